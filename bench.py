@@ -1,0 +1,286 @@
+#!/usr/bin/env python3
+"""Benchmark of the RdfPlan / RmsdPlan hot path on one MI355X.
+
+One step = one pass of the hot path over one batch of synthetic input, with the
+inputs already resident in HBM:
+  * RdfPlan, OW-OW: 100,000 waters (the OW atoms of a 300,000-atom system) in a
+    cubic 144.3 A box, 1,000 float32 frames (47^3 lattice sites, a seeded choice
+    of 100,000, sigma = 0.3 A jitter per frame), 500 bins, r_max 15 A;
+  * batched RmsdPlan: 5,000 frames of a 20,000-atom chain, fit on the 2,000 CA
+    atoms, RMSD over all atoms, float32 coordinates, FP64 QCP.
+value = reference-equivalent RDF pair evaluations (frames x (N_a N_b - common))
+per second of the whole step.  The RMSD frames/s, the triclinic RDF rate and
+the per-kernel roofline figures are reported beside it.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import ctypes
+import json
+import math
+import os
+import platform
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "warp-md_amd"))
+
+VALU_PEAK_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12  # lane-ops/s: 256 CU x 4 SIMD x 16 lanes x 2.4 GHz
+HBM_PEAK_GBS = 8000.0
+RDF_OPS_PER_PAIR = 24  # algorithmic VALU lane-ops per unordered in-range pair
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_model():
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name"):
+                return line.split(":", 1)[1].strip()
+    except Exception:
+        pass
+    return platform.processor()
+
+
+def cpu_baseline(n_side=47, box=144.3, n_waters=100000):
+    """The oracle (a C restatement of the reference's single-threaded CPU pair
+    loop) timed on a bounded sample: frame 0 of the same water box, the first
+    rows of sel_a against all of sel_b."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+
+    import oracle
+    from warp_md_amd import workloads
+
+    w = workloads.water_lattice(n_side=n_side, box=box, n_frames=1, n_waters=n_waters, seed=1,
+                                with_h=False)
+    n = w["coords"].shape[1]
+    allat = np.arange(n)
+    # calibrate to ~10 s of single-thread work
+    probe = 500
+    t0 = time.perf_counter()
+    oracle.rdf_counts(w["coords"], allat[:probe], allat, 500, 15.0, boxes=w["box"])
+    dt = max(time.perf_counter() - t0, 1e-6)
+    rate = probe * (n - 1) / dt
+    rows = int(min(n, max(probe, 10.0 * rate / n)))
+    t0 = time.perf_counter()
+    oracle.rdf_counts(w["coords"], allat[:rows], allat, 500, 15.0, boxes=w["box"])
+    dt = time.perf_counter() - t0
+    evals = rows * (n - 1)
+    out = {"value": evals / dt, "unit": "pair-evals/s", "cores": 1, "kind": "port",
+           "sample": f"frame 0 of the 100,000-OW box, sel_a = first {rows} OW x all {n} OW "
+                     f"({evals:.3g} pair-evals, {dt:.1f} s, 1 thread, brute force in the "
+                     f"reference loop order)",
+           "cpu": cpu_model()}
+    # all-cores line of the same loop (OpenMP over sel_a), labelled as such
+    threads = max(1, min(16, int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))))
+    rows2 = int(min(n, rows * threads))
+    t0 = time.perf_counter()
+    oracle.rdf_counts(w["coords"], allat[:rows2], allat, 500, 15.0, boxes=w["box"], method="omp",
+                      threads=threads)
+    dt2 = time.perf_counter() - t0
+    out["all_cores"] = {"value": rows2 * (n - 1) / dt2, "threads": threads}
+    # RMSD frames/s of the oracle's FP64 Kabsch (fit on CA, RMSD over all atoms)
+    c = workloads.chain(n_frames=100, seed=2)
+    t0 = time.perf_counter()
+    oracle.fit_rmsd(c["coords"], c["ca"], np.arange(c["ref"].shape[0]), c["ref"])
+    out["rmsd_frames_per_s"] = 100 / (time.perf_counter() - t0)
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--frames", type=int, default=1000)
+    ap.add_argument("--rmsd-frames", type=int, default=5000)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-triclinic", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for "
+              f"--gpus {args.gpus}); the frame-sharded design is described in DESIGN.md.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from warp_md_amd import _native, workloads
+    from warp_md_amd._native import check, lib, ptr
+
+    dev = "cuda:0"
+    ctx = _native.context(0)
+    L = lib()
+    V = lambda t: ctypes.c_void_p(t.data_ptr())
+
+    # ---- inputs resident in HBM ------------------------------------------
+    F = args.frames
+    box_len = 144.3
+    log(f"generating {F} frames of the 100,000-water box on the GPU")
+    store, n, stride, _ = workloads.water_lattice_device(47, box_len, F, n_waters=100000, seed=1,
+                                                         device=dev)
+    boxes = np.repeat(np.diag([box_len] * 3).reshape(1, 9), F, 0).astype(np.float32)
+    sel = np.arange(n, dtype=np.uint32)
+    h_rdf = ctypes.c_void_p()
+    check(L.wmd_rdf_create(ctx.handle, ptr(sel), n, ptr(sel), n, 500, ctypes.c_float(15.0), 1, 0,
+                           ctypes.byref(h_rdf)))
+
+    c = workloads.chain(n_frames=0, seed=2)
+    n_chain = c["ref"].shape[0]
+    r4 = np.ones((n_chain, 4), np.float32)
+    r4[:, :3] = c["ref"]
+    allat = np.arange(n_chain, dtype=np.uint32)
+    h_rmsd = ctypes.c_void_p()
+    check(L.wmd_rmsd_create(ctx.handle, ptr(c["ca"]), c["ca"].size, ptr(allat), allat.size, ptr(r4),
+                            n_chain, 1, ctypes.byref(h_rmsd)))
+    ns, st = ctypes.c_size_t(), ctypes.c_size_t()
+    check(L.wmd_rmsd_store_layout(h_rmsd, None, 0, ctypes.byref(ns), ctypes.byref(st)))
+    atoms = np.zeros(st.value, np.uint32)
+    check(L.wmd_rmsd_store_layout(h_rmsd, ptr(atoms), atoms.size, ctypes.byref(ns), ctypes.byref(st)))
+    RF = args.rmsd_frames
+    log(f"generating {RF} chain frames on the GPU")
+    chain_store = workloads.chain_store_device(c["ref"], atoms, RF, seed=2, device=dev)
+    o32 = torch.zeros(RF, dtype=torch.float32, device=dev)
+    o64 = torch.zeros(RF, dtype=torch.float64, device=dev)
+    oerr = torch.zeros(RF, dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+
+    def step():
+        check(L.wmd_rdf_reset(h_rdf))
+        check(L.wmd_rdf_accumulate_device(h_rdf, V(store), stride, F, ptr(boxes), None))
+        check(L.wmd_rmsd_run_device(h_rmsd, V(chain_store), st.value, RF, V(o32), V(o64), V(oerr),
+                                    None, None, None))
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    ctx.profile(True)
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    pair_ms, pair_n = ctx.profile_read("rdf_pair")
+    build_ms, build_n = ctx.profile_read("cell_build")
+    mom_ms, mom_n = ctx.profile_read("rmsd_moments")
+    fin_ms, fin_n = ctx.profile_read("rmsd_finalize")
+    ctx.profile(False)
+    counts = np.zeros(500, np.uint64)
+    check(L.wmd_rdf_read(h_rdf, ptr(counts), None))
+    rmsd = o64.cpu().numpy()
+    path = ctypes.c_int()
+    ncs = (ctypes.c_int * 3)()
+    check(L.wmd_rdf_last_path(h_rdf, ctypes.byref(path), ncs))
+    K = args.steps
+    ms_per_step = elapsed / K * 1e3
+    pair_evals = F * (n * n - n)  # reference-equivalent ordered pair evaluations
+    value = pair_evals / (ms_per_step / 1e3)
+    rdf_ms = (pair_ms + build_ms) / K
+    rmsd_ms = (mom_ms + fin_ms) / K
+    p_in = float(counts.sum()) / 2.0  # unordered in-range pairs per step
+    rdf_ach = RDF_OPS_PER_PAIR * p_in * K / (pair_ms / 1e3) / 1e12
+    rmsd_bytes = 12.0 * n_chain * RF + 12.0 * n_chain + 4.0 * RF
+    rmsd_ach = rmsd_bytes * K / (mom_ms / 1e3) / 1e9
+    log(f"rdf path={path.value} cells={list(ncs)} pair {pair_ms / K:.2f} ms/step over "
+        f"{pair_n // K} launches, build {build_ms / K:.2f} ms; rmsd moments {mom_ms / K:.3f} ms, "
+        f"finalize {fin_ms / K:.3f} ms; step {ms_per_step:.2f} ms")
+
+    # triclinic variant of the RDF workload (10 deg xy tilt), timed separately
+    tric = None
+    if not args.no_triclinic:
+        del store
+        torch.cuda.empty_cache()
+        log("triclinic variant")
+        tstore, tn, tstride, M = workloads.water_lattice_device(47, box_len, F, n_waters=100000,
+                                                                seed=1, tilt_deg=10.0, device=dev)
+        tboxes = np.repeat(M.reshape(1, 9), F, 0).astype(np.float32)
+        h_tri = ctypes.c_void_p()
+        check(L.wmd_rdf_create(ctx.handle, ptr(sel), n, ptr(sel), n, 500, ctypes.c_float(15.0), 2, 0,
+                               ctypes.byref(h_tri)))
+        torch.cuda.synchronize()
+        check(L.wmd_rdf_accumulate_device(h_tri, V(tstore), tstride, F, ptr(tboxes), None))
+        torch.cuda.synchronize()
+        check(L.wmd_rdf_reset(h_tri))
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        check(L.wmd_rdf_accumulate_device(h_tri, V(tstore), tstride, F, ptr(tboxes), None))
+        torch.cuda.synchronize()
+        tdt = time.perf_counter() - t1
+        tcounts = np.zeros(500, np.uint64)
+        check(L.wmd_rdf_read(h_tri, ptr(tcounts), None))
+        L.wmd_rdf_destroy(h_tri)
+        tric = {"pair_evals_per_s": pair_evals / tdt, "ms": tdt * 1e3,
+                "total_pairs": int(tcounts.sum())}
+        del tstore
+        torch.cuda.empty_cache()
+
+    L.wmd_rdf_destroy(h_rdf)
+    L.wmd_rmsd_destroy(h_rmsd)
+    cpu = None if args.no_cpu else cpu_baseline()
+
+    line = {
+        "metric": "RDF pair-evals/s + RMSD frames/s on 1 MI355X; soft-RDF fwd+bwd vs plain",
+        "value": value,
+        "unit": "pair-evals/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "RdfPlan OW-OW, 100,000 waters, cubic 144.3 A, 1,000 frames, 500 bins, "
+                        "r_max 15 A + batched RmsdPlan 5,000 frames x 20,000 atoms, fit on 2,000 CA, "
+                        "RMSD over all atoms",
+            "rdf_frames": F, "rdf_atoms": n, "bins": 500, "r_max": 15.0,
+            "rmsd_frames": RF, "rmsd_atoms": n_chain, "rmsd_fit_atoms": int(c["ca"].size),
+            "parallelism": "single GPU",
+        },
+        "rdf_pair_evals_per_s": pair_evals / (rdf_ms / 1e3),
+        "rmsd_frames_per_s": RF / (rmsd_ms / 1e3),
+        "rdf_triclinic": tric,
+        "softrdf_fwd_bwd_over_plain": None,
+        "roofline": {
+            "kernel": "rdf_cell_kernel",
+            "bound": "valu",
+            "achieved": rdf_ach,
+            "peak": VALU_PEAK_TOPS,
+            "unit": "TOP/s",
+            "frac": rdf_ach / VALU_PEAK_TOPS,
+            "traffic": None,
+            "algorithmic": f"{RDF_OPS_PER_PAIR} VALU lane-ops per unordered in-range pair x "
+                           f"{p_in:.4g} pairs per step",
+            "avg_launch_ms": pair_ms / max(pair_n, 1),
+        },
+        "roofline_rmsd": {
+            "kernel": "rmsd_moments_kernel",
+            "bound": "hbm",
+            "achieved": rmsd_ach,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": rmsd_ach / HBM_PEAK_GBS,
+            "traffic": None,
+            "avg_launch_ms": mom_ms / max(mom_n, 1),
+        },
+        "checks": {
+            "rdf_total_pairs_per_step": int(counts.sum()),
+            "rmsd_mean": float(np.mean(rmsd)),
+            "rmsd_min": float(np.min(rmsd)),
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,194 @@
+/*
+ * wmd_hip.h -- C ABI of the MI355X (gfx950) engine for warp-md's frame-batched
+ * RdfPlan / RmsdPlan hot path.
+ *
+ * This library replaces the RDF/RMSD slice of crates/traj-gpu (the `GpuContext`
+ * handle, crates/traj-gpu/src/cuda.rs:52-159) and the corresponding kernels of
+ * crates/traj-kernels (rdf_hist part4.cu:29-70; rmsd_centroid / rmsd_cov /
+ * rmsd_raw_accum / rmsd_finalize part1.cu:299-393).  Every entry point takes
+ * plain pointers and sizes; no C++ exception crosses it.
+ *
+ * Conventions
+ *  - Host buffers are owned by the caller and only read (or written, for
+ *    outputs) during the call.  Handles are owned by the library.
+ *  - One context = one device + one blocking HIP stream (host-buffer entry
+ *    points run on it and return when done); a context and the handles made
+ *    from it are not reentrant.  Distinct contexts may be used from distinct
+ *    threads.  Device-buffer entry points run on the stream the caller passes.
+ *  - Every function returns a wmd_status; on failure wmd_last_error() returns a
+ *    thread-local message.  Status codes mirror traj_core's TrajError variants
+ *    (crates/traj-core/src/error.rs) so a Rust shim maps them one to one.
+ *  - "xyzw" host coordinates are the reference FrameChunk layout
+ *    (crates/traj-core/src/frame.rs:10-20): [n_frames][n_atoms][4] float, w ignored.
+ *  - "boxes" are [n_frames][9] row-major box vectors (a, b, c rows).  An
+ *    orthorhombic box is diag(lx, ly, lz); the engine reads m[0], m[4], m[8].
+ *  - Device-resident stores ("*_device" entry points) are SoA frames:
+ *    [n_frames][3][stride] float (x plane, y plane, z plane), stride a
+ *    multiple of 4 and >= the plan's store size (see *_store_layout).
+ */
+#ifndef WMD_HIP_H
+#define WMD_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum wmd_status {
+    WMD_OK = 0,
+    WMD_ERR_UNSUPPORTED = 1, /* TrajError::Unsupported (cuda.rs:384-390) */
+    WMD_ERR_MISMATCH = 2,    /* TrajError::Mismatch (runtime_core.rs:61-62, rdf.rs:894-908) */
+    WMD_ERR_PARSE = 3,       /* TrajError::Parse (rdf.rs:663-667) */
+    WMD_ERR_DEVICE = 4,      /* HIP runtime failure */
+    WMD_ERR_OOM = 5,         /* device allocation failure */
+    WMD_ERR_INVALID = 6      /* null pointer / index out of range / bad size */
+} wmd_status;
+
+enum { WMD_PBC_NONE = 0, WMD_PBC_ORTHO = 1, WMD_PBC_TRICLINIC = 2 };
+
+/* wmd_rdf_create flags */
+#define WMD_RDF_EDGE_DIAGNOSTICS 0x1u /* count counted pairs within 1e-5 A of a bin edge */
+#define WMD_RDF_FORCE_BRUTE 0x2u      /* never use the cell list (testing / autotuning) */
+
+/* wmd_rdf_finalize flags (RdfPlan output options, rdf.rs:204-227) */
+#define WMD_RDF_USE_VOLUME 0x1u
+#define WMD_RDF_RAW 0x2u
+#define WMD_RDF_NUMBER_DENSITY 0x4u
+#define WMD_RDF_INTEGRAL 0x8u
+
+typedef struct wmd_ctx wmd_ctx;
+typedef struct wmd_rdf wmd_rdf;
+typedef struct wmd_rmsd wmd_rmsd;
+
+/* ---- library / context --------------------------------------------------- */
+
+/* Thread-local message of the last failing call on this thread ("" if none). */
+const char *wmd_last_error(void);
+/* Library version string. */
+const char *wmd_version(void);
+/* Number of visible HIP devices (0 when no GPU / no driver; never fails). */
+wmd_status wmd_device_count(int *out);
+
+/* GpuContext::new (crates/traj-gpu/src/cuda.rs:52-159, runtime_core.rs:4-17).
+ * Code objects are loaded once per process; nothing is compiled at run time. */
+wmd_status wmd_ctx_create(int device, wmd_ctx **out);
+void wmd_ctx_destroy(wmd_ctx *ctx);
+/* Waits for all work queued on the context's stream. */
+wmd_status wmd_ctx_synchronize(wmd_ctx *ctx);
+/* The context's hipStream_t (as void*), for callers that enqueue around it. */
+void *wmd_ctx_stream(wmd_ctx *ctx);
+/* Number of compute units of the context's device. */
+int wmd_ctx_cu_count(const wmd_ctx *ctx);
+/* Kernel timing with HIP events recorded around the engine's main kernels on
+ * the stream they run on (enable = 1 starts recording and clears old marks).
+ * wmd_ctx_profile_read synchronises and returns, for kernel family `name`
+ * ("rdf_pair", "rmsd_moments", "rmsd_finalize", "cell_build"), the summed
+ * duration in ms and the number of launches since profiling was enabled. */
+wmd_status wmd_ctx_profile(wmd_ctx *ctx, int enable);
+wmd_status wmd_ctx_profile_read(wmd_ctx *ctx, const char *name, double *total_ms, int *launches);
+
+/* ---- RDF ------------------------------------------------------------------ */
+
+/* RdfPlan::init GPU branch (crates/traj-engine/src/plans/rdf.rs:692-714):
+ * GpuContext::selection(sel_a), selection(sel_b) (runtime_core.rs:19-39) and
+ * alloc_counts + reset_counts (packing_constraints.rs:891-914).
+ * sel_a / sel_b: atom indices (duplicates allowed, they weight pairs like the
+ * reference's loops).  pbc: WMD_PBC_*.  Pairs with the same atom index are
+ * skipped (rdf.rs:917-919).  The IO selection is the order-preserving union of
+ * sel_a then sel_b (rdf.rs:498-526). */
+wmd_status wmd_rdf_create(wmd_ctx *ctx, const uint32_t *sel_a, size_t n_a, const uint32_t *sel_b,
+                          size_t n_b, size_t bins, float r_max, int pbc, uint32_t flags,
+                          wmd_rdf **out);
+/* The IO selection (union of sel_a and sel_b, first-seen order). *n gets its
+ * length; out may be NULL to query the length. */
+wmd_status wmd_rdf_io_selection(const wmd_rdf *rdf, uint32_t *out, size_t cap, size_t *n);
+/* GpuContext::rdf_accum (crates/traj-gpu/src/cuda/structure_polymer.rs:246-313)
+ * on a full-system host chunk: coords [n_frames][n_atoms][4], boxes
+ * [n_frames][9] (may be NULL for WMD_PBC_NONE).  Counts persist across calls. */
+wmd_status wmd_rdf_accumulate(wmd_rdf *rdf, const float *coords_xyzw, size_t n_atoms,
+                              size_t n_frames, const float *boxes);
+/* RdfPlan::process_chunk_selected (rdf.rs:882-941): the chunk holds only the IO
+ * selection atoms, in IO-selection order: coords [n_frames][n_io][4]. */
+wmd_status wmd_rdf_accumulate_selected(wmd_rdf *rdf, const float *coords_xyzw, size_t n_frames,
+                                       const float *boxes);
+/* Hot path on device-resident coordinates: d_soa is [n_frames][3][stride] float
+ * holding the IO selection atoms in IO-selection order; boxes are host
+ * [n_frames][9].  Work is enqueued on `stream` (a hipStream_t; NULL = the null
+ * stream), after the work already queued there, and the call returns before it
+ * completes. */
+wmd_status wmd_rdf_accumulate_device(wmd_rdf *rdf, const float *d_soa, size_t stride,
+                                     size_t n_frames, const float *boxes, void *stream);
+/* GpuContext::read_counts (packing_constraints.rs:891-898): copies the u64
+ * counts [bins] out (synchronises).  edge: NULL or [1], the number of counted
+ * pairs whose f32 distance lies within 1e-5 A of a bin edge (requires
+ * WMD_RDF_EDGE_DIAGNOSTICS, else 0). */
+wmd_status wmd_rdf_read(wmd_rdf *rdf, uint64_t *counts, uint64_t *edge);
+/* GpuContext::reset_counts (packing_constraints.rs:900-914). */
+wmd_status wmd_rdf_reset(wmd_rdf *rdf);
+void wmd_rdf_destroy(wmd_rdf *rdf);
+/* Which pair kernel the last accumulate used: 0 none yet, 1 brute force,
+ * 2 cell list; *cells_per_side receives the last cell grid (or zeros). */
+wmd_status wmd_rdf_last_path(const wmd_rdf *rdf, int *path, int *cells_per_side);
+/* Cell-list shape override from an autotuner: k = cells per r_max (1..3, 0 =
+ * automatic).  Takes effect on the next accumulate. */
+wmd_status wmd_rdf_set_tuning(wmd_rdf *rdf, int cell_k, int block_threads);
+
+/* RdfPlan::finalize normalisation (rdf.rs:943-1016) on host data: r, g [bins]
+ * and, with WMD_RDF_INTEGRAL, integral [bins] (else may be NULL).  flags:
+ * WMD_RDF_USE_VOLUME / RAW / NUMBER_DENSITY / INTEGRAL.  Returns
+ * WMD_ERR_MISMATCH for USE_VOLUME with volume_sum <= 0 (rdf.rs:962-966). */
+wmd_status wmd_rdf_finalize(const uint64_t *counts, size_t bins, float r_max, double n_a,
+                            double n_b, double common, size_t frames, double volume_sum,
+                            double density, uint32_t flags, float *r, float *g, float *integral);
+
+/* ---- RMSD ----------------------------------------------------------------- */
+
+/* RmsdPlan::init GPU branch (crates/traj-engine/src/plans/rmsd/plan.rs:205-220):
+ * GpuContext::selection + GpuContext::reference (runtime_core.rs:127-137).
+ * fit_sel: atoms of the superposition fit; rmsd_sel: atoms the RMSD is taken
+ * over (NULL => same as fit_sel, the RmsdPlan case; the split form is the
+ * fit-on-A / RMSD-over-B composition of rms.rs:643-676).  ref_xyzw:
+ * [n_ref_atoms][4] reference coordinates indexed by atom.  align = 0 gives the
+ * unfitted RMSD (rmsd/plan.rs:1093-1110; requires rmsd_sel == fit_sel). */
+wmd_status wmd_rmsd_create(wmd_ctx *ctx, const uint32_t *fit_sel, size_t n_fit,
+                           const uint32_t *rmsd_sel, size_t n_rmsd, const float *ref_xyzw,
+                           size_t n_ref_atoms, int align, wmd_rmsd **out);
+/* Device store layout: atoms[slot] = atom index held in store slot `slot`
+ * (0xFFFFFFFF = zero padding); *n_store = slots used; *stride = minimum SoA
+ * plane stride.  atoms may be NULL to query sizes. */
+wmd_status wmd_rmsd_store_layout(const wmd_rmsd *rmsd, uint32_t *atoms, size_t cap,
+                                 size_t *n_store, size_t *stride);
+/* GpuContext::rmsd_covariance + host rmsd_from_cov, or rmsd_raw
+ * (runtime_core.rs:375-535, rmsd/math.rs:7-38) on a host chunk
+ * [n_frames][n_atoms][4]: out_rmsd [n_frames] f32 (synchronous).  out_err:
+ * NULL or [n_frames] estimated absolute error bound (A) of each RMSD against
+ * exact arithmetic on the same f32 inputs. */
+wmd_status wmd_rmsd_run(wmd_rmsd *rmsd, const float *coords_xyzw, size_t n_atoms,
+                        size_t n_frames, float *out_rmsd, double *out_err);
+/* Hot path on a device-resident store [n_frames][3][stride] (see
+ * wmd_rmsd_store_layout).  Device outputs (each may be NULL except d_rmsd):
+ * d_rmsd [F] f32, d_rmsd64 [F] f64, d_err [F] f64, d_rot [F][9] f64 (row-major
+ * R with R (x - cx) ~ (y - cy)), d_cx [F][3] f64 fit centroid of the frame.
+ * Enqueued on `stream` (a hipStream_t; NULL = the null stream) after the work
+ * already queued there; returns before it completes. */
+wmd_status wmd_rmsd_run_device(wmd_rmsd *rmsd, const float *d_store, size_t stride,
+                               size_t n_frames, float *d_rmsd, double *d_rmsd64, double *d_err,
+                               double *d_rot, double *d_cx, void *stream);
+void wmd_rmsd_destroy(wmd_rmsd *rmsd);
+
+/* QCP superposition of one set of centred moments, evaluated on the host with
+ * the same code the device runs (for testing without a GPU).
+ * m: [9] row-major M = sum (x - cx)(y - cy)^T, gx = sum |x - cx|^2,
+ * gy = sum |y - cy|^2, n = atom count.  Outputs: *rmsd (fitted RMSD over the
+ * same atoms), rot [9] row-major R, *lambda (largest eigenvalue of the key
+ * matrix), *iters (Newton iterations; -1 when the Jacobi fallback ran). */
+wmd_status wmd_qcp_host(const double *m, double gx, double gy, double n, double *rmsd,
+                        double *rot, double *lambda, int *iters);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* WMD_HIP_H */

@@ -208,7 +208,7 @@ static inline int near_edge(double r, double w, size_t bins, double tol) {
 #define EDGE_TOL 1.0e-5
 
 typedef struct {
-    uint64_t edge32;   /* pairs whose f32 distance lies within 1e-5 A of a bin edge */
+    uint64_t edge32;   /* counted pairs whose f32 distance lies within 1e-5 A of a bin edge */
     uint64_t edge64;   /* pairs whose f64 distance lies within 1e-5 A of a bin edge */
     uint64_t mismatch; /* pairs whose f32 and f64 bins differ */
 } edge_stats;
@@ -221,8 +221,7 @@ static inline void pair_account(const float *pa, const float *pb, const frame_bo
     if (es) {
         double w = (double)bin_width;
         double r64 = pair_r64(pa, pb, fb);
-        if ((double)r < (double)r_max + 2.0 * EDGE_TOL && near_edge((double)r, w, bins, EDGE_TOL))
-            es->edge32 += 1;
+        if (bin >= 0 && near_edge((double)r, w, bins, EDGE_TOL)) es->edge32 += 1;
         if (r64 < (double)r_max + 2.0 * EDGE_TOL && near_edge(r64, w, bins, EDGE_TOL))
             es->edge64 += 1;
         long b64 = -1;

@@ -1,0 +1,143 @@
+// Internal declarations shared by the engine's translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "wmd_hip.h"
+
+namespace wmd {
+
+// ---------------------------------------------------------------------------
+// Errors: a thread-local message plus a status code; no exception crosses the
+// C ABI (every extern "C" body is wrapped in WMD_API_BEGIN / WMD_API_END).
+// ---------------------------------------------------------------------------
+void set_error(const std::string &msg);
+void clear_error();
+
+struct Error {
+    wmd_status status;
+    std::string msg;
+};
+
+[[noreturn]] void fail(wmd_status st, const std::string &msg);
+
+#define WMD_HIP_CHECK(expr)                                                                   \
+    do {                                                                                      \
+        hipError_t _e = (expr);                                                               \
+        if (_e != hipSuccess) {                                                               \
+            ::wmd::fail(_e == hipErrorOutOfMemory ? WMD_ERR_OOM : WMD_ERR_DEVICE,             \
+                        std::string(#expr) + ": " + hipGetErrorString(_e));                   \
+        }                                                                                     \
+    } while (0)
+
+#define WMD_REQUIRE(cond, st, msg)                                                            \
+    do {                                                                                      \
+        if (!(cond)) ::wmd::fail((st), (msg));                                                \
+    } while (0)
+
+#define WMD_API_BEGIN                                                                         \
+    ::wmd::clear_error();                                                                     \
+    try {
+#define WMD_API_END                                                                           \
+    }                                                                                         \
+    catch (const ::wmd::Error &e) {                                                           \
+        ::wmd::set_error(e.msg);                                                              \
+        return e.status;                                                                      \
+    }                                                                                         \
+    catch (const std::exception &e) {                                                         \
+        ::wmd::set_error(e.what());                                                           \
+        return WMD_ERR_INVALID;                                                               \
+    }                                                                                         \
+    catch (...) {                                                                             \
+        ::wmd::set_error("unknown internal error");                                           \
+        return WMD_ERR_INVALID;                                                               \
+    }                                                                                         \
+    return WMD_OK;
+
+// ---------------------------------------------------------------------------
+// Device memory owned by the library: a growable buffer freed on destruction.
+// ---------------------------------------------------------------------------
+struct DevBuf {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    int device = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf();
+    // Ensures at least `n` bytes; contents are not preserved when it grows.
+    void *reserve(size_t n);
+    template <class T> T *as() const { return static_cast<T *>(ptr); }
+};
+
+// Pinned host staging buffer (hipHostMalloc), growable.
+struct HostBuf {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    HostBuf() = default;
+    HostBuf(const HostBuf &) = delete;
+    HostBuf &operator=(const HostBuf &) = delete;
+    ~HostBuf();
+    void *reserve(size_t n);
+};
+
+// ---------------------------------------------------------------------------
+// Context (the GpuContext replacement): device, one default-priority stream.
+// ---------------------------------------------------------------------------
+}  // namespace wmd
+
+struct wmd_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int cu_count = 0;
+    int xcd_count = 8;
+    size_t lds_per_block = 0;
+    std::string arch;
+    // kernel timing (wmd_ctx_profile)
+    bool profile = false;
+    struct Mark {
+        std::string name;
+        hipEvent_t start, stop;
+    };
+    std::vector<Mark> marks;
+};
+
+namespace wmd {
+
+// Makes the context's device current for the calling thread.
+void bind(const wmd_ctx *ctx);
+// The stream a device-buffer entry point runs on: the caller's (NULL = null stream).
+hipStream_t pick_stream(const wmd_ctx *ctx, void *stream);
+
+inline size_t round_up(size_t x, size_t m) { return (x + m - 1) / m * m; }
+
+// Records HIP events around a launch sequence on `st` when profiling is on.
+struct ProfScope {
+    wmd_ctx *ctx;
+    hipStream_t st;
+    int idx = -1;
+    ProfScope(wmd_ctx *c, hipStream_t s, const char *name);
+    ~ProfScope();
+};
+
+// Host-side minimum-image support shared by the RDF paths.
+struct BoxInfo {
+    // f32 parameters the pair kernels use (one per frame).
+    float lx, ly, lz;       // orthorhombic lengths
+    float c[9];             // triclinic rows (box vectors), f32
+    float r[9];             // triclinic inverse rows (f32 casts of the f64 inverse)
+    double cell64[9], inv64[9];
+    double width[3];        // plane spacings (A) along the three fractional axes
+};
+
+// Builds BoxInfo from a row-major box for the given pbc kind.
+// Returns false for a singular / non-positive box.
+bool make_box(int pbc, const float *m, BoxInfo *out);
+
+}  // namespace wmd

@@ -1,0 +1,511 @@
+// RMSD superposition for gfx950: the engine behind RmsdPlan's device branch
+// (crates/traj-engine/src/plans/rmsd/plan.rs:361-392) replacing
+// GpuContext::rmsd_covariance / rmsd_raw (crates/traj-gpu/src/cuda/runtime_core.rs:375-535),
+// the rmsd_centroid / rmsd_cov / rmsd_raw_accum / rmsd_finalize kernels
+// (crates/traj-kernels/src/kernels/part1.cu:299-393) and the host-side
+// rmsd_from_cov (plans/rmsd/math.rs:7-38).
+//
+// One pass over HBM.  A frame's store is SoA planes x[stride], y[stride],
+// z[stride] of the plan's atoms, permuted into three 4-aligned segments:
+// atoms in both the fit and the RMSD selection, fit-only atoms, RMSD-only
+// atoms.  Each wave owns one (frame, tile) and streams float4 groups of the
+// three planes; the reference planes stay L2-resident.  Per atom it adds the
+// raw moments  Sx += x,  Sxx += |x|^2,  Sxy += x y^T  in FP64: an f32 x f32
+// product is exact in FP64 and v_fma_f64 issues at the same rate as a
+// non-packed FP32 FMA on gfx950, so this is both cheaper and more accurate
+// than a compensated FP32 chain.  Lanes are combined by a wave tree reduction
+// and tiles by a fixed-order sum, so results are deterministic.
+//
+// The finalize kernel (one thread per frame) centres the moments, solves the
+// QCP polynomial in FP64 (wmd_qcp.hpp) and evaluates
+//   RMSD^2 = (Gx_B + Gy_B - 2 tr(R M_B)) / n_B
+// with the fit set's centroids and rotation, i.e. the composition of
+// kabsch_rotation (geometry_math.rs:169-297) and the residual of rms.rs:656-672;
+// for fit == RMSD selection it is 2 (E0 - lambda) / n, the RmsdPlan value.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+
+#include "wmd_internal.hpp"
+#include "wmd_qcp.hpp"
+
+using namespace wmd;
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr uint32_t kPad = 0xFFFFFFFFu;
+constexpr int kMom = 13;  // Sx[3], Sxy[9], Sxx
+
+enum { MODE_SAME = 0, MODE_SPLIT = 1, MODE_RAW = 2 };
+
+struct RefMoments {
+    double n_a, n_b;
+    double sy_a[3], syy_a;  // reference moments over the fit set
+    double sy_b[3], syy_b;  // over the RMSD set
+};
+
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__device__ __forceinline__ void acc_atom(double *a, float xf, float yf, float zf, float rxf,
+                                         float ryf, float rzf) {
+    const double x = xf, y = yf, z = zf;
+    const double rx = rxf, ry = ryf, rz = rzf;
+    a[0] += x;
+    a[1] += y;
+    a[2] += z;
+    a[3] = fma(x, rx, a[3]);
+    a[4] = fma(x, ry, a[4]);
+    a[5] = fma(x, rz, a[5]);
+    a[6] = fma(y, rx, a[6]);
+    a[7] = fma(y, ry, a[7]);
+    a[8] = fma(y, rz, a[8]);
+    a[9] = fma(z, rx, a[9]);
+    a[10] = fma(z, ry, a[10]);
+    a[11] = fma(z, rz, a[11]);
+    a[12] = fma(x, x, fma(y, y, fma(z, z, a[12])));
+}
+
+__device__ __forceinline__ void acc_raw(double *a, float xf, float yf, float zf, float rxf,
+                                        float ryf, float rzf) {
+    // (f32 - f32) is exact in FP64, like the reference's `as f64` differences
+    const double dx = (double)xf - (double)rxf;
+    const double dy = (double)yf - (double)ryf;
+    const double dz = (double)zf - (double)rzf;
+    a[0] = fma(dx, dx, fma(dy, dy, fma(dz, dz, a[0])));
+}
+
+template <int MODE>
+__device__ __forceinline__ void acc_group(double *A, double *B, int seg, const float4 &x4,
+                                          const float4 &y4, const float4 &z4, const float4 &rx,
+                                          const float4 &ry, const float4 &rz) {
+    if constexpr (MODE == MODE_RAW) {
+        acc_raw(A, x4.x, y4.x, z4.x, rx.x, ry.x, rz.x);
+        acc_raw(A, x4.y, y4.y, z4.y, rx.y, ry.y, rz.y);
+        acc_raw(A, x4.z, y4.z, z4.z, rx.z, ry.z, rz.z);
+        acc_raw(A, x4.w, y4.w, z4.w, rx.w, ry.w, rz.w);
+    } else if constexpr (MODE == MODE_SAME) {
+        acc_atom(A, x4.x, y4.x, z4.x, rx.x, ry.x, rz.x);
+        acc_atom(A, x4.y, y4.y, z4.y, rx.y, ry.y, rz.y);
+        acc_atom(A, x4.z, y4.z, z4.z, rx.z, ry.z, rz.z);
+        acc_atom(A, x4.w, y4.w, z4.w, rx.w, ry.w, rz.w);
+    } else {
+        if (seg != 2) {  // fit set: both or fit-only
+            acc_atom(A, x4.x, y4.x, z4.x, rx.x, ry.x, rz.x);
+            acc_atom(A, x4.y, y4.y, z4.y, rx.y, ry.y, rz.y);
+            acc_atom(A, x4.z, y4.z, z4.z, rx.z, ry.z, rz.z);
+            acc_atom(A, x4.w, y4.w, z4.w, rx.w, ry.w, rz.w);
+        }
+        if (seg != 1) {  // RMSD set: both or RMSD-only
+            acc_atom(B, x4.x, y4.x, z4.x, rx.x, ry.x, rz.x);
+            acc_atom(B, x4.y, y4.y, z4.y, rx.y, ry.y, rz.y);
+            acc_atom(B, x4.z, y4.z, z4.z, rx.z, ry.z, rz.z);
+            acc_atom(B, x4.w, y4.w, z4.w, rx.w, ry.w, rz.w);
+        }
+    }
+}
+
+// partials: [F][n_tiles][NV] doubles, NV = 1 (raw), 13 (same), 26 (split)
+template <int MODE>
+__global__ __launch_bounds__(kBlock) void rmsd_moments_kernel(
+    const float *__restrict__ store, long long frame_stride, int stride, int n_frames,
+    const float *__restrict__ ref, int n_groups, int seg1, int seg2, int n_tiles, int gpt,
+    double *__restrict__ partials) {
+    constexpr int NV = MODE == MODE_RAW ? 1 : (MODE == MODE_SAME ? kMom : 2 * kMom);
+    const int wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const long long unit = (long long)blockIdx.x * (kBlock / 64) + wave;
+    if (unit >= (long long)n_frames * n_tiles) return;
+    const int f = (int)(unit / n_tiles);
+    const int t = (int)(unit - (long long)f * n_tiles);
+    const int g0 = t * gpt;
+    const int g1 = min(n_groups, g0 + gpt);
+    const float4 *X = (const float4 *)(store + (long long)f * frame_stride);
+    const float4 *Y = (const float4 *)(store + (long long)f * frame_stride + stride);
+    const float4 *Z = (const float4 *)(store + (long long)f * frame_stride + 2 * stride);
+    const float4 *RX = (const float4 *)ref;
+    const float4 *RY = (const float4 *)(ref + stride);
+    const float4 *RZ = (const float4 *)(ref + 2 * stride);
+    double A[MODE == MODE_RAW ? 1 : kMom];
+    double B[MODE == MODE_SPLIT ? kMom : 1];
+#pragma unroll
+    for (int k = 0; k < (MODE == MODE_RAW ? 1 : kMom); ++k) A[k] = 0.0;
+#pragma unroll
+    for (int k = 0; k < (MODE == MODE_SPLIT ? kMom : 1); ++k) B[k] = 0.0;
+    int g = g0 + lane;
+    // two groups in flight per lane
+    for (; g + 64 < g1; g += 128) {
+        float4 x0 = X[g], y0 = Y[g], z0 = Z[g];
+        float4 x1 = X[g + 64], y1 = Y[g + 64], z1 = Z[g + 64];
+        float4 a0 = RX[g], b0 = RY[g], c0 = RZ[g];
+        float4 a1 = RX[g + 64], b1 = RY[g + 64], c1 = RZ[g + 64];
+        const int s0 = (g >= seg1) + (g >= seg2);
+        const int s1 = (g + 64 >= seg1) + (g + 64 >= seg2);
+        acc_group<MODE>(A, B, s0, x0, y0, z0, a0, b0, c0);
+        acc_group<MODE>(A, B, s1, x1, y1, z1, a1, b1, c1);
+    }
+    if (g < g1) {
+        float4 x0 = X[g], y0 = Y[g], z0 = Z[g];
+        float4 a0 = RX[g], b0 = RY[g], c0 = RZ[g];
+        const int s0 = (g >= seg1) + (g >= seg2);
+        acc_group<MODE>(A, B, s0, x0, y0, z0, a0, b0, c0);
+    }
+    double *out = partials + ((long long)f * n_tiles + t) * NV;
+#pragma unroll
+    for (int k = 0; k < (MODE == MODE_RAW ? 1 : kMom); ++k) {
+        double v = wave_sum(A[k]);
+        if (lane == 0) out[k] = v;
+    }
+    if constexpr (MODE == MODE_SPLIT) {
+#pragma unroll
+        for (int k = 0; k < kMom; ++k) {
+            double v = wave_sum(B[k]);
+            if (lane == 0) out[kMom + k] = v;
+        }
+    }
+}
+
+// Centred moments of a set relative to centroids (cx of the frame set, cy of
+// the reference set): M = Sxy - cx Sy^T - Sx cy^T + n cx cy^T, G likewise.
+__device__ __host__ inline void centred(const double *S, const double *sy, double syy, double n,
+                                        const double *cx, const double *cy, double *M, double *gx,
+                                        double *gy) {
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b)
+            M[a * 3 + b] = S[3 + a * 3 + b] - cx[a] * sy[b] - S[a] * cy[b] + n * cx[a] * cy[b];
+    *gx = S[12] - 2.0 * (cx[0] * S[0] + cx[1] * S[1] + cx[2] * S[2]) +
+          n * (cx[0] * cx[0] + cx[1] * cx[1] + cx[2] * cx[2]);
+    *gy = syy - 2.0 * (cy[0] * sy[0] + cy[1] * sy[1] + cy[2] * sy[2]) +
+          n * (cy[0] * cy[0] + cy[1] * cy[1] + cy[2] * cy[2]);
+}
+
+template <int MODE>
+__global__ void rmsd_finalize_kernel(const double *__restrict__ partials, int n_frames, int n_tiles,
+                                     RefMoments rm, double lane_terms, float *__restrict__ out,
+                                     double *__restrict__ out64, double *__restrict__ err,
+                                     double *__restrict__ rot_out, double *__restrict__ cx_out) {
+    constexpr int NV = MODE == MODE_RAW ? 1 : (MODE == MODE_SAME ? kMom : 2 * kMom);
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= n_frames) return;
+    double S[NV];
+    for (int k = 0; k < NV; ++k) S[k] = 0.0;
+    for (int t = 0; t < n_tiles; ++t) {
+        const double *p = partials + ((long long)f * n_tiles + t) * NV;
+        for (int k = 0; k < NV; ++k) S[k] += p[k];
+    }
+    const double eps = 1.1102230246251565e-16;
+    const double growth = (lane_terms + 8.0 + n_tiles) * eps;
+    double rmsd = 0.0, e = 0.0;
+    double R[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    double cx[3] = {0, 0, 0};
+    if constexpr (MODE == MODE_RAW) {
+        if (rm.n_b > 0) {
+            rmsd = sqrt(S[0] / rm.n_b);
+            e = growth * S[0] / (2.0 * rm.n_b * fmax(rmsd, 1e-300));
+        }
+    } else {
+        const double *SA = S;
+        const double *SB = MODE == MODE_SPLIT ? S + kMom : S;
+        double cy[3] = {0, 0, 0};
+        if (rm.n_a > 0) {
+            for (int d = 0; d < 3; ++d) {
+                cx[d] = SA[d] / rm.n_a;
+                cy[d] = rm.sy_a[d] / rm.n_a;
+            }
+        }
+        double MA[9], gxa, gya;
+        centred(SA, rm.sy_a, rm.syy_a, rm.n_a, cx, cy, MA, &gxa, &gya);
+        QcpResult qr;
+        double resid, mag;
+        if constexpr (MODE == MODE_SAME) {
+            qcp_solve(MA, gxa, gya, rot_out != nullptr, &qr);
+            resid = gxa + gya - 2.0 * qr.lambda;
+            mag = SA[12] + rm.syy_a + 2.0 * fabs(qr.lambda);
+            if (rot_out) for (int k = 0; k < 9; ++k) R[k] = qr.rot[k];
+        } else {
+            if (rm.n_a > 0) {
+                qcp_solve(MA, gxa, gya, true, &qr);
+                for (int k = 0; k < 9; ++k) R[k] = qr.rot[k];
+            }
+            double MB[9], gxb, gyb;
+            centred(SB, rm.sy_b, rm.syy_b, rm.n_b, cx, cy, MB, &gxb, &gyb);
+            double tr = 0.0;
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b) tr += R[a * 3 + b] * MB[b * 3 + a];
+            resid = gxb + gyb - 2.0 * tr;
+            mag = SB[12] + rm.syy_b + 2.0 * fabs(tr);
+        }
+        const double n = MODE == MODE_SAME ? rm.n_a : rm.n_b;
+        if (n > 0) {
+            double r2 = resid / n;
+            rmsd = r2 > 0.0 ? sqrt(r2) : 0.0;
+            double de = growth * mag / n;  // error of r2
+            e = de / (rmsd + sqrt(de));
+        }
+    }
+    out[f] = (float)rmsd;
+    if (out64) out64[f] = rmsd;
+    if (err) err[f] = e;
+    if (rot_out) for (int k = 0; k < 9; ++k) rot_out[(long long)f * 9 + k] = R[k];
+    if (cx_out) for (int d = 0; d < 3; ++d) cx_out[(long long)f * 3 + d] = cx[d];
+}
+
+// AoS [F][n_src][4] -> SoA [F][3][stride] through the store's atom list.
+__global__ __launch_bounds__(kBlock) void rmsd_gather_kernel(const float4 *__restrict__ aos,
+                                                             long long n_src,
+                                                             const unsigned *__restrict__ idx,
+                                                             int stride, int n_frames,
+                                                             float *__restrict__ soa) {
+    long long total = (long long)stride * n_frames;
+    for (long long t = (long long)blockIdx.x * kBlock + threadIdx.x; t < total;
+         t += (long long)gridDim.x * kBlock) {
+        int f = (int)(t / stride);
+        int k = (int)(t - (long long)f * stride);
+        unsigned a = idx[k];
+        float x = 0.f, y = 0.f, z = 0.f;
+        if (a != kPad) {
+            float4 v = aos[(long long)f * n_src + a];
+            x = v.x; y = v.y; z = v.z;
+        }
+        float *o = soa + (long long)f * 3 * stride;
+        o[k] = x;
+        o[stride + k] = y;
+        o[2 * stride + k] = z;
+    }
+}
+
+}  // namespace
+
+struct wmd_rmsd {
+    wmd_ctx *ctx = nullptr;
+    int mode = MODE_SAME;
+    std::vector<uint32_t> atoms;  // store slot -> atom (kPad = zero padding)
+    size_t n_store = 0, stride = 0;
+    int seg1 = 0, seg2 = 0;  // group indices where fit-only / RMSD-only segments start
+    size_t n_ref_atoms = 0;
+    RefMoments rm{};
+    DevBuf d_ref, d_atoms, d_partials, d_aos, d_soa, d_out32, d_out64, d_err;
+    HostBuf h_stage;
+};
+
+namespace {
+
+template <int MODE>
+void launch_rmsd(wmd_rmsd *h, const float *d_store, size_t stride, size_t n_frames, float *d_rmsd,
+                 double *d_r64, double *d_err, double *d_rot, double *d_cx, hipStream_t st) {
+    constexpr int NV = MODE == MODE_RAW ? 1 : (MODE == MODE_SAME ? kMom : 2 * kMom);
+    const int n_groups = (int)(h->stride / 4);
+    // tiles per frame: aim for ~12k waves in flight, each lane streaming >= 32 groups
+    long long want = std::max<long long>(1, (12288 + (long long)n_frames - 1) / (long long)n_frames);
+    long long cap = std::max<long long>(1, n_groups / (64 * 32));
+    int n_tiles = (int)std::min(want, cap);
+    int gpt = (n_groups + n_tiles - 1) / n_tiles;
+    n_tiles = (n_groups + gpt - 1) / std::max(gpt, 1);
+    n_tiles = std::max(n_tiles, 1);
+    double *partials = (double *)h->d_partials.reserve((size_t)n_frames * n_tiles * NV * 8);
+    long long waves = (long long)n_frames * n_tiles;
+    long long blocks = (waves + (kBlock / 64) - 1) / (kBlock / 64);
+    WMD_REQUIRE(blocks < (1ll << 31), WMD_ERR_UNSUPPORTED, "too many frames for one launch");
+    {
+    ProfScope prof(h->ctx, st, "rmsd_moments");
+    hipLaunchKernelGGL(rmsd_moments_kernel<MODE>, dim3((unsigned)blocks), dim3(kBlock), 0, st,
+                       d_store, 3 * (long long)stride, (int)stride, (int)n_frames,
+                       h->d_ref.as<float>(), n_groups, h->seg1, h->seg2, n_tiles, gpt, partials);
+    }
+    WMD_HIP_CHECK(hipGetLastError());
+    double lane_terms = 4.0 * std::ceil((double)gpt / 64.0) + 6.0;
+    int fb = (int)((n_frames + 63) / 64);
+    {
+    ProfScope prof(h->ctx, st, "rmsd_finalize");
+    hipLaunchKernelGGL(rmsd_finalize_kernel<MODE>, dim3(fb), dim3(64), 0, st, partials,
+                       (int)n_frames, n_tiles, h->rm, lane_terms, d_rmsd, d_r64, d_err, d_rot, d_cx);
+    }
+    WMD_HIP_CHECK(hipGetLastError());
+}
+
+void run_device(wmd_rmsd *h, const float *d_store, size_t stride, size_t n_frames, float *d_rmsd,
+                double *d_r64, double *d_err, double *d_rot, double *d_cx, hipStream_t st) {
+    if (n_frames == 0) return;
+    WMD_REQUIRE(stride == h->stride, WMD_ERR_INVALID,
+                "store stride must equal the plan's stride (" + std::to_string(h->stride) + ")");
+    WMD_REQUIRE(d_store && d_rmsd, WMD_ERR_INVALID, "null device buffer");
+    if (h->mode == MODE_SAME) launch_rmsd<MODE_SAME>(h, d_store, stride, n_frames, d_rmsd, d_r64, d_err, d_rot, d_cx, st);
+    else if (h->mode == MODE_SPLIT) launch_rmsd<MODE_SPLIT>(h, d_store, stride, n_frames, d_rmsd, d_r64, d_err, d_rot, d_cx, st);
+    else launch_rmsd<MODE_RAW>(h, d_store, stride, n_frames, d_rmsd, d_r64, d_err, d_rot, d_cx, st);
+}
+
+}  // namespace
+
+extern "C" {
+
+wmd_status wmd_rmsd_create(wmd_ctx *ctx, const uint32_t *fit_sel, size_t n_fit,
+                           const uint32_t *rmsd_sel, size_t n_rmsd, const float *ref_xyzw,
+                           size_t n_ref_atoms, int align, wmd_rmsd **out) {
+    WMD_API_BEGIN
+    WMD_REQUIRE(ctx && out, WMD_ERR_INVALID, "null context or output handle");
+    *out = nullptr;
+    WMD_REQUIRE(n_fit == 0 || fit_sel, WMD_ERR_INVALID, "null fit selection");
+    WMD_REQUIRE(ref_xyzw || n_ref_atoms == 0, WMD_ERR_INVALID, "null reference");
+    const bool same = rmsd_sel == nullptr;
+    if (same) n_rmsd = n_fit;
+    WMD_REQUIRE(same || n_rmsd == 0 || rmsd_sel, WMD_ERR_INVALID, "null rmsd selection");
+    if (!align) WMD_REQUIRE(same, WMD_ERR_UNSUPPORTED, "align=false needs a single selection");
+    for (size_t i = 0; i < n_fit; ++i)
+        WMD_REQUIRE(fit_sel[i] < n_ref_atoms, WMD_ERR_MISMATCH, "selection index out of range of the reference");
+    if (!same)
+        for (size_t i = 0; i < n_rmsd; ++i)
+            WMD_REQUIRE(rmsd_sel[i] < n_ref_atoms, WMD_ERR_MISMATCH, "selection index out of range of the reference");
+    bind(ctx);
+    auto *h = new wmd_rmsd();
+    try {
+        h->ctx = ctx;
+        h->n_ref_atoms = n_ref_atoms;
+        h->mode = !align ? MODE_RAW : (same ? MODE_SAME : MODE_SPLIT);
+        std::vector<uint32_t> both, fit_only, rmsd_only;
+        if (same) {
+            both.assign(fit_sel, fit_sel + n_fit);  // list order, duplicates kept
+        } else {
+            // per atom: min(mA, mB) copies in both, the rest in the one-sided segments
+            std::map<uint32_t, std::pair<size_t, size_t>> mult;
+            for (size_t i = 0; i < n_fit; ++i) mult[fit_sel[i]].first += 1;
+            for (size_t i = 0; i < n_rmsd; ++i) mult[rmsd_sel[i]].second += 1;
+            for (auto &kv : mult) {
+                size_t a = kv.second.first, b = kv.second.second, m = std::min(a, b);
+                for (size_t k = 0; k < m; ++k) both.push_back(kv.first);
+                for (size_t k = m; k < a; ++k) fit_only.push_back(kv.first);
+                for (size_t k = m; k < b; ++k) rmsd_only.push_back(kv.first);
+            }
+        }
+        auto pad4 = [](std::vector<uint32_t> &v) { while (v.size() % 4) v.push_back(kPad); };
+        pad4(both);
+        pad4(fit_only);
+        pad4(rmsd_only);
+        h->atoms = both;
+        h->seg1 = (int)(h->atoms.size() / 4);
+        h->atoms.insert(h->atoms.end(), fit_only.begin(), fit_only.end());
+        h->seg2 = (int)(h->atoms.size() / 4);
+        h->atoms.insert(h->atoms.end(), rmsd_only.begin(), rmsd_only.end());
+        h->n_store = h->atoms.size();
+        h->stride = std::max<size_t>(4, round_up(h->n_store, 4));
+        while (h->atoms.size() < h->stride) h->atoms.push_back(kPad);
+        // reference store + reference moments (FP64, exact products)
+        std::vector<float> refs(3 * h->stride, 0.0f);
+        RefMoments rm{};
+        rm.n_a = (double)n_fit;
+        rm.n_b = (double)n_rmsd;
+        for (size_t k = 0; k < h->stride; ++k) {
+            uint32_t a = h->atoms[k];
+            if (a == kPad) continue;
+            float x = ref_xyzw[4 * (size_t)a], y = ref_xyzw[4 * (size_t)a + 1], z = ref_xyzw[4 * (size_t)a + 2];
+            refs[k] = x;
+            refs[h->stride + k] = y;
+            refs[2 * h->stride + k] = z;
+            int seg = ((int)(k / 4) >= h->seg1) + ((int)(k / 4) >= h->seg2);
+            double yy = (double)x * x + (double)y * y + (double)z * z;
+            if (seg != 2) {
+                rm.sy_a[0] += x; rm.sy_a[1] += y; rm.sy_a[2] += z; rm.syy_a += yy;
+            }
+            if (seg != 1) {
+                rm.sy_b[0] += x; rm.sy_b[1] += y; rm.sy_b[2] += z; rm.syy_b += yy;
+            }
+        }
+        h->rm = rm;
+        h->d_ref.reserve(refs.size() * 4);
+        WMD_HIP_CHECK(hipMemcpy(h->d_ref.ptr, refs.data(), refs.size() * 4, hipMemcpyHostToDevice));
+        h->d_atoms.reserve(h->atoms.size() * 4);
+        WMD_HIP_CHECK(hipMemcpy(h->d_atoms.ptr, h->atoms.data(), h->atoms.size() * 4, hipMemcpyHostToDevice));
+    } catch (...) {
+        delete h;
+        throw;
+    }
+    *out = h;
+    WMD_API_END
+}
+
+wmd_status wmd_rmsd_store_layout(const wmd_rmsd *h, uint32_t *atoms, size_t cap, size_t *n_store,
+                                 size_t *stride) {
+    WMD_API_BEGIN
+    WMD_REQUIRE(h != nullptr, WMD_ERR_INVALID, "null handle");
+    if (n_store) *n_store = h->n_store;
+    if (stride) *stride = h->stride;
+    if (atoms) {
+        WMD_REQUIRE(cap >= h->stride, WMD_ERR_INVALID, "output buffer too small");
+        std::memcpy(atoms, h->atoms.data(), h->stride * 4);
+    }
+    WMD_API_END
+}
+
+wmd_status wmd_rmsd_run(wmd_rmsd *h, const float *coords, size_t n_atoms, size_t n_frames,
+                        float *out_rmsd, double *out_err) {
+    WMD_API_BEGIN
+    WMD_REQUIRE(h != nullptr, WMD_ERR_INVALID, "null handle");
+    if (n_frames == 0) return WMD_OK;
+    WMD_REQUIRE(coords && out_rmsd, WMD_ERR_INVALID, "null buffer");
+    for (uint32_t a : h->atoms)
+        WMD_REQUIRE(a == kPad || a < n_atoms, WMD_ERR_MISMATCH,
+                    "selection index out of range for the chunk's atom count");
+    bind(h->ctx);
+    hipStream_t st = h->ctx->stream;
+    size_t fbytes = n_atoms * 16;
+    size_t batch = std::max<size_t>(1, std::min<size_t>(n_frames, (256ull << 20) / std::max<size_t>(fbytes, 1)));
+    float4 *d_aos = (float4 *)h->d_aos.reserve(batch * fbytes);
+    float *d_soa = (float *)h->d_soa.reserve(batch * 3 * h->stride * 4);
+    float *d_o = (float *)h->d_out32.reserve(batch * 4);
+    double *d_e = out_err ? (double *)h->d_err.reserve(batch * 8) : nullptr;
+    void *stage = h->h_stage.reserve(batch * fbytes);
+    for (size_t f0 = 0; f0 < n_frames; f0 += batch) {
+        size_t nf = std::min(batch, n_frames - f0);
+        WMD_HIP_CHECK(hipStreamSynchronize(st));
+        std::memcpy(stage, coords + f0 * n_atoms * 4, nf * fbytes);
+        WMD_HIP_CHECK(hipMemcpyAsync(d_aos, stage, nf * fbytes, hipMemcpyHostToDevice, st));
+        long long total = (long long)h->stride * nf;
+        int g = (int)std::min<long long>(65536, (total + kBlock - 1) / kBlock);
+        hipLaunchKernelGGL(rmsd_gather_kernel, dim3(g), dim3(kBlock), 0, st, (const float4 *)d_aos,
+                           (long long)n_atoms, h->d_atoms.as<unsigned>(), (int)h->stride, (int)nf, d_soa);
+        WMD_HIP_CHECK(hipGetLastError());
+        run_device(h, d_soa, h->stride, nf, d_o, nullptr, d_e, nullptr, nullptr, st);
+        WMD_HIP_CHECK(hipMemcpyAsync(out_rmsd + f0, d_o, nf * 4, hipMemcpyDeviceToHost, st));
+        if (out_err) WMD_HIP_CHECK(hipMemcpyAsync(out_err + f0, d_e, nf * 8, hipMemcpyDeviceToHost, st));
+    }
+    WMD_HIP_CHECK(hipStreamSynchronize(st));
+    WMD_API_END
+}
+
+wmd_status wmd_rmsd_run_device(wmd_rmsd *h, const float *d_store, size_t stride, size_t n_frames,
+                               float *d_rmsd, double *d_rmsd64, double *d_err, double *d_rot,
+                               double *d_cx, void *stream) {
+    WMD_API_BEGIN
+    WMD_REQUIRE(h != nullptr, WMD_ERR_INVALID, "null handle");
+    bind(h->ctx);
+    run_device(h, d_store, stride, n_frames, d_rmsd, d_rmsd64, d_err, d_rot, d_cx,
+               pick_stream(h->ctx, stream));
+    WMD_API_END
+}
+
+void wmd_rmsd_destroy(wmd_rmsd *h) {
+    if (!h) return;
+    (void)hipSetDevice(h->ctx->device);
+    // work may still be queued on a caller's stream: wait for the whole device
+    (void)hipDeviceSynchronize();
+    delete h;
+}
+
+wmd_status wmd_qcp_host(const double *m, double gx, double gy, double n, double *rmsd, double *rot,
+                        double *lambda, int *iters) {
+    WMD_API_BEGIN
+    WMD_REQUIRE(m != nullptr, WMD_ERR_INVALID, "null moments");
+    QcpResult qr;
+    qcp_solve(m, gx, gy, rot != nullptr, &qr);
+    double r2 = n > 0 ? (gx + gy - 2.0 * qr.lambda) / n : 0.0;
+    if (rmsd) *rmsd = r2 > 0.0 ? std::sqrt(r2) : 0.0;
+    if (rot) for (int k = 0; k < 9; ++k) rot[k] = qr.rot[k];
+    if (lambda) *lambda = qr.lambda;
+    if (iters) *iters = qr.iters;
+    WMD_API_END
+}
+
+}  // extern "C"

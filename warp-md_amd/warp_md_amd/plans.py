@@ -1,0 +1,255 @@
+"""RdfPlan and RmsdPlan with warp-md's Python signatures, running on the
+gfx950 engine through its C ABI.
+
+RdfPlan mirrors PyRdfPlan (crates/traj-py/src/analysis.rs:2298-2399) over
+RdfPlan (crates/traj-engine/src/plans/rdf.rs:141-236, 653-1016); RmsdPlan
+mirrors PyRmsdPlan (crates/traj-py/src/io.rs:2617-2659) over RmsdPlan
+(crates/traj-engine/src/plans/rmsd/plan.rs:164-409).
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import _native
+from ._native import check, lib, ptr
+from .executor import Device, resolve_chunk_frames, select_frames, xyzw
+from .system import Selection, System, Trajectory
+
+
+def _parse_pbc(value: str) -> int:
+    # traj-py pack.rs:2665-2673 parse_pbc, plus the triclinic mode of this engine
+    table = {"orthorhombic": _native.PBC_ORTHO, "none": _native.PBC_NONE,
+             "triclinic": _native.PBC_TRICLINIC}
+    if value not in table:
+        raise RuntimeError("pbc must be 'orthorhombic', 'triclinic' or 'none'")
+    return table[value]
+
+
+def _parse_dimension(value: str) -> str:
+    # analysis.rs:2303-2309 parse_rdf_dimension
+    v = str(value).lower()
+    if v in ("3d", "xyz"):
+        return "3d"
+    if v == "xy":
+        return "xy"
+    raise ValueError("rdf dimension must be '3d' or 'xy'")
+
+
+def _indices(sel) -> np.ndarray:
+    if isinstance(sel, Selection):
+        return sel.indices
+    return np.ascontiguousarray(np.asarray(sel, dtype=np.int64).astype(np.uint32))
+
+
+class RdfPlan:
+    """Pair-distance histogram g(r) between two selections."""
+
+    def __init__(self, sel_a, sel_b, bins: int, r_max: float, pbc: str = "orthorhombic",
+                 center1=False, center2=False, byres1=False, byres2=False, bymol1=False,
+                 bymol2=False, no_intramol=False, mass_weighted=True, density=0.033456,
+                 volume=False, raw_rdf=False, intrdf=False, dimension="3d",
+                 number_density=False, edge_diagnostics=False):
+        self.pbc = _parse_pbc(pbc)
+        self.dimension = _parse_dimension(dimension)
+        self.sel_a = _indices(sel_a)
+        self.sel_b = _indices(sel_b)
+        self.bins = int(bins)
+        if self.bins < 0:
+            raise OverflowError("bins must be non-negative")
+        self.r_max = float(np.float32(r_max))
+        self.grouped = any([center1, center2, byres1, byres2, bymol1, bymol2, no_intramol])
+        self.mass_weighted = bool(mass_weighted)
+        self.density = float(density)
+        self.use_volume = bool(volume)
+        self.raw = bool(raw_rdf)
+        self.intrdf = bool(intrdf)
+        self.number_density = bool(number_density)
+        self.edge_diagnostics = bool(edge_diagnostics)
+        self.last_edge_pairs = 0
+        self.last_path = None
+        self._cell_k = 0
+
+    # rdf.rs:378-385 common_atom_count
+    def _common_count(self) -> int:
+        left = set(self.sel_a.tolist())
+        return int(sum(1 for i in self.sel_b.tolist() if i in left))
+
+    def set_tuning(self, cell_k: int = 0):
+        """Cell-list shape (cells per r_max, 1..3; 0 = automatic)."""
+        self._cell_k = int(cell_k)
+
+    def run(self, traj: Trajectory, system: System, chunk_frames: Optional[int] = None,
+            device: str = "auto", frame_indices: Optional[Sequence[int]] = None):
+        dev = Device.from_spec(device)
+        if traj.n_atoms() != system.n_atoms():
+            raise RuntimeError("mismatch: trajectory atom count does not match system")
+        # rdf.rs:663-667
+        if not np.isfinite(self.density) or self.density <= 0.0:
+            raise RuntimeError("parse error: rdf density must be finite and positive")
+        if self.grouped or self.dimension == "xy":
+            # rdf.rs:697-699 keeps grouped and XY modes on the CPU path
+            raise RuntimeError("unsupported: grouped and XY RDF modes run on warp-md's CPU path")
+        ctx = dev.context()
+        n_atoms = system.n_atoms()
+        for sel in (self.sel_a, self.sel_b):
+            if sel.size and int(sel.max()) >= n_atoms:
+                raise RuntimeError("invalid selection: atom index out of range")
+        flags = _native.RDF_EDGE_DIAGNOSTICS if self.edge_diagnostics else 0
+        h = ctypes.c_void_p()
+        check(lib().wmd_rdf_create(ctx.handle, ptr(self.sel_a), self.sel_a.size, ptr(self.sel_b),
+                                   self.sel_b.size, self.bins, ctypes.c_float(self.r_max),
+                                   self.pbc, flags, ctypes.byref(h)))
+        try:
+            if self._cell_k:
+                check(lib().wmd_rdf_set_tuning(h, self._cell_k, 0))
+            n_io = ctypes.c_size_t()
+            check(lib().wmd_rdf_io_selection(h, None, 0, ctypes.byref(n_io)))
+            io = np.zeros(n_io.value, np.uint32)
+            check(lib().wmd_rdf_io_selection(h, ptr(io), io.size, ctypes.byref(n_io)))
+            tr = select_frames(traj, frame_indices)
+            tr.reset()
+            frames = 0
+            volume_sum = 0.0
+            step = resolve_chunk_frames(chunk_frames, max(1, io.size))
+            need_box = self.pbc != _native.PBC_NONE or self.use_volume
+            while True:
+                chunk = tr.read_chunk(step)
+                if chunk is None:
+                    break
+                coords, boxes = chunk
+                nf = coords.shape[0]
+                if need_box:
+                    if boxes is None:
+                        raise RuntimeError(
+                            "mismatch: RDF requires orthorhombic box for PBC or volume normalization")
+                    for m in boxes:
+                        if self.pbc == _native.PBC_TRICLINIC:
+                            volume_sum += abs(float(np.linalg.det(m.astype(np.float64).reshape(3, 3))))
+                        else:
+                            off = m[[1, 2, 3, 5, 6, 7]]
+                            if np.any(off != 0) or np.any(m[[0, 4, 8]] <= 0):
+                                raise RuntimeError("mismatch: RDF requires orthorhombic box for PBC or volume normalization")
+                            # rdf_box_measure (rdf.rs:480-485): f64 product of the f32 lengths
+                            volume_sum += float(m[0]) * float(m[4]) * float(m[8])
+                frames += nf
+                sel_coords = xyzw(np.ascontiguousarray(coords[:, io, :]))
+                bx = boxes if boxes is not None else None
+                check(lib().wmd_rdf_accumulate_selected(h, ptr(sel_coords), nf, ptr(bx)))
+            counts = np.zeros(self.bins, np.uint64)
+            edge = np.zeros(1, np.uint64)
+            check(lib().wmd_rdf_read(h, ptr(counts), ptr(edge)))
+            path = ctypes.c_int()
+            check(lib().wmd_rdf_last_path(h, ctypes.byref(path), None))
+            self.last_path = {0: None, 1: "brute", 2: "cells"}[path.value]
+            self.last_edge_pairs = int(edge[0])
+        finally:
+            lib().wmd_rdf_destroy(h)
+        return finalize_rdf(counts, self.r_max, float(self.sel_a.size), float(self.sel_b.size),
+                            float(self._common_count()), frames, volume_sum, self.density,
+                            self.use_volume, self.raw, self.number_density, self.intrdf)
+
+
+def finalize_rdf(counts, r_max, n_a, n_b, common, frames, volume_sum, density, use_volume,
+                 raw, number_density, intrdf):
+    """RdfPlan::finalize (rdf.rs:943-1016) through the engine's host routine."""
+    counts = np.ascontiguousarray(counts, dtype=np.uint64)
+    bins = counts.size
+    r = np.zeros(bins, np.float32)
+    g = np.zeros(bins, np.float32)
+    it = np.zeros(bins, np.float32)
+    flags = ((_native.RDF_USE_VOLUME if use_volume else 0) | (_native.RDF_RAW if raw else 0)
+             | (_native.RDF_NUMBER_DENSITY if number_density else 0)
+             | (_native.RDF_INTEGRAL if intrdf else 0))
+    if bins:
+        check(lib().wmd_rdf_finalize(ptr(counts), bins, ctypes.c_float(r_max), n_a, n_b, common,
+                                     int(frames), float(volume_sum), float(density), flags,
+                                     ptr(r), ptr(g), ptr(it)))
+    if intrdf:
+        return r, g, counts, it
+    return r, g, counts
+
+
+def _parse_reference(value: str) -> str:
+    # traj-py pack.rs:2627-2635
+    if value not in ("topology", "frame0"):
+        raise RuntimeError("reference must be 'topology' or 'frame0'")
+    return value
+
+
+class RmsdPlan:
+    """Per-frame RMSD to a reference after optimal superposition.
+
+    ``fit_selection`` (an addition of this engine) fits on one selection and
+    takes the RMSD over ``selection`` -- the composition of kabsch_rotation
+    and the residual of rms.rs:643-676.  ``None`` keeps RmsdPlan's behaviour
+    (fit and RMSD over the same atoms).
+    """
+
+    def __init__(self, selection, reference: str = "topology", align: bool = True,
+                 fit_selection=None):
+        self.selection = _indices(selection)
+        self.reference_mode = _parse_reference(reference)
+        self.align = bool(align)
+        self.fit_selection = None if fit_selection is None else _indices(fit_selection)
+        self.last_error_bound = None
+
+    def run(self, traj: Trajectory, system: System, chunk_frames: Optional[int] = None,
+            device: str = "auto", frame_indices: Optional[Sequence[int]] = None):
+        dev = Device.from_spec(device)
+        if traj.n_atoms() != system.n_atoms():
+            raise RuntimeError("mismatch: trajectory atom count does not match system")
+        if self.fit_selection is not None and not self.align:
+            raise RuntimeError("unsupported: fit_selection requires align=True")
+        ctx = dev.context()
+        n_atoms = system.n_atoms()
+        fit = self.fit_selection if self.fit_selection is not None else self.selection
+        for sel in (self.selection, fit):
+            if sel.size and int(sel.max()) >= n_atoms:
+                raise RuntimeError("invalid selection: atom index out of range")
+        # the executor reads selected chunks (rmsd/plan.rs:224-230): gather the
+        # union of the selections and express both in its local indices
+        union, inv = np.unique(np.concatenate([self.selection, fit]).astype(np.int64),
+                               return_inverse=True)
+        loc_rmsd = inv[: self.selection.size].astype(np.uint32)
+        loc_fit = inv[self.selection.size:].astype(np.uint32)
+        tr = select_frames(traj, frame_indices)
+        tr.reset()
+        if self.reference_mode == "topology":
+            if system.positions0 is None:
+                raise RuntimeError("mismatch: no topology reference coords")
+            ref = system.positions0[union]
+        else:
+            if tr.n_frames() == 0:
+                return np.zeros(0, np.float32)
+            ref = tr.coords[0][union]  # rmsd/plan.rs:336-354 first frame seen
+        ref4 = xyzw(np.ascontiguousarray(ref, dtype=np.float32)[None])[0]
+        h = ctypes.c_void_p()
+        if self.fit_selection is None:
+            check(lib().wmd_rmsd_create(ctx.handle, ptr(loc_rmsd), loc_rmsd.size, None, 0,
+                                        ptr(ref4), union.size, int(self.align), ctypes.byref(h)))
+        else:
+            check(lib().wmd_rmsd_create(ctx.handle, ptr(loc_fit), loc_fit.size, ptr(loc_rmsd),
+                                        loc_rmsd.size, ptr(ref4), union.size, 1, ctypes.byref(h)))
+        try:
+            out = np.zeros(tr.n_frames(), np.float32)
+            err = np.zeros(tr.n_frames(), np.float64)
+            step = resolve_chunk_frames(chunk_frames, max(1, union.size))
+            f0 = 0
+            while True:
+                chunk = tr.read_chunk(step)
+                if chunk is None:
+                    break
+                coords, _ = chunk
+                nf = coords.shape[0]
+                sel_coords = xyzw(np.ascontiguousarray(coords[:, union, :]))
+                o = out[f0:f0 + nf]
+                e = err[f0:f0 + nf]
+                check(lib().wmd_rmsd_run(h, ptr(sel_coords), union.size, nf, ptr(o), ptr(e)))
+                f0 += nf
+        finally:
+            lib().wmd_rmsd_destroy(h)
+        self.last_error_bound = err
+        return out
