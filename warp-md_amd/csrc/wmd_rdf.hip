@@ -9,31 +9,43 @@
 // tests r < r_max and bins (r / bin_width) as usize.  That map s -> bin is a
 // monotone step function of the f32 value s, so the host precomputes its exact
 // breakpoints T[k] = min{ s : bin(s) >= k } by bisection over f32 bit patterns
-// with IEEE sqrtf / division.  A pair is counted iff s < T[bins]; its bin comes
-// from a fast v_sqrt estimate corrected by one comparison against T on each
-// side.  Counts are therefore bit-identical to the reference with neither the
-// IEEE divide nor the IEEE square root in the pair loop.
+// with IEEE sqrtf / division.  A pair is counted iff s < T[bins].  Its bin is
+// read from a table indexed by the high bits of s (built so that every bucket
+// holds at most one breakpoint) and corrected by one comparison with T; when
+// the bins are too fine for such a table, a v_sqrt estimate corrected on both
+// sides is used instead.  Counts are therefore bit-identical to the reference
+// with neither the IEEE divide nor the IEEE square root in the pair loop.
 //
-// Minimum image.  When r_max < half the smallest plane spacing (less 1e-5
-// relative), every in-range pair's image index round(dx/L) is far from a .5
-// tie, so rint(dx * (1/L)) gives the same integer as the reference's
-// roundf(dx / L) and the wrapped dx (dx - n*L, not fused) is bit-identical; a
-// pair where the two could differ lies near L/2 > r_max and is not counted by
-// either.  Otherwise the EXACT variant evaluates roundf(dx / L) with IEEE
-// division.  The triclinic expression is the reference's f32 device formula
-// wrap_triclinic (part1.cu:12-27), without FMA contraction.
+// Minimum image.  The reference wraps each component with round(dx / L)
+// (roundf).  Three exact ways are used:
+//  * shift: with a periodic cell grid of >= 2k+3 cells per side, every pair the
+//    stencil visits has round(dx / L) = m_j - w - m_i exactly, where m is the
+//    atom's image index floor(x / L) and w the stencil's periodic wrap, because
+//    the image separation is < (k+1)/nc < 1/2 box.  The per-j value n*L (the
+//    reference's own f32 product round(dx/L)*L) is staged in LDS and the pair
+//    costs dx = (xj - xi) - S: the reference's two f32 roundings, nothing else.
+//    Triclinic boxes stage the fractional image n and evaluate the reference's
+//    f32 device formula wrap_triclinic (part1.cu:12-27) with f - n in place of
+//    f - round(f), skipping the zero entries of a lower-triangular box (adding
+//    or multiplying by an exact zero changes at most the sign of a zero).
+//  * rint: when r_max < half the smallest plane spacing (less 1e-5 relative),
+//    rint(dx * (1/L)) equals roundf(dx / L) for every pair in range; a pair
+//    where they could differ lies near L/2 > r_max and is counted by neither.
+//  * exact: IEEE division and roundf, for small boxes.
 //
 // Enumeration.  Counts are integer sums, so any enumeration that visits every
-// ordered pair's multiset once reproduces the reference loop.  Two pair kernels:
+// ordered pair's multiset once reproduces the reference loop.  Two kernels:
 //  * brute force over 256x256 atom tiles (upper triangle of tile pairs; diagonal
 //    tiles as ordered pairs, off-diagonal as unordered pairs of weight 2);
 //  * a periodic cell list, cells >= (r_max + 1e-2 A) / k wide along each plane
 //    spacing, home cell x half stencil of (2k+1)^3 neighbour cells (unordered
 //    pairs, weight 2) plus the home cell itself (ordered pairs, weight 1).
-// Both are persistent: each workgroup keeps wave-private u32 sub-histograms in
-// LDS for all the units it processes and merges them into the u64 global
-// counts once at the end.  Units are assigned in contiguous ranges per XCD so
-// that an XCD's L2 holds the frames it works on.
+//    Neighbour atoms farther than r_max + 0.05 A from the home atoms' bounding
+//    box are dropped while staging (their pairs are provably out of range).
+// Both are persistent: each workgroup keeps u32 sub-histograms in LDS for all
+// the units it processes and merges them into the u64 global counts when a
+// uniform bound says a slot could wrap, and at the end.  Units are assigned in
+// contiguous ranges per XCD so that an XCD's L2 holds the frames it works on.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -48,9 +60,9 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kWaves = kBlock / 64;
 constexpr int kTile = 256;        // brute-force tile
-constexpr int kMaxJ = 1024;       // cell-list j atoms staged per pass
-constexpr int kMaxStencil = 172;  // (7^3 - 1) / 2 + 1 for k = 3
+constexpr int kMaxTab = 8192;     // s-bits bin table entries (u16)
 constexpr uint32_t kPad = 0xFFFFFFFFu;
+constexpr int kImageLimit = 8;  // |image index| beyond which a frame uses the rint path
 
 struct FrameBox {
     float lx, ly, lz, ilx, ily, ilz;
@@ -62,16 +74,118 @@ struct HistParams {
     int bins;
     int hist_copies;  // 4 = one sub-histogram per wave, 1 = shared
     float cut;        // counted iff s < cut
-    float inv_w;      // approximate 1 / bin_width for the bin estimate
+    float inv_w;      // approximate 1 / bin_width for the sqrt estimate
     const float *thr; // [bins + 1]
     const float *elo; // [bins + 1] edge diagnostics (EDGE variants)
     const float *ehi;
+    const unsigned short *tab;  // [tab_n] s-bits bin table (TABLE variants)
+    int tab_n, tab_sh, tab_base;
     unsigned long long *counts;  // [bins]
     unsigned long long *edge;    // [1]
 };
 
+// LDS layout shared by the pair kernels:
+//   thr[bins+1] | elo, ehi (EDGE) | tab[tab_n] (TABLE) | hist[copies][bins] | kernel-specific
+struct LdsHist {
+    float *thr, *elo, *ehi;
+    unsigned short *tab;
+    unsigned *hist;
+    char *rest;
+};
+
+__host__ __device__ inline size_t align16(size_t x) { return (x + 15) / 16 * 16; }
+
+__host__ __device__ inline size_t lds_hist_size(int bins, int copies, bool edge, int tab_n) {
+    size_t nb1 = (size_t)bins + 1;
+    return align16(nb1 * 4 * (edge ? 3 : 1)) + align16((size_t)tab_n * 2) +
+           align16((size_t)copies * bins * 4);
+}
+
+__device__ __forceinline__ LdsHist lds_hist_layout(char *base, const HistParams &p, bool edge,
+                                                   bool table) {
+    LdsHist L;
+    size_t nb1 = (size_t)p.bins + 1;
+    L.thr = (float *)base;
+    L.elo = L.thr + nb1;
+    L.ehi = L.elo + nb1;
+    size_t off = align16(nb1 * 4 * (edge ? 3 : 1));
+    L.tab = (unsigned short *)(base + off);
+    off += align16((size_t)(table ? p.tab_n : 0) * 2);
+    L.hist = (unsigned *)(base + off);
+    off += align16((size_t)p.hist_copies * p.bins * 4);
+    L.rest = base + off;
+    return L;
+}
+
+template <bool EDGE, bool TABLE>
+__device__ void lds_hist_init(const LdsHist &L, const HistParams &p) {
+    for (int i = threadIdx.x; i <= p.bins; i += kBlock) {
+        L.thr[i] = p.thr[i];
+        if constexpr (EDGE) {
+            L.elo[i] = p.elo[i];
+            L.ehi[i] = p.ehi[i];
+        }
+    }
+    if constexpr (TABLE)
+        for (int i = threadIdx.x; i < p.tab_n; i += kBlock) L.tab[i] = p.tab[i];
+    for (int i = threadIdx.x; i < p.hist_copies * p.bins; i += kBlock) L.hist[i] = 0u;
+}
+
+// Merges the LDS sub-histograms into the global u64 counts (and clears them).
+template <bool EDGE>
+__device__ void lds_hist_flush(const LdsHist &L, const HistParams &p, unsigned &edge_local) {
+    __syncthreads();
+    for (int i = threadIdx.x; i < p.bins; i += kBlock) {
+        unsigned long long sum = 0;
+        for (int c = 0; c < p.hist_copies; ++c) {
+            sum += L.hist[c * p.bins + i];
+            L.hist[c * p.bins + i] = 0u;
+        }
+        if (sum) atomicAdd(&p.counts[i], sum);
+    }
+    if constexpr (EDGE) {
+        if (edge_local) atomicAdd(p.edge, (unsigned long long)edge_local);
+        edge_local = 0;
+    }
+    __syncthreads();
+}
+
+// Exact bin of a counted pair (s < cut).
+template <bool TABLE>
+__device__ __forceinline__ int exact_bin(float s, const LdsHist &L, const HistParams &p) {
+    if constexpr (TABLE) {
+        // bucket of s: saturating (bits - base) >> sh; s < cut keeps it below tab_n
+        const unsigned idx = __builtin_elementwise_sub_sat(__float_as_uint(s), (unsigned)p.tab_base) >> p.tab_sh;
+        int b = L.tab[idx];
+        b += (int)(s >= L.thr[b + 1]);
+        return b;
+    } else {
+        float r_est = __builtin_amdgcn_sqrtf(s);
+        int b = (int)(r_est * p.inv_w);
+        b = b < p.bins - 1 ? b : p.bins - 1;
+        float lo = L.thr[b];
+        float hi = L.thr[b + 1];
+        b += (int)(s >= hi) - (int)(s < lo);
+        return b;
+    }
+}
+
+template <bool EDGE, bool TABLE>
+__device__ __forceinline__ void count_pair(float s, unsigned w, const HistParams &p,
+                                           const LdsHist &L, unsigned *hist,
+                                           unsigned &edge_local) {
+    if (s < p.cut) {
+        int b = exact_bin<TABLE>(s, L, p);
+        atomicAdd(&hist[b], w);
+        if constexpr (EDGE) {
+            bool near = (s >= L.elo[b] && s < L.ehi[b]) || (s >= L.elo[b + 1] && s < L.ehi[b + 1]);
+            edge_local += near ? w : 0u;
+        }
+    }
+}
+
 // -------------------------------------------------------------------------
-// The per-pair squared distance, reference operation order.
+// The per-pair squared distance, reference operation order (rint / exact wrap).
 // -------------------------------------------------------------------------
 template <int PBC, bool EXACT>
 __device__ __forceinline__ float pair_s(float xi, float yi, float zi, float xj, float yj, float zj,
@@ -109,83 +223,45 @@ __device__ __forceinline__ float pair_s(float xi, float yi, float zi, float xj, 
     return dx * dx + dy * dy + dz * dz;
 }
 
-// Exact bin of a counted pair (s < cut): estimate then one-step correction.
-__device__ __forceinline__ int exact_bin(float s, const float *thr_lds, float inv_w, int bins) {
-    float r_est = __builtin_amdgcn_sqrtf(s);
-    int b = (int)(r_est * inv_w);
-    b = b < bins - 1 ? b : bins - 1;
-    float lo = thr_lds[b];
-    float hi = thr_lds[b + 1];
-    b += (int)(s >= hi) - (int)(s < lo);
-    return b;
-}
-
-template <bool EDGE>
-__device__ __forceinline__ void count_pair(float s, unsigned w, const HistParams &p,
-                                           const float *thr_lds, const float *elo_lds,
-                                           const float *ehi_lds, unsigned *hist,
-                                           unsigned &edge_local) {
-    if (s < p.cut) {
-        int b = exact_bin(s, thr_lds, p.inv_w, p.bins);
-        atomicAdd(&hist[b], w);
-        if constexpr (EDGE) {
-            bool near = (s >= elo_lds[b] && s < ehi_lds[b]) || (s >= elo_lds[b + 1] && s < ehi_lds[b + 1]);
-            edge_local += near ? w : 0u;
+// Shift path: the image is known; (sx, sy, sz) hold n*L (orthorhombic) or n
+// (triclinic fractional).  LT: lower-triangular box (r[1] = r[2] = r[5] = 0 and
+// c[1] = c[2] = c[5] = 0), whose zero terms are skipped.
+template <int PBC, bool LT>
+__device__ __forceinline__ float pair_s_shift(float xi, float yi, float zi, float xj, float yj,
+                                              float zj, float sx, float sy, float sz,
+                                              const FrameBox &b) {
+    float dx = xj - xi;
+    float dy = yj - yi;
+    float dz = zj - zi;
+    if constexpr (PBC == WMD_PBC_ORTHO) {
+        dx -= sx;
+        dy -= sy;
+        dz -= sz;
+    } else {
+        float f0, f1, f2;
+        if constexpr (LT) {
+            f0 = b.r[0] * dx + b.r[3] * dy + b.r[6] * dz;
+            f1 = b.r[4] * dy + b.r[7] * dz;
+            f2 = b.r[8] * dz;
+        } else {
+            f0 = b.r[0] * dx + b.r[3] * dy + b.r[6] * dz;
+            f1 = b.r[1] * dx + b.r[4] * dy + b.r[7] * dz;
+            f2 = b.r[2] * dx + b.r[5] * dy + b.r[8] * dz;
+        }
+        f0 -= sx;
+        f1 -= sy;
+        f2 -= sz;
+        if constexpr (LT) {
+            dx = f0 * b.c[0] + f1 * b.c[3] + f2 * b.c[6];
+            dy = f1 * b.c[4] + f2 * b.c[7];
+            dz = f2 * b.c[8];
+        } else {
+            dx = f0 * b.c[0] + f1 * b.c[3] + f2 * b.c[6];
+            dy = f0 * b.c[1] + f1 * b.c[4] + f2 * b.c[7];
+            dz = f0 * b.c[2] + f1 * b.c[5] + f2 * b.c[8];
         }
     }
-}
-
-// LDS layout shared by the pair kernels:
-//   thr[bins+1] | elo[bins+1] | ehi[bins+1] (EDGE) | hist[copies][bins] | kernel-specific
-struct LdsHist {
-    float *thr, *elo, *ehi;
-    unsigned *hist;
-    char *rest;
-};
-
-__device__ __forceinline__ LdsHist lds_hist_layout(char *base, const HistParams &p, bool edge) {
-    LdsHist L;
-    size_t nb1 = (size_t)p.bins + 1;
-    L.thr = (float *)base;
-    L.elo = L.thr + nb1;
-    L.ehi = L.elo + nb1;
-    size_t nf = edge ? 3 * nb1 : nb1;
-    size_t off = ((nf * 4 + 15) / 16) * 16;
-    L.hist = (unsigned *)(base + off);
-    size_t hb = (size_t)p.hist_copies * p.bins * 4;
-    L.rest = base + off + ((hb + 15) / 16) * 16;
-    return L;
-}
-
-template <bool EDGE>
-__device__ void lds_hist_init(const LdsHist &L, const HistParams &p) {
-    for (int i = threadIdx.x; i <= p.bins; i += kBlock) {
-        L.thr[i] = p.thr[i];
-        if constexpr (EDGE) {
-            L.elo[i] = p.elo[i];
-            L.ehi[i] = p.ehi[i];
-        }
-    }
-    for (int i = threadIdx.x; i < p.hist_copies * p.bins; i += kBlock) L.hist[i] = 0u;
-}
-
-// Merges the LDS sub-histograms into the global u64 counts (and clears them).
-template <bool EDGE>
-__device__ void lds_hist_flush(const LdsHist &L, const HistParams &p, unsigned &edge_local) {
-    __syncthreads();
-    for (int i = threadIdx.x; i < p.bins; i += kBlock) {
-        unsigned long long sum = 0;
-        for (int c = 0; c < p.hist_copies; ++c) {
-            sum += L.hist[c * p.bins + i];
-            L.hist[c * p.bins + i] = 0u;
-        }
-        if (sum) atomicAdd(&p.counts[i], sum);
-    }
-    if constexpr (EDGE) {
-        if (edge_local) atomicAdd(p.edge, (unsigned long long)edge_local);
-        edge_local = 0;
-    }
-    __syncthreads();
+    return dx * dx + dy * dy + dz * dz;
 }
 
 // XCD-aware persistent schedule: WG g runs on XCD g % 8 (round-robin dispatch);
@@ -210,19 +286,19 @@ __device__ __forceinline__ UnitRange xcd_units(long long n_units, int n_xcd) {
 // soa: [F][3][stride]; weights (WEIGHTED): packed mA | mB << 8 per IO atom.
 // tile_pairs: (it, jt) with it <= jt, upper-triangle order.
 // -------------------------------------------------------------------------
-template <int PBC, bool EXACT, bool WEIGHTED, bool EDGE>
+template <int PBC, bool EXACT, bool WEIGHTED, bool EDGE, bool TABLE>
 __global__ __launch_bounds__(kBlock) void rdf_brute_kernel(
     const float *__restrict__ soa, long long frame_stride, int stride, int n, int n_frames,
     const FrameBox *__restrict__ boxes, const unsigned *__restrict__ wts,
     const int2 *__restrict__ tile_pairs, int n_tile_pairs, long long frame0, HistParams p,
     int n_xcd, unsigned w_max) {
     extern __shared__ __attribute__((aligned(16))) char lds[];
-    LdsHist L = lds_hist_layout(lds, p, EDGE);
+    LdsHist L = lds_hist_layout(lds, p, EDGE, TABLE);
     float *jx = (float *)L.rest;
     float *jy = jx + kTile;
     float *jz = jy + kTile;
     unsigned *jw = (unsigned *)(jz + kTile);
-    lds_hist_init<EDGE>(L, p);
+    lds_hist_init<EDGE, TABLE>(L, p);
     unsigned edge_local = 0;
     const int wave = threadIdx.x >> 6;
     unsigned *hist = L.hist + (p.hist_copies == 1 ? 0 : wave) * p.bins;
@@ -264,7 +340,7 @@ __global__ __launch_bounds__(kBlock) void rdf_brute_kernel(
                 if (jj == (int)threadIdx.x) s = nanv;
                 unsigned w = 1u;
                 if constexpr (WEIGHTED) w = (wi & 0xFFu) * (jw[jj] >> 8);
-                count_pair<EDGE>(s, w, p, L.thr, L.elo, L.ehi, hist, edge_local);
+                count_pair<EDGE, TABLE>(s, w, p, L, hist, edge_local);
             }
         } else {
             // unordered pairs: both orders at once
@@ -275,7 +351,7 @@ __global__ __launch_bounds__(kBlock) void rdf_brute_kernel(
                     unsigned wj = jw[jj];
                     w = (wi & 0xFFu) * (wj >> 8) + (wj & 0xFFu) * (wi >> 8);
                 }
-                count_pair<EDGE>(s, w, p, L.thr, L.elo, L.ehi, hist, edge_local);
+                count_pair<EDGE, TABLE>(s, w, p, L, hist, edge_local);
             }
         }
         // at most kTile pairs per thread column and sub-histogram copy per unit
@@ -290,28 +366,21 @@ __global__ __launch_bounds__(kBlock) void rdf_brute_kernel(
 
 // -------------------------------------------------------------------------
 // Cell list construction.
-// cell key = (cz * ncy + cy) * ncx + cx from the wrapped fractional position.
+// cell key = (cz * ncy + cy) * ncx + cx from the wrapped fractional position;
+// meta word of a sorted atom: image index (mx, my, mz) as 3 x int8 in bits
+// 0..23, selection multiplicities mA in bits 24..27 and mB in bits 28..31.
 // -------------------------------------------------------------------------
 template <int PBC>
-__device__ __forceinline__ int cell_of_pos(float x, float y, float z, const FrameBox &b, int ncx,
-                                           int ncy, int ncz) {
-    float f0, f1, f2;
+__device__ __forceinline__ void frac_of(float x, float y, float z, const FrameBox &b, float *f) {
     if constexpr (PBC == WMD_PBC_ORTHO) {
-        f0 = x * b.ilx;
-        f1 = y * b.ily;
-        f2 = z * b.ilz;
+        f[0] = x * b.ilx;
+        f[1] = y * b.ily;
+        f[2] = z * b.ilz;
     } else {
-        f0 = b.r[0] * x + b.r[3] * y + b.r[6] * z;
-        f1 = b.r[1] * x + b.r[4] * y + b.r[7] * z;
-        f2 = b.r[2] * x + b.r[5] * y + b.r[8] * z;
+        f[0] = b.r[0] * x + b.r[3] * y + b.r[6] * z;
+        f[1] = b.r[1] * x + b.r[4] * y + b.r[7] * z;
+        f[2] = b.r[2] * x + b.r[5] * y + b.r[8] * z;
     }
-    f0 -= floorf(f0);
-    f1 -= floorf(f1);
-    f2 -= floorf(f2);
-    int cx = min(max((int)(f0 * (float)ncx), 0), ncx - 1);
-    int cy = min(max((int)(f1 * (float)ncy), 0), ncy - 1);
-    int cz = min(max((int)(f2 * (float)ncz), 0), ncz - 1);
-    return (cz * ncy + cy) * ncx + cx;
 }
 
 __global__ void fill_u32_kernel(unsigned *p, long long n, unsigned v) {
@@ -329,19 +398,39 @@ template <int PBC>
 __global__ __launch_bounds__(kBlock) void cell_assign_kernel(
     const float *__restrict__ soa, long long frame_stride, int stride, int n, int n_frames,
     long long frame0, const FrameBox *__restrict__ boxes, int ncx, int ncy, int ncz,
-    unsigned *__restrict__ cell_of, unsigned *__restrict__ cell_slot, unsigned *__restrict__ cell_count) {
+    unsigned *__restrict__ cell_of, unsigned *__restrict__ cell_slot,
+    unsigned *__restrict__ cell_count, unsigned *__restrict__ image_of,
+    unsigned *__restrict__ frame_flag) {
     const int ncell = ncx * ncy * ncz;
+    const int nc[3] = {ncx, ncy, ncz};
     long long total = (long long)n * n_frames;
     for (long long t = (long long)blockIdx.x * kBlock + threadIdx.x; t < total;
          t += (long long)gridDim.x * kBlock) {
         int f = (int)(t / n);
         int i = (int)(t - (long long)f * n);
         const float *fx = soa + (frame0 + f) * frame_stride;
-        float x = fx[i], y = fx[stride + i], z = fx[2 * stride + i];
-        int c = cell_of_pos<PBC>(x, y, z, boxes[f], ncx, ncy, ncz);
-        unsigned slot = atomicAdd(&cell_count[(long long)f * ncell + c], 1u);
-        cell_of[t] = (unsigned)c;
+        float fr[3];
+        frac_of<PBC>(fx[i], fx[stride + i], fx[2 * stride + i], boxes[f], fr);
+        int c[3];
+        unsigned img = 0;
+        bool far_out = false;
+        for (int d = 0; d < 3; ++d) {
+            float fl = floorf(fr[d]);
+            float w = fr[d] - fl;  // in [0, 1) (NaN for non-finite input)
+            int cc = (int)(w * (float)nc[d]);
+            c[d] = min(max(cc, 0), nc[d] - 1);
+            // image index; non-finite or far-away atoms force the rint path
+            bool ok = fl >= -(float)kImageLimit && fl <= (float)kImageLimit;
+            int m = ok ? (int)fl : 0;
+            far_out |= !ok;
+            img |= ((unsigned)(m & 0xFF)) << (8 * d);
+        }
+        if (far_out) atomicOr(&frame_flag[f], 1u);
+        int cid = (c[2] * ncy + c[1]) * ncx + c[0];
+        unsigned slot = atomicAdd(&cell_count[(long long)f * ncell + cid], 1u);
+        cell_of[t] = (unsigned)cid;
         cell_slot[t] = slot;
+        image_of[t] = img;
     }
 }
 
@@ -375,12 +464,12 @@ __global__ __launch_bounds__(kBlock) void cell_scan_kernel(const unsigned *__res
     if (threadIdx.x == kBlock - 1) s[ncell] = part[kBlock - 1];
 }
 
-// sorted[f][start[c] + slot] = (x, y, z, packed weights)
+// sorted[f][start[c] + slot] = (x, y, z, meta)
 __global__ __launch_bounds__(kBlock) void cell_scatter_kernel(
     const float *__restrict__ soa, long long frame_stride, int stride, int n, int n_frames,
     long long frame0, const unsigned *__restrict__ cell_of, const unsigned *__restrict__ cell_slot,
-    const unsigned *__restrict__ start, int ncell, const unsigned *__restrict__ wts,
-    float4 *__restrict__ sorted) {
+    const unsigned *__restrict__ image_of, const unsigned *__restrict__ start, int ncell,
+    const unsigned *__restrict__ wts, float4 *__restrict__ sorted) {
     long long total = (long long)n * n_frames;
     for (long long t = (long long)blockIdx.x * kBlock + threadIdx.x; t < total;
          t += (long long)gridDim.x * kBlock) {
@@ -388,48 +477,184 @@ __global__ __launch_bounds__(kBlock) void cell_scatter_kernel(
         int i = (int)(t - (long long)f * n);
         const float *fx = soa + (frame0 + f) * frame_stride;
         unsigned pos = start[(long long)f * (ncell + 1) + cell_of[t]] + cell_slot[t];
+        unsigned w = wts ? wts[i] : 0x0101u;  // mA | mB << 8
+        unsigned meta = (image_of[t] & 0xFFFFFFu) | ((w & 0xFu) << 24) | (((w >> 8) & 0xFu) << 28);
         float4 v;
         v.x = fx[i];
         v.y = fx[stride + i];
         v.z = fx[2 * stride + i];
-        v.w = __uint_as_float(wts ? wts[i] : 0x0101u);
+        v.w = __uint_as_float(meta);
         sorted[(long long)f * n + pos] = v;
     }
 }
 
+__device__ __forceinline__ int sext8(unsigned v) { return (int)(signed char)(v & 0xFFu); }
+
 // -------------------------------------------------------------------------
-// Cell-list pair kernel.  Unit = (frame, home cell).  The j list is the home
-// cell followed by the half-stencil neighbour cells, staged into LDS in
-// passes of kMaxJ atoms.  Lanes: i = t % I over the home atoms (I = min(n_home,
-// 256)), j phase = t / I.
+// Cell-list pair kernel.  One wave per unit (frame, home cell); no block-level
+// synchronisation between units.  The unit's j stream is the home atoms, then
+// the neighbour cells of the half stencil taken as rows along x: for each
+// (dy, dz) with dz > 0, or dz = 0 and dy > 0, the cells hx-k..hx+k, and in the
+// home row the cells hx+1..hx+k.  Such a row is one contiguous run of the
+// cell-sorted atoms (two runs where it wraps in x).  Lanes load 64 stream
+// atoms at a time, drop neighbours farther than the cull radius from the home
+// atoms' bounding box (shift path), and append the rest to a per-wave LDS ring;
+// every full ring chunk of 64 j atoms is paired with all home atoms i, which
+// are uniform across the wave and read with scalar loads.  Home atoms in the
+// stream pair only with home atoms of lower index (unordered pairs, weight 2).
+// The histogram is a u64 LDS array shared by the workgroup's waves.
 // -------------------------------------------------------------------------
-template <int PBC, bool EXACT, bool WEIGHTED, bool EDGE>
-__global__ __launch_bounds__(kBlock) void rdf_cell_kernel(
-    const float4 *__restrict__ sorted, const unsigned *__restrict__ start, int n, int n_frames,
-    const FrameBox *__restrict__ boxes, int ncx, int ncy, int ncz,
-    const int4 *__restrict__ offsets, int n_off, HistParams p, int n_xcd, unsigned w_max) {
+constexpr int kCellBlock = 512;
+constexpr int kCellWaves = kCellBlock / 64;
+constexpr int kRing = 128;  // per-wave ring entries (float4 each)
+
+struct CellLds {
+    unsigned long long *hist;
+    float *thr, *elo, *ehi;
+    unsigned short *tab;
+    float4 *ring;  // [kCellWaves][kRing]
+};
+
+__host__ __device__ inline size_t cell_lds_bytes(int bins, bool edge, int tab_n) {
+    return align16((size_t)bins * 8) + align16(((size_t)bins + 1) * 4 * (edge ? 3 : 1)) +
+           align16((size_t)tab_n * 2) + (size_t)kCellWaves * kRing * 16;
+}
+
+__device__ __forceinline__ CellLds cell_lds_layout(char *base, const HistParams &p, bool edge) {
+    CellLds L;
+    const size_t nb1 = (size_t)p.bins + 1;
+    size_t off = 0;
+    L.hist = (unsigned long long *)base;
+    off += align16((size_t)p.bins * 8);
+    L.thr = (float *)(base + off);
+    L.elo = L.thr + nb1;
+    L.ehi = L.elo + nb1;
+    off += align16(nb1 * 4 * (edge ? 3 : 1));
+    L.tab = (unsigned short *)(base + off);
+    off += align16((size_t)p.tab_n * 2);
+    L.ring = (float4 *)(base + off);
+    return L;
+}
+
+// Exact bin from LDS copies of the thresholds / table (s < cut).
+template <bool TABLE>
+__device__ __forceinline__ int exact_bin_lds(float s, const float *thr, const unsigned short *tab,
+                                             const HistParams &p) {
+    if constexpr (TABLE) {
+        const unsigned idx =
+            __builtin_elementwise_sub_sat(__float_as_uint(s), (unsigned)p.tab_base) >> p.tab_sh;
+        int b = tab[idx];
+        b += (int)(s >= thr[b + 1]);
+        return b;
+    } else {
+        float r_est = __builtin_amdgcn_sqrtf(s);
+        int b = (int)(r_est * p.inv_w);
+        b = b < p.bins - 1 ? b : p.bins - 1;
+        b += (int)(s >= thr[b + 1]) - (int)(s < thr[b]);
+        return b;
+    }
+}
+
+__device__ __forceinline__ float wave_min(float v) {
+    for (int off = 32; off > 0; off >>= 1) v = fminf(v, __shfl_xor(v, off, 64));
+    return v;
+}
+__device__ __forceinline__ float wave_max(float v) {
+    for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+// Number of set bits of m below this lane.
+__device__ __forceinline__ int lanes_below(unsigned long long m) {
+    return (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
+                                          __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+}
+
+// Pairs every j of one ring chunk with every home atom i.
+// SHIFT: j carries its image shift n (3 x int8 in the meta word); otherwise
+// the rint wrap.  HOME: the chunk holds home atoms (stream positions below
+// n_home), which pair only with home atoms of lower index.
+template <int PBC, bool LT, bool WEIGHTED, bool EDGE, bool TABLE, bool SHIFT, bool HOME>
+__device__ __forceinline__ void cell_chunk(const float4 *__restrict__ hp, int n_home, float xj,
+                                           float yj, float zj, unsigned mj, int rank,
+                                           const FrameBox &b, const HistParams &p,
+                                           const CellLds &L, unsigned long long &edge_local) {
+    float sx = 0.f, sy = 0.f, sz = 0.f;
+    if constexpr (SHIFT) {
+        sx = (float)sext8(mj);
+        sy = (float)sext8(mj >> 8);
+        sz = (float)sext8(mj >> 16);
+        if constexpr (PBC == WMD_PBC_ORTHO) {
+            sx *= b.lx;  // the reference's f32 product round(dx/L) * L
+            sy *= b.ly;
+            sz *= b.lz;
+        }
+    }
+    const unsigned mAj = (mj >> 24) & 0xFu, mBj = mj >> 28;
+    for (int i = 0; i < n_home; ++i) {
+        const float4 pi = hp[i];
+        float s;
+        if constexpr (SHIFT)
+            s = pair_s_shift<PBC, LT>(pi.x, pi.y, pi.z, xj, yj, zj, sx, sy, sz, b);
+        else
+            s = pair_s<PBC, false>(pi.x, pi.y, pi.z, xj, yj, zj, b);
+        if constexpr (HOME) {
+            if (i >= rank) s = __builtin_nanf("");
+        }
+        if (s < p.cut) {
+            unsigned w = 2u;
+            if constexpr (WEIGHTED) {
+                const unsigned mi = __float_as_uint(pi.w);
+                w = ((mi >> 24) & 0xFu) * mBj + (mi >> 28) * mAj;
+            }
+            const int bin = exact_bin_lds<TABLE>(s, L.thr, L.tab, p);
+            atomicAdd(&L.hist[bin], (unsigned long long)w);
+            if constexpr (EDGE) {
+                bool near = (s >= L.elo[bin] && s < L.ehi[bin]) ||
+                            (s >= L.elo[bin + 1] && s < L.ehi[bin + 1]);
+                edge_local += near ? w : 0u;
+            }
+        }
+    }
+}
+
+template <int PBC, bool LT, bool WEIGHTED, bool EDGE, bool TABLE>
+__global__ __launch_bounds__(kCellBlock) void rdf_cell_kernel(
+    const float4 *__restrict__ sorted, const unsigned *__restrict__ start,
+    const unsigned *__restrict__ frame_flag, int n, int n_frames,
+    const FrameBox *__restrict__ boxes, int ncx, int ncy, int ncz, int k, int shift_ok,
+    float cull2, HistParams p, int n_xcd) {
     extern __shared__ __attribute__((aligned(16))) char lds[];
-    __shared__ int wave_tot[kWaves];
-    LdsHist L = lds_hist_layout(lds, p, EDGE);
-    float4 *jpos = (float4 *)L.rest;
-    int *c_start = (int *)(jpos + kMaxJ);
-    int *c_pref = c_start + kMaxStencil;  // kMaxStencil + 1 entries
-    lds_hist_init<EDGE>(L, p);
-    unsigned edge_local = 0;
-    const int wave = threadIdx.x >> 6;
-    const int lane = threadIdx.x & 63;
-    unsigned *hist = L.hist + (p.hist_copies == 1 ? 0 : wave) * p.bins;
-    const int ncell = ncx * ncy * ncz;
-    const int n_sten = n_off + 1;
-    const long long n_units = (long long)n_frames * ncell;
-    UnitRange ur = xcd_units(n_units, n_xcd);
-    const float nanv = __builtin_nanf("");
-    // Upper bound of what any u32 sub-histogram slot has received since the
-    // last flush (workgroup-uniform); flush before it could wrap.
-    unsigned long long budget = 0;
-    const unsigned long long budget_cap = 0x7FFFFFFFull;
+    const CellLds L = cell_lds_layout(lds, p, EDGE);
+    for (int i = threadIdx.x; i < p.bins; i += kCellBlock) L.hist[i] = 0ull;
+    for (int i = threadIdx.x; i <= p.bins; i += kCellBlock) {
+        L.thr[i] = p.thr[i];
+        if constexpr (EDGE) {
+            L.elo[i] = p.elo[i];
+            L.ehi[i] = p.ehi[i];
+        }
+    }
+    if constexpr (TABLE)
+        for (int i = threadIdx.x; i < p.tab_n; i += kCellBlock) L.tab[i] = p.tab[i];
     __syncthreads();
-    for (long long u = ur.begin; u < ur.end; u += ur.step) {
+
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform: unit indices in SGPRs
+    const int lane = threadIdx.x & 63;
+    float4 *ring = L.ring + wave * kRing;
+    unsigned long long edge_local = 0;
+    const float nanv = __builtin_nanf("");
+    const int ncell = ncx * ncy * ncz;
+    // rows of the half stencil: k rows (0, dy > 0), k (2k + 1) rows dz > 0, the home row
+    const int n_rows = 2 * k * k + 2 * k + 1;
+    const int n_pieces = 2 * n_rows;
+    // XCD-aware static schedule over waves: workgroup g runs on XCD g % n_xcd
+    const long long n_units = (long long)n_frames * ncell;
+    const int xcd = blockIdx.x % n_xcd;
+    const long long wave_local = (long long)(blockIdx.x / n_xcd) * kCellWaves + wave;
+    const long long waves_per_xcd = (long long)(gridDim.x / n_xcd) * kCellWaves;
+    const long long u_hi = n_units * (xcd + 1) / n_xcd;
+
+    for (long long u = n_units * xcd / n_xcd + wave_local; u < u_hi; u += waves_per_xcd) {
         const int f = (int)(u / ncell);
         const int h = (int)(u - (long long)f * ncell);
         const int hx = h % ncx;
@@ -437,99 +662,154 @@ __global__ __launch_bounds__(kBlock) void rdf_cell_kernel(
         const int hz = h / (ncx * ncy);
         const unsigned *fs = start + (long long)f * (ncell + 1);
         const float4 *fp = sorted + (long long)f * n;
-        const FrameBox b = boxes[f];
-        __syncthreads();  // previous unit done with LDS
-        int cnt = 0;
-        if ((int)threadIdx.x < n_sten) {
-            int cid = h;
-            if (threadIdx.x > 0) {
-                int4 o = offsets[threadIdx.x - 1];
-                int cx = hx + o.x, cy = hy + o.y, cz = hz + o.z;
-                cx += cx < 0 ? ncx : 0; cx -= cx >= ncx ? ncx : 0;
-                cy += cy < 0 ? ncy : 0; cy -= cy >= ncy ? ncy : 0;
-                cz += cz < 0 ? ncz : 0; cz -= cz >= ncz ? ncz : 0;
-                cid = (cz * ncy + cy) * ncx + cx;
-            }
-            int s0 = (int)fs[cid];
-            cnt = (int)fs[cid + 1] - s0;
-            c_start[threadIdx.x] = s0;
-        }
-        // block-wide exclusive prefix of the stencil cell counts
-        int v = cnt;
-        for (int off = 1; off < 64; off <<= 1) {
-            int t = __shfl_up(v, off, 64);
-            if (lane >= off) v += t;
-        }
-        if (lane == 63) wave_tot[wave] = v;
-        __syncthreads();
-        int add = 0;
-        for (int w = 0; w < wave; ++w) add += wave_tot[w];
-        if ((int)threadIdx.x < n_sten) c_pref[threadIdx.x + 1] = v + add;
-        if (threadIdx.x == 0) c_pref[0] = 0;
-        __syncthreads();
-        const int n_home = c_pref[1];
-        const int total = c_pref[n_sten];
+        const int hs = (int)fs[h];
+        const int n_home = (int)fs[h + 1] - hs;
         if (n_home == 0) continue;
-        const int I = min(n_home, kBlock);
-        const int J = kBlock / I;
-        const int islot = (int)threadIdx.x % I;
-        const int jph = (int)threadIdx.x / I;
-        const bool lane_on = jph < J;
-        for (int ib = 0; ib < n_home; ib += I) {
-            const int i = ib + islot;
-            float xi = nanv, yi = nanv, zi = nanv;
-            unsigned wi = 0;
-            if (lane_on && i < n_home) {
-                float4 pi = fp[c_start[0] + i];
-                xi = pi.x; yi = pi.y; zi = pi.z;
-                wi = __float_as_uint(pi.w);
+        const float4 *hp = fp + hs;
+        const FrameBox b = boxes[f];
+
+        // home atoms: bounding box and image uniformity
+        const unsigned m0 = __float_as_uint(hp[0].w) & 0xFFFFFFu;
+        float mn0 = INFINITY, mn1 = INFINITY, mn2 = INFINITY;
+        float mx0 = -INFINITY, mx1 = -INFINITY, mx2 = -INFINITY;
+        bool diff = false;
+        for (int hb = 0; hb < n_home; hb += 64) {
+            if (hb + lane < n_home) {
+                const float4 q = hp[hb + lane];
+                mn0 = fminf(mn0, q.x); mx0 = fmaxf(mx0, q.x);
+                mn1 = fminf(mn1, q.y); mx1 = fmaxf(mx1, q.y);
+                mn2 = fminf(mn2, q.z); mx2 = fmaxf(mx2, q.z);
+                diff |= (__float_as_uint(q.w) & 0xFFFFFFu) != m0;
             }
-            for (int c0 = 0; c0 < total; c0 += kMaxJ) {
-                const int c1 = min(total, c0 + kMaxJ);
-                __syncthreads();
-                for (int idx = c0 + (int)threadIdx.x; idx < c1; idx += kBlock) {
-                    // stencil cell holding list position idx: c_pref[lo] <= idx < c_pref[lo+1]
-                    int lo = 0, hi = n_sten;
-                    while (hi - lo > 1) {
-                        int mid = (lo + hi) >> 1;
-                        if (c_pref[mid] <= idx) lo = mid; else hi = mid;
-                    }
-                    jpos[idx - c0] = fp[c_start[lo] + (idx - c_pref[lo])];
-                }
-                __syncthreads();
-                if (lane_on) {
-                    // home cell: ordered pairs, skipping the atom itself
-                    const int own_end = min(c1, n_home);
-                    for (int jj = c0 + jph; jj < own_end; jj += J) {
-                        float4 pj = jpos[jj - c0];
-                        float s = pair_s<PBC, EXACT>(xi, yi, zi, pj.x, pj.y, pj.z, b);
-                        if (jj == i) s = nanv;
-                        unsigned w = 1u;
-                        if constexpr (WEIGHTED) w = (wi & 0xFFu) * (__float_as_uint(pj.w) >> 8);
-                        count_pair<EDGE>(s, w, p, L.thr, L.elo, L.ehi, hist, edge_local);
-                    }
-                    // neighbour cells: unordered pairs, both orders at once
-                    for (int jj = max(c0, n_home) + jph; jj < c1; jj += J) {
-                        float4 pj = jpos[jj - c0];
-                        float s = pair_s<PBC, EXACT>(xi, yi, zi, pj.x, pj.y, pj.z, b);
-                        unsigned w = 2u;
-                        if constexpr (WEIGHTED) {
-                            unsigned wj = __float_as_uint(pj.w);
-                            w = (wi & 0xFFu) * (wj >> 8) + (wj & 0xFFu) * (wi >> 8);
-                        }
-                        count_pair<EDGE>(s, w, p, L.thr, L.elo, L.ehi, hist, edge_local);
-                    }
-                }
-                // at most I * (c1 - c0) pairs per sub-histogram copy in this pass
-                budget += (unsigned long long)I * (unsigned long long)(c1 - c0) * w_max;
-                if (budget > budget_cap) {
-                    budget = 0;
-                    lds_hist_flush<EDGE>(L, p, edge_local);
-                }
+        }
+        const bool use_shift = shift_ok && frame_flag[f] == 0u && !__any(diff);
+        mn0 = wave_min(mn0); mn1 = wave_min(mn1); mn2 = wave_min(mn2);
+        mx0 = wave_max(mx0); mx1 = wave_max(mx1); mx2 = wave_max(mx2);
+        const int mhx = sext8(m0), mhy = sext8(m0 >> 8), mhz = sext8(m0 >> 16);
+
+        // the runs of the half stencil, one per lane
+        int pbeg = 0, plen = 0, pwrap = 0x15;
+        if (lane < n_pieces) {
+            const int r = lane >> 1;
+            int dy, dz, xa, xb;
+            if (r < k) {
+                dz = 0; dy = r + 1; xa = hx - k; xb = hx + k;
+            } else if (r < n_rows - 1) {
+                const int q = r - k;
+                dz = 1 + q / (2 * k + 1); dy = q % (2 * k + 1) - k; xa = hx - k; xb = hx + k;
+            } else {
+                dz = 0; dy = 0; xa = hx + 1; xb = hx + k;
             }
+            int cy = hy + dy, cz = hz + dz;
+            const int wy = cy < 0 ? -1 : (cy >= ncy ? 1 : 0);
+            const int wz = cz < 0 ? -1 : (cz >= ncz ? 1 : 0);
+            cy -= wy * ncy;
+            cz -= wz * ncz;
+            int lo, hi, wx = 0;
+            if ((lane & 1) == 0) {
+                lo = max(xa, 0); hi = min(xb, ncx - 1);
+            } else if (xa < 0) {
+                lo = xa + ncx; hi = ncx - 1; wx = -1;
+            } else if (xb >= ncx) {
+                lo = 0; hi = xb - ncx; wx = 1;
+            } else {
+                lo = 1; hi = 0;
+            }
+            if (lo <= hi) {
+                const int rowbase = (cz * ncy + cy) * ncx;
+                pbeg = (int)fs[rowbase + lo];
+                plen = (int)fs[rowbase + hi + 1] - pbeg;
+            }
+            pwrap = (wx + 1) | ((wy + 1) << 2) | ((wz + 1) << 4);
+        }
+
+        // stream: home atoms (piece -1), then the runs; ring positions head..tail
+        int head = 0, tail = 0;
+        int l = -1, b0 = 0, base = hs, len = n_home, wrap = 0x15;
+        while (true) {
+            while (b0 >= len) {
+                if (++l >= n_pieces) break;
+                base = __builtin_amdgcn_readlane(pbeg, l);
+                len = __builtin_amdgcn_readlane(plen, l);
+                wrap = __builtin_amdgcn_readlane(pwrap, l);
+                b0 = 0;
+            }
+            const bool more = l < n_pieces;
+            if (more) {
+                const int idx = b0 + lane;
+                const bool valid = idx < len;
+                float4 q = make_float4(nanv, nanv, nanv, 0.f);
+                if (valid) q = fp[base + idx];
+                unsigned meta = __float_as_uint(q.w);
+                bool keep = valid;
+                if (l < 0 || !use_shift) {
+                    meta &= 0xFF000000u;  // multiplicities only (zero shift)
+                } else {
+                    const int nx = sext8(meta) - ((wrap & 3) - 1) - mhx;
+                    const int ny = sext8(meta >> 8) - (((wrap >> 2) & 3) - 1) - mhy;
+                    const int nz = sext8(meta >> 16) - (((wrap >> 4) & 3) - 1) - mhz;
+                    float px, py, pz;  // image of j next to the home atoms (approximate)
+                    if constexpr (PBC == WMD_PBC_ORTHO) {
+                        px = q.x - (float)nx * b.lx;
+                        py = q.y - (float)ny * b.ly;
+                        pz = q.z - (float)nz * b.lz;
+                    } else {
+                        const float fx = (float)nx, fy = (float)ny, fz = (float)nz;
+                        px = q.x - (fx * b.c[0] + fy * b.c[3] + fz * b.c[6]);
+                        py = q.y - (fx * b.c[1] + fy * b.c[4] + fz * b.c[7]);
+                        pz = q.z - (fx * b.c[2] + fy * b.c[5] + fz * b.c[8]);
+                    }
+                    const float ex = fmaxf(fmaxf(mn0 - px, px - mx0), 0.f);
+                    const float ey = fmaxf(fmaxf(mn1 - py, py - mx1), 0.f);
+                    const float ez = fmaxf(fmaxf(mn2 - pz, pz - mx2), 0.f);
+                    keep = valid && ex * ex + ey * ey + ez * ez <= cull2;
+                    meta = (meta & 0xFF000000u) | ((unsigned)nx & 0xFFu) |
+                           (((unsigned)ny & 0xFFu) << 8) | (((unsigned)nz & 0xFFu) << 16);
+                }
+                const unsigned long long bal = __ballot(keep);
+                if (keep)
+                    ring[(tail + lanes_below(bal)) & (kRing - 1)] =
+                        make_float4(q.x, q.y, q.z, __uint_as_float(meta));
+                tail += (int)__popcll(bal);
+                b0 += 64;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            while (tail - head >= 64 || (!more && tail > head)) {
+                const int cnt = min(64, tail - head);
+                const float4 e = ring[(head + lane) & (kRing - 1)];
+                const bool v = lane < cnt;
+                const float xj = v ? e.x : nanv;
+                const float yj = v ? e.y : nanv;
+                const float zj = v ? e.z : nanv;
+                const unsigned mj = __float_as_uint(e.w);
+                if (head < n_home) {
+                    const int rank = head + lane < n_home ? head + lane : 0x7FFFFFFF;
+                    if (use_shift)
+                        cell_chunk<PBC, LT, WEIGHTED, EDGE, TABLE, true, true>(hp, n_home, xj, yj, zj, mj, rank, b, p, L, edge_local);
+                    else
+                        cell_chunk<PBC, LT, WEIGHTED, EDGE, TABLE, false, true>(hp, n_home, xj, yj, zj, mj, rank, b, p, L, edge_local);
+                } else {
+                    if (use_shift)
+                        cell_chunk<PBC, LT, WEIGHTED, EDGE, TABLE, true, false>(hp, n_home, xj, yj, zj, mj, 0, b, p, L, edge_local);
+                    else
+                        cell_chunk<PBC, LT, WEIGHTED, EDGE, TABLE, false, false>(hp, n_home, xj, yj, zj, mj, 0, b, p, L, edge_local);
+                }
+                head += cnt;
+            }
+            if (!more) break;
         }
     }
-    lds_hist_flush<EDGE>(L, p, edge_local);
+    __syncthreads();
+    for (int i = threadIdx.x; i < p.bins; i += kCellBlock) {
+        const unsigned long long v = L.hist[i];
+        if (v) atomicAdd(&p.counts[i], v);
+    }
+    if constexpr (EDGE) {
+        for (int off = 32; off > 0; off >>= 1) edge_local += __shfl_xor(edge_local, off, 64);
+        if (lane == 0 && edge_local) atomicAdd(p.edge, edge_local);
+    }
 }
 
 // AoS [F][n_src][4] -> SoA [F][3][stride] through a gather list (kPad -> 0).
@@ -564,6 +844,12 @@ static inline float bits_f(uint32_t u) {
     float f;
     std::memcpy(&f, &u, 4);
     return f;
+}
+
+static inline uint32_t f_bits(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    return u;
 }
 
 // Reference key of a squared distance: its bin if counted, else bins.
@@ -605,36 +891,51 @@ struct wmd_rdf {
     int pbc = 0;
     uint32_t flags = 0;
     std::vector<float> thr, elo, ehi;
+    std::vector<unsigned short> tab;
+    int tab_sh = 0, tab_base = 0;
     int cell_k = 0;        // 0 = automatic
     int last_path = 0;
     int last_nc[3] = {0, 0, 0};
-    DevBuf d_counts, d_thr, d_elo, d_ehi, d_wts, d_io, d_boxes, d_aos, d_soa;
-    DevBuf d_cell_of, d_cell_slot, d_cell_count, d_cell_start, d_sorted, d_offsets, d_tiles;
+    DevBuf d_counts, d_thr, d_elo, d_ehi, d_tab, d_wts, d_io, d_boxes, d_aos, d_soa;
+    DevBuf d_cell_of, d_cell_slot, d_cell_image, d_cell_count, d_cell_start, d_frame_flag,
+        d_sorted, d_tiles;
     HostBuf h_stage, h_boxes;
 };
 
 namespace {
 
-int hist_copies_for(int bins) { return bins * 4 * kWaves <= 32768 ? kWaves : 1; }
+// One shared LDS histogram: LDS atomics of different waves are separate
+// instructions, so per-wave copies would not reduce conflicts.
+int hist_copies_for(int) { return 1; }
 
-size_t lds_hist_bytes(int bins, bool edge) {
-    size_t nb1 = (size_t)bins + 1;
-    size_t nf = edge ? 3 * nb1 : nb1;
-    size_t off = ((nf * 4 + 15) / 16) * 16;
-    size_t hb = (size_t)hist_copies_for(bins) * bins * 4;
-    return off + ((hb + 15) / 16) * 16;
-}
-
-// Half stencil: offsets o in [-k, k]^3 with (oz, oy, ox) > (0, 0, 0) lexicographically.
-std::vector<int4> half_stencil(int k) {
-    std::vector<int4> out;
-    for (int oz = -k; oz <= k; ++oz)
-        for (int oy = -k; oy <= k; ++oy)
-            for (int ox = -k; ox <= k; ++ox) {
-                bool pos = oz > 0 || (oz == 0 && (oy > 0 || (oy == 0 && ox > 0)));
-                if (pos) out.push_back(make_int4(ox, oy, oz, 0));
-            }
-    return out;
+// s-bits bin table: the largest shift such that every bucket of 2^sh bit
+// patterns in [T[1], cut) holds at most one breakpoint T[1..bins-1].
+void build_table(wmd_rdf *h) {
+    h->tab.clear();
+    if (h->bins < 2) return;
+    const uint32_t lo_bits = f_bits(h->thr[1]);
+    const uint32_t hi_bits = f_bits(h->cut) - 1;  // largest s below the cut
+    for (int sh = 23; sh >= 8; --sh) {
+        const int base = (int)(lo_bits >> sh);
+        const int n = (int)(hi_bits >> sh) - base + 1;
+        if (n > kMaxTab) break;
+        bool ok = true;
+        for (int k = 1; k + 1 < h->bins && ok; ++k)
+            if ((f_bits(h->thr[k]) >> sh) == (f_bits(h->thr[k + 1]) >> sh)) ok = false;
+        if (!ok) continue;
+        std::vector<unsigned short> tab(n);
+        int b = 0;
+        for (int k = 0; k < n; ++k) {
+            const uint32_t start_bits = (uint32_t)(base + k) << sh;
+            const float s0 = k == 0 ? 0.0f : bits_f(start_bits);
+            while (b + 1 < h->bins && s0 >= h->thr[b + 1]) ++b;
+            tab[k] = (unsigned short)b;
+        }
+        h->tab.swap(tab);
+        h->tab_sh = sh;
+        h->tab_base = base << sh;  // bit pattern where bucket 0 starts
+        return;
+    }
 }
 
 template <class K> int occupancy(K kernel, size_t lds) {
@@ -645,66 +946,15 @@ template <class K> int occupancy(K kernel, size_t lds) {
     return nb;
 }
 
-// Dispatch helpers over the template flags.
-template <int PBC, bool EXACT, bool WEIGHTED, bool EDGE>
-void launch_brute_t(wmd_rdf *h, hipStream_t st, const float *soa, long long frame_stride,
-                    int stride, int n, int nf, long long frame0, const FrameBox *boxes,
-                    const int2 *tiles, int n_tp, const HistParams &p) {
-    auto kern = rdf_brute_kernel<PBC, EXACT, WEIGHTED, EDGE>;
-    ProfScope prof(h->ctx, st, "rdf_pair");
-    size_t lds = lds_hist_bytes(p.bins, EDGE) + kTile * 16;
-    int occ = occupancy(kern, lds);
-    long long n_units = (long long)nf * n_tp;
+int persistent_grid(wmd_rdf *h, int occ, long long n_units) {
     int grid = h->ctx->cu_count * occ;
     grid = std::max(8, grid / 8 * 8);
     long long max_grid = (n_units + 7) / 8 * 8;
     if (grid > max_grid) grid = (int)std::max<long long>(8, max_grid);
-    WMD_REQUIRE(lds <= h->ctx->lds_per_block, WMD_ERR_UNSUPPORTED, "too many bins for LDS");
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), lds, st, soa, frame_stride, stride, n, nf,
-                       boxes, (const unsigned *)h->d_wts.ptr, tiles, n_tp, frame0, p, 8,
-                       h->w_max);
-    WMD_HIP_CHECK(hipGetLastError());
+    return grid;
 }
 
-template <int PBC, bool EXACT, bool WEIGHTED, bool EDGE>
-void launch_cells_t(wmd_rdf *h, hipStream_t st, const float4 *sorted, const unsigned *start, int n,
-                    int nf, const FrameBox *boxes, const int *nc, const int4 *offs, int n_off,
-                    const HistParams &p) {
-    auto kern = rdf_cell_kernel<PBC, EXACT, WEIGHTED, EDGE>;
-    ProfScope prof(h->ctx, st, "rdf_pair");
-    size_t lds = lds_hist_bytes(p.bins, EDGE) + kMaxJ * 16 + (2 * kMaxStencil + 4) * 4;
-    int occ = occupancy(kern, lds);
-    long long n_units = (long long)nf * nc[0] * nc[1] * nc[2];
-    int grid = h->ctx->cu_count * occ;
-    grid = std::max(8, grid / 8 * 8);
-    long long max_grid = (n_units + 7) / 8 * 8;
-    if (grid > max_grid) grid = (int)std::max<long long>(8, max_grid);
-    WMD_REQUIRE(lds <= h->ctx->lds_per_block, WMD_ERR_UNSUPPORTED, "too many bins for LDS");
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), lds, st, sorted, start, n, nf, boxes, nc[0],
-                       nc[1], nc[2], offs, n_off, p, 8, h->w_max);
-    WMD_HIP_CHECK(hipGetLastError());
-}
-
-template <bool WEIGHTED, bool EDGE, class... A>
-void brute_by_pbc(int pbc, bool exact, A... a) {
-    if (pbc == WMD_PBC_NONE) launch_brute_t<WMD_PBC_NONE, false, WEIGHTED, EDGE>(a...);
-    else if (pbc == WMD_PBC_ORTHO) {
-        if (exact) launch_brute_t<WMD_PBC_ORTHO, true, WEIGHTED, EDGE>(a...);
-        else launch_brute_t<WMD_PBC_ORTHO, false, WEIGHTED, EDGE>(a...);
-    } else {
-        if (exact) launch_brute_t<WMD_PBC_TRICLINIC, true, WEIGHTED, EDGE>(a...);
-        else launch_brute_t<WMD_PBC_TRICLINIC, false, WEIGHTED, EDGE>(a...);
-    }
-}
-
-template <bool WEIGHTED, bool EDGE, class... A>
-void cells_by_pbc(int pbc, A... a) {
-    // the cell list is only used when r_max < half the plane spacing: fast wrap
-    if (pbc == WMD_PBC_ORTHO) launch_cells_t<WMD_PBC_ORTHO, false, WEIGHTED, EDGE>(a...);
-    else launch_cells_t<WMD_PBC_TRICLINIC, false, WEIGHTED, EDGE>(a...);
-}
-
-HistParams hist_params(wmd_rdf *h) {
+HistParams hist_params(wmd_rdf *h, bool table) {
     HistParams p;
     p.bins = h->bins;
     p.hist_copies = hist_copies_for(h->bins);
@@ -713,9 +963,92 @@ HistParams hist_params(wmd_rdf *h) {
     p.thr = h->d_thr.as<float>();
     p.elo = h->d_elo.as<float>();
     p.ehi = h->d_ehi.as<float>();
+    p.tab = table ? h->d_tab.as<unsigned short>() : nullptr;
+    p.tab_n = table ? (int)h->tab.size() : 0;
+    p.tab_sh = h->tab_sh;
+    p.tab_base = h->tab_base;
     p.counts = h->d_counts.as<unsigned long long>();
     p.edge = h->d_counts.as<unsigned long long>() + h->bins;
     return p;
+}
+
+size_t lds_hist_bytes(const HistParams &p, bool edge) {
+    return lds_hist_size(p.bins, p.hist_copies, edge, p.tab_n);
+}
+
+// Dispatch helpers over the template flags.
+template <int PBC, bool EXACT, bool WEIGHTED, bool EDGE, bool TABLE>
+void launch_brute_t(wmd_rdf *h, hipStream_t st, const float *soa, long long frame_stride,
+                    int stride, int n, int nf, long long frame0, const FrameBox *boxes,
+                    const int2 *tiles, int n_tp, const HistParams &p) {
+    auto kern = rdf_brute_kernel<PBC, EXACT, WEIGHTED, EDGE, TABLE>;
+    ProfScope prof(h->ctx, st, "rdf_pair");
+    size_t lds = lds_hist_bytes(p, EDGE) + kTile * 16;
+    WMD_REQUIRE(lds <= h->ctx->lds_per_block, WMD_ERR_UNSUPPORTED, "too many bins for LDS");
+    int grid = persistent_grid(h, occupancy(kern, lds), (long long)nf * n_tp);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), lds, st, soa, frame_stride, stride, n, nf,
+                       boxes, (const unsigned *)h->d_wts.ptr, tiles, n_tp, frame0, p, 8, h->w_max);
+    WMD_HIP_CHECK(hipGetLastError());
+}
+
+template <int PBC, bool LT, bool WEIGHTED, bool EDGE, bool TABLE>
+void launch_cells_t(wmd_rdf *h, hipStream_t st, const float4 *sorted, const unsigned *start,
+                    const unsigned *frame_flag, int n, int nf, const FrameBox *boxes,
+                    const int *nc, int k, int shift_ok, float cull2, const HistParams &p) {
+    auto kern = rdf_cell_kernel<PBC, LT, WEIGHTED, EDGE, TABLE>;
+    ProfScope prof(h->ctx, st, "rdf_pair");
+    size_t lds = cell_lds_bytes(p.bins, EDGE, p.tab_n);
+    WMD_REQUIRE(lds <= h->ctx->lds_per_block, WMD_ERR_UNSUPPORTED, "too many bins for LDS");
+    int occ = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, kCellBlock, lds) != hipSuccess || occ < 1)
+        occ = 1;
+    const long long n_units = (long long)nf * nc[0] * nc[1] * nc[2];
+    int grid = std::max(8, h->ctx->cu_count * occ / 8 * 8);
+    const long long max_grid = (n_units + kCellWaves * 8 - 1) / (kCellWaves * 8) * 8;
+    if (grid > max_grid) grid = (int)std::max<long long>(8, max_grid);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(kCellBlock), lds, st, sorted, start, frame_flag, n, nf,
+                       boxes, nc[0], nc[1], nc[2], k, shift_ok, cull2, p, 8);
+    WMD_HIP_CHECK(hipGetLastError());
+}
+
+template <bool EXACT, bool WEIGHTED, bool EDGE, bool TABLE, class... A>
+void brute_pbc(int pbc, A... a) {
+    if (pbc == WMD_PBC_NONE) launch_brute_t<WMD_PBC_NONE, false, WEIGHTED, EDGE, TABLE>(a...);
+    else if (pbc == WMD_PBC_ORTHO) launch_brute_t<WMD_PBC_ORTHO, EXACT, WEIGHTED, EDGE, TABLE>(a...);
+    else launch_brute_t<WMD_PBC_TRICLINIC, EXACT, WEIGHTED, EDGE, TABLE>(a...);
+}
+
+template <class... A>
+void brute_dispatch(bool exact, bool weighted, bool edge, bool table, int pbc, A... a) {
+#define WMD_BRUTE(E, W, D, T) \
+    if (exact == E && weighted == W && edge == D && table == T) return brute_pbc<E, W, D, T>(pbc, a...);
+    WMD_BRUTE(false, false, false, false) WMD_BRUTE(false, false, false, true)
+    WMD_BRUTE(false, false, true, false) WMD_BRUTE(false, false, true, true)
+    WMD_BRUTE(false, true, false, false) WMD_BRUTE(false, true, false, true)
+    WMD_BRUTE(false, true, true, false) WMD_BRUTE(false, true, true, true)
+    WMD_BRUTE(true, false, false, false) WMD_BRUTE(true, false, false, true)
+    WMD_BRUTE(true, false, true, false) WMD_BRUTE(true, false, true, true)
+    WMD_BRUTE(true, true, false, false) WMD_BRUTE(true, true, false, true)
+    WMD_BRUTE(true, true, true, false) WMD_BRUTE(true, true, true, true)
+#undef WMD_BRUTE
+}
+
+template <class... A>
+void cells_dispatch(int pbc, bool lt, bool weighted, bool edge, bool table, A... a) {
+#define WMD_CELLS(P, LT_, W, D, T)                                                \
+    if (pbc == P && lt == LT_ && weighted == W && edge == D && table == T)        \
+        return launch_cells_t<P, LT_, W, D, T>(a...);
+#define WMD_CELLS_P(P, LT_) \
+    WMD_CELLS(P, LT_, false, false, false) WMD_CELLS(P, LT_, false, false, true) \
+    WMD_CELLS(P, LT_, false, true, false) WMD_CELLS(P, LT_, false, true, true)   \
+    WMD_CELLS(P, LT_, true, false, false) WMD_CELLS(P, LT_, true, false, true)   \
+    WMD_CELLS(P, LT_, true, true, false) WMD_CELLS(P, LT_, true, true, true)
+    WMD_CELLS_P(WMD_PBC_ORTHO, false)
+    WMD_CELLS_P(WMD_PBC_TRICLINIC, false)
+    WMD_CELLS_P(WMD_PBC_TRICLINIC, true)
+#undef WMD_CELLS_P
+#undef WMD_CELLS
+    fail(WMD_ERR_INVALID, "no cell-list kernel for this combination");
 }
 
 void grid1d(long long total, int *grid) {
@@ -736,14 +1069,17 @@ void accumulate_device(wmd_rdf *h, const float *d_soa, size_t stride, size_t n_f
     // per-frame box parameters
     std::vector<FrameBox> fb(n_frames);
     double min_width[3] = {1e300, 1e300, 1e300};
+    bool lower_tri = true;
     for (size_t f = 0; f < n_frames; ++f) {
         BoxInfo bi;
         if (h->pbc != WMD_PBC_NONE) {
-            if (!make_box(h->pbc, boxes_host + 9 * f, &bi))
+            const float *m = boxes_host + 9 * f;
+            if (!make_box(h->pbc, m, &bi))
                 fail(WMD_ERR_MISMATCH, h->pbc == WMD_PBC_ORTHO
                                            ? "RDF requires a positive orthorhombic box"
                                            : "box matrix not invertible");
             for (int d = 0; d < 3; ++d) min_width[d] = std::min(min_width[d], bi.width[d]);
+            if (m[1] != 0.0f || m[2] != 0.0f || m[5] != 0.0f) lower_tri = false;
         } else {
             std::memset(&bi, 0, sizeof(bi));
         }
@@ -753,7 +1089,6 @@ void accumulate_device(wmd_rdf *h, const float *d_soa, size_t stride, size_t n_f
         b.ily = bi.ly > 0 ? 1.0f / bi.ly : 0.f;
         b.ilz = bi.lz > 0 ? 1.0f / bi.lz : 0.f;
         if (h->pbc == WMD_PBC_ORTHO) {
-            // cell assignment uses the inverse rows for both kinds
             for (int i = 0; i < 9; ++i) b.r[i] = 0.f, b.c[i] = 0.f;
             b.r[0] = b.ilx; b.r[4] = b.ily; b.r[8] = b.ilz;
             b.c[0] = b.lx; b.c[4] = b.ly; b.c[8] = b.lz;
@@ -770,7 +1105,8 @@ void accumulate_device(wmd_rdf *h, const float *d_soa, size_t stride, size_t n_f
     WMD_HIP_CHECK(hipMemcpyAsync(d_boxes, hb, box_bytes, hipMemcpyHostToDevice, st));
 
     const bool edge = h->flags & WMD_RDF_EDGE_DIAGNOSTICS;
-    HistParams p = hist_params(h);
+    const bool table = !h->tab.empty();
+    HistParams p = hist_params(h, table);
     double mw = std::min(min_width[0], std::min(min_width[1], min_width[2]));
     bool fast_wrap = h->pbc == WMD_PBC_NONE || (double)h->r_max <= 0.5 * mw * (1.0 - 1e-5);
 
@@ -812,62 +1148,68 @@ void accumulate_device(wmd_rdf *h, const float *d_soa, size_t stride, size_t n_f
         // the vector dies at scope exit: make the copy complete first
         WMD_HIP_CHECK(hipStreamSynchronize(st));
         long long fs = 3 * (long long)stride;
-        if (h->weighted) {
-            if (edge) brute_by_pbc<true, true>(h->pbc, !fast_wrap, h, st, d_soa, fs, (int)stride, n, (int)n_frames, 0LL, (const FrameBox *)d_boxes, (const int2 *)d_tp, n_tp, p);
-            else brute_by_pbc<true, false>(h->pbc, !fast_wrap, h, st, d_soa, fs, (int)stride, n, (int)n_frames, 0LL, (const FrameBox *)d_boxes, (const int2 *)d_tp, n_tp, p);
-        } else {
-            if (edge) brute_by_pbc<false, true>(h->pbc, !fast_wrap, h, st, d_soa, fs, (int)stride, n, (int)n_frames, 0LL, (const FrameBox *)d_boxes, (const int2 *)d_tp, n_tp, p);
-            else brute_by_pbc<false, false>(h->pbc, !fast_wrap, h, st, d_soa, fs, (int)stride, n, (int)n_frames, 0LL, (const FrameBox *)d_boxes, (const int2 *)d_tp, n_tp, p);
-        }
+        brute_dispatch(!fast_wrap, h->weighted, edge, table, h->pbc, h, st, d_soa, fs, (int)stride,
+                       n, (int)n_frames, 0LL, (const FrameBox *)d_boxes, (const int2 *)d_tp, n_tp, p);
         h->last_path = 1;
         h->last_nc[0] = h->last_nc[1] = h->last_nc[2] = 0;
         return;
     }
 
     // ---- cell list ----------------------------------------------------------
-    std::vector<int4> offs = half_stencil(k_use);
-    int n_off = (int)offs.size();
-    int4 *d_offs = (int4 *)h->d_offsets.reserve(sizeof(int4) * n_off);
-    WMD_HIP_CHECK(hipMemcpyAsync(d_offs, offs.data(), sizeof(int4) * n_off, hipMemcpyHostToDevice, st));
-    WMD_HIP_CHECK(hipStreamSynchronize(st));
     const int ncell = nc[0] * nc[1] * nc[2];
-    // frames per batch: bound the scratch (sorted float4 + 2 u32 per atom)
-    size_t per_frame = (size_t)n * (16 + 8) + (size_t)(2 * ncell + 1) * 4;
+    // the image index of every stencil pair is determined by the cells when the
+    // stencil spans less than half the box: (k + 1) / nc < 1/2
+    const int shift_ok = (nc[0] >= 2 * k_use + 3 && nc[1] >= 2 * k_use + 3 && nc[2] >= 2 * k_use + 3) ? 1 : 0;
+    // Cull radius: r_max plus a margin covering the f32 rounding of the image
+    // positions and of the reference's own difference, for coordinates bounded
+    // by (kImageLimit + 1) box lengths (frames beyond that use the rint path).
+    double coord_bound = 0.0;
+    for (size_t f = 0; f < n_frames; ++f) {
+        double row = 0.0;
+        for (int i = 0; i < 9; ++i) row += std::fabs((double)fb[f].c[i]);
+        coord_bound = std::max(coord_bound, row);
+    }
+    coord_bound *= (double)(kImageLimit + 1);
+    const double margin = 0.05 + 32.0 * coord_bound * std::ldexp(1.0, -24);
+    const float cull = (float)((double)h->r_max + margin);
+    const float cull2 = cull * cull;
+    // frames per batch: bound the scratch (sorted float4 + 3 u32 per atom)
+    size_t per_frame = (size_t)n * (16 + 12) + (size_t)(2 * ncell + 2) * 4;
     size_t batch = std::max<size_t>(1, std::min<size_t>(n_frames, (size_t)(768ull << 20) / per_frame));
     unsigned *d_cell_of = (unsigned *)h->d_cell_of.reserve((size_t)n * batch * 4);
     unsigned *d_cell_slot = (unsigned *)h->d_cell_slot.reserve((size_t)n * batch * 4);
+    unsigned *d_image = (unsigned *)h->d_cell_image.reserve((size_t)n * batch * 4);
     unsigned *d_count = (unsigned *)h->d_cell_count.reserve((size_t)ncell * batch * 4);
     unsigned *d_start = (unsigned *)h->d_cell_start.reserve((size_t)(ncell + 1) * batch * 4);
+    unsigned *d_fflag = (unsigned *)h->d_frame_flag.reserve(batch * 4);
     float4 *d_sorted = (float4 *)h->d_sorted.reserve((size_t)n * batch * 16);
     const unsigned *d_w = h->weighted ? h->d_wts.as<unsigned>() : nullptr;
     long long fs = 3 * (long long)stride;
+    const bool lt = h->pbc == WMD_PBC_TRICLINIC && lower_tri;
     for (size_t f0 = 0; f0 < n_frames; f0 += batch) {
         int nf = (int)std::min(batch, n_frames - f0);
         int g;
         const FrameBox *bx = (const FrameBox *)d_boxes + f0;
         {
-        ProfScope build_prof(h->ctx, st, "cell_build");
-        grid1d((long long)ncell * nf, &g);
-        hipLaunchKernelGGL(fill_u32_kernel, dim3(g), dim3(kBlock), 0, st, d_count, (long long)ncell * nf, 0u);
-        WMD_HIP_CHECK(hipGetLastError());
-        grid1d((long long)n * nf, &g);
-        if (h->pbc == WMD_PBC_ORTHO)
-            hipLaunchKernelGGL(cell_assign_kernel<WMD_PBC_ORTHO>, dim3(g), dim3(kBlock), 0, st, d_soa, fs, (int)stride, n, nf, (long long)f0, bx, nc[0], nc[1], nc[2], d_cell_of, d_cell_slot, d_count);
-        else
-            hipLaunchKernelGGL(cell_assign_kernel<WMD_PBC_TRICLINIC>, dim3(g), dim3(kBlock), 0, st, d_soa, fs, (int)stride, n, nf, (long long)f0, bx, nc[0], nc[1], nc[2], d_cell_of, d_cell_slot, d_count);
-        WMD_HIP_CHECK(hipGetLastError());
-        hipLaunchKernelGGL(cell_scan_kernel, dim3(nf), dim3(kBlock), 0, st, d_count, d_start, ncell);
-        WMD_HIP_CHECK(hipGetLastError());
-        hipLaunchKernelGGL(cell_scatter_kernel, dim3(g), dim3(kBlock), 0, st, d_soa, fs, (int)stride, n, nf, (long long)f0, d_cell_of, d_cell_slot, d_start, ncell, d_w, d_sorted);
-        WMD_HIP_CHECK(hipGetLastError());
+            ProfScope build_prof(h->ctx, st, "cell_build");
+            grid1d((long long)ncell * nf, &g);
+            hipLaunchKernelGGL(fill_u32_kernel, dim3(g), dim3(kBlock), 0, st, d_count, (long long)ncell * nf, 0u);
+            hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(kBlock), 0, st, d_fflag, (long long)nf, 0u);
+            WMD_HIP_CHECK(hipGetLastError());
+            grid1d((long long)n * nf, &g);
+            if (h->pbc == WMD_PBC_ORTHO)
+                hipLaunchKernelGGL(cell_assign_kernel<WMD_PBC_ORTHO>, dim3(g), dim3(kBlock), 0, st, d_soa, fs, (int)stride, n, nf, (long long)f0, bx, nc[0], nc[1], nc[2], d_cell_of, d_cell_slot, d_count, d_image, d_fflag);
+            else
+                hipLaunchKernelGGL(cell_assign_kernel<WMD_PBC_TRICLINIC>, dim3(g), dim3(kBlock), 0, st, d_soa, fs, (int)stride, n, nf, (long long)f0, bx, nc[0], nc[1], nc[2], d_cell_of, d_cell_slot, d_count, d_image, d_fflag);
+            WMD_HIP_CHECK(hipGetLastError());
+            hipLaunchKernelGGL(cell_scan_kernel, dim3(nf), dim3(kBlock), 0, st, d_count, d_start, ncell);
+            WMD_HIP_CHECK(hipGetLastError());
+            hipLaunchKernelGGL(cell_scatter_kernel, dim3(g), dim3(kBlock), 0, st, d_soa, fs, (int)stride, n, nf, (long long)f0, d_cell_of, d_cell_slot, d_image, d_start, ncell, d_w, d_sorted);
+            WMD_HIP_CHECK(hipGetLastError());
         }
-        if (h->weighted) {
-            if (edge) cells_by_pbc<true, true>(h->pbc, h, st, (const float4 *)d_sorted, (const unsigned *)d_start, n, nf, bx, (const int *)nc, (const int4 *)d_offs, n_off, p);
-            else cells_by_pbc<true, false>(h->pbc, h, st, (const float4 *)d_sorted, (const unsigned *)d_start, n, nf, bx, (const int *)nc, (const int4 *)d_offs, n_off, p);
-        } else {
-            if (edge) cells_by_pbc<false, true>(h->pbc, h, st, (const float4 *)d_sorted, (const unsigned *)d_start, n, nf, bx, (const int *)nc, (const int4 *)d_offs, n_off, p);
-            else cells_by_pbc<false, false>(h->pbc, h, st, (const float4 *)d_sorted, (const unsigned *)d_start, n, nf, bx, (const int *)nc, (const int4 *)d_offs, n_off, p);
-        }
+        cells_dispatch(h->pbc, lt, h->weighted, edge, table, h, st, (const float4 *)d_sorted,
+                       (const unsigned *)d_start, (const unsigned *)d_fflag, n, nf, bx,
+                       (const int *)nc, k_use, shift_ok, cull2, p);
     }
     h->last_path = 2;
     h->last_nc[0] = nc[0]; h->last_nc[1] = nc[1]; h->last_nc[2] = nc[2];
@@ -906,6 +1248,7 @@ void accumulate_host(wmd_rdf *h, const float *coords, size_t n_src, size_t n_fra
         WMD_HIP_CHECK(hipGetLastError());
         accumulate_device(h, d_soa, stride, nf, boxes ? boxes + 9 * f0 : nullptr, st);
     }
+    WMD_HIP_CHECK(hipStreamSynchronize(st));
 }
 
 }  // namespace
@@ -949,8 +1292,8 @@ wmd_status wmd_rdf_create(wmd_ctx *ctx, const uint32_t *sel_a, size_t n_a, const
         unsigned max_a = 0, max_b = 0;
         h->wts.resize(h->io.size());
         for (size_t i = 0; i < h->io.size(); ++i) {
-            WMD_REQUIRE(ma[i] <= 64 && mb[i] <= 64, WMD_ERR_UNSUPPORTED,
-                        "an atom repeated more than 64 times in a selection");
+            WMD_REQUIRE(ma[i] <= 15 && mb[i] <= 15, WMD_ERR_UNSUPPORTED,
+                        "an atom repeated more than 15 times in a selection");
             h->wts[i] = (unsigned)ma[i] | ((unsigned)mb[i] << 8);
             if (ma[i] != 1 || mb[i] != 1) unit = false;
             max_a = std::max<unsigned>(max_a, ma[i]);
@@ -969,6 +1312,7 @@ wmd_status wmd_rdf_create(wmd_ctx *ctx, const uint32_t *sel_a, size_t n_a, const
             for (int k = 1; k <= B; ++k)
                 h->thr[k] = min_s([&](float s) { return ref_key(s, r_max, w, B) >= k; });
             h->cut = h->thr[B];
+            build_table(h);
             h->elo.assign(bins + 1, 0.0f);
             h->ehi.assign(bins + 1, 0.0f);
             if (flags & WMD_RDF_EDGE_DIAGNOSTICS) {
@@ -979,14 +1323,21 @@ wmd_status wmd_rdf_create(wmd_ctx *ctx, const uint32_t *sel_a, size_t n_a, const
                     h->ehi[k] = min_s([&](float s) { volatile float r = std::sqrt(s); return (double)r >= e + 1e-5; });
                 }
             }
+            hipStream_t st = ctx->stream;
             h->d_counts.reserve((bins + 1) * 8);
-            WMD_HIP_CHECK(hipMemsetAsync(h->d_counts.ptr, 0, (bins + 1) * 8, ctx->stream));
+            hipLaunchKernelGGL(fill_u64_kernel, dim3((bins + 1 + kBlock - 1) / kBlock), dim3(kBlock), 0, st,
+                               h->d_counts.as<unsigned long long>(), (long long)bins + 1);
+            WMD_HIP_CHECK(hipGetLastError());
             h->d_thr.reserve((bins + 1) * 4);
             h->d_elo.reserve((bins + 1) * 4);
             h->d_ehi.reserve((bins + 1) * 4);
-            WMD_HIP_CHECK(hipMemcpyAsync(h->d_thr.ptr, h->thr.data(), (bins + 1) * 4, hipMemcpyHostToDevice, ctx->stream));
-            WMD_HIP_CHECK(hipMemcpyAsync(h->d_elo.ptr, h->elo.data(), (bins + 1) * 4, hipMemcpyHostToDevice, ctx->stream));
-            WMD_HIP_CHECK(hipMemcpyAsync(h->d_ehi.ptr, h->ehi.data(), (bins + 1) * 4, hipMemcpyHostToDevice, ctx->stream));
+            WMD_HIP_CHECK(hipMemcpyAsync(h->d_thr.ptr, h->thr.data(), (bins + 1) * 4, hipMemcpyHostToDevice, st));
+            WMD_HIP_CHECK(hipMemcpyAsync(h->d_elo.ptr, h->elo.data(), (bins + 1) * 4, hipMemcpyHostToDevice, st));
+            WMD_HIP_CHECK(hipMemcpyAsync(h->d_ehi.ptr, h->ehi.data(), (bins + 1) * 4, hipMemcpyHostToDevice, st));
+            if (!h->tab.empty()) {
+                h->d_tab.reserve(h->tab.size() * 2);
+                WMD_HIP_CHECK(hipMemcpyAsync(h->d_tab.ptr, h->tab.data(), h->tab.size() * 2, hipMemcpyHostToDevice, st));
+            }
         }
         size_t nio = std::max<size_t>(h->io.size(), 1);
         h->d_wts.reserve(nio * 4);
