@@ -79,6 +79,7 @@ struct HistParams {
     const float *elo; // [bins + 1] edge diagnostics (EDGE variants)
     const float *ehi;
     const unsigned short *tab;  // [tab_n] s-bits bin table (TABLE variants)
+    const uint2 *tab2;          // [tab_n] per bucket (split threshold bits, bin below it)
     int tab_n, tab_sh, tab_base;
     unsigned long long *counts;  // [bins]
     unsigned long long *edge;    // [1]
@@ -496,72 +497,88 @@ __device__ __forceinline__ int sext8(unsigned v) { return (int)(signed char)(v &
 // the neighbour cells of the half stencil taken as rows along x: for each
 // (dy, dz) with dz > 0, or dz = 0 and dy > 0, the cells hx-k..hx+k, and in the
 // home row the cells hx+1..hx+k.  Such a row is one contiguous run of the
-// cell-sorted atoms (two runs where it wraps in x).  Lanes load 64 stream
-// atoms at a time, drop neighbours farther than the cull radius from the home
-// atoms' bounding box (shift path), and append the rest to a per-wave LDS ring;
-// every full ring chunk of 64 j atoms is paired with all home atoms i, which
-// are uniform across the wave and read with scalar loads.  Home atoms in the
-// stream pair only with home atoms of lower index (unordered pairs, weight 2).
-// The histogram is a u64 LDS array shared by the workgroup's waves.
+// cell-sorted atoms (two runs where it wraps in x); runs that wrap the box are
+// streamed after those that do not.  Lanes load 64 stream atoms at a time,
+// drop neighbours farther than the cull radius from the home atoms' bounding
+// box (shift path), and append the rest to a per-wave LDS ring; every full
+// ring chunk of 64 j atoms is paired with all home atoms i, which are uniform
+// across the wave and read with scalar loads.  Home atoms in the stream pair
+// only with home atoms of lower index (unordered pairs, weight 2).  The
+// histogram is a u64 LDS array shared by the workgroup's waves.
 // -------------------------------------------------------------------------
-constexpr int kCellBlock = 512;
+constexpr int kCellBlock = 768;  // 12 waves, 3 per SIMD: two workgroups per CU within the kernel's SGPR budget
 constexpr int kCellWaves = kCellBlock / 64;
-constexpr int kRing = 128;  // per-wave ring entries (float4 each)
+constexpr int kRing = 128;       // per-wave ring entries (float4 each)
+constexpr int kMaxTab2 = 5120;   // split-table entries the cell kernel keeps in LDS
 
 struct CellLds {
     unsigned long long *hist;
+    uint2 *tab2;
     float *thr, *elo, *ehi;
-    unsigned short *tab;
     float4 *ring;  // [kCellWaves][kRing]
 };
 
 __host__ __device__ inline size_t cell_lds_bytes(int bins, bool edge, int tab_n) {
-    return align16((size_t)bins * 8) + align16(((size_t)bins + 1) * 4 * (edge ? 3 : 1)) +
-           align16((size_t)tab_n * 2) + (size_t)kCellWaves * kRing * 16;
+    return align16((size_t)bins * 8) + align16((size_t)tab_n * 8) +
+           align16(((size_t)bins + 1) * 4 * (edge ? 3 : 1)) + (size_t)kCellWaves * kRing * 16;
 }
 
-__device__ __forceinline__ CellLds cell_lds_layout(char *base, const HistParams &p, bool edge) {
+__device__ __forceinline__ CellLds cell_lds_layout(char *base, const HistParams &p, bool edge,
+                                                   bool table) {
     CellLds L;
     const size_t nb1 = (size_t)p.bins + 1;
     size_t off = 0;
     L.hist = (unsigned long long *)base;
     off += align16((size_t)p.bins * 8);
+    L.tab2 = (uint2 *)(base + off);
+    off += align16((size_t)(table ? p.tab_n : 0) * 8);
     L.thr = (float *)(base + off);
     L.elo = L.thr + nb1;
     L.ehi = L.elo + nb1;
     off += align16(nb1 * 4 * (edge ? 3 : 1));
-    L.tab = (unsigned short *)(base + off);
-    off += align16((size_t)p.tab_n * 2);
     L.ring = (float4 *)(base + off);
     return L;
 }
 
-// Exact bin from LDS copies of the thresholds / table (s < cut).
+// Exact bin of a counted pair (s < cut) from the LDS tables.
 template <bool TABLE>
-__device__ __forceinline__ int exact_bin_lds(float s, const float *thr, const unsigned short *tab,
-                                             const HistParams &p) {
+__device__ __forceinline__ int cell_bin(float s, const CellLds &L, const HistParams &p) {
     if constexpr (TABLE) {
-        const unsigned idx =
-            __builtin_elementwise_sub_sat(__float_as_uint(s), (unsigned)p.tab_base) >> p.tab_sh;
-        int b = tab[idx];
-        b += (int)(s >= thr[b + 1]);
-        return b;
+        const unsigned t = __builtin_elementwise_sub_sat(__float_as_uint(s), (unsigned)p.tab_base);
+        const uint2 e = L.tab2[t >> p.tab_sh];
+        return (int)e.y + (int)(s >= __uint_as_float(e.x));
     } else {
         float r_est = __builtin_amdgcn_sqrtf(s);
         int b = (int)(r_est * p.inv_w);
         b = b < p.bins - 1 ? b : p.bins - 1;
-        b += (int)(s >= thr[b + 1]) - (int)(s < thr[b]);
+        b += (int)(s >= L.thr[b + 1]) - (int)(s < L.thr[b]);
         return b;
     }
 }
 
+// Wave-wide min / max through DPP row operations; every lane must be active.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ float dpp_f(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), CTRL,
+                                                      ROWS, 0xF, false));
+}
 __device__ __forceinline__ float wave_min(float v) {
-    for (int off = 32; off > 0; off >>= 1) v = fminf(v, __shfl_xor(v, off, 64));
-    return v;
+    v = fminf(v, dpp_f<0xB1, 0xF>(v));   // quad_perm [1,0,3,2]
+    v = fminf(v, dpp_f<0x4E, 0xF>(v));   // quad_perm [2,3,0,1]
+    v = fminf(v, dpp_f<0x141, 0xF>(v));  // row_half_mirror
+    v = fminf(v, dpp_f<0x140, 0xF>(v));  // row_mirror
+    v = fminf(v, dpp_f<0x142, 0xA>(v));  // row_bcast:15 into rows 1, 3
+    v = fminf(v, dpp_f<0x143, 0xC>(v));  // row_bcast:31 into rows 2, 3
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
 }
 __device__ __forceinline__ float wave_max(float v) {
-    for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
-    return v;
+    v = fmaxf(v, dpp_f<0xB1, 0xF>(v));
+    v = fmaxf(v, dpp_f<0x4E, 0xF>(v));
+    v = fmaxf(v, dpp_f<0x141, 0xF>(v));
+    v = fmaxf(v, dpp_f<0x140, 0xF>(v));
+    v = fmaxf(v, dpp_f<0x142, 0xA>(v));
+    v = fmaxf(v, dpp_f<0x143, 0xC>(v));
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
 }
 
 // Number of set bits of m below this lane.
@@ -570,17 +587,37 @@ __device__ __forceinline__ int lanes_below(unsigned long long m) {
                                           __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
 }
 
-// Pairs every j of one ring chunk with every home atom i.
-// SHIFT: j carries its image shift n (3 x int8 in the meta word); otherwise
-// the rint wrap.  HOME: the chunk holds home atoms (stream positions below
-// n_home), which pair only with home atoms of lower index.
-template <int PBC, bool LT, bool WEIGHTED, bool EDGE, bool TABLE, bool SHIFT, bool HOME>
+// Counts one pair (s, i's meta word) of a chunk.
+template <bool WEIGHTED, bool EDGE, bool TABLE>
+__device__ __forceinline__ void cell_count(float s, unsigned mi, unsigned mAj, unsigned mBj,
+                                           const HistParams &p, const CellLds &L,
+                                           unsigned long long &edge_local) {
+    if (s < p.cut) {
+        unsigned w = 2u;
+        if constexpr (WEIGHTED) w = ((mi >> 24) & 0xFu) * mBj + (mi >> 28) * mAj;
+        const int bin = cell_bin<TABLE>(s, L, p);
+        atomicAdd(&L.hist[bin], (unsigned long long)w);
+        if constexpr (EDGE) {
+            bool near = (s >= L.elo[bin] && s < L.ehi[bin]) ||
+                        (s >= L.elo[bin + 1] && s < L.ehi[bin + 1]);
+            edge_local += near ? w : 0u;
+        }
+    }
+}
+
+// Pairs every j of one ring chunk with every home atom i (i unrolled by 4:
+// one 64-byte scalar load per group).
+// MODE 0: zero image shift (orthorhombic: dx = xj - xi); 1: j carries its
+// image shift n (3 x int8 in the meta word); 2: the rint wrap.
+// HOME: the chunk holds home atoms (stream positions below n_home), which
+// pair only with home atoms of lower index.
+template <int PBC, bool LT, bool WEIGHTED, bool EDGE, bool TABLE, int MODE, bool HOME>
 __device__ __forceinline__ void cell_chunk(const float4 *__restrict__ hp, int n_home, float xj,
                                            float yj, float zj, unsigned mj, int rank,
                                            const FrameBox &b, const HistParams &p,
                                            const CellLds &L, unsigned long long &edge_local) {
     float sx = 0.f, sy = 0.f, sz = 0.f;
-    if constexpr (SHIFT) {
+    if constexpr (MODE == 1) {
         sx = (float)sext8(mj);
         sy = (float)sext8(mj >> 8);
         sz = (float)sext8(mj >> 16);
@@ -591,31 +628,51 @@ __device__ __forceinline__ void cell_chunk(const float4 *__restrict__ hp, int n_
         }
     }
     const unsigned mAj = (mj >> 24) & 0xFu, mBj = mj >> 28;
-    for (int i = 0; i < n_home; ++i) {
-        const float4 pi = hp[i];
-        float s;
-        if constexpr (SHIFT)
-            s = pair_s_shift<PBC, LT>(pi.x, pi.y, pi.z, xj, yj, zj, sx, sy, sz, b);
-        else
-            s = pair_s<PBC, false>(pi.x, pi.y, pi.z, xj, yj, zj, b);
+    const float nanv = __builtin_nanf("");
+    auto dist = [&](const float4 &pi) {
+        if constexpr (MODE == 0 && PBC == WMD_PBC_ORTHO) {
+            const float dx = xj - pi.x, dy = yj - pi.y, dz = zj - pi.z;
+            return dx * dx + dy * dy + dz * dz;
+        } else if constexpr (MODE <= 1) {
+            return pair_s_shift<PBC, LT>(pi.x, pi.y, pi.z, xj, yj, zj, sx, sy, sz, b);
+        } else {
+            return pair_s<PBC, false>(pi.x, pi.y, pi.z, xj, yj, zj, b);
+        }
+    };
+    int i = 0;
+    for (; i + 4 <= n_home; i += 4) {
+        const float4 a0 = hp[i], a1 = hp[i + 1], a2 = hp[i + 2], a3 = hp[i + 3];
+        float s0 = dist(a0), s1 = dist(a1), s2 = dist(a2), s3 = dist(a3);
         if constexpr (HOME) {
-            if (i >= rank) s = __builtin_nanf("");
+            s0 = i >= rank ? nanv : s0;
+            s1 = i + 1 >= rank ? nanv : s1;
+            s2 = i + 2 >= rank ? nanv : s2;
+            s3 = i + 3 >= rank ? nanv : s3;
         }
-        if (s < p.cut) {
-            unsigned w = 2u;
-            if constexpr (WEIGHTED) {
-                const unsigned mi = __float_as_uint(pi.w);
-                w = ((mi >> 24) & 0xFu) * mBj + (mi >> 28) * mAj;
-            }
-            const int bin = exact_bin_lds<TABLE>(s, L.thr, L.tab, p);
-            atomicAdd(&L.hist[bin], (unsigned long long)w);
-            if constexpr (EDGE) {
-                bool near = (s >= L.elo[bin] && s < L.ehi[bin]) ||
-                            (s >= L.elo[bin + 1] && s < L.ehi[bin + 1]);
-                edge_local += near ? w : 0u;
-            }
-        }
+        cell_count<WEIGHTED, EDGE, TABLE>(s0, __float_as_uint(a0.w), mAj, mBj, p, L, edge_local);
+        cell_count<WEIGHTED, EDGE, TABLE>(s1, __float_as_uint(a1.w), mAj, mBj, p, L, edge_local);
+        cell_count<WEIGHTED, EDGE, TABLE>(s2, __float_as_uint(a2.w), mAj, mBj, p, L, edge_local);
+        cell_count<WEIGHTED, EDGE, TABLE>(s3, __float_as_uint(a3.w), mAj, mBj, p, L, edge_local);
     }
+    for (; i < n_home; ++i) {
+        const float4 a0 = hp[i];
+        float s0 = dist(a0);
+        if constexpr (HOME) s0 = i >= rank ? nanv : s0;
+        cell_count<WEIGHTED, EDGE, TABLE>(s0, __float_as_uint(a0.w), mAj, mBj, p, L, edge_local);
+    }
+}
+
+template <int PBC, bool LT, bool WEIGHTED, bool EDGE, bool TABLE, bool HOME>
+__device__ __forceinline__ void cell_chunk_mode(int mode, const float4 *__restrict__ hp, int n_home,
+                                                float xj, float yj, float zj, unsigned mj, int rank,
+                                                const FrameBox &b, const HistParams &p,
+                                                const CellLds &L, unsigned long long &edge_local) {
+    if (mode == 0)
+        cell_chunk<PBC, LT, WEIGHTED, EDGE, TABLE, 0, HOME>(hp, n_home, xj, yj, zj, mj, rank, b, p, L, edge_local);
+    else if (mode == 1)
+        cell_chunk<PBC, LT, WEIGHTED, EDGE, TABLE, 1, HOME>(hp, n_home, xj, yj, zj, mj, rank, b, p, L, edge_local);
+    else
+        cell_chunk<PBC, LT, WEIGHTED, EDGE, TABLE, 2, HOME>(hp, n_home, xj, yj, zj, mj, rank, b, p, L, edge_local);
 }
 
 template <int PBC, bool LT, bool WEIGHTED, bool EDGE, bool TABLE>
@@ -625,7 +682,7 @@ __global__ __launch_bounds__(kCellBlock) void rdf_cell_kernel(
     const FrameBox *__restrict__ boxes, int ncx, int ncy, int ncz, int k, int shift_ok,
     float cull2, HistParams p, int n_xcd) {
     extern __shared__ __attribute__((aligned(16))) char lds[];
-    const CellLds L = cell_lds_layout(lds, p, EDGE);
+    const CellLds L = cell_lds_layout(lds, p, EDGE, TABLE);
     for (int i = threadIdx.x; i < p.bins; i += kCellBlock) L.hist[i] = 0ull;
     for (int i = threadIdx.x; i <= p.bins; i += kCellBlock) {
         L.thr[i] = p.thr[i];
@@ -635,7 +692,7 @@ __global__ __launch_bounds__(kCellBlock) void rdf_cell_kernel(
         }
     }
     if constexpr (TABLE)
-        for (int i = threadIdx.x; i < p.tab_n; i += kCellBlock) L.tab[i] = p.tab[i];
+        for (int i = threadIdx.x; i < p.tab_n; i += kCellBlock) L.tab2[i] = p.tab2[i];
     __syncthreads();
 
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform: unit indices in SGPRs
@@ -647,6 +704,20 @@ __global__ __launch_bounds__(kCellBlock) void rdf_cell_kernel(
     // rows of the half stencil: k rows (0, dy > 0), k (2k + 1) rows dz > 0, the home row
     const int n_rows = 2 * k * k + 2 * k + 1;
     const int n_pieces = 2 * n_rows;
+    // this lane's piece geometry (lanes 1..n_pieces; odd: the in-range part of
+    // a row, even: its wrapped part): row offsets and the x range relative to hx
+    int g_dy = 0, g_dz = 0, g_xa = 0, g_xb = -1;
+    if (lane >= 1 && lane <= n_pieces) {
+        const int r = (lane - 1) >> 1;
+        if (r < k) {
+            g_dz = 0; g_dy = r + 1; g_xa = -k; g_xb = k;
+        } else if (r < n_rows - 1) {
+            const int q = r - k;
+            g_dz = 1 + q / (2 * k + 1); g_dy = q % (2 * k + 1) - k; g_xa = -k; g_xb = k;
+        } else {
+            g_dz = 0; g_dy = 0; g_xa = 1; g_xb = k;
+        }
+    }
     // XCD-aware static schedule over waves: workgroup g runs on XCD g % n_xcd
     const long long n_units = (long long)n_frames * ncell;
     const int xcd = blockIdx.x % n_xcd;
@@ -687,26 +758,17 @@ __global__ __launch_bounds__(kCellBlock) void rdf_cell_kernel(
         mx0 = wave_max(mx0); mx1 = wave_max(mx1); mx2 = wave_max(mx2);
         const int mhx = sext8(m0), mhy = sext8(m0 >> 8), mhz = sext8(m0 >> 16);
 
-        // the runs of the half stencil, one per lane
-        int pbeg = 0, plen = 0, pwrap = 0x15;
-        if (lane < n_pieces) {
-            const int r = lane >> 1;
-            int dy, dz, xa, xb;
-            if (r < k) {
-                dz = 0; dy = r + 1; xa = hx - k; xb = hx + k;
-            } else if (r < n_rows - 1) {
-                const int q = r - k;
-                dz = 1 + q / (2 * k + 1); dy = q % (2 * k + 1) - k; xa = hx - k; xb = hx + k;
-            } else {
-                dz = 0; dy = 0; xa = hx + 1; xb = hx + k;
-            }
-            int cy = hy + dy, cz = hz + dz;
+        // the stream's pieces: lane 0 the home cell, lanes 1..n_pieces the runs
+        int pbeg = hs, plen = n_home, pwrap = 0x15;
+        if (lane >= 1 && lane <= n_pieces) {
+            int cy = hy + g_dy, cz = hz + g_dz;
             const int wy = cy < 0 ? -1 : (cy >= ncy ? 1 : 0);
             const int wz = cz < 0 ? -1 : (cz >= ncz ? 1 : 0);
             cy -= wy * ncy;
             cz -= wz * ncz;
+            const int xa = hx + g_xa, xb = hx + g_xb;
             int lo, hi, wx = 0;
-            if ((lane & 1) == 0) {
+            if (lane & 1) {
                 lo = max(xa, 0); hi = min(xb, ncx - 1);
             } else if (xa < 0) {
                 lo = xa + ncx; hi = ncx - 1; wx = -1;
@@ -715,6 +777,8 @@ __global__ __launch_bounds__(kCellBlock) void rdf_cell_kernel(
             } else {
                 lo = 1; hi = 0;
             }
+            pbeg = 0;
+            plen = 0;
             if (lo <= hi) {
                 const int rowbase = (cz * ncy + cy) * ncx;
                 pbeg = (int)fs[rowbase + lo];
@@ -722,27 +786,65 @@ __global__ __launch_bounds__(kCellBlock) void rdf_cell_kernel(
             }
             pwrap = (wx + 1) | ((wy + 1) << 2) | ((wz + 1) << 4);
         }
-
-        // stream: home atoms (piece -1), then the runs; ring positions head..tail
-        int head = 0, tail = 0;
-        int l = -1, b0 = 0, base = hs, len = n_home, wrap = 0x15;
-        while (true) {
-            while (b0 >= len) {
-                if (++l >= n_pieces) break;
-                base = __builtin_amdgcn_readlane(pbeg, l);
-                len = __builtin_amdgcn_readlane(plen, l);
-                wrap = __builtin_amdgcn_readlane(pwrap, l);
-                b0 = 0;
+        // stream order: home, runs without a box wrap, runs with one (so that
+        // chunks of unshifted atoms come first); permuted through the empty ring
+        {
+            const bool act = lane <= n_pieces;
+            const bool wrapped = pwrap != 0x15;
+            const unsigned long long b0m = __ballot(act && !wrapped);
+            const unsigned long long b1m = __ballot(act && wrapped);
+            const int dst = wrapped ? (int)__popcll(b0m) + lanes_below(b1m) : lanes_below(b0m);
+            if (act) ring[dst] = make_float4(__int_as_float(pbeg), __int_as_float(plen), __int_as_float(pwrap), 0.f);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            if (act) {
+                const float4 t = ring[lane];
+                pbeg = __float_as_int(t.x);
+                plen = __float_as_int(t.y);
+                pwrap = __float_as_int(t.z);
             }
-            const bool more = l < n_pieces;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+
+        // Stream positions P0 .. P0 + 63 per step, across piece boundaries.
+        // Uniform cursor: piece c covers stream positions [c_off, c_end).
+        int head = 0, tail = 0;  // ring positions
+        int c = 0, c_beg = hs, c_off = 0, c_end = n_home, c_wrap = 0x15;
+        int P0 = 0;
+        while (true) {
+            while (c_end <= P0 && c < n_pieces) {
+                ++c;
+                c_beg = __builtin_amdgcn_readlane(pbeg, c);
+                c_off = c_end;
+                c_end += __builtin_amdgcn_readlane(plen, c);
+                c_wrap = __builtin_amdgcn_readlane(pwrap, c);
+            }
+            const bool more = c_end > P0;
             if (more) {
-                const int idx = b0 + lane;
-                const bool valid = idx < len;
+                const int pos = P0 + lane;
+                int src = c_beg + (pos - c_off);
+                int wrap = c_wrap;
+                int e = c, e_end = c_end;
+                while (e_end < P0 + 64 && e < n_pieces) {
+                    ++e;
+                    const int eb = __builtin_amdgcn_readlane(pbeg, e);
+                    const int el = __builtin_amdgcn_readlane(plen, e);
+                    const int ew = __builtin_amdgcn_readlane(pwrap, e);
+                    if (pos >= e_end) {
+                        src = eb + (pos - e_end);
+                        wrap = ew;
+                    }
+                    e_end += el;
+                }
+                const bool valid = pos < e_end;
                 float4 q = make_float4(nanv, nanv, nanv, 0.f);
-                if (valid) q = fp[base + idx];
+                if (valid) q = fp[src];
                 unsigned meta = __float_as_uint(q.w);
                 bool keep = valid;
-                if (l < 0 || !use_shift) {
+                if (pos < n_home || !use_shift) {
                     meta &= 0xFF000000u;  // multiplicities only (zero shift)
                 } else {
                     const int nx = sext8(meta) - ((wrap & 3) - 1) - mhx;
@@ -771,30 +873,26 @@ __global__ __launch_bounds__(kCellBlock) void rdf_cell_kernel(
                     ring[(tail + lanes_below(bal)) & (kRing - 1)] =
                         make_float4(q.x, q.y, q.z, __uint_as_float(meta));
                 tail += (int)__popcll(bal);
-                b0 += 64;
+                P0 += 64;
             }
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             while (tail - head >= 64 || (!more && tail > head)) {
                 const int cnt = min(64, tail - head);
-                const float4 e = ring[(head + lane) & (kRing - 1)];
+                const float4 ej = ring[(head + lane) & (kRing - 1)];
                 const bool v = lane < cnt;
-                const float xj = v ? e.x : nanv;
-                const float yj = v ? e.y : nanv;
-                const float zj = v ? e.z : nanv;
-                const unsigned mj = __float_as_uint(e.w);
+                const float xj = v ? ej.x : nanv;
+                const float yj = v ? ej.y : nanv;
+                const float zj = v ? ej.z : nanv;
+                const unsigned mj = v ? __float_as_uint(ej.w) : 0u;
+                // 0: no lane has an image shift; 1: some lane has one; 2: rint wrap
+                const int mode = !use_shift ? 2 : (__any((mj & 0xFFFFFFu) != 0u) ? 1 : 0);
                 if (head < n_home) {
                     const int rank = head + lane < n_home ? head + lane : 0x7FFFFFFF;
-                    if (use_shift)
-                        cell_chunk<PBC, LT, WEIGHTED, EDGE, TABLE, true, true>(hp, n_home, xj, yj, zj, mj, rank, b, p, L, edge_local);
-                    else
-                        cell_chunk<PBC, LT, WEIGHTED, EDGE, TABLE, false, true>(hp, n_home, xj, yj, zj, mj, rank, b, p, L, edge_local);
+                    cell_chunk_mode<PBC, LT, WEIGHTED, EDGE, TABLE, true>(mode, hp, n_home, xj, yj, zj, mj, rank, b, p, L, edge_local);
                 } else {
-                    if (use_shift)
-                        cell_chunk<PBC, LT, WEIGHTED, EDGE, TABLE, true, false>(hp, n_home, xj, yj, zj, mj, 0, b, p, L, edge_local);
-                    else
-                        cell_chunk<PBC, LT, WEIGHTED, EDGE, TABLE, false, false>(hp, n_home, xj, yj, zj, mj, 0, b, p, L, edge_local);
+                    cell_chunk_mode<PBC, LT, WEIGHTED, EDGE, TABLE, false>(mode, hp, n_home, xj, yj, zj, mj, 0, b, p, L, edge_local);
                 }
                 head += cnt;
             }
@@ -892,11 +990,12 @@ struct wmd_rdf {
     uint32_t flags = 0;
     std::vector<float> thr, elo, ehi;
     std::vector<unsigned short> tab;
+    std::vector<uint2> tab2;
     int tab_sh = 0, tab_base = 0;
     int cell_k = 0;        // 0 = automatic
     int last_path = 0;
     int last_nc[3] = {0, 0, 0};
-    DevBuf d_counts, d_thr, d_elo, d_ehi, d_tab, d_wts, d_io, d_boxes, d_aos, d_soa;
+    DevBuf d_counts, d_thr, d_elo, d_ehi, d_tab, d_tab2, d_wts, d_io, d_boxes, d_aos, d_soa;
     DevBuf d_cell_of, d_cell_slot, d_cell_image, d_cell_count, d_cell_start, d_frame_flag,
         d_sorted, d_tiles;
     HostBuf h_stage, h_boxes;
@@ -912,6 +1011,7 @@ int hist_copies_for(int) { return 1; }
 // patterns in [T[1], cut) holds at most one breakpoint T[1..bins-1].
 void build_table(wmd_rdf *h) {
     h->tab.clear();
+    h->tab2.clear();
     if (h->bins < 2) return;
     const uint32_t lo_bits = f_bits(h->thr[1]);
     const uint32_t hi_bits = f_bits(h->cut) - 1;  // largest s below the cut
@@ -931,7 +1031,18 @@ void build_table(wmd_rdf *h) {
             while (b + 1 < h->bins && s0 >= h->thr[b + 1]) ++b;
             tab[k] = (unsigned short)b;
         }
+        // split table: the breakpoint inside each bucket (+inf if none) and the
+        // bin of the bucket's values below it
+        std::vector<uint2> tab2(n);
+        for (int k = 0; k < n; ++k) {
+            const int bk = tab[k];
+            uint32_t split = 0x7F800000u;
+            if (bk + 1 <= h->bins && (f_bits(h->thr[bk + 1]) >> sh) == (uint32_t)(base + k))
+                split = f_bits(h->thr[bk + 1]);
+            tab2[k] = make_uint2(split, (unsigned)bk);
+        }
         h->tab.swap(tab);
+        h->tab2.swap(tab2);
         h->tab_sh = sh;
         h->tab_base = base << sh;  // bit pattern where bucket 0 starts
         return;
@@ -964,6 +1075,7 @@ HistParams hist_params(wmd_rdf *h, bool table) {
     p.elo = h->d_elo.as<float>();
     p.ehi = h->d_ehi.as<float>();
     p.tab = table ? h->d_tab.as<unsigned short>() : nullptr;
+    p.tab2 = table ? h->d_tab2.as<uint2>() : nullptr;
     p.tab_n = table ? (int)h->tab.size() : 0;
     p.tab_sh = h->tab_sh;
     p.tab_base = h->tab_base;
@@ -997,7 +1109,7 @@ void launch_cells_t(wmd_rdf *h, hipStream_t st, const float4 *sorted, const unsi
                     const int *nc, int k, int shift_ok, float cull2, const HistParams &p) {
     auto kern = rdf_cell_kernel<PBC, LT, WEIGHTED, EDGE, TABLE>;
     ProfScope prof(h->ctx, st, "rdf_pair");
-    size_t lds = cell_lds_bytes(p.bins, EDGE, p.tab_n);
+    size_t lds = cell_lds_bytes(p.bins, EDGE, TABLE ? p.tab_n : 0);
     WMD_REQUIRE(lds <= h->ctx->lds_per_block, WMD_ERR_UNSUPPORTED, "too many bins for LDS");
     int occ = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, kCellBlock, lds) != hipSuccess || occ < 1)
@@ -1005,6 +1117,7 @@ void launch_cells_t(wmd_rdf *h, hipStream_t st, const float4 *sorted, const unsi
     const long long n_units = (long long)nf * nc[0] * nc[1] * nc[2];
     int grid = std::max(8, h->ctx->cu_count * occ / 8 * 8);
     const long long max_grid = (n_units + kCellWaves * 8 - 1) / (kCellWaves * 8) * 8;
+    WMD_REQUIRE(k >= 1 && 2 * (2 * k * k + 2 * k + 1) < 64, WMD_ERR_INVALID, "cell stencil reach out of range");
     if (grid > max_grid) grid = (int)std::max<long long>(8, max_grid);
     hipLaunchKernelGGL(kern, dim3(grid), dim3(kCellBlock), lds, st, sorted, start, frame_flag, n, nf,
                        boxes, nc[0], nc[1], nc[2], k, shift_ok, cull2, p, 8);
@@ -1186,6 +1299,7 @@ void accumulate_device(wmd_rdf *h, const float *d_soa, size_t stride, size_t n_f
     const unsigned *d_w = h->weighted ? h->d_wts.as<unsigned>() : nullptr;
     long long fs = 3 * (long long)stride;
     const bool lt = h->pbc == WMD_PBC_TRICLINIC && lower_tri;
+    const bool cell_table = table && (int)h->tab.size() <= kMaxTab2;
     for (size_t f0 = 0; f0 < n_frames; f0 += batch) {
         int nf = (int)std::min(batch, n_frames - f0);
         int g;
@@ -1207,7 +1321,7 @@ void accumulate_device(wmd_rdf *h, const float *d_soa, size_t stride, size_t n_f
             hipLaunchKernelGGL(cell_scatter_kernel, dim3(g), dim3(kBlock), 0, st, d_soa, fs, (int)stride, n, nf, (long long)f0, d_cell_of, d_cell_slot, d_image, d_start, ncell, d_w, d_sorted);
             WMD_HIP_CHECK(hipGetLastError());
         }
-        cells_dispatch(h->pbc, lt, h->weighted, edge, table, h, st, (const float4 *)d_sorted,
+        cells_dispatch(h->pbc, lt, h->weighted, edge, cell_table, h, st, (const float4 *)d_sorted,
                        (const unsigned *)d_start, (const unsigned *)d_fflag, n, nf, bx,
                        (const int *)nc, k_use, shift_ok, cull2, p);
     }
@@ -1337,6 +1451,8 @@ wmd_status wmd_rdf_create(wmd_ctx *ctx, const uint32_t *sel_a, size_t n_a, const
             if (!h->tab.empty()) {
                 h->d_tab.reserve(h->tab.size() * 2);
                 WMD_HIP_CHECK(hipMemcpyAsync(h->d_tab.ptr, h->tab.data(), h->tab.size() * 2, hipMemcpyHostToDevice, st));
+                h->d_tab2.reserve(h->tab2.size() * 8);
+                WMD_HIP_CHECK(hipMemcpyAsync(h->d_tab2.ptr, h->tab2.data(), h->tab2.size() * 8, hipMemcpyHostToDevice, st));
             }
         }
         size_t nio = std::max<size_t>(h->io.size(), 1);
