@@ -63,6 +63,7 @@ constexpr int kTile = 256;        // brute-force tile
 constexpr int kMaxTab = 8192;     // s-bits bin table entries (u16)
 constexpr uint32_t kPad = 0xFFFFFFFFu;
 constexpr int kImageLimit = 8;  // |image index| beyond which a frame uses the rint path
+constexpr int kImageBias = 32;  // image indices are stored as m + 32 in one byte
 
 struct FrameBox {
     float lx, ly, lz, ilx, ily, ilz;
@@ -79,7 +80,7 @@ struct HistParams {
     const float *elo; // [bins + 1] edge diagnostics (EDGE variants)
     const float *ehi;
     const unsigned short *tab;  // [tab_n] s-bits bin table (TABLE variants)
-    const uint2 *tab2;          // [tab_n] per bucket (split threshold bits, bin below it)
+    const uint2 *tab2;          // [tab_n] per bucket (split threshold bits, 8 x bin below it)
     int tab_n, tab_sh, tab_base;
     unsigned long long *counts;  // [bins]
     unsigned long long *edge;    // [1]
@@ -368,7 +369,7 @@ __global__ __launch_bounds__(kBlock) void rdf_brute_kernel(
 // -------------------------------------------------------------------------
 // Cell list construction.
 // cell key = (cz * ncy + cy) * ncx + cx from the wrapped fractional position;
-// meta word of a sorted atom: image index (mx, my, mz) as 3 x int8 in bits
+// meta word of a sorted atom: image index (mx, my, mz) + 32 as 3 x u8 in bits
 // 0..23, selection multiplicities mA in bits 24..27 and mB in bits 28..31.
 // -------------------------------------------------------------------------
 template <int PBC>
@@ -424,7 +425,7 @@ __global__ __launch_bounds__(kBlock) void cell_assign_kernel(
             bool ok = fl >= -(float)kImageLimit && fl <= (float)kImageLimit;
             int m = ok ? (int)fl : 0;
             far_out |= !ok;
-            img |= ((unsigned)(m & 0xFF)) << (8 * d);
+            img |= ((unsigned)(m + kImageBias) & 0xFFu) << (8 * d);
         }
         if (far_out) atomicOr(&frame_flag[f], 1u);
         int cid = (c[2] * ncy + c[1]) * ncx + c[0];
@@ -489,7 +490,6 @@ __global__ __launch_bounds__(kBlock) void cell_scatter_kernel(
     }
 }
 
-__device__ __forceinline__ int sext8(unsigned v) { return (int)(signed char)(v & 0xFFu); }
 
 // -------------------------------------------------------------------------
 // Cell-list pair kernel.  One wave per unit (frame, home cell); no block-level
@@ -540,45 +540,50 @@ __device__ __forceinline__ CellLds cell_lds_layout(char *base, const HistParams 
     return L;
 }
 
-// Exact bin of a counted pair (s < cut) from the LDS tables.
+// Byte offset (8 x bin) of the exact bin of a counted pair (s < cut).
 template <bool TABLE>
-__device__ __forceinline__ int cell_bin(float s, const CellLds &L, const HistParams &p) {
+__device__ __forceinline__ unsigned cell_bin8(float s, const CellLds &L, const HistParams &p) {
     if constexpr (TABLE) {
         const unsigned t = __builtin_elementwise_sub_sat(__float_as_uint(s), (unsigned)p.tab_base);
         const uint2 e = L.tab2[t >> p.tab_sh];
-        return (int)e.y + (int)(s >= __uint_as_float(e.x));
+        return e.y + (s >= __uint_as_float(e.x) ? 8u : 0u);
     } else {
         float r_est = __builtin_amdgcn_sqrtf(s);
         int b = (int)(r_est * p.inv_w);
         b = b < p.bins - 1 ? b : p.bins - 1;
         b += (int)(s >= L.thr[b + 1]) - (int)(s < L.thr[b]);
-        return b;
+        return (unsigned)b * 8u;
     }
 }
 
-// Wave-wide min / max through DPP row operations; every lane must be active.
+// Wave-wide min / max of non-NaN floats through DPP row operations on their
+// order-preserving integer keys; every lane must be active.
+__device__ __forceinline__ int f_key(float x) {
+    const int b = __float_as_int(x);
+    return b ^ ((b >> 31) & 0x7FFFFFFF);
+}
+__device__ __forceinline__ float key_f(int k) { return __int_as_float(k ^ ((k >> 31) & 0x7FFFFFFF)); }
 template <int CTRL, int ROWS>
-__device__ __forceinline__ float dpp_f(float v) {
-    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), CTRL,
-                                                      ROWS, 0xF, false));
+__device__ __forceinline__ int dpp_i(int v) {
+    return __builtin_amdgcn_update_dpp(v, v, CTRL, ROWS, 0xF, false);
 }
-__device__ __forceinline__ float wave_min(float v) {
-    v = fminf(v, dpp_f<0xB1, 0xF>(v));   // quad_perm [1,0,3,2]
-    v = fminf(v, dpp_f<0x4E, 0xF>(v));   // quad_perm [2,3,0,1]
-    v = fminf(v, dpp_f<0x141, 0xF>(v));  // row_half_mirror
-    v = fminf(v, dpp_f<0x140, 0xF>(v));  // row_mirror
-    v = fminf(v, dpp_f<0x142, 0xA>(v));  // row_bcast:15 into rows 1, 3
-    v = fminf(v, dpp_f<0x143, 0xC>(v));  // row_bcast:31 into rows 2, 3
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+__device__ __forceinline__ float wave_min_key(int v) {
+    v = min(v, dpp_i<0xB1, 0xF>(v));   // quad_perm [1,0,3,2]
+    v = min(v, dpp_i<0x4E, 0xF>(v));   // quad_perm [2,3,0,1]
+    v = min(v, dpp_i<0x141, 0xF>(v));  // row_half_mirror
+    v = min(v, dpp_i<0x140, 0xF>(v));  // row_mirror
+    v = min(v, dpp_i<0x142, 0xA>(v));  // row_bcast:15 into rows 1, 3
+    v = min(v, dpp_i<0x143, 0xC>(v));  // row_bcast:31 into rows 2, 3
+    return key_f(__builtin_amdgcn_readlane(v, 63));
 }
-__device__ __forceinline__ float wave_max(float v) {
-    v = fmaxf(v, dpp_f<0xB1, 0xF>(v));
-    v = fmaxf(v, dpp_f<0x4E, 0xF>(v));
-    v = fmaxf(v, dpp_f<0x141, 0xF>(v));
-    v = fmaxf(v, dpp_f<0x140, 0xF>(v));
-    v = fmaxf(v, dpp_f<0x142, 0xA>(v));
-    v = fmaxf(v, dpp_f<0x143, 0xC>(v));
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+__device__ __forceinline__ float wave_max_key(int v) {
+    v = max(v, dpp_i<0xB1, 0xF>(v));
+    v = max(v, dpp_i<0x4E, 0xF>(v));
+    v = max(v, dpp_i<0x141, 0xF>(v));
+    v = max(v, dpp_i<0x140, 0xF>(v));
+    v = max(v, dpp_i<0x142, 0xA>(v));
+    v = max(v, dpp_i<0x143, 0xC>(v));
+    return key_f(__builtin_amdgcn_readlane(v, 63));
 }
 
 // Number of set bits of m below this lane.
@@ -595,9 +600,10 @@ __device__ __forceinline__ void cell_count(float s, unsigned mi, unsigned mAj, u
     if (s < p.cut) {
         unsigned w = 2u;
         if constexpr (WEIGHTED) w = ((mi >> 24) & 0xFu) * mBj + (mi >> 28) * mAj;
-        const int bin = cell_bin<TABLE>(s, L, p);
-        atomicAdd(&L.hist[bin], (unsigned long long)w);
+        const unsigned off = cell_bin8<TABLE>(s, L, p);
+        atomicAdd((unsigned long long *)((char *)L.hist + off), (unsigned long long)w);
         if constexpr (EDGE) {
+            const unsigned bin = off >> 3;
             bool near = (s >= L.elo[bin] && s < L.ehi[bin]) ||
                         (s >= L.elo[bin + 1] && s < L.ehi[bin + 1]);
             edge_local += near ? w : 0u;
@@ -608,7 +614,7 @@ __device__ __forceinline__ void cell_count(float s, unsigned mi, unsigned mAj, u
 // Pairs every j of one ring chunk with every home atom i (i unrolled by 4:
 // one 64-byte scalar load per group).
 // MODE 0: zero image shift (orthorhombic: dx = xj - xi); 1: j carries its
-// image shift n (3 x int8 in the meta word); 2: the rint wrap.
+// image shift n (n + 128 as 3 x u8 in the meta word); 2: the rint wrap.
 // HOME: the chunk holds home atoms (stream positions below n_home), which
 // pair only with home atoms of lower index.
 template <int PBC, bool LT, bool WEIGHTED, bool EDGE, bool TABLE, int MODE, bool HOME>
@@ -618,9 +624,9 @@ __device__ __forceinline__ void cell_chunk(const float4 *__restrict__ hp, int n_
                                            const CellLds &L, unsigned long long &edge_local) {
     float sx = 0.f, sy = 0.f, sz = 0.f;
     if constexpr (MODE == 1) {
-        sx = (float)sext8(mj);
-        sy = (float)sext8(mj >> 8);
-        sz = (float)sext8(mj >> 16);
+        sx = (float)((int)(mj & 0xFFu) - 128);  // image shift n + 128 per byte
+        sy = (float)((int)((mj >> 8) & 0xFFu) - 128);
+        sz = (float)((int)((mj >> 16) & 0xFFu) - 128);
         if constexpr (PBC == WMD_PBC_ORTHO) {
             sx *= b.lx;  // the reference's f32 product round(dx/L) * L
             sy *= b.ly;
@@ -739,27 +745,33 @@ __global__ __launch_bounds__(kCellBlock) void rdf_cell_kernel(
         const float4 *hp = fp + hs;
         const FrameBox b = boxes[f];
 
-        // home atoms: bounding box and image uniformity
+        // home atoms: bounding box and image uniformity (the box is only used
+        // on the shift path, whose frames hold finite coordinates)
         const unsigned m0 = __float_as_uint(hp[0].w) & 0xFFFFFFu;
-        float mn0 = INFINITY, mn1 = INFINITY, mn2 = INFINITY;
-        float mx0 = -INFINITY, mx1 = -INFINITY, mx2 = -INFINITY;
+        int kn0 = 0x7FFFFFFF, kn1 = 0x7FFFFFFF, kn2 = 0x7FFFFFFF;
+        int kx0 = (int)0x80000000, kx1 = (int)0x80000000, kx2 = (int)0x80000000;
         bool diff = false;
         for (int hb = 0; hb < n_home; hb += 64) {
             if (hb + lane < n_home) {
                 const float4 q = hp[hb + lane];
-                mn0 = fminf(mn0, q.x); mx0 = fmaxf(mx0, q.x);
-                mn1 = fminf(mn1, q.y); mx1 = fmaxf(mx1, q.y);
-                mn2 = fminf(mn2, q.z); mx2 = fmaxf(mx2, q.z);
+                const int k0 = f_key(q.x), k1 = f_key(q.y), k2 = f_key(q.z);
+                kn0 = min(kn0, k0); kx0 = max(kx0, k0);
+                kn1 = min(kn1, k1); kx1 = max(kx1, k1);
+                kn2 = min(kn2, k2); kx2 = max(kx2, k2);
                 diff |= (__float_as_uint(q.w) & 0xFFFFFFu) != m0;
             }
         }
         const bool use_shift = shift_ok && frame_flag[f] == 0u && !__any(diff);
-        mn0 = wave_min(mn0); mn1 = wave_min(mn1); mn2 = wave_min(mn2);
-        mx0 = wave_max(mx0); mx1 = wave_max(mx1); mx2 = wave_max(mx2);
-        const int mhx = sext8(m0), mhy = sext8(m0 >> 8), mhz = sext8(m0 >> 16);
+        const float mn0 = wave_min_key(kn0), mn1 = wave_min_key(kn1), mn2 = wave_min_key(kn2);
+        const float mx0 = wave_max_key(kx0), mx1 = wave_max_key(kx1), mx2 = wave_max_key(kx2);
+        // per-piece byte constants K: n + 128 = (m_j + 32) + K per byte, with
+        // K = 128 - w - (m_home + 32) (no carries: every byte stays below 256)
+        const unsigned k_home = ((128u - (m0 & 0xFFu)) & 0xFFu) | (((128u - ((m0 >> 8) & 0xFFu)) & 0xFFu) << 8) |
+                                (((128u - ((m0 >> 16) & 0xFFu)) & 0xFFu) << 16);
 
         // the stream's pieces: lane 0 the home cell, lanes 1..n_pieces the runs
-        int pbeg = hs, plen = n_home, pwrap = 0x15;
+        int pbeg = hs, plen = n_home;
+        unsigned pk = k_home;
         if (lane >= 1 && lane <= n_pieces) {
             int cy = hy + g_dy, cz = hz + g_dz;
             const int wy = cy < 0 ? -1 : (cy >= ncy ? 1 : 0);
@@ -784,17 +796,18 @@ __global__ __launch_bounds__(kCellBlock) void rdf_cell_kernel(
                 pbeg = (int)fs[rowbase + lo];
                 plen = (int)fs[rowbase + hi + 1] - pbeg;
             }
-            pwrap = (wx + 1) | ((wy + 1) << 2) | ((wz + 1) << 4);
+            // K per byte: 128 - w - (m_home + 32); w = +1 subtracts 1, w = -1 adds 1
+            pk = k_home + (unsigned)(-wx) + ((unsigned)(-wy) << 8) + ((unsigned)(-wz) << 16);
         }
         // stream order: home, runs without a box wrap, runs with one (so that
         // chunks of unshifted atoms come first); permuted through the empty ring
         {
             const bool act = lane <= n_pieces;
-            const bool wrapped = pwrap != 0x15;
+            const bool wrapped = pk != k_home;
             const unsigned long long b0m = __ballot(act && !wrapped);
             const unsigned long long b1m = __ballot(act && wrapped);
             const int dst = wrapped ? (int)__popcll(b0m) + lanes_below(b1m) : lanes_below(b0m);
-            if (act) ring[dst] = make_float4(__int_as_float(pbeg), __int_as_float(plen), __int_as_float(pwrap), 0.f);
+            if (act) ring[dst] = make_float4(__int_as_float(pbeg), __int_as_float(plen), __uint_as_float(pk), 0.f);
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -802,7 +815,7 @@ __global__ __launch_bounds__(kCellBlock) void rdf_cell_kernel(
                 const float4 t = ring[lane];
                 pbeg = __float_as_int(t.x);
                 plen = __float_as_int(t.y);
-                pwrap = __float_as_int(t.z);
+                pk = __float_as_uint(t.z);
             }
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
@@ -812,7 +825,8 @@ __global__ __launch_bounds__(kCellBlock) void rdf_cell_kernel(
         // Stream positions P0 .. P0 + 63 per step, across piece boundaries.
         // Uniform cursor: piece c covers stream positions [c_off, c_end).
         int head = 0, tail = 0;  // ring positions
-        int c = 0, c_beg = hs, c_off = 0, c_end = n_home, c_wrap = 0x15;
+        int c = 0, c_beg = hs, c_off = 0, c_end = n_home;
+        unsigned c_k = k_home;
         int P0 = 0;
         while (true) {
             while (c_end <= P0 && c < n_pieces) {
@@ -820,22 +834,22 @@ __global__ __launch_bounds__(kCellBlock) void rdf_cell_kernel(
                 c_beg = __builtin_amdgcn_readlane(pbeg, c);
                 c_off = c_end;
                 c_end += __builtin_amdgcn_readlane(plen, c);
-                c_wrap = __builtin_amdgcn_readlane(pwrap, c);
+                c_k = __builtin_amdgcn_readlane(pk, c);
             }
             const bool more = c_end > P0;
             if (more) {
                 const int pos = P0 + lane;
                 int src = c_beg + (pos - c_off);
-                int wrap = c_wrap;
+                unsigned kk = c_k;
                 int e = c, e_end = c_end;
                 while (e_end < P0 + 64 && e < n_pieces) {
                     ++e;
                     const int eb = __builtin_amdgcn_readlane(pbeg, e);
                     const int el = __builtin_amdgcn_readlane(plen, e);
-                    const int ew = __builtin_amdgcn_readlane(pwrap, e);
+                    const unsigned ek = __builtin_amdgcn_readlane(pk, e);
                     if (pos >= e_end) {
                         src = eb + (pos - e_end);
-                        wrap = ew;
+                        kk = ek;
                     }
                     e_end += el;
                 }
@@ -844,29 +858,29 @@ __global__ __launch_bounds__(kCellBlock) void rdf_cell_kernel(
                 if (valid) q = fp[src];
                 unsigned meta = __float_as_uint(q.w);
                 bool keep = valid;
-                if (pos < n_home || !use_shift) {
-                    meta &= 0xFF000000u;  // multiplicities only (zero shift)
+                if (!use_shift) {
+                    meta = (meta & 0xFF000000u) | 0x808080u;  // multiplicities only
                 } else {
-                    const int nx = sext8(meta) - ((wrap & 3) - 1) - mhx;
-                    const int ny = sext8(meta >> 8) - (((wrap >> 2) & 3) - 1) - mhy;
-                    const int nz = sext8(meta >> 16) - (((wrap >> 4) & 3) - 1) - mhz;
+                    // n + 128 per byte; home atoms get n = 0 and lie inside the box
+                    const unsigned nb = (meta & 0xFFFFFFu) + kk;
+                    meta = (meta & 0xFF000000u) | nb;
+                    const float fx = (float)(nb & 0xFFu) - 128.f;
+                    const float fy = (float)((nb >> 8) & 0xFFu) - 128.f;
+                    const float fz = (float)((nb >> 16) & 0xFFu) - 128.f;
                     float px, py, pz;  // image of j next to the home atoms (approximate)
                     if constexpr (PBC == WMD_PBC_ORTHO) {
-                        px = q.x - (float)nx * b.lx;
-                        py = q.y - (float)ny * b.ly;
-                        pz = q.z - (float)nz * b.lz;
+                        px = __builtin_fmaf(-fx, b.lx, q.x);
+                        py = __builtin_fmaf(-fy, b.ly, q.y);
+                        pz = __builtin_fmaf(-fz, b.lz, q.z);
                     } else {
-                        const float fx = (float)nx, fy = (float)ny, fz = (float)nz;
-                        px = q.x - (fx * b.c[0] + fy * b.c[3] + fz * b.c[6]);
-                        py = q.y - (fx * b.c[1] + fy * b.c[4] + fz * b.c[7]);
-                        pz = q.z - (fx * b.c[2] + fy * b.c[5] + fz * b.c[8]);
+                        px = __builtin_fmaf(-fx, b.c[0], __builtin_fmaf(-fy, b.c[3], __builtin_fmaf(-fz, b.c[6], q.x)));
+                        py = __builtin_fmaf(-fx, b.c[1], __builtin_fmaf(-fy, b.c[4], __builtin_fmaf(-fz, b.c[7], q.y)));
+                        pz = __builtin_fmaf(-fx, b.c[2], __builtin_fmaf(-fy, b.c[5], __builtin_fmaf(-fz, b.c[8], q.z)));
                     }
                     const float ex = fmaxf(fmaxf(mn0 - px, px - mx0), 0.f);
                     const float ey = fmaxf(fmaxf(mn1 - py, py - mx1), 0.f);
                     const float ez = fmaxf(fmaxf(mn2 - pz, pz - mx2), 0.f);
                     keep = valid && ex * ex + ey * ey + ez * ez <= cull2;
-                    meta = (meta & 0xFF000000u) | ((unsigned)nx & 0xFFu) |
-                           (((unsigned)ny & 0xFFu) << 8) | (((unsigned)nz & 0xFFu) << 16);
                 }
                 const unsigned long long bal = __ballot(keep);
                 if (keep)
@@ -885,9 +899,9 @@ __global__ __launch_bounds__(kCellBlock) void rdf_cell_kernel(
                 const float xj = v ? ej.x : nanv;
                 const float yj = v ? ej.y : nanv;
                 const float zj = v ? ej.z : nanv;
-                const unsigned mj = v ? __float_as_uint(ej.w) : 0u;
+                const unsigned mj = v ? __float_as_uint(ej.w) : 0x808080u;
                 // 0: no lane has an image shift; 1: some lane has one; 2: rint wrap
-                const int mode = !use_shift ? 2 : (__any((mj & 0xFFFFFFu) != 0u) ? 1 : 0);
+                const int mode = !use_shift ? 2 : (__any((mj & 0xFFFFFFu) != 0x808080u) ? 1 : 0);
                 if (head < n_home) {
                     const int rank = head + lane < n_home ? head + lane : 0x7FFFFFFF;
                     cell_chunk_mode<PBC, LT, WEIGHTED, EDGE, TABLE, true>(mode, hp, n_home, xj, yj, zj, mj, rank, b, p, L, edge_local);
@@ -1039,7 +1053,7 @@ void build_table(wmd_rdf *h) {
             uint32_t split = 0x7F800000u;
             if (bk + 1 <= h->bins && (f_bits(h->thr[bk + 1]) >> sh) == (uint32_t)(base + k))
                 split = f_bits(h->thr[bk + 1]);
-            tab2[k] = make_uint2(split, (unsigned)bk);
+            tab2[k] = make_uint2(split, (unsigned)bk * 8u);  // byte offset of the bin
         }
         h->tab.swap(tab);
         h->tab2.swap(tab2);
