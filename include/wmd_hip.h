@@ -61,6 +61,7 @@ enum { WMD_PBC_NONE = 0, WMD_PBC_ORTHO = 1, WMD_PBC_TRICLINIC = 2 };
 typedef struct wmd_ctx wmd_ctx;
 typedef struct wmd_rdf wmd_rdf;
 typedef struct wmd_rmsd wmd_rmsd;
+typedef struct wmd_softrdf wmd_softrdf;
 
 /* ---- library / context --------------------------------------------------- */
 
@@ -176,7 +177,47 @@ wmd_status wmd_rmsd_run(wmd_rmsd *rmsd, const float *coords_xyzw, size_t n_atoms
 wmd_status wmd_rmsd_run_device(wmd_rmsd *rmsd, const float *d_store, size_t stride,
                                size_t n_frames, float *d_rmsd, double *d_rmsd64, double *d_err,
                                double *d_rot, double *d_cx, void *stream);
+/* Backward of wmd_rmsd_run_device (no counterpart in the reference: the adjoint
+ * of structure-matching losses, the differentiable companion of RmsdPlan).
+ * d_gout [F] f32 = dL/dRMSD per frame; d_rmsd64, d_rot, d_cx: the outputs of
+ * the forward run on the same store.  Writes d_grad [F][3][stride] (store
+ * layout, 0 in padding slots) = dL/dx: ((x - cx) - R^T (y - cy)) / (n RMSD)
+ * per atom for the fitted RMSD (envelope theorem; rmsd_sel must equal
+ * fit_sel, else WMD_ERR_UNSUPPORTED), (x - y) / (n RMSD) unfitted.  The
+ * reference coordinates are constants.  Asynchronous on `stream`. */
+wmd_status wmd_rmsd_backward_device(wmd_rmsd *rmsd, const float *d_store, size_t stride,
+                                    size_t n_frames, const double *d_rmsd64, const double *d_rot,
+                                    const double *d_cx, const float *d_gout, float *d_grad,
+                                    void *stream);
 void wmd_rmsd_destroy(wmd_rmsd *rmsd);
+
+/* Gaussian-smoothed RDF (no counterpart in the reference; the differentiable
+ * RDF of structure-matching losses such as warp-cg's objectives,
+ * crates/warp-cg/src/optimize/objective.rs:271-367).  Over the unordered
+ * pairs of one atom set with the RdfPlan minimum-image rule:
+ *   c_k = sum over ordered pairs i != j of w / (sigma sqrt(2 pi))
+ *         exp(-(r_k - r_ij)^2 / (2 sigma^2)),  r_k = (k + 1/2) w, w = r_max / bins,
+ * taken over the 17 bins k0 - 8 .. k0 + 8 with k0 = floor(r_ij / w).
+ * sigma must lie in [w / 2, 2 w].  pbc: WMD_PBC_*. */
+wmd_status wmd_softrdf_create(wmd_ctx *ctx, size_t bins, float r_max, float sigma, int pbc,
+                              wmd_softrdf **out);
+/* d_soa [F][3][stride] f32 device store of n_atoms atoms; boxes_host [F][9]
+ * row-major box vectors (NULL without PBC).  Adds c_k summed over the frames
+ * to d_counts [bins] (f64, device).  Asynchronous on `stream`. */
+wmd_status wmd_softrdf_forward(wmd_softrdf *h, const float *d_soa, size_t stride, size_t n_atoms,
+                               size_t n_frames, const float *boxes_host, double *d_counts,
+                               void *stream);
+/* Adds dL/dx to d_grad [F][3][stride] (f32, device) given d_dcounts [bins]
+ * (f64, device) = dL/dc_k.  The pairs are enumerated again (nothing is kept
+ * from the forward).  Asynchronous on `stream`. */
+wmd_status wmd_softrdf_backward(wmd_softrdf *h, const float *d_soa, size_t stride, size_t n_atoms,
+                                size_t n_frames, const float *boxes_host, const double *d_dcounts,
+                                float *d_grad, void *stream);
+/* Enumeration of the last call: *path 1 = brute force, 2 = cell list (nc [3]). */
+wmd_status wmd_softrdf_last_path(const wmd_softrdf *h, int *path, int *nc);
+/* cell_k: stencil reach in cells (0 = automatic, 1..3; -1 = brute force). */
+wmd_status wmd_softrdf_set_tuning(wmd_softrdf *h, int cell_k);
+void wmd_softrdf_destroy(wmd_softrdf *h);
 
 /* QCP superposition of one set of centred moments, evaluated on the host with
  * the same code the device runs (for testing without a GPU).

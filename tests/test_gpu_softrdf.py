@@ -1,0 +1,209 @@
+"""Soft RDF and RMSD adjoints on the GPU against the float64 definitions
+(oracle/softrdf.py) and FP64 central finite differences."""
+import ctypes
+import math
+
+import numpy as np
+import pytest
+
+import softrdf as S
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def _beads(n, L, frames, seed, jitter=0.2):
+    rng = np.random.default_rng(seed)
+    base = rng.random((n, 3)) * L
+    return (base[None] + rng.normal(scale=jitter, size=(frames, n, 3))).astype(np.float32)
+
+
+def _plan(**kw):
+    from warp_md_amd.autograd import SoftRdf
+    return SoftRdf(**kw)
+
+
+def _rel(a, b):
+    return float(np.abs(np.asarray(a) - np.asarray(b)).max() / max(np.abs(np.asarray(b)).max(), 1e-300))
+
+
+BINS, RMAX, SIG = 300, 12.0, 0.05
+
+
+@pytest.fixture(scope="module")
+def small():
+    # 500 beads, L = 18.42 A < 2 r_max: minimum image over every pair (brute force)
+    n = 500
+    L = (n / 0.08) ** (1.0 / 3.0)
+    x = _beads(n, L, 2, seed=4)
+    return x, L
+
+
+def test_brute_counts_match_oracle(small):
+    x, L = small
+    plan = _plan(bins=BINS, r_max=RMAX, sigma=SIG)
+    c = plan.counts(torch.tensor(x, device="cuda"), np.array([L, L, L], np.float32)).cpu().numpy()
+    assert plan.last_path[0] == 1
+    ref = S.soft_counts(x, BINS, RMAX, SIG, "orthorhombic", [np.array([L, L, L])] * len(x))
+    assert _rel(c, ref) < 1e-5
+
+
+def test_brute_gradient_matches_oracle(small):
+    x, L = small
+    plan = _plan(bins=BINS, r_max=RMAX, sigma=SIG)
+    rng = np.random.default_rng(1)
+    dldc = rng.normal(size=BINS)
+    xt = torch.tensor(x, device="cuda", requires_grad=True)
+    c = plan.counts(xt, np.array([L, L, L], np.float32))
+    c.backward(torch.tensor(dldc, device="cuda"))
+    g = xt.grad.cpu().numpy()
+    ref = S.soft_grad(x, BINS, RMAX, SIG, dldc, "orthorhombic", [np.array([L, L, L])] * len(x))
+    assert _rel(g, ref) < 1e-4
+
+
+def test_loss_gradient_matches_fp64_finite_differences(small):
+    x, L = small
+    plan = _plan(bins=BINS, r_max=RMAX, sigma=SIG)
+    box = np.array([L, L, L], np.float32)
+    xt = torch.tensor(x, device="cuda", requires_grad=True)
+    g = plan.g(xt, box)
+    t = torch.tensor(S.g_target(plan.bin_centers), device="cuda")
+    loss = ((g - t) ** 2).sum()
+    loss.backward()
+    grad = xt.grad.cpu().numpy().astype(np.float64)
+    x64 = x.astype(np.float64)
+    boxes = [np.array([L, L, L], np.float64)] * len(x)
+    vol = [L ** 3] * len(x)
+    # the GPU loss value against the float64 definition
+    l64, _ = S.rdf_loss(x64, BINS, RMAX, SIG, vol, "orthorhombic", boxes)
+    assert float(loss.detach()) == pytest.approx(l64, rel=1e-5)
+    rng = np.random.default_rng(2)
+    h = 1e-4
+    scale = np.abs(grad).max()
+    for _ in range(12):
+        f, a, d = rng.integers(len(x)), rng.integers(x.shape[1]), rng.integers(3)
+        xp = x64.copy(); xp[f, a, d] += h
+        xm = x64.copy(); xm[f, a, d] -= h
+        fd = (S.rdf_loss(xp, BINS, RMAX, SIG, vol, "orthorhombic", boxes)[0]
+              - S.rdf_loss(xm, BINS, RMAX, SIG, vol, "orthorhombic", boxes)[0]) / (2 * h)
+        assert abs(grad[f, a, d] - fd) <= 1e-4 * scale, (f, a, d, grad[f, a, d], fd)
+
+
+@pytest.mark.parametrize("n,tilt", [(6000, None), (12000, None), (12000, 10.0)])
+def test_cell_path_matches_brute(n, tilt):
+    L = (n / 0.08) ** (1.0 / 3.0)
+    x = _beads(n, L, 2, seed=n)
+    if tilt:
+        M = np.array([[L, 0, 0], [L * math.tan(math.radians(tilt)), L, 0], [0, 0, L]], np.float32)
+        x = (x / L) @ M  # fractional sites through the box
+        box, pbc = M, "triclinic"
+    else:
+        box, pbc = np.array([L, L, L], np.float32), "orthorhombic"
+    xt = torch.tensor(x, device="cuda")
+    cell = _plan(bins=BINS, r_max=RMAX, sigma=SIG, pbc=pbc)
+    brute = _plan(bins=BINS, r_max=RMAX, sigma=SIG, pbc=pbc, cell_k=-1)
+    cc = cell.counts(xt, box)
+    assert cell.last_path[0] == 2
+    cb = brute.counts(xt, box)
+    assert brute.last_path[0] == 1
+    assert _rel(cc.cpu().numpy(), cb.cpu().numpy()) < 1e-5
+    dldc = torch.tensor(np.random.default_rng(5).normal(size=BINS), device="cuda")
+    xa = xt.clone().requires_grad_(True)
+    cell.counts(xa, box).backward(dldc)
+    xb = xt.clone().requires_grad_(True)
+    brute.counts(xb, box).backward(dldc)
+    assert _rel(xa.grad.cpu().numpy(), xb.grad.cpu().numpy()) < 1e-4
+
+
+def test_counts_add_over_frame_chunks():
+    n = 6000
+    L = (n / 0.08) ** (1.0 / 3.0)
+    x = torch.tensor(_beads(n, L, 4, seed=8), device="cuda")
+    box = np.array([L, L, L], np.float32)
+    plan = _plan(bins=BINS, r_max=RMAX, sigma=SIG)
+    whole = plan.counts(x, box)
+    parts = plan.counts(x[:1], box) + plan.counts(x[1:], box)
+    torch.testing.assert_close(whole, parts, rtol=1e-12, atol=1e-9)
+
+
+def _chain(n, frames, seed):
+    rng = np.random.default_rng(seed)
+    y = np.cumsum(rng.normal(size=(n, 3)), 0) * 1.5
+    out = []
+    for _ in range(frames):
+        R = np.linalg.qr(rng.normal(size=(3, 3)))[0]
+        if np.linalg.det(R) < 0:
+            R[:, 0] *= -1
+        out.append(y @ R.T + rng.uniform(-5, 5, 3) + rng.normal(scale=0.5, size=y.shape))
+    return y.astype(np.float32), np.array(out, np.float32)
+
+
+@pytest.mark.parametrize("align", [True, False])
+def test_rmsd_backward_matches_oracle(align):
+    from warp_md_amd.autograd import FittedRmsd
+    y, x = _chain(300, 3, seed=6)
+    plan = FittedRmsd(y, align=align)
+    xt = torch.tensor(x, device="cuda", requires_grad=True)
+    r = plan(xt)
+    w = torch.tensor([0.5, -1.0, 2.0], dtype=torch.float64, device="cuda")
+    (r * w).sum().backward()
+    g = xt.grad.cpu().numpy()
+    for f in range(3):
+        x64, y64 = x[f].astype(np.float64), y.astype(np.float64)
+        if align:
+            ref_r = S.fit_rmsd_parts(x64, y64)[0]
+            ref_g = S.fit_rmsd_grad(x64, y64)
+        else:
+            diff = x64 - y64
+            ref_r = math.sqrt((diff * diff).sum() / len(x64))
+            ref_g = diff / (len(x64) * ref_r)
+        assert float(r[f]) == pytest.approx(ref_r, abs=1e-5)
+        assert _rel(g[f], float(w[f]) * ref_g) < 1e-4
+
+
+def test_rmsd_backward_fd():
+    from warp_md_amd.autograd import FittedRmsd
+    y, x = _chain(120, 1, seed=3)
+    plan = FittedRmsd(y)
+    xt = torch.tensor(x, device="cuda", requires_grad=True)
+    plan(xt).sum().backward()
+    g = xt.grad.cpu().numpy()[0]
+    x64, y64 = x[0].astype(np.float64), y.astype(np.float64)
+    h = 1e-5
+    for a in range(0, 120, 11):
+        for d in range(3):
+            xp = x64.copy(); xp[a, d] += h
+            xm = x64.copy(); xm[a, d] -= h
+            fd = (S.fit_rmsd_parts(xp, y64)[0] - S.fit_rmsd_parts(xm, y64)[0]) / (2 * h)
+            assert abs(g[a, d] - fd) <= 1e-4 * np.abs(g).max()
+
+
+def test_split_rmsd_backward_unsupported():
+    from warp_md_amd import _native as N
+    ctx = N.context(0)
+    y, x = _chain(40, 1, seed=1)
+    xyzw = np.zeros((40, 4), np.float32)
+    xyzw[:, :3] = y
+    fit = np.arange(0, 40, 2, dtype=np.uint32)
+    allat = np.arange(40, dtype=np.uint32)
+    h = ctypes.c_void_p()
+    N.check(N.lib().wmd_rmsd_create(ctx.handle, N.ptr(fit), fit.size, N.ptr(allat), allat.size,
+                                    N.ptr(xyzw), 40, 1, ctypes.byref(h)))
+    try:
+        st = torch.zeros((1, 3, 64), device="cuda")
+        one = torch.ones(1, device="cuda")
+        r64 = torch.ones(1, dtype=torch.float64, device="cuda")
+        rot = torch.zeros((1, 9), dtype=torch.float64, device="cuda")
+        status = N.lib().wmd_rmsd_backward_device(
+            h, ctypes.c_void_p(st.data_ptr()), 64, 1, ctypes.c_void_p(r64.data_ptr()),
+            ctypes.c_void_p(rot.data_ptr()), ctypes.c_void_p(rot.data_ptr()),
+            ctypes.c_void_p(one.data_ptr()), ctypes.c_void_p(st.data_ptr()), None)
+        assert status == N.WMD_ERR_UNSUPPORTED
+    finally:
+        N.lib().wmd_rmsd_destroy(h)
+
+
+def test_sigma_outside_window_rejected():
+    with pytest.raises(RuntimeError, match="unsupported"):
+        _plan(bins=300, r_max=12.0, sigma=0.2)

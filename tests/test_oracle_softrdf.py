@@ -1,0 +1,117 @@
+"""Pins of the float64 soft-RDF / RMSD-adjoint definitions (oracle/softrdf.py)."""
+import math
+
+import numpy as np
+import pytest
+
+import softrdf as S
+
+
+def _beads(n, L, frames, seed):
+    rng = np.random.default_rng(seed)
+    base = rng.random((n, 3)) * L
+    return base[None] + rng.normal(scale=0.3, size=(frames, n, 3))
+
+
+def test_single_pair_closed_form():
+    # one pair at r: 17 Gaussian samples around r, nothing else
+    bins, r_max, sigma = 100, 4.0, 0.05
+    w = r_max / bins
+    r = 1.2345
+    x = np.array([[[0.0, 0.0, 0.0], [r, 0.0, 0.0]]])
+    c = S.soft_counts(x, bins, r_max, sigma, pbc="none")
+    k0 = int(math.floor(r / w))
+    for k in range(bins):
+        rk = (k + 0.5) * w
+        want = 2 * w / (sigma * math.sqrt(2 * math.pi)) * math.exp(-(rk - r) ** 2 / (2 * sigma ** 2))
+        want = want if abs(k - k0) <= S.TAPS else 0.0
+        assert c[k] == pytest.approx(want, rel=1e-12, abs=1e-300)
+
+
+@pytest.mark.parametrize("sigma_over_w", [0.6, 1.25, 2.0])
+def test_mass_conservation(sigma_over_w):
+    # a pair away from 0 and r_max adds exactly 2 (both orders): Poisson
+    # summation of the sampled Gaussian, error 2 exp(-2 pi^2 sigma^2 / w^2)
+    bins, r_max = 300, 12.0
+    w = r_max / bins
+    sigma = sigma_over_w * w
+    rng = np.random.default_rng(7)
+    for r in rng.uniform(1.0, 11.0, size=20):
+        x = np.array([[[0.0, 0.0, 0.0], [0.0, r, 0.0]]])
+        tot = S.soft_counts(x, bins, r_max, sigma, pbc="none").sum()
+        # truncation of the 17-tap window: |d| >= 7.5 w dropped
+        trunc = math.erfc(7.5 * w / (sigma * math.sqrt(2))) * 2
+        alias = 4 * math.exp(-2 * math.pi ** 2 * sigma ** 2 / w ** 2)
+        assert abs(tot - 2.0) <= 2 * (trunc + alias) + 1e-12
+
+
+def test_min_image_matches_round_half_away():
+    L = 10.0
+    x = np.array([[0.0, 0.0, 0.0], [5.0, 0.0, 0.0], [9.9, 0.0, 0.0]])
+    _, _, d = S.pair_vectors(x, "orthorhombic", np.array([L, L, L]))
+    # 5.0 / 10 = 0.5 rounds away from zero -> -5.0
+    assert d[0, 0] == -5.0
+    assert d[1, 0] == pytest.approx(-0.1)
+    assert d[2, 0] == pytest.approx(4.9)
+
+
+@pytest.mark.parametrize("pbc", ["none", "orthorhombic", "triclinic"])
+def test_gradient_matches_finite_differences(pbc):
+    n, L, frames = 40, 7.0, 2
+    bins, r_max, sigma = 60, 3.3, 0.07
+    x = _beads(n, L, frames, seed=11)
+    if pbc == "triclinic":
+        M = np.array([[L, 0, 0], [1.2, L, 0], [0.4, -0.7, L]])
+        boxes = [M] * frames
+        vol = [abs(np.linalg.det(M))] * frames
+    else:
+        boxes = [np.array([L, L, L])] * frames
+        vol = [L ** 3] * frames
+    loss, dldc = S.rdf_loss(x, bins, r_max, sigma, vol, pbc, boxes)
+    g = S.soft_grad(x, bins, r_max, sigma, dldc, pbc, boxes)
+    rng = np.random.default_rng(3)
+    h = 1e-5
+    for _ in range(25):
+        f, a, d = rng.integers(frames), rng.integers(n), rng.integers(3)
+        xp = x.copy(); xp[f, a, d] += h
+        xm = x.copy(); xm[f, a, d] -= h
+        fd = (S.rdf_loss(xp, bins, r_max, sigma, vol, pbc, boxes)[0]
+              - S.rdf_loss(xm, bins, r_max, sigma, vol, pbc, boxes)[0]) / (2 * h)
+        scale = np.abs(g).max()
+        assert abs(fd - g[f, a, d]) <= 1e-6 * scale + 1e-6 * abs(fd), (fd, g[f, a, d])
+
+
+def test_translation_invariance_periodic():
+    n, L = 50, 8.0
+    x = _beads(n, L, 1, seed=5)
+    bins, r_max, sigma = 50, 3.5, 0.08
+    box = [np.array([L, L, L])]
+    c0 = S.soft_counts(x, bins, r_max, sigma, "orthorhombic", box)
+    c1 = S.soft_counts(x + np.array([3.1, -7.7, 0.4]), bins, r_max, sigma, "orthorhombic", box)
+    np.testing.assert_allclose(c0, c1, rtol=1e-9, atol=1e-9)
+
+
+def test_fit_rmsd_gradient_matches_finite_differences():
+    rng = np.random.default_rng(9)
+    y = rng.normal(size=(30, 3)) * 4
+    R = np.linalg.qr(rng.normal(size=(3, 3)))[0]
+    if np.linalg.det(R) < 0:
+        R[:, 0] *= -1
+    x = y @ R.T + rng.normal(scale=0.4, size=y.shape) + np.array([2.0, -1.0, 0.5])
+    g = S.fit_rmsd_grad(x, y)
+    h = 1e-6
+    for a in range(0, 30, 3):
+        for d in range(3):
+            xp = x.copy(); xp[a, d] += h
+            xm = x.copy(); xm[a, d] -= h
+            fd = (S.fit_rmsd_parts(xp, y)[0] - S.fit_rmsd_parts(xm, y)[0]) / (2 * h)
+            assert fd == pytest.approx(g[a, d], rel=1e-6, abs=1e-9)
+
+
+def test_fit_rmsd_zero_for_rigid_transform():
+    rng = np.random.default_rng(1)
+    y = rng.normal(size=(20, 3))
+    R = np.linalg.qr(rng.normal(size=(3, 3)))[0]
+    if np.linalg.det(R) < 0:
+        R[:, 0] *= -1
+    assert S.fit_rmsd_parts(y @ R.T + 3.0, y)[0] < 1e-12

@@ -58,7 +58,6 @@ using namespace wmd;
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kWaves = kBlock / 64;
 constexpr int kTile = 256;        // brute-force tile
 constexpr int kMaxTab = 8192;     // s-bits bin table entries (u16)
 constexpr uint32_t kPad = 0xFFFFFFFFu;
@@ -471,7 +470,7 @@ __global__ __launch_bounds__(kBlock) void cell_scatter_kernel(
     const float *__restrict__ soa, long long frame_stride, int stride, int n, int n_frames,
     long long frame0, const unsigned *__restrict__ cell_of, const unsigned *__restrict__ cell_slot,
     const unsigned *__restrict__ image_of, const unsigned *__restrict__ start, int ncell,
-    const unsigned *__restrict__ wts, float4 *__restrict__ sorted) {
+    const unsigned *__restrict__ wts, float4 *__restrict__ sorted, unsigned *__restrict__ order) {
     long long total = (long long)n * n_frames;
     for (long long t = (long long)blockIdx.x * kBlock + threadIdx.x; t < total;
          t += (long long)gridDim.x * kBlock) {
@@ -487,6 +486,7 @@ __global__ __launch_bounds__(kBlock) void cell_scatter_kernel(
         v.z = fx[2 * stride + i];
         v.w = __uint_as_float(meta);
         sorted[(long long)f * n + pos] = v;
+        if (order) order[(long long)f * n + pos] = (unsigned)i;  // sorted position -> store slot
     }
 }
 
@@ -668,44 +668,43 @@ __device__ __forceinline__ void cell_chunk(const float4 *__restrict__ hp, int n_
     }
 }
 
-template <int PBC, bool LT, bool WEIGHTED, bool EDGE, bool TABLE, bool HOME>
-__device__ __forceinline__ void cell_chunk_mode(int mode, const float4 *__restrict__ hp, int n_home,
-                                                float xj, float yj, float zj, unsigned mj, int rank,
-                                                const FrameBox &b, const HistParams &p,
-                                                const CellLds &L, unsigned long long &edge_local) {
-    if (mode == 0)
-        cell_chunk<PBC, LT, WEIGHTED, EDGE, TABLE, 0, HOME>(hp, n_home, xj, yj, zj, mj, rank, b, p, L, edge_local);
-    else if (mode == 1)
-        cell_chunk<PBC, LT, WEIGHTED, EDGE, TABLE, 1, HOME>(hp, n_home, xj, yj, zj, mj, rank, b, p, L, edge_local);
-    else
-        cell_chunk<PBC, LT, WEIGHTED, EDGE, TABLE, 2, HOME>(hp, n_home, xj, yj, zj, mj, rank, b, p, L, edge_local);
+// Inputs of the cell-list pair kernels: a batch of cell-sorted frames.
+struct CellGrid {
+    const float4 *sorted;       // [F][n] (x, y, z, meta) in cell order
+    const unsigned *start;      // [F][ncell + 1]
+    const unsigned *frame_flag; // [F] nonzero: the frame takes the rint path
+    const FrameBox *boxes;      // [F]
+    int n, n_frames, ncx, ncy, ncz, k, shift_ok, n_xcd;
+    float cull2;                // squared cull radius (pair reach plus rounding margin)
+};
+
+// Dispatches one chunk to the consumer's pair loop for its image mode.
+template <class C>
+__device__ __forceinline__ void cell_dispatch(C &cons, int mode, bool home, const float4 *hp,
+                                              int n_home, float xj, float yj, float zj,
+                                              unsigned mj, int rank, unsigned jpos, int f,
+                                              const FrameBox &b) {
+    if (home) {
+        if (mode == 0) cons.template chunk<0, true>(hp, n_home, xj, yj, zj, mj, rank, jpos, f, b);
+        else if (mode == 1) cons.template chunk<1, true>(hp, n_home, xj, yj, zj, mj, rank, jpos, f, b);
+        else cons.template chunk<2, true>(hp, n_home, xj, yj, zj, mj, rank, jpos, f, b);
+    } else {
+        if (mode == 0) cons.template chunk<0, false>(hp, n_home, xj, yj, zj, mj, rank, jpos, f, b);
+        else if (mode == 1) cons.template chunk<1, false>(hp, n_home, xj, yj, zj, mj, rank, jpos, f, b);
+        else cons.template chunk<2, false>(hp, n_home, xj, yj, zj, mj, rank, jpos, f, b);
+    }
 }
 
-template <int PBC, bool LT, bool WEIGHTED, bool EDGE, bool TABLE>
-__global__ __launch_bounds__(kCellBlock) void rdf_cell_kernel(
-    const float4 *__restrict__ sorted, const unsigned *__restrict__ start,
-    const unsigned *__restrict__ frame_flag, int n, int n_frames,
-    const FrameBox *__restrict__ boxes, int ncx, int ncy, int ncz, int k, int shift_ok,
-    float cull2, HistParams p, int n_xcd) {
-    extern __shared__ __attribute__((aligned(16))) char lds[];
-    const CellLds L = cell_lds_layout(lds, p, EDGE, TABLE);
-    for (int i = threadIdx.x; i < p.bins; i += kCellBlock) L.hist[i] = 0ull;
-    for (int i = threadIdx.x; i <= p.bins; i += kCellBlock) {
-        L.thr[i] = p.thr[i];
-        if constexpr (EDGE) {
-            L.elo[i] = p.elo[i];
-            L.ehi[i] = p.ehi[i];
-        }
-    }
-    if constexpr (TABLE)
-        for (int i = threadIdx.x; i < p.tab_n; i += kCellBlock) L.tab2[i] = p.tab2[i];
-    __syncthreads();
-
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform: unit indices in SGPRs
+// The unit loop of one wave.  For every unit it streams the half-stencil j
+// list through the wave's ring (POS: also each j's position in the sorted
+// frame) and hands every chunk of up to 64 j atoms to cons.chunk<MODE, HOME>;
+// cons.unit_end(f, hs, n_home) follows each non-empty unit.
+template <int PBC, bool POS, class C>
+__device__ __forceinline__ void cell_waves(const CellGrid &G, float4 *ring, unsigned *ringp,
+                                           int wave, C &cons) {
     const int lane = threadIdx.x & 63;
-    float4 *ring = L.ring + wave * kRing;
-    unsigned long long edge_local = 0;
     const float nanv = __builtin_nanf("");
+    const int ncx = G.ncx, ncy = G.ncy, ncz = G.ncz, k = G.k;
     const int ncell = ncx * ncy * ncz;
     // rows of the half stencil: k rows (0, dy > 0), k (2k + 1) rows dz > 0, the home row
     const int n_rows = 2 * k * k + 2 * k + 1;
@@ -725,11 +724,14 @@ __global__ __launch_bounds__(kCellBlock) void rdf_cell_kernel(
         }
     }
     // XCD-aware static schedule over waves: workgroup g runs on XCD g % n_xcd
-    const long long n_units = (long long)n_frames * ncell;
+    const int n_xcd = G.n_xcd;
+    const long long n_units = (long long)G.n_frames * ncell;
     const int xcd = blockIdx.x % n_xcd;
-    const long long wave_local = (long long)(blockIdx.x / n_xcd) * kCellWaves + wave;
-    const long long waves_per_xcd = (long long)(gridDim.x / n_xcd) * kCellWaves;
+    const int waves_per_wg = blockDim.x >> 6;
+    const long long wave_local = (long long)(blockIdx.x / n_xcd) * waves_per_wg + wave;
+    const long long waves_per_xcd = (long long)(gridDim.x / n_xcd) * waves_per_wg;
     const long long u_hi = n_units * (xcd + 1) / n_xcd;
+    const float cull2 = G.cull2;
 
     for (long long u = n_units * xcd / n_xcd + wave_local; u < u_hi; u += waves_per_xcd) {
         const int f = (int)(u / ncell);
@@ -737,13 +739,14 @@ __global__ __launch_bounds__(kCellBlock) void rdf_cell_kernel(
         const int hx = h % ncx;
         const int hy = (h / ncx) % ncy;
         const int hz = h / (ncx * ncy);
-        const unsigned *fs = start + (long long)f * (ncell + 1);
-        const float4 *fp = sorted + (long long)f * n;
+        const unsigned *fs = G.start + (long long)f * (ncell + 1);
+        const float4 *fp = G.sorted + (long long)f * G.n;
         const int hs = (int)fs[h];
         const int n_home = (int)fs[h + 1] - hs;
         if (n_home == 0) continue;
         const float4 *hp = fp + hs;
-        const FrameBox b = boxes[f];
+        const FrameBox b = G.boxes[f];
+        cons.unit_begin(f, hs, n_home);
 
         // home atoms: bounding box and image uniformity (the box is only used
         // on the shift path, whose frames hold finite coordinates)
@@ -761,7 +764,7 @@ __global__ __launch_bounds__(kCellBlock) void rdf_cell_kernel(
                 diff |= (__float_as_uint(q.w) & 0xFFFFFFu) != m0;
             }
         }
-        const bool use_shift = shift_ok && frame_flag[f] == 0u && !__any(diff);
+        const bool use_shift = G.shift_ok && G.frame_flag[f] == 0u && !__any(diff);
         const float mn0 = wave_min_key(kn0), mn1 = wave_min_key(kn1), mn2 = wave_min_key(kn2);
         const float mx0 = wave_max_key(kx0), mx1 = wave_max_key(kx1), mx2 = wave_max_key(kx2);
         // per-piece byte constants K: n + 128 = (m_j + 32) + K per byte, with
@@ -883,9 +886,11 @@ __global__ __launch_bounds__(kCellBlock) void rdf_cell_kernel(
                     keep = valid && ex * ex + ey * ey + ez * ez <= cull2;
                 }
                 const unsigned long long bal = __ballot(keep);
-                if (keep)
-                    ring[(tail + lanes_below(bal)) & (kRing - 1)] =
-                        make_float4(q.x, q.y, q.z, __uint_as_float(meta));
+                if (keep) {
+                    const int slot = (tail + lanes_below(bal)) & (kRing - 1);
+                    ring[slot] = make_float4(q.x, q.y, q.z, __uint_as_float(meta));
+                    if constexpr (POS) ringp[slot] = (unsigned)src;
+                }
                 tail += (int)__popcll(bal);
                 P0 += 64;
             }
@@ -894,34 +899,479 @@ __global__ __launch_bounds__(kCellBlock) void rdf_cell_kernel(
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             while (tail - head >= 64 || (!more && tail > head)) {
                 const int cnt = min(64, tail - head);
-                const float4 ej = ring[(head + lane) & (kRing - 1)];
+                const int slot = (head + lane) & (kRing - 1);
+                const float4 ej = ring[slot];
                 const bool v = lane < cnt;
                 const float xj = v ? ej.x : nanv;
                 const float yj = v ? ej.y : nanv;
                 const float zj = v ? ej.z : nanv;
                 const unsigned mj = v ? __float_as_uint(ej.w) : 0x808080u;
+                unsigned jpos = 0;
+                if constexpr (POS) jpos = v ? ringp[slot] : 0xFFFFFFFFu;
                 // 0: no lane has an image shift; 1: some lane has one; 2: rint wrap
                 const int mode = !use_shift ? 2 : (__any((mj & 0xFFFFFFu) != 0x808080u) ? 1 : 0);
-                if (head < n_home) {
-                    const int rank = head + lane < n_home ? head + lane : 0x7FFFFFFF;
-                    cell_chunk_mode<PBC, LT, WEIGHTED, EDGE, TABLE, true>(mode, hp, n_home, xj, yj, zj, mj, rank, b, p, L, edge_local);
-                } else {
-                    cell_chunk_mode<PBC, LT, WEIGHTED, EDGE, TABLE, false>(mode, hp, n_home, xj, yj, zj, mj, 0, b, p, L, edge_local);
-                }
+                const bool home = head < n_home;
+                const int rank = home && head + lane < n_home ? head + lane : 0x7FFFFFFF;
+                cell_dispatch(cons, mode, home, hp, n_home, xj, yj, zj, mj, rank, jpos, f, b);
                 head += cnt;
             }
             if (!more) break;
         }
+        cons.unit_end(f, hs, n_home);
     }
+}
+
+// Plain histogram consumer: exact bins, u64 LDS counts.
+template <int PBC, bool LT, bool WEIGHTED, bool EDGE, bool TABLE>
+struct PlainCons {
+    HistParams p;
+    CellLds L;
+    unsigned long long edge_local;
+    template <int MODE, bool HOME>
+    __device__ __forceinline__ void chunk(const float4 *hp, int n_home, float xj, float yj, float zj,
+                                          unsigned mj, int rank, unsigned, int, const FrameBox &b) {
+        cell_chunk<PBC, LT, WEIGHTED, EDGE, TABLE, MODE, HOME>(hp, n_home, xj, yj, zj, mj, rank, b, p,
+                                                                L, edge_local);
+    }
+    __device__ __forceinline__ void unit_begin(int, int, int) {}
+    __device__ __forceinline__ void unit_end(int, int, int) {}
+};
+
+template <int PBC, bool LT, bool WEIGHTED, bool EDGE, bool TABLE>
+__global__ __launch_bounds__(kCellBlock) void rdf_cell_kernel(CellGrid G, HistParams p) {
+    extern __shared__ __attribute__((aligned(16))) char lds[];
+    const CellLds L = cell_lds_layout(lds, p, EDGE, TABLE);
+    for (int i = threadIdx.x; i < p.bins; i += kCellBlock) L.hist[i] = 0ull;
+    for (int i = threadIdx.x; i <= p.bins; i += kCellBlock) {
+        L.thr[i] = p.thr[i];
+        if constexpr (EDGE) {
+            L.elo[i] = p.elo[i];
+            L.ehi[i] = p.ehi[i];
+        }
+    }
+    if constexpr (TABLE)
+        for (int i = threadIdx.x; i < p.tab_n; i += kCellBlock) L.tab2[i] = p.tab2[i];
+    __syncthreads();
+
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform: unit indices in SGPRs
+    const int lane = threadIdx.x & 63;
+    PlainCons<PBC, LT, WEIGHTED, EDGE, TABLE> cons{p, L, 0ull};
+    cell_waves<PBC, false>(G, L.ring + wave * kRing, nullptr, wave, cons);
+
     __syncthreads();
     for (int i = threadIdx.x; i < p.bins; i += kCellBlock) {
         const unsigned long long v = L.hist[i];
         if (v) atomicAdd(&p.counts[i], v);
     }
     if constexpr (EDGE) {
-        for (int off = 32; off > 0; off >>= 1) edge_local += __shfl_xor(edge_local, off, 64);
-        if (lane == 0 && edge_local) atomicAdd(p.edge, edge_local);
+        unsigned long long e = cons.edge_local;
+        for (int off = 32; off > 0; off >>= 1) e += __shfl_xor(e, off, 64);
+        if (lane == 0 && e) atomicAdd(p.edge, e);
     }
+}
+
+// -------------------------------------------------------------------------
+// Gaussian-smoothed RDF (no counterpart in the reference; the differentiable
+// companion of the histogram above, over the same unordered-pair set and the
+// same minimum-image rule).
+//
+//   c_k = sum over ordered pairs (i, j), i != j, of  w / (sigma sqrt(2 pi)) *
+//         exp(-(r_k - r_ij)^2 / (2 sigma^2)),  r_k = (k + 1/2) w,
+//
+// restricted to the 17 bins k0 - 8 .. k0 + 8 around k0 = floor(r_ij / w)
+// (the dropped terms are below exp(-(7.5 w)^2 / (2 sigma^2)) of the peak, and
+// a bin leaves the window at |r_k - r_ij| = 8.5 w).  Pairs with
+// r_ij >= (bins + 8) w reach no bin.  The backward pass gives dL/dx from
+// dL/dc through F(r) = dL/dr of one unordered pair, tabulated with its
+// derivative on nodes spaced w / 8 and read by cubic Hermite interpolation;
+// the pairs are enumerated again rather than stored.
+// -------------------------------------------------------------------------
+constexpr int kSoftTaps = 8;          // taps on each side of the centre bin
+constexpr int kSoftPadL = 8;          // LDS histogram padding below bin 0
+constexpr int kSoftPadR = 17;         // and above the last bin (centre bin <= bins + 8)
+constexpr int kSoftNodes = 8;         // Hermite nodes per bin
+constexpr int kSoftMaxHome = 64;      // home atoms with an LDS gradient slot per wave
+
+struct SoftParams {
+    int bins;
+    float w, inv_w;     // bin width
+    float cut2;         // squared pair reach ((bins + 8) w)^2
+    float nh;           // -1 / (2 sigma^2)
+    float peak;         // 2 w / (sigma sqrt(2 pi)): one unordered pair's centre weight
+    float bstep;        // exp(2 w^2 nh): ratio of successive tap ratios
+    float inv_h;        // 1 / node spacing
+    double *counts;     // [bins] (forward output, accumulated)
+    const float2 *table;  // [n_nodes] (F, h F') (backward)
+    int n_nodes;
+    float *grad;        // [F][3][stride] (backward output, accumulated)
+    const unsigned *order;  // [F][n] sorted position -> store slot (cell path)
+    long long grad_frame0;  // frame offset of this batch in grad
+    int stride;
+    int n;              // atoms per frame
+};
+
+// Adds one pair at squared distance s to the padded f64 histogram (hist[k +
+// kSoftPadL] holds bin k).  The 17 Gaussian values come from the centre value
+// by the recurrence G(t + 1) = G(t) A(t), A(t + 1) = A(t) B.
+__device__ __forceinline__ void soft_taps(float s, const SoftParams &sp, double *hist) {
+    const float r = __builtin_amdgcn_sqrtf(s);
+    // s < cut2 gives r * inv_w < bins + 8 up to rounding; the clamp keeps the
+    // top tap inside the padding
+    const int kc = min((int)(r * sp.inv_w), sp.bins + kSoftTaps);
+    const float dc = ((float)kc + 0.5f) * sp.w - r;
+    const float gc = sp.peak * __expf(dc * dc * sp.nh);
+    const float aup0 = __expf((2.0f * dc * sp.w + sp.w * sp.w) * sp.nh);
+    const float adn0 = __expf((-2.0f * dc * sp.w + sp.w * sp.w) * sp.nh);
+    double *c = hist + kSoftPadL + kc;
+    atomicAdd(c, (double)gc);
+    float g = gc, a = aup0;
+#pragma unroll
+    for (int t = 1; t <= kSoftTaps; ++t) {
+        g *= a;
+        a *= sp.bstep;
+        atomicAdd(c + t, (double)g);
+    }
+    g = gc;
+    a = adn0;
+#pragma unroll
+    for (int t = 1; t <= kSoftTaps; ++t) {
+        g *= a;
+        a *= sp.bstep;
+        atomicAdd(c - t, (double)g);
+    }
+}
+
+// F(r) = dL/dr of one unordered pair, from the node table.
+__device__ __forceinline__ float soft_force(float r, const SoftParams &sp, const float2 *tab) {
+    const float u = r * sp.inv_h;
+    const int q = (int)u;
+    const float t = u - (float)q;
+    const float2 e0 = tab[q], e1 = tab[q + 1];
+    const float t2 = t * t, t3 = t2 * t;
+    const float h00 = 2.0f * t3 - 3.0f * t2 + 1.0f;
+    const float h10 = t3 - 2.0f * t2 + t;
+    const float h01 = 3.0f * t2 - 2.0f * t3;
+    const float h11 = t3 - t2;
+    return h00 * e0.x + h10 * e0.y + h01 * e1.x + h11 * e1.y;
+}
+
+// Minimum-image displacement x_j - x_i of the cell stream's pair (MODE as in
+// cell_chunk), returned with its squared length.
+template <int PBC, bool LT, int MODE>
+__device__ __forceinline__ float soft_disp(const float4 &pi, float xj, float yj, float zj, float sx,
+                                           float sy, float sz, const FrameBox &b, float &dx,
+                                           float &dy, float &dz) {
+    dx = xj - pi.x;
+    dy = yj - pi.y;
+    dz = zj - pi.z;
+    if constexpr (PBC == WMD_PBC_ORTHO) {
+        if constexpr (MODE == 1) {
+            dx -= sx; dy -= sy; dz -= sz;
+        } else if constexpr (MODE == 2) {
+            dx -= rintf(dx * b.ilx) * b.lx;
+            dy -= rintf(dy * b.ily) * b.ly;
+            dz -= rintf(dz * b.ilz) * b.lz;
+        }
+    } else if constexpr (MODE != 0) {
+        float f0 = b.r[0] * dx + b.r[3] * dy + b.r[6] * dz;
+        float f1 = b.r[1] * dx + b.r[4] * dy + b.r[7] * dz;
+        float f2 = b.r[2] * dx + b.r[5] * dy + b.r[8] * dz;
+        if constexpr (MODE == 1) {
+            f0 -= sx; f1 -= sy; f2 -= sz;
+        } else {
+            f0 -= rintf(f0); f1 -= rintf(f1); f2 -= rintf(f2);
+        }
+        dx = f0 * b.c[0] + f1 * b.c[3] + f2 * b.c[6];
+        dy = f0 * b.c[1] + f1 * b.c[4] + f2 * b.c[7];
+        dz = f0 * b.c[2] + f1 * b.c[5] + f2 * b.c[8];
+    }
+    return dx * dx + dy * dy + dz * dz;
+}
+
+template <int MODE, int PBC>
+__device__ __forceinline__ void soft_shift(unsigned mj, const FrameBox &b, float &sx, float &sy,
+                                           float &sz) {
+    sx = sy = sz = 0.f;
+    if constexpr (MODE == 1) {
+        sx = (float)((int)(mj & 0xFFu) - 128);
+        sy = (float)((int)((mj >> 8) & 0xFFu) - 128);
+        sz = (float)((int)((mj >> 16) & 0xFFu) - 128);
+        if constexpr (PBC == WMD_PBC_ORTHO) {
+            sx *= b.lx;
+            sy *= b.ly;
+            sz *= b.lz;
+        }
+    }
+}
+
+// Forward consumer: in-range pairs are compacted into a per-wave buffer of
+// squared distances and spread onto the histogram 64 at a time.
+template <int PBC, bool LT>
+struct SoftFwdCons {
+    SoftParams sp;
+    double *hist;   // LDS, padded
+    float *pbuf;    // this wave's 128-entry pair buffer
+    int ph, pt;     // buffer head / tail (uniform)
+    __device__ __forceinline__ void drain(int cnt) {
+        const int lane = threadIdx.x & 63;
+        if (lane < cnt) soft_taps(pbuf[(ph + lane) & 127], sp, hist);
+        ph += cnt;
+    }
+    template <int MODE, bool HOME>
+    __device__ __forceinline__ void chunk(const float4 *hp, int n_home, float xj, float yj, float zj,
+                                          unsigned mj, int rank, unsigned, int, const FrameBox &b) {
+        float sx, sy, sz;
+        soft_shift<MODE, PBC>(mj, b, sx, sy, sz);
+        for (int i = 0; i < n_home; ++i) {
+            float dx, dy, dz;
+            const float s = soft_disp<PBC, LT, MODE>(hp[i], xj, yj, zj, sx, sy, sz, b, dx, dy, dz);
+            const bool in = s < sp.cut2 && (!HOME || i < rank);
+            const unsigned long long bal = __ballot(in);
+            if (in) pbuf[(pt + lanes_below(bal)) & 127] = s;
+            pt += (int)__popcll(bal);
+            if (pt - ph >= 64) {
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                drain(64);
+            }
+        }
+    }
+    __device__ __forceinline__ void unit_begin(int, int, int) {}
+    __device__ __forceinline__ void unit_end(int, int, int) {}
+};
+
+// Backward consumer: dL/dx_j accumulates in the j lane over the home atoms and
+// is added to the output once per chunk; dL/dx_i is summed over the 16 lanes
+// of each row by DPP and collected per home atom in LDS until the unit ends
+// (home atoms beyond kSoftMaxHome add their row sums to the output directly).
+template <int PBC, bool LT>
+struct SoftBwdCons {
+    SoftParams sp;
+    const float2 *tab;  // LDS node table
+    float4 *gacc;       // this wave's [kSoftMaxHome] home-atom gradients
+    const unsigned *ord;  // the current frame's sorted position -> store slot
+    int hs;               // the current unit's first home position
+    __device__ __forceinline__ static float row_sum(float v) {
+        v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0xB1, 0xF, 0xF, false));
+        v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x4E, 0xF, 0xF, false));
+        v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x141, 0xF, 0xF, false));
+        v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x140, 0xF, 0xF, false));
+        return v;
+    }
+    __device__ __forceinline__ void add_grad(int f, unsigned slot, float gx, float gy, float gz) {
+        float *g = sp.grad + (sp.grad_frame0 + f) * 3 * (long long)sp.stride + slot;
+        atomicAdd(g, gx);
+        atomicAdd(g + sp.stride, gy);
+        atomicAdd(g + 2 * sp.stride, gz);
+    }
+    __device__ __forceinline__ void unit_begin(int f, int h0, int) {
+        ord = sp.order + (long long)f * sp.n;
+        hs = h0;
+    }
+    template <int MODE, bool HOME>
+    __device__ __forceinline__ void chunk(const float4 *hp, int n_home, float xj, float yj, float zj,
+                                          unsigned mj, int rank, unsigned jpos, int f,
+                                          const FrameBox &b) {
+        float sx, sy, sz;
+        soft_shift<MODE, PBC>(mj, b, sx, sy, sz);
+        const int lane = threadIdx.x & 63;
+        float gjx = 0.f, gjy = 0.f, gjz = 0.f;
+        for (int i = 0; i < n_home; ++i) {
+            float dx, dy, dz;
+            const float s = soft_disp<PBC, LT, MODE>(hp[i], xj, yj, zj, sx, sy, sz, b, dx, dy, dz);
+            float gx = 0.f, gy = 0.f, gz = 0.f;
+            if (s < sp.cut2 && (!HOME || i < rank) && s > 0.f) {
+                const float rinv = __builtin_amdgcn_rsqf(s);
+                const float fr = soft_force(s * rinv, sp, tab) * rinv;  // F(r) / r
+                gx = fr * dx;
+                gy = fr * dy;
+                gz = fr * dz;
+            }
+            gjx += gx;
+            gjy += gy;
+            gjz += gz;
+            const float rx = row_sum(gx), ry = row_sum(gy), rz = row_sum(gz);
+            if ((lane & 15) == 0) {
+                if (i < kSoftMaxHome) {
+                    atomicAdd(&gacc[i].x, -rx);
+                    atomicAdd(&gacc[i].y, -ry);
+                    atomicAdd(&gacc[i].z, -rz);
+                } else {
+                    add_grad(f, ord[hs + i], -rx, -ry, -rz);
+                }
+            }
+        }
+        if (jpos != 0xFFFFFFFFu) add_grad(f, ord[jpos], gjx, gjy, gjz);
+    }
+    __device__ __forceinline__ void unit_end(int f, int, int n_home) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const int lane = threadIdx.x & 63;
+        if (lane < min(n_home, kSoftMaxHome)) {
+            const float4 g = gacc[lane];
+            add_grad(f, ord[hs + lane], g.x, g.y, g.z);
+            gacc[lane] = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+};
+
+constexpr int kSoftBlock = 768;  // 12 waves
+constexpr int kSoftWaves = kSoftBlock / 64;
+
+template <int PBC, bool LT>
+__global__ __launch_bounds__(kSoftBlock) void soft_cell_fwd_kernel(CellGrid G, SoftParams sp) {
+    extern __shared__ __attribute__((aligned(16))) char lds[];
+    const int hn = sp.bins + kSoftPadL + kSoftPadR;
+    double *hist = (double *)lds;
+    float *pbufs = (float *)(lds + align16((size_t)hn * 8));
+    float4 *rings = (float4 *)((char *)pbufs + kSoftWaves * 128 * 4);
+    for (int i = threadIdx.x; i < hn; i += kSoftBlock) hist[i] = 0.0;
+    __syncthreads();
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    SoftFwdCons<PBC, LT> cons{sp, hist, pbufs + wave * 128, 0, 0};
+    cell_waves<PBC, false>(G, rings + wave * kRing, nullptr, wave, cons);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (cons.pt > cons.ph) cons.drain(cons.pt - cons.ph);
+    __syncthreads();
+    for (int i = threadIdx.x; i < sp.bins; i += kSoftBlock) {
+        const double v = hist[kSoftPadL + i];
+        if (v != 0.0) atomicAdd(&sp.counts[i], v);
+    }
+}
+
+template <int PBC, bool LT>
+__global__ __launch_bounds__(kSoftBlock) void soft_cell_bwd_kernel(CellGrid G, SoftParams sp) {
+    extern __shared__ __attribute__((aligned(16))) char lds[];
+    float2 *tab = (float2 *)lds;
+    float4 *gaccs = (float4 *)(lds + align16((size_t)sp.n_nodes * 8));
+    unsigned *ringps = (unsigned *)(gaccs + kSoftWaves * kSoftMaxHome);
+    float4 *rings = (float4 *)(ringps + kSoftWaves * kRing);
+    for (int i = threadIdx.x; i < sp.n_nodes; i += kSoftBlock) tab[i] = sp.table[i];
+    for (int i = threadIdx.x; i < kSoftWaves * kSoftMaxHome; i += kSoftBlock)
+        gaccs[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    __syncthreads();
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    SoftBwdCons<PBC, LT> cons{sp, tab, gaccs + wave * kSoftMaxHome, sp.order, 0};
+    cell_waves<PBC, true>(G, rings + wave * kRing, ringps + wave * kRing, wave, cons);
+}
+
+// Brute-force soft kernels (boxes too small for the cell list): one thread per
+// atom i of a frame; the forward takes j > i, the backward every j != i and
+// writes dL/dx_i itself.  Minimum image by roundf(d / L), the reference rule.
+template <int PBC>
+__device__ __forceinline__ float soft_pair_vec(float xi, float yi, float zi, float xj, float yj,
+                                               float zj, const FrameBox &b, float &dx, float &dy,
+                                               float &dz) {
+    dx = xj - xi;
+    dy = yj - yi;
+    dz = zj - zi;
+    if constexpr (PBC == WMD_PBC_ORTHO) {
+        dx -= roundf(dx / b.lx) * b.lx;
+        dy -= roundf(dy / b.ly) * b.ly;
+        dz -= roundf(dz / b.lz) * b.lz;
+    } else if constexpr (PBC == WMD_PBC_TRICLINIC) {
+        float f0 = b.r[0] * dx + b.r[3] * dy + b.r[6] * dz;
+        float f1 = b.r[1] * dx + b.r[4] * dy + b.r[7] * dz;
+        float f2 = b.r[2] * dx + b.r[5] * dy + b.r[8] * dz;
+        f0 -= roundf(f0);
+        f1 -= roundf(f1);
+        f2 -= roundf(f2);
+        dx = f0 * b.c[0] + f1 * b.c[3] + f2 * b.c[6];
+        dy = f0 * b.c[1] + f1 * b.c[4] + f2 * b.c[7];
+        dz = f0 * b.c[2] + f1 * b.c[5] + f2 * b.c[8];
+    }
+    return dx * dx + dy * dy + dz * dz;
+}
+
+template <int PBC>
+__global__ __launch_bounds__(kBlock) void soft_brute_fwd_kernel(const float *__restrict__ soa,
+                                                                int stride, int n,
+                                                                const FrameBox *__restrict__ boxes,
+                                                                SoftParams sp) {
+    extern __shared__ __attribute__((aligned(16))) char lds[];
+    const int hn = sp.bins + kSoftPadL + kSoftPadR;
+    double *hist = (double *)lds;
+    for (int t = threadIdx.x; t < hn; t += kBlock) hist[t] = 0.0;
+    __syncthreads();
+    const int f = blockIdx.y;
+    const float *X = soa + (long long)f * 3 * stride;
+    const FrameBox b = boxes[f];
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) {
+        const float xi = X[i], yi = X[stride + i], zi = X[2 * stride + i];
+        for (int j = i + 1; j < n; ++j) {
+            float dx, dy, dz;
+            const float s = soft_pair_vec<PBC>(xi, yi, zi, X[j], X[stride + j], X[2 * stride + j], b, dx, dy, dz);
+            if (s < sp.cut2) soft_taps(s, sp, hist);
+        }
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < sp.bins; t += kBlock) {
+        const double v = hist[kSoftPadL + t];
+        if (v != 0.0) atomicAdd(&sp.counts[t], v);
+    }
+}
+
+template <int PBC>
+__global__ __launch_bounds__(kBlock) void soft_brute_bwd_kernel(const float *__restrict__ soa,
+                                                                int stride, int n,
+                                                                const FrameBox *__restrict__ boxes,
+                                                                SoftParams sp) {
+    extern __shared__ __attribute__((aligned(16))) char lds[];
+    float2 *tab = (float2 *)lds;
+    for (int t = threadIdx.x; t < sp.n_nodes; t += kBlock) tab[t] = sp.table[t];
+    __syncthreads();
+    const int f = blockIdx.y;
+    const float *X = soa + (long long)f * 3 * stride;
+    const FrameBox b = boxes[f];
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const float xi = X[i], yi = X[stride + i], zi = X[2 * stride + i];
+    float gx = 0.f, gy = 0.f, gz = 0.f;
+    for (int j = 0; j < n; ++j) {
+        if (j == i) continue;
+        float dx, dy, dz;
+        const float s = soft_pair_vec<PBC>(xi, yi, zi, X[j], X[stride + j], X[2 * stride + j], b, dx, dy, dz);
+        if (s < sp.cut2 && s > 0.f) {
+            const float rinv = __builtin_amdgcn_rsqf(s);
+            const float fr = soft_force(s * rinv, sp, tab) * rinv;
+            gx -= fr * dx;
+            gy -= fr * dy;
+            gz -= fr * dz;
+        }
+    }
+    float *g = sp.grad + (sp.grad_frame0 + f) * 3 * (long long)stride + i;
+    g[0] += gx;
+    g[stride] += gy;
+    g[2 * stride] += gz;
+}
+
+// Node table of F(r) = dL/dr for one unordered pair and h dF/dr, from dL/dc
+// (f64), over the same 17-bin windows as the forward.
+__global__ void soft_table_kernel(const double *__restrict__ dldc, int bins, double w, double sigma,
+                                  double h, int n_nodes, float2 *__restrict__ table) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n_nodes) return;
+    const double r = q * h;
+    const int k0 = (int)floor(r / w);
+    const double peak = 2.0 * w / (sigma * 2.5066282746310002);
+    const double is2 = 1.0 / (sigma * sigma);
+    double F = 0.0, dF = 0.0;
+    for (int k = k0 - kSoftTaps; k <= k0 + kSoftTaps; ++k) {
+        if (k < 0 || k >= bins) continue;
+        const double d = (k + 0.5) * w - r;
+        const double g = peak * exp(-0.5 * d * d * is2);
+        F += dldc[k] * d * is2 * g;                       // d/dr of g(r_k - r)
+        dF += dldc[k] * (-(1.0 - d * d * is2) * is2) * g;  // d2/dr2
+    }
+    table[q] = make_float2((float)F, (float)(h * dF));
 }
 
 // AoS [F][n_src][4] -> SoA [F][3][stride] through a gather list (kPad -> 0).
@@ -987,6 +1437,25 @@ template <class P> static float min_s(P pred) {
     return bits_f(hi);
 }
 
+// Scratch of the cell build.
+struct CellScratch {
+    DevBuf cell_of, cell_slot, image, count, start, flag, sorted, order;
+    size_t frames_for(int n, int ncell, size_t n_frames, bool want_order) const {
+        const size_t per_frame = (size_t)n * (16 + 12 + (want_order ? 4 : 0)) + (size_t)(2 * ncell + 2) * 4;
+        return std::max<size_t>(1, std::min<size_t>(n_frames, (size_t)(768ull << 20) / per_frame));
+    }
+    void reserve(int n, int ncell, size_t batch, bool want_order) {
+        cell_of.reserve((size_t)n * batch * 4);
+        cell_slot.reserve((size_t)n * batch * 4);
+        image.reserve((size_t)n * batch * 4);
+        count.reserve((size_t)ncell * batch * 4);
+        start.reserve((size_t)(ncell + 1) * batch * 4);
+        flag.reserve(batch * 4);
+        sorted.reserve((size_t)n * batch * 16);
+        if (want_order) order.reserve((size_t)n * batch * 4);
+    }
+};
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1010,8 +1479,8 @@ struct wmd_rdf {
     int last_path = 0;
     int last_nc[3] = {0, 0, 0};
     DevBuf d_counts, d_thr, d_elo, d_ehi, d_tab, d_tab2, d_wts, d_io, d_boxes, d_aos, d_soa;
-    DevBuf d_cell_of, d_cell_slot, d_cell_image, d_cell_count, d_cell_start, d_frame_flag,
-        d_sorted, d_tiles;
+    DevBuf d_tiles;
+    CellScratch cells;
     HostBuf h_stage, h_boxes;
 };
 
@@ -1133,8 +1602,8 @@ void launch_cells_t(wmd_rdf *h, hipStream_t st, const float4 *sorted, const unsi
     const long long max_grid = (n_units + kCellWaves * 8 - 1) / (kCellWaves * 8) * 8;
     WMD_REQUIRE(k >= 1 && 2 * (2 * k * k + 2 * k + 1) < 64, WMD_ERR_INVALID, "cell stencil reach out of range");
     if (grid > max_grid) grid = (int)std::max<long long>(8, max_grid);
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(kCellBlock), lds, st, sorted, start, frame_flag, n, nf,
-                       boxes, nc[0], nc[1], nc[2], k, shift_ok, cull2, p, 8);
+    CellGrid G{sorted, start, frame_flag, boxes, n, nf, nc[0], nc[1], nc[2], k, shift_ok, 8, cull2};
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(kCellBlock), lds, st, G, p);
     WMD_HIP_CHECK(hipGetLastError());
 }
 
@@ -1185,6 +1654,122 @@ void grid1d(long long total, int *grid) {
     *grid = (int)g;
 }
 
+// Per-frame box parameters of a batch (throws on an invalid box).
+struct BoxSet {
+    std::vector<FrameBox> fb;
+    double min_width[3] = {1e300, 1e300, 1e300};
+    bool lower_tri = true;
+};
+
+BoxSet make_boxes(int pbc, const float *boxes_host, size_t n_frames) {
+    BoxSet B;
+    B.fb.resize(n_frames);
+    for (size_t f = 0; f < n_frames; ++f) {
+        BoxInfo bi;
+        if (pbc != WMD_PBC_NONE) {
+            const float *m = boxes_host + 9 * f;
+            if (!make_box(pbc, m, &bi))
+                fail(WMD_ERR_MISMATCH, pbc == WMD_PBC_ORTHO ? "RDF requires a positive orthorhombic box"
+                                                            : "box matrix not invertible");
+            for (int d = 0; d < 3; ++d) B.min_width[d] = std::min(B.min_width[d], bi.width[d]);
+            if (m[1] != 0.0f || m[2] != 0.0f || m[5] != 0.0f) B.lower_tri = false;
+        } else {
+            std::memset(&bi, 0, sizeof(bi));
+        }
+        FrameBox &b = B.fb[f];
+        b.lx = bi.lx; b.ly = bi.ly; b.lz = bi.lz;
+        b.ilx = bi.lx > 0 ? 1.0f / bi.lx : 0.f;
+        b.ily = bi.ly > 0 ? 1.0f / bi.ly : 0.f;
+        b.ilz = bi.lz > 0 ? 1.0f / bi.lz : 0.f;
+        if (pbc == WMD_PBC_ORTHO) {
+            for (int i = 0; i < 9; ++i) b.r[i] = 0.f, b.c[i] = 0.f;
+            b.r[0] = b.ilx; b.r[4] = b.ily; b.r[8] = b.ilz;
+            b.c[0] = b.lx; b.c[4] = b.ly; b.c[8] = b.lz;
+        } else {
+            for (int i = 0; i < 9; ++i) b.r[i] = bi.r[i], b.c[i] = bi.c[i];
+        }
+    }
+    return B;
+}
+
+// Copies the batch's boxes to the device through a pinned staging buffer.
+FrameBox *upload_boxes(const BoxSet &B, HostBuf &hbuf, DevBuf &dbuf, hipStream_t st) {
+    const size_t bytes = sizeof(FrameBox) * std::max<size_t>(B.fb.size(), 1);
+    void *hb = hbuf.reserve(bytes);
+    WMD_HIP_CHECK(hipStreamSynchronize(st));  // the staging buffer may still be in use
+    std::memcpy(hb, B.fb.data(), sizeof(FrameBox) * B.fb.size());
+    FrameBox *d = (FrameBox *)dbuf.reserve(bytes);
+    WMD_HIP_CHECK(hipMemcpyAsync(d, hb, sizeof(FrameBox) * B.fb.size(), hipMemcpyHostToDevice, st));
+    return d;
+}
+
+// Cell grid for pair reach `reach`: cells at least (reach + 1e-2) / k wide per
+// plane spacing, >= 2k + 1 per side, several atoms per stencil.  Returns k
+// (0 = use brute force); forced_k in 1..3 restricts the choice.
+int choose_cells(double reach, const double *min_width, int n, int forced_k, int *nc) {
+    int ks[3] = {2, 1, 3};
+    if (forced_k >= 1 && forced_k <= 3) { ks[0] = forced_k; ks[1] = 0; ks[2] = 0; }
+    for (int kk : ks) {
+        if (kk == 0) continue;
+        const double cw = (reach + 1e-2) / kk;
+        int c[3];
+        bool ok = true;
+        for (int d = 0; d < 3; ++d) {
+            c[d] = (int)std::min(std::floor(min_width[d] / cw), 96.0);
+            if (c[d] < 2 * kk + 1) ok = false;
+        }
+        if (!ok) continue;
+        if ((double)n / ((double)c[0] * c[1] * c[2]) * std::pow(2 * kk + 1, 3) < 48.0) return 0;
+        nc[0] = c[0]; nc[1] = c[1]; nc[2] = c[2];
+        return kk;
+    }
+    return 0;
+}
+
+// Squared cull radius: reach plus a margin covering the f32 rounding of image
+// positions and pair differences for coordinates within (kImageLimit + 1) box
+// lengths (frames beyond that take the rint path, which does not cull).
+float cull2_for(double reach, const BoxSet &B) {
+    double coord_bound = 0.0;
+    for (const FrameBox &b : B.fb) {
+        double row = 0.0;
+        for (int i = 0; i < 9; ++i) row += std::fabs((double)b.c[i]);
+        coord_bound = std::max(coord_bound, row);
+    }
+    coord_bound *= (double)(kImageLimit + 1);
+    const double margin = 0.05 + 32.0 * coord_bound * std::ldexp(1.0, -24);
+    const float cull = (float)(reach + margin);
+    return cull * cull;
+}
+
+// Sorts frames [f0, f0 + nf) of an SoA store into cells (bx: their boxes).
+void build_cells(wmd_ctx *ctx, hipStream_t st, int pbc, const float *d_soa, int stride, int n,
+                 int nf, long long f0, const FrameBox *bx, const int *nc, const unsigned *d_w,
+                 CellScratch &S, bool want_order) {
+    ProfScope build_prof(ctx, st, "cell_build");
+    const int ncell = nc[0] * nc[1] * nc[2];
+    const long long fs = 3 * (long long)stride;
+    unsigned *d_count = S.count.as<unsigned>(), *d_fflag = S.flag.as<unsigned>();
+    int g;
+    grid1d((long long)ncell * nf, &g);
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(g), dim3(kBlock), 0, st, d_count, (long long)ncell * nf, 0u);
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(kBlock), 0, st, d_fflag, (long long)nf, 0u);
+    WMD_HIP_CHECK(hipGetLastError());
+    grid1d((long long)n * nf, &g);
+    if (pbc == WMD_PBC_ORTHO)
+        hipLaunchKernelGGL(cell_assign_kernel<WMD_PBC_ORTHO>, dim3(g), dim3(kBlock), 0, st, d_soa, fs, stride, n, nf, f0, bx, nc[0], nc[1], nc[2], S.cell_of.as<unsigned>(), S.cell_slot.as<unsigned>(), d_count, S.image.as<unsigned>(), d_fflag);
+    else
+        hipLaunchKernelGGL(cell_assign_kernel<WMD_PBC_TRICLINIC>, dim3(g), dim3(kBlock), 0, st, d_soa, fs, stride, n, nf, f0, bx, nc[0], nc[1], nc[2], S.cell_of.as<unsigned>(), S.cell_slot.as<unsigned>(), d_count, S.image.as<unsigned>(), d_fflag);
+    WMD_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(cell_scan_kernel, dim3(nf), dim3(kBlock), 0, st, d_count, S.start.as<unsigned>(), ncell);
+    WMD_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(cell_scatter_kernel, dim3(g), dim3(kBlock), 0, st, d_soa, fs, stride, n, nf, f0,
+                       (const unsigned *)S.cell_of.ptr, (const unsigned *)S.cell_slot.ptr,
+                       (const unsigned *)S.image.ptr, (const unsigned *)S.start.ptr, ncell, d_w,
+                       S.sorted.as<float4>(), want_order ? S.order.as<unsigned>() : (unsigned *)nullptr);
+    WMD_HIP_CHECK(hipGetLastError());
+}
+
 // Core: accumulate frames of an IO-selection SoA store already on the device.
 void accumulate_device(wmd_rdf *h, const float *d_soa, size_t stride, size_t n_frames,
                        const float *boxes_host, hipStream_t st) {
@@ -1193,74 +1778,19 @@ void accumulate_device(wmd_rdf *h, const float *d_soa, size_t stride, size_t n_f
     WMD_REQUIRE(stride >= (size_t)n && stride % 4 == 0, WMD_ERR_INVALID,
                 "stride must be a multiple of 4 and >= the IO selection size");
     WMD_REQUIRE(h->pbc == WMD_PBC_NONE || boxes_host, WMD_ERR_INVALID, "boxes required for PBC");
-    // per-frame box parameters
-    std::vector<FrameBox> fb(n_frames);
-    double min_width[3] = {1e300, 1e300, 1e300};
-    bool lower_tri = true;
-    for (size_t f = 0; f < n_frames; ++f) {
-        BoxInfo bi;
-        if (h->pbc != WMD_PBC_NONE) {
-            const float *m = boxes_host + 9 * f;
-            if (!make_box(h->pbc, m, &bi))
-                fail(WMD_ERR_MISMATCH, h->pbc == WMD_PBC_ORTHO
-                                           ? "RDF requires a positive orthorhombic box"
-                                           : "box matrix not invertible");
-            for (int d = 0; d < 3; ++d) min_width[d] = std::min(min_width[d], bi.width[d]);
-            if (m[1] != 0.0f || m[2] != 0.0f || m[5] != 0.0f) lower_tri = false;
-        } else {
-            std::memset(&bi, 0, sizeof(bi));
-        }
-        FrameBox &b = fb[f];
-        b.lx = bi.lx; b.ly = bi.ly; b.lz = bi.lz;
-        b.ilx = bi.lx > 0 ? 1.0f / bi.lx : 0.f;
-        b.ily = bi.ly > 0 ? 1.0f / bi.ly : 0.f;
-        b.ilz = bi.lz > 0 ? 1.0f / bi.lz : 0.f;
-        if (h->pbc == WMD_PBC_ORTHO) {
-            for (int i = 0; i < 9; ++i) b.r[i] = 0.f, b.c[i] = 0.f;
-            b.r[0] = b.ilx; b.r[4] = b.ily; b.r[8] = b.ilz;
-            b.c[0] = b.lx; b.c[4] = b.ly; b.c[8] = b.lz;
-        } else {
-            for (int i = 0; i < 9; ++i) b.r[i] = bi.r[i], b.c[i] = bi.c[i];
-        }
-    }
-    size_t box_bytes = sizeof(FrameBox) * n_frames;
-    void *hb = h->h_boxes.reserve(box_bytes);
-    // the pinned buffer may still be read by an earlier async copy
-    WMD_HIP_CHECK(hipStreamSynchronize(st));
-    std::memcpy(hb, fb.data(), box_bytes);
-    FrameBox *d_boxes = (FrameBox *)h->d_boxes.reserve(box_bytes);
-    WMD_HIP_CHECK(hipMemcpyAsync(d_boxes, hb, box_bytes, hipMemcpyHostToDevice, st));
+    const BoxSet B = make_boxes(h->pbc, boxes_host, n_frames);
+    const FrameBox *d_boxes = upload_boxes(B, h->h_boxes, h->d_boxes, st);
 
     const bool edge = h->flags & WMD_RDF_EDGE_DIAGNOSTICS;
     const bool table = !h->tab.empty();
     HistParams p = hist_params(h, table);
-    double mw = std::min(min_width[0], std::min(min_width[1], min_width[2]));
-    bool fast_wrap = h->pbc == WMD_PBC_NONE || (double)h->r_max <= 0.5 * mw * (1.0 - 1e-5);
+    const double mw = std::min(B.min_width[0], std::min(B.min_width[1], B.min_width[2]));
+    const bool fast_wrap = h->pbc == WMD_PBC_NONE || (double)h->r_max <= 0.5 * mw * (1.0 - 1e-5);
 
     // ---- choose the enumeration -------------------------------------------
     int k_use = 0, nc[3] = {0, 0, 0};
-    if (h->pbc != WMD_PBC_NONE && fast_wrap && !(h->flags & WMD_RDF_FORCE_BRUTE) && n >= 64) {
-        int ks[3] = {2, 1, 3};
-        if (h->cell_k >= 1 && h->cell_k <= 3) { ks[0] = h->cell_k; ks[1] = 0; ks[2] = 0; }
-        for (int kk : ks) {
-            if (kk == 0) continue;
-            double reach = ((double)h->r_max + 1e-2) / kk;
-            int c[3];
-            bool ok = true;
-            for (int d = 0; d < 3; ++d) {
-                double m = std::floor(min_width[d] / reach);
-                c[d] = (int)std::min(m, 96.0);
-                if (c[d] < 2 * kk + 1) ok = false;
-            }
-            if (!ok) continue;
-            k_use = kk;
-            nc[0] = c[0]; nc[1] = c[1]; nc[2] = c[2];
-            break;
-        }
-        // a cell list only pays off with several atoms per stencil
-        if (k_use && (double)n / ((double)nc[0] * nc[1] * nc[2]) * std::pow(2 * k_use + 1, 3) < 48.0)
-            k_use = 0;
-    }
+    if (h->pbc != WMD_PBC_NONE && fast_wrap && !(h->flags & WMD_RDF_FORCE_BRUTE) && n >= 64)
+        k_use = choose_cells((double)h->r_max, B.min_width, n, h->cell_k, nc);
 
     if (k_use == 0) {
         // ---- brute force ----------------------------------------------------
@@ -1276,7 +1806,7 @@ void accumulate_device(wmd_rdf *h, const float *d_soa, size_t stride, size_t n_f
         WMD_HIP_CHECK(hipStreamSynchronize(st));
         long long fs = 3 * (long long)stride;
         brute_dispatch(!fast_wrap, h->weighted, edge, table, h->pbc, h, st, d_soa, fs, (int)stride,
-                       n, (int)n_frames, 0LL, (const FrameBox *)d_boxes, (const int2 *)d_tp, n_tp, p);
+                       n, (int)n_frames, 0LL, d_boxes, (const int2 *)d_tp, n_tp, p);
         h->last_path = 1;
         h->last_nc[0] = h->last_nc[1] = h->last_nc[2] = 0;
         return;
@@ -1287,57 +1817,19 @@ void accumulate_device(wmd_rdf *h, const float *d_soa, size_t stride, size_t n_f
     // the image index of every stencil pair is determined by the cells when the
     // stencil spans less than half the box: (k + 1) / nc < 1/2
     const int shift_ok = (nc[0] >= 2 * k_use + 3 && nc[1] >= 2 * k_use + 3 && nc[2] >= 2 * k_use + 3) ? 1 : 0;
-    // Cull radius: r_max plus a margin covering the f32 rounding of the image
-    // positions and of the reference's own difference, for coordinates bounded
-    // by (kImageLimit + 1) box lengths (frames beyond that use the rint path).
-    double coord_bound = 0.0;
-    for (size_t f = 0; f < n_frames; ++f) {
-        double row = 0.0;
-        for (int i = 0; i < 9; ++i) row += std::fabs((double)fb[f].c[i]);
-        coord_bound = std::max(coord_bound, row);
-    }
-    coord_bound *= (double)(kImageLimit + 1);
-    const double margin = 0.05 + 32.0 * coord_bound * std::ldexp(1.0, -24);
-    const float cull = (float)((double)h->r_max + margin);
-    const float cull2 = cull * cull;
-    // frames per batch: bound the scratch (sorted float4 + 3 u32 per atom)
-    size_t per_frame = (size_t)n * (16 + 12) + (size_t)(2 * ncell + 2) * 4;
-    size_t batch = std::max<size_t>(1, std::min<size_t>(n_frames, (size_t)(768ull << 20) / per_frame));
-    unsigned *d_cell_of = (unsigned *)h->d_cell_of.reserve((size_t)n * batch * 4);
-    unsigned *d_cell_slot = (unsigned *)h->d_cell_slot.reserve((size_t)n * batch * 4);
-    unsigned *d_image = (unsigned *)h->d_cell_image.reserve((size_t)n * batch * 4);
-    unsigned *d_count = (unsigned *)h->d_cell_count.reserve((size_t)ncell * batch * 4);
-    unsigned *d_start = (unsigned *)h->d_cell_start.reserve((size_t)(ncell + 1) * batch * 4);
-    unsigned *d_fflag = (unsigned *)h->d_frame_flag.reserve(batch * 4);
-    float4 *d_sorted = (float4 *)h->d_sorted.reserve((size_t)n * batch * 16);
+    const float cull2 = cull2_for((double)h->r_max, B);
+    const size_t batch = h->cells.frames_for(n, ncell, n_frames, false);
+    h->cells.reserve(n, ncell, batch, false);
     const unsigned *d_w = h->weighted ? h->d_wts.as<unsigned>() : nullptr;
-    long long fs = 3 * (long long)stride;
-    const bool lt = h->pbc == WMD_PBC_TRICLINIC && lower_tri;
+    const bool lt = h->pbc == WMD_PBC_TRICLINIC && B.lower_tri;
     const bool cell_table = table && (int)h->tab.size() <= kMaxTab2;
     for (size_t f0 = 0; f0 < n_frames; f0 += batch) {
-        int nf = (int)std::min(batch, n_frames - f0);
-        int g;
-        const FrameBox *bx = (const FrameBox *)d_boxes + f0;
-        {
-            ProfScope build_prof(h->ctx, st, "cell_build");
-            grid1d((long long)ncell * nf, &g);
-            hipLaunchKernelGGL(fill_u32_kernel, dim3(g), dim3(kBlock), 0, st, d_count, (long long)ncell * nf, 0u);
-            hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(kBlock), 0, st, d_fflag, (long long)nf, 0u);
-            WMD_HIP_CHECK(hipGetLastError());
-            grid1d((long long)n * nf, &g);
-            if (h->pbc == WMD_PBC_ORTHO)
-                hipLaunchKernelGGL(cell_assign_kernel<WMD_PBC_ORTHO>, dim3(g), dim3(kBlock), 0, st, d_soa, fs, (int)stride, n, nf, (long long)f0, bx, nc[0], nc[1], nc[2], d_cell_of, d_cell_slot, d_count, d_image, d_fflag);
-            else
-                hipLaunchKernelGGL(cell_assign_kernel<WMD_PBC_TRICLINIC>, dim3(g), dim3(kBlock), 0, st, d_soa, fs, (int)stride, n, nf, (long long)f0, bx, nc[0], nc[1], nc[2], d_cell_of, d_cell_slot, d_count, d_image, d_fflag);
-            WMD_HIP_CHECK(hipGetLastError());
-            hipLaunchKernelGGL(cell_scan_kernel, dim3(nf), dim3(kBlock), 0, st, d_count, d_start, ncell);
-            WMD_HIP_CHECK(hipGetLastError());
-            hipLaunchKernelGGL(cell_scatter_kernel, dim3(g), dim3(kBlock), 0, st, d_soa, fs, (int)stride, n, nf, (long long)f0, d_cell_of, d_cell_slot, d_image, d_start, ncell, d_w, d_sorted);
-            WMD_HIP_CHECK(hipGetLastError());
-        }
-        cells_dispatch(h->pbc, lt, h->weighted, edge, cell_table, h, st, (const float4 *)d_sorted,
-                       (const unsigned *)d_start, (const unsigned *)d_fflag, n, nf, bx,
-                       (const int *)nc, k_use, shift_ok, cull2, p);
+        const int nf = (int)std::min(batch, n_frames - f0);
+        const FrameBox *bx = d_boxes + f0;
+        build_cells(h->ctx, st, h->pbc, d_soa, (int)stride, n, nf, (long long)f0, bx, nc, d_w, h->cells, false);
+        cells_dispatch(h->pbc, lt, h->weighted, edge, cell_table, h, st, h->cells.sorted.as<float4>(),
+                       (const unsigned *)h->cells.start.ptr, (const unsigned *)h->cells.flag.ptr, n, nf,
+                       bx, (const int *)nc, k_use, shift_ok, cull2, p);
     }
     h->last_path = 2;
     h->last_nc[0] = nc[0]; h->last_nc[1] = nc[1]; h->last_nc[2] = nc[2];
@@ -1580,6 +2072,253 @@ wmd_status wmd_rdf_set_tuning(wmd_rdf *h, int cell_k, int block_threads) {
                 "only 256-thread workgroups are built");
     h->cell_k = cell_k;
     WMD_API_END
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Gaussian-smoothed RDF handle
+// ---------------------------------------------------------------------------
+struct wmd_softrdf {
+    wmd_ctx *ctx = nullptr;
+    int bins = 0;
+    float r_max = 0.f, sigma = 0.f, w = 0.f;
+    int pbc = 0;
+    int cell_k = 0;
+    int last_path = 0;
+    int last_nc[3] = {0, 0, 0};
+    DevBuf d_boxes, d_table;
+    HostBuf h_boxes;
+    CellScratch cells;
+};
+
+namespace {
+
+SoftParams soft_params(const wmd_softrdf *h) {
+    SoftParams sp{};
+    sp.bins = h->bins;
+    sp.w = h->w;
+    sp.inv_w = 1.0f / h->w;
+    const float reach = (float)(h->bins + kSoftTaps) * h->w;
+    sp.cut2 = reach * reach;
+    const double s2 = (double)h->sigma * (double)h->sigma;
+    sp.nh = (float)(-0.5 / s2);
+    sp.peak = (float)(2.0 * (double)h->w / ((double)h->sigma * 2.5066282746310002));
+    sp.bstep = (float)std::exp(2.0 * (double)h->w * (double)h->w * (-0.5 / s2));
+    const double hn = (double)h->w / kSoftNodes;
+    sp.inv_h = (float)(1.0 / hn);
+    sp.n_nodes = (int)std::ceil((double)reach / hn) + 2;
+    return sp;
+}
+
+double soft_reach(const wmd_softrdf *h) { return (double)(h->bins + kSoftTaps) * (double)h->w; }
+
+struct SoftPlanOut {
+    int k = 0;
+    int nc[3] = {0, 0, 0};
+    bool lt = false;
+};
+
+// Cell list when the pair reach is below half the smallest plane spacing.
+SoftPlanOut soft_plan(wmd_softrdf *h, const BoxSet &B, int n) {
+    SoftPlanOut o;
+    if (h->pbc == WMD_PBC_NONE || h->cell_k < 0) return o;
+    const double mw = std::min(B.min_width[0], std::min(B.min_width[1], B.min_width[2]));
+    if (soft_reach(h) > 0.5 * mw * (1.0 - 1e-5) || n < 64) return o;
+    o.k = choose_cells(soft_reach(h), B.min_width, n, h->cell_k, o.nc);
+    o.lt = h->pbc == WMD_PBC_TRICLINIC && B.lower_tri;
+    return o;
+}
+
+template <class K>
+void launch_soft_cells(wmd_softrdf *h, hipStream_t st, K kern, size_t lds, const CellGrid &G,
+                       const SoftParams &sp, const char *name) {
+    ProfScope prof(h->ctx, st, name);
+    WMD_REQUIRE(lds <= h->ctx->lds_per_block, WMD_ERR_UNSUPPORTED, "too many bins for LDS");
+    int occ = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, kSoftBlock, lds) != hipSuccess || occ < 1)
+        occ = 1;
+    const long long n_units = (long long)G.n_frames * G.ncx * G.ncy * G.ncz;
+    int grid = std::max(8, h->ctx->cu_count * occ / 8 * 8);
+    const long long max_grid = (n_units + kSoftWaves * 8 - 1) / (kSoftWaves * 8) * 8;
+    if (grid > max_grid) grid = (int)std::max<long long>(8, max_grid);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(kSoftBlock), lds, st, G, sp);
+    WMD_HIP_CHECK(hipGetLastError());
+}
+
+void soft_run(wmd_softrdf *h, bool backward, const float *d_soa, size_t stride, size_t n_atoms,
+              size_t n_frames, const float *boxes_host, double *d_counts, const double *d_dc,
+              float *d_grad, hipStream_t st) {
+    if (n_frames == 0 || n_atoms < 2) return;
+    WMD_REQUIRE(d_soa != nullptr, WMD_ERR_INVALID, "null device store");
+    WMD_REQUIRE(stride >= n_atoms && stride % 4 == 0, WMD_ERR_INVALID,
+                "stride must be a multiple of 4 and >= the atom count");
+    WMD_REQUIRE(n_atoms < (1u << 24), WMD_ERR_UNSUPPORTED, "at most 2^24 atoms per frame");
+    WMD_REQUIRE(h->pbc == WMD_PBC_NONE || boxes_host, WMD_ERR_INVALID, "boxes required for PBC");
+    const int n = (int)n_atoms;
+    const BoxSet B = make_boxes(h->pbc, boxes_host, n_frames);
+    const FrameBox *d_boxes = upload_boxes(B, h->h_boxes, h->d_boxes, st);
+    SoftParams sp = soft_params(h);
+    sp.counts = d_counts;
+    sp.grad = d_grad;
+    sp.stride = (int)stride;
+    sp.n = n;
+    if (backward) {
+        float2 *tab = (float2 *)h->d_table.reserve((size_t)sp.n_nodes * 8);
+        const double hn = (double)h->w / kSoftNodes;
+        hipLaunchKernelGGL(soft_table_kernel, dim3((sp.n_nodes + 255) / 256), dim3(256), 0, st, d_dc,
+                           h->bins, (double)h->w, (double)h->sigma, hn, sp.n_nodes, tab);
+        WMD_HIP_CHECK(hipGetLastError());
+        sp.table = tab;
+    }
+    const SoftPlanOut pl = soft_plan(h, B, n);
+    if (pl.k == 0) {
+        // brute force: one thread per atom, frames along y
+        for (size_t f0 = 0; f0 < n_frames; f0 += 65535) {
+            const int nf = (int)std::min<size_t>(65535, n_frames - f0);
+            dim3 grid((n + kBlock - 1) / kBlock, nf);
+            const float *soa = d_soa + f0 * 3 * stride;
+            const FrameBox *bx = d_boxes + f0;
+            sp.grad_frame0 = (long long)f0;
+            ProfScope prof(h->ctx, st, backward ? "softrdf_bwd" : "softrdf_fwd");
+            if (!backward) {
+                const size_t lds = (size_t)(h->bins + kSoftPadL + kSoftPadR) * 8;
+                WMD_REQUIRE(lds <= h->ctx->lds_per_block, WMD_ERR_UNSUPPORTED, "too many bins for LDS");
+                if (h->pbc == WMD_PBC_NONE)
+                    hipLaunchKernelGGL(soft_brute_fwd_kernel<WMD_PBC_NONE>, grid, dim3(kBlock), lds, st, soa, (int)stride, n, bx, sp);
+                else if (h->pbc == WMD_PBC_ORTHO)
+                    hipLaunchKernelGGL(soft_brute_fwd_kernel<WMD_PBC_ORTHO>, grid, dim3(kBlock), lds, st, soa, (int)stride, n, bx, sp);
+                else
+                    hipLaunchKernelGGL(soft_brute_fwd_kernel<WMD_PBC_TRICLINIC>, grid, dim3(kBlock), lds, st, soa, (int)stride, n, bx, sp);
+            } else {
+                sp.grad_frame0 = 0;
+                SoftParams spb = sp;
+                spb.grad = d_grad + f0 * 3 * stride;
+                const size_t lds = (size_t)sp.n_nodes * 8;
+                WMD_REQUIRE(lds <= h->ctx->lds_per_block, WMD_ERR_UNSUPPORTED, "too many bins for LDS");
+                if (h->pbc == WMD_PBC_NONE)
+                    hipLaunchKernelGGL(soft_brute_bwd_kernel<WMD_PBC_NONE>, grid, dim3(kBlock), lds, st, soa, (int)stride, n, bx, spb);
+                else if (h->pbc == WMD_PBC_ORTHO)
+                    hipLaunchKernelGGL(soft_brute_bwd_kernel<WMD_PBC_ORTHO>, grid, dim3(kBlock), lds, st, soa, (int)stride, n, bx, spb);
+                else
+                    hipLaunchKernelGGL(soft_brute_bwd_kernel<WMD_PBC_TRICLINIC>, grid, dim3(kBlock), lds, st, soa, (int)stride, n, bx, spb);
+            }
+            WMD_HIP_CHECK(hipGetLastError());
+        }
+        h->last_path = 1;
+        h->last_nc[0] = h->last_nc[1] = h->last_nc[2] = 0;
+        return;
+    }
+    const int ncell = pl.nc[0] * pl.nc[1] * pl.nc[2];
+    const int shift_ok = (pl.nc[0] >= 2 * pl.k + 3 && pl.nc[1] >= 2 * pl.k + 3 && pl.nc[2] >= 2 * pl.k + 3) ? 1 : 0;
+    const float cull2 = cull2_for(soft_reach(h), B);
+    const size_t batch = h->cells.frames_for(n, ncell, n_frames, backward);
+    h->cells.reserve(n, ncell, batch, backward);
+    for (size_t f0 = 0; f0 < n_frames; f0 += batch) {
+        const int nf = (int)std::min(batch, n_frames - f0);
+        const FrameBox *bx = d_boxes + f0;
+        build_cells(h->ctx, st, h->pbc, d_soa, (int)stride, n, nf, (long long)f0, bx, pl.nc, nullptr,
+                    h->cells, backward);
+        CellGrid G{h->cells.sorted.as<float4>(), (const unsigned *)h->cells.start.ptr,
+                   (const unsigned *)h->cells.flag.ptr, bx, n, nf, pl.nc[0], pl.nc[1], pl.nc[2], pl.k,
+                   shift_ok, 8, cull2};
+        sp.grad_frame0 = (long long)f0;
+        sp.order = (const unsigned *)h->cells.order.ptr;
+        if (!backward) {
+            const size_t lds = align16((size_t)(h->bins + kSoftPadL + kSoftPadR) * 8) +
+                               (size_t)kSoftWaves * 128 * 4 + (size_t)kSoftWaves * kRing * 16;
+            if (h->pbc == WMD_PBC_ORTHO)
+                launch_soft_cells(h, st, soft_cell_fwd_kernel<WMD_PBC_ORTHO, false>, lds, G, sp, "softrdf_fwd");
+            else if (pl.lt)
+                launch_soft_cells(h, st, soft_cell_fwd_kernel<WMD_PBC_TRICLINIC, true>, lds, G, sp, "softrdf_fwd");
+            else
+                launch_soft_cells(h, st, soft_cell_fwd_kernel<WMD_PBC_TRICLINIC, false>, lds, G, sp, "softrdf_fwd");
+        } else {
+            const size_t lds = align16((size_t)sp.n_nodes * 8) + (size_t)kSoftWaves * kSoftMaxHome * 16 +
+                               (size_t)kSoftWaves * kRing * 4 + (size_t)kSoftWaves * kRing * 16;
+            if (h->pbc == WMD_PBC_ORTHO)
+                launch_soft_cells(h, st, soft_cell_bwd_kernel<WMD_PBC_ORTHO, false>, lds, G, sp, "softrdf_bwd");
+            else if (pl.lt)
+                launch_soft_cells(h, st, soft_cell_bwd_kernel<WMD_PBC_TRICLINIC, true>, lds, G, sp, "softrdf_bwd");
+            else
+                launch_soft_cells(h, st, soft_cell_bwd_kernel<WMD_PBC_TRICLINIC, false>, lds, G, sp, "softrdf_bwd");
+        }
+    }
+    h->last_path = 2;
+    h->last_nc[0] = pl.nc[0]; h->last_nc[1] = pl.nc[1]; h->last_nc[2] = pl.nc[2];
+}
+
+}  // namespace
+
+extern "C" {
+
+wmd_status wmd_softrdf_create(wmd_ctx *ctx, size_t bins, float r_max, float sigma, int pbc,
+                              wmd_softrdf **out) {
+    WMD_API_BEGIN
+    WMD_REQUIRE(ctx && out, WMD_ERR_INVALID, "null argument");
+    *out = nullptr;
+    WMD_REQUIRE(bins >= 1 && bins <= 8192, WMD_ERR_INVALID, "bins must be in 1..8192");
+    WMD_REQUIRE(std::isfinite(r_max) && r_max > 0.f, WMD_ERR_INVALID, "r_max must be positive");
+    WMD_REQUIRE(pbc >= WMD_PBC_NONE && pbc <= WMD_PBC_TRICLINIC, WMD_ERR_INVALID, "unknown pbc kind");
+    const float w = r_max / (float)bins;
+    WMD_REQUIRE(std::isfinite(sigma) && sigma >= 0.5f * w && sigma <= 2.0f * w, WMD_ERR_UNSUPPORTED,
+                "sigma must lie in [0.5, 2] bin widths (the 17-bin kernel window)");
+    bind(ctx);
+    wmd_softrdf *h = new wmd_softrdf;
+    h->ctx = ctx;
+    h->bins = (int)bins;
+    h->r_max = r_max;
+    h->sigma = sigma;
+    h->w = w;
+    h->pbc = pbc;
+    *out = h;
+    WMD_API_END
+}
+
+wmd_status wmd_softrdf_forward(wmd_softrdf *h, const float *d_soa, size_t stride, size_t n_atoms,
+                               size_t n_frames, const float *boxes, double *d_counts, void *stream) {
+    WMD_API_BEGIN
+    WMD_REQUIRE(h != nullptr, WMD_ERR_INVALID, "null handle");
+    WMD_REQUIRE(d_counts != nullptr || n_frames == 0, WMD_ERR_INVALID, "null counts buffer");
+    bind(h->ctx);
+    soft_run(h, false, d_soa, stride, n_atoms, n_frames, boxes, d_counts, nullptr, nullptr,
+             pick_stream(h->ctx, stream));
+    WMD_API_END
+}
+
+wmd_status wmd_softrdf_backward(wmd_softrdf *h, const float *d_soa, size_t stride, size_t n_atoms,
+                                size_t n_frames, const float *boxes, const double *d_dcounts,
+                                float *d_grad, void *stream) {
+    WMD_API_BEGIN
+    WMD_REQUIRE(h != nullptr, WMD_ERR_INVALID, "null handle");
+    WMD_REQUIRE((d_dcounts && d_grad) || n_frames == 0, WMD_ERR_INVALID, "null device buffer");
+    bind(h->ctx);
+    soft_run(h, true, d_soa, stride, n_atoms, n_frames, boxes, nullptr, d_dcounts, d_grad,
+             pick_stream(h->ctx, stream));
+    WMD_API_END
+}
+
+wmd_status wmd_softrdf_last_path(const wmd_softrdf *h, int *path, int *nc) {
+    WMD_API_BEGIN
+    WMD_REQUIRE(h && path, WMD_ERR_INVALID, "null handle");
+    *path = h->last_path;
+    if (nc) for (int d = 0; d < 3; ++d) nc[d] = h->last_nc[d];
+    WMD_API_END
+}
+
+wmd_status wmd_softrdf_set_tuning(wmd_softrdf *h, int cell_k) {
+    WMD_API_BEGIN
+    WMD_REQUIRE(h != nullptr, WMD_ERR_INVALID, "null handle");
+    WMD_REQUIRE(cell_k >= -1 && cell_k <= 3, WMD_ERR_INVALID, "cell_k must be -1..3");
+    h->cell_k = cell_k;
+    WMD_API_END
+}
+
+void wmd_softrdf_destroy(wmd_softrdf *h) {
+    if (!h) return;
+    (void)hipSetDevice(h->ctx->device);
+    (void)hipDeviceSynchronize();
+    delete h;
 }
 
 }  // extern "C"

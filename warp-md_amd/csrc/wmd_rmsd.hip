@@ -278,6 +278,51 @@ __global__ __launch_bounds__(kBlock) void rmsd_gather_kernel(const float4 *__res
     }
 }
 
+// Backward of the per-frame RMSD (no counterpart in the reference; the
+// adjoint for structure-matching losses).  For the fitted RMSD over the fit
+// set itself, the envelope theorem gives dRMSD/dx_i = ((x_i - cx) -
+// R^T (y_i - cy)) / (n RMSD) with the optimal R and centroids (the centroid
+// and rotation terms vanish at the optimum); for the unfitted RMSD,
+// (x_i - y_i) / (n RMSD).  grad[f][d][slot] = gout[f] * dRMSD_f/dx (0 for
+// padding slots and frames with RMSD = 0).
+template <bool FIT>
+__global__ __launch_bounds__(kBlock) void rmsd_backward_kernel(
+    const float *__restrict__ store, int stride, int n_frames, const float *__restrict__ ref,
+    const unsigned *__restrict__ atoms, double cy0, double cy1, double cy2, double n,
+    const double *__restrict__ rmsd64, const double *__restrict__ rot,
+    const double *__restrict__ cx, const float *__restrict__ gout, float *__restrict__ grad) {
+    const long long total = (long long)n_frames * stride;
+    for (long long t = (long long)blockIdx.x * kBlock + threadIdx.x; t < total;
+         t += (long long)gridDim.x * kBlock) {
+        const int f = (int)(t / stride);
+        const int k = (int)(t - (long long)f * stride);
+        const float *X = store + (long long)f * 3 * stride;
+        float *G = grad + (long long)f * 3 * stride;
+        const double r = rmsd64[f];
+        double g0 = 0.0, g1 = 0.0, g2 = 0.0;
+        if (atoms[k] != kPad && r > 0.0) {
+            const double sc = (double)gout[f] / (n * r);
+            const double x0 = X[k], x1 = X[stride + k], x2 = X[2 * stride + k];
+            const double y0 = ref[k], y1 = ref[stride + k], y2 = ref[2 * stride + k];
+            if constexpr (FIT) {
+                const double *R = rot + (long long)f * 9;
+                const double *c = cx + (long long)f * 3;
+                const double v0 = y0 - cy0, v1 = y1 - cy1, v2 = y2 - cy2;
+                g0 = sc * ((x0 - c[0]) - (R[0] * v0 + R[3] * v1 + R[6] * v2));
+                g1 = sc * ((x1 - c[1]) - (R[1] * v0 + R[4] * v1 + R[7] * v2));
+                g2 = sc * ((x2 - c[2]) - (R[2] * v0 + R[5] * v1 + R[8] * v2));
+            } else {
+                g0 = sc * (x0 - y0);
+                g1 = sc * (x1 - y1);
+                g2 = sc * (x2 - y2);
+            }
+        }
+        G[k] = (float)g0;
+        G[stride + k] = (float)g1;
+        G[2 * stride + k] = (float)g2;
+    }
+}
+
 }  // namespace
 
 struct wmd_rmsd {
@@ -483,6 +528,41 @@ wmd_status wmd_rmsd_run_device(wmd_rmsd *h, const float *d_store, size_t stride,
     bind(h->ctx);
     run_device(h, d_store, stride, n_frames, d_rmsd, d_rmsd64, d_err, d_rot, d_cx,
                pick_stream(h->ctx, stream));
+    WMD_API_END
+}
+
+wmd_status wmd_rmsd_backward_device(wmd_rmsd *h, const float *d_store, size_t stride,
+                                    size_t n_frames, const double *d_rmsd64, const double *d_rot,
+                                    const double *d_cx, const float *d_gout, float *d_grad,
+                                    void *stream) {
+    WMD_API_BEGIN
+    WMD_REQUIRE(h != nullptr, WMD_ERR_INVALID, "null handle");
+    if (n_frames == 0) return WMD_OK;
+    WMD_REQUIRE(h->mode != MODE_SPLIT, WMD_ERR_UNSUPPORTED,
+                "the RMSD backward needs rmsd_sel == fit_sel (a split fit needs the adjoint through the rotation)");
+    WMD_REQUIRE(stride == h->stride, WMD_ERR_INVALID,
+                "store stride must equal the plan's stride (" + std::to_string(h->stride) + ")");
+    WMD_REQUIRE(d_store && d_rmsd64 && d_gout && d_grad, WMD_ERR_INVALID, "null device buffer");
+    WMD_REQUIRE(h->mode == MODE_RAW || (d_rot && d_cx), WMD_ERR_INVALID,
+                "the fitted RMSD backward needs the rotations and centroids of the forward run");
+    bind(h->ctx);
+    hipStream_t st = pick_stream(h->ctx, stream);
+    const double n = h->mode == MODE_SAME ? h->rm.n_a : h->rm.n_b;
+    double cy[3] = {0, 0, 0};
+    if (h->rm.n_a > 0)
+        for (int d = 0; d < 3; ++d) cy[d] = h->rm.sy_a[d] / h->rm.n_a;
+    const long long total = (long long)n_frames * (long long)stride;
+    const int g = (int)std::min<long long>(65536, (total + kBlock - 1) / kBlock);
+    ProfScope prof(h->ctx, st, "rmsd_backward");
+    if (h->mode == MODE_SAME)
+        hipLaunchKernelGGL(rmsd_backward_kernel<true>, dim3(g), dim3(kBlock), 0, st, d_store, (int)stride,
+                           (int)n_frames, h->d_ref.as<float>(), h->d_atoms.as<unsigned>(), cy[0], cy[1], cy[2],
+                           n, d_rmsd64, d_rot, d_cx, d_gout, d_grad);
+    else
+        hipLaunchKernelGGL(rmsd_backward_kernel<false>, dim3(g), dim3(kBlock), 0, st, d_store, (int)stride,
+                           (int)n_frames, h->d_ref.as<float>(), h->d_atoms.as<unsigned>(), cy[0], cy[1], cy[2],
+                           n, d_rmsd64, d_rot, d_cx, d_gout, d_grad);
+    WMD_HIP_CHECK(hipGetLastError());
     WMD_API_END
 }
 

@@ -51,7 +51,9 @@ SYMBOLS = (
     "wmd_rdf_reset", "wmd_rdf_destroy", "wmd_rdf_last_path", "wmd_rdf_set_tuning",
     "wmd_rdf_finalize",
     "wmd_rmsd_create", "wmd_rmsd_store_layout", "wmd_rmsd_run", "wmd_rmsd_run_device",
-    "wmd_rmsd_destroy", "wmd_qcp_host",
+    "wmd_rmsd_backward_device", "wmd_rmsd_destroy", "wmd_qcp_host",
+    "wmd_softrdf_create", "wmd_softrdf_forward", "wmd_softrdf_backward",
+    "wmd_softrdf_last_path", "wmd_softrdf_set_tuning", "wmd_softrdf_destroy",
 )
 
 
@@ -129,7 +131,14 @@ def lib():
             "wmd_rmsd_store_layout": ([P, P, S, ctypes.POINTER(S), ctypes.POINTER(S)], St),
             "wmd_rmsd_run": ([P, P, S, S, P, P], St),
             "wmd_rmsd_run_device": ([P, P, S, S, P, P, P, P, P, P], St),
+            "wmd_rmsd_backward_device": ([P, P, S, S, P, P, P, P, P, P], St),
             "wmd_rmsd_destroy": ([P], None),
+            "wmd_softrdf_create": ([P, S, ctypes.c_float, ctypes.c_float, I, PP], St),
+            "wmd_softrdf_forward": ([P, P, S, S, S, P, P, P], St),
+            "wmd_softrdf_backward": ([P, P, S, S, S, P, P, P, P], St),
+            "wmd_softrdf_last_path": ([P, ctypes.POINTER(I), P], St),
+            "wmd_softrdf_set_tuning": ([P, I], St),
+            "wmd_softrdf_destroy": ([P], None),
             "wmd_qcp_host": ([P, ctypes.c_double, ctypes.c_double, ctypes.c_double, P, P, P,
                               ctypes.POINTER(I)], St),
         }
