@@ -93,6 +93,83 @@ def cpu_baseline(n_side=47, box=144.3, n_waters=100000):
     return out
 
 
+def soft_rdf_ratio(ctx, dev, n=20000, frames=64, reps=3):
+    """Soft-RDF forward + backward against the plain RDF forward on the same
+    input: 20,000 beads at 0.08 / A^3 (cubic 62.996 A), 64 frames of a uniform
+    base configuration plus N(0, 0.2^2) jitter, 300 bins, r_max 12 A, kernel
+    sigma 0.05 A; the loss is sum_k (g_k - g_t(r_k))^2 plus the fitted RMSD to
+    frame 0 (whose forward + backward is reported beside)."""
+    import numpy as np
+    import torch
+
+    from warp_md_amd import autograd
+    from warp_md_amd._native import check, lib, ptr
+
+    L = lib()
+    V = lambda t: ctypes.c_void_p(t.data_ptr())
+    box = (n / 0.08) ** (1.0 / 3.0)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(4)
+    base = torch.rand((n, 3), generator=gen, device=dev, dtype=torch.float32) * box
+    x = base[None] + 0.2 * torch.randn((frames, n, 3), generator=gen, device=dev, dtype=torch.float32)
+    stride = (n + 3) // 4 * 4
+    soa = torch.zeros((frames, 3, stride), dtype=torch.float32, device=dev)
+    soa[:, :, :n] = x.permute(0, 2, 1)
+    boxes = np.repeat(np.diag([box] * 3).reshape(1, 9), frames, 0).astype(np.float32)
+    sel = np.arange(n, dtype=np.uint32)
+    h = ctypes.c_void_p()
+    check(L.wmd_rdf_create(ctx.handle, ptr(sel), n, ptr(sel), n, 300, ctypes.c_float(12.0), 1, 0,
+                           ctypes.byref(h)))
+    plan = autograd.SoftRdf(bins=300, r_max=12.0, sigma=0.05, pbc="orthorhombic")
+    norm = torch.as_tensor(plan.normalisation(n, frames, np.array([box] * 3)), device=dev)
+    target = torch.as_tensor(autograd.g_target(plan.bin_centers), device=dev)
+    rmsd_plan = autograd.FittedRmsd(x[0])
+
+    def plain():
+        check(L.wmd_rdf_reset(h))
+        check(L.wmd_rdf_accumulate_device(h, V(soa), stride, frames, ptr(boxes), None))
+
+    def soft_step():
+        c = plan.forward_store(soa, stride, n, boxes)
+        g = c / norm
+        dldc = 2.0 * (g - target) / norm
+        return plan.backward_store(soa, stride, n, boxes, dldc), float(((g - target) ** 2).sum())
+
+    def rmsd_step():
+        xr = x.clone().requires_grad_(True)
+        r = rmsd_plan(xr)
+        r.sum().backward()
+        return xr.grad
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            out = fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / reps * 1e3, out
+
+    plain_ms, _ = timed(plain)
+    soft_ms, (grad, loss) = timed(soft_step)
+    rmsd_ms, _ = timed(rmsd_step)
+    path = plan.last_path
+    # kernel breakdown of one more soft step (HIP events)
+    ctx.profile(True)
+    soft_step()
+    torch.cuda.synchronize()
+    parts = {k: ctx.profile_read(k)[0] for k in ("softrdf_fwd", "softrdf_bwd", "cell_build")}
+    ctx.profile(False)
+    L.wmd_rdf_destroy(h)
+    log(f"soft rdf: plain forward {plain_ms:.2f} ms, soft fwd+bwd {soft_ms:.2f} ms "
+        f"(ratio {soft_ms / plain_ms:.2f}), fitted RMSD fwd+bwd {rmsd_ms:.2f} ms, path {path}")
+    return {"ratio": soft_ms / plain_ms, "plain_fwd_ms": plain_ms, "soft_fwd_bwd_ms": soft_ms,
+            "fitted_rmsd_fwd_bwd_ms": rmsd_ms, "loss": loss,
+            "grad_norm": float(grad.norm()), "path": path[0], "cells": path[1],
+            "kernel_ms": parts,
+            "workload": "20,000 beads, cubic 62.996 A, 64 frames, 300 bins, r_max 12 A, sigma 0.05 A"}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -102,6 +179,7 @@ def main():
     ap.add_argument("--rmsd-frames", type=int, default=5000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-triclinic", action="store_true")
+    ap.add_argument("--no-soft", action="store_true", help="skip the soft-RDF ratio")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for "
@@ -223,6 +301,7 @@ def main():
 
     L.wmd_rdf_destroy(h_rdf)
     L.wmd_rmsd_destroy(h_rmsd)
+    soft = None if args.no_soft else soft_rdf_ratio(ctx, dev)
     cpu = None if args.no_cpu else cpu_baseline()
 
     line = {
@@ -249,7 +328,8 @@ def main():
         "rdf_pair_evals_per_s": pair_evals / (rdf_ms / 1e3),
         "rmsd_frames_per_s": RF / (rmsd_ms / 1e3),
         "rdf_triclinic": tric,
-        "softrdf_fwd_bwd_over_plain": None,
+        "softrdf_fwd_bwd_over_plain": None if soft is None else soft["ratio"],
+        "softrdf": soft,
         "roofline": {
             "kernel": "rdf_cell_kernel",
             "bound": "valu",

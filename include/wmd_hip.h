@@ -194,11 +194,13 @@ void wmd_rmsd_destroy(wmd_rmsd *rmsd);
 /* Gaussian-smoothed RDF (no counterpart in the reference; the differentiable
  * RDF of structure-matching losses such as warp-cg's objectives,
  * crates/warp-cg/src/optimize/objective.rs:271-367).  Over the unordered
- * pairs of one atom set with the RdfPlan minimum-image rule:
- *   c_k = sum over ordered pairs i != j of w / (sigma sqrt(2 pi))
- *         exp(-(r_k - r_ij)^2 / (2 sigma^2)),  r_k = (k + 1/2) w, w = r_max / bins,
- * taken over the 17 bins k0 - 8 .. k0 + 8 with k0 = floor(r_ij / w).
- * sigma must lie in [w / 2, 2 w].  pbc: WMD_PBC_*. */
+ * pairs of one atom set with the RdfPlan minimum-image rule, in
+ * particle-mesh form: every ordered pair deposits the quadratic B-spline
+ * B(r_ij / h - m) on nodes m h, h = w / 4, w = r_max / bins, giving H_m; then
+ *   c_k = sum over |m h - r_k| <= 6 sigma of H_m w phi_sigma(r_k - m h),
+ * r_k = (k + 1/2) w, phi_sigma the N(0, sigma^2) density (so each pair
+ * contributes the Gaussian convolved with B; c is C1 in the positions).
+ * sigma must lie in [w / 4, 4 w].  pbc: WMD_PBC_*. */
 wmd_status wmd_softrdf_create(wmd_ctx *ctx, size_t bins, float r_max, float sigma, int pbc,
                               wmd_softrdf **out);
 /* d_soa [F][3][stride] f32 device store of n_atoms atoms; boxes_host [F][9]

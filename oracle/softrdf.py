@@ -14,18 +14,22 @@ reference's pair set and minimum-image rule:
   d -= round(d / L) * L per axis, round half away from zero
   (crates/traj-engine/src/plans/rdf.rs:924-928), or f -= round(f) on the
   fractional components of a triclinic box (crates/traj-core/src/pbc_math.rs:92-164);
-* soft count of bin k (centre r_k = (k + 1/2) w, w = r_max / bins):
-      c_k = sum over ordered pairs of w / (sigma sqrt(2 pi)) exp(-(r_k - r)^2 / (2 sigma^2)),
-  restricted to the 17 bins k0 - 8 .. k0 + 8, k0 = floor(r / w);
+* particle-mesh smoothing: nodes m h with h = w / 4 (w = r_max / bins); every
+  ordered pair deposits the quadratic B-spline B(r / h - m) onto its 3 nearest
+  nodes, H_m = sum of the deposits; bin k (centre r_k = (k + 1/2) w) collects
+      c_k = sum over |m h - r_k| <= 6 sigma of H_m w phi(r_k - m h),
+  phi the N(0, sigma^2) density.  The pair kernel is the Gaussian convolved
+  with B (variance sigma^2 + h^2 / 4); c is C1 in the positions;
 * normalisation as RdfPlan::finalize (rdf.rs:943-1016) with use_volume:
       g_k = c_k / (frames * shell_k * rho),  rho = N (N - 1) / mean volume,
   shell_k = 4/3 pi ((r_k + w/2)^3 - (r_k - w/2)^3) (evaluated here in float64);
 * loss L = sum_k (g_k - g_t(r_k))^2 with g_t(r) = 1 + 0.3 exp(-(r - 4)^2) cos(2 r).
 
 Pins (tests/test_oracle_softrdf.py): the gradient below against central
-finite differences of the loss; mass conservation (a pair far from r = 0 and
-r_max adds exactly 2 to sum_k c_k up to e^-30, by Poisson summation of the
-Gaussian sampled at spacing w <= 2 sigma); the closed form of a single pair.
+finite differences of the loss; mass conservation (the B-spline is a
+partition of unity and the Gaussian sampled at spacing w <= sigma / 4...4 sigma
+sums to 1 up to the Poisson-summation alias); the closed form of a single
+pair; the effective kernel against the Gaussian of variance sigma^2 + h^2 / 4.
 """
 from __future__ import annotations
 
@@ -33,7 +37,8 @@ import math
 
 import numpy as np
 
-TAPS = 8
+NODES_PER_BIN = 4
+REACH_SIGMAS = 6.0
 SQRT_2PI = math.sqrt(2.0 * math.pi)
 
 
@@ -66,44 +71,67 @@ def pair_vectors(x, pbc, box=None):
     return i, j, d
 
 
-def _taps(r, bins, r_max):
+def geometry(bins, r_max, sigma):
+    """(w, h, node count, bin -> node map matrix G [bins, nodes])."""
     w = r_max / bins
-    k0 = np.floor(r / w).astype(np.int64)
-    ks = k0[:, None] + np.arange(-TAPS, TAPS + 1)[None, :]
-    valid = (ks >= 0) & (ks < bins)
-    rk = (ks + 0.5) * w
-    return w, ks, valid, rk
+    h = w / NODES_PER_BIN
+    n_nodes = int(math.ceil(((bins - 0.5) * w + REACH_SIGMAS * sigma) / h)) + 2
+    rk = (np.arange(bins) + 0.5) * w
+    rm = np.arange(n_nodes) * h
+    d = rk[:, None] - rm[None, :]
+    G = w / (sigma * SQRT_2PI) * np.exp(-0.5 * d * d / (sigma * sigma))
+    G[np.abs(d) > REACH_SIGMAS * sigma] = 0.0
+    return w, h, n_nodes, G
 
 
-def soft_counts(frames, bins, r_max, sigma, pbc="orthorhombic", boxes=None):
-    """c [bins] float64 summed over the frames (F, N, 3)."""
+def bspline2(u):
+    """Nearest node n0 = rint(u), t = u - n0, weights and u-derivatives on n0-1, n0, n0+1."""
+    n0 = np.rint(u).astype(np.int64)
+    t = u - n0
+    wts = np.stack([0.5 * (0.5 - t) ** 2, 0.75 - t * t, 0.5 * (0.5 + t) ** 2], 1)
+    der = np.stack([t - 0.5, -2.0 * t, t + 0.5], 1)
+    return n0, wts, der
+
+
+def node_sums(frames, bins, r_max, sigma, pbc="orthorhombic", boxes=None):
+    """H [n_nodes] (ordered-pair B-spline deposits) summed over the frames."""
+    w, h, n_nodes, _ = geometry(bins, r_max, sigma)
     frames = np.asarray(frames, np.float64)
-    c = np.zeros(bins)
+    H = np.zeros(n_nodes + 4)
     for f in range(len(frames)):
         box = None if boxes is None else boxes[f]
         _, _, d = pair_vectors(frames[f], pbc, box)
         r = np.sqrt((d * d).sum(1))
-        w, ks, valid, rk = _taps(r, bins, r_max)
-        val = 2.0 * w / (sigma * SQRT_2PI) * np.exp(-(rk - r[:, None]) ** 2 / (2.0 * sigma * sigma))
-        np.add.at(c, ks[valid], val[valid])
-    return c
+        n0, wts, _ = bspline2(r / h)
+        ok = n0 + 1 < n_nodes + 3
+        for o in range(3):
+            np.add.at(H, n0[ok] - 1 + o + 1, 2.0 * wts[ok, o])
+    return H[1:n_nodes + 1]
+
+
+def soft_counts(frames, bins, r_max, sigma, pbc="orthorhombic", boxes=None):
+    """c [bins] float64 summed over the frames (F, N, 3)."""
+    _, _, _, G = geometry(bins, r_max, sigma)
+    return G @ node_sums(frames, bins, r_max, sigma, pbc, boxes)
 
 
 def soft_grad(frames, bins, r_max, sigma, dldc, pbc="orthorhombic", boxes=None):
     """dL/dx (F, N, 3) float64 given dL/dc [bins]."""
+    w, h, n_nodes, G = geometry(bins, r_max, sigma)
+    D = np.zeros(n_nodes + 4)
+    D[1:n_nodes + 1] = np.asarray(dldc, np.float64) @ G
     frames = np.asarray(frames, np.float64)
-    dldc = np.asarray(dldc, np.float64)
     out = np.zeros_like(frames)
     for f in range(len(frames)):
         box = None if boxes is None else boxes[f]
         i, j, d = pair_vectors(frames[f], pbc, box)
         r = np.sqrt((d * d).sum(1))
-        w, ks, valid, rk = _taps(r, bins, r_max)
-        kk = np.where(valid, ks, 0)
-        dd = rk - r[:, None]
-        g = 2.0 * w / (sigma * SQRT_2PI) * np.exp(-dd ** 2 / (2.0 * sigma * sigma))
-        # d/dr of g(r_k - r) = (r_k - r) / sigma^2 * g
-        F = (np.where(valid, dldc[kk], 0.0) * dd / (sigma * sigma) * g).sum(1)
+        n0, _, der = bspline2(r / h)
+        n0 = np.minimum(n0, n_nodes + 1)
+        F = np.zeros_like(r)
+        for o in range(3):
+            F += D[n0 - 1 + o + 1] * der[:, o]
+        F *= 2.0 / h                  # both orders, du/dr = 1/h
         fr = np.where(r > 0, F / np.where(r > 0, r, 1.0), 0.0)
         gv = fr[:, None] * d          # dL/dx_j; dL/dx_i is its negative
         np.add.at(out[f], j, gv)

@@ -14,24 +14,31 @@ def _beads(n, L, frames, seed):
 
 
 def test_single_pair_closed_form():
-    # one pair at r: 17 Gaussian samples around r, nothing else
+    # one pair at r: 2 x B-spline deposits on 3 nodes, each spread by the Gaussian
     bins, r_max, sigma = 100, 4.0, 0.05
     w = r_max / bins
+    h = w / S.NODES_PER_BIN
     r = 1.2345
     x = np.array([[[0.0, 0.0, 0.0], [r, 0.0, 0.0]]])
     c = S.soft_counts(x, bins, r_max, sigma, pbc="none")
-    k0 = int(math.floor(r / w))
+    n0 = int(round(r / h))
+    t = r / h - n0
+    B = [0.5 * (0.5 - t) ** 2, 0.75 - t * t, 0.5 * (0.5 + t) ** 2]
     for k in range(bins):
         rk = (k + 0.5) * w
-        want = 2 * w / (sigma * math.sqrt(2 * math.pi)) * math.exp(-(rk - r) ** 2 / (2 * sigma ** 2))
-        want = want if abs(k - k0) <= S.TAPS else 0.0
+        want = 0.0
+        for o in range(3):
+            d = rk - (n0 - 1 + o) * h
+            if abs(d) <= 6 * sigma:
+                want += 2 * B[o] * w / (sigma * math.sqrt(2 * math.pi)) * math.exp(-d * d / (2 * sigma ** 2))
         assert c[k] == pytest.approx(want, rel=1e-12, abs=1e-300)
 
 
 @pytest.mark.parametrize("sigma_over_w", [0.6, 1.25, 2.0])
 def test_mass_conservation(sigma_over_w):
-    # a pair away from 0 and r_max adds exactly 2 (both orders): Poisson
-    # summation of the sampled Gaussian, error 2 exp(-2 pi^2 sigma^2 / w^2)
+    # a pair away from 0 and r_max adds 2 (both orders): the B-spline is a
+    # partition of unity and the sampled Gaussian sums to 1 up to the
+    # Poisson-summation alias 2 exp(-2 pi^2 sigma^2 / w^2) and the 6 sigma cut
     bins, r_max = 300, 12.0
     w = r_max / bins
     sigma = sigma_over_w * w
@@ -39,10 +46,24 @@ def test_mass_conservation(sigma_over_w):
     for r in rng.uniform(1.0, 11.0, size=20):
         x = np.array([[[0.0, 0.0, 0.0], [0.0, r, 0.0]]])
         tot = S.soft_counts(x, bins, r_max, sigma, pbc="none").sum()
-        # truncation of the 17-tap window: |d| >= 7.5 w dropped
-        trunc = math.erfc(7.5 * w / (sigma * math.sqrt(2))) * 2
+        trunc = math.erfc(6.0 / math.sqrt(2)) * 2
         alias = 4 * math.exp(-2 * math.pi ** 2 * sigma ** 2 / w ** 2)
-        assert abs(tot - 2.0) <= 2 * (trunc + alias) + 1e-12
+        assert abs(tot - 2.0) <= 3 * (trunc + alias) + 1e-12
+
+
+def test_effective_kernel_is_gaussian():
+    # the deposit-then-spread kernel is phi_sigma convolved with the quadratic
+    # B-spline of spacing h: close to the Gaussian of variance sigma^2 + h^2 / 4
+    bins, r_max, sigma = 300, 12.0, 0.05
+    w = r_max / bins
+    h = w / S.NODES_PER_BIN
+    se = math.sqrt(sigma ** 2 + h * h / 4)
+    for r in (3.01, 5.5437, 8.123):
+        x = np.array([[[0.0, 0.0, 0.0], [r, 0.0, 0.0]]])
+        c = S.soft_counts(x, bins, r_max, sigma, pbc="none")
+        rk = (np.arange(bins) + 0.5) * w
+        g = 2 * w / (se * math.sqrt(2 * math.pi)) * np.exp(-(rk - r) ** 2 / (2 * se * se))
+        assert np.abs(c - g).max() <= 2e-3 * g.max()
 
 
 def test_min_image_matches_round_half_away():

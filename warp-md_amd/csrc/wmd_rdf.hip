@@ -973,36 +973,32 @@ __global__ __launch_bounds__(kCellBlock) void rdf_cell_kernel(CellGrid G, HistPa
 // -------------------------------------------------------------------------
 // Gaussian-smoothed RDF (no counterpart in the reference; the differentiable
 // companion of the histogram above, over the same unordered-pair set and the
-// same minimum-image rule).
+// same minimum-image rule), in particle-mesh form:
 //
-//   c_k = sum over ordered pairs (i, j), i != j, of  w / (sigma sqrt(2 pi)) *
-//         exp(-(r_k - r_ij)^2 / (2 sigma^2)),  r_k = (k + 1/2) w,
+//   H_m = sum over ordered pairs (i, j), i != j, of B(r_ij / h - m)
+//   c_k = sum over nodes m with |m h - r_k| <= 6 sigma of H_m w phi(r_k - m h),
 //
-// restricted to the 17 bins k0 - 8 .. k0 + 8 around k0 = floor(r_ij / w)
-// (the dropped terms are below exp(-(7.5 w)^2 / (2 sigma^2)) of the peak, and
-// a bin leaves the window at |r_k - r_ij| = 8.5 w).  Pairs with
-// r_ij >= (bins + 8) w reach no bin.  The backward pass gives dL/dx from
-// dL/dc through F(r) = dL/dr of one unordered pair, tabulated with its
-// derivative on nodes spaced w / 8 and read by cubic Hermite interpolation;
-// the pairs are enumerated again rather than stored.
+// where B is the quadratic B-spline (support 3 nodes, a partition of unity),
+// h = w / 4 the node spacing, r_k = (k + 1/2) w, w = r_max / bins and phi the
+// N(0, sigma^2) density.  c is a fixed linear map of H, so it is C1 in the
+// positions; the effective kernel is the Gaussian convolved with B
+// (variance sigma^2 + h^2 / 4).  Each pair deposits onto 3 nodes (f64 LDS
+// atomics); a small kernel applies the map.  The backward pass tabulates
+// D_m = sum_k dL/dc_k w phi(r_k - m h) on the nodes, so that one unordered
+// pair's dL/dr = 2 sum_m D_m B'(r / h - m) / h, and enumerates the pairs again
+// rather than storing them.
 // -------------------------------------------------------------------------
-constexpr int kSoftTaps = 8;          // taps on each side of the centre bin
-constexpr int kSoftPadL = 8;          // LDS histogram padding below bin 0
-constexpr int kSoftPadR = 17;         // and above the last bin (centre bin <= bins + 8)
-constexpr int kSoftNodes = 8;         // Hermite nodes per bin
+constexpr int kSoftNodesPerBin = 4;   // h = w / 4
+constexpr int kSoftPadL = 1;          // node padding below node 0
+constexpr int kSoftPadR = 3;          // and above the last node
 constexpr int kSoftMaxHome = 64;      // home atoms with an LDS gradient slot per wave
 
 struct SoftParams {
-    int bins;
-    float w, inv_w;     // bin width
-    float cut2;         // squared pair reach ((bins + 8) w)^2
-    float nh;           // -1 / (2 sigma^2)
-    float peak;         // 2 w / (sigma sqrt(2 pi)): one unordered pair's centre weight
-    float bstep;        // exp(2 w^2 nh): ratio of successive tap ratios
+    int n_nodes;        // nodes 0 .. n_nodes - 1 carry weight
     float inv_h;        // 1 / node spacing
-    double *counts;     // [bins] (forward output, accumulated)
-    const float2 *table;  // [n_nodes] (F, h F') (backward)
-    int n_nodes;
+    float cut2;         // squared pair reach
+    double *nodes;      // [n_nodes] H (forward output, accumulated)
+    const float *table; // [n_nodes] D (backward)
     float *grad;        // [F][3][stride] (backward output, accumulated)
     const unsigned *order;  // [F][n] sorted position -> store slot (cell path)
     long long grad_frame0;  // frame offset of this batch in grad
@@ -1010,49 +1006,37 @@ struct SoftParams {
     int n;              // atoms per frame
 };
 
-// Adds one pair at squared distance s to the padded f64 histogram (hist[k +
-// kSoftPadL] holds bin k).  The 17 Gaussian values come from the centre value
-// by the recurrence G(t + 1) = G(t) A(t), A(t + 1) = A(t) B.
-__device__ __forceinline__ void soft_taps(float s, const SoftParams &sp, double *hist) {
-    const float r = __builtin_amdgcn_sqrtf(s);
-    // s < cut2 gives r * inv_w < bins + 8 up to rounding; the clamp keeps the
-    // top tap inside the padding
-    const int kc = min((int)(r * sp.inv_w), sp.bins + kSoftTaps);
-    const float dc = ((float)kc + 0.5f) * sp.w - r;
-    const float gc = sp.peak * __expf(dc * dc * sp.nh);
-    const float aup0 = __expf((2.0f * dc * sp.w + sp.w * sp.w) * sp.nh);
-    const float adn0 = __expf((-2.0f * dc * sp.w + sp.w * sp.w) * sp.nh);
-    double *c = hist + kSoftPadL + kc;
-    atomicAdd(c, (double)gc);
-    float g = gc, a = aup0;
-#pragma unroll
-    for (int t = 1; t <= kSoftTaps; ++t) {
-        g *= a;
-        a *= sp.bstep;
-        atomicAdd(c + t, (double)g);
-    }
-    g = gc;
-    a = adn0;
-#pragma unroll
-    for (int t = 1; t <= kSoftTaps; ++t) {
-        g *= a;
-        a *= sp.bstep;
-        atomicAdd(c - t, (double)g);
-    }
+// Quadratic B-spline weights of position u on nodes n0 - 1, n0, n0 + 1
+// (n0 = rint(u), t = u - n0 in [-1/2, 1/2]) and their u-derivatives.
+__device__ __forceinline__ int bspline2(float u, int n_max, float &t) {
+    const int n0 = min((int)rintf(u), n_max);
+    t = u - (float)n0;
+    return n0;
 }
 
-// F(r) = dL/dr of one unordered pair, from the node table.
-__device__ __forceinline__ float soft_force(float r, const SoftParams &sp, const float2 *tab) {
-    const float u = r * sp.inv_h;
-    const int q = (int)u;
-    const float t = u - (float)q;
-    const float2 e0 = tab[q], e1 = tab[q + 1];
-    const float t2 = t * t, t3 = t2 * t;
-    const float h00 = 2.0f * t3 - 3.0f * t2 + 1.0f;
-    const float h10 = t3 - 2.0f * t2 + t;
-    const float h01 = 3.0f * t2 - 2.0f * t3;
-    const float h11 = t3 - t2;
-    return h00 * e0.x + h10 * e0.y + h01 * e1.x + h11 * e1.y;
+// Deposits one unordered pair (both orders) at squared distance s onto the
+// padded node array (nodes[m + kSoftPadL] holds node m).
+__device__ __forceinline__ void soft_deposit(float s, const SoftParams &sp, double *nodes) {
+    const float u = __builtin_amdgcn_sqrtf(s) * sp.inv_h;
+    float t;
+    const int n0 = bspline2(u, sp.n_nodes, t);
+    const float a = 0.5f - t, c = 0.5f + t;
+    double *p = nodes + kSoftPadL + n0;
+    atomicAdd(p - 1, (double)(a * a));              // 2 x (1/2 - t)^2 / 2
+    atomicAdd(p, (double)(1.5f - 2.0f * t * t));    // 2 x (3/4 - t^2)
+    atomicAdd(p + 1, (double)(c * c));              // 2 x (1/2 + t)^2 / 2
+}
+
+// One unordered pair's dL/dr divided by r, from the padded node table.
+__device__ __forceinline__ float soft_force_over_r(float s, const SoftParams &sp, const float *tab) {
+    const float rinv = __builtin_amdgcn_rsqf(s);
+    const float u = s * rinv * sp.inv_h;
+    float t;
+    const int n0 = bspline2(u, sp.n_nodes, t);
+    const float *q = tab + kSoftPadL + n0;
+    // 2 (D_{n0-1} (t - 1/2) - 2 t D_{n0} + D_{n0+1} (t + 1/2)) / h
+    const float du = (q[-1] * (t - 0.5f) - 2.0f * t * q[0] + q[1] * (t + 0.5f)) * 2.0f * sp.inv_h;
+    return du * rinv;
 }
 
 // Minimum-image displacement x_j - x_i of the cell stream's pair (MODE as in
@@ -1114,7 +1098,7 @@ struct SoftFwdCons {
     int ph, pt;     // buffer head / tail (uniform)
     __device__ __forceinline__ void drain(int cnt) {
         const int lane = threadIdx.x & 63;
-        if (lane < cnt) soft_taps(pbuf[(ph + lane) & 127], sp, hist);
+        if (lane < cnt) soft_deposit(pbuf[(ph + lane) & 127], sp, hist);
         ph += cnt;
     }
     template <int MODE, bool HOME>
@@ -1148,7 +1132,7 @@ struct SoftFwdCons {
 template <int PBC, bool LT>
 struct SoftBwdCons {
     SoftParams sp;
-    const float2 *tab;  // LDS node table
+    const float *tab;   // LDS node table (padded)
     float4 *gacc;       // this wave's [kSoftMaxHome] home-atom gradients
     const unsigned *ord;  // the current frame's sorted position -> store slot
     int hs;               // the current unit's first home position
@@ -1176,14 +1160,15 @@ struct SoftBwdCons {
         float sx, sy, sz;
         soft_shift<MODE, PBC>(mj, b, sx, sy, sz);
         const int lane = threadIdx.x & 63;
+        // the j slot is needed only after the loop: issue its load first
+        const unsigned jslot = jpos != 0xFFFFFFFFu ? ord[jpos] : 0u;
         float gjx = 0.f, gjy = 0.f, gjz = 0.f;
         for (int i = 0; i < n_home; ++i) {
             float dx, dy, dz;
             const float s = soft_disp<PBC, LT, MODE>(hp[i], xj, yj, zj, sx, sy, sz, b, dx, dy, dz);
             float gx = 0.f, gy = 0.f, gz = 0.f;
             if (s < sp.cut2 && (!HOME || i < rank) && s > 0.f) {
-                const float rinv = __builtin_amdgcn_rsqf(s);
-                const float fr = soft_force(s * rinv, sp, tab) * rinv;  // F(r) / r
+                const float fr = soft_force_over_r(s, sp, tab);  // (dL/dr) / r
                 gx = fr * dx;
                 gy = fr * dy;
                 gz = fr * dz;
@@ -1202,7 +1187,7 @@ struct SoftBwdCons {
                 }
             }
         }
-        if (jpos != 0xFFFFFFFFu) add_grad(f, ord[jpos], gjx, gjy, gjz);
+        if (jpos != 0xFFFFFFFFu) add_grad(f, jslot, gjx, gjy, gjz);
     }
     __device__ __forceinline__ void unit_end(int f, int, int n_home) {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -1226,7 +1211,7 @@ constexpr int kSoftWaves = kSoftBlock / 64;
 template <int PBC, bool LT>
 __global__ __launch_bounds__(kSoftBlock) void soft_cell_fwd_kernel(CellGrid G, SoftParams sp) {
     extern __shared__ __attribute__((aligned(16))) char lds[];
-    const int hn = sp.bins + kSoftPadL + kSoftPadR;
+    const int hn = sp.n_nodes + kSoftPadL + kSoftPadR;
     double *hist = (double *)lds;
     float *pbufs = (float *)(lds + align16((size_t)hn * 8));
     float4 *rings = (float4 *)((char *)pbufs + kSoftWaves * 128 * 4);
@@ -1240,20 +1225,21 @@ __global__ __launch_bounds__(kSoftBlock) void soft_cell_fwd_kernel(CellGrid G, S
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     if (cons.pt > cons.ph) cons.drain(cons.pt - cons.ph);
     __syncthreads();
-    for (int i = threadIdx.x; i < sp.bins; i += kSoftBlock) {
+    for (int i = threadIdx.x; i < sp.n_nodes; i += kSoftBlock) {
         const double v = hist[kSoftPadL + i];
-        if (v != 0.0) atomicAdd(&sp.counts[i], v);
+        if (v != 0.0) atomicAdd(&sp.nodes[i], v);
     }
 }
 
 template <int PBC, bool LT>
 __global__ __launch_bounds__(kSoftBlock) void soft_cell_bwd_kernel(CellGrid G, SoftParams sp) {
     extern __shared__ __attribute__((aligned(16))) char lds[];
-    float2 *tab = (float2 *)lds;
-    float4 *gaccs = (float4 *)(lds + align16((size_t)sp.n_nodes * 8));
+    const int tn = sp.n_nodes + kSoftPadL + kSoftPadR;
+    float *tab = (float *)lds;
+    float4 *gaccs = (float4 *)(lds + align16((size_t)tn * 4));
     unsigned *ringps = (unsigned *)(gaccs + kSoftWaves * kSoftMaxHome);
     float4 *rings = (float4 *)(ringps + kSoftWaves * kRing);
-    for (int i = threadIdx.x; i < sp.n_nodes; i += kSoftBlock) tab[i] = sp.table[i];
+    for (int i = threadIdx.x; i < tn; i += kSoftBlock) tab[i] = sp.table[i];
     for (int i = threadIdx.x; i < kSoftWaves * kSoftMaxHome; i += kSoftBlock)
         gaccs[i] = make_float4(0.f, 0.f, 0.f, 0.f);
     __syncthreads();
@@ -1296,7 +1282,7 @@ __global__ __launch_bounds__(kBlock) void soft_brute_fwd_kernel(const float *__r
                                                                 const FrameBox *__restrict__ boxes,
                                                                 SoftParams sp) {
     extern __shared__ __attribute__((aligned(16))) char lds[];
-    const int hn = sp.bins + kSoftPadL + kSoftPadR;
+    const int hn = sp.n_nodes + kSoftPadL + kSoftPadR;
     double *hist = (double *)lds;
     for (int t = threadIdx.x; t < hn; t += kBlock) hist[t] = 0.0;
     __syncthreads();
@@ -1309,13 +1295,13 @@ __global__ __launch_bounds__(kBlock) void soft_brute_fwd_kernel(const float *__r
         for (int j = i + 1; j < n; ++j) {
             float dx, dy, dz;
             const float s = soft_pair_vec<PBC>(xi, yi, zi, X[j], X[stride + j], X[2 * stride + j], b, dx, dy, dz);
-            if (s < sp.cut2) soft_taps(s, sp, hist);
+            if (s < sp.cut2) soft_deposit(s, sp, hist);
         }
     }
     __syncthreads();
-    for (int t = threadIdx.x; t < sp.bins; t += kBlock) {
+    for (int t = threadIdx.x; t < sp.n_nodes; t += kBlock) {
         const double v = hist[kSoftPadL + t];
-        if (v != 0.0) atomicAdd(&sp.counts[t], v);
+        if (v != 0.0) atomicAdd(&sp.nodes[t], v);
     }
 }
 
@@ -1325,8 +1311,9 @@ __global__ __launch_bounds__(kBlock) void soft_brute_bwd_kernel(const float *__r
                                                                 const FrameBox *__restrict__ boxes,
                                                                 SoftParams sp) {
     extern __shared__ __attribute__((aligned(16))) char lds[];
-    float2 *tab = (float2 *)lds;
-    for (int t = threadIdx.x; t < sp.n_nodes; t += kBlock) tab[t] = sp.table[t];
+    const int tn = sp.n_nodes + kSoftPadL + kSoftPadR;
+    float *tab = (float *)lds;
+    for (int t = threadIdx.x; t < tn; t += kBlock) tab[t] = sp.table[t];
     __syncthreads();
     const int f = blockIdx.y;
     const float *X = soa + (long long)f * 3 * stride;
@@ -1340,8 +1327,7 @@ __global__ __launch_bounds__(kBlock) void soft_brute_bwd_kernel(const float *__r
         float dx, dy, dz;
         const float s = soft_pair_vec<PBC>(xi, yi, zi, X[j], X[stride + j], X[2 * stride + j], b, dx, dy, dz);
         if (s < sp.cut2 && s > 0.f) {
-            const float rinv = __builtin_amdgcn_rsqf(s);
-            const float fr = soft_force(s * rinv, sp, tab) * rinv;
+            const float fr = soft_force_over_r(s, sp, tab);
             gx -= fr * dx;
             gy -= fr * dy;
             gz -= fr * dz;
@@ -1353,25 +1339,44 @@ __global__ __launch_bounds__(kBlock) void soft_brute_bwd_kernel(const float *__r
     g[2 * stride] += gz;
 }
 
-// Node table of F(r) = dL/dr for one unordered pair and h dF/dr, from dL/dc
-// (f64), over the same 17-bin windows as the forward.
-__global__ void soft_table_kernel(const double *__restrict__ dldc, int bins, double w, double sigma,
-                                  double h, int n_nodes, float2 *__restrict__ table) {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= n_nodes) return;
-    const double r = q * h;
-    const int k0 = (int)floor(r / w);
-    const double peak = 2.0 * w / (sigma * 2.5066282746310002);
-    const double is2 = 1.0 / (sigma * sigma);
-    double F = 0.0, dF = 0.0;
-    for (int k = k0 - kSoftTaps; k <= k0 + kSoftTaps; ++k) {
-        if (k < 0 || k >= bins) continue;
-        const double d = (k + 0.5) * w - r;
-        const double g = peak * exp(-0.5 * d * d * is2);
-        F += dldc[k] * d * is2 * g;                       // d/dr of g(r_k - r)
-        dF += dldc[k] * (-(1.0 - d * d * is2) * is2) * g;  // d2/dr2
+// c_k += sum over |m h - r_k| <= 6 sigma of H_m w phi(r_k - m h), one thread per bin.
+__global__ void soft_conv_kernel(const double *__restrict__ nodes, int n_nodes, int bins, double w,
+                                 double h, double sigma, double *__restrict__ counts) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= bins) return;
+    const double rk = (k + 0.5) * w;
+    const double reach = 6.0 * sigma;
+    const int m0 = max(0, (int)ceil((rk - reach) / h));
+    const int m1 = min(n_nodes - 1, (int)floor((rk + reach) / h));
+    const double norm = w / (sigma * 2.5066282746310002);
+    double c = 0.0;
+    for (int m = m0; m <= m1; ++m) {
+        const double d = rk - m * h;
+        c += nodes[m] * norm * exp(-0.5 * d * d / (sigma * sigma));
     }
-    table[q] = make_float2((float)F, (float)(h * dF));
+    counts[k] += c;
+}
+
+// D_m = sum over |m h - r_k| <= 6 sigma of dL/dc_k w phi(r_k - m h) into the
+// padded table (zero in the padding), one thread per table entry.
+__global__ void soft_node_table_kernel(const double *__restrict__ dldc, int bins, double w, double h,
+                                       double sigma, int n_nodes, float *__restrict__ table) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n_nodes + kSoftPadL + kSoftPadR) return;
+    const int m = e - kSoftPadL;
+    double D = 0.0;
+    if (m >= 0 && m < n_nodes) {
+        const double rm = m * h;
+        const double reach = 6.0 * sigma;
+        const int k0 = max(0, (int)ceil((rm - reach) / w - 0.5));
+        const int k1 = min(bins - 1, (int)floor((rm + reach) / w - 0.5));
+        const double norm = w / (sigma * 2.5066282746310002);
+        for (int k = k0; k <= k1; ++k) {
+            const double d = (k + 0.5) * w - rm;
+            if (fabs(d) <= reach) D += dldc[k] * norm * exp(-0.5 * d * d / (sigma * sigma));
+        }
+    }
+    table[e] = (float)D;
 }
 
 // AoS [F][n_src][4] -> SoA [F][3][stride] through a gather list (kPad -> 0).
@@ -2087,31 +2092,30 @@ struct wmd_softrdf {
     int cell_k = 0;
     int last_path = 0;
     int last_nc[3] = {0, 0, 0};
-    DevBuf d_boxes, d_table;
+    DevBuf d_boxes, d_table, d_nodes;
     HostBuf h_boxes;
     CellScratch cells;
 };
 
 namespace {
 
-SoftParams soft_params(const wmd_softrdf *h) {
-    SoftParams sp{};
-    sp.bins = h->bins;
-    sp.w = h->w;
-    sp.inv_w = 1.0f / h->w;
-    const float reach = (float)(h->bins + kSoftTaps) * h->w;
-    sp.cut2 = reach * reach;
-    const double s2 = (double)h->sigma * (double)h->sigma;
-    sp.nh = (float)(-0.5 / s2);
-    sp.peak = (float)(2.0 * (double)h->w / ((double)h->sigma * 2.5066282746310002));
-    sp.bstep = (float)std::exp(2.0 * (double)h->w * (double)h->w * (-0.5 / s2));
-    const double hn = (double)h->w / kSoftNodes;
-    sp.inv_h = (float)(1.0 / hn);
-    sp.n_nodes = (int)std::ceil((double)reach / hn) + 2;
-    return sp;
+double soft_h(const wmd_softrdf *h) { return (double)h->w / kSoftNodesPerBin; }
+
+// Pair reach: a pair deposits onto nodes within 1.5 h of r; the last bin uses
+// nodes up to r_last + 6 sigma.
+double soft_reach(const wmd_softrdf *h) {
+    return ((double)h->bins - 0.5) * (double)h->w + 6.0 * (double)h->sigma + 2.0 * soft_h(h);
 }
 
-double soft_reach(const wmd_softrdf *h) { return (double)(h->bins + kSoftTaps) * (double)h->w; }
+SoftParams soft_params(const wmd_softrdf *h) {
+    SoftParams sp{};
+    const double hn = soft_h(h);
+    sp.n_nodes = (int)std::ceil((((double)h->bins - 0.5) * (double)h->w + 6.0 * (double)h->sigma) / hn) + 2;
+    sp.inv_h = (float)(1.0 / hn);
+    const float reach = (float)soft_reach(h);
+    sp.cut2 = reach * reach;
+    return sp;
+}
 
 struct SoftPlanOut {
     int k = 0;
@@ -2159,18 +2163,30 @@ void soft_run(wmd_softrdf *h, bool backward, const float *d_soa, size_t stride, 
     const BoxSet B = make_boxes(h->pbc, boxes_host, n_frames);
     const FrameBox *d_boxes = upload_boxes(B, h->h_boxes, h->d_boxes, st);
     SoftParams sp = soft_params(h);
-    sp.counts = d_counts;
     sp.grad = d_grad;
     sp.stride = (int)stride;
     sp.n = n;
+    const int tn = sp.n_nodes + kSoftPadL + kSoftPadR;
     if (backward) {
-        float2 *tab = (float2 *)h->d_table.reserve((size_t)sp.n_nodes * 8);
-        const double hn = (double)h->w / kSoftNodes;
-        hipLaunchKernelGGL(soft_table_kernel, dim3((sp.n_nodes + 255) / 256), dim3(256), 0, st, d_dc,
-                           h->bins, (double)h->w, (double)h->sigma, hn, sp.n_nodes, tab);
+        float *tab = (float *)h->d_table.reserve((size_t)tn * 4);
+        hipLaunchKernelGGL(soft_node_table_kernel, dim3((tn + 255) / 256), dim3(256), 0, st, d_dc,
+                           h->bins, (double)h->w, soft_h(h), (double)h->sigma, sp.n_nodes, tab);
         WMD_HIP_CHECK(hipGetLastError());
         sp.table = tab;
+    } else {
+        sp.nodes = (double *)h->d_nodes.reserve((size_t)sp.n_nodes * 8);
+        hipLaunchKernelGGL(fill_u64_kernel, dim3((sp.n_nodes + kBlock - 1) / kBlock), dim3(kBlock), 0, st,
+                           (unsigned long long *)sp.nodes, (long long)sp.n_nodes);
+        WMD_HIP_CHECK(hipGetLastError());
     }
+    // node sums -> bins (forward), after the pair kernels
+    auto conv = [&]() {
+        if (backward) return;
+        hipLaunchKernelGGL(soft_conv_kernel, dim3((h->bins + 255) / 256), dim3(256), 0, st,
+                           (const double *)sp.nodes, sp.n_nodes, h->bins, (double)h->w, soft_h(h),
+                           (double)h->sigma, d_counts);
+        WMD_HIP_CHECK(hipGetLastError());
+    };
     const SoftPlanOut pl = soft_plan(h, B, n);
     if (pl.k == 0) {
         // brute force: one thread per atom, frames along y
@@ -2182,7 +2198,7 @@ void soft_run(wmd_softrdf *h, bool backward, const float *d_soa, size_t stride, 
             sp.grad_frame0 = (long long)f0;
             ProfScope prof(h->ctx, st, backward ? "softrdf_bwd" : "softrdf_fwd");
             if (!backward) {
-                const size_t lds = (size_t)(h->bins + kSoftPadL + kSoftPadR) * 8;
+                const size_t lds = (size_t)tn * 8;
                 WMD_REQUIRE(lds <= h->ctx->lds_per_block, WMD_ERR_UNSUPPORTED, "too many bins for LDS");
                 if (h->pbc == WMD_PBC_NONE)
                     hipLaunchKernelGGL(soft_brute_fwd_kernel<WMD_PBC_NONE>, grid, dim3(kBlock), lds, st, soa, (int)stride, n, bx, sp);
@@ -2194,7 +2210,7 @@ void soft_run(wmd_softrdf *h, bool backward, const float *d_soa, size_t stride, 
                 sp.grad_frame0 = 0;
                 SoftParams spb = sp;
                 spb.grad = d_grad + f0 * 3 * stride;
-                const size_t lds = (size_t)sp.n_nodes * 8;
+                const size_t lds = (size_t)tn * 4;
                 WMD_REQUIRE(lds <= h->ctx->lds_per_block, WMD_ERR_UNSUPPORTED, "too many bins for LDS");
                 if (h->pbc == WMD_PBC_NONE)
                     hipLaunchKernelGGL(soft_brute_bwd_kernel<WMD_PBC_NONE>, grid, dim3(kBlock), lds, st, soa, (int)stride, n, bx, spb);
@@ -2205,6 +2221,7 @@ void soft_run(wmd_softrdf *h, bool backward, const float *d_soa, size_t stride, 
             }
             WMD_HIP_CHECK(hipGetLastError());
         }
+        conv();
         h->last_path = 1;
         h->last_nc[0] = h->last_nc[1] = h->last_nc[2] = 0;
         return;
@@ -2225,8 +2242,8 @@ void soft_run(wmd_softrdf *h, bool backward, const float *d_soa, size_t stride, 
         sp.grad_frame0 = (long long)f0;
         sp.order = (const unsigned *)h->cells.order.ptr;
         if (!backward) {
-            const size_t lds = align16((size_t)(h->bins + kSoftPadL + kSoftPadR) * 8) +
-                               (size_t)kSoftWaves * 128 * 4 + (size_t)kSoftWaves * kRing * 16;
+            const size_t lds = align16((size_t)tn * 8) + (size_t)kSoftWaves * 128 * 4 +
+                               (size_t)kSoftWaves * kRing * 16;
             if (h->pbc == WMD_PBC_ORTHO)
                 launch_soft_cells(h, st, soft_cell_fwd_kernel<WMD_PBC_ORTHO, false>, lds, G, sp, "softrdf_fwd");
             else if (pl.lt)
@@ -2234,7 +2251,7 @@ void soft_run(wmd_softrdf *h, bool backward, const float *d_soa, size_t stride, 
             else
                 launch_soft_cells(h, st, soft_cell_fwd_kernel<WMD_PBC_TRICLINIC, false>, lds, G, sp, "softrdf_fwd");
         } else {
-            const size_t lds = align16((size_t)sp.n_nodes * 8) + (size_t)kSoftWaves * kSoftMaxHome * 16 +
+            const size_t lds = align16((size_t)tn * 4) + (size_t)kSoftWaves * kSoftMaxHome * 16 +
                                (size_t)kSoftWaves * kRing * 4 + (size_t)kSoftWaves * kRing * 16;
             if (h->pbc == WMD_PBC_ORTHO)
                 launch_soft_cells(h, st, soft_cell_bwd_kernel<WMD_PBC_ORTHO, false>, lds, G, sp, "softrdf_bwd");
@@ -2244,6 +2261,7 @@ void soft_run(wmd_softrdf *h, bool backward, const float *d_soa, size_t stride, 
                 launch_soft_cells(h, st, soft_cell_bwd_kernel<WMD_PBC_TRICLINIC, false>, lds, G, sp, "softrdf_bwd");
         }
     }
+    conv();
     h->last_path = 2;
     h->last_nc[0] = pl.nc[0]; h->last_nc[1] = pl.nc[1]; h->last_nc[2] = pl.nc[2];
 }
@@ -2261,8 +2279,8 @@ wmd_status wmd_softrdf_create(wmd_ctx *ctx, size_t bins, float r_max, float sigm
     WMD_REQUIRE(std::isfinite(r_max) && r_max > 0.f, WMD_ERR_INVALID, "r_max must be positive");
     WMD_REQUIRE(pbc >= WMD_PBC_NONE && pbc <= WMD_PBC_TRICLINIC, WMD_ERR_INVALID, "unknown pbc kind");
     const float w = r_max / (float)bins;
-    WMD_REQUIRE(std::isfinite(sigma) && sigma >= 0.5f * w && sigma <= 2.0f * w, WMD_ERR_UNSUPPORTED,
-                "sigma must lie in [0.5, 2] bin widths (the 17-bin kernel window)");
+    WMD_REQUIRE(std::isfinite(sigma) && sigma >= 0.25f * w && sigma <= 4.0f * w, WMD_ERR_UNSUPPORTED,
+                "sigma must lie in [1/4, 4] bin widths");
     bind(ctx);
     wmd_softrdf *h = new wmd_softrdf;
     h->ctx = ctx;

@@ -82,9 +82,10 @@ def _check_positions(x, name="positions"):
 class SoftRdf:
     """Gaussian-smoothed RDF of one atom set (all atoms of the positions).
 
-    c_k = sum over ordered pairs i != j of w / (sigma sqrt(2 pi))
-    exp(-(r_k - r_ij)^2 / (2 sigma^2)), r_k = (k + 1/2) w, over the 17 bins
-    around floor(r_ij / w); g_k = c_k / (frames shell_k rho) with
+    Every ordered pair i != j deposits the quadratic B-spline B(r_ij / h - m)
+    on nodes m h (h = w / 4, w = r_max / bins); bin k collects
+    c_k = sum over |m h - r_k| <= 6 sigma of H_m w phi_sigma(r_k - m h),
+    r_k = (k + 1/2) w; g_k = c_k / (frames shell_k rho) with
     rho = N (N - 1) / mean box volume.
     """
 
