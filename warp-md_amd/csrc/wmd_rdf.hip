@@ -999,9 +999,8 @@ struct SoftParams {
     float cut2;         // squared pair reach
     double *nodes;      // [n_nodes] H (forward output, accumulated)
     const float *table; // [n_nodes] D (backward)
-    float *grad;        // [F][3][stride] (backward output, accumulated)
-    const unsigned *order;  // [F][n] sorted position -> store slot (cell path)
-    long long grad_frame0;  // frame offset of this batch in grad
+    float *grad;        // brute force: [F][3][stride] output; cell list: [F][3][n] in cell order
+    long long grad_frame0;  // frame offset of this batch in grad (brute force)
     int stride;
     int n;              // atoms per frame
 };
@@ -1126,16 +1125,17 @@ struct SoftFwdCons {
 };
 
 // Backward consumer: dL/dx_j accumulates in the j lane over the home atoms and
-// is added to the output once per chunk; dL/dx_i is summed over the 16 lanes
-// of each row by DPP and collected per home atom in LDS until the unit ends
-// (home atoms beyond kSoftMaxHome add their row sums to the output directly).
+// is added once per chunk; dL/dx_i is summed over the 16 lanes of each row by
+// DPP and collected per home atom in LDS until the unit ends (home atoms
+// beyond kSoftMaxHome add their row sums directly).  The output is indexed by
+// cell-sorted position, where a chunk's lanes mostly hold consecutive atoms,
+// so the atomics of one instruction fall on a few cache lines.
 template <int PBC, bool LT>
 struct SoftBwdCons {
     SoftParams sp;
     const float *tab;   // LDS node table (padded)
     float4 *gacc;       // this wave's [kSoftMaxHome] home-atom gradients
-    const unsigned *ord;  // the current frame's sorted position -> store slot
-    int hs;               // the current unit's first home position
+    int hs;             // the current unit's first home position
     __device__ __forceinline__ static float row_sum(float v) {
         v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0xB1, 0xF, 0xF, false));
         v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x4E, 0xF, 0xF, false));
@@ -1143,16 +1143,13 @@ struct SoftBwdCons {
         v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x140, 0xF, 0xF, false));
         return v;
     }
-    __device__ __forceinline__ void add_grad(int f, unsigned slot, float gx, float gy, float gz) {
-        float *g = sp.grad + (sp.grad_frame0 + f) * 3 * (long long)sp.stride + slot;
+    __device__ __forceinline__ void add_grad(int f, unsigned pos, float gx, float gy, float gz) {
+        float *g = sp.grad + (long long)f * 3 * sp.n + pos;
         atomicAdd(g, gx);
-        atomicAdd(g + sp.stride, gy);
-        atomicAdd(g + 2 * sp.stride, gz);
+        atomicAdd(g + sp.n, gy);
+        atomicAdd(g + 2 * sp.n, gz);
     }
-    __device__ __forceinline__ void unit_begin(int f, int h0, int) {
-        ord = sp.order + (long long)f * sp.n;
-        hs = h0;
-    }
+    __device__ __forceinline__ void unit_begin(int, int h0, int) { hs = h0; }
     template <int MODE, bool HOME>
     __device__ __forceinline__ void chunk(const float4 *hp, int n_home, float xj, float yj, float zj,
                                           unsigned mj, int rank, unsigned jpos, int f,
@@ -1160,8 +1157,6 @@ struct SoftBwdCons {
         float sx, sy, sz;
         soft_shift<MODE, PBC>(mj, b, sx, sy, sz);
         const int lane = threadIdx.x & 63;
-        // the j slot is needed only after the loop: issue its load first
-        const unsigned jslot = jpos != 0xFFFFFFFFu ? ord[jpos] : 0u;
         float gjx = 0.f, gjy = 0.f, gjz = 0.f;
         for (int i = 0; i < n_home; ++i) {
             float dx, dy, dz;
@@ -1183,11 +1178,11 @@ struct SoftBwdCons {
                     atomicAdd(&gacc[i].y, -ry);
                     atomicAdd(&gacc[i].z, -rz);
                 } else {
-                    add_grad(f, ord[hs + i], -rx, -ry, -rz);
+                    add_grad(f, (unsigned)(hs + i), -rx, -ry, -rz);
                 }
             }
         }
-        if (jpos != 0xFFFFFFFFu) add_grad(f, jslot, gjx, gjy, gjz);
+        if (jpos != 0xFFFFFFFFu) add_grad(f, jpos, gjx, gjy, gjz);
     }
     __device__ __forceinline__ void unit_end(int f, int, int n_home) {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -1196,7 +1191,7 @@ struct SoftBwdCons {
         const int lane = threadIdx.x & 63;
         if (lane < min(n_home, kSoftMaxHome)) {
             const float4 g = gacc[lane];
-            add_grad(f, ord[hs + lane], g.x, g.y, g.z);
+            add_grad(f, (unsigned)(hs + lane), g.x, g.y, g.z);
             gacc[lane] = make_float4(0.f, 0.f, 0.f, 0.f);
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -1204,6 +1199,25 @@ struct SoftBwdCons {
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
 };
+
+// grad[f][d][order[f][p]] += gs[f][d][p]: cell order back to store slots.
+__global__ __launch_bounds__(kBlock) void soft_unsort_kernel(const float *__restrict__ gs,
+                                                             const unsigned *__restrict__ order,
+                                                             int n, int n_frames, int stride,
+                                                             float *__restrict__ grad) {
+    const long long total = (long long)n * n_frames;
+    for (long long t = (long long)blockIdx.x * kBlock + threadIdx.x; t < total;
+         t += (long long)gridDim.x * kBlock) {
+        const int f = (int)(t / n);
+        const int p = (int)(t - (long long)f * n);
+        const unsigned slot = order[t];
+        const float *src = gs + (long long)f * 3 * n + p;
+        float *dst = grad + (long long)f * 3 * stride + slot;
+        dst[0] += src[0];
+        dst[stride] += src[n];
+        dst[2 * stride] += src[2 * n];
+    }
+}
 
 constexpr int kSoftBlock = 768;  // 12 waves
 constexpr int kSoftWaves = kSoftBlock / 64;
@@ -1244,7 +1258,7 @@ __global__ __launch_bounds__(kSoftBlock) void soft_cell_bwd_kernel(CellGrid G, S
         gaccs[i] = make_float4(0.f, 0.f, 0.f, 0.f);
     __syncthreads();
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    SoftBwdCons<PBC, LT> cons{sp, tab, gaccs + wave * kSoftMaxHome, sp.order, 0};
+    SoftBwdCons<PBC, LT> cons{sp, tab, gaccs + wave * kSoftMaxHome, 0};
     cell_waves<PBC, true>(G, rings + wave * kRing, ringps + wave * kRing, wave, cons);
 }
 
@@ -2092,7 +2106,7 @@ struct wmd_softrdf {
     int cell_k = 0;
     int last_path = 0;
     int last_nc[3] = {0, 0, 0};
-    DevBuf d_boxes, d_table, d_nodes;
+    DevBuf d_boxes, d_table, d_nodes, d_gsorted;
     HostBuf h_boxes;
     CellScratch cells;
 };
@@ -2239,8 +2253,13 @@ void soft_run(wmd_softrdf *h, bool backward, const float *d_soa, size_t stride, 
         CellGrid G{h->cells.sorted.as<float4>(), (const unsigned *)h->cells.start.ptr,
                    (const unsigned *)h->cells.flag.ptr, bx, n, nf, pl.nc[0], pl.nc[1], pl.nc[2], pl.k,
                    shift_ok, 8, cull2};
-        sp.grad_frame0 = (long long)f0;
-        sp.order = (const unsigned *)h->cells.order.ptr;
+        if (backward) {
+            // per-batch gradient in cell order, scattered to the store afterwards
+            sp.grad = (float *)h->d_gsorted.reserve((size_t)3 * n * nf * 4);
+            hipLaunchKernelGGL(fill_u32_kernel, dim3(std::min<long long>(65536, ((long long)3 * n * nf + kBlock - 1) / kBlock)),
+                               dim3(kBlock), 0, st, (unsigned *)sp.grad, (long long)3 * n * nf, 0u);
+            WMD_HIP_CHECK(hipGetLastError());
+        }
         if (!backward) {
             const size_t lds = align16((size_t)tn * 8) + (size_t)kSoftWaves * 128 * 4 +
                                (size_t)kSoftWaves * kRing * 16;
@@ -2259,6 +2278,12 @@ void soft_run(wmd_softrdf *h, bool backward, const float *d_soa, size_t stride, 
                 launch_soft_cells(h, st, soft_cell_bwd_kernel<WMD_PBC_TRICLINIC, true>, lds, G, sp, "softrdf_bwd");
             else
                 launch_soft_cells(h, st, soft_cell_bwd_kernel<WMD_PBC_TRICLINIC, false>, lds, G, sp, "softrdf_bwd");
+            int g;
+            grid1d((long long)n * nf, &g);
+            hipLaunchKernelGGL(soft_unsort_kernel, dim3(g), dim3(kBlock), 0, st, (const float *)sp.grad,
+                               (const unsigned *)h->cells.order.ptr, n, nf, (int)stride,
+                               d_grad + f0 * 3 * stride);
+            WMD_HIP_CHECK(hipGetLastError());
         }
     }
     conv();
