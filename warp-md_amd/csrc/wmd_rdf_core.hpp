@@ -1,0 +1,1137 @@
+// Shared core of the RDF engine (included by wmd_rdf.hip and wmd_batch.hip):
+// the per-frame box parameters, the exact-threshold histogram parameters and
+// their LDS layout, the reference-order pair expressions, the brute-force and
+// cell-list pair kernels with their unit loop, the cell-list construction
+// kernels, and the host helpers that choose and build a cell grid.  The
+// exactness and enumeration arguments are stated at the top of wmd_rdf.hip.
+#pragma once
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <unordered_map>
+
+#include "wmd_internal.hpp"
+
+using namespace wmd;
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kTile = 256;        // brute-force tile
+constexpr int kMaxTab = 8192;     // s-bits bin table entries (u16)
+constexpr uint32_t kPad = 0xFFFFFFFFu;
+constexpr int kImageLimit = 8;  // |image index| beyond which a frame uses the rint path
+constexpr int kImageBias = 32;  // image indices are stored as m + 32 in one byte
+
+struct FrameBox {
+    float lx, ly, lz, ilx, ily, ilz;
+    float r[9];  // inverse rows: f_d = r[0*3+d]*dx + r[1*3+d]*dy + r[2*3+d]*dz
+    float c[9];  // box rows: d_d = f0*c[0*3+d] + f1*c[1*3+d] + f2*c[2*3+d]
+};
+
+struct HistParams {
+    int bins;
+    int hist_copies;  // 4 = one sub-histogram per wave, 1 = shared
+    float cut;        // counted iff s < cut
+    float inv_w;      // approximate 1 / bin_width for the sqrt estimate
+    const float *thr; // [bins + 1]
+    const float *elo; // [bins + 1] edge diagnostics (EDGE variants)
+    const float *ehi;
+    const unsigned short *tab;  // [tab_n] s-bits bin table (TABLE variants)
+    const uint2 *tab2;          // [tab_n] per bucket (split threshold bits, 8 x bin below it)
+    int tab_n, tab_sh, tab_base;
+    unsigned long long *counts;  // [bins]
+    unsigned long long *edge;    // [1]
+};
+
+// LDS layout shared by the pair kernels:
+//   thr[bins+1] | elo, ehi (EDGE) | tab[tab_n] (TABLE) | hist[copies][bins] | kernel-specific
+struct LdsHist {
+    float *thr, *elo, *ehi;
+    unsigned short *tab;
+    unsigned *hist;
+    char *rest;
+};
+
+__host__ __device__ inline size_t align16(size_t x) { return (x + 15) / 16 * 16; }
+
+__host__ __device__ inline size_t lds_hist_size(int bins, int copies, bool edge, int tab_n) {
+    size_t nb1 = (size_t)bins + 1;
+    return align16(nb1 * 4 * (edge ? 3 : 1)) + align16((size_t)tab_n * 2) +
+           align16((size_t)copies * bins * 4);
+}
+
+__device__ __forceinline__ LdsHist lds_hist_layout(char *base, const HistParams &p, bool edge,
+                                                   bool table) {
+    LdsHist L;
+    size_t nb1 = (size_t)p.bins + 1;
+    L.thr = (float *)base;
+    L.elo = L.thr + nb1;
+    L.ehi = L.elo + nb1;
+    size_t off = align16(nb1 * 4 * (edge ? 3 : 1));
+    L.tab = (unsigned short *)(base + off);
+    off += align16((size_t)(table ? p.tab_n : 0) * 2);
+    L.hist = (unsigned *)(base + off);
+    off += align16((size_t)p.hist_copies * p.bins * 4);
+    L.rest = base + off;
+    return L;
+}
+
+template <bool EDGE, bool TABLE>
+__device__ void lds_hist_init(const LdsHist &L, const HistParams &p) {
+    for (int i = threadIdx.x; i <= p.bins; i += kBlock) {
+        L.thr[i] = p.thr[i];
+        if constexpr (EDGE) {
+            L.elo[i] = p.elo[i];
+            L.ehi[i] = p.ehi[i];
+        }
+    }
+    if constexpr (TABLE)
+        for (int i = threadIdx.x; i < p.tab_n; i += kBlock) L.tab[i] = p.tab[i];
+    for (int i = threadIdx.x; i < p.hist_copies * p.bins; i += kBlock) L.hist[i] = 0u;
+}
+
+// Merges the LDS sub-histograms into the global u64 counts (and clears them).
+template <bool EDGE>
+__device__ void lds_hist_flush(const LdsHist &L, const HistParams &p, unsigned &edge_local) {
+    __syncthreads();
+    for (int i = threadIdx.x; i < p.bins; i += kBlock) {
+        unsigned long long sum = 0;
+        for (int c = 0; c < p.hist_copies; ++c) {
+            sum += L.hist[c * p.bins + i];
+            L.hist[c * p.bins + i] = 0u;
+        }
+        if (sum) atomicAdd(&p.counts[i], sum);
+    }
+    if constexpr (EDGE) {
+        if (edge_local) atomicAdd(p.edge, (unsigned long long)edge_local);
+        edge_local = 0;
+    }
+    __syncthreads();
+}
+
+// Exact bin of a counted pair (s < cut).
+template <bool TABLE>
+__device__ __forceinline__ int exact_bin(float s, const LdsHist &L, const HistParams &p) {
+    if constexpr (TABLE) {
+        // bucket of s: saturating (bits - base) >> sh; s < cut keeps it below tab_n
+        const unsigned idx = __builtin_elementwise_sub_sat(__float_as_uint(s), (unsigned)p.tab_base) >> p.tab_sh;
+        int b = L.tab[idx];
+        b += (int)(s >= L.thr[b + 1]);
+        return b;
+    } else {
+        float r_est = __builtin_amdgcn_sqrtf(s);
+        int b = (int)(r_est * p.inv_w);
+        b = b < p.bins - 1 ? b : p.bins - 1;
+        float lo = L.thr[b];
+        float hi = L.thr[b + 1];
+        b += (int)(s >= hi) - (int)(s < lo);
+        return b;
+    }
+}
+
+template <bool EDGE, bool TABLE>
+__device__ __forceinline__ void count_pair(float s, unsigned w, const HistParams &p,
+                                           const LdsHist &L, unsigned *hist,
+                                           unsigned &edge_local) {
+    if (s < p.cut) {
+        int b = exact_bin<TABLE>(s, L, p);
+        atomicAdd(&hist[b], w);
+        if constexpr (EDGE) {
+            bool near = (s >= L.elo[b] && s < L.ehi[b]) || (s >= L.elo[b + 1] && s < L.ehi[b + 1]);
+            edge_local += near ? w : 0u;
+        }
+    }
+}
+
+// -------------------------------------------------------------------------
+// The per-pair squared distance, reference operation order (rint / exact wrap).
+// -------------------------------------------------------------------------
+template <int PBC, bool EXACT>
+__device__ __forceinline__ float pair_s(float xi, float yi, float zi, float xj, float yj, float zj,
+                                        const FrameBox &b) {
+    float dx = xj - xi;
+    float dy = yj - yi;
+    float dz = zj - zi;
+    if constexpr (PBC == WMD_PBC_ORTHO) {
+        if constexpr (EXACT) {
+            dx -= roundf(dx / b.lx) * b.lx;
+            dy -= roundf(dy / b.ly) * b.ly;
+            dz -= roundf(dz / b.lz) * b.lz;
+        } else {
+            dx -= rintf(dx * b.ilx) * b.lx;
+            dy -= rintf(dy * b.ily) * b.ly;
+            dz -= rintf(dz * b.ilz) * b.lz;
+        }
+    } else if constexpr (PBC == WMD_PBC_TRICLINIC) {
+        float f0 = b.r[0] * dx + b.r[3] * dy + b.r[6] * dz;
+        float f1 = b.r[1] * dx + b.r[4] * dy + b.r[7] * dz;
+        float f2 = b.r[2] * dx + b.r[5] * dy + b.r[8] * dz;
+        if constexpr (EXACT) {
+            f0 -= roundf(f0);
+            f1 -= roundf(f1);
+            f2 -= roundf(f2);
+        } else {
+            f0 -= rintf(f0);
+            f1 -= rintf(f1);
+            f2 -= rintf(f2);
+        }
+        dx = f0 * b.c[0] + f1 * b.c[3] + f2 * b.c[6];
+        dy = f0 * b.c[1] + f1 * b.c[4] + f2 * b.c[7];
+        dz = f0 * b.c[2] + f1 * b.c[5] + f2 * b.c[8];
+    }
+    return dx * dx + dy * dy + dz * dz;
+}
+
+// Shift path: the image is known; (sx, sy, sz) hold n*L (orthorhombic) or n
+// (triclinic fractional).  LT: lower-triangular box (r[1] = r[2] = r[5] = 0 and
+// c[1] = c[2] = c[5] = 0), whose zero terms are skipped.
+template <int PBC, bool LT>
+__device__ __forceinline__ float pair_s_shift(float xi, float yi, float zi, float xj, float yj,
+                                              float zj, float sx, float sy, float sz,
+                                              const FrameBox &b) {
+    float dx = xj - xi;
+    float dy = yj - yi;
+    float dz = zj - zi;
+    if constexpr (PBC == WMD_PBC_ORTHO) {
+        dx -= sx;
+        dy -= sy;
+        dz -= sz;
+    } else {
+        float f0, f1, f2;
+        if constexpr (LT) {
+            f0 = b.r[0] * dx + b.r[3] * dy + b.r[6] * dz;
+            f1 = b.r[4] * dy + b.r[7] * dz;
+            f2 = b.r[8] * dz;
+        } else {
+            f0 = b.r[0] * dx + b.r[3] * dy + b.r[6] * dz;
+            f1 = b.r[1] * dx + b.r[4] * dy + b.r[7] * dz;
+            f2 = b.r[2] * dx + b.r[5] * dy + b.r[8] * dz;
+        }
+        f0 -= sx;
+        f1 -= sy;
+        f2 -= sz;
+        if constexpr (LT) {
+            dx = f0 * b.c[0] + f1 * b.c[3] + f2 * b.c[6];
+            dy = f1 * b.c[4] + f2 * b.c[7];
+            dz = f2 * b.c[8];
+        } else {
+            dx = f0 * b.c[0] + f1 * b.c[3] + f2 * b.c[6];
+            dy = f0 * b.c[1] + f1 * b.c[4] + f2 * b.c[7];
+            dz = f0 * b.c[2] + f1 * b.c[5] + f2 * b.c[8];
+        }
+    }
+    return dx * dx + dy * dy + dz * dz;
+}
+
+// XCD-aware persistent schedule: WG g runs on XCD g % 8 (round-robin dispatch);
+// each XCD owns a contiguous slice of the unit range.
+struct UnitRange {
+    long long begin, end, step;
+};
+
+__device__ __forceinline__ UnitRange xcd_units(long long n_units, int n_xcd) {
+    int g = blockIdx.x;
+    int G = gridDim.x;
+    int xcd = g % n_xcd;
+    int local = g / n_xcd;
+    int per_xcd = G / n_xcd;  // host launches a multiple of n_xcd workgroups
+    long long lo = n_units * xcd / n_xcd;
+    long long hi = n_units * (xcd + 1) / n_xcd;
+    return {lo + local, hi, (long long)per_xcd};
+}
+
+// -------------------------------------------------------------------------
+// Brute force over 256-atom tiles of the IO selection.
+// soa: [F][3][stride]; weights (WEIGHTED): packed mA | mB << 8 per IO atom.
+// tile_pairs: (it, jt) with it <= jt, upper-triangle order.
+// -------------------------------------------------------------------------
+template <int PBC, bool EXACT, bool WEIGHTED, bool EDGE, bool TABLE>
+__global__ __launch_bounds__(kBlock) void rdf_brute_kernel(
+    const float *__restrict__ soa, long long frame_stride, int stride, int n, int n_frames,
+    const FrameBox *__restrict__ boxes, const unsigned *__restrict__ wts,
+    const int2 *__restrict__ tile_pairs, int n_tile_pairs, long long frame0, HistParams p,
+    int n_xcd, unsigned w_max) {
+    extern __shared__ __attribute__((aligned(16))) char lds[];
+    LdsHist L = lds_hist_layout(lds, p, EDGE, TABLE);
+    float *jx = (float *)L.rest;
+    float *jy = jx + kTile;
+    float *jz = jy + kTile;
+    unsigned *jw = (unsigned *)(jz + kTile);
+    lds_hist_init<EDGE, TABLE>(L, p);
+    unsigned edge_local = 0;
+    const int wave = threadIdx.x >> 6;
+    unsigned *hist = L.hist + (p.hist_copies == 1 ? 0 : wave) * p.bins;
+    const long long n_units = (long long)n_frames * n_tile_pairs;
+    UnitRange ur = xcd_units(n_units, n_xcd);
+    unsigned long long budget = 0;
+    __syncthreads();
+    for (long long u = ur.begin; u < ur.end; u += ur.step) {
+        const int f = (int)(u / n_tile_pairs);
+        const int2 tp = tile_pairs[u - (long long)f * n_tile_pairs];
+        const float *fx = soa + (frame0 + f) * frame_stride;
+        const float *fy = fx + stride;
+        const float *fz = fy + stride;
+        const FrameBox b = boxes[f];
+        const int i = tp.x * kTile + threadIdx.x;
+        const float nanv = __builtin_nanf("");
+        float xi = nanv, yi = nanv, zi = nanv;
+        unsigned wi = 0;
+        if (i < n) {
+            xi = fx[i];
+            yi = fy[i];
+            zi = fz[i];
+            if constexpr (WEIGHTED) wi = wts[i];
+        }
+        const int j0 = tp.y * kTile;
+        const int nj = min(kTile, n - j0);
+        __syncthreads();  // previous unit done with the j tile
+        if ((int)threadIdx.x < nj) {
+            jx[threadIdx.x] = fx[j0 + threadIdx.x];
+            jy[threadIdx.x] = fy[j0 + threadIdx.x];
+            jz[threadIdx.x] = fz[j0 + threadIdx.x];
+            if constexpr (WEIGHTED) jw[threadIdx.x] = wts[j0 + threadIdx.x];
+        }
+        __syncthreads();
+        if (tp.x == tp.y) {
+            // ordered pairs (i, j), i != j
+            for (int jj = 0; jj < nj; ++jj) {
+                float s = pair_s<PBC, EXACT>(xi, yi, zi, jx[jj], jy[jj], jz[jj], b);
+                if (jj == (int)threadIdx.x) s = nanv;
+                unsigned w = 1u;
+                if constexpr (WEIGHTED) w = (wi & 0xFFu) * (jw[jj] >> 8);
+                count_pair<EDGE, TABLE>(s, w, p, L, hist, edge_local);
+            }
+        } else {
+            // unordered pairs: both orders at once
+            for (int jj = 0; jj < nj; ++jj) {
+                float s = pair_s<PBC, EXACT>(xi, yi, zi, jx[jj], jy[jj], jz[jj], b);
+                unsigned w = 2u;
+                if constexpr (WEIGHTED) {
+                    unsigned wj = jw[jj];
+                    w = (wi & 0xFFu) * (wj >> 8) + (wj & 0xFFu) * (wi >> 8);
+                }
+                count_pair<EDGE, TABLE>(s, w, p, L, hist, edge_local);
+            }
+        }
+        // at most kTile pairs per thread column and sub-histogram copy per unit
+        budget += (unsigned long long)kTile * kTile * w_max;
+        if (budget > 0x7FFFFFFFull) {
+            budget = 0;
+            lds_hist_flush<EDGE>(L, p, edge_local);
+        }
+    }
+    lds_hist_flush<EDGE>(L, p, edge_local);
+}
+
+// -------------------------------------------------------------------------
+// Cell list construction.
+// cell key = (cz * ncy + cy) * ncx + cx from the wrapped fractional position;
+// meta word of a sorted atom: image index (mx, my, mz) + 32 as 3 x u8 in bits
+// 0..23, selection multiplicities mA in bits 24..27 and mB in bits 28..31.
+// -------------------------------------------------------------------------
+template <int PBC>
+__device__ __forceinline__ void frac_of(float x, float y, float z, const FrameBox &b, float *f) {
+    if constexpr (PBC == WMD_PBC_ORTHO) {
+        f[0] = x * b.ilx;
+        f[1] = y * b.ily;
+        f[2] = z * b.ilz;
+    } else {
+        f[0] = b.r[0] * x + b.r[3] * y + b.r[6] * z;
+        f[1] = b.r[1] * x + b.r[4] * y + b.r[7] * z;
+        f[2] = b.r[2] * x + b.r[5] * y + b.r[8] * z;
+    }
+}
+
+__global__ void fill_u32_kernel(unsigned *p, long long n, unsigned v) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long step = (long long)gridDim.x * blockDim.x;
+    for (; i < n; i += step) p[i] = v;
+}
+
+__global__ void fill_u64_kernel(unsigned long long *p, long long n) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
+template <int PBC>
+__global__ __launch_bounds__(kBlock) void cell_assign_kernel(
+    const float *__restrict__ soa, long long frame_stride, int stride, int n, int n_frames,
+    long long frame0, const FrameBox *__restrict__ boxes, int ncx, int ncy, int ncz,
+    unsigned *__restrict__ cell_of, unsigned *__restrict__ cell_slot,
+    unsigned *__restrict__ cell_count, unsigned *__restrict__ image_of,
+    unsigned *__restrict__ frame_flag) {
+    const int ncell = ncx * ncy * ncz;
+    const int nc[3] = {ncx, ncy, ncz};
+    long long total = (long long)n * n_frames;
+    for (long long t = (long long)blockIdx.x * kBlock + threadIdx.x; t < total;
+         t += (long long)gridDim.x * kBlock) {
+        int f = (int)(t / n);
+        int i = (int)(t - (long long)f * n);
+        const float *fx = soa + (frame0 + f) * frame_stride;
+        float fr[3];
+        frac_of<PBC>(fx[i], fx[stride + i], fx[2 * stride + i], boxes[f], fr);
+        int c[3];
+        unsigned img = 0;
+        bool far_out = false;
+        for (int d = 0; d < 3; ++d) {
+            float fl = floorf(fr[d]);
+            float w = fr[d] - fl;  // in [0, 1) (NaN for non-finite input)
+            int cc = (int)(w * (float)nc[d]);
+            c[d] = min(max(cc, 0), nc[d] - 1);
+            // image index; non-finite or far-away atoms force the rint path
+            bool ok = fl >= -(float)kImageLimit && fl <= (float)kImageLimit;
+            int m = ok ? (int)fl : 0;
+            far_out |= !ok;
+            img |= ((unsigned)(m + kImageBias) & 0xFFu) << (8 * d);
+        }
+        if (far_out) atomicOr(&frame_flag[f], 1u);
+        int cid = (c[2] * ncy + c[1]) * ncx + c[0];
+        unsigned slot = atomicAdd(&cell_count[(long long)f * ncell + cid], 1u);
+        cell_of[t] = (unsigned)cid;
+        cell_slot[t] = slot;
+        image_of[t] = img;
+    }
+}
+
+// Exclusive scan of each frame's cell counts (one workgroup per frame);
+// start has ncell + 1 entries per frame.
+__global__ __launch_bounds__(kBlock) void cell_scan_kernel(const unsigned *__restrict__ count,
+                                                           unsigned *__restrict__ start, int ncell) {
+    __shared__ unsigned part[kBlock];
+    const long long f = blockIdx.x;
+    const unsigned *c = count + f * ncell;
+    unsigned *s = start + f * (ncell + 1);
+    int per = (ncell + kBlock - 1) / kBlock;
+    int lo = threadIdx.x * per;
+    int hi = min(lo + per, ncell);
+    unsigned sum = 0;
+    for (int k = lo; k < hi; ++k) sum += c[k];
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    // Hillis-Steele inclusive scan over 256 partial sums
+    for (int off = 1; off < kBlock; off <<= 1) {
+        unsigned v = threadIdx.x >= (unsigned)off ? part[threadIdx.x - off] : 0u;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    unsigned run = threadIdx.x ? part[threadIdx.x - 1] : 0u;
+    for (int k = lo; k < hi; ++k) {
+        s[k] = run;
+        run += c[k];
+    }
+    if (threadIdx.x == kBlock - 1) s[ncell] = part[kBlock - 1];
+}
+
+// sorted[f][start[c] + slot] = (x, y, z, meta)
+__global__ __launch_bounds__(kBlock) void cell_scatter_kernel(
+    const float *__restrict__ soa, long long frame_stride, int stride, int n, int n_frames,
+    long long frame0, const unsigned *__restrict__ cell_of, const unsigned *__restrict__ cell_slot,
+    const unsigned *__restrict__ image_of, const unsigned *__restrict__ start, int ncell,
+    const unsigned *__restrict__ wts, float4 *__restrict__ sorted, unsigned *__restrict__ order) {
+    long long total = (long long)n * n_frames;
+    for (long long t = (long long)blockIdx.x * kBlock + threadIdx.x; t < total;
+         t += (long long)gridDim.x * kBlock) {
+        int f = (int)(t / n);
+        int i = (int)(t - (long long)f * n);
+        const float *fx = soa + (frame0 + f) * frame_stride;
+        unsigned pos = start[(long long)f * (ncell + 1) + cell_of[t]] + cell_slot[t];
+        unsigned w = wts ? wts[i] : 0x0101u;  // mA | mB << 8
+        unsigned meta = (image_of[t] & 0xFFFFFFu) | ((w & 0xFu) << 24) | (((w >> 8) & 0xFu) << 28);
+        float4 v;
+        v.x = fx[i];
+        v.y = fx[stride + i];
+        v.z = fx[2 * stride + i];
+        v.w = __uint_as_float(meta);
+        sorted[(long long)f * n + pos] = v;
+        if (order) order[(long long)f * n + pos] = (unsigned)i;  // sorted position -> store slot
+    }
+}
+
+
+// -------------------------------------------------------------------------
+// Cell-list pair kernel.  One wave per unit (frame, home cell); no block-level
+// synchronisation between units.  The unit's j stream is the home atoms, then
+// the neighbour cells of the half stencil taken as rows along x: for each
+// (dy, dz) with dz > 0, or dz = 0 and dy > 0, the cells hx-k..hx+k, and in the
+// home row the cells hx+1..hx+k.  Such a row is one contiguous run of the
+// cell-sorted atoms (two runs where it wraps in x); runs that wrap the box are
+// streamed after those that do not.  Lanes load 64 stream atoms at a time,
+// drop neighbours farther than the cull radius from the home atoms' bounding
+// box (shift path), and append the rest to a per-wave LDS ring; every full
+// ring chunk of 64 j atoms is paired with all home atoms i, which are uniform
+// across the wave and read with scalar loads.  Home atoms in the stream pair
+// only with home atoms of lower index (unordered pairs, weight 2).  The
+// histogram is a u64 LDS array shared by the workgroup's waves.
+// -------------------------------------------------------------------------
+constexpr int kCellBlock = 768;  // 12 waves, 3 per SIMD: two workgroups per CU within the kernel's SGPR budget
+constexpr int kCellWaves = kCellBlock / 64;
+constexpr int kRing = 128;       // per-wave ring entries (float4 each)
+constexpr int kMaxTab2 = 5120;   // split-table entries the cell kernel keeps in LDS
+
+struct CellLds {
+    unsigned long long *hist;
+    uint2 *tab2;
+    float *thr, *elo, *ehi;
+    float4 *ring;  // [kCellWaves][kRing]
+};
+
+__host__ __device__ inline size_t cell_lds_bytes(int bins, bool edge, int tab_n) {
+    return align16((size_t)bins * 8) + align16((size_t)tab_n * 8) +
+           align16(((size_t)bins + 1) * 4 * (edge ? 3 : 1)) + (size_t)kCellWaves * kRing * 16;
+}
+
+__device__ __forceinline__ CellLds cell_lds_layout(char *base, const HistParams &p, bool edge,
+                                                   bool table) {
+    CellLds L;
+    const size_t nb1 = (size_t)p.bins + 1;
+    size_t off = 0;
+    L.hist = (unsigned long long *)base;
+    off += align16((size_t)p.bins * 8);
+    L.tab2 = (uint2 *)(base + off);
+    off += align16((size_t)(table ? p.tab_n : 0) * 8);
+    L.thr = (float *)(base + off);
+    L.elo = L.thr + nb1;
+    L.ehi = L.elo + nb1;
+    off += align16(nb1 * 4 * (edge ? 3 : 1));
+    L.ring = (float4 *)(base + off);
+    return L;
+}
+
+// Byte offset (8 x bin) of the exact bin of a counted pair (s < cut).
+template <bool TABLE>
+__device__ __forceinline__ unsigned cell_bin8(float s, const CellLds &L, const HistParams &p) {
+    if constexpr (TABLE) {
+        const unsigned t = __builtin_elementwise_sub_sat(__float_as_uint(s), (unsigned)p.tab_base);
+        const uint2 e = L.tab2[t >> p.tab_sh];
+        return e.y + (s >= __uint_as_float(e.x) ? 8u : 0u);
+    } else {
+        float r_est = __builtin_amdgcn_sqrtf(s);
+        int b = (int)(r_est * p.inv_w);
+        b = b < p.bins - 1 ? b : p.bins - 1;
+        b += (int)(s >= L.thr[b + 1]) - (int)(s < L.thr[b]);
+        return (unsigned)b * 8u;
+    }
+}
+
+// Wave-wide min / max of non-NaN floats through DPP row operations on their
+// order-preserving integer keys; every lane must be active.
+__device__ __forceinline__ int f_key(float x) {
+    const int b = __float_as_int(x);
+    return b ^ ((b >> 31) & 0x7FFFFFFF);
+}
+__device__ __forceinline__ float key_f(int k) { return __int_as_float(k ^ ((k >> 31) & 0x7FFFFFFF)); }
+template <int CTRL, int ROWS>
+__device__ __forceinline__ int dpp_i(int v) {
+    return __builtin_amdgcn_update_dpp(v, v, CTRL, ROWS, 0xF, false);
+}
+__device__ __forceinline__ float wave_min_key(int v) {
+    v = min(v, dpp_i<0xB1, 0xF>(v));   // quad_perm [1,0,3,2]
+    v = min(v, dpp_i<0x4E, 0xF>(v));   // quad_perm [2,3,0,1]
+    v = min(v, dpp_i<0x141, 0xF>(v));  // row_half_mirror
+    v = min(v, dpp_i<0x140, 0xF>(v));  // row_mirror
+    v = min(v, dpp_i<0x142, 0xA>(v));  // row_bcast:15 into rows 1, 3
+    v = min(v, dpp_i<0x143, 0xC>(v));  // row_bcast:31 into rows 2, 3
+    return key_f(__builtin_amdgcn_readlane(v, 63));
+}
+__device__ __forceinline__ float wave_max_key(int v) {
+    v = max(v, dpp_i<0xB1, 0xF>(v));
+    v = max(v, dpp_i<0x4E, 0xF>(v));
+    v = max(v, dpp_i<0x141, 0xF>(v));
+    v = max(v, dpp_i<0x140, 0xF>(v));
+    v = max(v, dpp_i<0x142, 0xA>(v));
+    v = max(v, dpp_i<0x143, 0xC>(v));
+    return key_f(__builtin_amdgcn_readlane(v, 63));
+}
+
+// Number of set bits of m below this lane.
+__device__ __forceinline__ int lanes_below(unsigned long long m) {
+    return (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
+                                          __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+}
+
+// Counts one pair (s, i's meta word) of a chunk.
+template <bool WEIGHTED, bool EDGE, bool TABLE>
+__device__ __forceinline__ void cell_count(float s, unsigned mi, unsigned mAj, unsigned mBj,
+                                           const HistParams &p, const CellLds &L,
+                                           unsigned long long &edge_local) {
+    if (s < p.cut) {
+        unsigned w = 2u;
+        if constexpr (WEIGHTED) w = ((mi >> 24) & 0xFu) * mBj + (mi >> 28) * mAj;
+        const unsigned off = cell_bin8<TABLE>(s, L, p);
+        atomicAdd((unsigned long long *)((char *)L.hist + off), (unsigned long long)w);
+        if constexpr (EDGE) {
+            const unsigned bin = off >> 3;
+            bool near = (s >= L.elo[bin] && s < L.ehi[bin]) ||
+                        (s >= L.elo[bin + 1] && s < L.ehi[bin + 1]);
+            edge_local += near ? w : 0u;
+        }
+    }
+}
+
+// Pairs every j of one ring chunk with every home atom i (i unrolled by 4:
+// one 64-byte scalar load per group).
+// MODE 0: zero image shift (orthorhombic: dx = xj - xi); 1: j carries its
+// image shift n (n + 128 as 3 x u8 in the meta word); 2: the rint wrap.
+// HOME: the chunk holds home atoms (stream positions below n_home), which
+// pair only with home atoms of lower index.
+template <int PBC, bool LT, bool WEIGHTED, bool EDGE, bool TABLE, int MODE, bool HOME>
+__device__ __forceinline__ void cell_chunk(const float4 *__restrict__ hp, int n_home, float xj,
+                                           float yj, float zj, unsigned mj, int rank,
+                                           const FrameBox &b, const HistParams &p,
+                                           const CellLds &L, unsigned long long &edge_local) {
+    float sx = 0.f, sy = 0.f, sz = 0.f;
+    if constexpr (MODE == 1) {
+        sx = (float)((int)(mj & 0xFFu) - 128);  // image shift n + 128 per byte
+        sy = (float)((int)((mj >> 8) & 0xFFu) - 128);
+        sz = (float)((int)((mj >> 16) & 0xFFu) - 128);
+        if constexpr (PBC == WMD_PBC_ORTHO) {
+            sx *= b.lx;  // the reference's f32 product round(dx/L) * L
+            sy *= b.ly;
+            sz *= b.lz;
+        }
+    }
+    const unsigned mAj = (mj >> 24) & 0xFu, mBj = mj >> 28;
+    const float nanv = __builtin_nanf("");
+    auto dist = [&](const float4 &pi) {
+        if constexpr (MODE == 0 && PBC == WMD_PBC_ORTHO) {
+            const float dx = xj - pi.x, dy = yj - pi.y, dz = zj - pi.z;
+            return dx * dx + dy * dy + dz * dz;
+        } else if constexpr (MODE <= 1) {
+            return pair_s_shift<PBC, LT>(pi.x, pi.y, pi.z, xj, yj, zj, sx, sy, sz, b);
+        } else {
+            return pair_s<PBC, false>(pi.x, pi.y, pi.z, xj, yj, zj, b);
+        }
+    };
+    int i = 0;
+    for (; i + 4 <= n_home; i += 4) {
+        const float4 a0 = hp[i], a1 = hp[i + 1], a2 = hp[i + 2], a3 = hp[i + 3];
+        float s0 = dist(a0), s1 = dist(a1), s2 = dist(a2), s3 = dist(a3);
+        if constexpr (HOME) {
+            s0 = i >= rank ? nanv : s0;
+            s1 = i + 1 >= rank ? nanv : s1;
+            s2 = i + 2 >= rank ? nanv : s2;
+            s3 = i + 3 >= rank ? nanv : s3;
+        }
+        cell_count<WEIGHTED, EDGE, TABLE>(s0, __float_as_uint(a0.w), mAj, mBj, p, L, edge_local);
+        cell_count<WEIGHTED, EDGE, TABLE>(s1, __float_as_uint(a1.w), mAj, mBj, p, L, edge_local);
+        cell_count<WEIGHTED, EDGE, TABLE>(s2, __float_as_uint(a2.w), mAj, mBj, p, L, edge_local);
+        cell_count<WEIGHTED, EDGE, TABLE>(s3, __float_as_uint(a3.w), mAj, mBj, p, L, edge_local);
+    }
+    for (; i < n_home; ++i) {
+        const float4 a0 = hp[i];
+        float s0 = dist(a0);
+        if constexpr (HOME) s0 = i >= rank ? nanv : s0;
+        cell_count<WEIGHTED, EDGE, TABLE>(s0, __float_as_uint(a0.w), mAj, mBj, p, L, edge_local);
+    }
+}
+
+// Inputs of the cell-list pair kernels: a batch of cell-sorted frames.
+struct CellGrid {
+    const float4 *sorted;       // [F][n] (x, y, z, meta) in cell order
+    const unsigned *start;      // [F][ncell + 1]
+    const unsigned *frame_flag; // [F] nonzero: the frame takes the rint path
+    const FrameBox *boxes;      // [F]
+    int n, n_frames, ncx, ncy, ncz, k, shift_ok, n_xcd;
+    float cull2;                // squared cull radius (pair reach plus rounding margin)
+};
+
+// Dispatches one chunk to the consumer's pair loop for its image mode.
+template <class C>
+__device__ __forceinline__ void cell_dispatch(C &cons, int mode, bool home, const float4 *hp,
+                                              int n_home, float xj, float yj, float zj,
+                                              unsigned mj, int rank, unsigned jpos, int f,
+                                              const FrameBox &b) {
+    if (home) {
+        if (mode == 0) cons.template chunk<0, true>(hp, n_home, xj, yj, zj, mj, rank, jpos, f, b);
+        else if (mode == 1) cons.template chunk<1, true>(hp, n_home, xj, yj, zj, mj, rank, jpos, f, b);
+        else cons.template chunk<2, true>(hp, n_home, xj, yj, zj, mj, rank, jpos, f, b);
+    } else {
+        if (mode == 0) cons.template chunk<0, false>(hp, n_home, xj, yj, zj, mj, rank, jpos, f, b);
+        else if (mode == 1) cons.template chunk<1, false>(hp, n_home, xj, yj, zj, mj, rank, jpos, f, b);
+        else cons.template chunk<2, false>(hp, n_home, xj, yj, zj, mj, rank, jpos, f, b);
+    }
+}
+
+// Geometry of the half stencil's pieces for reach k: lanes 1..n_pieces each
+// own one piece (odd lane: the in-range part of a row of cells along x, even
+// lane: its part wrapped across the box): row offsets (dy, dz) and the x range
+// [xa, xb] relative to the home cell.  Rows: k rows (dz = 0, dy > 0),
+// k (2k + 1) rows with dz > 0, and the home row's cells hx + 1 .. hx + k.
+struct StencilLane {
+    int dy, dz, xa, xb;
+};
+
+__device__ __forceinline__ int stencil_pieces(int k) { return 2 * (2 * k * k + 2 * k + 1); }
+
+__device__ __forceinline__ StencilLane stencil_lane(int lane, int k) {
+    StencilLane s{0, 0, 0, -1};
+    const int n_rows = 2 * k * k + 2 * k + 1;
+    if (lane >= 1 && lane <= 2 * n_rows) {
+        const int r = (lane - 1) >> 1;
+        if (r < k) {
+            s.dz = 0; s.dy = r + 1; s.xa = -k; s.xb = k;
+        } else if (r < n_rows - 1) {
+            const int q = r - k;
+            s.dz = 1 + q / (2 * k + 1); s.dy = q % (2 * k + 1) - k; s.xa = -k; s.xb = k;
+        } else {
+            s.dz = 0; s.dy = 0; s.xa = 1; s.xb = k;
+        }
+    }
+    return s;
+}
+
+// One unit (frame f, home cell h) of one wave.  It streams the half-stencil j
+// list through the wave's ring (POS: also each j's position in the sorted
+// frame) and hands every chunk of up to 64 j atoms to cons.chunk<MODE, HOME>;
+// cons.unit_begin(f, hs, n_home) / cons.unit_end(f, hs, n_home) bracket a
+// non-empty unit.  sl: this lane's stencil_lane(lane, G.k).
+template <int PBC, bool POS, class C>
+__device__ __forceinline__ void cell_unit(const CellGrid &G, int f, int h, const StencilLane &sl,
+                                          float4 *ring, unsigned *ringp, C &cons) {
+    const int lane = threadIdx.x & 63;
+    const float nanv = __builtin_nanf("");
+    const int ncx = G.ncx, ncy = G.ncy, ncz = G.ncz;
+    const int ncell = ncx * ncy * ncz;
+    const int n_pieces = stencil_pieces(G.k);
+    const float cull2 = G.cull2;
+    {
+        const int hx = h % ncx;
+        const int hy = (h / ncx) % ncy;
+        const int hz = h / (ncx * ncy);
+        const unsigned *fs = G.start + (long long)f * (ncell + 1);
+        const float4 *fp = G.sorted + (long long)f * G.n;
+        const int hs = (int)fs[h];
+        const int n_home = (int)fs[h + 1] - hs;
+        if (n_home == 0) return;
+        const float4 *hp = fp + hs;
+        const FrameBox b = G.boxes[f];
+        cons.unit_begin(f, hs, n_home);
+
+        // home atoms: bounding box and image uniformity (the box is only used
+        // on the shift path, whose frames hold finite coordinates)
+        const unsigned m0 = __float_as_uint(hp[0].w) & 0xFFFFFFu;
+        int kn0 = 0x7FFFFFFF, kn1 = 0x7FFFFFFF, kn2 = 0x7FFFFFFF;
+        int kx0 = (int)0x80000000, kx1 = (int)0x80000000, kx2 = (int)0x80000000;
+        bool diff = false;
+        for (int hb = 0; hb < n_home; hb += 64) {
+            if (hb + lane < n_home) {
+                const float4 q = hp[hb + lane];
+                const int k0 = f_key(q.x), k1 = f_key(q.y), k2 = f_key(q.z);
+                kn0 = min(kn0, k0); kx0 = max(kx0, k0);
+                kn1 = min(kn1, k1); kx1 = max(kx1, k1);
+                kn2 = min(kn2, k2); kx2 = max(kx2, k2);
+                diff |= (__float_as_uint(q.w) & 0xFFFFFFu) != m0;
+            }
+        }
+        const bool use_shift = G.shift_ok && G.frame_flag[f] == 0u && !__any(diff);
+        const float mn0 = wave_min_key(kn0), mn1 = wave_min_key(kn1), mn2 = wave_min_key(kn2);
+        const float mx0 = wave_max_key(kx0), mx1 = wave_max_key(kx1), mx2 = wave_max_key(kx2);
+        // per-piece byte constants K: n + 128 = (m_j + 32) + K per byte, with
+        // K = 128 - w - (m_home + 32) (no carries: every byte stays below 256)
+        const unsigned k_home = ((128u - (m0 & 0xFFu)) & 0xFFu) | (((128u - ((m0 >> 8) & 0xFFu)) & 0xFFu) << 8) |
+                                (((128u - ((m0 >> 16) & 0xFFu)) & 0xFFu) << 16);
+
+        // the stream's pieces: lane 0 the home cell, lanes 1..n_pieces the runs
+        int pbeg = hs, plen = n_home;
+        unsigned pk = k_home;
+        if (lane >= 1 && lane <= n_pieces) {
+            int cy = hy + sl.dy, cz = hz + sl.dz;
+            const int wy = cy < 0 ? -1 : (cy >= ncy ? 1 : 0);
+            const int wz = cz < 0 ? -1 : (cz >= ncz ? 1 : 0);
+            cy -= wy * ncy;
+            cz -= wz * ncz;
+            const int xa = hx + sl.xa, xb = hx + sl.xb;
+            int lo, hi, wx = 0;
+            if (lane & 1) {
+                lo = max(xa, 0); hi = min(xb, ncx - 1);
+            } else if (xa < 0) {
+                lo = xa + ncx; hi = ncx - 1; wx = -1;
+            } else if (xb >= ncx) {
+                lo = 0; hi = xb - ncx; wx = 1;
+            } else {
+                lo = 1; hi = 0;
+            }
+            pbeg = 0;
+            plen = 0;
+            if (lo <= hi) {
+                const int rowbase = (cz * ncy + cy) * ncx;
+                pbeg = (int)fs[rowbase + lo];
+                plen = (int)fs[rowbase + hi + 1] - pbeg;
+            }
+            // K per byte: 128 - w - (m_home + 32); w = +1 subtracts 1, w = -1 adds 1
+            pk = k_home + (unsigned)(-wx) + ((unsigned)(-wy) << 8) + ((unsigned)(-wz) << 16);
+        }
+        // stream order: home, runs without a box wrap, runs with one (so that
+        // chunks of unshifted atoms come first); permuted through the empty ring
+        {
+            const bool act = lane <= n_pieces;
+            const bool wrapped = pk != k_home;
+            const unsigned long long b0m = __ballot(act && !wrapped);
+            const unsigned long long b1m = __ballot(act && wrapped);
+            const int dst = wrapped ? (int)__popcll(b0m) + lanes_below(b1m) : lanes_below(b0m);
+            if (act) ring[dst] = make_float4(__int_as_float(pbeg), __int_as_float(plen), __uint_as_float(pk), 0.f);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            if (act) {
+                const float4 t = ring[lane];
+                pbeg = __float_as_int(t.x);
+                plen = __float_as_int(t.y);
+                pk = __float_as_uint(t.z);
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+
+        // Stream positions P0 .. P0 + 63 per step, across piece boundaries.
+        // Uniform cursor: piece c covers stream positions [c_off, c_end).
+        int head = 0, tail = 0;  // ring positions
+        int c = 0, c_beg = hs, c_off = 0, c_end = n_home;
+        unsigned c_k = k_home;
+        int P0 = 0;
+        while (true) {
+            while (c_end <= P0 && c < n_pieces) {
+                ++c;
+                c_beg = __builtin_amdgcn_readlane(pbeg, c);
+                c_off = c_end;
+                c_end += __builtin_amdgcn_readlane(plen, c);
+                c_k = __builtin_amdgcn_readlane(pk, c);
+            }
+            const bool more = c_end > P0;
+            if (more) {
+                const int pos = P0 + lane;
+                int src = c_beg + (pos - c_off);
+                unsigned kk = c_k;
+                int e = c, e_end = c_end;
+                while (e_end < P0 + 64 && e < n_pieces) {
+                    ++e;
+                    const int eb = __builtin_amdgcn_readlane(pbeg, e);
+                    const int el = __builtin_amdgcn_readlane(plen, e);
+                    const unsigned ek = __builtin_amdgcn_readlane(pk, e);
+                    if (pos >= e_end) {
+                        src = eb + (pos - e_end);
+                        kk = ek;
+                    }
+                    e_end += el;
+                }
+                const bool valid = pos < e_end;
+                float4 q = make_float4(nanv, nanv, nanv, 0.f);
+                if (valid) q = fp[src];
+                unsigned meta = __float_as_uint(q.w);
+                bool keep = valid;
+                if (!use_shift) {
+                    meta = (meta & 0xFF000000u) | 0x808080u;  // multiplicities only
+                } else {
+                    // n + 128 per byte; home atoms get n = 0 and lie inside the box
+                    const unsigned nb = (meta & 0xFFFFFFu) + kk;
+                    meta = (meta & 0xFF000000u) | nb;
+                    const float fx = (float)(nb & 0xFFu) - 128.f;
+                    const float fy = (float)((nb >> 8) & 0xFFu) - 128.f;
+                    const float fz = (float)((nb >> 16) & 0xFFu) - 128.f;
+                    float px, py, pz;  // image of j next to the home atoms (approximate)
+                    if constexpr (PBC == WMD_PBC_ORTHO) {
+                        px = __builtin_fmaf(-fx, b.lx, q.x);
+                        py = __builtin_fmaf(-fy, b.ly, q.y);
+                        pz = __builtin_fmaf(-fz, b.lz, q.z);
+                    } else {
+                        px = __builtin_fmaf(-fx, b.c[0], __builtin_fmaf(-fy, b.c[3], __builtin_fmaf(-fz, b.c[6], q.x)));
+                        py = __builtin_fmaf(-fx, b.c[1], __builtin_fmaf(-fy, b.c[4], __builtin_fmaf(-fz, b.c[7], q.y)));
+                        pz = __builtin_fmaf(-fx, b.c[2], __builtin_fmaf(-fy, b.c[5], __builtin_fmaf(-fz, b.c[8], q.z)));
+                    }
+                    const float ex = fmaxf(fmaxf(mn0 - px, px - mx0), 0.f);
+                    const float ey = fmaxf(fmaxf(mn1 - py, py - mx1), 0.f);
+                    const float ez = fmaxf(fmaxf(mn2 - pz, pz - mx2), 0.f);
+                    keep = valid && ex * ex + ey * ey + ez * ez <= cull2;
+                }
+                const unsigned long long bal = __ballot(keep);
+                if (keep) {
+                    const int slot = (tail + lanes_below(bal)) & (kRing - 1);
+                    ring[slot] = make_float4(q.x, q.y, q.z, __uint_as_float(meta));
+                    if constexpr (POS) ringp[slot] = (unsigned)src;
+                }
+                tail += (int)__popcll(bal);
+                P0 += 64;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            while (tail - head >= 64 || (!more && tail > head)) {
+                const int cnt = min(64, tail - head);
+                const int slot = (head + lane) & (kRing - 1);
+                const float4 ej = ring[slot];
+                const bool v = lane < cnt;
+                const float xj = v ? ej.x : nanv;
+                const float yj = v ? ej.y : nanv;
+                const float zj = v ? ej.z : nanv;
+                const unsigned mj = v ? __float_as_uint(ej.w) : 0x808080u;
+                unsigned jpos = 0;
+                if constexpr (POS) jpos = v ? ringp[slot] : 0xFFFFFFFFu;
+                // 0: no lane has an image shift; 1: some lane has one; 2: rint wrap
+                const int mode = !use_shift ? 2 : (__any((mj & 0xFFFFFFu) != 0x808080u) ? 1 : 0);
+                const bool home = head < n_home;
+                const int rank = home && head + lane < n_home ? head + lane : 0x7FFFFFFF;
+                cell_dispatch(cons, mode, home, hp, n_home, xj, yj, zj, mj, rank, jpos, f, b);
+                head += cnt;
+            }
+            if (!more) break;
+        }
+        cons.unit_end(f, hs, n_home);
+    }
+}
+
+// The unit loop of one wave over a batch of frames: XCD-aware static schedule
+// (workgroup g runs on XCD g % n_xcd and takes a contiguous slice of the
+// units; the slice's units are dealt to its waves round robin).
+template <int PBC, bool POS, class C>
+__device__ __forceinline__ void cell_waves(const CellGrid &G, float4 *ring, unsigned *ringp,
+                                           int wave, C &cons) {
+    const StencilLane sl = stencil_lane(threadIdx.x & 63, G.k);
+    const int ncell = G.ncx * G.ncy * G.ncz;
+    const int n_xcd = G.n_xcd;
+    const long long n_units = (long long)G.n_frames * ncell;
+    const int xcd = blockIdx.x % n_xcd;
+    const int waves_per_wg = blockDim.x >> 6;
+    const long long wave_local = (long long)(blockIdx.x / n_xcd) * waves_per_wg + wave;
+    const long long waves_per_xcd = (long long)(gridDim.x / n_xcd) * waves_per_wg;
+    const long long u_hi = n_units * (xcd + 1) / n_xcd;
+    for (long long u = n_units * xcd / n_xcd + wave_local; u < u_hi; u += waves_per_xcd) {
+        const int f = (int)(u / ncell);
+        const int h = (int)(u - (long long)f * ncell);
+        cell_unit<PBC, POS>(G, f, h, sl, ring, ringp, cons);
+    }
+}
+
+// Plain histogram consumer: exact bins, u64 LDS counts.
+template <int PBC, bool LT, bool WEIGHTED, bool EDGE, bool TABLE>
+struct PlainCons {
+    HistParams p;
+    CellLds L;
+    unsigned long long edge_local;
+    template <int MODE, bool HOME>
+    __device__ __forceinline__ void chunk(const float4 *hp, int n_home, float xj, float yj, float zj,
+                                          unsigned mj, int rank, unsigned, int, const FrameBox &b) {
+        cell_chunk<PBC, LT, WEIGHTED, EDGE, TABLE, MODE, HOME>(hp, n_home, xj, yj, zj, mj, rank, b, p,
+                                                                L, edge_local);
+    }
+    __device__ __forceinline__ void unit_begin(int, int, int) {}
+    __device__ __forceinline__ void unit_end(int, int, int) {}
+};
+
+template <int PBC, bool LT, bool WEIGHTED, bool EDGE, bool TABLE>
+__global__ __launch_bounds__(kCellBlock) void rdf_cell_kernel(CellGrid G, HistParams p) {
+    extern __shared__ __attribute__((aligned(16))) char lds[];
+    const CellLds L = cell_lds_layout(lds, p, EDGE, TABLE);
+    for (int i = threadIdx.x; i < p.bins; i += kCellBlock) L.hist[i] = 0ull;
+    for (int i = threadIdx.x; i <= p.bins; i += kCellBlock) {
+        L.thr[i] = p.thr[i];
+        if constexpr (EDGE) {
+            L.elo[i] = p.elo[i];
+            L.ehi[i] = p.ehi[i];
+        }
+    }
+    if constexpr (TABLE)
+        for (int i = threadIdx.x; i < p.tab_n; i += kCellBlock) L.tab2[i] = p.tab2[i];
+    __syncthreads();
+
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform: unit indices in SGPRs
+    const int lane = threadIdx.x & 63;
+    PlainCons<PBC, LT, WEIGHTED, EDGE, TABLE> cons{p, L, 0ull};
+    cell_waves<PBC, false>(G, L.ring + wave * kRing, nullptr, wave, cons);
+
+    __syncthreads();
+    for (int i = threadIdx.x; i < p.bins; i += kCellBlock) {
+        const unsigned long long v = L.hist[i];
+        if (v) atomicAdd(&p.counts[i], v);
+    }
+    if constexpr (EDGE) {
+        unsigned long long e = cons.edge_local;
+        for (int off = 32; off > 0; off >>= 1) e += __shfl_xor(e, off, 64);
+        if (lane == 0 && e) atomicAdd(p.edge, e);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host-side exact thresholds.
+// ---------------------------------------------------------------------------
+static inline float bits_f(uint32_t u) {
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+
+static inline uint32_t f_bits(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    return u;
+}
+
+// Reference key of a squared distance: its bin if counted, else bins.
+// IEEE single sqrt and division (x86 SSE), each rounding once (rdf.rs:929-935).
+static int ref_key(float s, float r_max, float w, int bins) {
+    volatile float sv = s;
+    volatile float r = std::sqrt((float)sv);
+    if (!(r < r_max)) return bins;
+    volatile float q = r / w;
+    double qd = (double)q;
+    if (qd >= (double)bins) return bins;
+    return (int)qd;
+}
+
+// min{ f32 s >= 0 : pred(s) } by bisection over bit patterns (pred monotone).
+template <class P> static float min_s(P pred) {
+    uint32_t lo = 0, hi = 0x7F800000u;  // pred(+inf) assumed true
+    if (pred(bits_f(lo))) return 0.0f;
+    while (hi - lo > 1) {
+        uint32_t mid = lo + (hi - lo) / 2;
+        if (pred(bits_f(mid))) hi = mid; else lo = mid;
+    }
+    return bits_f(hi);
+}
+
+// Scratch of the cell build.
+struct CellScratch {
+    DevBuf cell_of, cell_slot, image, count, start, flag, sorted, order;
+    size_t frames_for(int n, int ncell, size_t n_frames, bool want_order) const {
+        const size_t per_frame = (size_t)n * (16 + 12 + (want_order ? 4 : 0)) + (size_t)(2 * ncell + 2) * 4;
+        return std::max<size_t>(1, std::min<size_t>(n_frames, (size_t)(768ull << 20) / per_frame));
+    }
+    void reserve(int n, int ncell, size_t batch, bool want_order) {
+        cell_of.reserve((size_t)n * batch * 4);
+        cell_slot.reserve((size_t)n * batch * 4);
+        image.reserve((size_t)n * batch * 4);
+        count.reserve((size_t)ncell * batch * 4);
+        start.reserve((size_t)(ncell + 1) * batch * 4);
+        flag.reserve(batch * 4);
+        sorted.reserve((size_t)n * batch * 16);
+        if (want_order) order.reserve((size_t)n * batch * 4);
+    }
+};
+
+void grid1d(long long total, int *grid) {
+    long long g = (total + kBlock - 1) / kBlock;
+    if (g > 65536) g = 65536;
+    if (g < 1) g = 1;
+    *grid = (int)g;
+}
+
+// Per-frame box parameters of a batch (throws on an invalid box).
+struct BoxSet {
+    std::vector<FrameBox> fb;
+    double min_width[3] = {1e300, 1e300, 1e300};
+    bool lower_tri = true;
+};
+
+BoxSet make_boxes(int pbc, const float *boxes_host, size_t n_frames) {
+    BoxSet B;
+    B.fb.resize(n_frames);
+    for (size_t f = 0; f < n_frames; ++f) {
+        BoxInfo bi;
+        if (pbc != WMD_PBC_NONE) {
+            const float *m = boxes_host + 9 * f;
+            if (!make_box(pbc, m, &bi))
+                fail(WMD_ERR_MISMATCH, pbc == WMD_PBC_ORTHO ? "RDF requires a positive orthorhombic box"
+                                                            : "box matrix not invertible");
+            for (int d = 0; d < 3; ++d) B.min_width[d] = std::min(B.min_width[d], bi.width[d]);
+            if (m[1] != 0.0f || m[2] != 0.0f || m[5] != 0.0f) B.lower_tri = false;
+        } else {
+            std::memset(&bi, 0, sizeof(bi));
+        }
+        FrameBox &b = B.fb[f];
+        b.lx = bi.lx; b.ly = bi.ly; b.lz = bi.lz;
+        b.ilx = bi.lx > 0 ? 1.0f / bi.lx : 0.f;
+        b.ily = bi.ly > 0 ? 1.0f / bi.ly : 0.f;
+        b.ilz = bi.lz > 0 ? 1.0f / bi.lz : 0.f;
+        if (pbc == WMD_PBC_ORTHO) {
+            for (int i = 0; i < 9; ++i) b.r[i] = 0.f, b.c[i] = 0.f;
+            b.r[0] = b.ilx; b.r[4] = b.ily; b.r[8] = b.ilz;
+            b.c[0] = b.lx; b.c[4] = b.ly; b.c[8] = b.lz;
+        } else {
+            for (int i = 0; i < 9; ++i) b.r[i] = bi.r[i], b.c[i] = bi.c[i];
+        }
+    }
+    return B;
+}
+
+// Copies the batch's boxes to the device through a pinned staging buffer.
+FrameBox *upload_boxes(const BoxSet &B, HostBuf &hbuf, DevBuf &dbuf, hipStream_t st) {
+    const size_t bytes = sizeof(FrameBox) * std::max<size_t>(B.fb.size(), 1);
+    void *hb = hbuf.reserve(bytes);
+    WMD_HIP_CHECK(hipStreamSynchronize(st));  // the staging buffer may still be in use
+    std::memcpy(hb, B.fb.data(), sizeof(FrameBox) * B.fb.size());
+    FrameBox *d = (FrameBox *)dbuf.reserve(bytes);
+    WMD_HIP_CHECK(hipMemcpyAsync(d, hb, sizeof(FrameBox) * B.fb.size(), hipMemcpyHostToDevice, st));
+    return d;
+}
+
+// Cell grid for pair reach `reach`: cells at least (reach + 1e-2) / k wide per
+// plane spacing, >= 2k + 1 per side, several atoms per stencil.  Returns k
+// (0 = use brute force); forced_k in 1..3 restricts the choice.
+int choose_cells(double reach, const double *min_width, int n, int forced_k, int *nc) {
+    int ks[3] = {2, 1, 3};
+    if (forced_k >= 1 && forced_k <= 3) { ks[0] = forced_k; ks[1] = 0; ks[2] = 0; }
+    for (int kk : ks) {
+        if (kk == 0) continue;
+        const double cw = (reach + 1e-2) / kk;
+        int c[3];
+        bool ok = true;
+        for (int d = 0; d < 3; ++d) {
+            c[d] = (int)std::min(std::floor(min_width[d] / cw), 96.0);
+            if (c[d] < 2 * kk + 1) ok = false;
+        }
+        if (!ok) continue;
+        if ((double)n / ((double)c[0] * c[1] * c[2]) * std::pow(2 * kk + 1, 3) < 48.0) return 0;
+        nc[0] = c[0]; nc[1] = c[1]; nc[2] = c[2];
+        return kk;
+    }
+    return 0;
+}
+
+// Squared cull radius: reach plus a margin covering the f32 rounding of image
+// positions and pair differences for coordinates within (kImageLimit + 1) box
+// lengths (frames beyond that take the rint path, which does not cull).
+float cull2_for(double reach, const BoxSet &B) {
+    double coord_bound = 0.0;
+    for (const FrameBox &b : B.fb) {
+        double row = 0.0;
+        for (int i = 0; i < 9; ++i) row += std::fabs((double)b.c[i]);
+        coord_bound = std::max(coord_bound, row);
+    }
+    coord_bound *= (double)(kImageLimit + 1);
+    const double margin = 0.05 + 32.0 * coord_bound * std::ldexp(1.0, -24);
+    const float cull = (float)(reach + margin);
+    return cull * cull;
+}
+
+// Sorts frames [f0, f0 + nf) of an SoA store into cells (bx: their boxes).
+void build_cells(wmd_ctx *ctx, hipStream_t st, int pbc, const float *d_soa, int stride, int n,
+                 int nf, long long f0, const FrameBox *bx, const int *nc, const unsigned *d_w,
+                 CellScratch &S, bool want_order) {
+    ProfScope build_prof(ctx, st, "cell_build");
+    const int ncell = nc[0] * nc[1] * nc[2];
+    const long long fs = 3 * (long long)stride;
+    unsigned *d_count = S.count.as<unsigned>(), *d_fflag = S.flag.as<unsigned>();
+    int g;
+    grid1d((long long)ncell * nf, &g);
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(g), dim3(kBlock), 0, st, d_count, (long long)ncell * nf, 0u);
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(kBlock), 0, st, d_fflag, (long long)nf, 0u);
+    WMD_HIP_CHECK(hipGetLastError());
+    grid1d((long long)n * nf, &g);
+    if (pbc == WMD_PBC_ORTHO)
+        hipLaunchKernelGGL(cell_assign_kernel<WMD_PBC_ORTHO>, dim3(g), dim3(kBlock), 0, st, d_soa, fs, stride, n, nf, f0, bx, nc[0], nc[1], nc[2], S.cell_of.as<unsigned>(), S.cell_slot.as<unsigned>(), d_count, S.image.as<unsigned>(), d_fflag);
+    else
+        hipLaunchKernelGGL(cell_assign_kernel<WMD_PBC_TRICLINIC>, dim3(g), dim3(kBlock), 0, st, d_soa, fs, stride, n, nf, f0, bx, nc[0], nc[1], nc[2], S.cell_of.as<unsigned>(), S.cell_slot.as<unsigned>(), d_count, S.image.as<unsigned>(), d_fflag);
+    WMD_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(cell_scan_kernel, dim3(nf), dim3(kBlock), 0, st, d_count, S.start.as<unsigned>(), ncell);
+    WMD_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(cell_scatter_kernel, dim3(g), dim3(kBlock), 0, st, d_soa, fs, stride, n, nf, f0,
+                       (const unsigned *)S.cell_of.ptr, (const unsigned *)S.cell_slot.ptr,
+                       (const unsigned *)S.image.ptr, (const unsigned *)S.start.ptr, ncell, d_w,
+                       S.sorted.as<float4>(), want_order ? S.order.as<unsigned>() : (unsigned *)nullptr);
+    WMD_HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace
