@@ -221,6 +221,66 @@ wmd_status wmd_softrdf_last_path(const wmd_softrdf *h, int *path, int *nc);
 wmd_status wmd_softrdf_set_tuning(wmd_softrdf *h, int cell_k);
 void wmd_softrdf_destroy(wmd_softrdf *h);
 
+/* ---- uneven batches ---------------------------------------------------------
+ * Many independent trajectories (uneven atom counts, own boxes and frame
+ * counts) through one RDF pair launch and one RMSD launch, replayed as a
+ * captured HIP graph.  No single counterpart in the reference, which runs one
+ * trajectory per Plan.run() with a fresh context (crates/traj-py/src/
+ * pack.rs:1320-1323) and tunes only its chunk size (pack.rs:1366-1405); per
+ * trajectory the results equal RdfPlan's counts over all atoms of the
+ * trajectory (rdf.rs:882-941; bit-exact) and RmsdPlan(all atoms,
+ * reference="frame0", align=True) (rmsd/plan.rs:336-392). */
+#define WMD_BATCH_RDF 0x1u      /* histogram every trajectory (all atoms, A = B) */
+#define WMD_BATCH_RMSD 0x2u     /* RMSD of every frame to its trajectory's frame 0 */
+#define WMD_BATCH_NO_GRAPH 0x4u /* launch eagerly instead of replaying a HIP graph */
+
+typedef struct wmd_batch wmd_batch;
+
+/* n_atoms, n_frames: [n_traj].  bins <= 1024, r_max > 0, pbc WMD_PBC_*. */
+wmd_status wmd_batch_create(wmd_ctx *ctx, size_t n_traj, const uint32_t *n_atoms,
+                            const uint32_t *n_frames, size_t bins, float r_max, int pbc,
+                            uint32_t flags, wmd_batch **out);
+/* Store layout: trajectory t's frames are SoA planes [n_frames[t]][3][strides[t]]
+ * (float) starting at float offset offsets[t] of one device buffer of
+ * *total_floats floats; padding slots (atom >= n_atoms[t]) must hold 0.
+ * offsets / strides: [n_traj] or NULL. */
+wmd_status wmd_batch_store_layout(const wmd_batch *b, uint64_t *offsets, uint32_t *strides,
+                                  uint64_t *total_floats);
+/* Boxes of every frame, [sum n_frames][9] row-major (trajectory order; NULL for
+ * WMD_PBC_NONE).  Plans the enumeration (cell list or brute force per
+ * trajectory, units, XCD slices); must precede the first run. */
+wmd_status wmd_batch_set_boxes(wmd_batch *b, const float *boxes);
+/* Per-trajectory enumeration override: cell_k[t] = 0 automatic, -1 brute
+ * force, 1..3 cells per r_max (NULL = all automatic).  Re-plans. */
+wmd_status wmd_batch_set_tuning(wmd_batch *b, const int *cell_k);
+/* Project autotuner: for every class of trajectories sharing (device arch,
+ * pbc, bins, r_max, atom-count bucket, density bucket) times brute force and
+ * cells per r_max k = 1, 2, 3 on the class's median trajectory (reps timed
+ * runs each, after a warm-up) and keeps the fastest; choices are cached per
+ * process (see wmd_autotune_export / import) and applied to the batch. */
+wmd_status wmd_batch_autotune(wmd_batch *b, const float *d_store, int reps, void *stream);
+/* One step over the device store: adds every trajectory's counts to the
+ * batch's u64 histograms [n_traj][bins] (WMD_BATCH_RDF) and writes d_rmsd
+ * [sum n_frames] f32 (and, if not NULL, d_rmsd64 f64 and d_err f64 error
+ * bounds) (WMD_BATCH_RMSD).  Without WMD_BATCH_NO_GRAPH the step is captured
+ * into a HIP graph on first use (and again when a pointer changes) and
+ * replayed.  Asynchronous on `stream` (NULL = the null stream). */
+wmd_status wmd_batch_run_device(wmd_batch *b, const float *d_store, float *d_rmsd,
+                                double *d_rmsd64, double *d_err, void *stream);
+/* Copies the histograms out, [n_traj][bins] u64 (synchronises). */
+wmd_status wmd_batch_read_counts(wmd_batch *b, uint64_t *counts);
+/* Zeroes the histograms. */
+wmd_status wmd_batch_reset(wmd_batch *b);
+/* The plan: cell_k[t] (0 = brute force) and the number of pair units. */
+wmd_status wmd_batch_plan_info(const wmd_batch *b, int *cell_k, long long *n_units);
+void wmd_batch_destroy(wmd_batch *b);
+/* The autotuner's process-wide cache as text, one choice per line
+ * ("arch pbc bins r_max_bits n_bucket density_bucket k"); *len gets the full
+ * length, buf (cap bytes, NUL-terminated) may be NULL. */
+wmd_status wmd_autotune_export(char *buf, size_t cap, size_t *len);
+/* Adds lines in the export format to the cache. */
+wmd_status wmd_autotune_import(const char *text);
+
 /* QCP superposition of one set of centred moments, evaluated on the host with
  * the same code the device runs (for testing without a GPU).
  * m: [9] row-major M = sum (x - cx)(y - cy)^T, gx = sum |x - cx|^2,

@@ -1,0 +1,916 @@
+// Uneven batches: many independent trajectories (different atom counts, boxes
+// and frame counts) through one RDF pair launch and one RMSD launch, replayed
+// as a captured HIP graph.  The reference processes one trajectory per run()
+// with a fresh context (crates/traj-py/src/pack.rs:1320-1323) and tunes only
+// its chunk size (pack.rs:1366-1405); per trajectory the results here are the
+// RdfPlan counts (rdf.rs:882-941, bit-exact through the thresholds of
+// wmd_rdf_core.hpp) and the RmsdPlan(frame0, align=true) series
+// (rmsd/plan.rs:336-392).
+//
+// Layout.  The batch store holds trajectory t's frames as SoA planes
+// [F_t][3][stride_t] (stride_t = N_t rounded up to 4, padding zero) starting at
+// float offset off_t; boxes are [sum F_t][9] row-major.
+//
+// Work.  Every trajectory is enumerated either by the cell list of the single
+// plan (frame x home cell units, shift / rint image modes) or, when its box is
+// too small for a stencil of at least 2k+1 cells per side (r_max near or above
+// half the box), by brute force: units (frame, block of 64 atoms i) pair the
+// block with every atom j > i under the reference's roundf(dx / L) wrap
+// (pair_s_checked).  A cell-sorted AoS copy of every frame (store order for
+// brute-force trajectories) is built by batched assign / scan / scatter
+// kernels over all trajectories at once.
+//
+// The pair kernel is persistent with a work queue: the unit range is split
+// into 8 slices of equal estimated cost (slice q is served by workgroups
+// g % 8 == q, i.e. one XCD under round-robin dispatch), and each wave takes
+// kGrab consecutive units at a time from its slice's atomic counter until the
+// counter passes the slice's end (every wave reaches that exit).  Each wave
+// keeps a private u64 LDS histogram for the trajectory it is on and merges it
+// into that trajectory's global counts when it moves to another trajectory
+// and at the end.
+//
+// The whole step (clears by fill kernels, cell build, pair kernel, RMSD
+// moments and finalize) is captured once into a HIP graph and replayed;
+// eager launches give the same results.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <sstream>
+#include <tuple>
+
+#include "wmd_rdf_core.hpp"
+
+using namespace wmd;
+
+namespace {
+
+constexpr int kBatchBlock = 512;  // 8 waves
+constexpr int kBatchWaves = kBatchBlock / 64;
+constexpr int kBruteHome = 64;    // i atoms per brute-force unit
+constexpr int kGrab = 4;          // units per queue grab
+constexpr int kSlices = 8;        // queue slices (one per XCD)
+
+enum { KIND_CELLS = 0, KIND_BRUTE = 1 };
+
+struct BatchTraj {
+    long long store_off;  // float offset of frame 0 in the batch store
+    long long af0;        // first atom-frame; also the float4 offset of its sorted atoms
+    long long start_off;  // u32 offset of frame 0's cell starts [F][ncell + 1] (cells)
+    long long count_off;  // u32 offset of frame 0's cell counts [F][ncell] (cells)
+    long long unit0;      // first pair unit
+    long long scan0;      // first scan workgroup (cells)
+    int stride, n, n_frames, box0;  // box0: index of frame 0's box and frame flag
+    int kind, k, ncx, ncy, ncz, shift_ok, upf;  // upf: units per frame
+    float cull2;
+};
+
+template <class K>
+__device__ __forceinline__ int find_last_le(int n, long long x, K key) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (key(mid) <= x) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// ---- batched cell build ---------------------------------------------------
+template <int PBC>
+__global__ __launch_bounds__(kBlock) void batch_assign_kernel(
+    const BatchTraj *__restrict__ traj, int n_traj, long long total_af,
+    const float *__restrict__ store, const FrameBox *__restrict__ boxes,
+    unsigned *__restrict__ cell_of, unsigned *__restrict__ cell_slot,
+    unsigned *__restrict__ cell_count, unsigned *__restrict__ image_of,
+    unsigned *__restrict__ frame_flag) {
+    for (long long g = (long long)blockIdx.x * kBlock + threadIdx.x; g < total_af;
+         g += (long long)gridDim.x * kBlock) {
+        const int t = find_last_le(n_traj, g, [&](int i) { return traj[i].af0; });
+        const BatchTraj &T = traj[t];
+        if (T.kind != KIND_CELLS) continue;
+        const long long local = g - T.af0;
+        const int f = (int)(local / T.n);
+        const int i = (int)(local - (long long)f * T.n);
+        const float *X = store + T.store_off + (long long)f * 3 * T.stride;
+        unsigned img;
+        bool far_out;
+        const int cid = assign_atom<PBC>(X[i], X[T.stride + i], X[2 * T.stride + i], boxes[T.box0 + f],
+                                         T.ncx, T.ncy, T.ncz, img, far_out);
+        if (far_out) atomicOr(&frame_flag[T.box0 + f], 1u);
+        const int ncell = T.ncx * T.ncy * T.ncz;
+        cell_slot[g] = atomicAdd(&cell_count[T.count_off + (long long)f * ncell + cid], 1u);
+        cell_of[g] = (unsigned)cid;
+        image_of[g] = img;
+    }
+}
+
+// One workgroup per (cell-list trajectory, frame).
+__global__ __launch_bounds__(kBlock) void batch_scan_kernel(const BatchTraj *__restrict__ traj,
+                                                            int n_traj,
+                                                            const unsigned *__restrict__ count,
+                                                            unsigned *__restrict__ start) {
+    const long long s = blockIdx.x;
+    const int t = find_last_le(n_traj, s, [&](int i) { return traj[i].scan0; });
+    const BatchTraj &T = traj[t];
+    const int f = (int)(s - T.scan0);
+    const int ncell = T.ncx * T.ncy * T.ncz;
+    scan_cells(count + T.count_off + (long long)f * ncell,
+               start + T.start_off + (long long)f * (ncell + 1), ncell);
+}
+
+// sorted[af0 + f n + pos] = (x, y, z, meta); pos = cell position (cells) or
+// the atom index (brute force).  Unit selection multiplicities (mA = mB = 1).
+__global__ __launch_bounds__(kBlock) void batch_scatter_kernel(
+    const BatchTraj *__restrict__ traj, int n_traj, long long total_af,
+    const float *__restrict__ store, const unsigned *__restrict__ cell_of,
+    const unsigned *__restrict__ cell_slot, const unsigned *__restrict__ image_of,
+    const unsigned *__restrict__ start, float4 *__restrict__ sorted) {
+    for (long long g = (long long)blockIdx.x * kBlock + threadIdx.x; g < total_af;
+         g += (long long)gridDim.x * kBlock) {
+        const int t = find_last_le(n_traj, g, [&](int i) { return traj[i].af0; });
+        const BatchTraj &T = traj[t];
+        const long long local = g - T.af0;
+        const int f = (int)(local / T.n);
+        const int i = (int)(local - (long long)f * T.n);
+        const float *X = store + T.store_off + (long long)f * 3 * T.stride;
+        unsigned pos, meta;
+        if (T.kind == KIND_CELLS) {
+            const int ncell = T.ncx * T.ncy * T.ncz;
+            pos = start[T.start_off + (long long)f * (ncell + 1) + cell_of[g]] + cell_slot[g];
+            meta = (image_of[g] & 0xFFFFFFu) | 0x11000000u;
+        } else {
+            pos = (unsigned)i;
+            meta = 0x11202020u;  // image bytes m + 32 with m = 0 (unused by brute force)
+        }
+        sorted[T.af0 + (long long)f * T.n + pos] =
+            make_float4(X[i], X[T.stride + i], X[2 * T.stride + i], __uint_as_float(meta));
+    }
+}
+
+// ---- persistent pair kernel -------------------------------------------------
+struct BatchPairArgs {
+    const BatchTraj *traj;
+    int n_traj;
+    const float4 *sorted;
+    const unsigned *start;
+    const unsigned *frame_flag;
+    const FrameBox *boxes;
+    unsigned long long *counts;  // [n_traj][bins]
+    unsigned long long *queue;   // [kSlices] grab counters, zero at launch
+    long long slice_lo[kSlices + 1];
+    HistParams p;
+};
+
+__host__ __device__ inline size_t batch_lds_bytes(int bins, int tab_n) {
+    return align16((size_t)kBatchWaves * bins * 8) + align16((size_t)tab_n * 8) +
+           align16(((size_t)bins + 1) * 4) + (size_t)kBatchWaves * kRing * 16;
+}
+
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Adds a wave's LDS histogram to out[bins] and clears it.
+__device__ __forceinline__ void wave_hist_flush(unsigned long long *hist,
+                                                unsigned long long *out, int bins, int lane) {
+    wave_lds_sync();
+    for (int i = lane; i < bins; i += 64) {
+        const unsigned long long v = hist[i];
+        if (v) {
+            atomicAdd(out + i, v);
+            hist[i] = 0ull;
+        }
+    }
+    wave_lds_sync();
+}
+
+// Brute-force unit: home block [I0, I0 + 64) of frame f against every j > i.
+template <class C>
+__device__ __forceinline__ void brute_unit(const CellGrid &G, int f, int blk, C &cons) {
+    const int lane = threadIdx.x & 63;
+    const float nanv = __builtin_nanf("");
+    const float4 *fp = G.sorted + (long long)f * G.n;
+    const int I0 = blk * kBruteHome;
+    const int n_home = min(kBruteHome, G.n - I0);
+    const float4 *hp = fp + I0;
+    const FrameBox b = G.boxes[f];
+    for (int j0 = I0; j0 < G.n; j0 += 64) {
+        const int j = j0 + lane;
+        float4 q = make_float4(nanv, nanv, nanv, __uint_as_float(0x11202020u));
+        if (j < G.n) q = fp[j];
+        const unsigned mj = __float_as_uint(q.w);
+        if (j0 == I0)  // the home block itself: pairs (i, j) with i < j
+            cons.template chunk<3, true>(hp, n_home, q.x, q.y, q.z, mj, lane, 0u, f, b);
+        else
+            cons.template chunk<3, false>(hp, n_home, q.x, q.y, q.z, mj, 0x7FFFFFFF, 0u, f, b);
+    }
+}
+
+template <int PBC, bool TABLE>
+__global__ __launch_bounds__(kBatchBlock) void batch_pair_kernel(BatchPairArgs a) {
+    extern __shared__ __attribute__((aligned(16))) char lds[];
+    const HistParams &p = a.p;
+    const int bins = p.bins;
+    unsigned long long *hist_all = (unsigned long long *)lds;
+    size_t off = align16((size_t)kBatchWaves * bins * 8);
+    uint2 *tab2 = (uint2 *)(lds + off);
+    off += align16((size_t)(TABLE ? p.tab_n : 0) * 8);
+    float *thr = (float *)(lds + off);
+    off += align16(((size_t)bins + 1) * 4);
+    float4 *rings = (float4 *)(lds + off);
+    for (int i = threadIdx.x; i < kBatchWaves * bins; i += kBatchBlock) hist_all[i] = 0ull;
+    for (int i = threadIdx.x; i <= bins; i += kBatchBlock) thr[i] = p.thr[i];
+    if constexpr (TABLE)
+        for (int i = threadIdx.x; i < p.tab_n; i += kBatchBlock) tab2[i] = p.tab2[i];
+    __syncthreads();
+
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    CellLds L;
+    L.hist = hist_all + wave * bins;
+    L.tab2 = tab2;
+    L.thr = thr;
+    L.elo = thr;
+    L.ehi = thr;
+    L.ring = rings + wave * kRing;
+    PlainCons<PBC, false, false, false, TABLE> cons{p, L, 0ull};
+
+    const int q = blockIdx.x % kSlices;
+    const long long lo = a.slice_lo[q], hi = a.slice_lo[q + 1];
+    int cur = -1;
+    long long cur_end = 0;
+    BatchTraj T{};
+    CellGrid G{};
+    StencilLane sl{0, 0, 0, -1};
+    while (true) {
+        unsigned long long g0 = 0;
+        if (lane == 0) g0 = atomicAdd(&a.queue[q], (unsigned long long)kGrab);
+        const unsigned g_lo = __builtin_amdgcn_readfirstlane((unsigned)g0);
+        const unsigned g_hi = __builtin_amdgcn_readfirstlane((unsigned)(g0 >> 32));
+        const long long u0 = lo + (long long)(((unsigned long long)g_hi << 32) | g_lo);
+        if (u0 >= hi) break;
+        const long long u1 = min(u0 + (long long)kGrab, hi);
+        for (long long u = u0; u < u1; ++u) {
+            if (u >= cur_end) {
+                if (cur >= 0) wave_hist_flush(L.hist, a.counts + (long long)cur * bins, bins, lane);
+                cur = __builtin_amdgcn_readfirstlane(
+                    find_last_le(a.n_traj, u, [&](int i) { return a.traj[i].unit0; }));
+                T = a.traj[cur];
+                cur_end = T.unit0 + (long long)T.n_frames * T.upf;
+                G = CellGrid{a.sorted + T.af0, a.start + T.start_off, a.frame_flag + T.box0,
+                             a.boxes + T.box0, T.n, T.n_frames, T.ncx, T.ncy, T.ncz, T.k,
+                             T.shift_ok, kSlices, T.cull2};
+                sl = stencil_lane(lane, T.kind == KIND_CELLS ? T.k : 1);
+            }
+            const long long local = u - T.unit0;
+            const int f = (int)(local / T.upf);
+            const int h = (int)(local - (long long)f * T.upf);
+            if constexpr (PBC != WMD_PBC_NONE) {
+                if (T.kind == KIND_CELLS) {
+                    cell_unit<PBC, false>(G, f, h, sl, L.ring, nullptr, cons);
+                    continue;
+                }
+            }
+            brute_unit(G, f, h, cons);
+        }
+    }
+    if (cur >= 0) wave_hist_flush(L.hist, a.counts + (long long)cur * bins, bins, lane);
+}
+
+// ---- autotuner cache ---------------------------------------------------------
+// key: (arch, pbc, bins, r_max bits, atom-count bucket, density bucket) -> k
+// (-1 brute force, 1..3 cells per r_max).  Buckets: quarter steps of log2.
+using TuneKey = std::tuple<std::string, int, int, uint32_t, int, int>;
+std::mutex g_tune_mu;
+std::map<TuneKey, int> g_tune;
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Batch handle
+// ---------------------------------------------------------------------------
+struct wmd_batch {
+    wmd_ctx *ctx = nullptr;
+    int n_traj = 0;
+    std::vector<uint32_t> n_atoms, n_frames;
+    std::vector<long long> store_off;
+    std::vector<uint32_t> stride;
+    long long store_floats = 0, total_frames = 0;
+    int bins = 0;
+    float r_max = 0.f;
+    int pbc = 0;
+    uint32_t flags = 0;
+    BinTables tables;
+    bool table = false;
+    std::vector<int> tune_k;   // per trajectory: 0 automatic, -1 brute, 1..3
+    bool use_graph = true;
+    // plan (set_boxes)
+    bool planned = false;
+    std::vector<float> boxes_host;
+    std::vector<BatchTraj> traj;
+    std::vector<BatchRmsdTraj> rtraj;
+    long long total_af = 0, n_units = 0, scan_blocks = 0, cell_counts = 0, cell_starts = 0;
+    long long rmsd_units = 0;
+    long long slice_lo[kSlices + 1] = {};
+    bool any_cells = false;
+    // device
+    DevBuf d_counts, d_thr, d_tab2, d_traj, d_rtraj, d_boxes, d_flag, d_count, d_start, d_cell_of,
+        d_slot, d_image, d_sorted, d_queue, d_partials;
+    HostBuf h_boxes;
+    // graph
+    hipStream_t cap = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    const void *g_store = nullptr;
+    const void *g_out32 = nullptr, *g_out64 = nullptr, *g_err = nullptr;
+    ~wmd_batch() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (cap) (void)hipStreamDestroy(cap);
+    }
+};
+
+namespace {
+
+void drop_graph(wmd_batch *b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    b->g_store = nullptr;
+}
+
+int grid_1d(long long total) {
+    long long g = (total + kBlock - 1) / kBlock;
+    return (int)std::max<long long>(1, std::min<long long>(g, 65536));
+}
+
+// Enumeration of one trajectory: k > 0 cells per r_max, or 0 (brute force).
+int traj_cells(const wmd_batch *b, int t, const BoxSet &B, int *nc) {
+    const int n = (int)b->n_atoms[t];
+    if (b->pbc == WMD_PBC_NONE || n < 64) return 0;
+    const double mw = std::min(B.min_width[0], std::min(B.min_width[1], B.min_width[2]));
+    if (!((double)b->r_max <= 0.5 * mw * (1.0 - 1e-5))) return 0;
+    const int forced = b->tune_k[t];
+    if (forced < 0) return 0;
+    return choose_cells((double)b->r_max, B.min_width, n, forced, nc);
+}
+
+void plan_batch(wmd_batch *b) {
+    const int T = b->n_traj;
+    hipStream_t st = b->ctx->stream;
+    b->traj.assign(T, BatchTraj{});
+    b->rtraj.assign(T, BatchRmsdTraj{});
+    std::vector<FrameBox> fb;
+    fb.reserve((size_t)b->total_frames);
+    std::vector<double> cost(T, 0.0);
+    long long af = 0, units = 0, scan = 0, cnt = 0, sta = 0, box0 = 0, runits = 0, out0 = 0;
+    b->any_cells = false;
+    for (int t = 0; t < T; ++t) {
+        const int n = (int)b->n_atoms[t], F = (int)b->n_frames[t];
+        BatchTraj &d = b->traj[t];
+        d.store_off = b->store_off[t];
+        d.stride = (int)b->stride[t];
+        d.n = n;
+        d.n_frames = F;
+        d.af0 = af;
+        d.box0 = (int)box0;
+        d.unit0 = units;
+        d.scan0 = scan;
+        d.count_off = cnt;
+        d.start_off = sta;
+        BoxSet B;
+        if (b->pbc != WMD_PBC_NONE && F > 0)
+            B = make_boxes(b->pbc, b->boxes_host.data() + 9 * box0, (size_t)F);
+        else
+            B = make_boxes(WMD_PBC_NONE, nullptr, (size_t)F);
+        for (const FrameBox &x : B.fb) fb.push_back(x);
+        int nc[3] = {0, 0, 0};
+        const int k = F > 0 && n > 1 ? traj_cells(b, t, B, nc) : 0;
+        if (k > 0) {
+            const int ncell = nc[0] * nc[1] * nc[2];
+            d.kind = KIND_CELLS;
+            d.k = k;
+            d.ncx = nc[0]; d.ncy = nc[1]; d.ncz = nc[2];
+            d.shift_ok = (nc[0] >= 2 * k + 3 && nc[1] >= 2 * k + 3 && nc[2] >= 2 * k + 3) ? 1 : 0;
+            d.cull2 = cull2_for((double)b->r_max, B);
+            d.upf = ncell;
+            scan += F;
+            cnt += (long long)F * ncell;
+            sta += (long long)F * (ncell + 1);
+            b->any_cells = true;
+            // per-unit cost: home atoms x half-stencil atoms, plus a fixed part
+            const double nh = (double)n / ncell;
+            const double st_cells = (std::pow(2.0 * k + 1.0, 3) - 1.0) / 2.0 + 1.0;
+            cost[t] = (double)F * ncell * (nh * nh * st_cells + 2000.0);
+        } else {
+            d.kind = KIND_BRUTE;
+            d.k = 0;
+            d.ncx = d.ncy = d.ncz = 1;
+            d.upf = n > 1 ? (n + kBruteHome - 1) / kBruteHome : 0;
+            cost[t] = (double)F * d.upf * (kBruteHome * 0.5 * n * 1.5 + 2000.0);
+        }
+        if (F == 0) d.upf = 0;
+        units += (long long)F * d.upf;
+        af += (long long)F * n;
+        box0 += F;
+        // RMSD descriptor: tiles of up to 2048 float4 groups
+        BatchRmsdTraj &r = b->rtraj[t];
+        const int groups = (int)(b->stride[t] / 4);
+        r.store_off = b->store_off[t];
+        r.unit0 = runits;
+        r.out0 = out0;
+        r.stride = (int)b->stride[t];
+        r.n = n;
+        r.n_frames = F;
+        r.tiles = std::max(1, (groups + 2047) / 2048);
+        r.gpt = std::max(1, (groups + r.tiles - 1) / r.tiles);
+        runits += (long long)F * r.tiles;
+        out0 += F;
+    }
+    b->total_af = af;
+    b->n_units = units;
+    b->scan_blocks = scan;
+    b->cell_counts = cnt;
+    b->cell_starts = sta;
+    b->rmsd_units = runits;
+    // slices of equal estimated cost, cut at unit granularity
+    double total = 0.0;
+    for (double c : cost) total += c;
+    b->slice_lo[0] = 0;
+    {
+        int t = 0;
+        double acc = 0.0;
+        for (int q = 1; q < kSlices; ++q) {
+            const double target = total * q / kSlices;
+            while (t < T && acc + cost[t] < target) acc += cost[t++];
+            long long cut = units;
+            if (t < T) {
+                const long long tu = (long long)b->traj[t].n_frames * b->traj[t].upf;
+                const double frac = cost[t] > 0 ? (target - acc) / cost[t] : 0.0;
+                cut = b->traj[t].unit0 + (long long)std::llround(frac * (double)tu);
+            }
+            b->slice_lo[q] = std::max(b->slice_lo[q - 1], std::min(cut, units));
+        }
+        b->slice_lo[kSlices] = units;
+    }
+    // device buffers
+    const size_t nT = std::max(T, 1);
+    b->d_traj.reserve(sizeof(BatchTraj) * nT);
+    b->d_rtraj.reserve(sizeof(BatchRmsdTraj) * nT);
+    b->d_boxes.reserve(sizeof(FrameBox) * std::max<size_t>(fb.size(), 1));
+    b->d_flag.reserve(4 * std::max<long long>(b->total_frames, 1));
+    b->d_count.reserve(4 * std::max<long long>(cnt, 1));
+    b->d_start.reserve(4 * std::max<long long>(sta, 1));
+    const long long afc = std::max<long long>(af, 1);
+    if (b->flags & WMD_BATCH_RDF) {
+        if (b->any_cells) {
+            b->d_cell_of.reserve(4 * afc);
+            b->d_slot.reserve(4 * afc);
+            b->d_image.reserve(4 * afc);
+        }
+        b->d_sorted.reserve(16 * afc);
+    }
+    b->d_queue.reserve(8 * kSlices);
+    b->d_partials.reserve(8 * 13 * std::max<long long>(runits, 1));
+    const size_t tb = sizeof(BatchTraj) * T, rb = sizeof(BatchRmsdTraj) * T, bb = sizeof(FrameBox) * fb.size();
+    char *hs = (char *)b->h_boxes.reserve(std::max<size_t>(tb + rb + bb, 1));
+    WMD_HIP_CHECK(hipStreamSynchronize(st));  // staging free
+    std::memcpy(hs, b->traj.data(), tb);
+    std::memcpy(hs + tb, b->rtraj.data(), rb);
+    std::memcpy(hs + tb + rb, fb.data(), bb);
+    if (tb) WMD_HIP_CHECK(hipMemcpyAsync(b->d_traj.ptr, hs, tb, hipMemcpyHostToDevice, st));
+    if (rb) WMD_HIP_CHECK(hipMemcpyAsync(b->d_rtraj.ptr, hs + tb, rb, hipMemcpyHostToDevice, st));
+    if (bb) WMD_HIP_CHECK(hipMemcpyAsync(b->d_boxes.ptr, hs + tb + rb, bb, hipMemcpyHostToDevice, st));
+    WMD_HIP_CHECK(hipStreamSynchronize(st));
+    b->planned = true;
+    drop_graph(b);
+}
+
+template <int PBC>
+void enqueue_pairs(wmd_batch *b, hipStream_t st, const float *d_store) {
+    const int T = b->n_traj;
+    const BatchTraj *dt = b->d_traj.as<BatchTraj>();
+    if (b->any_cells && PBC != WMD_PBC_NONE) {
+        hipLaunchKernelGGL(batch_assign_kernel<PBC == WMD_PBC_NONE ? WMD_PBC_ORTHO : PBC>,
+                           dim3(grid_1d(b->total_af)), dim3(kBlock), 0, st, dt, T, b->total_af,
+                           d_store, (const FrameBox *)b->d_boxes.ptr, b->d_cell_of.as<unsigned>(),
+                           b->d_slot.as<unsigned>(), b->d_count.as<unsigned>(),
+                           b->d_image.as<unsigned>(), b->d_flag.as<unsigned>());
+        WMD_HIP_CHECK(hipGetLastError());
+        hipLaunchKernelGGL(batch_scan_kernel, dim3((unsigned)b->scan_blocks), dim3(kBlock), 0, st, dt, T,
+                           (const unsigned *)b->d_count.ptr, b->d_start.as<unsigned>());
+        WMD_HIP_CHECK(hipGetLastError());
+    }
+    hipLaunchKernelGGL(batch_scatter_kernel, dim3(grid_1d(b->total_af)), dim3(kBlock), 0, st, dt, T,
+                       b->total_af, d_store, (const unsigned *)b->d_cell_of.ptr,
+                       (const unsigned *)b->d_slot.ptr, (const unsigned *)b->d_image.ptr,
+                       (const unsigned *)b->d_start.ptr, b->d_sorted.as<float4>());
+    WMD_HIP_CHECK(hipGetLastError());
+
+    BatchPairArgs a;
+    a.traj = dt;
+    a.n_traj = T;
+    a.sorted = b->d_sorted.as<float4>();
+    a.start = b->d_start.as<unsigned>();
+    a.frame_flag = b->d_flag.as<unsigned>();
+    a.boxes = b->d_boxes.as<FrameBox>();
+    a.counts = b->d_counts.as<unsigned long long>();
+    a.queue = b->d_queue.as<unsigned long long>();
+    for (int q = 0; q <= kSlices; ++q) a.slice_lo[q] = b->slice_lo[q];
+    HistParams &p = a.p;
+    std::memset(&p, 0, sizeof(p));
+    p.bins = b->bins;
+    p.hist_copies = 1;
+    p.cut = b->tables.cut;
+    p.inv_w = 1.0f / b->tables.w;
+    p.thr = b->d_thr.as<float>();
+    p.tab2 = b->table ? b->d_tab2.as<uint2>() : nullptr;
+    p.tab_n = b->table ? (int)b->tables.tab2.size() : 0;
+    p.tab_sh = b->tables.tab_sh;
+    p.tab_base = b->tables.tab_base;
+    p.counts = a.counts;
+    const size_t lds = batch_lds_bytes(b->bins, p.tab_n);
+    auto launch = [&](auto kern) {
+        int occ = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, kBatchBlock, lds) != hipSuccess || occ < 1)
+            occ = 1;
+        int grid = std::max(kSlices, b->ctx->cu_count * occ / kSlices * kSlices);
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(kBatchBlock), lds, st, a);
+        WMD_HIP_CHECK(hipGetLastError());
+    };
+    if (b->table) launch(batch_pair_kernel<PBC, true>);
+    else launch(batch_pair_kernel<PBC, false>);
+}
+
+// The whole step on `st`: clears, cell build, pair kernel, RMSD.
+void enqueue_step(wmd_batch *b, hipStream_t st, const float *d_store, float *d_out32, double *d_out64,
+                  double *d_err) {
+    const bool rdf = (b->flags & WMD_BATCH_RDF) && b->bins > 0 && b->n_units > 0;
+    if (rdf) {
+        hipLaunchKernelGGL(fill_u32_kernel, dim3(grid_1d(std::max<long long>(b->cell_counts, 1))), dim3(kBlock), 0, st,
+                           b->d_count.as<unsigned>(), b->cell_counts, 0u);
+        hipLaunchKernelGGL(fill_u32_kernel, dim3(grid_1d(std::max<long long>(b->total_frames, 1))), dim3(kBlock), 0, st,
+                           b->d_flag.as<unsigned>(), b->total_frames, 0u);
+        hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(kBlock), 0, st, b->d_queue.as<unsigned long long>(),
+                           (long long)kSlices);
+        WMD_HIP_CHECK(hipGetLastError());
+        if (b->pbc == WMD_PBC_NONE) enqueue_pairs<WMD_PBC_NONE>(b, st, d_store);
+        else if (b->pbc == WMD_PBC_ORTHO) enqueue_pairs<WMD_PBC_ORTHO>(b, st, d_store);
+        else enqueue_pairs<WMD_PBC_TRICLINIC>(b, st, d_store);
+    }
+    if ((b->flags & WMD_BATCH_RMSD) && d_out32)
+        batch_rmsd_enqueue(st, b->d_rtraj.as<BatchRmsdTraj>(), b->n_traj, b->rmsd_units,
+                           b->total_frames, d_store, b->d_partials.as<double>(), d_out32, d_out64, d_err);
+}
+
+void run_batch(wmd_batch *b, const float *d_store, float *d_out32, double *d_out64, double *d_err,
+               hipStream_t st, bool graph_ok) {
+    WMD_REQUIRE(b->planned, WMD_ERR_INVALID, "wmd_batch_set_boxes must be called before a run");
+    WMD_REQUIRE(d_store != nullptr || b->store_floats == 0, WMD_ERR_INVALID, "null store");
+    WMD_REQUIRE(!(b->flags & WMD_BATCH_RMSD) || d_out32 || b->total_frames == 0, WMD_ERR_INVALID,
+                "RMSD output required (the batch was created with WMD_BATCH_RMSD)");
+    if (!(b->use_graph && graph_ok)) {
+        ProfScope prof(b->ctx, st, "batch_step");
+        enqueue_step(b, st, d_store, d_out32, d_out64, d_err);
+        return;
+    }
+    if (!b->exec || b->g_store != d_store || b->g_out32 != d_out32 || b->g_out64 != d_out64 ||
+        b->g_err != d_err) {
+        drop_graph(b);
+        if (!b->cap) WMD_HIP_CHECK(hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking));
+        WMD_HIP_CHECK(hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal));
+        try {
+            enqueue_step(b, b->cap, d_store, d_out32, d_out64, d_err);
+        } catch (...) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(b->cap, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        WMD_HIP_CHECK(hipStreamEndCapture(b->cap, &b->graph));
+        WMD_HIP_CHECK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+        b->g_store = d_store;
+        b->g_out32 = d_out32;
+        b->g_out64 = d_out64;
+        b->g_err = d_err;
+    }
+    ProfScope prof(b->ctx, st, "batch_step");
+    WMD_HIP_CHECK(hipGraphLaunch(b->exec, st));
+}
+
+TuneKey tune_key(const wmd_batch *b, int t) {
+    const int n = (int)b->n_atoms[t];
+    double vol = 0.0;
+    long long box0 = 0;
+    for (int s = 0; s < t; ++s) box0 += b->n_frames[s];
+    if (b->pbc != WMD_PBC_NONE && b->n_frames[t] > 0) {
+        const float *m = b->boxes_host.data() + 9 * box0;
+        const double a[9] = {m[0], m[1], m[2], m[3], m[4], m[5], m[6], m[7], m[8]};
+        vol = std::fabs(a[0] * (a[4] * a[8] - a[5] * a[7]) - a[1] * (a[3] * a[8] - a[5] * a[6]) +
+                        a[2] * (a[3] * a[7] - a[4] * a[6]));
+    }
+    const int nb = n > 0 ? (int)std::floor(std::log2((double)n) * 4.0) : -1;
+    const int db = vol > 0 ? (int)std::floor(std::log2((double)n / vol) * 4.0) : -999;
+    uint32_t rb;
+    std::memcpy(&rb, &b->r_max, 4);
+    return TuneKey(b->ctx->arch, b->pbc, b->bins, rb, nb, db);
+}
+
+wmd_batch *create_batch(wmd_ctx *ctx, size_t n_traj, const uint32_t *n_atoms, const uint32_t *n_frames,
+                        size_t bins, float r_max, int pbc, uint32_t flags) {
+    WMD_REQUIRE(ctx, WMD_ERR_INVALID, "null context");
+    WMD_REQUIRE(n_traj == 0 || (n_atoms && n_frames), WMD_ERR_INVALID, "null size arrays");
+    WMD_REQUIRE(n_traj < (1u << 20), WMD_ERR_UNSUPPORTED, "too many trajectories");
+    WMD_REQUIRE(pbc >= WMD_PBC_NONE && pbc <= WMD_PBC_TRICLINIC, WMD_ERR_PARSE,
+                "pbc must be none, orthorhombic or triclinic");
+    WMD_REQUIRE(bins <= 1024, WMD_ERR_UNSUPPORTED, "a batch supports at most 1024 bins");
+    WMD_REQUIRE(std::isfinite(r_max) && r_max > 0.0f, WMD_ERR_PARSE, "r_max must be finite and positive");
+    WMD_REQUIRE((flags & ~(uint32_t)(WMD_BATCH_RDF | WMD_BATCH_RMSD | WMD_BATCH_NO_GRAPH)) == 0,
+                WMD_ERR_INVALID, "unknown batch flags");
+    bind(ctx);
+    auto *b = new wmd_batch();
+    try {
+        b->ctx = ctx;
+        b->n_traj = (int)n_traj;
+        b->n_atoms.assign(n_atoms, n_atoms + n_traj);
+        b->n_frames.assign(n_frames, n_frames + n_traj);
+        b->bins = (int)bins;
+        b->r_max = r_max;
+        b->pbc = pbc;
+        b->flags = flags;
+        b->use_graph = !(flags & WMD_BATCH_NO_GRAPH);
+        b->tune_k.assign(n_traj, 0);
+        long long off = 0, frames = 0;
+        for (size_t t = 0; t < n_traj; ++t) {
+            WMD_REQUIRE(n_atoms[t] < (1u << 27), WMD_ERR_UNSUPPORTED, "trajectory too large");
+            const uint32_t s = (uint32_t)round_up(std::max<uint32_t>(n_atoms[t], 1), 4);
+            b->store_off.push_back(off);
+            b->stride.push_back(s);
+            off += 3LL * s * n_frames[t];
+            frames += n_frames[t];
+        }
+        b->store_floats = off;
+        b->total_frames = frames;
+        WMD_REQUIRE(frames < (1ll << 31), WMD_ERR_UNSUPPORTED, "too many frames");
+        hipStream_t st = ctx->stream;
+        b->d_counts.reserve(8 * std::max<size_t>(n_traj * bins, 1));
+        hipLaunchKernelGGL(fill_u64_kernel, dim3((unsigned)((std::max<size_t>(n_traj * bins, 1) + kBlock - 1) / kBlock)),
+                           dim3(kBlock), 0, st, b->d_counts.as<unsigned long long>(),
+                           (long long)(n_traj * bins));
+        WMD_HIP_CHECK(hipGetLastError());
+        if (bins > 0) {
+            b->tables = make_bin_tables((int)bins, r_max);
+            b->table = !b->tables.tab2.empty() && (int)b->tables.tab2.size() <= kMaxTab2;
+            b->d_thr.reserve((bins + 1) * 4);
+            WMD_HIP_CHECK(hipMemcpyAsync(b->d_thr.ptr, b->tables.thr.data(), (bins + 1) * 4,
+                                         hipMemcpyHostToDevice, st));
+            if (b->table) {
+                b->d_tab2.reserve(b->tables.tab2.size() * 8);
+                WMD_HIP_CHECK(hipMemcpyAsync(b->d_tab2.ptr, b->tables.tab2.data(), b->tables.tab2.size() * 8,
+                                             hipMemcpyHostToDevice, st));
+            }
+        }
+        WMD_HIP_CHECK(hipStreamSynchronize(st));
+        WMD_REQUIRE(batch_lds_bytes((int)bins, b->table ? (int)b->tables.tab2.size() : 0) <= ctx->lds_per_block,
+                    WMD_ERR_UNSUPPORTED, "too many bins for the batch kernel's LDS");
+    } catch (...) {
+        delete b;
+        throw;
+    }
+    return b;
+}
+
+void set_boxes(wmd_batch *b, const float *boxes) {
+    WMD_REQUIRE(b->pbc == WMD_PBC_NONE || boxes || b->total_frames == 0, WMD_ERR_INVALID,
+                "boxes required for PBC");
+    if (b->pbc != WMD_PBC_NONE && b->total_frames > 0)
+        b->boxes_host.assign(boxes, boxes + 9 * b->total_frames);
+    else
+        b->boxes_host.clear();
+    plan_batch(b);
+}
+
+}  // namespace
+
+extern "C" {
+
+wmd_status wmd_batch_create(wmd_ctx *ctx, size_t n_traj, const uint32_t *n_atoms,
+                            const uint32_t *n_frames, size_t bins, float r_max, int pbc,
+                            uint32_t flags, wmd_batch **out) {
+    WMD_API_BEGIN
+    WMD_REQUIRE(out, WMD_ERR_INVALID, "null output handle");
+    *out = nullptr;
+    *out = create_batch(ctx, n_traj, n_atoms, n_frames, bins, r_max, pbc, flags);
+    WMD_API_END
+}
+
+wmd_status wmd_batch_store_layout(const wmd_batch *b, uint64_t *offsets, uint32_t *strides,
+                                  uint64_t *total_floats) {
+    WMD_API_BEGIN
+    WMD_REQUIRE(b, WMD_ERR_INVALID, "null batch");
+    for (int t = 0; t < b->n_traj; ++t) {
+        if (offsets) offsets[t] = (uint64_t)b->store_off[t];
+        if (strides) strides[t] = b->stride[t];
+    }
+    if (total_floats) *total_floats = (uint64_t)b->store_floats;
+    WMD_API_END
+}
+
+wmd_status wmd_batch_set_boxes(wmd_batch *b, const float *boxes) {
+    WMD_API_BEGIN
+    WMD_REQUIRE(b, WMD_ERR_INVALID, "null batch");
+    bind(b->ctx);
+    set_boxes(b, boxes);
+    WMD_API_END
+}
+
+wmd_status wmd_batch_set_tuning(wmd_batch *b, const int *cell_k) {
+    WMD_API_BEGIN
+    WMD_REQUIRE(b, WMD_ERR_INVALID, "null batch");
+    for (int t = 0; t < b->n_traj; ++t) {
+        const int k = cell_k ? cell_k[t] : 0;
+        WMD_REQUIRE(k >= -1 && k <= 3, WMD_ERR_INVALID, "cell_k must be -1, 0 or 1..3");
+        b->tune_k[t] = k;
+    }
+    if (b->planned) {
+        bind(b->ctx);
+        plan_batch(b);
+    }
+    WMD_API_END
+}
+
+wmd_status wmd_batch_run_device(wmd_batch *b, const float *d_store, float *d_rmsd, double *d_rmsd64,
+                                double *d_err, void *stream) {
+    WMD_API_BEGIN
+    WMD_REQUIRE(b, WMD_ERR_INVALID, "null batch");
+    bind(b->ctx);
+    run_batch(b, d_store, d_rmsd, d_rmsd64, d_err, pick_stream(b->ctx, stream), true);
+    WMD_API_END
+}
+
+wmd_status wmd_batch_read_counts(wmd_batch *b, uint64_t *counts) {
+    WMD_API_BEGIN
+    WMD_REQUIRE(b && (counts || b->n_traj * b->bins == 0), WMD_ERR_INVALID, "null argument");
+    bind(b->ctx);
+    WMD_HIP_CHECK(hipDeviceSynchronize());
+    const size_t bytes = (size_t)b->n_traj * b->bins * 8;
+    if (bytes) WMD_HIP_CHECK(hipMemcpy(counts, b->d_counts.ptr, bytes, hipMemcpyDeviceToHost));
+    WMD_API_END
+}
+
+wmd_status wmd_batch_reset(wmd_batch *b) {
+    WMD_API_BEGIN
+    WMD_REQUIRE(b, WMD_ERR_INVALID, "null batch");
+    bind(b->ctx);
+    WMD_HIP_CHECK(hipDeviceSynchronize());
+    const long long n = (long long)b->n_traj * b->bins;
+    if (n) {
+        hipLaunchKernelGGL(fill_u64_kernel, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                           b->ctx->stream, b->d_counts.as<unsigned long long>(), n);
+        WMD_HIP_CHECK(hipGetLastError());
+        WMD_HIP_CHECK(hipStreamSynchronize(b->ctx->stream));
+    }
+    WMD_API_END
+}
+
+wmd_status wmd_batch_plan_info(const wmd_batch *b, int *cell_k, long long *n_units) {
+    WMD_API_BEGIN
+    WMD_REQUIRE(b, WMD_ERR_INVALID, "null batch");
+    WMD_REQUIRE(b->planned, WMD_ERR_INVALID, "wmd_batch_set_boxes must be called first");
+    for (int t = 0; t < b->n_traj; ++t)
+        if (cell_k) cell_k[t] = b->traj[t].kind == KIND_CELLS ? b->traj[t].k : 0;
+    if (n_units) *n_units = b->n_units;
+    WMD_API_END
+}
+
+wmd_status wmd_batch_autotune(wmd_batch *b, const float *d_store, int reps, void *stream) {
+    WMD_API_BEGIN
+    WMD_REQUIRE(b, WMD_ERR_INVALID, "null batch");
+    WMD_REQUIRE(b->planned, WMD_ERR_INVALID, "wmd_batch_set_boxes must be called first");
+    WMD_REQUIRE(d_store || b->store_floats == 0, WMD_ERR_INVALID, "null store");
+    bind(b->ctx);
+    hipStream_t st = pick_stream(b->ctx, stream);
+    reps = std::max(1, std::min(reps, 20));
+    // classes of trajectories sharing a key; representative = median atom count
+    std::map<TuneKey, std::vector<int>> classes;
+    for (int t = 0; t < b->n_traj; ++t)
+        if (b->n_atoms[t] > 1 && b->n_frames[t] > 0) classes[tune_key(b, t)].push_back(t);
+    std::vector<long long> box0(b->n_traj + 1, 0);
+    for (int t = 0; t < b->n_traj; ++t) box0[t + 1] = box0[t] + b->n_frames[t];
+    hipEvent_t e0, e1;
+    WMD_HIP_CHECK(hipEventCreate(&e0));
+    WMD_HIP_CHECK(hipEventCreate(&e1));
+    try {
+        for (auto &kv : classes) {
+            int choice;
+            {
+                std::lock_guard<std::mutex> lk(g_tune_mu);
+                auto it = g_tune.find(kv.first);
+                choice = it == g_tune.end() ? 0 : it->second;
+            }
+            if (choice == 0) {
+                std::vector<int> &ts = kv.second;
+                std::sort(ts.begin(), ts.end(), [&](int x, int y) { return b->n_atoms[x] < b->n_atoms[y]; });
+                const int t = ts[ts.size() / 2];
+                const uint32_t na = b->n_atoms[t], nf = b->n_frames[t];
+                double best = 1e300;
+                for (int cand : {-1, 1, 2, 3}) {
+                    wmd_batch *s = create_batch(b->ctx, 1, &na, &nf, b->bins, b->r_max, b->pbc,
+                                                WMD_BATCH_RDF | WMD_BATCH_NO_GRAPH);
+                    try {
+                        s->tune_k[0] = cand;
+                        set_boxes(s, b->boxes_host.empty() ? nullptr : b->boxes_host.data() + 9 * box0[t]);
+                        const bool taken = cand < 0 ? s->traj[0].kind == KIND_BRUTE
+                                                    : (s->traj[0].kind == KIND_CELLS && s->traj[0].k == cand);
+                        if (taken) {
+                            const float *sub = d_store + b->store_off[t];
+                            run_batch(s, sub, nullptr, nullptr, nullptr, st, false);
+                            WMD_HIP_CHECK(hipEventRecord(e0, st));
+                            for (int r = 0; r < reps; ++r) run_batch(s, sub, nullptr, nullptr, nullptr, st, false);
+                            WMD_HIP_CHECK(hipEventRecord(e1, st));
+                            WMD_HIP_CHECK(hipEventSynchronize(e1));
+                            float ms = 0.f;
+                            WMD_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+                            if (ms < best) {
+                                best = ms;
+                                choice = cand;
+                            }
+                        }
+                    } catch (...) {
+                        delete s;
+                        throw;
+                    }
+                    delete s;
+                }
+                if (choice == 0) choice = -1;
+                std::lock_guard<std::mutex> lk(g_tune_mu);
+                g_tune[kv.first] = choice;
+            }
+            for (int t : kv.second) b->tune_k[t] = choice;
+        }
+    } catch (...) {
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    plan_batch(b);
+    WMD_API_END
+}
+
+wmd_status wmd_autotune_export(char *buf, size_t cap, size_t *len) {
+    WMD_API_BEGIN
+    std::ostringstream os;
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        for (const auto &kv : g_tune)
+            os << std::get<0>(kv.first) << ' ' << std::get<1>(kv.first) << ' ' << std::get<2>(kv.first)
+               << ' ' << std::get<3>(kv.first) << ' ' << std::get<4>(kv.first) << ' '
+               << std::get<5>(kv.first) << ' ' << kv.second << '\n';
+    }
+    const std::string s = os.str();
+    if (len) *len = s.size();
+    if (buf && cap) {
+        const size_t n = std::min(cap - 1, s.size());
+        std::memcpy(buf, s.data(), n);
+        buf[n] = '\0';
+    }
+    WMD_API_END
+}
+
+wmd_status wmd_autotune_import(const char *text) {
+    WMD_API_BEGIN
+    WMD_REQUIRE(text, WMD_ERR_INVALID, "null text");
+    std::istringstream is(text);
+    std::string line;
+    std::map<TuneKey, int> add;
+    while (std::getline(is, line)) {
+        if (line.empty()) continue;
+        std::istringstream ls(line);
+        std::string arch;
+        int pbc, bins, nb, db, k;
+        uint32_t rb;
+        WMD_REQUIRE(bool(ls >> arch >> pbc >> bins >> rb >> nb >> db >> k), WMD_ERR_PARSE,
+                    "malformed autotune line: " + line);
+        WMD_REQUIRE(k == -1 || (k >= 1 && k <= 3), WMD_ERR_PARSE, "autotune choice out of range: " + line);
+        add[TuneKey(arch, pbc, bins, rb, nb, db)] = k;
+    }
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    for (auto &kv : add) g_tune[kv.first] = kv.second;
+    WMD_API_END
+}
+
+void wmd_batch_destroy(wmd_batch *b) {
+    if (!b) return;
+    bind(b->ctx);
+    (void)hipDeviceSynchronize();
+    delete b;
+}
+
+}  // extern "C"

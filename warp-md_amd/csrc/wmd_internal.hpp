@@ -126,6 +126,24 @@ struct ProfScope {
     ~ProfScope();
 };
 
+// Batched RMSD to frame 0 over the trajectories of an uneven batch
+// (wmd_batch): one descriptor per trajectory in the batch store.
+struct BatchRmsdTraj {
+    long long store_off;  // float offset of frame 0 ([F][3][stride] SoA)
+    long long unit0;      // first moments unit (frame, tile) of this trajectory
+    long long out0;       // index of frame 0 in the per-frame outputs
+    int stride, n, n_frames, tiles, gpt;  // gpt: float4 groups per tile
+    int pad;
+};
+
+// Moments (one wave per (trajectory, frame, tile), FP64) and the QCP finalize
+// (one thread per output frame): out32 / out64 [n_out] RMSD of every frame to
+// its trajectory's frame 0 after optimal superposition, err [n_out] an error
+// bound (out64 / err may be NULL).  partials holds n_units x 13 doubles.
+void batch_rmsd_enqueue(hipStream_t st, const BatchRmsdTraj *d_traj, int n_traj,
+                        long long n_units, long long n_out, const float *d_store,
+                        double *d_partials, float *d_out32, double *d_out64, double *d_err);
+
 // Host-side minimum-image support shared by the RDF paths.
 struct BoxInfo {
     // f32 parameters the pair kernels use (one per frame).

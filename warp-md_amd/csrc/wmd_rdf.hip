@@ -541,48 +541,6 @@ namespace {
 // instructions, so per-wave copies would not reduce conflicts.
 int hist_copies_for(int) { return 1; }
 
-// s-bits bin table: the largest shift such that every bucket of 2^sh bit
-// patterns in [T[1], cut) holds at most one breakpoint T[1..bins-1].
-void build_table(wmd_rdf *h) {
-    h->tab.clear();
-    h->tab2.clear();
-    if (h->bins < 2) return;
-    const uint32_t lo_bits = f_bits(h->thr[1]);
-    const uint32_t hi_bits = f_bits(h->cut) - 1;  // largest s below the cut
-    for (int sh = 23; sh >= 8; --sh) {
-        const int base = (int)(lo_bits >> sh);
-        const int n = (int)(hi_bits >> sh) - base + 1;
-        if (n > kMaxTab) break;
-        bool ok = true;
-        for (int k = 1; k + 1 < h->bins && ok; ++k)
-            if ((f_bits(h->thr[k]) >> sh) == (f_bits(h->thr[k + 1]) >> sh)) ok = false;
-        if (!ok) continue;
-        std::vector<unsigned short> tab(n);
-        int b = 0;
-        for (int k = 0; k < n; ++k) {
-            const uint32_t start_bits = (uint32_t)(base + k) << sh;
-            const float s0 = k == 0 ? 0.0f : bits_f(start_bits);
-            while (b + 1 < h->bins && s0 >= h->thr[b + 1]) ++b;
-            tab[k] = (unsigned short)b;
-        }
-        // split table: the breakpoint inside each bucket (+inf if none) and the
-        // bin of the bucket's values below it
-        std::vector<uint2> tab2(n);
-        for (int k = 0; k < n; ++k) {
-            const int bk = tab[k];
-            uint32_t split = 0x7F800000u;
-            if (bk + 1 <= h->bins && (f_bits(h->thr[bk + 1]) >> sh) == (uint32_t)(base + k))
-                split = f_bits(h->thr[bk + 1]);
-            tab2[k] = make_uint2(split, (unsigned)bk * 8u);  // byte offset of the bin
-        }
-        h->tab.swap(tab);
-        h->tab2.swap(tab2);
-        h->tab_sh = sh;
-        h->tab_base = base << sh;  // bit pattern where bucket 0 starts
-        return;
-    }
-}
-
 template <class K> int occupancy(K kernel, size_t lds) {
     int nb = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, kBlock, lds) != hipSuccess ||
@@ -853,15 +811,16 @@ wmd_status wmd_rdf_create(wmd_ctx *ctx, const uint32_t *sel_a, size_t n_a, const
         WMD_REQUIRE(h->io.size() < (1u << 31), WMD_ERR_UNSUPPORTED, "selection too large");
         if (bins > 0) {
             // exact thresholds (rdf.rs:896, 929-935)
-            h->bin_width = r_max / (float)bins;
+            BinTables T = make_bin_tables((int)bins, r_max);
+            h->bin_width = T.w;
             const float w = h->bin_width;
             const int B = (int)bins;
-            h->thr.resize(bins + 1);
-            h->thr[0] = 0.0f;
-            for (int k = 1; k <= B; ++k)
-                h->thr[k] = min_s([&](float s) { return ref_key(s, r_max, w, B) >= k; });
-            h->cut = h->thr[B];
-            build_table(h);
+            h->thr.swap(T.thr);
+            h->cut = T.cut;
+            h->tab.swap(T.tab);
+            h->tab2.swap(T.tab2);
+            h->tab_sh = T.tab_sh;
+            h->tab_base = T.tab_base;
             h->elo.assign(bins + 1, 0.0f);
             h->ehi.assign(bins + 1, 0.0f);
             if (flags & WMD_RDF_EDGE_DIAGNOSTICS) {

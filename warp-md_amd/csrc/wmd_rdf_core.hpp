@@ -225,6 +225,32 @@ __device__ __forceinline__ float pair_s_shift(float xi, float yi, float zi, floa
     return dx * dx + dy * dy + dz * dz;
 }
 
+// The reference wrap roundf(dx / L) for any pair, without the IEEE division
+// in the common case.  q = dx * (1 / L) differs from the correctly rounded
+// quotient by less than 5e-5 for |q| < 256, so when q lies more than 1e-4
+// from a half-integer, rint(q) equals roundf(dx / L); otherwise (or for
+// |q| >= 256, or NaN) the division is done.  Triclinic boxes use the
+// reference device formula (roundf of products, no division).
+__device__ __forceinline__ float wrap_checked(float d, float l, float il) {
+    const float q = d * il;
+    float n = rintf(q);
+    if (!(fabsf(q - n) <= 0.4999f) || !(fabsf(q) < 256.f)) n = roundf(d / l);
+    return d - n * l;
+}
+
+template <int PBC>
+__device__ __forceinline__ float pair_s_checked(float xi, float yi, float zi, float xj, float yj,
+                                                float zj, const FrameBox &b) {
+    if constexpr (PBC == WMD_PBC_ORTHO) {
+        const float dx = wrap_checked(xj - xi, b.lx, b.ilx);
+        const float dy = wrap_checked(yj - yi, b.ly, b.ily);
+        const float dz = wrap_checked(zj - zi, b.lz, b.ilz);
+        return dx * dx + dy * dy + dz * dz;
+    } else {
+        return pair_s<PBC, true>(xi, yi, zi, xj, yj, zj, b);
+    }
+}
+
 // XCD-aware persistent schedule: WG g runs on XCD g % 8 (round-robin dispatch);
 // each XCD owns a contiguous slice of the unit range.
 struct UnitRange {
@@ -355,6 +381,31 @@ __global__ void fill_u64_kernel(unsigned long long *p, long long n) {
     if (i < n) p[i] = 0ull;
 }
 
+// Cell of one atom and its image word ((m + 32) per byte, m = floor of the
+// fractional coordinate); far_out: an image index beyond kImageLimit or a
+// non-finite coordinate (the frame then takes the rint path).
+template <int PBC>
+__device__ __forceinline__ int assign_atom(float x, float y, float z, const FrameBox &b, int ncx,
+                                           int ncy, int ncz, unsigned &img, bool &far_out) {
+    const int nc[3] = {ncx, ncy, ncz};
+    float fr[3];
+    frac_of<PBC>(x, y, z, b, fr);
+    int c[3];
+    img = 0;
+    far_out = false;
+    for (int d = 0; d < 3; ++d) {
+        float fl = floorf(fr[d]);
+        float w = fr[d] - fl;  // in [0, 1) (NaN for non-finite input)
+        int cc = (int)(w * (float)nc[d]);
+        c[d] = min(max(cc, 0), nc[d] - 1);
+        bool ok = fl >= -(float)kImageLimit && fl <= (float)kImageLimit;
+        int m = ok ? (int)fl : 0;
+        far_out |= !ok;
+        img |= ((unsigned)(m + kImageBias) & 0xFFu) << (8 * d);
+    }
+    return (c[2] * ncy + c[1]) * ncx + c[0];
+}
+
 template <int PBC>
 __global__ __launch_bounds__(kBlock) void cell_assign_kernel(
     const float *__restrict__ soa, long long frame_stride, int stride, int n, int n_frames,
@@ -363,31 +414,17 @@ __global__ __launch_bounds__(kBlock) void cell_assign_kernel(
     unsigned *__restrict__ cell_count, unsigned *__restrict__ image_of,
     unsigned *__restrict__ frame_flag) {
     const int ncell = ncx * ncy * ncz;
-    const int nc[3] = {ncx, ncy, ncz};
     long long total = (long long)n * n_frames;
     for (long long t = (long long)blockIdx.x * kBlock + threadIdx.x; t < total;
          t += (long long)gridDim.x * kBlock) {
         int f = (int)(t / n);
         int i = (int)(t - (long long)f * n);
         const float *fx = soa + (frame0 + f) * frame_stride;
-        float fr[3];
-        frac_of<PBC>(fx[i], fx[stride + i], fx[2 * stride + i], boxes[f], fr);
-        int c[3];
-        unsigned img = 0;
-        bool far_out = false;
-        for (int d = 0; d < 3; ++d) {
-            float fl = floorf(fr[d]);
-            float w = fr[d] - fl;  // in [0, 1) (NaN for non-finite input)
-            int cc = (int)(w * (float)nc[d]);
-            c[d] = min(max(cc, 0), nc[d] - 1);
-            // image index; non-finite or far-away atoms force the rint path
-            bool ok = fl >= -(float)kImageLimit && fl <= (float)kImageLimit;
-            int m = ok ? (int)fl : 0;
-            far_out |= !ok;
-            img |= ((unsigned)(m + kImageBias) & 0xFFu) << (8 * d);
-        }
+        unsigned img;
+        bool far_out;
+        const int cid = assign_atom<PBC>(fx[i], fx[stride + i], fx[2 * stride + i], boxes[f], ncx,
+                                         ncy, ncz, img, far_out);
         if (far_out) atomicOr(&frame_flag[f], 1u);
-        int cid = (c[2] * ncy + c[1]) * ncx + c[0];
         unsigned slot = atomicAdd(&cell_count[(long long)f * ncell + cid], 1u);
         cell_of[t] = (unsigned)cid;
         cell_slot[t] = slot;
@@ -395,14 +432,11 @@ __global__ __launch_bounds__(kBlock) void cell_assign_kernel(
     }
 }
 
-// Exclusive scan of each frame's cell counts (one workgroup per frame);
-// start has ncell + 1 entries per frame.
-__global__ __launch_bounds__(kBlock) void cell_scan_kernel(const unsigned *__restrict__ count,
-                                                           unsigned *__restrict__ start, int ncell) {
+// Exclusive scan of one frame's cell counts by one workgroup of kBlock
+// threads; s has ncell + 1 entries.
+__device__ __forceinline__ void scan_cells(const unsigned *__restrict__ c, unsigned *__restrict__ s,
+                                           int ncell) {
     __shared__ unsigned part[kBlock];
-    const long long f = blockIdx.x;
-    const unsigned *c = count + f * ncell;
-    unsigned *s = start + f * (ncell + 1);
     int per = (ncell + kBlock - 1) / kBlock;
     int lo = threadIdx.x * per;
     int hi = min(lo + per, ncell);
@@ -423,6 +457,14 @@ __global__ __launch_bounds__(kBlock) void cell_scan_kernel(const unsigned *__res
         run += c[k];
     }
     if (threadIdx.x == kBlock - 1) s[ncell] = part[kBlock - 1];
+}
+
+// Exclusive scan of each frame's cell counts (one workgroup per frame);
+// start has ncell + 1 entries per frame.
+__global__ __launch_bounds__(kBlock) void cell_scan_kernel(const unsigned *__restrict__ count,
+                                                           unsigned *__restrict__ start, int ncell) {
+    const long long f = blockIdx.x;
+    scan_cells(count + f * ncell, start + f * (ncell + 1), ncell);
 }
 
 // sorted[f][start[c] + slot] = (x, y, z, meta)
@@ -574,7 +616,8 @@ __device__ __forceinline__ void cell_count(float s, unsigned mi, unsigned mAj, u
 // Pairs every j of one ring chunk with every home atom i (i unrolled by 4:
 // one 64-byte scalar load per group).
 // MODE 0: zero image shift (orthorhombic: dx = xj - xi); 1: j carries its
-// image shift n (n + 128 as 3 x u8 in the meta word); 2: the rint wrap.
+// image shift n (n + 128 as 3 x u8 in the meta word); 2: the rint wrap;
+// 3: the reference's roundf(dx / L) wrap (pair_s_checked).
 // HOME: the chunk holds home atoms (stream positions below n_home), which
 // pair only with home atoms of lower index.
 template <int PBC, bool LT, bool WEIGHTED, bool EDGE, bool TABLE, int MODE, bool HOME>
@@ -601,8 +644,10 @@ __device__ __forceinline__ void cell_chunk(const float4 *__restrict__ hp, int n_
             return dx * dx + dy * dy + dz * dz;
         } else if constexpr (MODE <= 1) {
             return pair_s_shift<PBC, LT>(pi.x, pi.y, pi.z, xj, yj, zj, sx, sy, sz, b);
-        } else {
+        } else if constexpr (MODE == 2) {
             return pair_s<PBC, false>(pi.x, pi.y, pi.z, xj, yj, zj, b);
+        } else {
+            return pair_s_checked<PBC>(pi.x, pi.y, pi.z, xj, yj, zj, b);
         }
     };
     int i = 0;
@@ -990,6 +1035,66 @@ template <class P> static float min_s(P pred) {
         if (pred(bits_f(mid))) hi = mid; else lo = mid;
     }
     return bits_f(hi);
+}
+
+// Exact bin breakpoints of the reference's map s -> bin (rdf.rs:896, 929-935):
+// thr[k] = min{ f32 s : bin(s) >= k }, cut = thr[bins]; and the s-bits bin
+// table: the largest shift such that every bucket of 2^sh bit patterns in
+// [thr[1], cut) holds at most one breakpoint thr[1..bins-1] (tab: the bin of
+// the bucket's first value; tab2: the breakpoint inside the bucket, +inf if
+// none, and 8 x the bin of the values below it).  Empty tables when no shift
+// keeps the table within kMaxTab entries.
+struct BinTables {
+    float w = 0.f, cut = 0.f;
+    std::vector<float> thr;
+    std::vector<unsigned short> tab;
+    std::vector<uint2> tab2;
+    int tab_sh = 0, tab_base = 0;
+};
+
+inline BinTables make_bin_tables(int bins, float r_max) {
+    BinTables T;
+    T.w = r_max / (float)bins;
+    const float w = T.w;
+    T.thr.resize((size_t)bins + 1);
+    T.thr[0] = 0.0f;
+    for (int k = 1; k <= bins; ++k)
+        T.thr[k] = min_s([&](float s) { return ref_key(s, r_max, w, bins) >= k; });
+    T.cut = T.thr[bins];
+    if (bins < 2) return T;
+    const uint32_t lo_bits = f_bits(T.thr[1]);
+    const uint32_t hi_bits = f_bits(T.cut) - 1;  // largest s below the cut
+    for (int sh = 23; sh >= 8; --sh) {
+        const int base = (int)(lo_bits >> sh);
+        const int n = (int)(hi_bits >> sh) - base + 1;
+        if (n > kMaxTab) break;
+        bool ok = true;
+        for (int k = 1; k + 1 < bins && ok; ++k)
+            if ((f_bits(T.thr[k]) >> sh) == (f_bits(T.thr[k + 1]) >> sh)) ok = false;
+        if (!ok) continue;
+        std::vector<unsigned short> tab(n);
+        int b = 0;
+        for (int k = 0; k < n; ++k) {
+            const uint32_t start_bits = (uint32_t)(base + k) << sh;
+            const float s0 = k == 0 ? 0.0f : bits_f(start_bits);
+            while (b + 1 < bins && s0 >= T.thr[b + 1]) ++b;
+            tab[k] = (unsigned short)b;
+        }
+        std::vector<uint2> tab2(n);
+        for (int k = 0; k < n; ++k) {
+            const int bk = tab[k];
+            uint32_t split = 0x7F800000u;
+            if (bk + 1 <= bins && (f_bits(T.thr[bk + 1]) >> sh) == (uint32_t)(base + k))
+                split = f_bits(T.thr[bk + 1]);
+            tab2[k] = make_uint2(split, (unsigned)bk * 8u);  // byte offset of the bin
+        }
+        T.tab.swap(tab);
+        T.tab2.swap(tab2);
+        T.tab_sh = sh;
+        T.tab_base = base << sh;  // bit pattern where bucket 0 starts
+        return T;
+    }
+    return T;
 }
 
 // Scratch of the cell build.

@@ -323,6 +323,119 @@ __global__ __launch_bounds__(kBlock) void rmsd_backward_kernel(
     }
 }
 
+// ---------------------------------------------------------------------------
+// Batched RMSD to frame 0 (the RMSD half of an uneven batch; RmsdPlan with
+// reference = frame0 and align = true, rmsd/plan.rs:336-354, per trajectory).
+// The reference moments of a trajectory are its frame 0's own moments, so
+// no separate reference pass is needed.
+// ---------------------------------------------------------------------------
+// Index t in [0, n) with key(t) <= x < key(t + 1) (keys non-decreasing,
+// key(0) <= x); trajectories without units share their successor's key and
+// are never returned.
+template <class K>
+__device__ __forceinline__ int find_last_le(int n, long long x, K key) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (key(mid) <= x) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(kBlock) void batch_rmsd_moments_kernel(
+    const BatchRmsdTraj *__restrict__ traj, int n_traj, long long n_units,
+    const float *__restrict__ store, double *__restrict__ partials) {
+    const int lane = threadIdx.x & 63;
+    const long long unit = (long long)blockIdx.x * (kBlock / 64) +
+                           __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (unit >= n_units) return;
+    const int t = find_last_le(n_traj, unit, [&](int i) { return traj[i].unit0; });
+    const BatchRmsdTraj T = traj[t];
+    const long long local = unit - T.unit0;
+    const int f = (int)(local / T.tiles);
+    const int tile = (int)(local - (long long)f * T.tiles);
+    const int n_groups = T.stride / 4;
+    const int g0 = tile * T.gpt;
+    const int g1 = min(n_groups, g0 + T.gpt);
+    const float *F0 = store + T.store_off;
+    const float *Fx = F0 + (long long)f * 3 * T.stride;
+    const float4 *X = (const float4 *)Fx;
+    const float4 *Y = (const float4 *)(Fx + T.stride);
+    const float4 *Z = (const float4 *)(Fx + 2 * T.stride);
+    const float4 *RX = (const float4 *)F0;
+    const float4 *RY = (const float4 *)(F0 + T.stride);
+    const float4 *RZ = (const float4 *)(F0 + 2 * T.stride);
+    double A[kMom];
+#pragma unroll
+    for (int k = 0; k < kMom; ++k) A[k] = 0.0;
+    int g = g0 + lane;
+    for (; g + 64 < g1; g += 128) {
+        float4 x0 = X[g], y0 = Y[g], z0 = Z[g];
+        float4 x1 = X[g + 64], y1 = Y[g + 64], z1 = Z[g + 64];
+        float4 a0 = RX[g], b0 = RY[g], c0 = RZ[g];
+        float4 a1 = RX[g + 64], b1 = RY[g + 64], c1 = RZ[g + 64];
+        acc_group<MODE_SAME>(A, nullptr, 0, x0, y0, z0, a0, b0, c0);
+        acc_group<MODE_SAME>(A, nullptr, 0, x1, y1, z1, a1, b1, c1);
+    }
+    if (g < g1) {
+        float4 x0 = X[g], y0 = Y[g], z0 = Z[g];
+        float4 a0 = RX[g], b0 = RY[g], c0 = RZ[g];
+        acc_group<MODE_SAME>(A, nullptr, 0, x0, y0, z0, a0, b0, c0);
+    }
+    double *out = partials + unit * kMom;
+#pragma unroll
+    for (int k = 0; k < kMom; ++k) {
+        const double v = wave_sum(A[k]);
+        if (lane == 0) out[k] = v;
+    }
+}
+
+__global__ void batch_rmsd_finalize_kernel(const BatchRmsdTraj *__restrict__ traj, int n_traj,
+                                           long long n_out, const double *__restrict__ partials,
+                                           float *__restrict__ out32, double *__restrict__ out64,
+                                           double *__restrict__ err) {
+    const long long o = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= n_out) return;
+    const int t = find_last_le(n_traj, o, [&](int i) { return traj[i].out0; });
+    const BatchRmsdTraj T = traj[t];
+    const int f = (int)(o - T.out0);
+    double S[kMom], R0[kMom];
+    for (int k = 0; k < kMom; ++k) S[k] = R0[k] = 0.0;
+    for (int tile = 0; tile < T.tiles; ++tile) {
+        const double *p = partials + (T.unit0 + (long long)f * T.tiles + tile) * kMom;
+        const double *q = partials + (T.unit0 + tile) * kMom;  // frame 0: x = y
+        for (int k = 0; k < kMom; ++k) {
+            S[k] += p[k];
+            R0[k] += q[k];
+        }
+    }
+    const double n = (double)T.n;
+    double rmsd = 0.0, e = 0.0;
+    if (T.n > 0) {
+        double cx[3], cy[3];
+        for (int d = 0; d < 3; ++d) {
+            cx[d] = S[d] / n;
+            cy[d] = R0[d] / n;
+        }
+        double M[9], gx, gy;
+        centred(S, R0, R0[12], n, cx, cy, M, &gx, &gy);
+        QcpResult qr;
+        qcp_solve(M, gx, gy, false, &qr);
+        const double resid = gx + gy - 2.0 * qr.lambda;
+        const double mag = S[12] + R0[12] + 2.0 * fabs(qr.lambda);
+        const double lane_terms = 4.0 * ceil((double)T.gpt / 64.0) + 6.0;
+        const double growth = (lane_terms + 8.0 + T.tiles) * 1.1102230246251565e-16;
+        const double r2 = resid / n;
+        rmsd = r2 > 0.0 ? sqrt(r2) : 0.0;
+        const double de = growth * mag / n;
+        e = de / (rmsd + sqrt(de));
+    }
+    out32[o] = (float)rmsd;
+    if (out64) out64[o] = rmsd;
+    if (err) err[o] = e;
+}
+
 }  // namespace
 
 struct wmd_rmsd {
@@ -384,6 +497,24 @@ void run_device(wmd_rmsd *h, const float *d_store, size_t stride, size_t n_frame
 }
 
 }  // namespace
+
+namespace wmd {
+
+void batch_rmsd_enqueue(hipStream_t st, const BatchRmsdTraj *d_traj, int n_traj,
+                        long long n_units, long long n_out, const float *d_store,
+                        double *d_partials, float *d_out32, double *d_out64, double *d_err) {
+    if (n_units <= 0 || n_out <= 0) return;
+    const long long blocks = (n_units + (kBlock / 64) - 1) / (kBlock / 64);
+    WMD_REQUIRE(blocks < (1ll << 31), WMD_ERR_UNSUPPORTED, "too many RMSD units for one launch");
+    hipLaunchKernelGGL(batch_rmsd_moments_kernel, dim3((unsigned)blocks), dim3(kBlock), 0, st, d_traj,
+                       n_traj, n_units, d_store, d_partials);
+    WMD_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(batch_rmsd_finalize_kernel, dim3((unsigned)((n_out + 63) / 64)), dim3(64), 0, st,
+                       d_traj, n_traj, n_out, (const double *)d_partials, d_out32, d_out64, d_err);
+    WMD_HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace wmd
 
 extern "C" {
 

@@ -23,6 +23,10 @@ WMD_ERR_INVALID = 6
 
 PBC_NONE, PBC_ORTHO, PBC_TRICLINIC = 0, 1, 2
 
+BATCH_RDF = 0x1
+BATCH_RMSD = 0x2
+BATCH_NO_GRAPH = 0x4
+
 RDF_EDGE_DIAGNOSTICS = 0x1
 RDF_FORCE_BRUTE = 0x2
 RDF_USE_VOLUME = 0x1
@@ -54,6 +58,9 @@ SYMBOLS = (
     "wmd_rmsd_backward_device", "wmd_rmsd_destroy", "wmd_qcp_host",
     "wmd_softrdf_create", "wmd_softrdf_forward", "wmd_softrdf_backward",
     "wmd_softrdf_last_path", "wmd_softrdf_set_tuning", "wmd_softrdf_destroy",
+    "wmd_batch_create", "wmd_batch_store_layout", "wmd_batch_set_boxes", "wmd_batch_set_tuning",
+    "wmd_batch_autotune", "wmd_batch_run_device", "wmd_batch_read_counts", "wmd_batch_reset",
+    "wmd_batch_plan_info", "wmd_batch_destroy", "wmd_autotune_export", "wmd_autotune_import",
 )
 
 
@@ -141,6 +148,18 @@ def lib():
             "wmd_softrdf_destroy": ([P], None),
             "wmd_qcp_host": ([P, ctypes.c_double, ctypes.c_double, ctypes.c_double, P, P, P,
                               ctypes.POINTER(I)], St),
+            "wmd_batch_create": ([P, S, P, P, S, ctypes.c_float, I, ctypes.c_uint32, PP], St),
+            "wmd_batch_store_layout": ([P, P, P, ctypes.POINTER(ctypes.c_uint64)], St),
+            "wmd_batch_set_boxes": ([P, P], St),
+            "wmd_batch_set_tuning": ([P, P], St),
+            "wmd_batch_autotune": ([P, P, I, P], St),
+            "wmd_batch_run_device": ([P, P, P, P, P, P], St),
+            "wmd_batch_read_counts": ([P, P], St),
+            "wmd_batch_reset": ([P], St),
+            "wmd_batch_plan_info": ([P, P, ctypes.POINTER(ctypes.c_longlong)], St),
+            "wmd_batch_destroy": ([P], None),
+            "wmd_autotune_export": ([ctypes.c_char_p, S, ctypes.POINTER(S)], St),
+            "wmd_autotune_import": ([ctypes.c_char_p], St),
         }
         for name, (args, res) in sig.items():
             fn = getattr(L, name)

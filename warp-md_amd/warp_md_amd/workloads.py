@@ -188,6 +188,24 @@ def uneven_batch(n_traj=256, n_min=500, n_max=50000, n_frames=50, density=0.1, s
     return out
 
 
+def uneven_store_device(batch, spec, seed=3):
+    """Uniform positions in each trajectory's cubic box, drawn on the GPU
+    directly into an UnevenBatch store (padding slots stay zero)."""
+    import torch
+
+    store = batch.new_store()
+    gen = torch.Generator(device=store.device)
+    gen.manual_seed(seed)
+    for t, item in enumerate(spec):
+        n = int(item["n"])
+        F = int(batch.n_frames[t])
+        v = batch.store_view(store, t)
+        L = float(item["box"][0])
+        v[:, :, :n] = torch.rand((F, 3, n), generator=gen, device=store.device,
+                                 dtype=torch.float32) * L
+    return store
+
+
 def beads(n=20000, density=0.08, n_frames=64, jitter=0.2, seed=4):
     rng = np.random.default_rng(seed)
     L = (n / density) ** (1.0 / 3.0)
