@@ -170,6 +170,88 @@ def soft_rdf_ratio(ctx, dev, n=20000, frames=64, reps=3):
             "workload": "20,000 beads, cubic 62.996 A, 64 frames, 300 bins, r_max 12 A, sigma 0.05 A"}
 
 
+def uneven_batch_bench(ctx, dev, reps=3, autotune=True):
+    """256 trajectories, atom counts log-uniform in [500, 50000], cubic boxes at
+    0.1 atoms/A^3, 50 uniform-random frames each; RDF (300 bins, r_max 12 A)
+    and RMSD to frame 0 of every trajectory in one captured-graph step, against
+    the same work as 256 single-trajectory plan calls."""
+    import numpy as np
+    import torch
+
+    from warp_md_amd import workloads
+    from warp_md_amd._native import check, lib, ptr
+    from warp_md_amd.batch import UnevenBatch
+
+    spec = workloads.uneven_batch(sizes_only=True)
+    F = 50
+    ns = np.array([s["n"] for s in spec], np.int64)
+    b = UnevenBatch(ns, [F] * len(spec), bins=300, r_max=12.0, device=dev)
+    b.set_boxes([s["box"] for s in spec])
+    store = workloads.uneven_store_device(b, spec, seed=3)
+    rm = torch.zeros(F * len(spec), dtype=torch.float32, device=dev)
+    torch.cuda.synchronize()
+    tune_s = None
+    if autotune:
+        t0 = time.perf_counter()
+        b.autotune(store, reps=2)
+        torch.cuda.synchronize()
+        tune_s = time.perf_counter() - t0
+    kinds, units = b.plan()
+    b.run(store, rm)  # capture + first replay
+    torch.cuda.synchronize()
+    b.reset()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        b.run(store, rm)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) / reps * 1e3
+    counts = b.counts()
+    pairs_in = float(counts.sum()) / 2.0 / reps  # unordered in-range pairs per step
+    ref_evals = float(np.sum(F * ns * (ns - 1)))
+    # the same work as one plan call per trajectory (RDF accumulate + RMSD run)
+    V = lambda t: ctypes.c_void_p(t.data_ptr())
+    handles = []
+    for t, s in enumerate(spec):
+        n = int(s["n"])
+        sel = np.arange(n, dtype=np.uint32)
+        h = ctypes.c_void_p()
+        check(lib().wmd_rdf_create(ctx.handle, ptr(sel), n, ptr(sel), n, 300, ctypes.c_float(12.0), 1,
+                                   0, ctypes.byref(h)))
+        x0 = b.store_view(store, t)[0, :, :n].t().cpu().numpy()
+        r4 = np.ones((n, 4), np.float32)
+        r4[:, :3] = x0
+        hr = ctypes.c_void_p()
+        check(lib().wmd_rmsd_create(ctx.handle, ptr(sel), n, None, 0, ptr(r4), n, 1, ctypes.byref(hr)))
+        bx = np.repeat(np.diag(s["box"]).reshape(1, 9), F, 0).astype(np.float32)
+        handles.append((h, hr, bx))
+
+    def single_pass():
+        for t, (h, hr, bx) in enumerate(handles):
+            v = b.store_view(store, t)
+            check(lib().wmd_rdf_accumulate_device(h, V(v), v.shape[2], F, ptr(bx), None))
+            check(lib().wmd_rmsd_run_device(hr, V(v), v.shape[2], F, V(rm[t * F:(t + 1) * F]), None,
+                                            None, None, None, None))
+
+    single_pass()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    single_pass()
+    torch.cuda.synchronize()
+    single_ms = (time.perf_counter() - t0) * 1e3
+    for h, hr, _ in handles:
+        lib().wmd_rdf_destroy(h)
+        lib().wmd_rmsd_destroy(hr)
+    log(f"uneven batch: {ms:.2f} ms per graph step ({units} pair units, "
+        f"{int((kinds == 0).sum())} brute-force trajectories), 256 single-plan calls {single_ms:.2f} ms")
+    return {"ms": ms, "pair_evals_per_s": ref_evals / (ms / 1e3),
+            "rmsd_frames_per_s": F * len(spec) / (ms / 1e3),
+            "unordered_pairs_in_range": pairs_in,
+            "single_plan_calls_ms": single_ms, "speedup_vs_single_calls": single_ms / ms,
+            "autotune_s": tune_s, "cell_k_histogram": {str(k): int((kinds == k).sum()) for k in sorted(set(kinds.tolist()))},
+            "workload": "256 trajectories, N log-uniform [500, 50000] (seed 3, sum N = 2,612,540), "
+                        "density 0.1 /A^3, 50 frames, RDF 300 bins r_max 12 A + RMSD to frame 0"}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -180,6 +262,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-triclinic", action="store_true")
     ap.add_argument("--no-soft", action="store_true", help="skip the soft-RDF ratio")
+    ap.add_argument("--no-batch", action="store_true", help="skip the uneven batch")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for "
@@ -302,6 +385,7 @@ def main():
     L.wmd_rdf_destroy(h_rdf)
     L.wmd_rmsd_destroy(h_rmsd)
     soft = None if args.no_soft else soft_rdf_ratio(ctx, dev)
+    batch = None if args.no_batch else uneven_batch_bench(ctx, dev)
     cpu = None if args.no_cpu else cpu_baseline()
 
     line = {
@@ -330,6 +414,7 @@ def main():
         "rdf_triclinic": tric,
         "softrdf_fwd_bwd_over_plain": None if soft is None else soft["ratio"],
         "softrdf": soft,
+        "uneven_batch": batch,
         "roofline": {
             "kernel": "rdf_cell_kernel",
             "bound": "valu",

@@ -197,7 +197,7 @@ __device__ __forceinline__ void brute_unit(const CellGrid &G, int f, int blk, C 
     const int I0 = blk * kBruteHome;
     const int n_home = min(kBruteHome, G.n - I0);
     const float4 *hp = fp + I0;
-    const FrameBox b = G.boxes[f];
+    const FrameBox b = uload(G.boxes, f);
     for (int j0 = I0; j0 < G.n; j0 += 64) {
         const int j = j0 + lane;
         float4 q = make_float4(nanv, nanv, nanv, __uint_as_float(0x11202020u));
@@ -259,7 +259,7 @@ __global__ __launch_bounds__(kBatchBlock) void batch_pair_kernel(BatchPairArgs a
                 if (cur >= 0) wave_hist_flush(L.hist, a.counts + (long long)cur * bins, bins, lane);
                 cur = __builtin_amdgcn_readfirstlane(
                     find_last_le(a.n_traj, u, [&](int i) { return a.traj[i].unit0; }));
-                T = a.traj[cur];
+                T = uload(a.traj, cur);
                 cur_end = T.unit0 + (long long)T.n_frames * T.upf;
                 G = CellGrid{a.sorted + T.af0, a.start + T.start_off, a.frame_flag + T.box0,
                              a.boxes + T.box0, T.n, T.n_frames, T.ncx, T.ncy, T.ncz, T.k,

@@ -194,9 +194,9 @@ struct SoftFwdCons {
                                           unsigned mj, int rank, unsigned, int, const FrameBox &b) {
         float sx, sy, sz;
         soft_shift<MODE, PBC>(mj, b, sx, sy, sz);
-        for (int i = 0; i < n_home; ++i) {
+        auto pair = [&](int i, const float4 &a) {
             float dx, dy, dz;
-            const float s = soft_disp<PBC, LT, MODE>(hp[i], xj, yj, zj, sx, sy, sz, b, dx, dy, dz);
+            const float s = soft_disp<PBC, LT, MODE>(a, xj, yj, zj, sx, sy, sz, b, dx, dy, dz);
             const bool in = s < sp.cut2 && (!HOME || i < rank);
             const unsigned long long bal = __ballot(in);
             if (in) pbuf[(pt + lanes_below(bal)) & 127] = s;
@@ -207,7 +207,18 @@ struct SoftFwdCons {
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
                 drain(64);
             }
+        };
+        // home atoms four at a time (scalar loads)
+        int i = 0;
+        for (; i + 4 <= n_home; i += 4) {
+            const float4 a0 = uload(hp, i), a1 = uload(hp, i + 1), a2 = uload(hp, i + 2),
+                         a3 = uload(hp, i + 3);
+            pair(i, a0);
+            pair(i + 1, a1);
+            pair(i + 2, a2);
+            pair(i + 3, a3);
         }
+        for (; i < n_home; ++i) pair(i, uload(hp, i));
     }
     __device__ __forceinline__ void unit_begin(int, int, int) {}
     __device__ __forceinline__ void unit_end(int, int, int) {}
@@ -247,9 +258,9 @@ struct SoftBwdCons {
         soft_shift<MODE, PBC>(mj, b, sx, sy, sz);
         const int lane = threadIdx.x & 63;
         float gjx = 0.f, gjy = 0.f, gjz = 0.f;
-        for (int i = 0; i < n_home; ++i) {
+        auto pair = [&](int i, const float4 &a) {
             float dx, dy, dz;
-            const float s = soft_disp<PBC, LT, MODE>(hp[i], xj, yj, zj, sx, sy, sz, b, dx, dy, dz);
+            const float s = soft_disp<PBC, LT, MODE>(a, xj, yj, zj, sx, sy, sz, b, dx, dy, dz);
             float gx = 0.f, gy = 0.f, gz = 0.f;
             if (s < sp.cut2 && (!HOME || i < rank) && s > 0.f) {
                 const float fr = soft_force_over_r(s, sp, tab);  // (dL/dr) / r
@@ -270,7 +281,18 @@ struct SoftBwdCons {
                     add_grad(f, (unsigned)(hs + i), -rx, -ry, -rz);
                 }
             }
+        };
+        // home atoms four at a time (scalar loads)
+        int i = 0;
+        for (; i + 4 <= n_home; i += 4) {
+            const float4 a0 = uload(hp, i), a1 = uload(hp, i + 1), a2 = uload(hp, i + 2),
+                         a3 = uload(hp, i + 3);
+            pair(i, a0);
+            pair(i + 1, a1);
+            pair(i + 2, a2);
+            pair(i + 3, a3);
         }
+        for (; i < n_home; ++i) pair(i, uload(hp, i));
         if (jpos != 0xFFFFFFFFu) add_grad(f, jpos, gjx, gjy, gjz);
     }
     __device__ __forceinline__ void unit_end(int f, int, int n_home) {

@@ -251,6 +251,20 @@ __device__ __forceinline__ float pair_s_checked(float xi, float yi, float zi, fl
     }
 }
 
+// Load through the constant address space: a wave-uniform load this way
+// becomes a scalar (SMEM) load even in kernels that also write global memory
+// (where the compiler cannot otherwise prove the data unchanged).  Only for
+// data that no kernel writes while the loading kernel runs.  (The host pass
+// of the compiler parses device functions too; it takes the plain load.)
+template <class T>
+__device__ __forceinline__ T uload(const T *p, long long i) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return ((const __attribute__((address_space(4))) T *)p)[i];
+#else
+    return p[i];
+#endif
+}
+
 // XCD-aware persistent schedule: WG g runs on XCD g % 8 (round-robin dispatch);
 // each XCD owns a contiguous slice of the unit range.
 struct UnitRange {
@@ -652,7 +666,8 @@ __device__ __forceinline__ void cell_chunk(const float4 *__restrict__ hp, int n_
     };
     int i = 0;
     for (; i + 4 <= n_home; i += 4) {
-        const float4 a0 = hp[i], a1 = hp[i + 1], a2 = hp[i + 2], a3 = hp[i + 3];
+        const float4 a0 = uload(hp, i), a1 = uload(hp, i + 1), a2 = uload(hp, i + 2),
+                     a3 = uload(hp, i + 3);
         float s0 = dist(a0), s1 = dist(a1), s2 = dist(a2), s3 = dist(a3);
         if constexpr (HOME) {
             s0 = i >= rank ? nanv : s0;
@@ -666,7 +681,7 @@ __device__ __forceinline__ void cell_chunk(const float4 *__restrict__ hp, int n_
         cell_count<WEIGHTED, EDGE, TABLE>(s3, __float_as_uint(a3.w), mAj, mBj, p, L, edge_local);
     }
     for (; i < n_home; ++i) {
-        const float4 a0 = hp[i];
+        const float4 a0 = uload(hp, i);
         float s0 = dist(a0);
         if constexpr (HOME) s0 = i >= rank ? nanv : s0;
         cell_count<WEIGHTED, EDGE, TABLE>(s0, __float_as_uint(a0.w), mAj, mBj, p, L, edge_local);
@@ -748,16 +763,16 @@ __device__ __forceinline__ void cell_unit(const CellGrid &G, int f, int h, const
         const int hz = h / (ncx * ncy);
         const unsigned *fs = G.start + (long long)f * (ncell + 1);
         const float4 *fp = G.sorted + (long long)f * G.n;
-        const int hs = (int)fs[h];
-        const int n_home = (int)fs[h + 1] - hs;
+        const int hs = (int)uload(fs, h);
+        const int n_home = (int)uload(fs, h + 1) - hs;
         if (n_home == 0) return;
         const float4 *hp = fp + hs;
-        const FrameBox b = G.boxes[f];
+        const FrameBox b = uload(G.boxes, f);
         cons.unit_begin(f, hs, n_home);
 
         // home atoms: bounding box and image uniformity (the box is only used
         // on the shift path, whose frames hold finite coordinates)
-        const unsigned m0 = __float_as_uint(hp[0].w) & 0xFFFFFFu;
+        const unsigned m0 = __float_as_uint(uload(hp, 0).w) & 0xFFFFFFu;
         int kn0 = 0x7FFFFFFF, kn1 = 0x7FFFFFFF, kn2 = 0x7FFFFFFF;
         int kx0 = (int)0x80000000, kx1 = (int)0x80000000, kx2 = (int)0x80000000;
         bool diff = false;
@@ -771,7 +786,7 @@ __device__ __forceinline__ void cell_unit(const CellGrid &G, int f, int h, const
                 diff |= (__float_as_uint(q.w) & 0xFFFFFFu) != m0;
             }
         }
-        const bool use_shift = G.shift_ok && G.frame_flag[f] == 0u && !__any(diff);
+        const bool use_shift = G.shift_ok && uload(G.frame_flag, f) == 0u && !__any(diff);
         const float mn0 = wave_min_key(kn0), mn1 = wave_min_key(kn1), mn2 = wave_min_key(kn2);
         const float mx0 = wave_max_key(kx0), mx1 = wave_max_key(kx1), mx2 = wave_max_key(kx2);
         // per-piece byte constants K: n + 128 = (m_j + 32) + K per byte, with
