@@ -136,3 +136,56 @@ def test_fit_rmsd_zero_for_rigid_transform():
     if np.linalg.det(R) < 0:
         R[:, 0] *= -1
     assert S.fit_rmsd_parts(y @ R.T + 3.0, y)[0] < 1e-12
+
+
+def _torch_loss(x, bins, r_max, sigma, L):
+    """The soft-RDF loss as dense float64 torch operations (autograd gives dL/dx).
+
+    Independent of the oracle's hand-written adjoint: pairs by broadcasting,
+    round-half-away minimum image, B-spline weights scattered onto the nodes
+    with index_add, the node-to-bin map and the normalisation from the
+    definitions."""
+    import torch
+
+    F, n, _ = x.shape
+    w, h, n_nodes, G = S.geometry(bins, r_max, sigma)
+    Gt = torch.as_tensor(G)
+    iu, ju = np.triu_indices(n, k=1)
+    iu = torch.as_tensor(iu)
+    ju = torch.as_tensor(ju)
+    H = torch.zeros(n_nodes + 4, dtype=torch.float64)
+    for f in range(F):
+        d = x[f, ju] - x[f, iu]
+        q = d / L
+        d = d - torch.sign(q) * torch.floor(torch.abs(q) + 0.5) * L
+        r = torch.sqrt((d * d).sum(1))
+        u = r / h
+        n0 = torch.round(u).detach()  # ties have measure zero
+        t = u - n0
+        idx = n0.long()
+        ok = idx + 1 < n_nodes + 3
+        wts = (0.5 * (0.5 - t) ** 2, 0.75 - t * t, 0.5 * (0.5 + t) ** 2)
+        for o in range(3):
+            H = H.index_add(0, idx[ok] - 1 + o + 1, 2.0 * wts[o][ok])
+    c = Gt @ H[1:n_nodes + 1]
+    norm = torch.as_tensor(S.normalisation(n, F, bins, r_max, [L ** 3] * F))
+    rk = torch.as_tensor((np.arange(bins) + 0.5) * w)
+    gt = 1.0 + 0.3 * torch.exp(-(rk - 4.0) ** 2) * torch.cos(2.0 * rk)
+    return ((c / norm - gt) ** 2).sum()
+
+
+def test_oracle_gradient_matches_dense_torch_autograd():
+    import torch
+
+    n, L, frames = 60, 8.0, 2
+    bins, r_max, sigma = 80, 3.9, 0.06
+    x = _beads(n, L, frames, seed=21)
+    boxes = [np.array([L, L, L])] * frames
+    loss, dldc = S.rdf_loss(x, bins, r_max, sigma, [L ** 3] * frames, "orthorhombic", boxes)
+    g = S.soft_grad(x, bins, r_max, sigma, dldc, "orthorhombic", boxes)
+    xt = torch.tensor(x, dtype=torch.float64, requires_grad=True)
+    lt = _torch_loss(xt, bins, r_max, sigma, L)
+    lt.backward()
+    assert float(lt) == pytest.approx(loss, rel=1e-12)
+    gt = xt.grad.numpy()
+    assert np.max(np.abs(gt - g)) <= 1e-10 * np.max(np.abs(g))

@@ -75,19 +75,23 @@ namespace {
 // atomics); a small kernel applies the map.  The backward pass tabulates
 // D_m = sum_k dL/dc_k w phi(r_k - m h) on the nodes, so that one unordered
 // pair's dL/dr = 2 sum_m D_m B'(r / h - m) / h, and enumerates the pairs again
-// rather than storing them.
+// rather than storing them.  B' is linear on each interval |u - n0| <= 1/2, so
+// dL/dr there is A_n0 + t S_n0 (t = u - n0) with
+//   A_n = (D_{n+1} - D_{n-1}) / h,  S_n = 2 (D_{n-1} - 2 D_n + D_{n+1}) / h,
+// one (A, S) table entry per node.
 // -------------------------------------------------------------------------
 constexpr int kSoftNodesPerBin = 4;   // h = w / 4
 constexpr int kSoftPadL = 1;          // node padding below node 0
 constexpr int kSoftPadR = 3;          // and above the last node
 constexpr int kSoftMaxHome = 64;      // home atoms with an LDS gradient slot per wave
+constexpr int kSoftJCopies = 1;       // copies of the j gradient sums (power of two)
 
 struct SoftParams {
     int n_nodes;        // nodes 0 .. n_nodes - 1 carry weight
     float inv_h;        // 1 / node spacing
     float cut2;         // squared pair reach
     double *nodes;      // [n_nodes] H (forward output, accumulated)
-    const float *table; // [n_nodes] D (backward)
+    const float2 *table; // padded (A, S) per node (backward)
     float *grad;        // brute force: [F][3][stride] output; cell list: [F][3][n] in cell order
     long long grad_frame0;  // frame offset of this batch in grad (brute force)
     int stride;
@@ -115,16 +119,14 @@ __device__ __forceinline__ void soft_deposit(float s, const SoftParams &sp, doub
     atomicAdd(p + 1, (double)(c * c));              // 2 x (1/2 + t)^2 / 2
 }
 
-// One unordered pair's dL/dr divided by r, from the padded node table.
-__device__ __forceinline__ float soft_force_over_r(float s, const SoftParams &sp, const float *tab) {
+// One unordered pair's dL/dr divided by r, from the padded (A, S) table.
+__device__ __forceinline__ float soft_force_over_r(float s, const SoftParams &sp, const float2 *tab) {
     const float rinv = __builtin_amdgcn_rsqf(s);
     const float u = s * rinv * sp.inv_h;
     float t;
     const int n0 = bspline2(u, sp.n_nodes, t);
-    const float *q = tab + kSoftPadL + n0;
-    // 2 (D_{n0-1} (t - 1/2) - 2 t D_{n0} + D_{n0+1} (t + 1/2)) / h
-    const float du = (q[-1] * (t - 0.5f) - 2.0f * t * q[0] + q[1] * (t + 0.5f)) * 2.0f * sp.inv_h;
-    return du * rinv;
+    const float2 as = tab[kSoftPadL + n0];
+    return __builtin_fmaf(t, as.y, as.x) * rinv;
 }
 
 // Minimum-image displacement x_j - x_i of the cell stream's pair (MODE as in
@@ -224,30 +226,95 @@ struct SoftFwdCons {
     __device__ __forceinline__ void unit_end(int, int, int) {}
 };
 
-// Backward consumer: dL/dx_j accumulates in the j lane over the home atoms and
-// is added once per chunk; dL/dx_i is summed over the 16 lanes of each row by
-// DPP and collected per home atom in LDS until the unit ends (home atoms
-// beyond kSoftMaxHome add their row sums directly).  The output is indexed by
-// cell-sorted position, where a chunk's lanes mostly hold consecutive atoms,
-// so the atomics of one instruction fall on a few cache lines.
+// Backward consumer.  For every home atom i of a chunk the in-range pairs are
+// compacted into a per-wave buffer (displacement, s, and i, j lane, rank in
+// i's run); 64 buffered pairs at a time then evaluate dL/dr with every lane
+// busy.  A pair's gradient g goes to the chunk's j slot in LDS (+g) and to
+// home atom i (-g).  The buffer holds each i's pairs as one contiguous run,
+// so i's share of a batch is a difference of two wave prefix sums, taken at
+// the run's last lane (one LDS add per run instead of one per pair on the
+// same address).  The j adds of one batch come from several runs that
+// mostly hit the same j slots; LDS float adds of one instruction to one
+// address are very slow, so run q of a batch adds into copy q mod
+// kSoftJCopies of the j sums (one run never repeats a j slot).  At the end of a chunk the buffer is drained and each j lane
+// adds its accumulated gradient to the output once; home atoms beyond
+// kSoftMaxHome add theirs directly.  The output is indexed by cell-sorted
+// position, where a chunk's lanes mostly hold consecutive atoms, so the
+// atomics of one instruction fall on a few cache lines.
+__device__ __forceinline__ float wave_prefix_sum(float v) {
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x111, 0xF, 0xF, false));  // row_shr:1
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x112, 0xF, 0xF, false));  // row_shr:2
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x114, 0xF, 0xF, false));  // row_shr:4
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x118, 0xF, 0xF, false));  // row_shr:8
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x142, 0xA, 0xF, false));  // row_bcast:15
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x143, 0xC, 0xF, false));  // row_bcast:31
+    return v;
+}
+
 template <int PBC, bool LT>
 struct SoftBwdCons {
     SoftParams sp;
-    const float *tab;   // LDS node table (padded)
-    float4 *gacc;       // this wave's [kSoftMaxHome] home-atom gradients
+    const float2 *tab;  // LDS (A, S) table (padded)
+    double *gi;         // this wave's [3][kSoftMaxHome] home-atom gradient sums
+    double *gj;         // this wave's [kSoftJCopies][3][64] gradient sums of the chunk's j atoms
+    float4 *cbuf;       // this wave's [128] buffered pairs (dx, dy, dz, s)
+    unsigned *cid;      // and their i | j lane << 20 | rank in i's run << 26
     int hs;             // the current unit's first home position
-    __device__ __forceinline__ static float row_sum(float v) {
-        v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0xB1, 0xF, 0xF, false));
-        v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x4E, 0xF, 0xF, false));
-        v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x141, 0xF, 0xF, false));
-        v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x140, 0xF, 0xF, false));
-        return v;
-    }
+    int ph, pt;         // buffer head / tail (uniform)
     __device__ __forceinline__ void add_grad(int f, unsigned pos, float gx, float gy, float gz) {
         float *g = sp.grad + (long long)f * 3 * sp.n + pos;
         atomicAdd(g, gx);
         atomicAdd(g + sp.n, gy);
         atomicAdd(g + 2 * sp.n, gz);
+    }
+    __device__ __forceinline__ static void wave_sync() {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    // Evaluates cnt (<= 64) buffered pairs.
+    __device__ __forceinline__ void drain(int cnt, int f) {
+        const int lane = threadIdx.x & 63;
+        const bool act = lane < cnt;
+        float gx = 0.f, gy = 0.f, gz = 0.f;
+        unsigned id = act ? cid[(ph + lane) & 127] : 0u;
+        // index of this lane's run within the batch (runs start at lane 0 and where rank = 0)
+        const unsigned long long starts = __ballot(act && (lane == 0 || (id >> 26) == 0u));
+        const int run = (int)__popcll(starts & ((2ull << lane) - 1ull)) - 1;
+        double *gjc = gj + (run & (kSoftJCopies - 1)) * 192;
+        if (act) {
+            const int slot = (ph + lane) & 127;
+            const float4 e = cbuf[slot];
+            const float fr = soft_force_over_r(e.w, sp, tab);  // (dL/dr) / r
+            gx = fr * e.x;
+            gy = fr * e.y;
+            gz = fr * e.z;
+            const unsigned jl = (id >> 20) & 63u;
+            atomicAdd(&gjc[jl], (double)gx);
+            atomicAdd(&gjc[64 + jl], (double)gy);
+            atomicAdd(&gjc[128 + jl], (double)gz);
+        }
+        // i's share: prefix sums at the run's last lane minus those before its first
+        const float px = wave_prefix_sum(gx), py = wave_prefix_sum(gy), pz = wave_prefix_sum(gz);
+        const int r = (int)(id >> 26);
+        const int first = lane - r;  // < 0 when the run began in an earlier batch
+        const int nr = __shfl_down((int)(id >> 26), 1, 64);
+        const int src = first > 0 ? first - 1 : lane;
+        const float bx = __shfl(px, src, 64), by = __shfl(py, src, 64), bz = __shfl(pz, src, 64);
+        if (act && (lane == cnt - 1 || nr == 0)) {
+            const float sx = first > 0 ? px - bx : px;
+            const float sy = first > 0 ? py - by : py;
+            const float sz = first > 0 ? pz - bz : pz;
+            const unsigned i = id & 0xFFFFFu;
+            if (i < (unsigned)kSoftMaxHome) {
+                atomicAdd(&gi[i], (double)sx);
+                atomicAdd(&gi[kSoftMaxHome + i], (double)sy);
+                atomicAdd(&gi[2 * kSoftMaxHome + i], (double)sz);
+            } else {
+                add_grad(f, (unsigned)hs + i, -sx, -sy, -sz);
+            }
+        }
+        ph += cnt;
     }
     __device__ __forceinline__ void unit_begin(int, int h0, int) { hs = h0; }
     template <int MODE, bool HOME>
@@ -257,28 +324,22 @@ struct SoftBwdCons {
         float sx, sy, sz;
         soft_shift<MODE, PBC>(mj, b, sx, sy, sz);
         const int lane = threadIdx.x & 63;
-        float gjx = 0.f, gjy = 0.f, gjz = 0.f;
         auto pair = [&](int i, const float4 &a) {
             float dx, dy, dz;
             const float s = soft_disp<PBC, LT, MODE>(a, xj, yj, zj, sx, sy, sz, b, dx, dy, dz);
-            float gx = 0.f, gy = 0.f, gz = 0.f;
-            if (s < sp.cut2 && (!HOME || i < rank) && s > 0.f) {
-                const float fr = soft_force_over_r(s, sp, tab);  // (dL/dr) / r
-                gx = fr * dx;
-                gy = fr * dy;
-                gz = fr * dz;
-            }
-            gjx += gx;
-            gjy += gy;
-            gjz += gz;
-            const float rx = row_sum(gx), ry = row_sum(gy), rz = row_sum(gz);
-            if ((lane & 15) == 0) {
-                if (i < kSoftMaxHome) {
-                    atomicAdd(&gacc[i].x, -rx);
-                    atomicAdd(&gacc[i].y, -ry);
-                    atomicAdd(&gacc[i].z, -rz);
-                } else {
-                    add_grad(f, (unsigned)(hs + i), -rx, -ry, -rz);
+            const bool in = s < sp.cut2 && (!HOME || i < rank) && s > 0.f;
+            const unsigned long long bal = __ballot(in);
+            if (bal) {
+                if (in) {
+                    const int r = lanes_below(bal);
+                    const int slot = (pt + r) & 127;
+                    cbuf[slot] = make_float4(dx, dy, dz, s);
+                    cid[slot] = (unsigned)i | ((unsigned)lane << 20) | ((unsigned)r << 26);
+                }
+                pt += (int)__popcll(bal);
+                if (pt - ph >= 64) {
+                    wave_sync();
+                    drain(64, f);
                 }
             }
         };
@@ -293,21 +354,37 @@ struct SoftBwdCons {
             pair(i + 3, a3);
         }
         for (; i < n_home; ++i) pair(i, uload(hp, i));
-        if (jpos != 0xFFFFFFFFu) add_grad(f, jpos, gjx, gjy, gjz);
+        // the chunk's remaining pairs, then its j gradients
+        wave_sync();
+        if (pt > ph) drain(pt - ph, f);
+        wave_sync();
+        double tx = 0.0, ty = 0.0, tz = 0.0;
+#pragma unroll
+        for (int c = 0; c < kSoftJCopies; ++c) {
+            double *q = gj + c * 192;
+            tx += q[lane];
+            ty += q[64 + lane];
+            tz += q[128 + lane];
+            q[lane] = 0.0;
+            q[64 + lane] = 0.0;
+            q[128 + lane] = 0.0;
+        }
+        const float gx = (float)tx, gy = (float)ty, gz = (float)tz;
+        if (jpos != 0xFFFFFFFFu && (gx != 0.f || gy != 0.f || gz != 0.f)) add_grad(f, jpos, gx, gy, gz);
+        wave_sync();
     }
     __device__ __forceinline__ void unit_end(int f, int, int n_home) {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        wave_sync();
         const int lane = threadIdx.x & 63;
         if (lane < min(n_home, kSoftMaxHome)) {
-            const float4 g = gacc[lane];
-            add_grad(f, (unsigned)(hs + lane), g.x, g.y, g.z);
-            gacc[lane] = make_float4(0.f, 0.f, 0.f, 0.f);
+            const float gx = (float)gi[lane], gy = (float)gi[kSoftMaxHome + lane],
+                        gz = (float)gi[2 * kSoftMaxHome + lane];
+            add_grad(f, (unsigned)(hs + lane), -gx, -gy, -gz);
+            gi[lane] = 0.0;
+            gi[kSoftMaxHome + lane] = 0.0;
+            gi[2 * kSoftMaxHome + lane] = 0.0;
         }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        wave_sync();
     }
 };
 
@@ -356,20 +433,36 @@ __global__ __launch_bounds__(kSoftBlock) void soft_cell_fwd_kernel(CellGrid G, S
     }
 }
 
+constexpr int kSoftBwdBlock = 512;  // 8 waves (two workgroups per CU within the LDS budget)
+constexpr int kSoftBwdWaves = kSoftBwdBlock / 64;
+
+// LDS of the backward cell kernel: (A, S) table | per wave: cbuf[128] float4,
+// ring[kRing] float4, gj[kSoftJCopies][3][64] and gi[3][kSoftMaxHome] f64,
+// cid[128], ring positions[kRing].
+__host__ __device__ inline size_t soft_bwd_lds_bytes(int tn) {
+    return align16((size_t)tn * 8) +
+           (size_t)kSoftBwdWaves * ((128 + kRing) * 16 + (kSoftJCopies * 192 + 3 * kSoftMaxHome) * 8 +
+                                    128 * 4 + kRing * 4);
+}
+
 template <int PBC, bool LT>
-__global__ __launch_bounds__(kSoftBlock) void soft_cell_bwd_kernel(CellGrid G, SoftParams sp) {
+__global__ __launch_bounds__(kSoftBwdBlock) void soft_cell_bwd_kernel(CellGrid G, SoftParams sp) {
     extern __shared__ __attribute__((aligned(16))) char lds[];
     const int tn = sp.n_nodes + kSoftPadL + kSoftPadR;
-    float *tab = (float *)lds;
-    float4 *gaccs = (float4 *)(lds + align16((size_t)tn * 4));
-    unsigned *ringps = (unsigned *)(gaccs + kSoftWaves * kSoftMaxHome);
-    float4 *rings = (float4 *)(ringps + kSoftWaves * kRing);
-    for (int i = threadIdx.x; i < tn; i += kSoftBlock) tab[i] = sp.table[i];
-    for (int i = threadIdx.x; i < kSoftWaves * kSoftMaxHome; i += kSoftBlock)
-        gaccs[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    float2 *tab = (float2 *)lds;
+    float4 *cbufs = (float4 *)(lds + align16((size_t)tn * 8));
+    float4 *rings = cbufs + kSoftBwdWaves * 128;
+    double *gjs = (double *)(rings + kSoftBwdWaves * kRing);
+    double *gis = gjs + kSoftBwdWaves * kSoftJCopies * 192;
+    unsigned *cids = (unsigned *)(gis + kSoftBwdWaves * 3 * kSoftMaxHome);
+    unsigned *ringps = cids + kSoftBwdWaves * 128;
+    for (int i = threadIdx.x; i < tn; i += kSoftBwdBlock) tab[i] = sp.table[i];
+    for (int i = threadIdx.x; i < kSoftBwdWaves * (kSoftJCopies * 192 + 3 * kSoftMaxHome); i += kSoftBwdBlock)
+        gjs[i] = 0.0;  // gj and gi are contiguous
     __syncthreads();
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    SoftBwdCons<PBC, LT> cons{sp, tab, gaccs + wave * kSoftMaxHome, 0};
+    SoftBwdCons<PBC, LT> cons{sp, tab, gis + wave * 3 * kSoftMaxHome, gjs + wave * kSoftJCopies * 192, cbufs + wave * 128,
+                              cids + wave * 128, 0, 0, 0};
     cell_waves<PBC, true>(G, rings + wave * kRing, ringps + wave * kRing, wave, cons);
 }
 
@@ -437,7 +530,7 @@ __global__ __launch_bounds__(kBlock) void soft_brute_bwd_kernel(const float *__r
                                                                 SoftParams sp) {
     extern __shared__ __attribute__((aligned(16))) char lds[];
     const int tn = sp.n_nodes + kSoftPadL + kSoftPadR;
-    float *tab = (float *)lds;
+    float2 *tab = (float2 *)lds;
     for (int t = threadIdx.x; t < tn; t += kBlock) tab[t] = sp.table[t];
     __syncthreads();
     const int f = blockIdx.y;
@@ -483,13 +576,13 @@ __global__ void soft_conv_kernel(const double *__restrict__ nodes, int n_nodes, 
 }
 
 // D_m = sum over |m h - r_k| <= 6 sigma of dL/dc_k w phi(r_k - m h) into the
-// padded table (zero in the padding), one thread per table entry.
+// padded node array (zero in the padding), one thread per entry.
 __global__ void soft_node_table_kernel(const double *__restrict__ dldc, int bins, double w, double h,
-                                       double sigma, int n_nodes, float *__restrict__ table) {
+                                       double sigma, int n_nodes, double *__restrict__ D) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= n_nodes + kSoftPadL + kSoftPadR) return;
     const int m = e - kSoftPadL;
-    double D = 0.0;
+    double v = 0.0;
     if (m >= 0 && m < n_nodes) {
         const double rm = m * h;
         const double reach = 6.0 * sigma;
@@ -498,10 +591,20 @@ __global__ void soft_node_table_kernel(const double *__restrict__ dldc, int bins
         const double norm = w / (sigma * 2.5066282746310002);
         for (int k = k0; k <= k1; ++k) {
             const double d = (k + 0.5) * w - rm;
-            if (fabs(d) <= reach) D += dldc[k] * norm * exp(-0.5 * d * d / (sigma * sigma));
+            if (fabs(d) <= reach) v += dldc[k] * norm * exp(-0.5 * d * d / (sigma * sigma));
         }
     }
-    table[e] = (float)D;
+    D[e] = v;
+}
+
+// (A, S) per padded node from D (entries outside the array count as zero).
+__global__ void soft_slope_kernel(const double *__restrict__ D, int tn, double inv_h,
+                                  float2 *__restrict__ table) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= tn) return;
+    const double dm = e > 0 ? D[e - 1] : 0.0;
+    const double dp = e + 1 < tn ? D[e + 1] : 0.0;
+    table[e] = make_float2((float)((dp - dm) * inv_h), (float)(2.0 * (dm - 2.0 * D[e] + dp) * inv_h));
 }
 
 // AoS [F][n_src][4] -> SoA [F][3][stride] through a gather list (kPad -> 0).
@@ -1040,18 +1143,19 @@ SoftPlanOut soft_plan(wmd_softrdf *h, const BoxSet &B, int n) {
 }
 
 template <class K>
-void launch_soft_cells(wmd_softrdf *h, hipStream_t st, K kern, size_t lds, const CellGrid &G,
+void launch_soft_cells(wmd_softrdf *h, hipStream_t st, K kern, int block, size_t lds, const CellGrid &G,
                        const SoftParams &sp, const char *name) {
     ProfScope prof(h->ctx, st, name);
     WMD_REQUIRE(lds <= h->ctx->lds_per_block, WMD_ERR_UNSUPPORTED, "too many bins for LDS");
     int occ = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, kSoftBlock, lds) != hipSuccess || occ < 1)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, block, lds) != hipSuccess || occ < 1)
         occ = 1;
     const long long n_units = (long long)G.n_frames * G.ncx * G.ncy * G.ncz;
+    const int waves = block / 64;
     int grid = std::max(8, h->ctx->cu_count * occ / 8 * 8);
-    const long long max_grid = (n_units + kSoftWaves * 8 - 1) / (kSoftWaves * 8) * 8;
+    const long long max_grid = (n_units + waves * 8 - 1) / (waves * 8) * 8;
     if (grid > max_grid) grid = (int)std::max<long long>(8, max_grid);
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(kSoftBlock), lds, st, G, sp);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(block), lds, st, G, sp);
     WMD_HIP_CHECK(hipGetLastError());
 }
 
@@ -1062,7 +1166,7 @@ void soft_run(wmd_softrdf *h, bool backward, const float *d_soa, size_t stride, 
     WMD_REQUIRE(d_soa != nullptr, WMD_ERR_INVALID, "null device store");
     WMD_REQUIRE(stride >= n_atoms && stride % 4 == 0, WMD_ERR_INVALID,
                 "stride must be a multiple of 4 and >= the atom count");
-    WMD_REQUIRE(n_atoms < (1u << 24), WMD_ERR_UNSUPPORTED, "at most 2^24 atoms per frame");
+    WMD_REQUIRE(n_atoms < (1u << 20), WMD_ERR_UNSUPPORTED, "at most 2^20 atoms per frame");
     WMD_REQUIRE(h->pbc == WMD_PBC_NONE || boxes_host, WMD_ERR_INVALID, "boxes required for PBC");
     const int n = (int)n_atoms;
     const BoxSet B = make_boxes(h->pbc, boxes_host, n_frames);
@@ -1073,9 +1177,12 @@ void soft_run(wmd_softrdf *h, bool backward, const float *d_soa, size_t stride, 
     sp.n = n;
     const int tn = sp.n_nodes + kSoftPadL + kSoftPadR;
     if (backward) {
-        float *tab = (float *)h->d_table.reserve((size_t)tn * 4);
+        double *D = (double *)h->d_nodes.reserve((size_t)tn * 8);
+        float2 *tab = (float2 *)h->d_table.reserve((size_t)tn * 8);
         hipLaunchKernelGGL(soft_node_table_kernel, dim3((tn + 255) / 256), dim3(256), 0, st, d_dc,
-                           h->bins, (double)h->w, soft_h(h), (double)h->sigma, sp.n_nodes, tab);
+                           h->bins, (double)h->w, soft_h(h), (double)h->sigma, sp.n_nodes, D);
+        hipLaunchKernelGGL(soft_slope_kernel, dim3((tn + 255) / 256), dim3(256), 0, st,
+                           (const double *)D, tn, 1.0 / soft_h(h), tab);
         WMD_HIP_CHECK(hipGetLastError());
         sp.table = tab;
     } else {
@@ -1115,7 +1222,7 @@ void soft_run(wmd_softrdf *h, bool backward, const float *d_soa, size_t stride, 
                 sp.grad_frame0 = 0;
                 SoftParams spb = sp;
                 spb.grad = d_grad + f0 * 3 * stride;
-                const size_t lds = (size_t)tn * 4;
+                const size_t lds = (size_t)tn * 8;
                 WMD_REQUIRE(lds <= h->ctx->lds_per_block, WMD_ERR_UNSUPPORTED, "too many bins for LDS");
                 if (h->pbc == WMD_PBC_NONE)
                     hipLaunchKernelGGL(soft_brute_bwd_kernel<WMD_PBC_NONE>, grid, dim3(kBlock), lds, st, soa, (int)stride, n, bx, spb);
@@ -1155,20 +1262,19 @@ void soft_run(wmd_softrdf *h, bool backward, const float *d_soa, size_t stride, 
             const size_t lds = align16((size_t)tn * 8) + (size_t)kSoftWaves * 128 * 4 +
                                (size_t)kSoftWaves * kRing * 16;
             if (h->pbc == WMD_PBC_ORTHO)
-                launch_soft_cells(h, st, soft_cell_fwd_kernel<WMD_PBC_ORTHO, false>, lds, G, sp, "softrdf_fwd");
+                launch_soft_cells(h, st, soft_cell_fwd_kernel<WMD_PBC_ORTHO, false>, kSoftBlock, lds, G, sp, "softrdf_fwd");
             else if (pl.lt)
-                launch_soft_cells(h, st, soft_cell_fwd_kernel<WMD_PBC_TRICLINIC, true>, lds, G, sp, "softrdf_fwd");
+                launch_soft_cells(h, st, soft_cell_fwd_kernel<WMD_PBC_TRICLINIC, true>, kSoftBlock, lds, G, sp, "softrdf_fwd");
             else
-                launch_soft_cells(h, st, soft_cell_fwd_kernel<WMD_PBC_TRICLINIC, false>, lds, G, sp, "softrdf_fwd");
+                launch_soft_cells(h, st, soft_cell_fwd_kernel<WMD_PBC_TRICLINIC, false>, kSoftBlock, lds, G, sp, "softrdf_fwd");
         } else {
-            const size_t lds = align16((size_t)tn * 4) + (size_t)kSoftWaves * kSoftMaxHome * 16 +
-                               (size_t)kSoftWaves * kRing * 4 + (size_t)kSoftWaves * kRing * 16;
+            const size_t lds = soft_bwd_lds_bytes(tn);
             if (h->pbc == WMD_PBC_ORTHO)
-                launch_soft_cells(h, st, soft_cell_bwd_kernel<WMD_PBC_ORTHO, false>, lds, G, sp, "softrdf_bwd");
+                launch_soft_cells(h, st, soft_cell_bwd_kernel<WMD_PBC_ORTHO, false>, kSoftBwdBlock, lds, G, sp, "softrdf_bwd");
             else if (pl.lt)
-                launch_soft_cells(h, st, soft_cell_bwd_kernel<WMD_PBC_TRICLINIC, true>, lds, G, sp, "softrdf_bwd");
+                launch_soft_cells(h, st, soft_cell_bwd_kernel<WMD_PBC_TRICLINIC, true>, kSoftBwdBlock, lds, G, sp, "softrdf_bwd");
             else
-                launch_soft_cells(h, st, soft_cell_bwd_kernel<WMD_PBC_TRICLINIC, false>, lds, G, sp, "softrdf_bwd");
+                launch_soft_cells(h, st, soft_cell_bwd_kernel<WMD_PBC_TRICLINIC, false>, kSoftBwdBlock, lds, G, sp, "softrdf_bwd");
             int g;
             grid1d((long long)n * nf, &g);
             hipLaunchKernelGGL(soft_unsort_kernel, dim3(g), dim3(kBlock), 0, st, (const float *)sp.grad,
