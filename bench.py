@@ -47,6 +47,22 @@ def cpu_model():
     return platform.processor()
 
 
+def traffic_per_launch(kernel, frames):
+    """HBM bytes per launch of `kernel` over `frames` frames, from the committed
+    FETCH_SIZE / WRITE_SIZE counter runs (profiles/hbm_traffic.json: KiB per
+    dispatch on a known frame count, scaled by what the counters read for a
+    plain copy of known size); None when the file is absent."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    try:
+        with open(path) as fh:
+            t = json.load(fh)
+        k = t[kernel]
+    except (OSError, KeyError, ValueError):
+        return None
+    per_frame = (k["fetch_kib"] * t["fetch_scale"] + k["write_kib"] * t["write_scale"]) * 1024.0 / k["frames"]
+    return per_frame * frames
+
+
 def cpu_baseline(n_side=47, box=144.3, n_waters=100000):
     """The oracle (a C restatement of the reference's single-threaded CPU pair
     loop) timed on a bounded sample: frame 0 of the same water box, the first
@@ -422,7 +438,7 @@ def main():
             "peak": VALU_PEAK_TOPS,
             "unit": "TOP/s",
             "frac": rdf_ach / VALU_PEAK_TOPS,
-            "traffic": None,
+            "traffic": traffic_per_launch("rdf_cell_kernel_100k_waters", F / max(pair_n // K, 1)),
             "algorithmic": f"{RDF_OPS_PER_PAIR} VALU lane-ops per unordered in-range pair x "
                            f"{p_in:.4g} pairs per step",
             "avg_launch_ms": pair_ms / max(pair_n, 1),
@@ -434,7 +450,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": rmsd_ach / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic_per_launch("rmsd_moments_kernel_chain_20k", RF),
             "avg_launch_ms": mom_ms / max(mom_n, 1),
         },
         "checks": {
