@@ -111,8 +111,9 @@ class SoftRdf:
 
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h is not None and _native._lib is not None:
-            _native._lib.wmd_softrdf_destroy(h)
+        nat = globals().get("_native")  # None while the interpreter shuts down
+        if h is not None and nat is not None and nat._lib is not None:
+            nat._lib.wmd_softrdf_destroy(h)
             self._h = None
 
     @property
@@ -278,8 +279,9 @@ class FittedRmsd:
 
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h is not None and _native._lib is not None:
-            _native._lib.wmd_rmsd_destroy(h)
+        nat = globals().get("_native")  # None while the interpreter shuts down
+        if h is not None and nat is not None and nat._lib is not None:
+            nat._lib.wmd_rmsd_destroy(h)
             self._h = None
 
     def store(self, x):

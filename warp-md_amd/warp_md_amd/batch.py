@@ -190,8 +190,9 @@ class UnevenBatch:
 
     def __del__(self):
         h = getattr(self, "handle", None)
-        if h is not None and _native._lib is not None:
-            _native._lib.wmd_batch_destroy(h)
+        nat = globals().get("_native")  # None while the interpreter shuts down
+        if h is not None and nat is not None and nat._lib is not None:
+            nat._lib.wmd_batch_destroy(h)
             self.handle = None
 
 
