@@ -149,6 +149,36 @@ __global__ __launch_bounds__(kBlock) void batch_scatter_kernel(
     }
 }
 
+// Fused build, one workgroup per frame of the batch (global frame index s =
+// the box index): cell-list trajectories are counting-sorted in LDS
+// (build_frame), brute-force trajectories copied in store order.  Used when
+// every cell-list trajectory has at most kBuildMaxCells cells.
+template <int PBC>
+__global__ __launch_bounds__(kBuildBlock) void batch_build_kernel(
+    const BatchTraj *__restrict__ traj, int n_traj, const float *__restrict__ store,
+    const FrameBox *__restrict__ boxes, unsigned *__restrict__ start, float4 *__restrict__ sorted,
+    unsigned *__restrict__ frame_flag) {
+    extern __shared__ __attribute__((aligned(16))) char lds[];
+    unsigned *part = (unsigned *)lds;
+    unsigned *far = part + kBuildBlock;
+    unsigned *cnt = far + 4;
+    const long long s = blockIdx.x;
+    const int t = find_last_le(n_traj, s, [&](int i) { return (long long)traj[i].box0; });
+    const BatchTraj T = uload(traj, t);
+    const int f = (int)(s - T.box0);
+    const float *X = store + T.store_off + (long long)f * 3 * T.stride;
+    float4 *dst = sorted + T.af0 + (long long)f * T.n;
+    if (T.kind == KIND_CELLS) {
+        const int ncell = T.ncx * T.ncy * T.ncz;
+        build_frame<PBC>(X, T.stride, T.n, boxes[s], T.ncx, T.ncy, T.ncz, nullptr,
+                         start + T.start_off + (long long)f * (ncell + 1), dst, nullptr, frame_flag + s,
+                         cnt, part, far);
+    } else {
+        for (int i = threadIdx.x; i < T.n; i += kBuildBlock)
+            dst[i] = make_float4(X[i], X[T.stride + i], X[2 * T.stride + i], __uint_as_float(0x11202020u));
+    }
+}
+
 // ---- persistent pair kernel -------------------------------------------------
 struct BatchPairArgs {
     const BatchTraj *traj;
@@ -317,6 +347,7 @@ struct wmd_batch {
     long long rmsd_units = 0;
     long long slice_lo[kSlices + 1] = {};
     bool any_cells = false;
+    int max_ncell = 0;  // largest cell grid of a cell-list trajectory
     // device
     DevBuf d_counts, d_thr, d_tab2, d_traj, d_rtraj, d_boxes, d_flag, d_count, d_start, d_cell_of,
         d_slot, d_image, d_sorted, d_queue, d_partials;
@@ -370,6 +401,7 @@ void plan_batch(wmd_batch *b) {
     std::vector<double> cost(T, 0.0);
     long long af = 0, units = 0, scan = 0, cnt = 0, sta = 0, box0 = 0, runits = 0, out0 = 0;
     b->any_cells = false;
+    b->max_ncell = 0;
     for (int t = 0; t < T; ++t) {
         const int n = (int)b->n_atoms[t], F = (int)b->n_frames[t];
         BatchTraj &d = b->traj[t];
@@ -403,6 +435,7 @@ void plan_batch(wmd_batch *b) {
             cnt += (long long)F * ncell;
             sta += (long long)F * (ncell + 1);
             b->any_cells = true;
+            b->max_ncell = std::max(b->max_ncell, ncell);
             // per-unit cost: home atoms x half-stencil atoms, plus a fixed part
             const double nh = (double)n / ncell;
             const double st_cells = (std::pow(2.0 * k + 1.0, 3) - 1.0) / 2.0 + 1.0;
@@ -468,7 +501,7 @@ void plan_batch(wmd_batch *b) {
     b->d_start.reserve(4 * std::max<long long>(sta, 1));
     const long long afc = std::max<long long>(af, 1);
     if (b->flags & WMD_BATCH_RDF) {
-        if (b->any_cells) {
+        if (b->any_cells && b->max_ncell > kBuildMaxCells) {
             b->d_cell_of.reserve(4 * afc);
             b->d_slot.reserve(4 * afc);
             b->d_image.reserve(4 * afc);
@@ -495,7 +528,14 @@ template <int PBC>
 void enqueue_pairs(wmd_batch *b, hipStream_t st, const float *d_store) {
     const int T = b->n_traj;
     const BatchTraj *dt = b->d_traj.as<BatchTraj>();
-    if (b->any_cells && PBC != WMD_PBC_NONE) {
+    const bool fused = b->max_ncell <= kBuildMaxCells;
+    if (fused) {
+        hipLaunchKernelGGL(batch_build_kernel<PBC == WMD_PBC_NONE ? WMD_PBC_ORTHO : PBC>,
+                           dim3((unsigned)b->total_frames), dim3(kBuildBlock), build_lds_bytes(b->max_ncell), st, dt, T,
+                           d_store, (const FrameBox *)b->d_boxes.ptr, b->d_start.as<unsigned>(),
+                           b->d_sorted.as<float4>(), b->d_flag.as<unsigned>());
+        WMD_HIP_CHECK(hipGetLastError());
+    } else if (b->any_cells && PBC != WMD_PBC_NONE) {
         hipLaunchKernelGGL(batch_assign_kernel<PBC == WMD_PBC_NONE ? WMD_PBC_ORTHO : PBC>,
                            dim3(grid_1d(b->total_af)), dim3(kBlock), 0, st, dt, T, b->total_af,
                            d_store, (const FrameBox *)b->d_boxes.ptr, b->d_cell_of.as<unsigned>(),
@@ -506,11 +546,13 @@ void enqueue_pairs(wmd_batch *b, hipStream_t st, const float *d_store) {
                            (const unsigned *)b->d_count.ptr, b->d_start.as<unsigned>());
         WMD_HIP_CHECK(hipGetLastError());
     }
-    hipLaunchKernelGGL(batch_scatter_kernel, dim3(grid_1d(b->total_af)), dim3(kBlock), 0, st, dt, T,
-                       b->total_af, d_store, (const unsigned *)b->d_cell_of.ptr,
-                       (const unsigned *)b->d_slot.ptr, (const unsigned *)b->d_image.ptr,
-                       (const unsigned *)b->d_start.ptr, b->d_sorted.as<float4>());
-    WMD_HIP_CHECK(hipGetLastError());
+    if (!fused) {
+        hipLaunchKernelGGL(batch_scatter_kernel, dim3(grid_1d(b->total_af)), dim3(kBlock), 0, st, dt, T,
+                           b->total_af, d_store, (const unsigned *)b->d_cell_of.ptr,
+                           (const unsigned *)b->d_slot.ptr, (const unsigned *)b->d_image.ptr,
+                           (const unsigned *)b->d_start.ptr, b->d_sorted.as<float4>());
+        WMD_HIP_CHECK(hipGetLastError());
+    }
 
     BatchPairArgs a;
     a.traj = dt;
@@ -552,8 +594,9 @@ void enqueue_step(wmd_batch *b, hipStream_t st, const float *d_store, float *d_o
                   double *d_err) {
     const bool rdf = (b->flags & WMD_BATCH_RDF) && b->bins > 0 && b->n_units > 0;
     if (rdf) {
-        hipLaunchKernelGGL(fill_u32_kernel, dim3(grid_1d(std::max<long long>(b->cell_counts, 1))), dim3(kBlock), 0, st,
-                           b->d_count.as<unsigned>(), b->cell_counts, 0u);
+        if (b->max_ncell > kBuildMaxCells)
+            hipLaunchKernelGGL(fill_u32_kernel, dim3(grid_1d(std::max<long long>(b->cell_counts, 1))), dim3(kBlock), 0, st,
+                               b->d_count.as<unsigned>(), b->cell_counts, 0u);
         hipLaunchKernelGGL(fill_u32_kernel, dim3(grid_1d(std::max<long long>(b->total_frames, 1))), dim3(kBlock), 0, st,
                            b->d_flag.as<unsigned>(), b->total_frames, 0u);
         hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(kBlock), 0, st, b->d_queue.as<unsigned long long>(),

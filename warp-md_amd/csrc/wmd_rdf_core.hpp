@@ -481,6 +481,102 @@ __global__ __launch_bounds__(kBlock) void cell_scan_kernel(const unsigned *__res
     scan_cells(count + f * ncell, start + f * (ncell + 1), ncell);
 }
 
+// Fused cell build of one frame by one workgroup of kBuildBlock threads: a
+// counting sort in LDS.  Pass 1 counts the atoms of every cell with LDS
+// atomics; a block scan gives the cell starts (written to start[0..ncell]);
+// pass 2 recomputes each atom's cell and takes its slot from an LDS cursor.
+// The frame is read twice from HBM and the sorted copy written once, with no
+// global atomics.  cnt: LDS [ncell] (ncell <= kBuildMaxCells).
+constexpr int kBuildBlock = 1024;
+constexpr int kBuildMaxCells = 16384;
+
+__device__ __forceinline__ unsigned block_exclusive_scan_1024(unsigned v, unsigned *part,
+                                                              unsigned *total) {
+    part[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 1; off < kBuildBlock; off <<= 1) {
+        const unsigned u = threadIdx.x >= (unsigned)off ? part[threadIdx.x - off] : 0u;
+        __syncthreads();
+        part[threadIdx.x] += u;
+        __syncthreads();
+    }
+    const unsigned incl = part[threadIdx.x];
+    *total = part[kBuildBlock - 1];
+    __syncthreads();
+    return incl - v;
+}
+
+template <int PBC>
+__device__ __forceinline__ void build_frame(const float *__restrict__ X, int stride, int n,
+                                            const FrameBox &b, int ncx, int ncy, int ncz,
+                                            const unsigned *__restrict__ wts,
+                                            unsigned *__restrict__ start, float4 *__restrict__ sorted,
+                                            unsigned *__restrict__ order, unsigned *frame_flag,
+                                            unsigned *cnt, unsigned *part, unsigned *far) {
+    const int ncell = ncx * ncy * ncz;
+    for (int c = threadIdx.x; c < ncell; c += kBuildBlock) cnt[c] = 0u;
+    if (threadIdx.x == 0) *far = 0u;
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += kBuildBlock) {
+        unsigned img;
+        bool far_out;
+        const int cid = assign_atom<PBC>(X[i], X[stride + i], X[2 * stride + i], b, ncx, ncy, ncz, img, far_out);
+        atomicAdd(&cnt[cid], 1u);
+        if (far_out) *far = 1u;
+    }
+    __syncthreads();
+    // exclusive scan of cnt: thread t owns cells [t * per, (t + 1) * per)
+    const int per = (ncell + kBuildBlock - 1) / kBuildBlock;
+    const int lo = min(ncell, (int)threadIdx.x * per), hi = min(ncell, lo + per);
+    unsigned sum = 0;
+    for (int c = lo; c < hi; ++c) sum += cnt[c];
+    unsigned total;
+    unsigned run = block_exclusive_scan_1024(sum, part, &total);
+    for (int c = lo; c < hi; ++c) {
+        const unsigned k = cnt[c];
+        start[c] = run;
+        cnt[c] = run;  // the cursor of pass 2
+        run += k;
+    }
+    if (threadIdx.x == 0) {
+        start[ncell] = total;
+        *frame_flag = *far;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += kBuildBlock) {
+        const float x = X[i], y = X[stride + i], z = X[2 * stride + i];
+        unsigned img;
+        bool far_out;
+        const int cid = assign_atom<PBC>(x, y, z, b, ncx, ncy, ncz, img, far_out);
+        const unsigned pos = atomicAdd(&cnt[cid], 1u);
+        const unsigned w = wts ? wts[i] : 0x0101u;  // mA | mB << 8
+        const unsigned meta = (img & 0xFFFFFFu) | ((w & 0xFu) << 24) | (((w >> 8) & 0xFu) << 28);
+        sorted[pos] = make_float4(x, y, z, __uint_as_float(meta));
+        if (order) order[pos] = (unsigned)i;
+    }
+}
+
+template <int PBC>
+__global__ __launch_bounds__(kBuildBlock) void cell_build_frame_kernel(
+    const float *__restrict__ soa, long long frame_stride, int stride, int n, long long frame0,
+    const FrameBox *__restrict__ boxes, int ncx, int ncy, int ncz, const unsigned *__restrict__ wts,
+    unsigned *__restrict__ start, float4 *__restrict__ sorted, unsigned *__restrict__ order,
+    unsigned *__restrict__ frame_flag) {
+    extern __shared__ __attribute__((aligned(16))) char lds[];
+    unsigned *part = (unsigned *)lds;
+    unsigned *far = part + kBuildBlock;
+    unsigned *cnt = far + 4;
+    const long long f = blockIdx.x;
+    const int ncell = ncx * ncy * ncz;
+    build_frame<PBC>(soa + (frame0 + f) * frame_stride, stride, n, boxes[f], ncx, ncy, ncz, wts,
+                     start + f * (ncell + 1), sorted + f * n, order ? order + f * n : nullptr,
+                     frame_flag + f, cnt, part, far);
+}
+
+__host__ __device__ inline size_t build_lds_bytes(int ncell) {
+    return (size_t)(kBuildBlock + 4 + ncell) * 4;
+}
+
 // sorted[f][start[c] + slot] = (x, y, z, meta)
 __global__ __launch_bounds__(kBlock) void cell_scatter_kernel(
     const float *__restrict__ soa, long long frame_stride, int stride, int n, int n_frames,
@@ -1233,6 +1329,16 @@ void build_cells(wmd_ctx *ctx, hipStream_t st, int pbc, const float *d_soa, int 
     ProfScope build_prof(ctx, st, "cell_build");
     const int ncell = nc[0] * nc[1] * nc[2];
     const long long fs = 3 * (long long)stride;
+    if (ncell <= kBuildMaxCells) {
+        const size_t lds = build_lds_bytes(ncell);
+        unsigned *ord = want_order ? S.order.as<unsigned>() : (unsigned *)nullptr;
+        if (pbc == WMD_PBC_ORTHO)
+            hipLaunchKernelGGL(cell_build_frame_kernel<WMD_PBC_ORTHO>, dim3(nf), dim3(kBuildBlock), lds, st, d_soa, fs, stride, n, f0, bx, nc[0], nc[1], nc[2], d_w, S.start.as<unsigned>(), S.sorted.as<float4>(), ord, S.flag.as<unsigned>());
+        else
+            hipLaunchKernelGGL(cell_build_frame_kernel<WMD_PBC_TRICLINIC>, dim3(nf), dim3(kBuildBlock), lds, st, d_soa, fs, stride, n, f0, bx, nc[0], nc[1], nc[2], d_w, S.start.as<unsigned>(), S.sorted.as<float4>(), ord, S.flag.as<unsigned>());
+        WMD_HIP_CHECK(hipGetLastError());
+        return;
+    }
     unsigned *d_count = S.count.as<unsigned>(), *d_fflag = S.flag.as<unsigned>();
     int g;
     grid1d((long long)ncell * nf, &g);
