@@ -293,7 +293,7 @@ __global__ __launch_bounds__(kBatchBlock) void batch_pair_kernel(BatchPairArgs a
                 cur_end = T.unit0 + (long long)T.n_frames * T.upf;
                 G = CellGrid{a.sorted + T.af0, a.start + T.start_off, a.frame_flag + T.box0,
                              a.boxes + T.box0, T.n, T.n_frames, T.ncx, T.ncy, T.ncz, T.k,
-                             T.shift_ok, kSlices, T.cull2};
+                             T.shift_ok, kSlices, T.cull2, nullptr};
                 sl = stencil_lane(lane, T.kind == KIND_CELLS ? T.k : 1);
             }
             const long long local = u - T.unit0;

@@ -42,10 +42,12 @@
 //    pairs, weight 2) plus the home cell itself (ordered pairs, weight 1).
 //    Neighbour atoms farther than r_max + 0.05 A from the home atoms' bounding
 //    box are dropped while staging (their pairs are provably out of range).
-// Both are persistent: each workgroup keeps u32 sub-histograms in LDS for all
-// the units it processes and merges them into the u64 global counts when a
-// uniform bound says a slot could wrap, and at the end.  Units are assigned in
-// contiguous ranges per XCD so that an XCD's L2 holds the frames it works on.
+// Both are persistent.  The brute-force kernel keeps u32 sub-histograms in LDS
+// and merges them into the u64 global counts when a uniform bound says a slot
+// could wrap, and at the end; the cell kernel keeps one u64 LDS histogram per
+// workgroup.  Units are split into contiguous ranges per XCD so that an XCD's
+// L2 holds the frames it works on; inside its range, each wave of the cell
+// kernel takes units from an atomic counter (units differ in cost).
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -655,7 +657,7 @@ struct wmd_rdf {
     int last_path = 0;
     int last_nc[3] = {0, 0, 0};
     DevBuf d_counts, d_thr, d_elo, d_ehi, d_tab, d_tab2, d_wts, d_io, d_boxes, d_aos, d_soa;
-    DevBuf d_tiles;
+    DevBuf d_tiles, d_queue;
     CellScratch cells;
     HostBuf h_stage, h_boxes;
 };
@@ -736,7 +738,11 @@ void launch_cells_t(wmd_rdf *h, hipStream_t st, const float4 *sorted, const unsi
     const long long max_grid = (n_units + kCellWaves * 8 - 1) / (kCellWaves * 8) * 8;
     WMD_REQUIRE(k >= 1 && 2 * (2 * k * k + 2 * k + 1) < 64, WMD_ERR_INVALID, "cell stencil reach out of range");
     if (grid > max_grid) grid = (int)std::max<long long>(8, max_grid);
-    CellGrid G{sorted, start, frame_flag, boxes, n, nf, nc[0], nc[1], nc[2], k, shift_ok, 8, cull2};
+    // dynamic unit schedule: the per-XCD grab counters start at zero
+    unsigned long long *queue = (unsigned long long *)h->d_queue.reserve(8 * 8);
+    hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(kBlock), 0, st, queue, 8LL);
+    WMD_HIP_CHECK(hipGetLastError());
+    CellGrid G{sorted, start, frame_flag, boxes, n, nf, nc[0], nc[1], nc[2], k, shift_ok, 8, cull2, queue};
     hipLaunchKernelGGL(kern, dim3(grid), dim3(kCellBlock), lds, st, G, p);
     WMD_HIP_CHECK(hipGetLastError());
 }
@@ -1100,7 +1106,7 @@ struct wmd_softrdf {
     int cell_k = 0;
     int last_path = 0;
     int last_nc[3] = {0, 0, 0};
-    DevBuf d_boxes, d_table, d_nodes, d_gsorted;
+    DevBuf d_boxes, d_table, d_nodes, d_gsorted, d_queue;
     HostBuf h_boxes;
     CellScratch cells;
 };
@@ -1248,9 +1254,13 @@ void soft_run(wmd_softrdf *h, bool backward, const float *d_soa, size_t stride, 
         const FrameBox *bx = d_boxes + f0;
         build_cells(h->ctx, st, h->pbc, d_soa, (int)stride, n, nf, (long long)f0, bx, pl.nc, nullptr,
                     h->cells, backward);
+        // dynamic unit schedule: the per-XCD grab counters start at zero
+        unsigned long long *queue = (unsigned long long *)h->d_queue.reserve(8 * 8);
+        hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(kBlock), 0, st, queue, 8LL);
+        WMD_HIP_CHECK(hipGetLastError());
         CellGrid G{h->cells.sorted.as<float4>(), (const unsigned *)h->cells.start.ptr,
                    (const unsigned *)h->cells.flag.ptr, bx, n, nf, pl.nc[0], pl.nc[1], pl.nc[2], pl.k,
-                   shift_ok, 8, cull2};
+                   shift_ok, 8, cull2, queue};
         if (backward) {
             // per-batch gradient in cell order, scattered to the store afterwards
             sp.grad = (float *)h->d_gsorted.reserve((size_t)3 * n * nf * 4);

@@ -792,6 +792,7 @@ struct CellGrid {
     const FrameBox *boxes;      // [F]
     int n, n_frames, ncx, ncy, ncz, k, shift_ok, n_xcd;
     float cull2;                // squared cull radius (pair reach plus rounding margin)
+    unsigned long long *queue;  // [n_xcd] zeroed grab counters (dynamic schedule), or null
 };
 
 // Dispatches one chunk to the consumer's pair loop for its image mode.
@@ -1039,9 +1040,15 @@ __device__ __forceinline__ void cell_unit(const CellGrid &G, int f, int h, const
     }
 }
 
-// The unit loop of one wave over a batch of frames: XCD-aware static schedule
-// (workgroup g runs on XCD g % n_xcd and takes a contiguous slice of the
-// units; the slice's units are dealt to its waves round robin).
+// The unit loop of one wave over a batch of frames.  XCD-aware: workgroup g
+// runs on XCD g % n_xcd and works on that XCD's contiguous slice of the
+// units.  Static schedule (G.queue null): the slice's units are dealt to its
+// waves round robin.  Dynamic schedule: each wave takes kCellGrab
+// consecutive units at a time from the slice's atomic counter until the
+// counter passes the slice end (every wave reaches that exit), which evens
+// out units of uneven cost.
+constexpr int kCellGrab = 2;
+
 template <int PBC, bool POS, class C>
 __device__ __forceinline__ void cell_waves(const CellGrid &G, float4 *ring, unsigned *ringp,
                                            int wave, C &cons) {
@@ -1050,11 +1057,30 @@ __device__ __forceinline__ void cell_waves(const CellGrid &G, float4 *ring, unsi
     const int n_xcd = G.n_xcd;
     const long long n_units = (long long)G.n_frames * ncell;
     const int xcd = blockIdx.x % n_xcd;
+    const long long u_lo = n_units * xcd / n_xcd;
+    const long long u_hi = n_units * (xcd + 1) / n_xcd;
+    if (G.queue) {
+        const int lane = threadIdx.x & 63;
+        while (true) {
+            unsigned long long g0 = 0;
+            if (lane == 0) g0 = atomicAdd(&G.queue[xcd], (unsigned long long)kCellGrab);
+            const unsigned glo = __builtin_amdgcn_readfirstlane((unsigned)g0);
+            const unsigned ghi = __builtin_amdgcn_readfirstlane((unsigned)(g0 >> 32));
+            const long long u0 = u_lo + (long long)(((unsigned long long)ghi << 32) | glo);
+            if (u0 >= u_hi) break;
+            const long long u1 = min(u0 + (long long)kCellGrab, u_hi);
+            for (long long u = u0; u < u1; ++u) {
+                const int f = (int)(u / ncell);
+                const int h = (int)(u - (long long)f * ncell);
+                cell_unit<PBC, POS>(G, f, h, sl, ring, ringp, cons);
+            }
+        }
+        return;
+    }
     const int waves_per_wg = blockDim.x >> 6;
     const long long wave_local = (long long)(blockIdx.x / n_xcd) * waves_per_wg + wave;
     const long long waves_per_xcd = (long long)(gridDim.x / n_xcd) * waves_per_wg;
-    const long long u_hi = n_units * (xcd + 1) / n_xcd;
-    for (long long u = n_units * xcd / n_xcd + wave_local; u < u_hi; u += waves_per_xcd) {
+    for (long long u = u_lo + wave_local; u < u_hi; u += waves_per_xcd) {
         const int f = (int)(u / ncell);
         const int h = (int)(u - (long long)f * ncell);
         cell_unit<PBC, POS>(G, f, h, sl, ring, ringp, cons);
