@@ -144,6 +144,24 @@ wmd_status wmd_rdf_finalize(const uint64_t *counts, size_t bins, float r_max, do
                             double n_b, double common, size_t frames, double volume_sum,
                             double density, uint32_t flags, float *r, float *g, float *integral);
 
+/* StructureFactorPlan's pair counts (crates/traj-engine/src/plans/analysis/
+ * structure_factor.rs:160-198; its GPU branch reuses rdf_accum, :128-149): an
+ * RDF plan over sel x sel (pairs of an atom with itself skipped) whose pair
+ * distances follow the reference's CPU path in f64 -- the f32 coordinate
+ * difference cast to f64 and multiplied by length_scale, the orthorhombic
+ * wrap in f64 on the scaled box, r = sqrt in f64, r < r_max and
+ * (r / (r_max / bins)) as usize in f64 -- so counts are bit-exact with it.
+ * pbc: WMD_PBC_NONE or WMD_PBC_ORTHO; bins <= 4096.  The handle is used with
+ * the wmd_rdf_* calls above (accumulate*, read, reset, destroy). */
+wmd_status wmd_rdf_create_f64(wmd_ctx *ctx, const uint32_t *sel, size_t n_sel, size_t bins,
+                              double r_max, int pbc, double length_scale, wmd_rdf **out);
+/* StructureFactorPlan::finalize (structure_factor.rs:200-258): r, g_r [bins]
+ * at bin centres, q, s_q [q_bins] on q = q_max i / (q_bins - 1).  n: selection
+ * size; volume_sum: summed scaled box volumes (required for WMD_PBC_ORTHO). */
+wmd_status wmd_sf_finalize(const uint64_t *counts, size_t bins, double r_max, double n,
+                           size_t frames, double volume_sum, int pbc, size_t q_bins, double q_max,
+                           float *r, float *g_r, float *q, float *s_q);
+
 /* ---- RMSD ----------------------------------------------------------------- */
 
 /* RmsdPlan::init GPU branch (crates/traj-engine/src/plans/rmsd/plan.rs:205-220):

@@ -13,6 +13,9 @@ The C file restates warp-md's CPU semantics (citations in ``wmd_oracle.c``):
 * ``fit_rmsd``        -- ``kabsch_rotation`` + per-group residual,
   ``plans/geometry/geometry_math.rs:169-297``, ``plans/geometry/rms.rs:643-676``
 * ``min_image_f64``   -- ``crates/traj-core/src/pbc_math.rs:17-164``
+* ``pairdist_counts`` -- ``PairDistPlan`` pair loop, ``rdf.rs:1120-1178``
+* ``sf_counts``, ``sf_finalize`` -- ``StructureFactorPlan``,
+  ``crates/traj-engine/src/plans/analysis/structure_factor.rs:160-258``
 
 ``rdf_finalize_py`` below is a second, independent float64/float32 numpy
 restatement of the normalisation used to cross-check the C version.
@@ -70,6 +73,15 @@ def lib():
     L.orc_svd3.restype = None
     L.orc_min_image_f64.argtypes = [P, P, ctypes.c_int, P, ctypes.c_double, P]
     L.orc_min_image_f64.restype = ctypes.c_int
+    L.orc_pairdist.argtypes = [P, S, S, S, P, S, P, S, S, ctypes.c_float, ctypes.c_int, P,
+                               ctypes.c_int, ctypes.c_int, P, P]
+    L.orc_pairdist.restype = ctypes.c_int
+    L.orc_sf_counts.argtypes = [P, S, S, S, P, S, S, ctypes.c_double, ctypes.c_int, P,
+                                ctypes.c_double, P, P]
+    L.orc_sf_counts.restype = ctypes.c_int
+    L.orc_sf_finalize.argtypes = [P, S, ctypes.c_double, ctypes.c_double, S, ctypes.c_double,
+                                  ctypes.c_int, S, ctypes.c_double, P, P, P, P]
+    L.orc_sf_finalize.restype = ctypes.c_int
     L.orc_omp_max_threads.argtypes = []
     L.orc_omp_max_threads.restype = ctypes.c_int
     _lib = L
@@ -276,3 +288,58 @@ def min_image_f64(a, b, box=None, kind="none", length_scale=1.0):
 
 def omp_max_threads() -> int:
     return int(lib().orc_omp_max_threads())
+
+
+def pairdist_counts(coords, sel_a, sel_b, bins, r_max, pbc="orthorhombic", boxes=None,
+                    same_sel=False, unique_pairs=False, per_frame=False):
+    """PairDistPlan counts (u64 [bins]); per_frame=True also returns [F][bins]."""
+    L = lib()
+    c = _coords(coords)
+    F, N, stride = c.shape
+    kind = PBC_KINDS[pbc] if isinstance(pbc, str) else int(pbc)
+    bx = _boxes(boxes, F, kind)
+    a = np.ascontiguousarray(sel_a, dtype=np.uint32)
+    b = np.ascontiguousarray(sel_b, dtype=np.uint32)
+    counts = np.zeros(int(bins), np.uint64)
+    fc = np.zeros((F, int(bins)), np.uint64) if per_frame else None
+    st = L.orc_pairdist(_ptr(c), stride, N, F, _ptr(a), a.size, _ptr(b), b.size, int(bins),
+                        float(r_max), kind, _ptr(bx), int(bool(same_sel)), int(bool(unique_pairs)),
+                        _ptr(counts), _ptr(fc))
+    if st != 0:
+        raise RuntimeError(f"oracle pairdist failed with status {st}")
+    return (counts, fc) if per_frame else counts
+
+
+def sf_counts(coords, sel, bins, r_max, pbc="orthorhombic", boxes=None, length_scale=1.0):
+    """StructureFactorPlan CPU counts (f64 distances) and the scaled volume sum."""
+    L = lib()
+    c = _coords(coords)
+    F, N, stride = c.shape
+    kind = PBC_KINDS[pbc] if isinstance(pbc, str) else int(pbc)
+    bx = _boxes(boxes, F, kind)
+    s = np.ascontiguousarray(sel, dtype=np.uint32)
+    counts = np.zeros(int(bins), np.uint64)
+    vol = ctypes.c_double(0.0)
+    st = L.orc_sf_counts(_ptr(c), stride, N, F, _ptr(s), s.size, int(bins), float(r_max), kind,
+                         _ptr(bx), float(length_scale), _ptr(counts), ctypes.byref(vol))
+    if st != 0:
+        raise RuntimeError(f"oracle structure factor failed with status {st}")
+    return counts, vol.value
+
+
+def sf_finalize(counts, r_max, n, frames, volume_sum, pbc, q_bins, q_max):
+    """(r, g_r, q, s_q) float32 of StructureFactorPlan::finalize."""
+    L = lib()
+    counts = np.ascontiguousarray(counts, np.uint64)
+    bins = counts.size
+    r = np.zeros(bins, np.float32)
+    g = np.zeros(bins, np.float32)
+    q = np.zeros(int(q_bins), np.float32)
+    sq = np.zeros(int(q_bins), np.float32)
+    kind = PBC_KINDS[pbc] if isinstance(pbc, str) else int(pbc)
+    st = L.orc_sf_finalize(_ptr(counts), bins, float(r_max), float(n), int(frames),
+                           float(volume_sum), kind, int(q_bins), float(q_max), _ptr(r), _ptr(g),
+                           _ptr(q), _ptr(sq))
+    if st != 0:
+        raise RuntimeError(f"oracle structure factor finalize failed with status {st}")
+    return r, g, q, sq

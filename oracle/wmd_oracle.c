@@ -24,6 +24,11 @@
  *   orc_fit_rmsd         numpy.linalg.svd Kabsch restated independently, rigid-motion zero.
  *   orc_svd3             numpy.linalg.svd on random and rank-deficient matrices.
  *   orc_min_image_f64    pbc_math.rs:166-221 known vectors.
+ *   orc_pairdist         reference known answers (correlation_structure_tests.rs:2193-2231,
+ *                        2313-2422); equal to orc_rdf_brute where the pair rules coincide.
+ *   orc_sf_counts / orc_sf_finalize
+ *                        structure_factor_shapes (correlation_structure_tests.rs:759-780),
+ *                        an independent numpy float64 restatement, and the ideal-gas limit.
  */
 #include <math.h>
 #include <stddef.h>
@@ -778,4 +783,137 @@ int orc_omp_max_threads(void) {
 #else
     return 1;
 #endif
+}
+
+/* ------------------------------------------------------------------------- */
+/* PairDistPlan (rdf.rs:1120-1178, CPU loop of process_chunk)                 */
+/* ------------------------------------------------------------------------- */
+/* Ordered pairs (a_pos, b_pos) of sel_a x sel_b with the reference's skip
+ * rules: unique_pairs keeps b_pos > a_pos only; otherwise a == b is skipped
+ * when same_sel (Arc::ptr_eq).  Same f32 distance and binning as RdfPlan
+ * (orthorhombic or none; other boxes are a Mismatch, rdf.rs:1125-1134).
+ * counts: [bins] accumulated; frame_counts: NULL or [n_frames][bins] (the
+ * per-frame histograms of output_distribution, rdf.rs:1121-1125). */
+int orc_pairdist(const float *coords, size_t stride, size_t n_atoms, size_t n_frames,
+                 const uint32_t *sel_a, size_t n_a, const uint32_t *sel_b, size_t n_b,
+                 size_t bins, float r_max, int pbc, const float *boxes, int same_sel,
+                 int unique_pairs, uint64_t *counts, uint64_t *frame_counts) {
+    if (pbc == PBC_TRICLINIC) return ORC_ERR_MISMATCH;
+    for (size_t i = 0; i < n_a; ++i)
+        if (sel_a[i] >= n_atoms) return ORC_ERR_INVALID;
+    for (size_t i = 0; i < n_b; ++i)
+        if (sel_b[i] >= n_atoms) return ORC_ERR_INVALID;
+    float bin_width = r_max / (float)bins;
+    for (size_t f = 0; f < n_frames; ++f) {
+        frame_box fb;
+        if (load_box(pbc, boxes ? boxes + 9 * f : NULL, &fb) != ORC_OK) return ORC_ERR_MISMATCH;
+        const float *base = coords + f * n_atoms * stride;
+        uint64_t *fc = frame_counts ? frame_counts + f * bins : NULL;
+        for (size_t ia = 0; ia < n_a; ++ia) {
+            uint32_t a = sel_a[ia];
+            for (size_t ib = 0; ib < n_b; ++ib) {
+                uint32_t b = sel_b[ib];
+                if (unique_pairs && ib <= ia) continue;
+                if (!unique_pairs && a == b && same_sel) continue;
+                float r = pair_r32(base + (size_t)a * stride, base + (size_t)b * stride, &fb);
+                long bin = bins ? bin_of(r, r_max, bin_width, bins) : -1;
+                if (bin >= 0) {
+                    counts[bin] += 1;
+                    if (fc) fc[bin] += 1;
+                }
+            }
+        }
+    }
+    return ORC_OK;
+}
+
+/* ------------------------------------------------------------------------- */
+/* StructureFactorPlan (analysis/structure_factor.rs:160-258)                 */
+/* ------------------------------------------------------------------------- */
+/* Counts of the CPU path: ordered pairs of one selection skipping a == b,
+ * with the displacement (f32 difference) cast to f64 and scaled by
+ * length_scale, the orthorhombic wrap in f64 with the scaled box, r = sqrt in
+ * f64, r < r_max and bin = (r / (r_max / bins)) as usize in f64.  volume_sum
+ * gets the scaled f64 box volume of every frame (orthorhombic). */
+int orc_sf_counts(const float *coords, size_t stride, size_t n_atoms, size_t n_frames,
+                  const uint32_t *sel, size_t n_sel, size_t bins, double r_max, int pbc,
+                  const float *boxes, double length_scale, uint64_t *counts, double *volume_sum) {
+    if (pbc == PBC_TRICLINIC) return ORC_ERR_MISMATCH;
+    for (size_t i = 0; i < n_sel; ++i)
+        if (sel[i] >= n_atoms) return ORC_ERR_INVALID;
+    const double w = r_max / (double)bins;
+    for (size_t f = 0; f < n_frames; ++f) {
+        double lx = 0.0, ly = 0.0, lz = 0.0;
+        if (pbc == PBC_ORTHO) {
+            const float *m = boxes + 9 * f;
+            lx = (double)m[0];
+            ly = (double)m[4];
+            lz = (double)m[8];
+            /* (lx * ly * lz) * scale^3, powi(3) = (s * s) * s */
+            *volume_sum += (lx * ly * lz) * (length_scale * length_scale * length_scale);
+        }
+        const double sx = lx * length_scale, sy = ly * length_scale, sz = lz * length_scale;
+        const float *base = coords + f * n_atoms * stride;
+        for (size_t ia = 0; ia < n_sel; ++ia) {
+            const float *pa = base + (size_t)sel[ia] * stride;
+            for (size_t ib = 0; ib < n_sel; ++ib) {
+                if (sel[ia] == sel[ib]) continue;
+                const float *pb = base + (size_t)sel[ib] * stride;
+                double dx = (double)(pb[0] - pa[0]) * length_scale;
+                double dy = (double)(pb[1] - pa[1]) * length_scale;
+                double dz = (double)(pb[2] - pa[2]) * length_scale;
+                if (pbc == PBC_ORTHO) {
+                    dx -= round(dx / sx) * sx;
+                    dy -= round(dy / sy) * sy;
+                    dz -= round(dz / sz) * sz;
+                }
+                double r = sqrt(dx * dx + dy * dy + dz * dz);
+                if (r < r_max) {
+                    size_t bin = (size_t)(r / w);
+                    if (bin < bins) counts[bin] += 1;
+                }
+            }
+        }
+    }
+    return ORC_OK;
+}
+
+/* StructureFactorPlan::finalize (structure_factor.rs:200-258): g(r) at bin
+ * centres (i + 1/2) w with shell 4 pi r0^2 w and rho = n / mean volume (1 without
+ * PBC), then S(q) = 1 + 4 pi rho sum (g - 1) sinc(q r0) r0^2 w on q = dq i,
+ * dq = q_max / (q_bins - 1).  Returns ORC_ERR_MISMATCH for PBC with no volume. */
+int orc_sf_finalize(const uint64_t *counts, size_t bins, double r_max, double n, size_t frames,
+                    double volume_sum, int pbc, size_t q_bins, double q_max, float *r, float *g_r,
+                    float *q, float *s_q) {
+    const double pi = 3.14159265358979323846;
+    const double w = r_max / (double)bins;
+    const double fr = (double)(frames > 0 ? frames : 1);
+    double volume = 1.0;
+    if (pbc == PBC_ORTHO) {
+        if (volume_sum == 0.0) return ORC_ERR_MISMATCH;
+        volume = volume_sum / fr;
+    }
+    const double rho = volume > 0.0 ? n / volume : 0.0;
+    for (size_t i = 0; i < bins; ++i) {
+        const double r0 = ((double)i + 0.5) * w;
+        const double shell = 4.0 * pi * r0 * r0 * w;
+        const double norm = shell > 0.0 ? rho * shell * n : 1.0;
+        const double g = (double)counts[i] / (norm * fr);
+        r[i] = (float)r0;
+        g_r[i] = (float)g;
+    }
+    const double dq = q_bins > 1 ? q_max / ((double)q_bins - 1.0) : 0.0;
+    for (size_t k = 0; k < q_bins; ++k) {
+        const double qv = dq * (double)k;
+        double sum = 0.0;
+        for (size_t i = 0; i < bins; ++i) {
+            const double r0 = ((double)i + 0.5) * w;
+            const double diff = (double)g_r[i] - 1.0;
+            const double weight = qv == 0.0 ? 1.0 : sin(qv * r0) / (qv * r0);
+            sum += diff * weight * r0 * r0 * w;
+        }
+        q[k] = (float)qv;
+        s_q[k] = (float)(1.0 + 4.0 * pi * rho * sum);
+    }
+    return ORC_OK;
 }

@@ -236,6 +236,45 @@ wmd_status wmd_ctx_profile_read(wmd_ctx *ctx, const char *name, double *total_ms
 }
 
 // rdf.rs:943-1016 RdfPlan::finalize with rdf_shell_measure (rdf.rs:487-496), 3-D.
+wmd_status wmd_sf_finalize(const uint64_t *counts, size_t bins, double r_max, double n,
+                           size_t frames_in, double volume_sum, int pbc, size_t q_bins, double q_max,
+                           float *r, float *g_r, float *q, float *s_q) {
+    WMD_API_BEGIN
+    WMD_REQUIRE((counts && r && g_r) || bins == 0, WMD_ERR_INVALID, "null buffer");
+    WMD_REQUIRE((q && s_q) || q_bins == 0, WMD_ERR_INVALID, "null buffer");
+    WMD_REQUIRE(bins > 0 && q_bins > 0, WMD_ERR_INVALID, "structure factor requires bins and q_bins > 0");
+    const double pi = 3.14159265358979323846;
+    const double w = r_max / (double)bins;
+    const double frames = (double)(frames_in > 0 ? frames_in : 1);
+    double volume = 1.0;
+    if (pbc == WMD_PBC_ORTHO) {
+        if (volume_sum == 0.0) fail(WMD_ERR_MISMATCH, "structure factor requires box volume");
+        volume = volume_sum / frames;
+    }
+    const double rho = volume > 0.0 ? n / volume : 0.0;
+    for (size_t i = 0; i < bins; ++i) {
+        const double r0 = ((double)i + 0.5) * w;
+        const double shell = 4.0 * pi * r0 * r0 * w;
+        const double norm = shell > 0.0 ? rho * shell * n : 1.0;
+        r[i] = (float)r0;
+        g_r[i] = (float)((double)counts[i] / (norm * frames));
+    }
+    const double dq = q_bins > 1 ? q_max / ((double)q_bins - 1.0) : 0.0;
+    for (size_t k = 0; k < q_bins; ++k) {
+        const double qv = dq * (double)k;
+        double sum = 0.0;
+        for (size_t i = 0; i < bins; ++i) {
+            const double r0 = ((double)i + 0.5) * w;
+            const double diff = (double)g_r[i] - 1.0;
+            const double weight = qv == 0.0 ? 1.0 : std::sin(qv * r0) / (qv * r0);
+            sum += diff * weight * r0 * r0 * w;
+        }
+        q[k] = (float)qv;
+        s_q[k] = (float)(1.0 + 4.0 * pi * rho * sum);
+    }
+    WMD_API_END
+}
+
 wmd_status wmd_rdf_finalize(const uint64_t *counts, size_t bins, float r_max, double n_a,
                             double n_b, double common, size_t frames_in, double volume_sum,
                             double density_opt, uint32_t flags, float *r, float *g,

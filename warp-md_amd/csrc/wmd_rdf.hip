@@ -634,6 +634,141 @@ __global__ __launch_bounds__(kBlock) void gather_soa_kernel(const float4 *__rest
     }
 }
 
+// -------------------------------------------------------------------------
+// f64 distances (StructureFactorPlan's CPU path, structure_factor.rs:160-198):
+// the f32 difference cast to f64 and scaled, the orthorhombic wrap in f64 on
+// the scaled box, s = dx^2 + dy^2 + dz^2 in f64 (left-to-right, unfused).  A
+// pair counts iff s < thr64[bins]; its bin comes from an estimate corrected
+// against the f64 breakpoints thr64 (exact, from host bisection over f64 bit
+// patterns with IEEE sqrt and division).  Brute force over 256-atom tiles.
+// -------------------------------------------------------------------------
+// round(d / l) * l subtracted from d, with the division only near a tie:
+// q = d * (1 / l) is within 5e-7 of the correctly rounded quotient for
+// |q| < 1e9, so rint(q) is exact when q lies more than 1e-4 from a half-integer.
+__device__ __forceinline__ double wrap64(double d, double l, double il) {
+    const double q = d * il;
+    double n = rint(q);
+    if (!(fabs(q - n) <= 0.4999) || !(fabs(q) < 1e9)) n = round(d / l);
+    return d - n * l;
+}
+
+template <int PBC>
+__device__ __forceinline__ double pair_s64(float xi, float yi, float zi, float xj, float yj, float zj,
+                                           double scale, double lx, double ly, double lz, double ilx,
+                                           double ily, double ilz) {
+    double dx = (double)(xj - xi) * scale;
+    double dy = (double)(yj - yi) * scale;
+    double dz = (double)(zj - zi) * scale;
+    if constexpr (PBC == WMD_PBC_ORTHO) {
+        dx = wrap64(dx, lx, ilx);
+        dy = wrap64(dy, ly, ily);
+        dz = wrap64(dz, lz, ilz);
+    }
+    return dx * dx + dy * dy + dz * dz;
+}
+
+template <int PBC, bool WEIGHTED>
+__global__ __launch_bounds__(kBlock) void rdf_brute64_kernel(
+    const float *__restrict__ soa, long long frame_stride, int stride, int n, int n_frames,
+    const FrameBox *__restrict__ boxes, const unsigned *__restrict__ wts,
+    const int2 *__restrict__ tile_pairs, int n_tile_pairs, long long frame0, int bins,
+    const double *__restrict__ thr64, double inv_w, double scale, unsigned long long *counts,
+    int n_xcd, unsigned w_max) {
+    extern __shared__ __attribute__((aligned(16))) char lds[];
+    double *thr = (double *)lds;                                  // [bins + 1]
+    unsigned *hist = (unsigned *)(thr + bins + 1);                // [bins]
+    float *jx = (float *)(hist + ((bins + 3) & ~3));              // [kTile] x 3
+    float *jy = jx + kTile;
+    float *jz = jy + kTile;
+    unsigned *jw = (unsigned *)(jz + kTile);
+    for (int i = threadIdx.x; i <= bins; i += kBlock) thr[i] = thr64[i];
+    for (int i = threadIdx.x; i < bins; i += kBlock) hist[i] = 0u;
+    const double cut = thr64[bins];
+    const long long n_units = (long long)n_frames * n_tile_pairs;
+    UnitRange ur = xcd_units(n_units, n_xcd);
+    unsigned long long budget = 0;
+    auto flush = [&]() {
+        __syncthreads();
+        for (int i = threadIdx.x; i < bins; i += kBlock) {
+            const unsigned v = hist[i];
+            if (v) {
+                atomicAdd(&counts[i], (unsigned long long)v);
+                hist[i] = 0u;
+            }
+        }
+        __syncthreads();
+    };
+    auto count = [&](double s, unsigned w) {
+        if (s < cut) {
+            int b = (int)(sqrt(s) * inv_w);
+            b = min(max(b, 0), bins - 1);
+            b += (int)(s >= thr[b + 1]) - (int)(s < thr[b]);
+            atomicAdd(&hist[b], w);
+        }
+    };
+    __syncthreads();
+    for (long long u = ur.begin; u < ur.end; u += ur.step) {
+        const int f = (int)(u / n_tile_pairs);
+        const int2 tp = tile_pairs[u - (long long)f * n_tile_pairs];
+        const float *fx = soa + (frame0 + f) * frame_stride;
+        const float *fy = fx + stride;
+        const float *fz = fy + stride;
+        const FrameBox b = boxes[f];
+        const double lx = (double)b.lx * scale, ly = (double)b.ly * scale, lz = (double)b.lz * scale;
+        const double ilx = PBC == WMD_PBC_ORTHO ? 1.0 / lx : 0.0;
+        const double ily = PBC == WMD_PBC_ORTHO ? 1.0 / ly : 0.0;
+        const double ilz = PBC == WMD_PBC_ORTHO ? 1.0 / lz : 0.0;
+        const int i = tp.x * kTile + threadIdx.x;
+        const float nanv = __builtin_nanf("");
+        float xi = nanv, yi = nanv, zi = nanv;
+        unsigned wi = 0;
+        if (i < n) {
+            xi = fx[i];
+            yi = fy[i];
+            zi = fz[i];
+            if constexpr (WEIGHTED) wi = wts[i];
+        }
+        const int j0 = tp.y * kTile;
+        const int nj = min(kTile, n - j0);
+        __syncthreads();  // previous unit done with the j tile
+        if ((int)threadIdx.x < nj) {
+            jx[threadIdx.x] = fx[j0 + threadIdx.x];
+            jy[threadIdx.x] = fy[j0 + threadIdx.x];
+            jz[threadIdx.x] = fz[j0 + threadIdx.x];
+            if constexpr (WEIGHTED) jw[threadIdx.x] = wts[j0 + threadIdx.x];
+        }
+        __syncthreads();
+        if (tp.x == tp.y) {
+            // ordered pairs (i, j), i != j
+            for (int jj = 0; jj < nj; ++jj) {
+                double s = pair_s64<PBC>(xi, yi, zi, jx[jj], jy[jj], jz[jj], scale, lx, ly, lz, ilx, ily, ilz);
+                if (jj == (int)threadIdx.x) s = __builtin_nan("");
+                unsigned w = 1u;
+                if constexpr (WEIGHTED) w = (wi & 0xFFu) * (jw[jj] >> 8);
+                count(s, w);
+            }
+        } else {
+            // unordered pairs: both orders give the same f64 s (the f32
+            // difference, the scaling and the wrap are all odd functions)
+            for (int jj = 0; jj < nj; ++jj) {
+                const double s = pair_s64<PBC>(xi, yi, zi, jx[jj], jy[jj], jz[jj], scale, lx, ly, lz, ilx, ily, ilz);
+                unsigned w = 2u;
+                if constexpr (WEIGHTED) {
+                    const unsigned wj = jw[jj];
+                    w = (wi & 0xFFu) * (wj >> 8) + (wj & 0xFFu) * (wi >> 8);
+                }
+                count(s, w);
+            }
+        }
+        budget += (unsigned long long)kTile * kTile * w_max;
+        if (budget > 0x7FFFFFFFull) {
+            budget = 0;
+            flush();
+        }
+    }
+    flush();
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -660,6 +795,12 @@ struct wmd_rdf {
     DevBuf d_tiles, d_queue;
     CellScratch cells;
     HostBuf h_stage, h_boxes;
+    // f64 distance mode (StructureFactorPlan): thresholds on the f64 squared
+    // distance, r_max and the length scale in f64
+    bool f64 = false;
+    double r_max64 = 0.0, scale64 = 1.0;
+    std::vector<double> thr64;
+    DevBuf d_thr64;
 };
 
 namespace {
@@ -788,6 +929,49 @@ void cells_dispatch(int pbc, bool lt, bool weighted, bool edge, bool table, A...
 }
 
 
+// Tile pairs (it <= jt, upper triangle) of the brute-force kernels, uploaded
+// to h->d_tiles; returns their number.
+int upload_tile_pairs(wmd_rdf *h, int n, hipStream_t st) {
+    const int tiles = (n + kTile - 1) / kTile;
+    const int n_tp = tiles * (tiles + 1) / 2;
+    std::vector<int2> tp;
+    tp.reserve(n_tp);
+    for (int i = 0; i < tiles; ++i)
+        for (int j = i; j < tiles; ++j) tp.push_back(make_int2(i, j));
+    int2 *d_tp = (int2 *)h->d_tiles.reserve(sizeof(int2) * std::max(n_tp, 1));
+    WMD_HIP_CHECK(hipMemcpyAsync(d_tp, tp.data(), sizeof(int2) * n_tp, hipMemcpyHostToDevice, st));
+    WMD_HIP_CHECK(hipStreamSynchronize(st));  // the vector dies at scope exit
+    return n_tp;
+}
+
+// f64-distance counts (StructureFactorPlan) on the device store.
+void accumulate_f64(wmd_rdf *h, const float *d_soa, size_t stride, size_t n_frames,
+                    const FrameBox *d_boxes, hipStream_t st) {
+    const int n = (int)h->io.size();
+    const int n_tp = upload_tile_pairs(h, n, st);
+    const size_t lds = (size_t)(h->bins + 1) * 8 + (size_t)((h->bins + 3) & ~3) * 4 + (size_t)kTile * 16;
+    WMD_REQUIRE(lds <= h->ctx->lds_per_block, WMD_ERR_UNSUPPORTED, "too many bins for LDS");
+    auto launch = [&](auto kern) {
+        ProfScope prof(h->ctx, st, "rdf_pair");
+        int grid = persistent_grid(h, occupancy(kern, lds), (long long)n_frames * n_tp);
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), lds, st, d_soa, 3 * (long long)stride,
+                           (int)stride, n, (int)n_frames, d_boxes, (const unsigned *)h->d_wts.ptr,
+                           (const int2 *)h->d_tiles.ptr, n_tp, 0LL, h->bins,
+                           (const double *)h->d_thr64.ptr, (double)h->bins / h->r_max64, h->scale64,
+                           h->d_counts.as<unsigned long long>(), 8, h->w_max);
+        WMD_HIP_CHECK(hipGetLastError());
+    };
+    if (h->pbc == WMD_PBC_ORTHO) {
+        if (h->weighted) launch(rdf_brute64_kernel<WMD_PBC_ORTHO, true>);
+        else launch(rdf_brute64_kernel<WMD_PBC_ORTHO, false>);
+    } else {
+        if (h->weighted) launch(rdf_brute64_kernel<WMD_PBC_NONE, true>);
+        else launch(rdf_brute64_kernel<WMD_PBC_NONE, false>);
+    }
+    h->last_path = 1;
+    h->last_nc[0] = h->last_nc[1] = h->last_nc[2] = 0;
+}
+
 // Core: accumulate frames of an IO-selection SoA store already on the device.
 void accumulate_device(wmd_rdf *h, const float *d_soa, size_t stride, size_t n_frames,
                        const float *boxes_host, hipStream_t st) {
@@ -799,6 +983,10 @@ void accumulate_device(wmd_rdf *h, const float *d_soa, size_t stride, size_t n_f
     const BoxSet B = make_boxes(h->pbc, boxes_host, n_frames);
     const FrameBox *d_boxes = upload_boxes(B, h->h_boxes, h->d_boxes, st);
 
+    if (h->f64) {
+        accumulate_f64(h, d_soa, stride, n_frames, d_boxes, st);
+        return;
+    }
     const bool edge = h->flags & WMD_RDF_EDGE_DIAGNOSTICS;
     const bool table = !h->tab.empty();
     HistParams p = hist_params(h, table);
@@ -988,6 +1176,60 @@ wmd_status wmd_rdf_create(wmd_ctx *ctx, const uint32_t *sel_a, size_t n_a, const
             WMD_HIP_CHECK(hipMemcpyAsync(h->d_io.ptr, h->io.data(), h->io.size() * 4, hipMemcpyHostToDevice, ctx->stream));
         }
         WMD_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    } catch (...) {
+        delete h;
+        throw;
+    }
+    *out = h;
+    WMD_API_END
+}
+
+wmd_status wmd_rdf_create_f64(wmd_ctx *ctx, const uint32_t *sel, size_t n_sel, size_t bins,
+                              double r_max, int pbc, double length_scale, wmd_rdf **out) {
+    WMD_API_BEGIN
+    WMD_REQUIRE(ctx && out, WMD_ERR_INVALID, "null context or output handle");
+    *out = nullptr;
+    WMD_REQUIRE(pbc == WMD_PBC_NONE || pbc == WMD_PBC_ORTHO, WMD_ERR_MISMATCH,
+                "structure factor requires orthorhombic box for PBC");
+    WMD_REQUIRE(bins <= 4096, WMD_ERR_UNSUPPORTED, "at most 4096 bins in f64 mode");
+    WMD_REQUIRE(std::isfinite(r_max) && r_max > 0.0, WMD_ERR_PARSE, "r_max must be finite and positive");
+    WMD_REQUIRE(std::isfinite(length_scale) && length_scale > 0.0, WMD_ERR_PARSE,
+                "length_scale must be finite and positive");
+    wmd_rdf *h = nullptr;
+    const wmd_status st = wmd_rdf_create(ctx, sel, n_sel, sel, n_sel, bins, (float)r_max, pbc, 0, &h);
+    if (st != WMD_OK) return st;
+    try {
+        h->f64 = true;
+        h->r_max64 = r_max;
+        h->scale64 = length_scale;
+        if (bins > 0) {
+            // thr64[k] = min{ f64 s >= 0 : bin(s) >= k }, bin(s) = the reference's
+            // r = sqrt(s) (f64), r < r_max, (r / (r_max / bins)) as usize
+            const double w = r_max / (double)bins;
+            auto key = [&](double s) -> size_t {
+                volatile double r = std::sqrt(s);
+                if (!(r < r_max)) return bins;
+                volatile double q = r / w;
+                const size_t b = (size_t)q;
+                return b < bins ? b : bins;
+            };
+            h->thr64.assign(bins + 1, 0.0);
+            for (size_t k = 1; k <= bins; ++k) {
+                uint64_t lo = 0, hi = 0x7FF0000000000000ull;  // key(+inf) = bins >= k
+                double lo_s;
+                std::memcpy(&lo_s, &lo, 8);
+                if (key(lo_s) >= k) { h->thr64[k] = 0.0; continue; }
+                while (hi - lo > 1) {
+                    const uint64_t mid = lo + (hi - lo) / 2;
+                    double ms;
+                    std::memcpy(&ms, &mid, 8);
+                    if (key(ms) >= k) hi = mid; else lo = mid;
+                }
+                std::memcpy(&h->thr64[k], &hi, 8);
+            }
+            h->d_thr64.reserve((bins + 1) * 8);
+            WMD_HIP_CHECK(hipMemcpy(h->d_thr64.ptr, h->thr64.data(), (bins + 1) * 8, hipMemcpyHostToDevice));
+        }
     } catch (...) {
         delete h;
         throw;

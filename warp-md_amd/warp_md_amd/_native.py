@@ -53,7 +53,7 @@ SYMBOLS = (
     "wmd_rdf_create", "wmd_rdf_io_selection", "wmd_rdf_accumulate",
     "wmd_rdf_accumulate_selected", "wmd_rdf_accumulate_device", "wmd_rdf_read",
     "wmd_rdf_reset", "wmd_rdf_destroy", "wmd_rdf_last_path", "wmd_rdf_set_tuning",
-    "wmd_rdf_finalize",
+    "wmd_rdf_finalize", "wmd_rdf_create_f64", "wmd_sf_finalize",
     "wmd_rmsd_create", "wmd_rmsd_store_layout", "wmd_rmsd_run", "wmd_rmsd_run_device",
     "wmd_rmsd_backward_device", "wmd_rmsd_destroy", "wmd_qcp_host",
     "wmd_softrdf_create", "wmd_softrdf_forward", "wmd_softrdf_backward",
@@ -134,6 +134,9 @@ def lib():
             "wmd_rdf_finalize": ([P, S, ctypes.c_float, ctypes.c_double, ctypes.c_double,
                                   ctypes.c_double, S, ctypes.c_double, ctypes.c_double,
                                   ctypes.c_uint32, P, P, P], St),
+            "wmd_rdf_create_f64": ([P, P, S, S, ctypes.c_double, I, ctypes.c_double, PP], St),
+            "wmd_sf_finalize": ([P, S, ctypes.c_double, ctypes.c_double, S, ctypes.c_double, I, S,
+                                 ctypes.c_double, P, P, P, P], St),
             "wmd_rmsd_create": ([P, P, S, P, S, P, S, I, PP], St),
             "wmd_rmsd_store_layout": ([P, P, S, ctypes.POINTER(S), ctypes.POINTER(S)], St),
             "wmd_rmsd_run": ([P, P, S, S, P, P], St),
@@ -176,10 +179,12 @@ def check(status: int) -> None:
 
 
 def ptr(a):
-    """Raw address of a numpy array (or None)."""
+    """Address of a numpy array (or None) as a c_void_p that holds a reference
+    to the array, so a temporary passed as ``f(ptr(np.ascontiguousarray(x)))``
+    stays alive until the call returns."""
     if a is None:
         return None
-    return ctypes.c_void_p(a.ctypes.data)
+    return a.ctypes.data_as(ctypes.c_void_p)
 
 
 def device_count() -> int:

@@ -253,3 +253,239 @@ class RmsdPlan:
             lib().wmd_rmsd_destroy(h)
         self.last_error_bound = err
         return out
+
+
+class PairDistPlan:
+    """Pair-distance histogram between two selections.
+
+    Mirrors PyPairDistPlan (crates/traj-py/src/analysis.rs:2427-2490) over
+    PairDistPlan (crates/traj-engine/src/plans/rdf.rs:541-577, 1019-1287), whose
+    GPU branch uses the same rdf_accum boundary as RdfPlan.  The pair set is
+    the reference CPU path's: ordered pairs (a, b) of sel_a x sel_b, skipping
+    a == b only when both selections are the same object (Arc::ptr_eq,
+    rdf.rs:1140-1142; ``sel_a is sel_b`` here); with ``unique_pairs`` only the
+    pairs whose position in sel_b exceeds the position in sel_a (rdf.rs:1136).
+    The engine counts the pairs of distinct atoms; pairs of an atom with itself
+    lie at r = 0 and are added to bin 0 on the host.
+    """
+
+    def __init__(self, sel_a, sel_b, bins: int, r_max: float, pbc: str = "orthorhombic",
+                 output_distribution: bool = False, unique_pairs: bool = False,
+                 compact_output: bool = False):
+        self.pbc = _parse_pbc(pbc)
+        self.same_sel = sel_a is sel_b
+        self.sel_a = _indices(sel_a)
+        self.sel_b = _indices(sel_b)
+        self.bins = int(bins)
+        if self.bins < 0:
+            raise OverflowError("bins must be non-negative")
+        self.r_max = float(np.float32(r_max))
+        self.output_distribution = bool(output_distribution)
+        self.unique_pairs = bool(unique_pairs)
+        self.compact_output = bool(compact_output)
+        self.last_path = None
+
+    def _self_pairs_per_frame(self) -> int:
+        """Pairs of an atom with itself in one frame (all at r = 0)."""
+        if self.unique_pairs:
+            # positions i < j holding the same atom
+            _, m = np.unique(self.sel_a, return_counts=True)
+            return int(np.sum(m * (m - 1) // 2))
+        if self.same_sel:
+            return 0
+        ua, ma = np.unique(self.sel_a, return_counts=True)
+        ub, mb = np.unique(self.sel_b, return_counts=True)
+        _, ia, ib = np.intersect1d(ua, ub, assume_unique=True, return_indices=True)
+        return int(np.sum(ma[ia].astype(np.int64) * mb[ib].astype(np.int64)))
+
+    def run(self, traj: Trajectory, system: System, chunk_frames: Optional[int] = None,
+            device: str = "auto", frame_indices: Optional[Sequence[int]] = None):
+        dev = Device.from_spec(device)
+        if traj.n_atoms() != system.n_atoms():
+            raise RuntimeError("mismatch: trajectory atom count does not match system")
+        if self.unique_pairs and not np.array_equal(self.sel_a, self.sel_b):
+            raise RuntimeError("unsupported: unique_pairs over two different selections runs on "
+                               "warp-md's CPU path")
+        ctx = dev.context()
+        n_atoms = system.n_atoms()
+        for sel in (self.sel_a, self.sel_b):
+            if sel.size and int(sel.max()) >= n_atoms:
+                raise RuntimeError("invalid selection: atom index out of range")
+        bins = self.bins
+        w = np.float32(self.r_max) / np.float32(bins) if bins else np.float32(0.0)
+        h = ctypes.c_void_p()
+        check(lib().wmd_rdf_create(ctx.handle, ptr(self.sel_a), self.sel_a.size, ptr(self.sel_b),
+                                   self.sel_b.size, bins, ctypes.c_float(self.r_max), self.pbc, 0,
+                                   ctypes.byref(h)))
+        self_pairs = self._self_pairs_per_frame()
+        counts = np.zeros(bins, np.uint64)
+        sums = np.zeros(bins, np.float64)
+        sumsq = np.zeros(bins, np.float64)
+        frames = 0
+        try:
+            n_io = ctypes.c_size_t()
+            check(lib().wmd_rdf_io_selection(h, None, 0, ctypes.byref(n_io)))
+            io = np.zeros(n_io.value, np.uint32)
+            check(lib().wmd_rdf_io_selection(h, ptr(io), io.size, ctypes.byref(n_io)))
+            tr = select_frames(traj, frame_indices)
+            tr.reset()
+            step = 1 if self.output_distribution else resolve_chunk_frames(chunk_frames, max(1, io.size))
+            while True:
+                chunk = tr.read_chunk(step)
+                if chunk is None:
+                    break
+                coords, boxes = chunk
+                nf = coords.shape[0]
+                if self.pbc != _native.PBC_NONE:
+                    if boxes is None:
+                        raise RuntimeError("mismatch: pairdist requires orthorhombic box for PBC")
+                    if self.pbc == _native.PBC_ORTHO:
+                        for m in boxes:
+                            if np.any(m[[1, 2, 3, 5, 6, 7]] != 0) or np.any(m[[0, 4, 8]] <= 0):
+                                raise RuntimeError("mismatch: pairdist requires orthorhombic box for PBC")
+                sel_coords = xyzw(np.ascontiguousarray(coords[:, io, :]))
+                check(lib().wmd_rdf_accumulate_selected(h, ptr(sel_coords), nf,
+                                                        ptr(boxes if self.pbc != _native.PBC_NONE else None)))
+                frames += nf
+                if self.output_distribution:
+                    fc = np.zeros(bins, np.uint64)
+                    check(lib().wmd_rdf_read(h, ptr(fc), None))
+                    check(lib().wmd_rdf_reset(h))
+                    fc = self._pair_counts(fc, 1, self_pairs)
+                    # accumulate_pairdist_frame (rdf.rs:420-433)
+                    counts += fc
+                    sums += fc.astype(np.float64)
+                    sumsq += fc.astype(np.float64) ** 2
+            if not self.output_distribution:
+                raw = np.zeros(bins, np.uint64)
+                check(lib().wmd_rdf_read(h, ptr(raw), None))
+                counts = self._pair_counts(raw, frames, self_pairs)
+            path = ctypes.c_int()
+            check(lib().wmd_rdf_last_path(h, ctypes.byref(path), None))
+            self.last_path = {0: None, 1: "brute", 2: "cells"}[path.value]
+        finally:
+            lib().wmd_rdf_destroy(h)
+        if not self.output_distribution:
+            return pairdist_centers(bins, w), counts
+        return pairdist_stats(counts, sums, sumsq, frames, w, self.compact_output)
+
+    def _pair_counts(self, raw, frames, self_pairs):
+        """Engine counts (ordered pairs of distinct atoms) -> the plan's pair counts."""
+        c = np.asarray(raw, np.uint64).copy()
+        if self.unique_pairs:
+            c //= np.uint64(2)  # each unordered pair of distinct atoms once
+        if self.bins and self_pairs:
+            c[0] += np.uint64(self_pairs * frames)
+        return c
+
+
+class StructureFactorPlan:
+    """g(r) and S(q) of one selection.
+
+    Mirrors PyStructureFactorPlan (crates/traj-py/src/correlation.rs:2638-2684)
+    over StructureFactorPlan (crates/traj-engine/src/plans/analysis/
+    structure_factor.rs).  The pair counts follow the reference's CPU path
+    bit for bit: distances in f64 from the f32 differences scaled by
+    ``length_scale`` (``wmd_rdf_create_f64``); the normalisation and the sine
+    transform are ``wmd_sf_finalize`` (structure_factor.rs:200-258).
+    Returns (r, g_r, q, s_q) float32, like structure_factor_to_py
+    (traj-py pack.rs:2236-2245).
+    """
+
+    def __init__(self, selection, bins: int, r_max: float, q_bins: int, q_max: float,
+                 pbc: str = "orthorhombic", length_scale: Optional[float] = None):
+        if pbc not in ("orthorhombic", "none"):
+            raise RuntimeError("pbc must be 'orthorhombic' or 'none'")  # pack.rs:2665-2673
+        self.pbc = _parse_pbc(pbc)
+        self.selection = _indices(selection)
+        self.bins = int(bins)
+        self.q_bins = int(q_bins)
+        if self.bins < 0 or self.q_bins < 0:
+            raise OverflowError("bins must be non-negative")
+        self.r_max = float(r_max)
+        self.q_max = float(q_max)
+        self.length_scale = 1.0 if length_scale is None else float(length_scale)
+        self.last_counts = None
+
+    def run(self, traj: Trajectory, system: System, chunk_frames: Optional[int] = None,
+            device: str = "auto"):
+        dev = Device.from_spec(device)
+        if traj.n_atoms() != system.n_atoms():
+            raise RuntimeError("mismatch: trajectory atom count does not match system")
+        if self.bins == 0 or self.q_bins == 0:
+            # structure_factor.rs:71-75
+            raise RuntimeError("invalid selection: structure factor requires bins and q_bins > 0")
+        ctx = dev.context()
+        n_atoms = system.n_atoms()
+        sel = self.selection
+        if sel.size and int(sel.max()) >= n_atoms:
+            raise RuntimeError("invalid selection: atom index out of range")
+        h = ctypes.c_void_p()
+        check(lib().wmd_rdf_create_f64(ctx.handle, ptr(sel), sel.size, self.bins,
+                                       ctypes.c_double(self.r_max), self.pbc,
+                                       ctypes.c_double(self.length_scale), ctypes.byref(h)))
+        frames = 0
+        volume_sum = 0.0
+        s3 = self.length_scale * self.length_scale * self.length_scale  # powi(3)
+        try:
+            tr = select_frames(traj, None)
+            tr.reset()
+            step = resolve_chunk_frames(chunk_frames, max(1, sel.size))
+            while True:
+                chunk = tr.read_chunk(step)
+                if chunk is None:
+                    break
+                coords, boxes = chunk
+                nf = coords.shape[0]
+                if self.pbc == _native.PBC_ORTHO:
+                    if boxes is None:
+                        raise RuntimeError("mismatch: structure factor requires orthorhombic box for PBC")
+                    for m in boxes:
+                        if np.any(m[[1, 2, 3, 5, 6, 7]] != 0):
+                            raise RuntimeError("mismatch: structure factor requires orthorhombic box for PBC")
+                        volume_sum += (float(m[0]) * float(m[4]) * float(m[8])) * s3
+                frames += nf
+                x4 = xyzw(coords)
+                check(lib().wmd_rdf_accumulate(h, ptr(x4), n_atoms, nf,
+                                               ptr(boxes if self.pbc != _native.PBC_NONE else None)))
+            counts = np.zeros(self.bins, np.uint64)
+            check(lib().wmd_rdf_read(h, ptr(counts), None))
+        finally:
+            lib().wmd_rdf_destroy(h)
+        self.last_counts = counts
+        r = np.zeros(self.bins, np.float32)
+        g = np.zeros(self.bins, np.float32)
+        q = np.zeros(self.q_bins, np.float32)
+        sq = np.zeros(self.q_bins, np.float32)
+        check(lib().wmd_sf_finalize(ptr(counts), self.bins, ctypes.c_double(self.r_max),
+                                    ctypes.c_double(float(sel.size)), frames,
+                                    ctypes.c_double(volume_sum), self.pbc, self.q_bins,
+                                    ctypes.c_double(self.q_max), ptr(r), ptr(g), ptr(q), ptr(sq)))
+        return r, g, q, sq
+
+
+def pairdist_centers(bins: int, w) -> np.ndarray:
+    """Bin centres i * w + 0.5 * w, each f32 operation rounded (rdf.rs:1276-1279, 459)."""
+    w = np.float32(w)
+    idx = np.arange(bins, dtype=np.float32)
+    return (idx * w + np.float32(0.5) * w).astype(np.float32)
+
+
+def pairdist_stats(counts, sums, sumsq, frames: int, w, compact: bool):
+    """pairdist_output_from_stats (rdf.rs:435-471): (centers, probability, std,
+    counts, frames) from the summed per-frame counts and their squares."""
+    counts = np.asarray(counts, np.uint64)
+    sums = np.asarray(sums, np.float64)
+    sumsq = np.asarray(sumsq, np.float64)
+    bins = counts.size
+    centers = pairdist_centers(bins, w)
+    ff = float(max(frames, 1))
+    keep = sums > 0.0 if compact else np.ones(bins, bool)
+    mean = sums / ff
+    if frames > 1:
+        var = np.maximum((sumsq - sums * sums / ff) / (ff - 1.0), 0.0)
+    else:
+        var = np.zeros(bins)
+    prob = (mean / float(np.float32(w))).astype(np.float32)
+    return (centers[keep], prob[keep], np.sqrt(var).astype(np.float32)[keep], counts[keep],
+            frames)
