@@ -62,6 +62,7 @@ typedef struct wmd_ctx wmd_ctx;
 typedef struct wmd_rdf wmd_rdf;
 typedef struct wmd_rmsd wmd_rmsd;
 typedef struct wmd_softrdf wmd_softrdf;
+typedef struct wmd_pairwise wmd_pairwise;
 
 /* ---- library / context --------------------------------------------------- */
 
@@ -208,6 +209,40 @@ wmd_status wmd_rmsd_backward_device(wmd_rmsd *rmsd, const float *d_store, size_t
                                     const double *d_cx, const float *d_gout, float *d_grad,
                                     void *stream);
 void wmd_rmsd_destroy(wmd_rmsd *rmsd);
+
+/* ---- Pairwise RMSD ---------------------------------------------------------- */
+
+/* PairwiseRmsdPlan metrics (crates/traj-engine/src/plans/rmsd/plan.rs:586-675). */
+enum { WMD_PAIRWISE_RMS = 0, WMD_PAIRWISE_NOFIT = 1, WMD_PAIRWISE_DME = 2 };
+
+/* PairwiseRmsdPlan::new + init (plan.rs:67-73, 514-519): the F x F RMSD matrix
+ * of one selection between every two frames, "rms" after optimal superposition
+ * (rmsd_aligned, plan.rs:1112-1159: f64 centroids, f64 cross-covariance, the
+ * superposition residual) or "nofit" (rmsd_raw, plan.rs:1081-1091).  The
+ * matrix is symmetric with a zero diagonal; a selection of fewer than 2 atoms
+ * gives the zero matrix (plan.rs:597-603).  The O(F^2 n) cross-covariances are
+ * one FP64 Gram product on the matrix cores.  WMD_PAIRWISE_DME has no device
+ * path (WMD_ERR_UNSUPPORTED). */
+wmd_status wmd_pairwise_create(wmd_ctx *ctx, const uint32_t *sel, size_t n_sel, int metric,
+                               wmd_pairwise **out);
+/* PairwiseRmsdPlan::process_chunk (plan.rs:533-558): appends the selected atoms
+ * of a host chunk [n_frames][n_atoms][4] to the plan's device store. */
+wmd_status wmd_pairwise_append(wmd_pairwise *pw, const float *coords_xyzw, size_t n_atoms,
+                               size_t n_frames);
+wmd_status wmd_pairwise_frames(const wmd_pairwise *pw, size_t *n_frames);
+/* PairwiseRmsdPlan::finalize (plan.rs:586-675) over the appended frames:
+ * out [F][F] f32, row-major (synchronous). */
+wmd_status wmd_pairwise_finalize(wmd_pairwise *pw, float *out);
+/* Store stride (floats per coordinate plane) of the appended store and of the
+ * stores wmd_pairwise_run_device expects at minimum. */
+wmd_status wmd_pairwise_store_stride(const wmd_pairwise *pw, size_t *stride);
+/* Hot path on a device store [n_frames][3][stride] holding the selection's
+ * atoms in slots 0 .. n_sel - 1 (16-byte aligned): d_out [F][F] f32 and
+ * d_out64 (NULL or [F][F] f64, the value before the cast).  Asynchronous on
+ * `stream`. */
+wmd_status wmd_pairwise_run_device(wmd_pairwise *pw, const float *d_store, size_t stride,
+                                   size_t n_frames, float *d_out, double *d_out64, void *stream);
+void wmd_pairwise_destroy(wmd_pairwise *pw);
 
 /* Gaussian-smoothed RDF (no counterpart in the reference; the differentiable
  * RDF of structure-matching losses such as warp-cg's objectives,

@@ -16,6 +16,8 @@ The C file restates warp-md's CPU semantics (citations in ``wmd_oracle.c``):
 * ``pairdist_counts`` -- ``PairDistPlan`` pair loop, ``rdf.rs:1120-1178``
 * ``sf_counts``, ``sf_finalize`` -- ``StructureFactorPlan``,
   ``crates/traj-engine/src/plans/analysis/structure_factor.rs:160-258``
+* ``pairwise_rmsd``   -- ``PairwiseRmsdPlan::finalize`` (rms / nofit),
+  ``rmsd/plan.rs:586-675, 1081-1159``
 
 ``rdf_finalize_py`` below is a second, independent float64/float32 numpy
 restatement of the normalisation used to cross-check the C version.
@@ -82,6 +84,8 @@ def lib():
     L.orc_sf_finalize.argtypes = [P, S, ctypes.c_double, ctypes.c_double, S, ctypes.c_double,
                                   ctypes.c_int, S, ctypes.c_double, P, P, P, P]
     L.orc_sf_finalize.restype = ctypes.c_int
+    L.orc_pairwise_rmsd.argtypes = [P, S, S, S, P, S, ctypes.c_int, P, P]
+    L.orc_pairwise_rmsd.restype = ctypes.c_int
     L.orc_omp_max_threads.argtypes = []
     L.orc_omp_max_threads.restype = ctypes.c_int
     _lib = L
@@ -236,6 +240,21 @@ def rmsd_plan(coords, sel, reference, align=True):
                          int(bool(align)), _ptr(out), _ptr(out64))
     if st != 0:
         raise RuntimeError(f"oracle rmsd failed {st}")
+    return out, out64
+
+
+def pairwise_rmsd(coords, sel, metric="rms"):
+    """PairwiseRmsdPlan CPU path; returns ((F, F) f32, (F, F) f64 before the cast)."""
+    L = lib()
+    c = _coords(coords)
+    F, N, stride = c.shape
+    s = np.ascontiguousarray(sel, dtype=np.uint32)
+    out = np.zeros((F, F), np.float32)
+    out64 = np.zeros((F, F), np.float64)
+    m = {"rms": 0, "nofit": 1}[metric]
+    st = L.orc_pairwise_rmsd(_ptr(c), stride, N, F, _ptr(s), s.size, m, _ptr(out), _ptr(out64))
+    if st != 0:
+        raise RuntimeError(f"oracle pairwise rmsd failed {st}")
     return out, out64
 
 

@@ -26,6 +26,9 @@
  *   orc_min_image_f64    pbc_math.rs:166-221 known vectors.
  *   orc_pairdist         reference known answers (correlation_structure_tests.rs:2193-2231,
  *                        2313-2422); equal to orc_rdf_brute where the pair rules coincide.
+ *   orc_pairwise_rmsd    pairwise_rmsd_plan_basic (geometry_matrix_tests.rs:1438-1460),
+ *                        equal to orc_rmsd_plan per pair where the two coincide, numpy.linalg.svd
+ *                        Kabsch, rigid-motion zero, symmetry, triangle inequality (nofit).
  *   orc_sf_counts / orc_sf_finalize
  *                        structure_factor_shapes (correlation_structure_tests.rs:759-780),
  *                        an independent numpy float64 restatement, and the ideal-gas limit.
@@ -724,6 +727,106 @@ int orc_fit_rmsd(const float *coords, size_t stride, size_t n_atoms, size_t n_fr
         if (cx_out) for (int d = 0; d < 3; ++d) cx_out[3 * f + d] = cx[d];
         if (cy_out) for (int d = 0; d < 3; ++d) cy_out[3 * f + d] = cy[d];
     }
+    return ORC_OK;
+}
+
+/* rmsd/plan.rs:1112-1159 rmsd_aligned(frame, reference): f64 centroids
+ * (math.rs:40-46, sum / n), H = sum (x - cx)(y - cy)^T, Kabsch R = V U^T with
+ * the reflection fix, residual sum |R (x - cx) - (y - cy)|^2, sqrt(sum / n). */
+static double rmsd_aligned_pair(const float *x, const float *y, size_t n, size_t stride) {
+    if (n == 0) return 0.0;
+    double cx[3] = {0, 0, 0}, cy[3] = {0, 0, 0};
+    for (size_t i = 0; i < n; ++i)
+        for (int d = 0; d < 3; ++d) {
+            cx[d] += (double)x[i * stride + d];
+            cy[d] += (double)y[i * stride + d];
+        }
+    for (int d = 0; d < 3; ++d) {
+        cx[d] = cx[d] / (double)n;
+        cy[d] = cy[d] / (double)n;
+    }
+    double h[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    for (size_t i = 0; i < n; ++i) {
+        double xr[3], yr[3];
+        for (int d = 0; d < 3; ++d) {
+            xr[d] = (double)x[i * stride + d] - cx[d];
+            yr[d] = (double)y[i * stride + d] - cy[d];
+        }
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) h[a][b] += xr[a] * yr[b];
+    }
+    double r[3][3];
+    if (!kabsch_from_h(h, r)) {
+        /* rmsd_raw fallback (plan.rs:1081-1091, 1137-1140) */
+        double sum = 0.0;
+        for (size_t i = 0; i < n; ++i)
+            for (int d = 0; d < 3; ++d) {
+                double e = (double)x[i * stride + d] - (double)y[i * stride + d];
+                sum += e * e;
+            }
+        return sqrt(sum / (double)n);
+    }
+    double sum = 0.0;
+    for (size_t i = 0; i < n; ++i) {
+        double xr[3], yr[3];
+        for (int d = 0; d < 3; ++d) {
+            xr[d] = (double)x[i * stride + d] - cx[d];
+            yr[d] = (double)y[i * stride + d] - cy[d];
+        }
+        double dd = 0.0;
+        for (int a = 0; a < 3; ++a) {
+            double al = r[a][0] * xr[0] + r[a][1] * xr[1] + r[a][2] * xr[2];
+            double e = al - yr[a];
+            dd += e * e;
+        }
+        sum += dd;
+    }
+    return sqrt(sum / (double)n);
+}
+
+/*
+ * PairwiseRmsdPlan::finalize (rmsd/plan.rs:586-675) for metric 0 = rms
+ * (rmsd_aligned) and 1 = nofit (rmsd_raw, :1081-1091, f64 sum, sqrt(sum / n)).
+ * coords: [n_frames][n_atoms][stride]; the plan keeps the selected atoms of
+ * every frame (:533-558).  out [F][F] f32 (symmetric, zero diagonal; all zero
+ * when n_sel < 2), out64 NULL or the f64 values before the cast.
+ */
+int orc_pairwise_rmsd(const float *coords, size_t stride, size_t n_atoms, size_t n_frames,
+                      const uint32_t *sel, size_t n_sel, int metric, float *out, double *out64) {
+    if (metric != 0 && metric != 1) return ORC_ERR_INVALID;
+    for (size_t i = 0; i < n_sel; ++i)
+        if (sel[i] >= n_atoms) return ORC_ERR_INVALID;
+    for (size_t t = 0; t < n_frames * n_frames; ++t) {
+        out[t] = 0.0f;
+        if (out64) out64[t] = 0.0;
+    }
+    if (n_sel < 2) return ORC_OK;
+    float *fs = (float *)malloc(n_frames * n_sel * 3 * sizeof(float));
+    if (!fs) return ORC_ERR_INVALID;
+    for (size_t f = 0; f < n_frames; ++f)
+        for (size_t i = 0; i < n_sel; ++i)
+            for (int d = 0; d < 3; ++d)
+                fs[(f * n_sel + i) * 3 + d] = coords[(f * n_atoms + sel[i]) * stride + d];
+    for (size_t i = 0; i < n_frames; ++i)
+        for (size_t j = i + 1; j < n_frames; ++j) {
+            const float *x = fs + i * n_sel * 3, *y = fs + j * n_sel * 3;
+            double v;
+            if (metric == 0) {
+                v = rmsd_aligned_pair(x, y, n_sel, 3);
+            } else {
+                double sum = 0.0;
+                for (size_t k = 0; k < n_sel; ++k) {
+                    double dx = (double)x[3 * k] - (double)y[3 * k];
+                    double dy = (double)x[3 * k + 1] - (double)y[3 * k + 1];
+                    double dz = (double)x[3 * k + 2] - (double)y[3 * k + 2];
+                    sum += dx * dx + dy * dy + dz * dz;
+                }
+                v = sqrt(sum / (double)n_sel);
+            }
+            out[i * n_frames + j] = out[j * n_frames + i] = (float)v;
+            if (out64) out64[i * n_frames + j] = out64[j * n_frames + i] = v;
+        }
+    free(fs);
     return ORC_OK;
 }
 

@@ -1,0 +1,434 @@
+// PairwiseRmsdPlan on the device: the F x F matrix of RMSDs between every two
+// frames of one selection, after optimal superposition ("rms",
+// crates/traj-engine/src/plans/rmsd/plan.rs:654-668 with rmsd_aligned
+// :1112-1159) or without it ("nofit", rmsd_raw :1081-1091).
+//
+// The reference fits every pair separately (O(F^2 n) scalar work on the CPU).
+// Here the O(F^2 n) part is one symmetric Gram product on the FP64 matrix
+// cores: with X the (3F x n) matrix of centred coordinates (row 3f + d holds
+// frame f's d coordinate of every atom, minus the frame's f64 centroid),
+// G = X X^T holds in its 3 x 3 block (i, j) exactly the cross-covariance
+// H_ij = sum_k (x_ik - c_i)(x_jk - c_j)^T the reference accumulates.  Each
+// workgroup computes a 96 x 96 tile of G (32 x 32 frame pairs) with
+// v_mfma_f64_16x16x4 and finishes its frame pairs in the epilogue: the FP64
+// QCP eigenvalue (wmd_qcp.hpp) for "rms", and
+//   sum |x_i - x_j|^2 = G_i + G_j - 2 tr H_ij + n |c_i - c_j|^2
+// for "nofit" (G_f = sum_k |x_fk - c_f|^2 from the centring pass).  Centring
+// before the product keeps every sum free of cancellation; all arithmetic is
+// f64 like the reference's, and the result is cast to f32 as it does.
+#include <algorithm>
+#include <cstring>
+#include <memory>
+
+#include "wmd_internal.hpp"
+#include "wmd_qcp.hpp"
+
+namespace {
+using namespace wmd;
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+constexpr int kPwBlock = 256;                      // 4 waves
+constexpr int kPwFrames = 32;                      // frames per tile side
+constexpr int kPwRows = 3 * kPwFrames;             // Gram rows per tile side
+constexpr int kPwKc = 32;                          // atoms per staged chunk
+constexpr int kPwQuads = kPwKc / 4;                // float4 groups per row and chunk
+constexpr int kPwS4 = kPwRows + 1;                 // quad-row stride (odd: conflict-free staging)
+constexpr int kPwPanel = kPwQuads * kPwS4 * 4;     // doubles per staged panel
+constexpr int kPwGs = kPwRows + 1;                 // row stride of the tile Gram in LDS
+constexpr int kPwLdsDoubles = (2 * kPwPanel > (kPwRows / 2) * kPwGs) ? 2 * kPwPanel : (kPwRows / 2) * kPwGs;
+
+// [F][n_src] xyzw chunk -> SoA [F][3][stride] of the selected atoms (pads 0).
+__global__ __launch_bounds__(kPwBlock) void pw_gather_kernel(const float4 *__restrict__ aos,
+                                                             long long n_src,
+                                                             const unsigned *__restrict__ idx,
+                                                             int n, int stride, int n_frames,
+                                                             float *__restrict__ soa) {
+    const long long total = (long long)stride * n_frames;
+    for (long long t = (long long)blockIdx.x * kPwBlock + threadIdx.x; t < total;
+         t += (long long)gridDim.x * kPwBlock) {
+        const int f = (int)(t / stride);
+        const int k = (int)(t - (long long)f * stride);
+        float x = 0.f, y = 0.f, z = 0.f;
+        if (k < n) {
+            const float4 v = aos[(long long)f * n_src + idx[k]];
+            x = v.x;
+            y = v.y;
+            z = v.z;
+        }
+        float *o = soa + (long long)f * 3 * stride;
+        o[k] = x;
+        o[stride + k] = y;
+        o[2 * stride + k] = z;
+    }
+}
+
+__device__ __forceinline__ double block_sum(double v, double *red) {
+    // 256 threads: wave sums through LDS, then one wave adds the four
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    __syncthreads();
+    if (lane == 0) red[w] = v;
+    __syncthreads();
+    return red[0] + red[1] + red[2] + red[3];
+}
+
+// Per frame: centroid c = (sum_k x_k) / n in f64 (rmsd/math.rs:40-46) and
+// G = sum_k |x_k - c|^2.  One workgroup per frame (grid-stride).
+__global__ __launch_bounds__(kPwBlock) void pw_center_kernel(const float *__restrict__ soa,
+                                                             int stride, int n, int n_frames,
+                                                             double *__restrict__ cen,
+                                                             double *__restrict__ gsum) {
+    __shared__ double red[4];
+    for (int f = blockIdx.x; f < n_frames; f += gridDim.x) {
+        const float *x = soa + (long long)f * 3 * stride;
+        double s[3] = {0.0, 0.0, 0.0};
+        for (int k = threadIdx.x; k < n; k += kPwBlock) {
+            s[0] += (double)x[k];
+            s[1] += (double)x[stride + k];
+            s[2] += (double)x[2 * stride + k];
+        }
+        double c[3];
+        for (int d = 0; d < 3; ++d) c[d] = block_sum(s[d], red) / (double)n;
+        double g = 0.0;
+        for (int k = threadIdx.x; k < n; k += kPwBlock) {
+            const double dx = (double)x[k] - c[0];
+            const double dy = (double)x[stride + k] - c[1];
+            const double dz = (double)x[2 * stride + k] - c[2];
+            g += dx * dx + dy * dy + dz * dz;
+        }
+        g = block_sum(g, red);
+        if (threadIdx.x == 0) {
+            cen[3 * f] = c[0];
+            cen[3 * f + 1] = c[1];
+            cen[3 * f + 2] = c[2];
+            gsum[f] = g;
+        }
+    }
+}
+
+// Stages rows [row0, row0 + 96) x atoms [k0, k0 + 32) of X (centred, f64) into
+// a panel laid out [quad][row][4]: element (k, r) at ((k / 4) * S4 + r) * 4 + k % 4.
+__device__ __forceinline__ void pw_stage(double *panel, const float *__restrict__ soa, int stride,
+                                         int n, int rows, int row0, int k0,
+                                         const double *__restrict__ cen) {
+#pragma unroll
+    for (int e = 0; e < (kPwRows * kPwQuads) / kPwBlock; ++e) {
+        const int idx = e * kPwBlock + threadIdx.x;
+        const int r = idx / kPwQuads;
+        const int q = idx - r * kPwQuads;
+        const int k = k0 + 4 * q;
+        double v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0;
+        if (row0 + r < rows && k < n) {
+            const float4 x = *(const float4 *)(soa + (long long)(row0 + r) * stride + k);
+            const double c = cen[row0 + r];
+            v0 = (double)x.x - c;
+            if (k + 1 < n) v1 = (double)x.y - c;
+            if (k + 2 < n) v2 = (double)x.z - c;
+            if (k + 3 < n) v3 = (double)x.w - c;
+        }
+        double *p = panel + (q * kPwS4 + r) * 4;
+        p[0] = v0;
+        p[1] = v1;
+        p[2] = v2;
+        p[3] = v3;
+    }
+}
+
+// One 96 x 96 tile of G per workgroup (tile list: upper triangle, ti <= tj),
+// then the RMSD of its frame pairs.  Wave w owns rows 48 (w / 2) .. + 48 and
+// columns 48 (w % 2) .. + 48 as 3 x 3 blocks of 16 x 16.
+template <bool FIT>
+__global__ __launch_bounds__(kPwBlock) void pw_gram_kernel(
+    const float *__restrict__ soa, int stride, int n, int n_frames,
+    const double *__restrict__ cen, const double *__restrict__ gsum,
+    const int2 *__restrict__ tiles, int n_tiles, int per_xcd, int n_xcd, float *__restrict__ out,
+    double *__restrict__ out64) {
+    extern __shared__ __attribute__((aligned(16))) double lds[];
+    // XCD-aware order: workgroups on one XCD take consecutive tiles (same row panel)
+    const int b = blockIdx.x;
+    const int t = (b % n_xcd) * per_xcd + b / n_xcd;
+    if (t >= n_tiles) return;
+    const int2 tp = tiles[t];
+    const int rows = 3 * n_frames;
+    const int ra = kPwRows * tp.x, rb = kPwRows * tp.y;
+    double *pa = lds;
+    double *pb = lds + kPwPanel;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int mb0 = 3 * (w >> 1), nb0 = 3 * (w & 1);
+    d4 acc[3][3];
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) acc[m][j] = (d4){0.0, 0.0, 0.0, 0.0};
+    const int lr = lane & 15, lk = lane >> 4;
+    for (int k0 = 0; k0 < n; k0 += kPwKc) {
+        __syncthreads();  // the previous chunk is consumed
+        pw_stage(pa, soa, stride, n, rows, ra, k0, cen);
+        pw_stage(pb, soa, stride, n, rows, rb, k0, cen);
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < kPwQuads; ++q) {
+            double av[3], bv[3];
+#pragma unroll
+            for (int m = 0; m < 3; ++m) av[m] = pa[(q * kPwS4 + 16 * (mb0 + m) + lr) * 4 + lk];
+#pragma unroll
+            for (int j = 0; j < 3; ++j) bv[j] = pb[(q * kPwS4 + 16 * (nb0 + j) + lr) * 4 + lk];
+#pragma unroll
+            for (int m = 0; m < 3; ++m)
+#pragma unroll
+                for (int j = 0; j < 3; ++j)
+                    acc[m][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[m], bv[j], acc[m][j], 0, 0, 0);
+        }
+    }
+    // Epilogue in two halves of 16 frames x 32 frames: the waves owning rows
+    // 48 h .. 48 h + 47 put their blocks in LDS (lane l, register r of a block
+    // holds row 4 r + l / 16, column l % 16), then every thread finishes pairs.
+    double *gt = lds;
+    const double nd = (double)n;
+    const int jj = threadIdx.x & 31;
+    for (int half = 0; half < 2; ++half) {
+        __syncthreads();
+        if ((w >> 1) == half) {
+#pragma unroll
+            for (int m = 0; m < 3; ++m)
+#pragma unroll
+                for (int j = 0; j < 3; ++j)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r)
+                        gt[(16 * m + 4 * r + lk) * kPwGs + 16 * (nb0 + j) + lr] = acc[m][j][r];
+        }
+        __syncthreads();
+        for (int il = threadIdx.x >> 5; il < kPwFrames / 2; il += kPwBlock / 32) {
+            const int ii = (kPwFrames / 2) * half + il;
+            const int i = kPwFrames * tp.x + ii, j = kPwFrames * tp.y + jj;
+            if (i >= n_frames || j >= n_frames) continue;
+            if (tp.x == tp.y && ii >= jj) {
+                if (ii == jj) {
+                    out[(long long)i * n_frames + i] = 0.f;
+                    if (out64) out64[(long long)i * n_frames + i] = 0.0;
+                }
+                continue;
+            }
+            double M[9];
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) M[3 * a + c] = gt[(3 * il + a) * kPwGs + 3 * jj + c];
+            const double gi = gsum[i], gj = gsum[j];
+            double msd;
+            if (FIT) {
+                QcpResult res;
+                qcp_solve(M, gi, gj, false, &res);
+                msd = 2.0 * (res.e0 - res.lambda) / nd;
+            } else {
+                const double dx = cen[3 * i] - cen[3 * j], dy = cen[3 * i + 1] - cen[3 * j + 1],
+                             dz = cen[3 * i + 2] - cen[3 * j + 2];
+                msd = (gi + gj - 2.0 * (M[0] + M[4] + M[8]) + nd * (dx * dx + dy * dy + dz * dz)) / nd;
+            }
+            const double v = sqrt(msd > 0.0 ? msd : 0.0);
+            out[(long long)i * n_frames + j] = (float)v;
+            out[(long long)j * n_frames + i] = (float)v;
+            if (out64) {
+                out64[(long long)i * n_frames + j] = v;
+                out64[(long long)j * n_frames + i] = v;
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(kPwBlock) void pw_zero_kernel(float *out, double *out64, long long n) {
+    for (long long t = (long long)blockIdx.x * kPwBlock + threadIdx.x; t < n;
+         t += (long long)gridDim.x * kPwBlock) {
+        out[t] = 0.f;
+        if (out64) out64[t] = 0.0;
+    }
+}
+
+int pw_grid(long long total) {
+    return (int)std::max<long long>(1, std::min<long long>((total + kPwBlock - 1) / kPwBlock, 65536));
+}
+
+}  // namespace
+
+struct wmd_pairwise {
+    wmd_ctx *ctx = nullptr;
+    std::vector<uint32_t> sel;
+    int metric = WMD_PAIRWISE_RMS;
+    int stride = 0;
+    size_t frames = 0, cap = 0;
+    DevBuf d_sel, d_aos, d_cen, d_g, d_out, d_tiles;
+    std::unique_ptr<DevBuf> d_store;
+    HostBuf h_stage;
+    std::vector<int2> tiles;
+};
+
+namespace {
+
+// The F x F matrix of a store [F][3][stride] on stream st.
+void pw_run(wmd_pairwise *h, const float *d_soa, size_t stride, size_t F, float *d_out,
+            double *d_out64, hipStream_t st) {
+    const int n = (int)h->sel.size();
+    if (F == 0) return;
+    WMD_REQUIRE(stride >= (size_t)n && stride % 4 == 0, WMD_ERR_INVALID,
+                "stride must be a multiple of 4 and >= the selection size");
+    WMD_REQUIRE(F <= 46000, WMD_ERR_UNSUPPORTED, "at most 46000 frames (F x F output)");
+    WMD_REQUIRE(3 * F * stride < (1ull << 40), WMD_ERR_UNSUPPORTED, "store too large");
+    if (n < 2) {  // plan.rs:597-603: a zero matrix
+        hipLaunchKernelGGL(pw_zero_kernel, dim3(pw_grid((long long)(F * F))), dim3(kPwBlock), 0, st,
+                           d_out, d_out64, (long long)(F * F));
+        WMD_HIP_CHECK(hipGetLastError());
+        return;
+    }
+    double *cen = (double *)h->d_cen.reserve(F * 3 * sizeof(double));
+    double *g = (double *)h->d_g.reserve(F * sizeof(double));
+    {
+        ProfScope prof(h->ctx, st, "pairwise_center");
+        hipLaunchKernelGGL(pw_center_kernel, dim3((unsigned)std::min<size_t>(F, 65536)), dim3(kPwBlock),
+                           0, st, d_soa, (int)stride, n, (int)F, cen, g);
+        WMD_HIP_CHECK(hipGetLastError());
+    }
+    const int T = (int)((F + kPwFrames - 1) / kPwFrames);
+    const size_t nt = (size_t)T * (T + 1) / 2;
+    if (h->tiles.size() != nt) {
+        h->tiles.clear();
+        h->tiles.reserve(nt);
+        for (int i = 0; i < T; ++i)
+            for (int j = i; j < T; ++j) h->tiles.push_back(make_int2(i, j));
+        int2 *d = (int2 *)h->d_tiles.reserve(nt * sizeof(int2));
+        WMD_HIP_CHECK(hipMemcpyAsync(d, h->tiles.data(), nt * sizeof(int2), hipMemcpyHostToDevice, st));
+        WMD_HIP_CHECK(hipStreamSynchronize(st));
+    }
+    const int n_xcd = std::max(1, h->ctx->xcd_count);
+    const int per_xcd = (int)((nt + n_xcd - 1) / n_xcd);
+    const size_t lds = (size_t)kPwLdsDoubles * sizeof(double);
+    ProfScope prof(h->ctx, st, "pairwise_gram");
+    auto kern = h->metric == WMD_PAIRWISE_RMS ? pw_gram_kernel<true> : pw_gram_kernel<false>;
+    hipLaunchKernelGGL(kern, dim3((unsigned)(per_xcd * n_xcd)), dim3(kPwBlock), lds, st, d_soa,
+                       (int)stride, n, (int)F, (const double *)cen, (const double *)g,
+                       (const int2 *)h->d_tiles.ptr, (int)nt, per_xcd, n_xcd, d_out, d_out64);
+    WMD_HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace
+
+extern "C" {
+
+wmd_status wmd_pairwise_create(wmd_ctx *ctx, const uint32_t *sel, size_t n_sel, int metric,
+                               wmd_pairwise **out) {
+    WMD_API_BEGIN
+    WMD_REQUIRE(ctx && out, WMD_ERR_INVALID, "null context or output handle");
+    *out = nullptr;
+    WMD_REQUIRE(n_sel == 0 || sel, WMD_ERR_INVALID, "null selection");
+    WMD_REQUIRE(metric == WMD_PAIRWISE_RMS || metric == WMD_PAIRWISE_NOFIT ||
+                    metric == WMD_PAIRWISE_DME,
+                WMD_ERR_PARSE, "metric must be rms, nofit or dme");
+    WMD_REQUIRE(metric != WMD_PAIRWISE_DME, WMD_ERR_UNSUPPORTED,
+                "the dme metric has no device path");
+    WMD_REQUIRE(n_sel < (1u << 30), WMD_ERR_UNSUPPORTED, "selection too large");
+    bind(ctx);
+    auto h = std::make_unique<wmd_pairwise>();
+    h->ctx = ctx;
+    h->metric = metric;
+    h->sel.assign(sel, sel + n_sel);
+    h->stride = (int)round_up(std::max<size_t>(n_sel, 1), 4);
+    h->d_sel.reserve(std::max<size_t>(n_sel, 1) * 4);
+    if (n_sel)
+        WMD_HIP_CHECK(hipMemcpy(h->d_sel.ptr, sel, n_sel * 4, hipMemcpyHostToDevice));
+    *out = h.release();
+    WMD_API_END
+}
+
+wmd_status wmd_pairwise_append(wmd_pairwise *h, const float *coords_xyzw, size_t n_atoms,
+                               size_t n_frames) {
+    WMD_API_BEGIN
+    WMD_REQUIRE(h != nullptr, WMD_ERR_INVALID, "null handle");
+    if (n_frames == 0) return WMD_OK;
+    WMD_REQUIRE(coords_xyzw != nullptr, WMD_ERR_INVALID, "null coords");
+    for (uint32_t a : h->sel)
+        WMD_REQUIRE(a < n_atoms, WMD_ERR_MISMATCH,
+                    "selection index " + std::to_string(a) + " out of range for " +
+                        std::to_string(n_atoms) + " atoms");
+    bind(h->ctx);
+    hipStream_t st = h->ctx->stream;
+    const size_t fl = (size_t)3 * h->stride;  // floats per stored frame
+    if (h->frames + n_frames > h->cap) {
+        const size_t cap = std::max({h->frames + n_frames, 2 * h->cap, (size_t)64});
+        auto nb = std::make_unique<DevBuf>();
+        nb->reserve(cap * fl * 4);
+        if (h->frames)
+            WMD_HIP_CHECK(hipMemcpyAsync(nb->ptr, h->d_store->ptr, h->frames * fl * 4,
+                                         hipMemcpyDeviceToDevice, st));
+        WMD_HIP_CHECK(hipStreamSynchronize(st));
+        h->d_store = std::move(nb);
+        h->cap = cap;
+    }
+    const size_t fbytes = n_atoms * 16;
+    const size_t batch = std::max<size_t>(1, std::min<size_t>(n_frames, (256ull << 20) / std::max<size_t>(fbytes, 1)));
+    float4 *d_aos = (float4 *)h->d_aos.reserve(batch * fbytes);
+    void *stage = h->h_stage.reserve(batch * fbytes);
+    for (size_t f0 = 0; f0 < n_frames; f0 += batch) {
+        const size_t nf = std::min(batch, n_frames - f0);
+        WMD_HIP_CHECK(hipStreamSynchronize(st));
+        std::memcpy(stage, coords_xyzw + f0 * n_atoms * 4, nf * fbytes);
+        WMD_HIP_CHECK(hipMemcpyAsync(d_aos, stage, nf * fbytes, hipMemcpyHostToDevice, st));
+        float *dst = (float *)h->d_store->ptr + (h->frames + f0) * fl;
+        hipLaunchKernelGGL(pw_gather_kernel, dim3(pw_grid((long long)h->stride * nf)), dim3(kPwBlock), 0,
+                           st, (const float4 *)d_aos, (long long)n_atoms, h->d_sel.as<unsigned>(),
+                           (int)h->sel.size(), h->stride, (int)nf, dst);
+        WMD_HIP_CHECK(hipGetLastError());
+    }
+    WMD_HIP_CHECK(hipStreamSynchronize(st));
+    h->frames += n_frames;
+    WMD_API_END
+}
+
+wmd_status wmd_pairwise_frames(const wmd_pairwise *h, size_t *n_frames) {
+    WMD_API_BEGIN
+    WMD_REQUIRE(h && n_frames, WMD_ERR_INVALID, "null handle");
+    *n_frames = h->frames;
+    WMD_API_END
+}
+
+wmd_status wmd_pairwise_finalize(wmd_pairwise *h, float *out) {
+    WMD_API_BEGIN
+    WMD_REQUIRE(h != nullptr, WMD_ERR_INVALID, "null handle");
+    const size_t F = h->frames;
+    if (F == 0) return WMD_OK;
+    WMD_REQUIRE(out != nullptr, WMD_ERR_INVALID, "null output");
+    bind(h->ctx);
+    hipStream_t st = h->ctx->stream;
+    float *d_out = (float *)h->d_out.reserve(F * F * 4);
+    pw_run(h, (const float *)h->d_store->ptr, (size_t)h->stride, F, d_out, nullptr, st);
+    WMD_HIP_CHECK(hipMemcpyAsync(out, d_out, F * F * 4, hipMemcpyDeviceToHost, st));
+    WMD_HIP_CHECK(hipStreamSynchronize(st));
+    WMD_API_END
+}
+
+wmd_status wmd_pairwise_store_stride(const wmd_pairwise *h, size_t *stride) {
+    WMD_API_BEGIN
+    WMD_REQUIRE(h && stride, WMD_ERR_INVALID, "null handle");
+    *stride = (size_t)h->stride;
+    WMD_API_END
+}
+
+wmd_status wmd_pairwise_run_device(wmd_pairwise *h, const float *d_store, size_t stride,
+                                   size_t n_frames, float *d_out, double *d_out64, void *stream) {
+    WMD_API_BEGIN
+    WMD_REQUIRE(h != nullptr, WMD_ERR_INVALID, "null handle");
+    WMD_REQUIRE(n_frames == 0 || (d_store && d_out), WMD_ERR_INVALID, "null device buffer");
+    WMD_REQUIRE(((uintptr_t)d_store & 15) == 0, WMD_ERR_INVALID, "device store must be 16-byte aligned");
+    bind(h->ctx);
+    pw_run(h, d_store, stride, n_frames, d_out, d_out64, pick_stream(h->ctx, stream));
+    WMD_API_END
+}
+
+void wmd_pairwise_destroy(wmd_pairwise *h) {
+    if (!h) return;
+    int cur = 0;
+    if (hipGetDevice(&cur) == hipSuccess && h->ctx) (void)hipSetDevice(h->ctx->device);
+    delete h;
+    (void)hipSetDevice(cur);
+}
+
+}  // extern "C"

@@ -12,11 +12,13 @@ from ._native import WmdError, device_count, library_built
 from .analysis import rdf
 from .executor import Device, normalize_frame_indices
 from .batch import UnevenBatch
-from .plans import PairDistPlan, RdfPlan, RmsdPlan, StructureFactorPlan, finalize_rdf
+from .plans import (PairDistPlan, PairwiseRmsdPlan, RdfPlan, RmsdPlan, StructureFactorPlan,
+                    finalize_rdf)
 from .system import Selection, System, Trajectory
 
 __all__ = [
-    "Device", "PairDistPlan", "RdfPlan", "RmsdPlan", "StructureFactorPlan", "Selection", "System", "Trajectory", "UnevenBatch", "WmdError",
+    "Device", "PairDistPlan", "PairwiseRmsdPlan", "RdfPlan", "RmsdPlan", "StructureFactorPlan",
+    "Selection", "System", "Trajectory", "UnevenBatch", "WmdError",
     "device_count", "finalize_rdf", "library_built", "normalize_frame_indices", "rdf",
 ]
 __version__ = "0.1.0"

@@ -1,5 +1,6 @@
-"""RdfPlan and RmsdPlan with warp-md's Python signatures, running on the
-gfx950 engine through its C ABI.
+"""RdfPlan, RmsdPlan and their neighbours (PairDistPlan, StructureFactorPlan,
+PairwiseRmsdPlan) with warp-md's Python signatures, running on the gfx950
+engine through its C ABI.
 
 RdfPlan mirrors PyRdfPlan (crates/traj-py/src/analysis.rs:2298-2399) over
 RdfPlan (crates/traj-engine/src/plans/rdf.rs:141-236, 653-1016); RmsdPlan
@@ -462,6 +463,60 @@ class StructureFactorPlan:
                                     ctypes.c_double(volume_sum), self.pbc, self.q_bins,
                                     ctypes.c_double(self.q_max), ptr(r), ptr(g), ptr(q), ptr(sq)))
         return r, g, q, sq
+
+
+_PAIRWISE_METRICS = {"rms": 0, "nofit": 1, "dme": 2}
+
+
+class PairwiseRmsdPlan:
+    """F x F RMSD matrix between every two frames of one selection.
+
+    Mirrors PyPairwiseRmsdPlan (crates/traj-py/src/io.rs:2759-2800) over
+    PairwiseRmsdPlan (crates/traj-engine/src/plans/rmsd/plan.rs:504-675):
+    metric "rms" (after optimal superposition) or "nofit"; "dme" has no
+    device path and is refused.  Frames go to a device store as they are
+    read; the matrix is one FP64 Gram product plus per-pair QCP
+    (``wmd_pairwise_*``).  Returns a (F, F) float32 array.
+    """
+
+    def __init__(self, selection, metric: str = "rms", pbc: str = "none"):
+        if metric not in _PAIRWISE_METRICS:
+            raise RuntimeError("metric must be 'rms', 'nofit', or 'dme'")  # pack.rs:2675-2684
+        self.metric = metric
+        self.pbc = _parse_pbc(pbc)
+        self.selection = _indices(selection)
+
+    def run(self, traj: Trajectory, system: System, chunk_frames: Optional[int] = None,
+            device: str = "auto", frame_indices: Optional[Sequence[int]] = None):
+        dev = Device.from_spec(device)
+        if traj.n_atoms() != system.n_atoms():
+            raise RuntimeError("mismatch: trajectory atom count does not match system")
+        ctx = dev.context()
+        n_atoms = system.n_atoms()
+        sel = self.selection
+        if sel.size and int(sel.max()) >= n_atoms:
+            raise RuntimeError("invalid selection: atom index out of range")
+        h = ctypes.c_void_p()
+        check(lib().wmd_pairwise_create(ctx.handle, ptr(sel), sel.size,
+                                        _PAIRWISE_METRICS[self.metric], ctypes.byref(h)))
+        try:
+            tr = select_frames(traj, frame_indices)
+            tr.reset()
+            step = resolve_chunk_frames(chunk_frames, max(1, sel.size))
+            while True:
+                chunk = tr.read_chunk(step)
+                if chunk is None:
+                    break
+                coords, _ = chunk
+                x4 = xyzw(coords)
+                check(lib().wmd_pairwise_append(h, ptr(x4), n_atoms, coords.shape[0]))
+            nf = ctypes.c_size_t()
+            check(lib().wmd_pairwise_frames(h, ctypes.byref(nf)))
+            out = np.zeros((nf.value, nf.value), np.float32)
+            check(lib().wmd_pairwise_finalize(h, ptr(out)))
+        finally:
+            lib().wmd_pairwise_destroy(h)
+        return out
 
 
 def pairdist_centers(bins: int, w) -> np.ndarray:
