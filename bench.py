@@ -29,6 +29,7 @@ sys.path.insert(0, os.path.join(ROOT, "warp-md_amd"))
 
 VALU_PEAK_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12  # lane-ops/s: 256 CU x 4 SIMD x 16 lanes x 2.4 GHz
 HBM_PEAK_GBS = 8000.0
+MFMA_F64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 matrix
 RDF_OPS_PER_PAIR = 24  # algorithmic VALU lane-ops per unordered in-range pair
 
 
@@ -186,6 +187,81 @@ def soft_rdf_ratio(ctx, dev, n=20000, frames=64, reps=3):
             "workload": "20,000 beads, cubic 62.996 A, 64 frames, 300 bins, r_max 12 A, sigma 0.05 A"}
 
 
+def pairwise_bench(ctx, dev, F=5000, reps=3, cpu_frames=600):
+    """PairwiseRmsdPlan (metric rms) over the 5,000 chain frames, fit on the
+    2,000 CA atoms: the 5000 x 5000 RMSD matrix from a resident store.  The
+    Gram kernel's algorithmic work is 9 cross-covariance entries x n atoms x
+    2 flops per frame pair i < j; the CPU sample is the oracle's
+    PairwiseRmsdPlan on the first `cpu_frames` frames (one thread)."""
+    import numpy as np
+    import torch
+
+    from warp_md_amd import workloads
+    from warp_md_amd._native import check, lib, ptr
+
+    L = lib()
+    c = workloads.chain(n_frames=0, seed=2)
+    ca = c["ca"]
+    n = int(ca.size)
+    store = workloads.chain_store_device(c["ref"], ca, F, seed=2, device=dev)
+    sel = np.arange(n, dtype=np.uint32)
+    h = ctypes.c_void_p()
+    check(L.wmd_pairwise_create(ctx.handle, ptr(sel), n, 0, ctypes.byref(h)))
+    out = torch.empty((F, F), dtype=torch.float32, device=dev)
+    stream = torch.cuda.current_stream().cuda_stream
+
+    def run():
+        check(L.wmd_pairwise_run_device(h, ctypes.c_void_p(store.data_ptr()), store.shape[2], F,
+                                        ctypes.c_void_p(out.data_ptr()), None,
+                                        ctypes.c_void_p(stream)))
+
+    run()
+    torch.cuda.synchronize()
+    ctx.profile(True)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        run()
+    torch.cuda.synchronize()
+    wall_ms = (time.perf_counter() - t0) / reps * 1e3
+    gram_ms, gram_n = ctx.profile_read("pairwise_gram")
+    cen_ms, _ = ctx.profile_read("pairwise_center")
+    ctx.profile(False)
+    L.wmd_pairwise_destroy(h)
+    gram_avg = gram_ms / max(gram_n, 1)
+    pairs = F * (F - 1) / 2
+    flops = pairs * 9 * n * 2
+    ach = flops / (gram_avg / 1e3) / 1e12
+    m = out.cpu().numpy()
+    frames = np.transpose(store[:cpu_frames].cpu().numpy(), (0, 2, 1))
+    del store, out
+    torch.cuda.empty_cache()
+    cpu = None
+    try:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        t0 = time.perf_counter()
+        oracle.pairwise_rmsd(frames, np.arange(n), "rms")
+        dt = time.perf_counter() - t0
+        cpu = {"value": cpu_frames * (cpu_frames - 1) / 2 / dt, "unit": "frame pairs/s",
+               "cores": 1, "kind": "port",
+               "sample": f"first {cpu_frames} frames ({cpu_frames * (cpu_frames - 1) // 2} pairs, "
+                         f"{dt:.1f} s, 1 thread, SVD Kabsch per pair as the reference)"}
+    except Exception as e:  # the oracle is a checker; its absence leaves the GPU line intact
+        log(f"pairwise cpu baseline skipped: {e}")
+    log(f"pairwise rmsd: {F} x {F} matrix, {n} atoms: {wall_ms:.2f} ms wall, gram kernel "
+        f"{gram_avg:.2f} ms ({ach:.1f} TFLOP/s f64), centring {cen_ms / max(reps, 1):.3f} ms")
+    return {"frames": F, "atoms": n, "ms": wall_ms, "pairs_per_s": pairs / (wall_ms / 1e3),
+            "checks": {"mean": float(m[np.triu_indices(F, 1)].mean()),
+                       "symmetric": bool(np.array_equal(m, m.T))},
+            "roofline": {"kernel": "pw_gram_kernel", "bound": "mfma", "achieved": ach,
+                         "peak": MFMA_F64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                         "frac": ach / MFMA_F64_PEAK_TFLOPS, "traffic": None,
+                         "algorithmic": f"9 x {n} x 2 f64 flops per frame pair x {pairs:.4g} pairs",
+                         "avg_launch_ms": gram_avg},
+            "cpu_baseline": cpu,
+            "workload": "PairwiseRmsdPlan metric rms, 5,000 chain frames, fit on 2,000 CA"}
+
+
 def uneven_batch_bench(ctx, dev, reps=3, autotune=True):
     """256 trajectories, atom counts log-uniform in [500, 50000], cubic boxes at
     0.1 atoms/A^3, 50 uniform-random frames each; RDF (300 bins, r_max 12 A)
@@ -279,6 +355,8 @@ def main():
     ap.add_argument("--no-triclinic", action="store_true")
     ap.add_argument("--no-soft", action="store_true", help="skip the soft-RDF ratio")
     ap.add_argument("--no-batch", action="store_true", help="skip the uneven batch")
+    ap.add_argument("--no-pairwise", action="store_true", help="skip the pairwise RMSD matrix")
+    ap.add_argument("--only-pairwise", action="store_true", help="time only the pairwise RMSD matrix")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for "
@@ -296,6 +374,10 @@ def main():
     ctx = _native.context(0)
     L = lib()
     V = lambda t: ctypes.c_void_p(t.data_ptr())
+
+    if args.only_pairwise:
+        print(json.dumps({"pairwise_rmsd": pairwise_bench(ctx, dev)}), flush=True)
+        return
 
     # ---- inputs resident in HBM ------------------------------------------
     F = args.frames
@@ -402,6 +484,7 @@ def main():
     L.wmd_rmsd_destroy(h_rmsd)
     soft = None if args.no_soft else soft_rdf_ratio(ctx, dev)
     batch = None if args.no_batch else uneven_batch_bench(ctx, dev)
+    pairwise = None if args.no_pairwise else pairwise_bench(ctx, dev)
     cpu = None if args.no_cpu else cpu_baseline()
 
     line = {
@@ -431,6 +514,7 @@ def main():
         "softrdf_fwd_bwd_over_plain": None if soft is None else soft["ratio"],
         "softrdf": soft,
         "uneven_batch": batch,
+        "pairwise_rmsd": pairwise,
         "roofline": {
             "kernel": "rdf_cell_kernel",
             "bound": "valu",
