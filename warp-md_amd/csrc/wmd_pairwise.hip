@@ -107,25 +107,40 @@ __global__ __launch_bounds__(kPwBlock) void pw_center_kernel(const float *__rest
     }
 }
 
-// Stages rows [row0, row0 + 96) x atoms [k0, k0 + 32) of X (centred, f64) into
-// a panel laid out [quad][row][4]: element (k, r) at ((k / 4) * S4 + r) * 4 + k % 4.
-__device__ __forceinline__ void pw_stage(double *panel, const float *__restrict__ soa, int stride,
-                                         int n, int rows, int row0, int k0,
-                                         const double *__restrict__ cen) {
+// Staging of rows [row0, row0 + 96) x atoms [k0, k0 + 32) of X (centred,
+// f64) in two halves so the global loads of the next chunk overlap the MFMAs
+// of the current one: pw_load fetches a thread's three float4 groups,
+// pw_store centres them and writes the panel, laid out [quad][row][4]
+// (element (k, r) at ((k / 4) * S4 + r) * 4 + k % 4).
+constexpr int kPwPer = (kPwRows * kPwQuads) / kPwBlock;  // float4 groups per thread and panel
+
+__device__ __forceinline__ void pw_load(float4 *v, const float *__restrict__ soa, int stride, int n,
+                                        int rows, int row0, int k0) {
 #pragma unroll
-    for (int e = 0; e < (kPwRows * kPwQuads) / kPwBlock; ++e) {
+    for (int e = 0; e < kPwPer; ++e) {
+        const int idx = e * kPwBlock + threadIdx.x;
+        const int r = idx / kPwQuads;
+        const int k = k0 + 4 * (idx - r * kPwQuads);
+        v[e] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (row0 + r < rows && k < n) v[e] = *(const float4 *)(soa + (long long)(row0 + r) * stride + k);
+    }
+}
+
+__device__ __forceinline__ void pw_store(double *panel, const float4 *v, int n, int rows, int row0,
+                                         int k0, const double *__restrict__ cen) {
+#pragma unroll
+    for (int e = 0; e < kPwPer; ++e) {
         const int idx = e * kPwBlock + threadIdx.x;
         const int r = idx / kPwQuads;
         const int q = idx - r * kPwQuads;
         const int k = k0 + 4 * q;
         double v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0;
         if (row0 + r < rows && k < n) {
-            const float4 x = *(const float4 *)(soa + (long long)(row0 + r) * stride + k);
             const double c = cen[row0 + r];
-            v0 = (double)x.x - c;
-            if (k + 1 < n) v1 = (double)x.y - c;
-            if (k + 2 < n) v2 = (double)x.z - c;
-            if (k + 3 < n) v3 = (double)x.w - c;
+            v0 = (double)v[e].x - c;
+            if (k + 1 < n) v1 = (double)v[e].y - c;
+            if (k + 2 < n) v2 = (double)v[e].z - c;
+            if (k + 3 < n) v3 = (double)v[e].w - c;
         }
         double *p = panel + (q * kPwS4 + r) * 4;
         p[0] = v0;
@@ -162,11 +177,18 @@ __global__ __launch_bounds__(kPwBlock) void pw_gram_kernel(
 #pragma unroll
         for (int j = 0; j < 3; ++j) acc[m][j] = (d4){0.0, 0.0, 0.0, 0.0};
     const int lr = lane & 15, lk = lane >> 4;
+    float4 va[kPwPer], vb[kPwPer];
+    pw_load(va, soa, stride, n, rows, ra, 0);
+    pw_load(vb, soa, stride, n, rows, rb, 0);
     for (int k0 = 0; k0 < n; k0 += kPwKc) {
         __syncthreads();  // the previous chunk is consumed
-        pw_stage(pa, soa, stride, n, rows, ra, k0, cen);
-        pw_stage(pb, soa, stride, n, rows, rb, k0, cen);
+        pw_store(pa, va, n, rows, ra, k0, cen);
+        pw_store(pb, vb, n, rows, rb, k0, cen);
         __syncthreads();
+        if (k0 + kPwKc < n) {  // next chunk in flight during this one's MFMAs
+            pw_load(va, soa, stride, n, rows, ra, k0 + kPwKc);
+            pw_load(vb, soa, stride, n, rows, rb, k0 + kPwKc);
+        }
 #pragma unroll
         for (int q = 0; q < kPwQuads; ++q) {
             double av[3], bv[3];
