@@ -63,6 +63,7 @@ typedef struct wmd_rdf wmd_rdf;
 typedef struct wmd_rmsd wmd_rmsd;
 typedef struct wmd_softrdf wmd_softrdf;
 typedef struct wmd_pairwise wmd_pairwise;
+typedef struct wmd_align wmd_align;
 
 /* ---- library / context --------------------------------------------------- */
 
@@ -243,6 +244,37 @@ wmd_status wmd_pairwise_store_stride(const wmd_pairwise *pw, size_t *stride);
 wmd_status wmd_pairwise_run_device(wmd_pairwise *pw, const float *d_store, size_t stride,
                                    size_t n_frames, float *d_out, double *d_out64, void *stream);
 void wmd_pairwise_destroy(wmd_pairwise *pw);
+
+/* ---- Align / Superpose ------------------------------------------------------- */
+
+/* AlignPlan / SuperposePlan::new + init (crates/traj-engine/src/plans/geometry/
+ * align/plan_exec.rs:3-43, 78-137, 263-315): superposition of `sel` onto a
+ * reference with kabsch_rotation_weighted (geometry/geometry_math.rs:235-297;
+ * f64 weighted centroids and cross-covariance, the optimal rotation from the
+ * FP64 QCP solve).  masses: NULL (unweighted) or [n_sel] weights (mass=True).
+ * ref_xyzw: NULL (reference="frame0": call wmd_align_set_reference with the
+ * first frame before running) or [n_ref_atoms][4] atom-indexed reference
+ * coordinates (reference="topology", positions0).  norotate: R = I. */
+wmd_status wmd_align_create(wmd_ctx *ctx, const uint32_t *sel, size_t n_sel, const float *masses,
+                            const float *ref_xyzw, size_t n_ref_atoms, int norotate,
+                            wmd_align **out);
+/* Sets the reference from atom-indexed coordinates [n_ref_atoms][4]
+ * (plan_exec.rs:126-139, frame0 mode: the first frame's selected atoms). */
+wmd_status wmd_align_set_reference(wmd_align *al, const float *ref_xyzw, size_t n_ref_atoms);
+/* AlignPlan / SuperposePlan::process_chunk on a host chunk
+ * [n_frames][n_atoms][4] (synchronous).  out_rt: NULL or [n_frames][12] f32 =
+ * R row-major then t = cy - R cx (AlignPlan's row, plan_exec.rs:228-243);
+ * out_xyz: NULL or [n_frames][n_atoms][3] f32 = R x + t for every atom in f64,
+ * cast to f32 (SuperposePlan's series, plan_exec.rs:441-456). */
+wmd_status wmd_align_run(wmd_align *al, const float *coords_xyzw, size_t n_atoms, size_t n_frames,
+                         float *out_rt, float *out_xyz);
+/* The same on device frames [n_frames][n_atoms][4] (16-byte aligned): d_rt64
+ * [F][12] f64 (required), d_rt32 [F][12] f32 and d_xyz [F][n_atoms][3] f32
+ * (each may be NULL).  Asynchronous on `stream`. */
+wmd_status wmd_align_run_device(wmd_align *al, const float *d_xyzw, size_t n_atoms,
+                                size_t n_frames, double *d_rt64, float *d_rt32, float *d_xyz,
+                                void *stream);
+void wmd_align_destroy(wmd_align *al);
 
 /* Gaussian-smoothed RDF (no counterpart in the reference; the differentiable
  * RDF of structure-matching losses such as warp-cg's objectives,

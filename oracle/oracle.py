@@ -18,6 +18,8 @@ The C file restates warp-md's CPU semantics (citations in ``wmd_oracle.c``):
   ``crates/traj-engine/src/plans/analysis/structure_factor.rs:160-258``
 * ``pairwise_rmsd``   -- ``PairwiseRmsdPlan::finalize`` (rms / nofit),
   ``rmsd/plan.rs:586-675, 1081-1159``
+* ``align``           -- ``AlignPlan`` / ``SuperposePlan`` CPU path,
+  ``plans/geometry/align/plan_exec.rs:215-243, 422-456``
 
 ``rdf_finalize_py`` below is a second, independent float64/float32 numpy
 restatement of the normalisation used to cross-check the C version.
@@ -86,6 +88,8 @@ def lib():
     L.orc_sf_finalize.restype = ctypes.c_int
     L.orc_pairwise_rmsd.argtypes = [P, S, S, S, P, S, ctypes.c_int, P, P]
     L.orc_pairwise_rmsd.restype = ctypes.c_int
+    L.orc_align.argtypes = [P, S, S, S, P, S, P, P, S, ctypes.c_int, P, P, P]
+    L.orc_align.restype = ctypes.c_int
     L.orc_omp_max_threads.argtypes = []
     L.orc_omp_max_threads.restype = ctypes.c_int
     _lib = L
@@ -256,6 +260,30 @@ def pairwise_rmsd(coords, sel, metric="rms"):
     if st != 0:
         raise RuntimeError(f"oracle pairwise rmsd failed {st}")
     return out, out64
+
+
+def align(coords, sel, ref_all, weights=None, norotate=False, superpose=False):
+    """AlignPlan rows (F, 12) f32, their f64 values, and (superpose) the
+    (F, n_atoms, 3) f32 transformed coordinates (else None)."""
+    L = lib()
+    c = _coords(coords)
+    F, N, stride = c.shape
+    ref = np.ascontiguousarray(ref_all, dtype=np.float32)
+    if ref.shape[-1] != stride:
+        pad = np.ones(ref.shape[:-1] + (stride,), np.float32)
+        pad[..., :3] = ref[..., :3]
+        ref = pad
+    s = np.ascontiguousarray(sel, dtype=np.uint32)
+    w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float32)
+    rt = np.zeros((F, 12), np.float32)
+    rt64 = np.zeros((F, 12), np.float64)
+    xyz = np.zeros((F, N, 3), np.float32) if superpose else None
+    st = L.orc_align(_ptr(c), stride, N, F, _ptr(s), s.size, _ptr(w) if w is not None else None,
+                     _ptr(ref), ref.shape[0], int(bool(norotate)), _ptr(rt), _ptr(rt64),
+                     _ptr(xyz) if xyz is not None else None)
+    if st != 0:
+        raise RuntimeError(f"oracle align failed {st}")
+    return rt, rt64, xyz
 
 
 def fit_rmsd(coords, fit_sel, rmsd_sel, ref_all):

@@ -29,6 +29,9 @@
  *   orc_pairwise_rmsd    pairwise_rmsd_plan_basic (geometry_matrix_tests.rs:1438-1460),
  *                        equal to orc_rmsd_plan per pair where the two coincide, numpy.linalg.svd
  *                        Kabsch, rigid-motion zero, symmetry, triangle inequality (nofit).
+ *   orc_align            superpose_plan_basic (analysis_smoke_tests.rs:171-195),
+ *                        test_superpose.py:41-67 (rotation + translation recovered),
+ *                        numpy.linalg.svd weighted Kabsch restated independently.
  *   orc_sf_counts / orc_sf_finalize
  *                        structure_factor_shapes (correlation_structure_tests.rs:759-780),
  *                        an independent numpy float64 restatement, and the ideal-gas limit.
@@ -827,6 +830,92 @@ int orc_pairwise_rmsd(const float *coords, size_t stride, size_t n_atoms, size_t
             if (out64) out64[i * n_frames + j] = out64[j * n_frames + i] = v;
         }
     free(fs);
+    return ORC_OK;
+}
+
+/*
+ * AlignPlan / SuperposePlan CPU path (geometry/align/plan_exec.rs:215-243 and
+ * 422-456): per frame kabsch_rotation_weighted (geometry_math.rs:235-297;
+ * centroid_weighted :202-233, H = sum (x - cx)(y - cy)^T [* w], SVD Kabsch
+ * with the reflection fix), or R = I and the two centroids when norotate;
+ * t = cy - R cx.  coords [n_frames][n_atoms][stride]; ref [n_ref][stride]
+ * atom-indexed; weights NULL or [n_sel].  out_rt [F][12] f32 (R row-major,
+ * t), out_rt64 NULL or the same in f64, out_xyz NULL or [F][n_atoms][3] f32 =
+ * R x + t in f64 for every atom (the SuperposePlan series).
+ */
+int orc_align(const float *coords, size_t stride, size_t n_atoms, size_t n_frames,
+              const uint32_t *sel, size_t n_sel, const float *weights, const float *ref,
+              size_t n_ref, int norotate, float *out_rt, double *out_rt64, float *out_xyz) {
+    for (size_t i = 0; i < n_sel; ++i)
+        if (sel[i] >= n_atoms || sel[i] >= n_ref) return ORC_ERR_INVALID;
+    const size_t n = n_sel;
+    for (size_t f = 0; f < n_frames; ++f) {
+        const float *frame = coords + f * n_atoms * stride;
+        double cx[3] = {0, 0, 0}, cy[3] = {0, 0, 0}, r[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+        if (n > 0) {
+            double sx[3] = {0, 0, 0}, sy[3] = {0, 0, 0}, wsum = 0.0;
+            for (size_t i = 0; i < n; ++i) {
+                const float *p = frame + (size_t)sel[i] * stride;
+                const float *q = ref + (size_t)sel[i] * stride;
+                if (weights) {
+                    double wi = (double)weights[i];
+                    for (int d = 0; d < 3; ++d) {
+                        sx[d] += (double)p[d] * wi;
+                        sy[d] += (double)q[d] * wi;
+                    }
+                    wsum += wi;
+                } else {
+                    for (int d = 0; d < 3; ++d) {
+                        sx[d] += (double)p[d];
+                        sy[d] += (double)q[d];
+                    }
+                }
+            }
+            if (!weights) wsum = (double)n;
+            if (wsum != 0.0)
+                for (int d = 0; d < 3; ++d) {
+                    cx[d] = sx[d] / wsum;
+                    cy[d] = sy[d] / wsum;
+                }
+            if (!norotate) {
+                double h[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+                for (size_t i = 0; i < n; ++i) {
+                    const float *p = frame + (size_t)sel[i] * stride;
+                    const float *q = ref + (size_t)sel[i] * stride;
+                    double xr[3], yr[3];
+                    for (int d = 0; d < 3; ++d) {
+                        xr[d] = (double)p[d] - cx[d];
+                        yr[d] = (double)q[d] - cy[d];
+                    }
+                    double wi = weights ? (double)weights[i] : 1.0;
+                    for (int a = 0; a < 3; ++a)
+                        for (int b = 0; b < 3; ++b)
+                            h[a][b] += weights ? (xr[a] * yr[b]) * wi : xr[a] * yr[b];
+                }
+                double rr[3][3];
+                if (kabsch_from_h(h, rr)) memcpy(r, rr, sizeof(r));
+            }
+        }
+        double t[3];
+        for (int a = 0; a < 3; ++a) t[a] = cy[a] - (r[a][0] * cx[0] + r[a][1] * cx[1] + r[a][2] * cx[2]);
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) {
+                out_rt[f * 12 + 3 * a + b] = (float)r[a][b];
+                if (out_rt64) out_rt64[f * 12 + 3 * a + b] = r[a][b];
+            }
+        for (int a = 0; a < 3; ++a) {
+            out_rt[f * 12 + 9 + a] = (float)t[a];
+            if (out_rt64) out_rt64[f * 12 + 9 + a] = t[a];
+        }
+        if (out_xyz)
+            for (size_t k = 0; k < n_atoms; ++k) {
+                const float *p = frame + k * stride;
+                double x = (double)p[0], y = (double)p[1], z = (double)p[2];
+                for (int a = 0; a < 3; ++a)
+                    out_xyz[(f * n_atoms + k) * 3 + a] =
+                        (float)(r[a][0] * x + r[a][1] * y + r[a][2] * z + t[a]);
+            }
+    }
     return ORC_OK;
 }
 

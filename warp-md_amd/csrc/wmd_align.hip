@@ -1,0 +1,313 @@
+// AlignPlan and SuperposePlan on the device
+// (crates/traj-engine/src/plans/geometry/align/plan_exec.rs:76-465): per frame,
+// the optimal rotation R and translation t = cy - R cx that superpose the
+// selection onto the reference (kabsch_rotation_weighted,
+// geometry/geometry_math.rs:235-297, optionally mass-weighted), and for
+// SuperposePlan R x + t applied to every atom of the frame.
+//
+// The reference's CPU path is all f64 (weighted centroids, H, SVD) and its
+// GPU path f32 covariance sums; this engine follows the CPU path: one
+// workgroup per frame reduces the weighted centroid and then the weighted
+// cross-covariance in f64, and the FP64 QCP solve (wmd_qcp.hpp) supplies R.
+// The transform of SuperposePlan is evaluated in f64 in the reference's
+// operation order (plan_exec.rs:399-405) and cast to f32.
+#include <algorithm>
+#include <cstring>
+#include <memory>
+
+#include "wmd_internal.hpp"
+#include "wmd_qcp.hpp"
+
+namespace {
+using namespace wmd;
+
+constexpr int kAlBlock = 256;
+
+// Sum of `cnt` f64 values over the workgroup (256 threads); every thread gets the totals.
+template <int CNT>
+__device__ __forceinline__ void al_block_sum(double *v, double *red) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int c = 0; c < CNT; ++c)
+        for (int off = 32; off > 0; off >>= 1) v[c] += __shfl_down(v[c], off, 64);
+    __syncthreads();
+    if (lane == 0)
+#pragma unroll
+        for (int c = 0; c < CNT; ++c) red[w * CNT + c] = v[c];
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < CNT; ++c) v[c] = red[c] + red[CNT + c] + red[2 * CNT + c] + red[3 * CNT + c];
+}
+
+// One workgroup per frame (grid-stride): weighted centroid cx = sum w x / sum w
+// (centroid_weighted, geometry_math.rs:202-233), H = sum w (x - cx)(y - cy)^T,
+// gx = sum w |x - cx|^2, then R (QCP, or I for norotate) and t = cy - R cx.
+// aos: [F][n_src] xyzw; sel [n]; wts [n] or NULL; ref_c [n][3] = y - cy
+// (host f64, centred), ref_g = sum w |y - cy|^2, cy [3].
+// rt64 [F][12] (R row-major, t), rt32 [F][12] or NULL.
+__global__ __launch_bounds__(kAlBlock) void al_fit_kernel(
+    const float4 *__restrict__ aos, long long n_src, int n_frames, const unsigned *__restrict__ sel,
+    const float *__restrict__ wts, int n, const double *__restrict__ ref_c, double ref_g,
+    double cy0, double cy1, double cy2, int norotate, double *__restrict__ rt64,
+    float *__restrict__ rt32) {
+    __shared__ double red[4 * 10];
+    for (int f = blockIdx.x; f < n_frames; f += gridDim.x) {
+        const float4 *x = aos + (long long)f * n_src;
+        double s[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int k = threadIdx.x; k < n; k += kAlBlock) {
+            const float4 p = x[sel[k]];
+            const double w = wts ? (double)wts[k] : 1.0;
+            if (wts) {
+                s[0] += (double)p.x * w;
+                s[1] += (double)p.y * w;
+                s[2] += (double)p.z * w;
+                s[3] += w;
+            } else {
+                s[0] += (double)p.x;
+                s[1] += (double)p.y;
+                s[2] += (double)p.z;
+            }
+        }
+        al_block_sum<4>(s, red);
+        const double wsum = wts ? s[3] : (double)n;
+        double cx[3] = {0.0, 0.0, 0.0};
+        if (wsum != 0.0)
+            for (int d = 0; d < 3; ++d) cx[d] = s[d] / wsum;
+        double m[10];
+#pragma unroll
+        for (int c = 0; c < 10; ++c) m[c] = 0.0;
+        if (!norotate) {
+            for (int k = threadIdx.x; k < n; k += kAlBlock) {
+                const float4 p = x[sel[k]];
+                const double w = wts ? (double)wts[k] : 1.0;
+                const double xr[3] = {(double)p.x - cx[0], (double)p.y - cx[1], (double)p.z - cx[2]};
+                const double yr[3] = {ref_c[3 * k], ref_c[3 * k + 1], ref_c[3 * k + 2]};
+#pragma unroll
+                for (int a = 0; a < 3; ++a)
+#pragma unroll
+                    for (int b = 0; b < 3; ++b) m[3 * a + b] += xr[a] * yr[b] * w;
+                m[9] += (xr[0] * xr[0] + xr[1] * xr[1] + xr[2] * xr[2]) * w;
+            }
+            al_block_sum<10>(m, red);
+        }
+        if (threadIdx.x == 0) {
+            double R[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+            if (!norotate && n > 0) {
+                QcpResult res;
+                qcp_solve(m, m[9], ref_g, true, &res);
+                for (int i = 0; i < 9; ++i) R[i] = res.rot[i];
+            }
+            const double cy[3] = {cy0, cy1, cy2};
+            double out[12];
+            for (int i = 0; i < 9; ++i) out[i] = R[i];
+            for (int a = 0; a < 3; ++a)
+                out[9 + a] = cy[a] - (R[3 * a] * cx[0] + R[3 * a + 1] * cx[1] + R[3 * a + 2] * cx[2]);
+            for (int i = 0; i < 12; ++i) {
+                rt64[(long long)f * 12 + i] = out[i];
+                if (rt32) rt32[(long long)f * 12 + i] = (float)out[i];
+            }
+        }
+        __syncthreads();  // red is reused by the next frame
+    }
+}
+
+// out[f][a] = R x + t for every atom, f64 in the reference's order, cast to f32.
+__global__ __launch_bounds__(kAlBlock) void al_apply_kernel(const float4 *__restrict__ aos,
+                                                            long long n_atoms, int n_frames,
+                                                            const double *__restrict__ rt64,
+                                                            float *__restrict__ out) {
+    const long long total = n_atoms * n_frames;
+    for (long long t = (long long)blockIdx.x * kAlBlock + threadIdx.x; t < total;
+         t += (long long)gridDim.x * kAlBlock) {
+        const int f = (int)(t / n_atoms);
+        const double *q = rt64 + (long long)f * 12;
+        const float4 p = aos[t];
+        const double x = (double)p.x, y = (double)p.y, z = (double)p.z;
+        const double rx = q[0] * x + q[1] * y + q[2] * z + q[9];
+        const double ry = q[3] * x + q[4] * y + q[5] * z + q[10];
+        const double rz = q[6] * x + q[7] * y + q[8] * z + q[11];
+        out[3 * t] = (float)rx;
+        out[3 * t + 1] = (float)ry;
+        out[3 * t + 2] = (float)rz;
+    }
+}
+
+int al_grid(long long total) {
+    return (int)std::max<long long>(1, std::min<long long>((total + kAlBlock - 1) / kAlBlock, 65536));
+}
+
+}  // namespace
+
+struct wmd_align {
+    wmd_ctx *ctx = nullptr;
+    std::vector<uint32_t> sel;
+    std::vector<float> wts;  // empty: unweighted
+    bool norotate = false;
+    bool have_ref = false;
+    double cy[3] = {0, 0, 0};
+    double ref_g = 0.0;
+    DevBuf d_sel, d_wts, d_refc, d_aos, d_rt64, d_rt32, d_xyz;
+    HostBuf h_stage;
+};
+
+namespace {
+
+// kabsch_rotation_weighted's reference half (geometry_math.rs:248-249): the
+// weighted centroid cy and the centred reference y - cy, in f64 on the host.
+void al_set_reference(wmd_align *h, const float *ref_xyzw, size_t ref_stride_atoms,
+                      const uint32_t *idx) {
+    const size_t n = h->sel.size();
+    std::vector<double> yc(std::max<size_t>(n, 1) * 3, 0.0);
+    double s[3] = {0, 0, 0}, wsum = 0.0;
+    const bool weighted = !h->wts.empty();
+    for (size_t i = 0; i < n; ++i) {
+        const float *p = ref_xyzw + (size_t)(idx ? idx[i] : i) * 4;
+        if (weighted) {
+            const double w = (double)h->wts[i];
+            for (int d = 0; d < 3; ++d) s[d] += (double)p[d] * w;
+            wsum += w;
+        } else {
+            for (int d = 0; d < 3; ++d) s[d] += (double)p[d];
+        }
+    }
+    if (!weighted) wsum = (double)n;
+    for (int d = 0; d < 3; ++d) h->cy[d] = (n == 0 || wsum == 0.0) ? 0.0 : s[d] / wsum;
+    double g = 0.0;
+    for (size_t i = 0; i < n; ++i) {
+        const float *p = ref_xyzw + (size_t)(idx ? idx[i] : i) * 4;
+        const double w = weighted ? (double)h->wts[i] : 1.0;
+        double e = 0.0;
+        for (int d = 0; d < 3; ++d) {
+            yc[3 * i + d] = (double)p[d] - h->cy[d];
+            e += yc[3 * i + d] * yc[3 * i + d];
+        }
+        g += e * w;
+    }
+    (void)ref_stride_atoms;
+    h->ref_g = g;
+    h->d_refc.reserve(yc.size() * 8);
+    WMD_HIP_CHECK(hipMemcpy(h->d_refc.ptr, yc.data(), yc.size() * 8, hipMemcpyHostToDevice));
+    h->have_ref = true;
+}
+
+void al_enqueue(wmd_align *h, const float4 *d_aos, size_t n_atoms, size_t F, double *d_rt64,
+                float *d_rt32, float *d_xyz, hipStream_t st) {
+    for (uint32_t a : h->sel)
+        WMD_REQUIRE(a < n_atoms, WMD_ERR_MISMATCH,
+                    "selection index " + std::to_string(a) + " out of range for " +
+                        std::to_string(n_atoms) + " atoms");
+    WMD_REQUIRE(h->have_ref, WMD_ERR_MISMATCH, "reference not initialized");
+    if (F == 0) return;
+    {
+        ProfScope prof(h->ctx, st, "align_fit");
+        hipLaunchKernelGGL(al_fit_kernel, dim3((unsigned)std::min<size_t>(F, 65536)), dim3(kAlBlock), 0,
+                           st, d_aos, (long long)n_atoms, (int)F, h->d_sel.as<unsigned>(),
+                           h->wts.empty() ? nullptr : h->d_wts.as<float>(), (int)h->sel.size(),
+                           h->d_refc.as<double>(), h->ref_g, h->cy[0], h->cy[1], h->cy[2],
+                           (int)h->norotate, d_rt64, d_rt32);
+        WMD_HIP_CHECK(hipGetLastError());
+    }
+    if (d_xyz) {
+        ProfScope prof(h->ctx, st, "align_apply");
+        hipLaunchKernelGGL(al_apply_kernel, dim3(al_grid((long long)(n_atoms * F))), dim3(kAlBlock), 0,
+                           st, d_aos, (long long)n_atoms, (int)F, (const double *)d_rt64, d_xyz);
+        WMD_HIP_CHECK(hipGetLastError());
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+wmd_status wmd_align_create(wmd_ctx *ctx, const uint32_t *sel, size_t n_sel, const float *masses,
+                            const float *ref_xyzw, size_t n_ref_atoms, int norotate,
+                            wmd_align **out) {
+    WMD_API_BEGIN
+    WMD_REQUIRE(ctx && out, WMD_ERR_INVALID, "null context or output handle");
+    *out = nullptr;
+    WMD_REQUIRE(n_sel == 0 || sel, WMD_ERR_INVALID, "null selection");
+    WMD_REQUIRE(n_sel < (1u << 30), WMD_ERR_UNSUPPORTED, "selection too large");
+    bind(ctx);
+    auto h = std::make_unique<wmd_align>();
+    h->ctx = ctx;
+    h->norotate = norotate != 0;
+    h->sel.assign(sel, sel + n_sel);
+    if (masses) h->wts.assign(masses, masses + n_sel);
+    const size_t nb = std::max<size_t>(n_sel, 1) * 4;
+    h->d_sel.reserve(nb);
+    if (n_sel) WMD_HIP_CHECK(hipMemcpy(h->d_sel.ptr, sel, n_sel * 4, hipMemcpyHostToDevice));
+    if (masses) {
+        h->d_wts.reserve(nb);
+        if (n_sel) WMD_HIP_CHECK(hipMemcpy(h->d_wts.ptr, masses, n_sel * 4, hipMemcpyHostToDevice));
+    }
+    if (ref_xyzw) {
+        for (uint32_t a : h->sel)
+            WMD_REQUIRE(a < n_ref_atoms, WMD_ERR_MISMATCH, "selection index out of range for the reference");
+        al_set_reference(h.get(), ref_xyzw, n_ref_atoms, sel);
+    }
+    *out = h.release();
+    WMD_API_END
+}
+
+wmd_status wmd_align_set_reference(wmd_align *h, const float *ref_xyzw, size_t n_ref_atoms) {
+    WMD_API_BEGIN
+    WMD_REQUIRE(h && ref_xyzw, WMD_ERR_INVALID, "null handle or reference");
+    for (uint32_t a : h->sel)
+        WMD_REQUIRE(a < n_ref_atoms, WMD_ERR_MISMATCH, "selection index out of range for the reference");
+    bind(h->ctx);
+    al_set_reference(h, ref_xyzw, n_ref_atoms, h->sel.data());
+    WMD_API_END
+}
+
+wmd_status wmd_align_run(wmd_align *h, const float *coords_xyzw, size_t n_atoms, size_t n_frames,
+                         float *out_rt, float *out_xyz) {
+    WMD_API_BEGIN
+    WMD_REQUIRE(h != nullptr, WMD_ERR_INVALID, "null handle");
+    if (n_frames == 0) return WMD_OK;
+    WMD_REQUIRE(coords_xyzw != nullptr, WMD_ERR_INVALID, "null coords");
+    bind(h->ctx);
+    hipStream_t st = h->ctx->stream;
+    const size_t fbytes = n_atoms * 16;
+    const size_t batch = std::max<size_t>(1, std::min<size_t>(n_frames, (256ull << 20) / std::max<size_t>(fbytes, 1)));
+    float4 *d_aos = (float4 *)h->d_aos.reserve(batch * fbytes);
+    double *d_rt64 = (double *)h->d_rt64.reserve(batch * 12 * 8);
+    float *d_rt32 = (float *)h->d_rt32.reserve(batch * 12 * 4);
+    float *d_xyz = out_xyz ? (float *)h->d_xyz.reserve(batch * n_atoms * 12) : nullptr;
+    void *stage = h->h_stage.reserve(batch * fbytes);
+    for (size_t f0 = 0; f0 < n_frames; f0 += batch) {
+        const size_t nf = std::min(batch, n_frames - f0);
+        WMD_HIP_CHECK(hipStreamSynchronize(st));
+        std::memcpy(stage, coords_xyzw + f0 * n_atoms * 4, nf * fbytes);
+        WMD_HIP_CHECK(hipMemcpyAsync(d_aos, stage, nf * fbytes, hipMemcpyHostToDevice, st));
+        al_enqueue(h, d_aos, n_atoms, nf, d_rt64, d_rt32, d_xyz, st);
+        if (out_rt)
+            WMD_HIP_CHECK(hipMemcpyAsync(out_rt + f0 * 12, d_rt32, nf * 12 * 4, hipMemcpyDeviceToHost, st));
+        if (out_xyz)
+            WMD_HIP_CHECK(hipMemcpyAsync(out_xyz + f0 * n_atoms * 3, d_xyz, nf * n_atoms * 12,
+                                         hipMemcpyDeviceToHost, st));
+    }
+    WMD_HIP_CHECK(hipStreamSynchronize(st));
+    WMD_API_END
+}
+
+wmd_status wmd_align_run_device(wmd_align *h, const float *d_xyzw, size_t n_atoms, size_t n_frames,
+                                double *d_rt64, float *d_rt32, float *d_xyz, void *stream) {
+    WMD_API_BEGIN
+    WMD_REQUIRE(h != nullptr, WMD_ERR_INVALID, "null handle");
+    WMD_REQUIRE(n_frames == 0 || (d_xyzw && d_rt64), WMD_ERR_INVALID, "null device buffer");
+    WMD_REQUIRE(((uintptr_t)d_xyzw & 15) == 0, WMD_ERR_INVALID, "device frames must be 16-byte aligned");
+    bind(h->ctx);
+    al_enqueue(h, (const float4 *)d_xyzw, n_atoms, n_frames, d_rt64, d_rt32, d_xyz,
+               pick_stream(h->ctx, stream));
+    WMD_API_END
+}
+
+void wmd_align_destroy(wmd_align *h) {
+    if (!h) return;
+    int cur = 0;
+    if (hipGetDevice(&cur) == hipSuccess && h->ctx) (void)hipSetDevice(h->ctx->device);
+    delete h;
+    (void)hipSetDevice(cur);
+}
+
+}  // extern "C"

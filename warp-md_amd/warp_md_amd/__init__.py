@@ -12,13 +12,13 @@ from ._native import WmdError, device_count, library_built
 from .analysis import rdf
 from .executor import Device, normalize_frame_indices
 from .batch import UnevenBatch
-from .plans import (PairDistPlan, PairwiseRmsdPlan, RdfPlan, RmsdPlan, StructureFactorPlan,
-                    finalize_rdf)
+from .plans import (AlignPlan, PairDistPlan, PairwiseRmsdPlan, RdfPlan, RmsdPlan,
+                    StructureFactorPlan, SuperposePlan, finalize_rdf)
 from .system import Selection, System, Trajectory
 
 __all__ = [
-    "Device", "PairDistPlan", "PairwiseRmsdPlan", "RdfPlan", "RmsdPlan", "StructureFactorPlan",
-    "Selection", "System", "Trajectory", "UnevenBatch", "WmdError",
+    "AlignPlan", "Device", "PairDistPlan", "PairwiseRmsdPlan", "RdfPlan", "RmsdPlan", "StructureFactorPlan",
+    "SuperposePlan", "Selection", "System", "Trajectory", "UnevenBatch", "WmdError",
     "device_count", "finalize_rdf", "library_built", "normalize_frame_indices", "rdf",
 ]
 __version__ = "0.1.0"

@@ -63,6 +63,8 @@ SYMBOLS = (
     "wmd_batch_plan_info", "wmd_batch_destroy", "wmd_autotune_export", "wmd_autotune_import",
     "wmd_pairwise_create", "wmd_pairwise_append", "wmd_pairwise_frames", "wmd_pairwise_finalize",
     "wmd_pairwise_store_stride", "wmd_pairwise_run_device", "wmd_pairwise_destroy",
+    "wmd_align_create", "wmd_align_set_reference", "wmd_align_run", "wmd_align_run_device",
+    "wmd_align_destroy",
 )
 
 
@@ -172,6 +174,11 @@ def lib():
             "wmd_pairwise_store_stride": ([P, ctypes.POINTER(S)], St),
             "wmd_pairwise_run_device": ([P, P, S, S, P, P, P], St),
             "wmd_pairwise_destroy": ([P], None),
+            "wmd_align_create": ([P, P, S, P, P, S, I, PP], St),
+            "wmd_align_set_reference": ([P, P, S], St),
+            "wmd_align_run": ([P, P, S, S, P, P], St),
+            "wmd_align_run_device": ([P, P, S, S, P, P, P, P], St),
+            "wmd_align_destroy": ([P], None),
         }
         for name, (args, res) in sig.items():
             fn = getattr(L, name)

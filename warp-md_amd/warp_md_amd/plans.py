@@ -1,5 +1,5 @@
 """RdfPlan, RmsdPlan and their neighbours (PairDistPlan, StructureFactorPlan,
-PairwiseRmsdPlan) with warp-md's Python signatures, running on the gfx950
+PairwiseRmsdPlan, AlignPlan, SuperposePlan) with warp-md's Python signatures, running on the gfx950
 engine through its C ABI.
 
 RdfPlan mirrors PyRdfPlan (crates/traj-py/src/analysis.rs:2298-2399) over
@@ -517,6 +517,98 @@ class PairwiseRmsdPlan:
         finally:
             lib().wmd_pairwise_destroy(h)
         return out
+
+
+class _SuperposeBase:
+    """Shared state of AlignPlan and SuperposePlan (align/plan_exec.rs:3-43)."""
+
+    def __init__(self, selection, reference: str = "topology", mass: bool = False,
+                 norotate: bool = False):
+        self.selection = _indices(selection)
+        self.reference_mode = _parse_reference(reference)
+        self.mass = bool(mass)
+        self.norotate = bool(norotate)
+
+    def _run(self, traj: Trajectory, system: System, chunk_frames, device, frame_indices,
+             superpose: bool):
+        dev = Device.from_spec(device)
+        if traj.n_atoms() != system.n_atoms():
+            raise RuntimeError("mismatch: trajectory atom count does not match system")
+        ctx = dev.context()
+        n_atoms = system.n_atoms()
+        sel = self.selection
+        if sel.size and int(sel.max()) >= n_atoms:
+            raise RuntimeError("invalid selection: atom index out of range")
+        masses = None
+        if self.mass:
+            masses = np.ascontiguousarray(np.asarray(system.atom_table()["mass"], np.float32)[sel])
+        ref4 = None
+        if self.reference_mode == "topology":
+            if system.positions0 is None:
+                raise RuntimeError("mismatch: no topology reference coords")  # plan_exec.rs:84-87
+            ref4 = xyzw(np.ascontiguousarray(system.positions0, np.float32)[None])[0]
+        h = ctypes.c_void_p()
+        check(lib().wmd_align_create(ctx.handle, ptr(sel), sel.size, ptr(masses), ptr(ref4),
+                                     n_atoms if ref4 is not None else 0, int(self.norotate),
+                                     ctypes.byref(h)))
+        rts, xyzs = [], []
+        try:
+            tr = select_frames(traj, frame_indices)
+            tr.reset()
+            step = resolve_chunk_frames(chunk_frames, max(1, n_atoms))
+            first = True
+            while True:
+                chunk = tr.read_chunk(step)
+                if chunk is None:
+                    break
+                coords, _ = chunk
+                nf = coords.shape[0]
+                x4 = xyzw(coords)
+                if first and ref4 is None:
+                    # frame0: the first frame read is the reference (plan_exec.rs:126-139)
+                    f0 = np.ascontiguousarray(x4[0])
+                    check(lib().wmd_align_set_reference(h, ptr(f0), n_atoms))
+                first = False
+                rt = np.zeros((nf, 12), np.float32)
+                xyz = np.zeros((nf, n_atoms, 3), np.float32) if superpose else None
+                check(lib().wmd_align_run(h, ptr(x4), n_atoms, nf, ptr(rt), ptr(xyz)))
+                rts.append(rt)
+                if superpose:
+                    xyzs.append(xyz)
+        finally:
+            lib().wmd_align_destroy(h)
+        if superpose:
+            return (np.concatenate(xyzs).reshape(-1) if xyzs else np.zeros(0, np.float32))
+        return np.concatenate(rts) if rts else np.zeros((0, 12), np.float32)
+
+
+class AlignPlan(_SuperposeBase):
+    """Per-frame superposition transform of a selection onto a reference.
+
+    Mirrors PyAlignPlan (crates/traj-py/src/transform.rs:1798-1836) over
+    AlignPlan (crates/traj-engine/src/plans/geometry/align/plan_exec.rs:76-255):
+    returns (F, 12) float32 rows R (row-major) then t = cy - R cx, with R the
+    optimal rotation of kabsch_rotation_weighted (mass=True weights by mass;
+    norotate keeps R = I) from the engine's f64 fit (``wmd_align_*``).
+    """
+
+    def run(self, traj: Trajectory, system: System, chunk_frames: Optional[int] = None,
+            device: str = "auto"):
+        return self._run(traj, system, chunk_frames, device, None, superpose=False)
+
+
+class SuperposePlan(_SuperposeBase):
+    """Superposed coordinates of every atom, R x + t per frame.
+
+    Mirrors PySuperposePlan (crates/traj-py/src/correlation.rs:265-305) over
+    SuperposePlan (plan_exec.rs:257-465): returns the flat float32 series
+    [F * n_atoms * 3], the transform evaluated in f64 and cast as the
+    reference's CPU path does.
+    """
+
+    def run(self, traj: Trajectory, system: System, chunk_frames: Optional[int] = None,
+            device: str = "auto", frame_indices: Optional[Sequence[int]] = None):
+        return self._run(traj, system, chunk_frames, device, frame_indices, superpose=True)
 
 
 def pairdist_centers(bins: int, w) -> np.ndarray:
