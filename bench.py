@@ -262,6 +262,80 @@ def pairwise_bench(ctx, dev, F=5000, reps=3, cpu_frames=600):
             "workload": "PairwiseRmsdPlan metric rms, 5,000 chain frames, fit on 2,000 CA"}
 
 
+def align_bench(ctx, dev, F=5000, reps=3, cpu_frames=40):
+    """SuperposePlan over the 5,000 frames of the 20,000-atom chain, fit on the
+    2,000 CA atoms, from device frames [F][n][4]: one workgroup per frame
+    fits in f64 and writes R x + t for every atom (al_fit_kernel; HBM-bound,
+    algorithmic 16 B read + 12 B written per atom)."""
+    import numpy as np
+    import torch
+
+    from warp_md_amd import workloads
+    from warp_md_amd._native import check, lib, ptr
+
+    L = lib()
+    c = workloads.chain(n_frames=0, seed=2)
+    ref, ca = c["ref"], c["ca"]
+    n = ref.shape[0]
+    store = workloads.chain_store_device(ref, np.arange(n, dtype=np.uint32), F, seed=2, device=dev)
+    x4 = torch.ones((F, n, 4), dtype=torch.float32, device=dev)
+    x4[:, :, :3] = store.permute(0, 2, 1)
+    del store
+    ref4 = np.ones((n, 4), np.float32)
+    ref4[:, :3] = ref
+    h = ctypes.c_void_p()
+    check(L.wmd_align_create(ctx.handle, ptr(ca), ca.size, None, ptr(ref4), n, 0, ctypes.byref(h)))
+    rt64 = torch.zeros((F, 12), dtype=torch.float64, device=dev)
+    xyz = torch.empty((F, n, 3), dtype=torch.float32, device=dev)
+    stream = torch.cuda.current_stream().cuda_stream
+
+    def run():
+        check(L.wmd_align_run_device(h, ctypes.c_void_p(x4.data_ptr()), n, F,
+                                     ctypes.c_void_p(rt64.data_ptr()), None,
+                                     ctypes.c_void_p(xyz.data_ptr()), ctypes.c_void_p(stream)))
+
+    run()
+    torch.cuda.synchronize()
+    ctx.profile(True)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        run()
+    torch.cuda.synchronize()
+    wall_ms = (time.perf_counter() - t0) / reps * 1e3
+    app_ms, app_n = ctx.profile_read("align_superpose")
+    ctx.profile(False)
+    L.wmd_align_destroy(h)
+    app_avg = app_ms / max(app_n, 1)
+    algo = 28.0 * n * F
+    ach = algo / (app_avg / 1e3) / 1e9
+    sample = x4[:cpu_frames, :, :3].cpu().numpy()
+    del x4, xyz
+    torch.cuda.empty_cache()
+    cpu = None
+    try:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        t0 = time.perf_counter()
+        oracle.align(sample, ca, ref, superpose=True)
+        dt = time.perf_counter() - t0
+        cpu = {"value": cpu_frames / dt, "unit": "frames/s", "cores": 1, "kind": "port",
+               "sample": f"first {cpu_frames} frames ({dt:.2f} s, 1 thread, SVD Kabsch + f64 transform)"}
+    except Exception as e:
+        log(f"align cpu baseline skipped: {e}")
+    log(f"superpose: {F} frames x {n} atoms (fit {ca.size}): {wall_ms:.2f} ms wall, fused "
+        f"fit + transform kernel {app_avg:.3f} ms ({ach:.0f} GB/s)")
+    return {"frames": F, "atoms": n, "fit_atoms": int(ca.size), "ms": wall_ms,
+            "frames_per_s": F / (wall_ms / 1e3),
+            "kernel_ms": {"align_superpose": app_avg},
+            "roofline": {"kernel": "al_fit_kernel (fit + transform)", "bound": "hbm", "achieved": ach,
+                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+                         "traffic": None,
+                         "algorithmic": f"28 B per atom (16 read, 12 written) x {n * F:.4g} atoms",
+                         "avg_launch_ms": app_avg},
+            "cpu_baseline": cpu,
+            "workload": "SuperposePlan, 5,000 chain frames x 20,000 atoms, fit on 2,000 CA"}
+
+
 def uneven_batch_bench(ctx, dev, reps=3, autotune=True):
     """256 trajectories, atom counts log-uniform in [500, 50000], cubic boxes at
     0.1 atoms/A^3, 50 uniform-random frames each; RDF (300 bins, r_max 12 A)
@@ -357,6 +431,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the uneven batch")
     ap.add_argument("--no-pairwise", action="store_true", help="skip the pairwise RMSD matrix")
     ap.add_argument("--only-pairwise", action="store_true", help="time only the pairwise RMSD matrix")
+    ap.add_argument("--no-align", action="store_true", help="skip the superposition leg")
+    ap.add_argument("--only-f2", action="store_true", help="time only the pairwise and superposition legs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for "
@@ -375,8 +451,11 @@ def main():
     L = lib()
     V = lambda t: ctypes.c_void_p(t.data_ptr())
 
-    if args.only_pairwise:
-        print(json.dumps({"pairwise_rmsd": pairwise_bench(ctx, dev)}), flush=True)
+    if args.only_pairwise or args.only_f2:
+        out = {"pairwise_rmsd": pairwise_bench(ctx, dev)}
+        if args.only_f2:
+            out["superpose"] = align_bench(ctx, dev)
+        print(json.dumps(out), flush=True)
         return
 
     # ---- inputs resident in HBM ------------------------------------------
@@ -485,6 +564,7 @@ def main():
     soft = None if args.no_soft else soft_rdf_ratio(ctx, dev)
     batch = None if args.no_batch else uneven_batch_bench(ctx, dev)
     pairwise = None if args.no_pairwise else pairwise_bench(ctx, dev)
+    superpose = None if args.no_align else align_bench(ctx, dev)
     cpu = None if args.no_cpu else cpu_baseline()
 
     line = {
@@ -515,6 +595,7 @@ def main():
         "softrdf": soft,
         "uneven_batch": batch,
         "pairwise_rmsd": pairwise,
+        "superpose": superpose,
         "roofline": {
             "kernel": "rdf_cell_kernel",
             "bound": "valu",

@@ -10,7 +10,8 @@
 // workgroup per frame reduces the weighted centroid and then the weighted
 // cross-covariance in f64, and the FP64 QCP solve (wmd_qcp.hpp) supplies R.
 // The transform of SuperposePlan is evaluated in f64 in the reference's
-// operation order (plan_exec.rs:399-405) and cast to f32.
+// operation order (plan_exec.rs:399-405) and cast to f32, by the same
+// workgroup right after the fit, while the frame's lines are still cached.
 #include <algorithm>
 #include <cstring>
 #include <memory>
@@ -44,13 +45,15 @@ __device__ __forceinline__ void al_block_sum(double *v, double *red) {
 // gx = sum w |x - cx|^2, then R (QCP, or I for norotate) and t = cy - R cx.
 // aos: [F][n_src] xyzw; sel [n]; wts [n] or NULL; ref_c [n][3] = y - cy
 // (host f64, centred), ref_g = sum w |y - cy|^2, cy [3].
-// rt64 [F][12] (R row-major, t), rt32 [F][12] or NULL.
+// rt64 [F][12] (R row-major, t), rt32 [F][12] or NULL; xyz NULL or
+// [F][n_src][3] = R x + t for every atom (f64, reference order, cast to f32).
 __global__ __launch_bounds__(kAlBlock) void al_fit_kernel(
     const float4 *__restrict__ aos, long long n_src, int n_frames, const unsigned *__restrict__ sel,
     const float *__restrict__ wts, int n, const double *__restrict__ ref_c, double ref_g,
     double cy0, double cy1, double cy2, int norotate, double *__restrict__ rt64,
-    float *__restrict__ rt32) {
+    float *__restrict__ rt32, float *__restrict__ xyz) {
     __shared__ double red[4 * 10];
+    __shared__ double rt_s[12];
     for (int f = blockIdx.x; f < n_frames; f += gridDim.x) {
         const float4 *x = aos + (long long)f * n_src;
         double s[4] = {0.0, 0.0, 0.0, 0.0};
@@ -105,35 +108,37 @@ __global__ __launch_bounds__(kAlBlock) void al_fit_kernel(
             for (int i = 0; i < 12; ++i) {
                 rt64[(long long)f * 12 + i] = out[i];
                 if (rt32) rt32[(long long)f * 12 + i] = (float)out[i];
+                rt_s[i] = out[i];
             }
         }
-        __syncthreads();  // red is reused by the next frame
+        __syncthreads();
+        if (xyz) {
+            // SuperposePlan: every atom of this frame while its lines are still cached
+            double q[12];
+#pragma unroll
+            for (int i = 0; i < 12; ++i) q[i] = rt_s[i];
+            float *o = xyz + (long long)f * n_src * 3;
+            constexpr int U = 4;  // loads in flight per thread
+            for (long long a0 = threadIdx.x; a0 < n_src; a0 += U * kAlBlock) {
+                float4 p[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const long long a = a0 + (long long)u * kAlBlock;
+                    p[u] = a < n_src ? x[a] : make_float4(0.f, 0.f, 0.f, 0.f);
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const long long a = a0 + (long long)u * kAlBlock;
+                    if (a >= n_src) break;
+                    const double px = (double)p[u].x, py = (double)p[u].y, pz = (double)p[u].z;
+                    o[3 * a] = (float)(q[0] * px + q[1] * py + q[2] * pz + q[9]);
+                    o[3 * a + 1] = (float)(q[3] * px + q[4] * py + q[5] * pz + q[10]);
+                    o[3 * a + 2] = (float)(q[6] * px + q[7] * py + q[8] * pz + q[11]);
+                }
+            }
+        }
+        __syncthreads();  // red and rt_s are reused by the next frame
     }
-}
-
-// out[f][a] = R x + t for every atom, f64 in the reference's order, cast to f32.
-__global__ __launch_bounds__(kAlBlock) void al_apply_kernel(const float4 *__restrict__ aos,
-                                                            long long n_atoms, int n_frames,
-                                                            const double *__restrict__ rt64,
-                                                            float *__restrict__ out) {
-    const long long total = n_atoms * n_frames;
-    for (long long t = (long long)blockIdx.x * kAlBlock + threadIdx.x; t < total;
-         t += (long long)gridDim.x * kAlBlock) {
-        const int f = (int)(t / n_atoms);
-        const double *q = rt64 + (long long)f * 12;
-        const float4 p = aos[t];
-        const double x = (double)p.x, y = (double)p.y, z = (double)p.z;
-        const double rx = q[0] * x + q[1] * y + q[2] * z + q[9];
-        const double ry = q[3] * x + q[4] * y + q[5] * z + q[10];
-        const double rz = q[6] * x + q[7] * y + q[8] * z + q[11];
-        out[3 * t] = (float)rx;
-        out[3 * t + 1] = (float)ry;
-        out[3 * t + 2] = (float)rz;
-    }
-}
-
-int al_grid(long long total) {
-    return (int)std::max<long long>(1, std::min<long long>((total + kAlBlock - 1) / kAlBlock, 65536));
 }
 
 }  // namespace
@@ -199,18 +204,12 @@ void al_enqueue(wmd_align *h, const float4 *d_aos, size_t n_atoms, size_t F, dou
     WMD_REQUIRE(h->have_ref, WMD_ERR_MISMATCH, "reference not initialized");
     if (F == 0) return;
     {
-        ProfScope prof(h->ctx, st, "align_fit");
+        ProfScope prof(h->ctx, st, d_xyz ? "align_superpose" : "align_fit");
         hipLaunchKernelGGL(al_fit_kernel, dim3((unsigned)std::min<size_t>(F, 65536)), dim3(kAlBlock), 0,
                            st, d_aos, (long long)n_atoms, (int)F, h->d_sel.as<unsigned>(),
                            h->wts.empty() ? nullptr : h->d_wts.as<float>(), (int)h->sel.size(),
                            h->d_refc.as<double>(), h->ref_g, h->cy[0], h->cy[1], h->cy[2],
-                           (int)h->norotate, d_rt64, d_rt32);
-        WMD_HIP_CHECK(hipGetLastError());
-    }
-    if (d_xyz) {
-        ProfScope prof(h->ctx, st, "align_apply");
-        hipLaunchKernelGGL(al_apply_kernel, dim3(al_grid((long long)(n_atoms * F))), dim3(kAlBlock), 0,
-                           st, d_aos, (long long)n_atoms, (int)F, (const double *)d_rt64, d_xyz);
+                           (int)h->norotate, d_rt64, d_rt32, d_xyz);
         WMD_HIP_CHECK(hipGetLastError());
     }
 }
