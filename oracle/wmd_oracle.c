@@ -34,7 +34,7 @@
  *                        numpy.linalg.svd weighted Kabsch restated independently.
  *   orc_sf_counts / orc_sf_finalize
  *                        structure_factor_shapes (correlation_structure_tests.rs:759-780),
- *                        an independent numpy float64 restatement, and the ideal-gas limit.
+ *                        an independent numpy float64 restatement, closed-form lattice shell counts.
  */
 #include <math.h>
 #include <stddef.h>
