@@ -336,6 +336,65 @@ def align_bench(ctx, dev, F=5000, reps=3, cpu_frames=40):
             "workload": "SuperposePlan, 5,000 chain frames x 20,000 atoms, fit on 2,000 CA"}
 
 
+def host_ingest_bench(ctx, frames=96, rmsd_frames=1000, reps=2):
+    """The host-buffer entry points end to end (PCIe and host copies included):
+    RdfPlan's full chunks of the 100,000-water system (300,000 atoms, OW
+    selected on the device) through wmd_rdf_accumulate, and RmsdPlan's
+    20,000-atom chain chunks through wmd_rmsd_run.  These rates are reported
+    beside the bench value, never as it."""
+    import numpy as np
+
+    from warp_md_amd import workloads
+    from warp_md_amd._native import check, lib, ptr
+
+    L = lib()
+    w = workloads.water_lattice(n_side=47, box=144.3, n_frames=8, n_waters=100000, seed=1)
+    N = w["coords"].shape[1]
+    x4 = np.ones((frames, N, 4), np.float32)
+    for f in range(frames):
+        x4[f, :, :3] = w["coords"][f % 8]
+    boxes = np.repeat(np.diag([144.3] * 3).reshape(1, 9), frames, 0).astype(np.float32)
+    ow = w["ow"]
+    h = ctypes.c_void_p()
+    check(L.wmd_rdf_create(ctx.handle, ptr(ow), ow.size, ptr(ow), ow.size, 500, ctypes.c_float(15.0),
+                           1, 0, ctypes.byref(h)))
+    check(L.wmd_rdf_accumulate(h, ptr(x4[:8]), N, 8, ptr(boxes[:8])))  # warm-up
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        check(L.wmd_rdf_accumulate(h, ptr(x4), N, frames, ptr(boxes)))
+    rdf_s = (time.perf_counter() - t0) / reps
+    L.wmd_rdf_destroy(h)
+    del x4
+    c = workloads.chain(n_frames=rmsd_frames, seed=2)
+    nc = c["ref"].shape[0]
+    cx4 = np.ones((rmsd_frames, nc, 4), np.float32)
+    cx4[:, :, :3] = c["coords"]
+    r4 = np.ones((nc, 4), np.float32)
+    r4[:, :3] = c["ref"]
+    allat = np.arange(nc, dtype=np.uint32)
+    hr = ctypes.c_void_p()
+    check(L.wmd_rmsd_create(ctx.handle, ptr(c["ca"]), c["ca"].size, ptr(allat), allat.size, ptr(r4), nc,
+                            1, ctypes.byref(hr)))
+    out = np.zeros(rmsd_frames, np.float32)
+    check(L.wmd_rmsd_run(hr, ptr(cx4[:8]), nc, 8, ptr(out[:8]), None))
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        check(L.wmd_rmsd_run(hr, ptr(cx4), nc, rmsd_frames, ptr(out), None))
+    rmsd_s = (time.perf_counter() - t0) / reps
+    L.wmd_rmsd_destroy(hr)
+    pair_evals = frames * (float(ow.size) * ow.size - ow.size)
+    res = {"rdf": {"frames": frames, "atoms_per_frame": N, "s": rdf_s,
+                   "host_GB_per_s": frames * N * 16 / rdf_s / 1e9,
+                   "pair_evals_per_s": pair_evals / rdf_s},
+           "rmsd": {"frames": rmsd_frames, "atoms_per_frame": nc, "s": rmsd_s,
+                    "host_GB_per_s": rmsd_frames * nc * 16 / rmsd_s / 1e9,
+                    "frames_per_s": rmsd_frames / rmsd_s}}
+    log(f"host ingest: RDF {frames} x {N} atoms {rdf_s * 1e3:.1f} ms "
+        f"({res['rdf']['host_GB_per_s']:.1f} GB/s of chunks, {res['rdf']['pair_evals_per_s']:.3g} pair-evals/s); "
+        f"RMSD {rmsd_frames} x {nc} atoms {rmsd_s * 1e3:.1f} ms ({res['rmsd']['host_GB_per_s']:.1f} GB/s)")
+    return res
+
+
 def uneven_batch_bench(ctx, dev, reps=3, autotune=True):
     """256 trajectories, atom counts log-uniform in [500, 50000], cubic boxes at
     0.1 atoms/A^3, 50 uniform-random frames each; RDF (300 bins, r_max 12 A)
@@ -432,6 +491,8 @@ def main():
     ap.add_argument("--no-pairwise", action="store_true", help="skip the pairwise RMSD matrix")
     ap.add_argument("--only-pairwise", action="store_true", help="time only the pairwise RMSD matrix")
     ap.add_argument("--no-align", action="store_true", help="skip the superposition leg")
+    ap.add_argument("--no-ingest", action="store_true", help="skip the host-buffer (PCIe-inclusive) leg")
+    ap.add_argument("--only-ingest", action="store_true", help="time only the host-buffer leg")
     ap.add_argument("--only-f2", action="store_true", help="time only the pairwise and superposition legs")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -451,6 +512,9 @@ def main():
     L = lib()
     V = lambda t: ctypes.c_void_p(t.data_ptr())
 
+    if args.only_ingest:
+        print(json.dumps({"host_ingest": host_ingest_bench(ctx)}), flush=True)
+        return
     if args.only_pairwise or args.only_f2:
         out = {"pairwise_rmsd": pairwise_bench(ctx, dev)}
         if args.only_f2:
@@ -565,6 +629,7 @@ def main():
     batch = None if args.no_batch else uneven_batch_bench(ctx, dev)
     pairwise = None if args.no_pairwise else pairwise_bench(ctx, dev)
     superpose = None if args.no_align else align_bench(ctx, dev)
+    ingest = None if args.no_ingest else host_ingest_bench(ctx)
     cpu = None if args.no_cpu else cpu_baseline()
 
     line = {
@@ -596,6 +661,7 @@ def main():
         "uneven_batch": batch,
         "pairwise_rmsd": pairwise,
         "superpose": superpose,
+        "host_ingest": ingest,
         "roofline": {
             "kernel": "rdf_cell_kernel",
             "bound": "valu",

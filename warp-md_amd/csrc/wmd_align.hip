@@ -40,64 +40,60 @@ __device__ __forceinline__ void al_block_sum(double *v, double *red) {
     for (int c = 0; c < CNT; ++c) v[c] = red[c] + red[CNT + c] + red[2 * CNT + c] + red[3 * CNT + c];
 }
 
-// One workgroup per frame (grid-stride): weighted centroid cx = sum w x / sum w
-// (centroid_weighted, geometry_math.rs:202-233), H = sum w (x - cx)(y - cy)^T,
-// gx = sum w |x - cx|^2, then R (QCP, or I for norotate) and t = cy - R cx.
+// One workgroup per frame (grid-stride), one pass over the selection:
+// W = sum w, Sx = sum w x, Sxy = sum w x (y - cy)^T and Sxx = sum w |x|^2 in
+// f64, then cx = Sx / W (centroid_weighted, geometry_math.rs:202-233),
+// H = sum w (x - cx)(y - cy)^T = Sxy - cx (sum w (y - cy))^T (the last sum is
+// a host constant, zero up to rounding), gx = Sxx - W |cx|^2 (only the start of
+// the QCP Newton iteration), R (QCP, or I for norotate) and t = cy - R cx.
 // aos: [F][n_src] xyzw; sel [n]; wts [n] or NULL; ref_c [n][3] = y - cy
-// (host f64, centred), ref_g = sum w |y - cy|^2, cy [3].
+// (host f64, centred), ref_s = sum w (y - cy), ref_g = sum w |y - cy|^2, cy [3].
 // rt64 [F][12] (R row-major, t), rt32 [F][12] or NULL; xyz NULL or
 // [F][n_src][3] = R x + t for every atom (f64, reference order, cast to f32).
 __global__ __launch_bounds__(kAlBlock) void al_fit_kernel(
     const float4 *__restrict__ aos, long long n_src, int n_frames, const unsigned *__restrict__ sel,
-    const float *__restrict__ wts, int n, const double *__restrict__ ref_c, double ref_g,
-    double cy0, double cy1, double cy2, int norotate, double *__restrict__ rt64,
-    float *__restrict__ rt32, float *__restrict__ xyz) {
-    __shared__ double red[4 * 10];
+    const float *__restrict__ wts, int n, const double *__restrict__ ref_c, double rs0, double rs1,
+    double rs2, double ref_g, double cy0, double cy1, double cy2, int norotate,
+    double *__restrict__ rt64, float *__restrict__ rt32, float *__restrict__ xyz) {
+    __shared__ double red[4 * 14];
     __shared__ double rt_s[12];
     for (int f = blockIdx.x; f < n_frames; f += gridDim.x) {
         const float4 *x = aos + (long long)f * n_src;
-        double s[4] = {0.0, 0.0, 0.0, 0.0};
+        // m: Sx[3], W, Sxy[9], Sxx
+        double m[14];
+#pragma unroll
+        for (int c = 0; c < 14; ++c) m[c] = 0.0;
         for (int k = threadIdx.x; k < n; k += kAlBlock) {
             const float4 p = x[sel[k]];
             const double w = wts ? (double)wts[k] : 1.0;
-            if (wts) {
-                s[0] += (double)p.x * w;
-                s[1] += (double)p.y * w;
-                s[2] += (double)p.z * w;
-                s[3] += w;
-            } else {
-                s[0] += (double)p.x;
-                s[1] += (double)p.y;
-                s[2] += (double)p.z;
-            }
+            const double xv[3] = {(double)p.x, (double)p.y, (double)p.z};
+            const double yr[3] = {ref_c[3 * k], ref_c[3 * k + 1], ref_c[3 * k + 2]};
+#pragma unroll
+            for (int a = 0; a < 3; ++a) m[a] += xv[a] * w;
+            m[3] += w;
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) m[4 + 3 * a + c] += xv[a] * yr[c] * w;
+            m[13] += (xv[0] * xv[0] + xv[1] * xv[1] + xv[2] * xv[2]) * w;
         }
-        al_block_sum<4>(s, red);
-        const double wsum = wts ? s[3] : (double)n;
+        al_block_sum<14>(m, red);
+        const double wsum = wts ? m[3] : (double)n;
         double cx[3] = {0.0, 0.0, 0.0};
         if (wsum != 0.0)
-            for (int d = 0; d < 3; ++d) cx[d] = s[d] / wsum;
-        double m[10];
+            for (int d = 0; d < 3; ++d) cx[d] = m[d] / wsum;
+        const double rs[3] = {rs0, rs1, rs2};
+        double H[10];
 #pragma unroll
-        for (int c = 0; c < 10; ++c) m[c] = 0.0;
-        if (!norotate) {
-            for (int k = threadIdx.x; k < n; k += kAlBlock) {
-                const float4 p = x[sel[k]];
-                const double w = wts ? (double)wts[k] : 1.0;
-                const double xr[3] = {(double)p.x - cx[0], (double)p.y - cx[1], (double)p.z - cx[2]};
-                const double yr[3] = {ref_c[3 * k], ref_c[3 * k + 1], ref_c[3 * k + 2]};
+        for (int a = 0; a < 3; ++a)
 #pragma unroll
-                for (int a = 0; a < 3; ++a)
-#pragma unroll
-                    for (int b = 0; b < 3; ++b) m[3 * a + b] += xr[a] * yr[b] * w;
-                m[9] += (xr[0] * xr[0] + xr[1] * xr[1] + xr[2] * xr[2]) * w;
-            }
-            al_block_sum<10>(m, red);
-        }
+            for (int c = 0; c < 3; ++c) H[3 * a + c] = m[4 + 3 * a + c] - cx[a] * rs[c];
+        H[9] = m[13] - wsum * (cx[0] * cx[0] + cx[1] * cx[1] + cx[2] * cx[2]);
         if (threadIdx.x == 0) {
             double R[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
             if (!norotate && n > 0) {
                 QcpResult res;
-                qcp_solve(m, m[9], ref_g, true, &res);
+                qcp_solve(H, H[9] > 0.0 ? H[9] : 0.0, ref_g, true, &res);
                 for (int i = 0; i < 9; ++i) R[i] = res.rot[i];
             }
             const double cy[3] = {cy0, cy1, cy2};
@@ -150,6 +146,7 @@ struct wmd_align {
     bool norotate = false;
     bool have_ref = false;
     double cy[3] = {0, 0, 0};
+    double ref_s[3] = {0, 0, 0};  // sum w (y - cy): zero up to rounding
     double ref_g = 0.0;
     DevBuf d_sel, d_wts, d_refc, d_aos, d_rt64, d_rt32, d_xyz;
     HostBuf h_stage;
@@ -177,7 +174,7 @@ void al_set_reference(wmd_align *h, const float *ref_xyzw, size_t ref_stride_ato
     }
     if (!weighted) wsum = (double)n;
     for (int d = 0; d < 3; ++d) h->cy[d] = (n == 0 || wsum == 0.0) ? 0.0 : s[d] / wsum;
-    double g = 0.0;
+    double g = 0.0, rs[3] = {0, 0, 0};
     for (size_t i = 0; i < n; ++i) {
         const float *p = ref_xyzw + (size_t)(idx ? idx[i] : i) * 4;
         const double w = weighted ? (double)h->wts[i] : 1.0;
@@ -185,9 +182,11 @@ void al_set_reference(wmd_align *h, const float *ref_xyzw, size_t ref_stride_ato
         for (int d = 0; d < 3; ++d) {
             yc[3 * i + d] = (double)p[d] - h->cy[d];
             e += yc[3 * i + d] * yc[3 * i + d];
+            rs[d] += yc[3 * i + d] * w;
         }
         g += e * w;
     }
+    for (int d = 0; d < 3; ++d) h->ref_s[d] = rs[d];
     (void)ref_stride_atoms;
     h->ref_g = g;
     h->d_refc.reserve(yc.size() * 8);
@@ -208,7 +207,8 @@ void al_enqueue(wmd_align *h, const float4 *d_aos, size_t n_atoms, size_t F, dou
         hipLaunchKernelGGL(al_fit_kernel, dim3((unsigned)std::min<size_t>(F, 65536)), dim3(kAlBlock), 0,
                            st, d_aos, (long long)n_atoms, (int)F, h->d_sel.as<unsigned>(),
                            h->wts.empty() ? nullptr : h->d_wts.as<float>(), (int)h->sel.size(),
-                           h->d_refc.as<double>(), h->ref_g, h->cy[0], h->cy[1], h->cy[2],
+                           h->d_refc.as<double>(), h->ref_s[0], h->ref_s[1], h->ref_s[2], h->ref_g,
+                           h->cy[0], h->cy[1], h->cy[2],
                            (int)h->norotate, d_rt64, d_rt32, d_xyz);
         WMD_HIP_CHECK(hipGetLastError());
     }

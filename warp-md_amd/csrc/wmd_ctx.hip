@@ -1,7 +1,9 @@
 // Context, error handling, buffers and host-side helpers of the engine.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <stdexcept>
+#include <thread>
 
 #include "wmd_internal.hpp"
 
@@ -35,6 +37,38 @@ void *DevBuf::reserve(size_t n) {
     WMD_HIP_CHECK(hipMalloc(&ptr, want));
     bytes = want;
     return ptr;
+}
+
+Event::~Event() {
+    if (ev) (void)hipEventDestroy(ev);
+}
+
+hipEvent_t Event::get() {
+    if (!ev) WMD_HIP_CHECK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    return ev;
+}
+
+void Event::wait() {
+    if (ev) WMD_HIP_CHECK(hipEventSynchronize(ev));
+}
+
+void host_copy(void *dst, const void *src, size_t bytes) {
+    const size_t kMin = 16ull << 20;  // below this one thread is as fast
+    unsigned nt = std::thread::hardware_concurrency();
+    nt = std::max(1u, std::min(nt, 8u));
+    if (bytes < kMin || nt == 1) {
+        std::memcpy(dst, src, bytes);
+        return;
+    }
+    const size_t part = round_up((bytes + nt - 1) / nt, 4096);
+    std::vector<std::thread> pool;
+    for (unsigned t = 0; t < nt; ++t) {
+        const size_t b0 = (size_t)t * part;
+        if (b0 >= bytes) break;
+        const size_t nb = std::min(part, bytes - b0);
+        pool.emplace_back([=] { std::memcpy((char *)dst + b0, (const char *)src + b0, nb); });
+    }
+    for (auto &th : pool) th.join();
 }
 
 HostBuf::~HostBuf() {

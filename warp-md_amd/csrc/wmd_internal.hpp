@@ -87,6 +87,21 @@ struct HostBuf {
     void *reserve(size_t n);
 };
 
+// A HIP event created on first use (timing disabled), destroyed with its owner.
+struct Event {
+    hipEvent_t ev = nullptr;
+    Event() = default;
+    Event(const Event &) = delete;
+    Event &operator=(const Event &) = delete;
+    ~Event();
+    hipEvent_t get();
+    // Blocks until the last record has completed (no-op before the first record).
+    void wait();
+};
+
+// memcpy of host memory with up to 8 threads for large copies (pinned staging).
+void host_copy(void *dst, const void *src, size_t bytes);
+
 // ---------------------------------------------------------------------------
 // Context (the GpuContext replacement): device, one default-priority stream.
 // ---------------------------------------------------------------------------

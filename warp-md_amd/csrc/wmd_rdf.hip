@@ -791,10 +791,15 @@ struct wmd_rdf {
     int cell_k = 0;        // 0 = automatic
     int last_path = 0;
     int last_nc[3] = {0, 0, 0};
-    DevBuf d_counts, d_thr, d_elo, d_ehi, d_tab, d_tab2, d_wts, d_io, d_boxes, d_aos, d_soa;
+    DevBuf d_counts, d_thr, d_elo, d_ehi, d_tab, d_tab2, d_wts, d_io, d_boxes;
     DevBuf d_tiles, d_queue;
+    int tiles_n = -1;  // atom count the tile pairs in d_tiles were built for
     CellScratch cells;
-    HostBuf h_stage, h_boxes;
+    HostBuf h_boxes;
+    // host-chunk pipeline: two staging / device buffers, events guarding reuse
+    HostBuf p_stage[2];
+    DevBuf p_aos[2], p_soa[2];
+    Event p_copied[2], ev_boxes;
     // f64 distance mode (StructureFactorPlan): thresholds on the f64 squared
     // distance, r_max and the length scale in f64
     bool f64 = false;
@@ -934,6 +939,7 @@ void cells_dispatch(int pbc, bool lt, bool weighted, bool edge, bool table, A...
 int upload_tile_pairs(wmd_rdf *h, int n, hipStream_t st) {
     const int tiles = (n + kTile - 1) / kTile;
     const int n_tp = tiles * (tiles + 1) / 2;
+    if (h->tiles_n == n) return n_tp;  // d_tiles already holds them
     std::vector<int2> tp;
     tp.reserve(n_tp);
     for (int i = 0; i < tiles; ++i)
@@ -941,6 +947,7 @@ int upload_tile_pairs(wmd_rdf *h, int n, hipStream_t st) {
     int2 *d_tp = (int2 *)h->d_tiles.reserve(sizeof(int2) * std::max(n_tp, 1));
     WMD_HIP_CHECK(hipMemcpyAsync(d_tp, tp.data(), sizeof(int2) * n_tp, hipMemcpyHostToDevice, st));
     WMD_HIP_CHECK(hipStreamSynchronize(st));  // the vector dies at scope exit
+    h->tiles_n = n;
     return n_tp;
 }
 
@@ -974,14 +981,14 @@ void accumulate_f64(wmd_rdf *h, const float *d_soa, size_t stride, size_t n_fram
 
 // Core: accumulate frames of an IO-selection SoA store already on the device.
 void accumulate_device(wmd_rdf *h, const float *d_soa, size_t stride, size_t n_frames,
-                       const float *boxes_host, hipStream_t st) {
+                       const float *boxes_host, hipStream_t st, bool pipelined = false) {
     const int n = (int)h->io.size();
     if (n_frames == 0 || h->bins == 0) return;
     WMD_REQUIRE(stride >= (size_t)n && stride % 4 == 0, WMD_ERR_INVALID,
                 "stride must be a multiple of 4 and >= the IO selection size");
     WMD_REQUIRE(h->pbc == WMD_PBC_NONE || boxes_host, WMD_ERR_INVALID, "boxes required for PBC");
     const BoxSet B = make_boxes(h->pbc, boxes_host, n_frames);
-    const FrameBox *d_boxes = upload_boxes(B, h->h_boxes, h->d_boxes, st);
+    const FrameBox *d_boxes = upload_boxes(B, h->h_boxes, h->d_boxes, st, pipelined ? &h->ev_boxes : nullptr);
 
     if (h->f64) {
         accumulate_f64(h, d_soa, stride, n_frames, d_boxes, st);
@@ -1000,16 +1007,8 @@ void accumulate_device(wmd_rdf *h, const float *d_soa, size_t stride, size_t n_f
 
     if (k_use == 0) {
         // ---- brute force ----------------------------------------------------
-        int tiles = (n + kTile - 1) / kTile;
-        int n_tp = tiles * (tiles + 1) / 2;
-        std::vector<int2> tp;
-        tp.reserve(n_tp);
-        for (int i = 0; i < tiles; ++i)
-            for (int j = i; j < tiles; ++j) tp.push_back(make_int2(i, j));
-        int2 *d_tp = (int2 *)h->d_tiles.reserve(sizeof(int2) * n_tp);
-        WMD_HIP_CHECK(hipMemcpyAsync(d_tp, tp.data(), sizeof(int2) * n_tp, hipMemcpyHostToDevice, st));
-        // the vector dies at scope exit: make the copy complete first
-        WMD_HIP_CHECK(hipStreamSynchronize(st));
+        const int n_tp = upload_tile_pairs(h, n, st);
+        const int2 *d_tp = (const int2 *)h->d_tiles.ptr;
         long long fs = 3 * (long long)stride;
         brute_dispatch(!fast_wrap, h->weighted, edge, table, h->pbc, h, st, d_soa, fs, (int)stride,
                        n, (int)n_frames, 0LL, d_boxes, (const int2 *)d_tp, n_tp, p);
@@ -1041,7 +1040,11 @@ void accumulate_device(wmd_rdf *h, const float *d_soa, size_t stride, size_t n_f
     h->last_nc[0] = nc[0]; h->last_nc[1] = nc[1]; h->last_nc[2] = nc[2];
 }
 
-// Host AoS chunk -> device SoA store of the IO atoms -> accumulate.
+// Host AoS chunk -> device SoA store of the IO atoms -> accumulate, as a
+// two-stage pipeline: batch k + 1 is copied into the other pinned staging
+// buffer (several host threads) while the GPU runs batch k's H2D copy, gather
+// and pair kernels; all device work stays in order on the context's stream,
+// and an event per staging buffer tells the host when it may be refilled.
 void accumulate_host(wmd_rdf *h, const float *coords, size_t n_src, size_t n_frames,
                      const float *boxes, bool selected) {
     if (n_frames == 0 || h->bins == 0) return;
@@ -1054,25 +1057,35 @@ void accumulate_host(wmd_rdf *h, const float *coords, size_t n_src, size_t n_fra
                             std::to_string(n_src) + " atoms");
     }
     hipStream_t st = h->ctx->stream;
-    size_t stride = round_up(std::max(n, 1), 4);
-    // frames per staging batch (~256 MB of AoS)
-    size_t fbytes = n_src * 16;
-    size_t batch = std::max<size_t>(1, std::min<size_t>(n_frames, (256ull << 20) / std::max<size_t>(fbytes, 1)));
-    float4 *d_aos = (float4 *)h->d_aos.reserve(batch * fbytes);
-    float *d_soa = (float *)h->d_soa.reserve(batch * 3 * stride * 4);
-    void *stage = h->h_stage.reserve(batch * fbytes);
-    for (size_t f0 = 0; f0 < n_frames; f0 += batch) {
-        size_t nf = std::min(batch, n_frames - f0);
-        WMD_HIP_CHECK(hipStreamSynchronize(st));  // staging and store free again
-        std::memcpy(stage, coords + f0 * n_src * 4, nf * fbytes);
-        WMD_HIP_CHECK(hipMemcpyAsync(d_aos, stage, nf * fbytes, hipMemcpyHostToDevice, st));
+    const size_t stride = round_up(std::max(n, 1), 4);
+    // frames per batch (~128 MB of AoS per staging buffer)
+    const size_t fbytes = n_src * 16;
+    const size_t batch = std::max<size_t>(1, std::min<size_t>(n_frames, (128ull << 20) / std::max<size_t>(fbytes, 1)));
+    for (int b = 0; b < 2; ++b) {
+        if (h->p_stage[b].bytes < batch * fbytes || h->p_soa[b].bytes < batch * 3 * stride * 4) {
+            WMD_HIP_CHECK(hipStreamSynchronize(st));  // growing frees buffers queued work may use
+            h->p_stage[b].reserve(batch * fbytes);
+            h->p_aos[b].reserve(batch * fbytes);
+            h->p_soa[b].reserve(batch * 3 * stride * 4);
+        }
+    }
+    size_t k = 0;
+    for (size_t f0 = 0; f0 < n_frames; f0 += batch, ++k) {
+        const int b = (int)(k & 1);
+        const size_t nf = std::min(batch, n_frames - f0);
+        h->p_copied[b].wait();  // the copy out of this staging buffer two batches ago is done
+        host_copy(h->p_stage[b].ptr, coords + f0 * n_src * 4, nf * fbytes);
+        WMD_HIP_CHECK(hipMemcpyAsync(h->p_aos[b].ptr, h->p_stage[b].ptr, nf * fbytes,
+                                     hipMemcpyHostToDevice, st));
+        WMD_HIP_CHECK(hipEventRecord(h->p_copied[b].get(), st));
+        float *d_soa = (float *)h->p_soa[b].ptr;
         int g;
         grid1d((long long)stride * nf, &g);
-        hipLaunchKernelGGL(gather_soa_kernel, dim3(g), dim3(kBlock), 0, st, (const float4 *)d_aos,
+        hipLaunchKernelGGL(gather_soa_kernel, dim3(g), dim3(kBlock), 0, st, (const float4 *)h->p_aos[b].ptr,
                            (long long)n_src, selected ? nullptr : h->d_io.as<unsigned>(), n,
                            (int)stride, (int)nf, d_soa);
         WMD_HIP_CHECK(hipGetLastError());
-        accumulate_device(h, d_soa, stride, nf, boxes ? boxes + 9 * f0 : nullptr, st);
+        accumulate_device(h, d_soa, stride, nf, boxes ? boxes + 9 * f0 : nullptr, st, true);
     }
     WMD_HIP_CHECK(hipStreamSynchronize(st));
 }

@@ -1299,13 +1299,20 @@ BoxSet make_boxes(int pbc, const float *boxes_host, size_t n_frames) {
 }
 
 // Copies the batch's boxes to the device through a pinned staging buffer.
-FrameBox *upload_boxes(const BoxSet &B, HostBuf &hbuf, DevBuf &dbuf, hipStream_t st) {
+// guard: NULL, or an event recorded after the previous copy out of hbuf; the
+// host then waits for that copy only, not for the whole stream.
+FrameBox *upload_boxes(const BoxSet &B, HostBuf &hbuf, DevBuf &dbuf, hipStream_t st,
+                       Event *guard = nullptr) {
     const size_t bytes = sizeof(FrameBox) * std::max<size_t>(B.fb.size(), 1);
+    if (guard) guard->wait();
+    else WMD_HIP_CHECK(hipStreamSynchronize(st));  // the staging buffer may still be in use
+    if (guard && (bytes > hbuf.bytes || bytes > dbuf.bytes))
+        WMD_HIP_CHECK(hipStreamSynchronize(st));  // growing frees buffers queued work may use
     void *hb = hbuf.reserve(bytes);
-    WMD_HIP_CHECK(hipStreamSynchronize(st));  // the staging buffer may still be in use
     std::memcpy(hb, B.fb.data(), sizeof(FrameBox) * B.fb.size());
     FrameBox *d = (FrameBox *)dbuf.reserve(bytes);
     WMD_HIP_CHECK(hipMemcpyAsync(d, hb, sizeof(FrameBox) * B.fb.size(), hipMemcpyHostToDevice, st));
+    if (guard) WMD_HIP_CHECK(hipEventRecord(guard->get(), st));
     return d;
 }
 

@@ -446,8 +446,11 @@ struct wmd_rmsd {
     int seg1 = 0, seg2 = 0;  // group indices where fit-only / RMSD-only segments start
     size_t n_ref_atoms = 0;
     RefMoments rm{};
-    DevBuf d_ref, d_atoms, d_partials, d_aos, d_soa, d_out32, d_out64, d_err;
-    HostBuf h_stage;
+    DevBuf d_ref, d_atoms, d_partials, d_out32, d_out64, d_err;
+    // host-chunk pipeline: two staging / device buffers, events guarding reuse
+    HostBuf p_stage[2];
+    DevBuf p_aos[2], p_soa[2];
+    Event p_copied[2];
 };
 
 namespace {
@@ -626,27 +629,41 @@ wmd_status wmd_rmsd_run(wmd_rmsd *h, const float *coords, size_t n_atoms, size_t
                     "selection index out of range for the chunk's atom count");
     bind(h->ctx);
     hipStream_t st = h->ctx->stream;
-    size_t fbytes = n_atoms * 16;
-    size_t batch = std::max<size_t>(1, std::min<size_t>(n_frames, (256ull << 20) / std::max<size_t>(fbytes, 1)));
-    float4 *d_aos = (float4 *)h->d_aos.reserve(batch * fbytes);
-    float *d_soa = (float *)h->d_soa.reserve(batch * 3 * h->stride * 4);
-    float *d_o = (float *)h->d_out32.reserve(batch * 4);
-    double *d_e = out_err ? (double *)h->d_err.reserve(batch * 8) : nullptr;
-    void *stage = h->h_stage.reserve(batch * fbytes);
-    for (size_t f0 = 0; f0 < n_frames; f0 += batch) {
-        size_t nf = std::min(batch, n_frames - f0);
+    // two-stage pipeline: the host copies batch k + 1 into the other pinned
+    // staging buffer while the GPU runs batch k (all device work in order on st)
+    const size_t fbytes = n_atoms * 16;
+    const size_t batch = std::max<size_t>(1, std::min<size_t>(n_frames, (128ull << 20) / std::max<size_t>(fbytes, 1)));
+    for (int b = 0; b < 2; ++b) {
+        if (h->p_stage[b].bytes < batch * fbytes || h->p_soa[b].bytes < batch * 3 * h->stride * 4) {
+            WMD_HIP_CHECK(hipStreamSynchronize(st));  // growing frees buffers queued work may use
+            h->p_stage[b].reserve(batch * fbytes);
+            h->p_aos[b].reserve(batch * fbytes);
+            h->p_soa[b].reserve(batch * 3 * h->stride * 4);
+        }
+    }
+    if (h->d_out32.bytes < n_frames * 4 || (out_err && h->d_err.bytes < n_frames * 8))
         WMD_HIP_CHECK(hipStreamSynchronize(st));
-        std::memcpy(stage, coords + f0 * n_atoms * 4, nf * fbytes);
-        WMD_HIP_CHECK(hipMemcpyAsync(d_aos, stage, nf * fbytes, hipMemcpyHostToDevice, st));
+    float *d_o = (float *)h->d_out32.reserve(n_frames * 4);
+    double *d_e = out_err ? (double *)h->d_err.reserve(n_frames * 8) : nullptr;
+    size_t k = 0;
+    for (size_t f0 = 0; f0 < n_frames; f0 += batch, ++k) {
+        const int b = (int)(k & 1);
+        const size_t nf = std::min(batch, n_frames - f0);
+        h->p_copied[b].wait();  // the copy out of this staging buffer two batches ago is done
+        host_copy(h->p_stage[b].ptr, coords + f0 * n_atoms * 4, nf * fbytes);
+        WMD_HIP_CHECK(hipMemcpyAsync(h->p_aos[b].ptr, h->p_stage[b].ptr, nf * fbytes,
+                                     hipMemcpyHostToDevice, st));
+        WMD_HIP_CHECK(hipEventRecord(h->p_copied[b].get(), st));
+        float *d_soa = (float *)h->p_soa[b].ptr;
         long long total = (long long)h->stride * nf;
         int g = (int)std::min<long long>(65536, (total + kBlock - 1) / kBlock);
-        hipLaunchKernelGGL(rmsd_gather_kernel, dim3(g), dim3(kBlock), 0, st, (const float4 *)d_aos,
+        hipLaunchKernelGGL(rmsd_gather_kernel, dim3(g), dim3(kBlock), 0, st, (const float4 *)h->p_aos[b].ptr,
                            (long long)n_atoms, h->d_atoms.as<unsigned>(), (int)h->stride, (int)nf, d_soa);
         WMD_HIP_CHECK(hipGetLastError());
-        run_device(h, d_soa, h->stride, nf, d_o, nullptr, d_e, nullptr, nullptr, st);
-        WMD_HIP_CHECK(hipMemcpyAsync(out_rmsd + f0, d_o, nf * 4, hipMemcpyDeviceToHost, st));
-        if (out_err) WMD_HIP_CHECK(hipMemcpyAsync(out_err + f0, d_e, nf * 8, hipMemcpyDeviceToHost, st));
+        run_device(h, d_soa, h->stride, nf, d_o + f0, nullptr, d_e ? d_e + f0 : nullptr, nullptr, nullptr, st);
     }
+    WMD_HIP_CHECK(hipMemcpyAsync(out_rmsd, d_o, n_frames * 4, hipMemcpyDeviceToHost, st));
+    if (out_err) WMD_HIP_CHECK(hipMemcpyAsync(out_err, d_e, n_frames * 8, hipMemcpyDeviceToHost, st));
     WMD_HIP_CHECK(hipStreamSynchronize(st));
     WMD_API_END
 }
