@@ -55,7 +55,7 @@ void Event::wait() {
 void host_copy(void *dst, const void *src, size_t bytes) {
     const size_t kMin = 16ull << 20;  // below this one thread is as fast
     unsigned nt = std::thread::hardware_concurrency();
-    nt = std::max(1u, std::min(nt, 8u));
+    nt = std::max(1u, std::min(nt, 16u));
     if (bytes < kMin || nt == 1) {
         std::memcpy(dst, src, bytes);
         return;

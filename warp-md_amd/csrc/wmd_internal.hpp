@@ -99,7 +99,7 @@ struct Event {
     void wait();
 };
 
-// memcpy of host memory with up to 8 threads for large copies (pinned staging).
+// memcpy of host memory with up to 16 threads for large copies (pinned staging).
 void host_copy(void *dst, const void *src, size_t bytes);
 
 // ---------------------------------------------------------------------------

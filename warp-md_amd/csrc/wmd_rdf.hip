@@ -1058,9 +1058,9 @@ void accumulate_host(wmd_rdf *h, const float *coords, size_t n_src, size_t n_fra
     }
     hipStream_t st = h->ctx->stream;
     const size_t stride = round_up(std::max(n, 1), 4);
-    // frames per batch (~128 MB of AoS per staging buffer)
+    // frames per batch (~48 MB of AoS per staging buffer)
     const size_t fbytes = n_src * 16;
-    const size_t batch = std::max<size_t>(1, std::min<size_t>(n_frames, (128ull << 20) / std::max<size_t>(fbytes, 1)));
+    const size_t batch = std::max<size_t>(1, std::min<size_t>(n_frames, (48ull << 20) / std::max<size_t>(fbytes, 1)));
     for (int b = 0; b < 2; ++b) {
         if (h->p_stage[b].bytes < batch * fbytes || h->p_soa[b].bytes < batch * 3 * stride * 4) {
             WMD_HIP_CHECK(hipStreamSynchronize(st));  // growing frees buffers queued work may use

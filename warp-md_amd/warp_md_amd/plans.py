@@ -106,18 +106,14 @@ class RdfPlan:
         try:
             if self._cell_k:
                 check(lib().wmd_rdf_set_tuning(h, self._cell_k, 0))
-            n_io = ctypes.c_size_t()
-            check(lib().wmd_rdf_io_selection(h, None, 0, ctypes.byref(n_io)))
-            io = np.zeros(n_io.value, np.uint32)
-            check(lib().wmd_rdf_io_selection(h, ptr(io), io.size, ctypes.byref(n_io)))
             tr = select_frames(traj, frame_indices)
             tr.reset()
             frames = 0
             volume_sum = 0.0
-            step = resolve_chunk_frames(chunk_frames, max(1, io.size))
+            step = resolve_chunk_frames(chunk_frames, max(1, n_atoms))
             need_box = self.pbc != _native.PBC_NONE or self.use_volume
             while True:
-                chunk = tr.read_chunk(step)
+                chunk = tr.read_chunk4(step)
                 if chunk is None:
                     break
                 coords, boxes = chunk
@@ -136,9 +132,8 @@ class RdfPlan:
                             # rdf_box_measure (rdf.rs:480-485): f64 product of the f32 lengths
                             volume_sum += float(m[0]) * float(m[4]) * float(m[8])
                 frames += nf
-                sel_coords = xyzw(np.ascontiguousarray(coords[:, io, :]))
-                bx = boxes if boxes is not None else None
-                check(lib().wmd_rdf_accumulate_selected(h, ptr(sel_coords), nf, ptr(bx)))
+                # the full chunk; the engine gathers the IO selection on the device
+                check(lib().wmd_rdf_accumulate(h, ptr(coords), n_atoms, nf, ptr(boxes)))
             counts = np.zeros(self.bins, np.uint64)
             edge = np.zeros(1, np.uint64)
             check(lib().wmd_rdf_read(h, ptr(counts), ptr(edge)))
@@ -210,45 +205,39 @@ class RmsdPlan:
         for sel in (self.selection, fit):
             if sel.size and int(sel.max()) >= n_atoms:
                 raise RuntimeError("invalid selection: atom index out of range")
-        # the executor reads selected chunks (rmsd/plan.rs:224-230): gather the
-        # union of the selections and express both in its local indices
-        union, inv = np.unique(np.concatenate([self.selection, fit]).astype(np.int64),
-                               return_inverse=True)
-        loc_rmsd = inv[: self.selection.size].astype(np.uint32)
-        loc_fit = inv[self.selection.size:].astype(np.uint32)
         tr = select_frames(traj, frame_indices)
         tr.reset()
         if self.reference_mode == "topology":
             if system.positions0 is None:
                 raise RuntimeError("mismatch: no topology reference coords")
-            ref = system.positions0[union]
+            ref4 = xyzw(np.ascontiguousarray(system.positions0, dtype=np.float32)[None])[0]
         else:
             if tr.n_frames() == 0:
                 return np.zeros(0, np.float32)
-            ref = tr.coords[0][union]  # rmsd/plan.rs:336-354 first frame seen
-        ref4 = xyzw(np.ascontiguousarray(ref, dtype=np.float32)[None])[0]
+            ref4 = np.ascontiguousarray(tr.coords4[0])  # rmsd/plan.rs:336-354 first frame seen
+        # atom-indexed reference and full chunks: the engine gathers the
+        # selections on the device (the executor's selected read, rmsd/plan.rs:224-230)
         h = ctypes.c_void_p()
         if self.fit_selection is None:
-            check(lib().wmd_rmsd_create(ctx.handle, ptr(loc_rmsd), loc_rmsd.size, None, 0,
-                                        ptr(ref4), union.size, int(self.align), ctypes.byref(h)))
+            check(lib().wmd_rmsd_create(ctx.handle, ptr(self.selection), self.selection.size, None, 0,
+                                        ptr(ref4), n_atoms, int(self.align), ctypes.byref(h)))
         else:
-            check(lib().wmd_rmsd_create(ctx.handle, ptr(loc_fit), loc_fit.size, ptr(loc_rmsd),
-                                        loc_rmsd.size, ptr(ref4), union.size, 1, ctypes.byref(h)))
+            check(lib().wmd_rmsd_create(ctx.handle, ptr(fit), fit.size, ptr(self.selection),
+                                        self.selection.size, ptr(ref4), n_atoms, 1, ctypes.byref(h)))
         try:
             out = np.zeros(tr.n_frames(), np.float32)
             err = np.zeros(tr.n_frames(), np.float64)
-            step = resolve_chunk_frames(chunk_frames, max(1, union.size))
+            step = resolve_chunk_frames(chunk_frames, max(1, n_atoms))
             f0 = 0
             while True:
-                chunk = tr.read_chunk(step)
+                chunk = tr.read_chunk4(step)
                 if chunk is None:
                     break
-                coords, _ = chunk
-                nf = coords.shape[0]
-                sel_coords = xyzw(np.ascontiguousarray(coords[:, union, :]))
+                x4, _ = chunk
+                nf = x4.shape[0]
                 o = out[f0:f0 + nf]
                 e = err[f0:f0 + nf]
-                check(lib().wmd_rmsd_run(h, ptr(sel_coords), union.size, nf, ptr(o), ptr(e)))
+                check(lib().wmd_rmsd_run(h, ptr(x4), n_atoms, nf, ptr(o), ptr(e)))
                 f0 += nf
         finally:
             lib().wmd_rmsd_destroy(h)
@@ -324,15 +313,11 @@ class PairDistPlan:
         sumsq = np.zeros(bins, np.float64)
         frames = 0
         try:
-            n_io = ctypes.c_size_t()
-            check(lib().wmd_rdf_io_selection(h, None, 0, ctypes.byref(n_io)))
-            io = np.zeros(n_io.value, np.uint32)
-            check(lib().wmd_rdf_io_selection(h, ptr(io), io.size, ctypes.byref(n_io)))
             tr = select_frames(traj, frame_indices)
             tr.reset()
-            step = 1 if self.output_distribution else resolve_chunk_frames(chunk_frames, max(1, io.size))
+            step = 1 if self.output_distribution else resolve_chunk_frames(chunk_frames, max(1, n_atoms))
             while True:
-                chunk = tr.read_chunk(step)
+                chunk = tr.read_chunk4(step)
                 if chunk is None:
                     break
                 coords, boxes = chunk
@@ -344,9 +329,8 @@ class PairDistPlan:
                         for m in boxes:
                             if np.any(m[[1, 2, 3, 5, 6, 7]] != 0) or np.any(m[[0, 4, 8]] <= 0):
                                 raise RuntimeError("mismatch: pairdist requires orthorhombic box for PBC")
-                sel_coords = xyzw(np.ascontiguousarray(coords[:, io, :]))
-                check(lib().wmd_rdf_accumulate_selected(h, ptr(sel_coords), nf,
-                                                        ptr(boxes if self.pbc != _native.PBC_NONE else None)))
+                check(lib().wmd_rdf_accumulate(h, ptr(coords), n_atoms, nf,
+                                               ptr(boxes if self.pbc != _native.PBC_NONE else None)))
                 frames += nf
                 if self.output_distribution:
                     fc = np.zeros(bins, np.uint64)
@@ -431,13 +415,13 @@ class StructureFactorPlan:
         try:
             tr = select_frames(traj, None)
             tr.reset()
-            step = resolve_chunk_frames(chunk_frames, max(1, sel.size))
+            step = resolve_chunk_frames(chunk_frames, max(1, n_atoms))
             while True:
-                chunk = tr.read_chunk(step)
+                chunk = tr.read_chunk4(step)
                 if chunk is None:
                     break
-                coords, boxes = chunk
-                nf = coords.shape[0]
+                x4, boxes = chunk
+                nf = x4.shape[0]
                 if self.pbc == _native.PBC_ORTHO:
                     if boxes is None:
                         raise RuntimeError("mismatch: structure factor requires orthorhombic box for PBC")
@@ -446,7 +430,6 @@ class StructureFactorPlan:
                             raise RuntimeError("mismatch: structure factor requires orthorhombic box for PBC")
                         volume_sum += (float(m[0]) * float(m[4]) * float(m[8])) * s3
                 frames += nf
-                x4 = xyzw(coords)
                 check(lib().wmd_rdf_accumulate(h, ptr(x4), n_atoms, nf,
                                                ptr(boxes if self.pbc != _native.PBC_NONE else None)))
             counts = np.zeros(self.bins, np.uint64)
@@ -502,14 +485,13 @@ class PairwiseRmsdPlan:
         try:
             tr = select_frames(traj, frame_indices)
             tr.reset()
-            step = resolve_chunk_frames(chunk_frames, max(1, sel.size))
+            step = resolve_chunk_frames(chunk_frames, max(1, n_atoms))
             while True:
-                chunk = tr.read_chunk(step)
+                chunk = tr.read_chunk4(step)
                 if chunk is None:
                     break
-                coords, _ = chunk
-                x4 = xyzw(coords)
-                check(lib().wmd_pairwise_append(h, ptr(x4), n_atoms, coords.shape[0]))
+                x4, _ = chunk
+                check(lib().wmd_pairwise_append(h, ptr(x4), n_atoms, x4.shape[0]))
             nf = ctypes.c_size_t()
             check(lib().wmd_pairwise_frames(h, ctypes.byref(nf)))
             out = np.zeros((nf.value, nf.value), np.float32)
@@ -558,12 +540,11 @@ class _SuperposeBase:
             step = resolve_chunk_frames(chunk_frames, max(1, n_atoms))
             first = True
             while True:
-                chunk = tr.read_chunk(step)
+                chunk = tr.read_chunk4(step)
                 if chunk is None:
                     break
-                coords, _ = chunk
-                nf = coords.shape[0]
-                x4 = xyzw(coords)
+                x4, _ = chunk
+                nf = x4.shape[0]
                 if first and ref4 is None:
                     # frame0: the first frame read is the reference (plan_exec.rs:126-139)
                     f0 = np.ascontiguousarray(x4[0])

@@ -125,13 +125,27 @@ class Trajectory:
     """In-memory trajectory (``Trajectory.from_numpy``, io.rs:669-686).
 
     coords: (frames, atoms, 3) or (atoms, 3), float32 or float64 (cast to f32).
+    The frames are held once in the reference's FrameChunk layout,
+    (frames, atoms, 4) float32 with w = 1 (io.rs:1995), so plans hand chunks
+    to the engine without a copy; ``coords`` is the (frames, atoms, 3) view.
     """
 
     def __init__(self, coords: np.ndarray, boxes: Optional[np.ndarray], time_ps=None):
-        self.coords = coords
+        c = np.asarray(coords, dtype=np.float32)
+        if c.shape[-1] == 3:
+            c4 = np.empty(c.shape[:-1] + (4,), np.float32)
+            c4[..., :3] = c
+            c4[..., 3] = 1.0
+        else:
+            c4 = np.ascontiguousarray(c)
+        self.coords4 = c4
         self.boxes = boxes
         self.time_ps = time_ps
         self._cursor = 0
+
+    @property
+    def coords(self) -> np.ndarray:
+        return self.coords4[..., :3]
 
     @staticmethod
     def from_numpy(coords, box=None, time_ps=None) -> "Trajectory":
@@ -142,16 +156,15 @@ class Trajectory:
             c = c[None]
         if c.ndim != 3 or c.shape[2] != 3:
             raise RuntimeError("coords must have final dimension 3")
-        c = np.ascontiguousarray(c, dtype=np.float32)
         boxes = _boxes_from(box, c.shape[0])
         t = None if time_ps is None else np.asarray(time_ps, dtype=np.float32)
         return Trajectory(c, boxes, t)
 
     def n_atoms(self) -> int:
-        return int(self.coords.shape[1])
+        return int(self.coords4.shape[1])
 
     def n_frames(self) -> int:
-        return int(self.coords.shape[0])
+        return int(self.coords4.shape[0])
 
     def reset(self):
         self._cursor = 0
@@ -163,14 +176,30 @@ class Trajectory:
         idx = np.asarray(list(frames), dtype=np.int64)
         boxes = None if self.boxes is None else self.boxes[idx]
         t = None if self.time_ps is None else self.time_ps[idx]
-        return Trajectory(self.coords[idx], boxes, t)
+        return Trajectory(self.coords4[idx], boxes, t)
 
-    def read_chunk(self, max_frames: int):
-        """Next (coords[f, N, 3], boxes[f, 9] or None); None at the end."""
+    def _advance(self, max_frames: int):
         if self._cursor >= self.n_frames():
             return None
         a = self._cursor
         b = min(self.n_frames(), a + max(1, int(max_frames)))
         self._cursor = b
+        return a, b
+
+    def read_chunk(self, max_frames: int):
+        """Next (coords[f, N, 3], boxes[f, 9] or None); None at the end."""
+        r = self._advance(max_frames)
+        if r is None:
+            return None
+        a, b = r
         boxes = None if self.boxes is None else self.boxes[a:b]
-        return self.coords[a:b], boxes
+        return self.coords4[a:b, :, :3], boxes
+
+    def read_chunk4(self, max_frames: int):
+        """Next (xyzw[f, N, 4] contiguous, boxes[f, 9] or None); None at the end."""
+        r = self._advance(max_frames)
+        if r is None:
+            return None
+        a, b = r
+        boxes = None if self.boxes is None else self.boxes[a:b]
+        return self.coords4[a:b], boxes
