@@ -51,12 +51,15 @@ enum { WMD_PBC_NONE = 0, WMD_PBC_ORTHO = 1, WMD_PBC_TRICLINIC = 2 };
 /* wmd_rdf_create flags */
 #define WMD_RDF_EDGE_DIAGNOSTICS 0x1u /* count counted pairs within 1e-5 A of a bin edge */
 #define WMD_RDF_FORCE_BRUTE 0x2u      /* never use the cell list (testing / autotuning) */
+#define WMD_RDF_PLANAR_XY 0x4u        /* dimension "xy": r = sqrt(dx^2 + dy^2) (rdf.rs:473-478) */
+#define WMD_RDF_NO_INTRAMOL 0x8u      /* wmd_rdf_create_groups: skip intramolecular group pairs */
 
 /* wmd_rdf_finalize flags (RdfPlan output options, rdf.rs:204-227) */
 #define WMD_RDF_USE_VOLUME 0x1u
 #define WMD_RDF_RAW 0x2u
 #define WMD_RDF_NUMBER_DENSITY 0x4u
 #define WMD_RDF_INTEGRAL 0x8u
+#define WMD_RDF_FINALIZE_XY 0x10u /* dimension "xy": shells PI (r_o^2 - r_i^2) */
 
 typedef struct wmd_ctx wmd_ctx;
 typedef struct wmd_rdf wmd_rdf;
@@ -140,11 +143,34 @@ wmd_status wmd_rdf_set_tuning(wmd_rdf *rdf, int cell_k, int block_threads);
 
 /* RdfPlan::finalize normalisation (rdf.rs:943-1016) on host data: r, g [bins]
  * and, with WMD_RDF_INTEGRAL, integral [bins] (else may be NULL).  flags:
- * WMD_RDF_USE_VOLUME / RAW / NUMBER_DENSITY / INTEGRAL.  Returns
+ * WMD_RDF_USE_VOLUME / RAW / NUMBER_DENSITY / INTEGRAL / FINALIZE_XY.  Returns
  * WMD_ERR_MISMATCH for USE_VOLUME with volume_sum <= 0 (rdf.rs:962-966). */
 wmd_status wmd_rdf_finalize(const uint64_t *counts, size_t bins, float r_max, double n_a,
                             double n_b, double common, size_t frames, double volume_sum,
                             double density, uint32_t flags, float *r, float *g, float *integral);
+
+/* RdfPlan's grouped modes (rdf.rs:279-376, 808-850; CPU-only in the
+ * reference, rdf.rs:697-699): counts over pairs of group positions.
+ * The caller builds the groups as rdf_groups does (centre: the whole
+ * selection; bymol / byres: one group per molecule / residue; else one per
+ * atom) and passes their union, identical atom lists merged:
+ * group_offsets [n_groups + 1] into group_atoms (atom indices of each group,
+ * in order), group_masses NULL (plain centres) or parallel to group_atoms
+ * (mass-weighted centres); groups_a / groups_b: the group ids of each side
+ * (a group paired with itself is skipped, rdf.rs:819-821).  With
+ * WMD_RDF_NO_INTRAMOL in flags, mol_key / res_key [2 n_groups] (INT64_MIN =
+ * none; [g] the keys of group g as a sel_a group, [n_groups + g] as a sel_b
+ * group) give same_intramolecular_group (rdf.rs:368-373) and those pairs are
+ * skipped.  Centres follow center_of_selection (traj-core/src/centers.rs:3-29)
+ * in f64 cast to f32; pair distances, wrap and binning are the plain RDF's, so
+ * counts are bit-exact.  WMD_RDF_PLANAR_XY selects the xy dimension.  The
+ * handle takes full chunks (wmd_rdf_accumulate) and is read, reset and
+ * destroyed with the wmd_rdf_* calls; pbc: none or orthorhombic. */
+wmd_status wmd_rdf_create_groups(wmd_ctx *ctx, size_t n_groups, const uint64_t *group_offsets,
+                                 const uint32_t *group_atoms, const float *group_masses,
+                                 const uint32_t *groups_a, size_t n_a, const uint32_t *groups_b,
+                                 size_t n_b, const int64_t *mol_key, const int64_t *res_key,
+                                 size_t bins, float r_max, int pbc, uint32_t flags, wmd_rdf **out);
 
 /* StructureFactorPlan's pair counts (crates/traj-engine/src/plans/analysis/
  * structure_factor.rs:160-198; its GPU branch reuses rdf_accum, :128-149): an

@@ -18,6 +18,7 @@ The C file restates warp-md's CPU semantics (citations in ``wmd_oracle.c``):
   ``crates/traj-engine/src/plans/analysis/structure_factor.rs:160-258``
 * ``pairwise_rmsd``   -- ``PairwiseRmsdPlan::finalize`` (rms / nofit),
   ``rmsd/plan.rs:586-675, 1081-1159``
+* ``rdf_grouped``     -- ``RdfPlan`` grouped modes and the xy dimension, ``rdf.rs:808-850``
 * ``align``           -- ``AlignPlan`` / ``SuperposePlan`` CPU path,
   ``plans/geometry/align/plan_exec.rs:215-243, 422-456``
 
@@ -90,6 +91,9 @@ def lib():
     L.orc_pairwise_rmsd.restype = ctypes.c_int
     L.orc_align.argtypes = [P, S, S, S, P, S, P, P, S, ctypes.c_int, P, P, P]
     L.orc_align.restype = ctypes.c_int
+    L.orc_rdf_grouped.argtypes = [P, S, S, S, P, P, S, P, P, P, P, S, P, P, P, ctypes.c_int,
+                                  ctypes.c_int, P, ctypes.c_int, S, ctypes.c_float, P]
+    L.orc_rdf_grouped.restype = ctypes.c_int
     L.orc_omp_max_threads.argtypes = []
     L.orc_omp_max_threads.restype = ctypes.c_int
     _lib = L
@@ -260,6 +264,41 @@ def pairwise_rmsd(coords, sel, metric="rms"):
     if st != 0:
         raise RuntimeError(f"oracle pairwise rmsd failed {st}")
     return out, out64
+
+
+def rdf_grouped(coords, groups_a, groups_b, bins, r_max, pbc="none", boxes=None, masses=None,
+                no_intramol=False, planar=False):
+    """RdfPlan grouped / xy counts.  groups: lists of (atom tuple, residue key or
+    None, molecule key or None)."""
+    L = lib()
+    c = _coords(coords)
+    F, N, stride = c.shape
+    rid = {}
+    none = -(2 ** 63)
+
+    def pack(groups):
+        off = np.zeros(len(groups) + 1, np.uint64)
+        off[1:] = np.cumsum([len(g[0]) for g in groups])
+        atoms = np.array([a for g in groups for a in g[0]], np.uint32)
+        if atoms.size == 0:
+            atoms = np.zeros(1, np.uint32)
+        mol = np.array([none if g[2] is None else int(g[2]) for g in groups] or [none], np.int64)
+        res = np.array([none if g[1] is None else rid.setdefault(g[1], len(rid)) for g in groups]
+                       or [none], np.int64)
+        return off, atoms, mol, res
+
+    oa, aa, ma, ra = pack(groups_a)
+    ob, ab, mb, rb = pack(groups_b)
+    bx = _boxes(boxes, F, PBC_KINDS[pbc])
+    m = None if masses is None else np.ascontiguousarray(masses, np.float32)
+    counts = np.zeros(bins, np.uint64)
+    st = L.orc_rdf_grouped(_ptr(c), stride, N, F, _ptr(oa), _ptr(aa), len(groups_a), _ptr(ma), _ptr(ra),
+                           _ptr(ob), _ptr(ab), len(groups_b), _ptr(mb), _ptr(rb), _ptr(m),
+                           int(bool(no_intramol)), PBC_KINDS[pbc], _ptr(bx), int(bool(planar)), bins,
+                           ctypes.c_float(r_max), _ptr(counts))
+    if st != 0:
+        raise RuntimeError(f"oracle rdf_grouped failed {st}")
+    return counts
 
 
 def align(coords, sel, ref_all, weights=None, norotate=False, superpose=False):

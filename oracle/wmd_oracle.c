@@ -29,6 +29,10 @@
  *   orc_pairwise_rmsd    pairwise_rmsd_plan_basic (geometry_matrix_tests.rs:1438-1460),
  *                        equal to orc_rmsd_plan per pair where the two coincide, numpy.linalg.svd
  *                        Kabsch, rigid-motion zero, symmetry, triangle inequality (nofit).
+ *   orc_rdf_grouped      the reference's grouped / xy known answers
+ *                        (correlation_structure_tests.rs:1979-2008, 2455-2588);
+ *                        equal to orc_rdf_brute for one-atom groups without exclusions;
+ *                        a pure-Python restatement on small inputs.
  *   orc_align            superpose_plan_basic (analysis_smoke_tests.rs:171-195),
  *                        test_superpose.py:41-67 (rotation + translation recovered),
  *                        numpy.linalg.svd weighted Kabsch restated independently.
@@ -916,6 +920,99 @@ int orc_align(const float *coords, size_t stride, size_t n_atoms, size_t n_frame
                         (float)(r[a][0] * x + r[a][1] * y + r[a][2] * z + t[a]);
             }
     }
+    return ORC_OK;
+}
+
+/* group_position (rdf.rs:353-366) over center_of_selection
+ * (traj-core/src/centers.rs:3-29): f32 position of one group in one frame. */
+static void group_position(const float *frame, size_t stride, const uint32_t *atoms, size_t n,
+                           const float *masses, float out[3]) {
+    out[0] = out[1] = out[2] = 0.0f;
+    if (n == 0) return;
+    if (n == 1) {
+        const float *p = frame + (size_t)atoms[0] * stride;
+        out[0] = p[0]; out[1] = p[1]; out[2] = p[2];
+        return;
+    }
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, ms = 0.0;
+    for (size_t i = 0; i < n; ++i) {
+        const float *p = frame + (size_t)atoms[i] * stride;
+        double m = masses ? (double)masses[atoms[i]] : 1.0;
+        s0 += (double)p[0] * m;
+        s1 += (double)p[1] * m;
+        s2 += (double)p[2] * m;
+        ms += m;
+    }
+    if (ms == 0.0) return;
+    out[0] = (float)(s0 / ms);
+    out[1] = (float)(s1 / ms);
+    out[2] = (float)(s2 / ms);
+}
+
+/*
+ * RdfPlan grouped modes and the xy dimension (rdf.rs:808-850, rdf_distance
+ * :473-478, same_intramolecular_group :368-373).  Groups are CSR lists of
+ * atom indices (off [n + 1], atoms); mol / res: per group key or INT64_MIN
+ * (none).  masses: per atom (mass-weighted centres) or NULL.  Pairs of groups
+ * with identical atom lists are skipped, and with no_intramol the
+ * intramolecular ones; the distance, wrap and binning are the atom loop's.
+ */
+int orc_rdf_grouped(const float *coords, size_t stride, size_t n_atoms, size_t n_frames,
+                    const uint64_t *off_a, const uint32_t *atoms_a, size_t n_ga,
+                    const int64_t *mol_a, const int64_t *res_a, const uint64_t *off_b,
+                    const uint32_t *atoms_b, size_t n_gb, const int64_t *mol_b,
+                    const int64_t *res_b, const float *masses, int no_intramol, int pbc,
+                    const float *boxes, int planar, size_t bins, float r_max, uint64_t *counts) {
+    const int64_t none = INT64_MIN;
+    if (pbc == PBC_TRICLINIC) return ORC_ERR_MISMATCH;
+    for (size_t i = 0; i < off_a[n_ga]; ++i)
+        if (atoms_a[i] >= n_atoms) return ORC_ERR_INVALID;
+    for (size_t i = 0; i < off_b[n_gb]; ++i)
+        if (atoms_b[i] >= n_atoms) return ORC_ERR_INVALID;
+    const float bin_width = r_max / (float)bins;
+    float *pa = (float *)malloc((n_ga + 1) * 3 * sizeof(float));
+    float *pb = (float *)malloc((n_gb + 1) * 3 * sizeof(float));
+    if (!pa || !pb) { free(pa); free(pb); return ORC_ERR_INVALID; }
+    for (size_t f = 0; f < n_frames; ++f) {
+        const float *frame = coords + f * n_atoms * stride;
+        float lx = 0.f, ly = 0.f, lz = 0.f;
+        if (pbc == PBC_ORTHO) {
+            const float *m = boxes + 9 * f;
+            lx = m[0]; ly = m[4]; lz = m[8];
+        }
+        for (size_t g = 0; g < n_ga; ++g)
+            group_position(frame, stride, atoms_a + off_a[g], off_a[g + 1] - off_a[g], masses, pa + 3 * g);
+        for (size_t g = 0; g < n_gb; ++g)
+            group_position(frame, stride, atoms_b + off_b[g], off_b[g + 1] - off_b[g], masses, pb + 3 * g);
+        for (size_t a = 0; a < n_ga; ++a) {
+            for (size_t b = 0; b < n_gb; ++b) {
+                size_t la = off_a[a + 1] - off_a[a], lb = off_b[b + 1] - off_b[b];
+                if (la == lb && memcmp(atoms_a + off_a[a], atoms_b + off_b[b], la * sizeof(uint32_t)) == 0)
+                    continue;
+                if (no_intramol) {
+                    int same;
+                    if (mol_a[a] != none && mol_b[b] != none) same = mol_a[a] == mol_b[b];
+                    else same = res_a[a] != none && res_b[b] != none && res_a[a] == res_b[b];
+                    if (same) continue;
+                }
+                float dx = pb[3 * b] - pa[3 * a];
+                float dy = pb[3 * b + 1] - pa[3 * a + 1];
+                float dz = pb[3 * b + 2] - pa[3 * a + 2];
+                if (pbc == PBC_ORTHO) {
+                    dx -= roundf(dx / lx) * lx;
+                    dy -= roundf(dy / ly) * ly;
+                    dz -= roundf(dz / lz) * lz;
+                }
+                float r = planar ? sqrtf(dx * dx + dy * dy) : sqrtf(dx * dx + dy * dy + dz * dz);
+                if (r < r_max) {
+                    size_t bin = (size_t)(r / bin_width);
+                    if (bin < bins) counts[bin] += 1;
+                }
+            }
+        }
+    }
+    free(pa);
+    free(pb);
     return ORC_OK;
 }
 

@@ -141,8 +141,8 @@ def test_plan_argument_errors():
     tr2 = Trajectory.from_numpy(np.zeros((1, 2, 3), np.float32))
     with pytest.raises(RuntimeError, match="unsupported"):
         RdfPlan([0], [1], bins=2, r_max=1.0, pbc="none").run(tr2, sysm, device="cpu")
-    with pytest.raises(RuntimeError, match="unsupported: grouped"):
-        RdfPlan([0], [1], bins=2, r_max=1.0, pbc="none", center1=True).run(tr2, sysm)
+    with pytest.raises(RuntimeError, match="unsupported: grouped and xy"):
+        RdfPlan([0], [1], bins=2, r_max=1.0, pbc="triclinic", center1=True).run(tr2, sysm)
 
 
 def test_system_selection_subset():

@@ -321,6 +321,7 @@ wmd_status wmd_rdf_finalize(const uint64_t *counts, size_t bins, float r_max, do
     const bool use_volume = flags & WMD_RDF_USE_VOLUME;
     const bool raw = flags & WMD_RDF_RAW;
     const bool nd_out = flags & WMD_RDF_NUMBER_DENSITY;
+    const bool planar = flags & WMD_RDF_FINALIZE_XY;
     // Arithmetic below is written so that each f32 operation rounds once, in
     // the reference's order (the shell volume is an f32 expression cast to f64).
     volatile float bin_width = r_max / (float)bins;
@@ -350,6 +351,11 @@ wmd_status wmd_rdf_finalize(const uint64_t *counts, size_t bins, float r_max, do
         volatile float diff = ro3 - ri3;
         volatile float t = four_thirds * pi32;
         volatile float shell32 = t * diff;
+        if (planar) {
+            // rdf_shell_measure, PlanarXY (rdf.rs:487-496): PI (r_o^2 - r_i^2) in f32
+            volatile float d2 = ro2 - ri2;
+            shell32 = pi32 * d2;
+        }
         const double shell = (double)shell32;
         const double expected = shell * density;
         const double norm = expected > 0.0 ? frames * expected : 0.0;

@@ -609,11 +609,13 @@ __global__ void soft_slope_kernel(const double *__restrict__ D, int tn, double i
     table[e] = make_float2((float)((dp - dm) * inv_h), (float)(2.0 * (dm - 2.0 * D[e] + dp) * inv_h));
 }
 
-// AoS [F][n_src][4] -> SoA [F][3][stride] through a gather list (kPad -> 0).
+// AoS [F][n_src][4] -> SoA [F][3][stride] through a gather list (kPad -> 0);
+// planar: z = 0, so s = dx^2 + dy^2 (+ 0) is the PlanarXY distance
+// (rdf_distance, rdf.rs:473-478) bit for bit.
 __global__ __launch_bounds__(kBlock) void gather_soa_kernel(const float4 *__restrict__ aos,
                                                             long long n_src, const unsigned *__restrict__ idx,
                                                             int n_store, int stride, int n_frames,
-                                                            float *__restrict__ soa) {
+                                                            float *__restrict__ soa, int planar = 0) {
     long long total = (long long)stride * n_frames;
     for (long long t = (long long)blockIdx.x * kBlock + threadIdx.x; t < total;
          t += (long long)gridDim.x * kBlock) {
@@ -624,13 +626,113 @@ __global__ __launch_bounds__(kBlock) void gather_soa_kernel(const float4 *__rest
             unsigned a = idx ? idx[k] : (unsigned)k;
             if (a != kPad) {
                 float4 v = aos[(long long)f * n_src + a];
-                x = v.x; y = v.y; z = v.z;
+                x = v.x; y = v.y; z = planar ? 0.f : v.z;
             }
         }
         float *o = soa + (long long)f * 3 * stride;
         o[k] = x;
         o[stride + k] = y;
         o[2 * stride + k] = z;
+    }
+}
+
+// Group centres of the grouped RDF modes (group_position, rdf.rs:353-366, over
+// center_of_selection, traj-core/src/centers.rs:3-29): per frame and IO group,
+// f64 sums of x * m over the group's atoms in order divided by the summed m
+// (m = 1 unless mass weighted) and cast to f32; a one-atom group is the atom
+// itself, an empty group or a zero mass sum the origin.  planar: z = 0.
+__global__ __launch_bounds__(kBlock) void group_center_kernel(
+    const float4 *__restrict__ aos, long long n_src, const unsigned long long *__restrict__ goff,
+    const unsigned *__restrict__ gatoms, const float *__restrict__ gmass, int n_groups, int stride,
+    int n_frames, int planar, float *__restrict__ soa) {
+    const long long total = (long long)stride * n_frames;
+    for (long long t = (long long)blockIdx.x * kBlock + threadIdx.x; t < total;
+         t += (long long)gridDim.x * kBlock) {
+        const int f = (int)(t / stride);
+        const int k = (int)(t - (long long)f * stride);
+        float x = 0.f, y = 0.f, z = 0.f;
+        if (k < n_groups) {
+            const float4 *fr = aos + (long long)f * n_src;
+            const unsigned long long b = goff[k], e = goff[k + 1];
+            if (e - b == 1) {
+                const float4 p = fr[gatoms[b]];
+                x = p.x; y = p.y; z = p.z;
+            } else if (e > b) {
+                double s0 = 0.0, s1 = 0.0, s2 = 0.0, ms = 0.0;
+                for (unsigned long long i = b; i < e; ++i) {
+                    const float4 p = fr[gatoms[i]];
+                    const double m = gmass ? (double)gmass[i] : 1.0;
+                    s0 += (double)p.x * m;
+                    s1 += (double)p.y * m;
+                    s2 += (double)p.z * m;
+                    ms += m;
+                }
+                if (ms != 0.0) {
+                    x = (float)(s0 / ms);
+                    y = (float)(s1 / ms);
+                    z = (float)(s2 / ms);
+                }
+            }
+        }
+        if (planar) z = 0.f;
+        float *o = soa + (long long)f * 3 * stride;
+        o[k] = x;
+        o[stride + k] = y;
+        o[2 * stride + k] = z;
+    }
+}
+
+// Copy of a SoA store with z = 0 (PlanarXY on a caller's device store).
+__global__ __launch_bounds__(kBlock) void planar_copy_kernel(const float *__restrict__ src,
+                                                             float *__restrict__ dst, long long stride,
+                                                             long long n_frames) {
+    for (long long t = (long long)blockIdx.x * kBlock + threadIdx.x; t < stride * n_frames;
+         t += (long long)gridDim.x * kBlock) {
+        const long long f = t / stride, k = t - f * stride;
+        const long long o = 3 * f * stride + k;
+        dst[o] = src[o];
+        dst[o + stride] = src[o + stride];
+        dst[o + 2 * stride] = 0.f;
+    }
+}
+
+// Removes the intramolecular pairs of no_intramol (rdf.rs:822-824,
+// same_intramolecular_group :368-373) from the counts the pair kernels made:
+// each excluded ordered pair (u from A, v from B, weight mA(u) mB(v)) is
+// evaluated with the reference's own expression (f32 difference, dx -=
+// round(dx / l) l, s = dx^2 + dy^2 + dz^2, no contraction) and its weight
+// subtracted from the bin the exact breakpoints thr give.
+template <int PBC>
+__global__ __launch_bounds__(kBlock) void rdf_excl_kernel(const float *__restrict__ soa, int stride,
+                                                          int n_frames, const FrameBox *__restrict__ boxes,
+                                                          const uint4 *__restrict__ excl, long long n_excl,
+                                                          const float *__restrict__ thr, int bins,
+                                                          unsigned long long *counts) {
+    const long long total = n_excl * n_frames;
+    for (long long t = (long long)blockIdx.x * kBlock + threadIdx.x; t < total;
+         t += (long long)gridDim.x * kBlock) {
+        const int f = (int)(t / n_excl);
+        const uint4 e = excl[t - (long long)f * n_excl];
+        const float *x = soa + (long long)f * 3 * stride;
+        float dx = x[e.y] - x[e.x];
+        float dy = x[stride + e.y] - x[stride + e.x];
+        float dz = x[2 * stride + e.y] - x[2 * stride + e.x];
+        if constexpr (PBC == WMD_PBC_ORTHO) {
+            const FrameBox b = boxes[f];
+            dx -= roundf(dx / b.lx) * b.lx;
+            dy -= roundf(dy / b.ly) * b.ly;
+            dz -= roundf(dz / b.lz) * b.lz;
+        }
+        const float s = dx * dx + dy * dy + dz * dz;
+        if (s < thr[bins]) {
+            int lo = 0, hi = bins;  // largest k < bins with thr[k] <= s
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (thr[mid] <= s) lo = mid;
+                else hi = mid;
+            }
+            atomicAdd(&counts[lo], (unsigned long long)0 - (unsigned long long)e.z);
+        }
     }
 }
 
@@ -800,6 +902,13 @@ struct wmd_rdf {
     HostBuf p_stage[2];
     DevBuf p_aos[2], p_soa[2];
     Event p_copied[2], ev_boxes;
+    // grouped modes (centre / byres / bymol / no_intramol) and PlanarXY: the
+    // IO "atoms" are group centres computed from full chunks; exclusions are
+    // the no_intramol ordered pairs (u, v, weight) in IO positions
+    bool grouped = false, planar = false;
+    DevBuf d_goff, d_gatoms, d_gmass, d_excl, d_pstore;
+    long long n_excl = 0;
+    uint64_t g_max_atom = 0;  // chunks must hold at least this many atoms
     // f64 distance mode (StructureFactorPlan): thresholds on the f64 squared
     // distance, r_max and the length scale in f64
     bool f64 = false;
@@ -1049,8 +1158,13 @@ void accumulate_host(wmd_rdf *h, const float *coords, size_t n_src, size_t n_fra
                      const float *boxes, bool selected) {
     if (n_frames == 0 || h->bins == 0) return;
     WMD_REQUIRE(coords != nullptr, WMD_ERR_INVALID, "null coords");
+    WMD_REQUIRE(!(selected && h->grouped), WMD_ERR_UNSUPPORTED,
+                "grouped RDF modes take full chunks (wmd_rdf_accumulate)");
     const int n = (int)h->io.size();
-    if (!selected) {
+    if (h->grouped)
+        WMD_REQUIRE(h->g_max_atom <= n_src, WMD_ERR_MISMATCH,
+                    "group atom index out of range for " + std::to_string(n_src) + " atoms");
+    if (!selected && !h->grouped) {
         for (uint32_t a : h->io)
             WMD_REQUIRE(a < n_src, WMD_ERR_MISMATCH,
                         "selection index " + std::to_string(a) + " out of range for " +
@@ -1081,11 +1195,26 @@ void accumulate_host(wmd_rdf *h, const float *coords, size_t n_src, size_t n_fra
         float *d_soa = (float *)h->p_soa[b].ptr;
         int g;
         grid1d((long long)stride * nf, &g);
-        hipLaunchKernelGGL(gather_soa_kernel, dim3(g), dim3(kBlock), 0, st, (const float4 *)h->p_aos[b].ptr,
-                           (long long)n_src, selected ? nullptr : h->d_io.as<unsigned>(), n,
-                           (int)stride, (int)nf, d_soa);
+        if (h->grouped) {
+            hipLaunchKernelGGL(group_center_kernel, dim3(g), dim3(kBlock), 0, st,
+                               (const float4 *)h->p_aos[b].ptr, (long long)n_src,
+                               h->d_goff.as<unsigned long long>(), h->d_gatoms.as<unsigned>(),
+                               h->d_gmass.as<float>(), n, (int)stride, (int)nf, (int)h->planar, d_soa);
+        } else {
+            hipLaunchKernelGGL(gather_soa_kernel, dim3(g), dim3(kBlock), 0, st, (const float4 *)h->p_aos[b].ptr,
+                               (long long)n_src, selected ? nullptr : h->d_io.as<unsigned>(), n,
+                               (int)stride, (int)nf, d_soa, (int)h->planar);
+        }
         WMD_HIP_CHECK(hipGetLastError());
         accumulate_device(h, d_soa, stride, nf, boxes ? boxes + 9 * f0 : nullptr, st, true);
+        if (h->n_excl > 0) {
+            grid1d(h->n_excl * (long long)nf, &g);
+            auto kern = h->pbc == WMD_PBC_ORTHO ? rdf_excl_kernel<WMD_PBC_ORTHO> : rdf_excl_kernel<WMD_PBC_NONE>;
+            hipLaunchKernelGGL(kern, dim3(g), dim3(kBlock), 0, st, (const float *)d_soa, (int)stride, (int)nf,
+                               h->d_boxes.as<FrameBox>(), h->d_excl.as<uint4>(), h->n_excl,
+                               h->d_thr.as<float>(), h->bins, h->d_counts.as<unsigned long long>());
+            WMD_HIP_CHECK(hipGetLastError());
+        }
     }
     WMD_HIP_CHECK(hipStreamSynchronize(st));
 }
@@ -1113,6 +1242,9 @@ wmd_status wmd_rdf_create(wmd_ctx *ctx, const uint32_t *sel_a, size_t n_a, const
         h->r_max = r_max;
         h->pbc = pbc;
         h->flags = flags;
+        h->planar = (flags & WMD_RDF_PLANAR_XY) != 0;
+        WMD_REQUIRE(!h->planar || pbc != WMD_PBC_TRICLINIC, WMD_ERR_UNSUPPORTED,
+                    "the xy dimension supports pbc none or orthorhombic");
         // rdf.rs:498-526: order-preserving union, sel_a first
         std::vector<uint16_t> ma, mb;
         {
@@ -1251,6 +1383,108 @@ wmd_status wmd_rdf_create_f64(wmd_ctx *ctx, const uint32_t *sel, size_t n_sel, s
     WMD_API_END
 }
 
+wmd_status wmd_rdf_create_groups(wmd_ctx *ctx, size_t n_groups, const uint64_t *group_offsets,
+                                 const uint32_t *group_atoms, const float *group_masses,
+                                 const uint32_t *groups_a, size_t n_a, const uint32_t *groups_b,
+                                 size_t n_b, const int64_t *mol_key, const int64_t *res_key,
+                                 size_t bins, float r_max, int pbc, uint32_t flags, wmd_rdf **out) {
+    WMD_API_BEGIN
+    WMD_REQUIRE(ctx && out, WMD_ERR_INVALID, "null context or output handle");
+    *out = nullptr;
+    WMD_REQUIRE(group_offsets && (n_groups == 0 || group_offsets[n_groups] == 0 || group_atoms),
+                WMD_ERR_INVALID, "null group layout");
+    WMD_REQUIRE(pbc == WMD_PBC_NONE || pbc == WMD_PBC_ORTHO, WMD_ERR_UNSUPPORTED,
+                "grouped RDF modes support pbc none or orthorhombic");
+    WMD_REQUIRE(!(flags & WMD_RDF_EDGE_DIAGNOSTICS), WMD_ERR_UNSUPPORTED,
+                "edge diagnostics are not available in grouped modes");
+    for (size_t g = 0; g < n_groups; ++g)
+        WMD_REQUIRE(group_offsets[g] <= group_offsets[g + 1], WMD_ERR_INVALID, "group offsets must not decrease");
+    for (size_t i = 0; i < n_a; ++i) WMD_REQUIRE(groups_a[i] < n_groups, WMD_ERR_INVALID, "group id out of range");
+    for (size_t i = 0; i < n_b; ++i) WMD_REQUIRE(groups_b[i] < n_groups, WMD_ERR_INVALID, "group id out of range");
+    const bool no_intramol = (flags & WMD_RDF_NO_INTRAMOL) != 0;
+    WMD_REQUIRE(!no_intramol || (mol_key && res_key), WMD_ERR_INVALID, "no_intramol needs group keys");
+    wmd_rdf *h = nullptr;
+    const wmd_status st0 = wmd_rdf_create(ctx, groups_a, n_a, groups_b, n_b, bins, r_max, pbc,
+                                          flags & ~WMD_RDF_NO_INTRAMOL, &h);
+    if (st0 != WMD_OK) return st0;
+    try {
+        h->grouped = true;
+        // the IO "atoms" are the groups in IO order: their atom lists as CSR
+        const size_t nio = h->io.size();
+        std::vector<unsigned long long> off(nio + 1, 0);
+        std::vector<uint32_t> atoms;
+        std::vector<float> masses;
+        for (size_t i = 0; i < nio; ++i) {
+            const uint32_t g = h->io[i];
+            for (uint64_t k = group_offsets[g]; k < group_offsets[g + 1]; ++k) {
+                atoms.push_back(group_atoms[k]);
+                if (group_masses) masses.push_back(group_masses[k]);
+            }
+            off[i + 1] = atoms.size();
+        }
+        h->d_goff.reserve(off.size() * 8);
+        WMD_HIP_CHECK(hipMemcpy(h->d_goff.ptr, off.data(), off.size() * 8, hipMemcpyHostToDevice));
+        h->d_gatoms.reserve(std::max<size_t>(atoms.size(), 1) * 4);
+        if (!atoms.empty())
+            WMD_HIP_CHECK(hipMemcpy(h->d_gatoms.ptr, atoms.data(), atoms.size() * 4, hipMemcpyHostToDevice));
+        if (group_masses) {
+            h->d_gmass.reserve(std::max<size_t>(masses.size(), 1) * 4);
+            if (!masses.empty())
+                WMD_HIP_CHECK(hipMemcpy(h->d_gmass.ptr, masses.data(), masses.size() * 4, hipMemcpyHostToDevice));
+        }
+        h->g_max_atom = 0;
+        for (uint32_t a : atoms) h->g_max_atom = std::max<uint64_t>(h->g_max_atom, (uint64_t)a + 1);
+        if (no_intramol) {
+            // same_intramolecular_group (rdf.rs:368-373) for u in its sel_a role and
+            // v in its sel_b role (keys [g] and [n_groups + g]): molecule keys equal
+            // when both have one, else residue keys equal when both have one
+            const int64_t kNone = INT64_MIN;
+            auto key = [&](const int64_t *k, uint32_t io_pos, int role) { return k[(size_t)role * n_groups + h->io[io_pos]]; };
+            std::unordered_map<int64_t, std::vector<uint32_t>> mol_a, mol_b, res_a, res_b;
+            for (uint32_t i = 0; i < (uint32_t)nio; ++i) {
+                if (h->wts[i] & 0xFFu) {
+                    if (key(mol_key, i, 0) != kNone) mol_a[key(mol_key, i, 0)].push_back(i);
+                    if (key(res_key, i, 0) != kNone) res_a[key(res_key, i, 0)].push_back(i);
+                }
+                if (h->wts[i] >> 8) {
+                    if (key(mol_key, i, 1) != kNone) mol_b[key(mol_key, i, 1)].push_back(i);
+                    if (key(res_key, i, 1) != kNone) res_b[key(res_key, i, 1)].push_back(i);
+                }
+            }
+            std::vector<uint4> ex;
+            auto emit = [&](uint32_t u, uint32_t v) {
+                const unsigned w = (h->wts[u] & 0xFFu) * (h->wts[v] >> 8);
+                if (u != v && w) ex.push_back(make_uint4(u, v, w, 0u));
+                WMD_REQUIRE(ex.size() <= (1ull << 27), WMD_ERR_UNSUPPORTED,
+                            "too many intramolecular pairs to exclude");
+            };
+            for (auto &kv : mol_a) {
+                auto it = mol_b.find(kv.first);
+                if (it == mol_b.end()) continue;
+                for (uint32_t u : kv.second)
+                    for (uint32_t v : it->second) emit(u, v);
+            }
+            for (auto &kv : res_a) {
+                auto it = res_b.find(kv.first);
+                if (it == res_b.end()) continue;
+                for (uint32_t u : kv.second)
+                    for (uint32_t v : it->second)
+                        if (!(key(mol_key, u, 0) != kNone && key(mol_key, v, 1) != kNone)) emit(u, v);
+            }
+            h->n_excl = (long long)ex.size();
+            if (!ex.empty()) {
+                h->d_excl.reserve(ex.size() * sizeof(uint4));
+                WMD_HIP_CHECK(hipMemcpy(h->d_excl.ptr, ex.data(), ex.size() * sizeof(uint4), hipMemcpyHostToDevice));
+            }
+        }
+    } catch (...) {
+        delete h;
+        throw;
+    }
+    *out = h;
+    WMD_API_END
+}
+
 wmd_status wmd_rdf_io_selection(const wmd_rdf *h, uint32_t *out, size_t cap, size_t *n) {
     WMD_API_BEGIN
     WMD_REQUIRE(h && n, WMD_ERR_INVALID, "null handle");
@@ -1285,8 +1519,22 @@ wmd_status wmd_rdf_accumulate_device(wmd_rdf *h, const float *d_soa, size_t stri
     WMD_API_BEGIN
     WMD_REQUIRE(h != nullptr, WMD_ERR_INVALID, "null handle");
     WMD_REQUIRE(d_soa != nullptr || n_frames == 0, WMD_ERR_INVALID, "null device store");
+    WMD_REQUIRE(!h->grouped, WMD_ERR_UNSUPPORTED, "grouped RDF modes take full chunks (wmd_rdf_accumulate)");
     bind(h->ctx);
     hipStream_t st = pick_stream(h->ctx, stream);
+    if (h->planar && n_frames > 0) {
+        // PlanarXY: the store's x and y planes with z = 0
+        WMD_REQUIRE(stride % 4 == 0, WMD_ERR_INVALID, "stride must be a multiple of 4");
+        const size_t fl = 3 * stride;
+        if (h->d_pstore.bytes < n_frames * fl * 4) WMD_HIP_CHECK(hipStreamSynchronize(st));
+        float *dst = (float *)h->d_pstore.reserve(n_frames * fl * 4);
+        int g;
+        grid1d((long long)n_frames * stride, &g);
+        hipLaunchKernelGGL(planar_copy_kernel, dim3(g), dim3(kBlock), 0, st, d_soa, dst, (long long)stride,
+                           (long long)n_frames);
+        WMD_HIP_CHECK(hipGetLastError());
+        d_soa = dst;
+    }
     accumulate_device(h, d_soa, stride, n_frames, boxes, st);
     WMD_API_END
 }

@@ -28,11 +28,14 @@ BATCH_RMSD = 0x2
 BATCH_NO_GRAPH = 0x4
 
 RDF_EDGE_DIAGNOSTICS = 0x1
+RDF_PLANAR_XY = 0x4
+RDF_NO_INTRAMOL = 0x8
 RDF_FORCE_BRUTE = 0x2
 RDF_USE_VOLUME = 0x1
 RDF_RAW = 0x2
 RDF_NUMBER_DENSITY = 0x4
 RDF_INTEGRAL = 0x8
+RDF_FINALIZE_XY = 0x10
 
 # TrajError Display prefixes (crates/traj-core/src/error.rs:13-24); the
 # reference surfaces every TrajError as RuntimeError(str(err)) (traj-py pack.rs:1188-1190).
@@ -64,7 +67,7 @@ SYMBOLS = (
     "wmd_pairwise_create", "wmd_pairwise_append", "wmd_pairwise_frames", "wmd_pairwise_finalize",
     "wmd_pairwise_store_stride", "wmd_pairwise_run_device", "wmd_pairwise_destroy",
     "wmd_align_create", "wmd_align_set_reference", "wmd_align_run", "wmd_align_run_device",
-    "wmd_align_destroy",
+    "wmd_align_destroy", "wmd_rdf_create_groups",
 )
 
 
@@ -179,6 +182,8 @@ def lib():
             "wmd_align_run": ([P, P, S, S, P, P], St),
             "wmd_align_run_device": ([P, P, S, S, P, P, P, P], St),
             "wmd_align_destroy": ([P], None),
+            "wmd_rdf_create_groups": ([P, S, P, P, P, P, S, P, S, P, P, S, ctypes.c_float, I,
+                                       ctypes.c_uint32, PP], St),
         }
         for name, (args, res) in sig.items():
             fn = getattr(L, name)

@@ -45,6 +45,81 @@ def _indices(sel) -> np.ndarray:
     return np.ascontiguousarray(np.asarray(sel, dtype=np.int64).astype(np.uint32))
 
 
+_NO_KEY = -(2 ** 63)  # INT64_MIN: no molecule / residue key
+
+
+def _residue_keys(system: System):
+    """residue_key (rdf.rs:264-277): (chain_id, resid, resname_id) per atom; the
+    resname id is the first-seen order of the upper-cased name (the interner)."""
+    names = {}
+    keys = []
+    for c, r, rn in zip(system.atoms["chain_id"], system.atoms["resid"], system.atoms["resname"]):
+        u = rn.upper()
+        if u not in names:
+            names[u] = len(names)
+        keys.append((int(c), int(r), names[u]))
+    return keys
+
+
+def rdf_groups(sel: np.ndarray, system: System, center: bool, byres: bool, bymol: bool):
+    """rdf_groups (rdf.rs:279-351): list of (atom index tuple, residue key or
+    None, molecule key or None)."""
+    idx = [int(i) for i in sel]
+    if center:
+        return [(tuple(idx), None, None)]
+    rkeys = _residue_keys(system)
+    if bymol:
+        grouped = {}
+        for i in idx:
+            m = system.molecule_id_for_atom(i)
+            if m is not None:
+                grouped.setdefault(m, []).append(i)
+        if grouped:
+            return [(tuple(grouped[k]), None, k) for k in sorted(grouped)]
+    if byres:
+        grouped = {}
+        for i in idx:
+            grouped.setdefault(rkeys[i], []).append(i)
+        if grouped:
+            return [(tuple(grouped[k]), k, None) for k in sorted(grouped)]
+    return [((i,), rkeys[i], system.molecule_id_for_atom(i)) for i in idx]
+
+
+def same_intramolecular_group(a, b) -> bool:
+    """rdf.rs:368-373."""
+    if a[2] is not None and b[2] is not None:
+        return a[2] == b[2]
+    return a[1] is not None and b[1] is not None and a[1] == b[1]
+
+
+def common_group_count(groups_a, groups_b, no_intramol: bool) -> int:
+    """rdf.rs:387-401."""
+    count = 0
+    if no_intramol:
+        mol_b, res_b = {}, {}
+        for g in groups_b:
+            if g[2] is not None:
+                mol_b[g[2]] = mol_b.get(g[2], 0) + 1
+            if g[1] is not None:
+                res_b[g[1]] = res_b.get(g[1], 0) + 1
+        res_b_mol = {}
+        for g in groups_b:
+            if g[1] is not None and g[2] is not None:
+                res_b_mol[g[1]] = res_b_mol.get(g[1], 0) + 1
+        for a in groups_a:
+            if a[2] is not None:
+                count += mol_b.get(a[2], 0)
+                if a[1] is not None:  # b without a molecule key falls back to residues
+                    count += res_b.get(a[1], 0) - res_b_mol.get(a[1], 0)
+            elif a[1] is not None:
+                count += res_b.get(a[1], 0)
+        return count
+    seen = {}
+    for g in groups_b:
+        seen[g[0]] = seen.get(g[0], 0) + 1
+    return int(sum(seen.get(g[0], 0) for g in groups_a))
+
+
 class RdfPlan:
     """Pair-distance histogram g(r) between two selections."""
 
@@ -62,6 +137,8 @@ class RdfPlan:
             raise OverflowError("bins must be non-negative")
         self.r_max = float(np.float32(r_max))
         self.grouped = any([center1, center2, byres1, byres2, bymol1, bymol2, no_intramol])
+        self.modes = (bool(center1), bool(center2), bool(byres1), bool(byres2), bool(bymol1),
+                      bool(bymol2), bool(no_intramol))
         self.mass_weighted = bool(mass_weighted)
         self.density = float(density)
         self.use_volume = bool(volume)
@@ -84,25 +161,38 @@ class RdfPlan:
 
     def run(self, traj: Trajectory, system: System, chunk_frames: Optional[int] = None,
             device: str = "auto", frame_indices: Optional[Sequence[int]] = None):
+        planar = self.dimension == "xy"
+        if (self.grouped or planar) and self.pbc == _native.PBC_TRICLINIC:
+            raise RuntimeError("unsupported: grouped and xy RDF modes need pbc none or orthorhombic")
         dev = Device.from_spec(device)
         if traj.n_atoms() != system.n_atoms():
             raise RuntimeError("mismatch: trajectory atom count does not match system")
         # rdf.rs:663-667
         if not np.isfinite(self.density) or self.density <= 0.0:
             raise RuntimeError("parse error: rdf density must be finite and positive")
-        if self.grouped or self.dimension == "xy":
-            # rdf.rs:697-699 keeps grouped and XY modes on the CPU path
-            raise RuntimeError("unsupported: grouped and XY RDF modes run on warp-md's CPU path")
         ctx = dev.context()
         n_atoms = system.n_atoms()
         for sel in (self.sel_a, self.sel_b):
             if sel.size and int(sel.max()) >= n_atoms:
                 raise RuntimeError("invalid selection: atom index out of range")
         flags = _native.RDF_EDGE_DIAGNOSTICS if self.edge_diagnostics else 0
+        if planar:
+            flags |= _native.RDF_PLANAR_XY
         h = ctypes.c_void_p()
-        check(lib().wmd_rdf_create(ctx.handle, ptr(self.sel_a), self.sel_a.size, ptr(self.sel_b),
-                                   self.sel_b.size, self.bins, ctypes.c_float(self.r_max),
-                                   self.pbc, flags, ctypes.byref(h)))
+        n_a, n_b, common = float(self.sel_a.size), float(self.sel_b.size), None
+        if self.grouped:
+            if self.edge_diagnostics:
+                raise RuntimeError("unsupported: edge diagnostics in grouped RDF modes")
+            c1, c2, r1, r2, m1, m2, nointra = self.modes
+            ga = rdf_groups(self.sel_a, system, c1, r1, m1)
+            gb = rdf_groups(self.sel_b, system, c2, r2, m2)
+            n_a, n_b = float(len(ga)), float(len(gb))
+            common = float(common_group_count(ga, gb, nointra))
+            self._create_grouped(ctx, system, ga, gb, nointra, flags, h)
+        else:
+            check(lib().wmd_rdf_create(ctx.handle, ptr(self.sel_a), self.sel_a.size, ptr(self.sel_b),
+                                       self.sel_b.size, self.bins, ctypes.c_float(self.r_max),
+                                       self.pbc, flags, ctypes.byref(h)))
         try:
             if self._cell_k:
                 check(lib().wmd_rdf_set_tuning(h, self._cell_k, 0))
@@ -130,7 +220,10 @@ class RdfPlan:
                             if np.any(off != 0) or np.any(m[[0, 4, 8]] <= 0):
                                 raise RuntimeError("mismatch: RDF requires orthorhombic box for PBC or volume normalization")
                             # rdf_box_measure (rdf.rs:480-485): f64 product of the f32 lengths
-                            volume_sum += float(m[0]) * float(m[4]) * float(m[8])
+                            if planar:
+                                volume_sum += float(m[0]) * float(m[4])
+                            else:
+                                volume_sum += float(m[0]) * float(m[4]) * float(m[8])
                 frames += nf
                 # the full chunk; the engine gathers the IO selection on the device
                 check(lib().wmd_rdf_accumulate(h, ptr(coords), n_atoms, nf, ptr(boxes)))
@@ -143,13 +236,54 @@ class RdfPlan:
             self.last_edge_pairs = int(edge[0])
         finally:
             lib().wmd_rdf_destroy(h)
-        return finalize_rdf(counts, self.r_max, float(self.sel_a.size), float(self.sel_b.size),
-                            float(self._common_count()), frames, volume_sum, self.density,
-                            self.use_volume, self.raw, self.number_density, self.intrdf)
+        if common is None:
+            common = float(self._common_count())
+        return finalize_rdf(counts, self.r_max, n_a, n_b, common, frames, volume_sum, self.density,
+                            self.use_volume, self.raw, self.number_density, self.intrdf,
+                            planar=planar)
+
+    def _create_grouped(self, ctx, system, ga, gb, no_intramol, flags, h):
+        """Union of the two group lists (identical atom lists merged,
+        rdf.rs:819-821) as CSR, per-role molecule / residue keys, and
+        wmd_rdf_create_groups."""
+        ids, members = {}, []
+        for g in ga + gb:
+            if g[0] not in ids:
+                ids[g[0]] = len(members)
+                members.append(g[0])
+        n_groups = len(members)
+        offsets = np.zeros(n_groups + 1, np.uint64)
+        offsets[1:] = np.cumsum([len(m) for m in members])
+        atoms = np.ascontiguousarray(np.fromiter((a for m in members for a in m), np.uint32,
+                                                 count=int(offsets[-1])))
+        masses = None
+        if self.mass_weighted:
+            mass = np.asarray(system.atoms["mass"], np.float32)
+            masses = np.ascontiguousarray(mass[atoms.astype(np.int64)]) if atoms.size else np.zeros(1, np.float32)
+        ida = np.ascontiguousarray([ids[g[0]] for g in ga], np.uint32)
+        idb = np.ascontiguousarray([ids[g[0]] for g in gb], np.uint32)
+        mol = res = None
+        if no_intramol:
+            # keys per role: [0, n) for groups in their sel_a role, [n, 2n) in their sel_b role
+            mol = np.full(2 * n_groups, _NO_KEY, np.int64)
+            res = np.full(2 * n_groups, _NO_KEY, np.int64)
+            rid = {}
+            for role, groups in ((0, ga), (1, gb)):
+                for g in groups:
+                    k = role * n_groups + ids[g[0]]
+                    if g[2] is not None:
+                        mol[k] = int(g[2])
+                    if g[1] is not None:
+                        res[k] = rid.setdefault(g[1], len(rid))
+            flags |= _native.RDF_NO_INTRAMOL
+        check(lib().wmd_rdf_create_groups(ctx.handle, n_groups, ptr(offsets), ptr(atoms), ptr(masses),
+                                          ptr(ida), ida.size, ptr(idb), idb.size, ptr(mol), ptr(res),
+                                          self.bins, ctypes.c_float(self.r_max), self.pbc, flags,
+                                          ctypes.byref(h)))
 
 
 def finalize_rdf(counts, r_max, n_a, n_b, common, frames, volume_sum, density, use_volume,
-                 raw, number_density, intrdf):
+                 raw, number_density, intrdf, planar=False):
     """RdfPlan::finalize (rdf.rs:943-1016) through the engine's host routine."""
     counts = np.ascontiguousarray(counts, dtype=np.uint64)
     bins = counts.size
@@ -158,7 +292,8 @@ def finalize_rdf(counts, r_max, n_a, n_b, common, frames, volume_sum, density, u
     it = np.zeros(bins, np.float32)
     flags = ((_native.RDF_USE_VOLUME if use_volume else 0) | (_native.RDF_RAW if raw else 0)
              | (_native.RDF_NUMBER_DENSITY if number_density else 0)
-             | (_native.RDF_INTEGRAL if intrdf else 0))
+             | (_native.RDF_INTEGRAL if intrdf else 0)
+             | (_native.RDF_FINALIZE_XY if planar else 0))
     if bins:
         check(lib().wmd_rdf_finalize(ptr(counts), bins, ctypes.c_float(r_max), n_a, n_b, common,
                                      int(frames), float(volume_sum), float(density), flags,

@@ -29,6 +29,31 @@ class Selection:
         return f"Selection({self.expr!r}, n={len(self)})"
 
 
+def _molecule_ids_from_bonds(n: int, bonds) -> list:
+    parent = list(range(n))
+
+    def find(x):
+        while parent[x] != x:
+            parent[x] = parent[parent[x]]
+            x = parent[x]
+        return x
+
+    for a, b in bonds:
+        a, b = int(a), int(b)
+        if not (0 <= a < n and 0 <= b < n):
+            raise RuntimeError("invalid bond: atom index out of range")
+        ra, rb = find(a), find(b)
+        if ra != rb:
+            parent[rb] = ra
+    ids, out = {}, []
+    for atom in range(n):
+        root = find(atom)
+        if root not in ids:
+            ids[root] = len(ids) + 1
+        out.append(ids[root])
+    return out
+
+
 class System:
     """Atom table plus optional topology coordinates ``positions0``."""
 
@@ -45,6 +70,19 @@ class System:
         for key, col in self.atoms.items():
             if len(col) != n:
                 raise RuntimeError(f"mismatch: atom column '{key}' has {len(col)} entries, expected {n}")
+        # molecule ids (traj-core/src/system.rs:73-91): given per atom, else the
+        # connected components of the bonds numbered from 1 in atom order
+        # (molecule_ids_from_bonds, :226-247), else none
+        mol = atoms.get("molecule_id")
+        bonds = atoms.get("bonds")
+        if mol is not None and len(mol):
+            if len(mol) != n:
+                raise RuntimeError("mismatch: atom_table.molecule_id must match atom count")
+            self.molecule_id = [int(x) for x in mol]
+        elif bonds is not None and len(bonds) and n:
+            self.molecule_id = _molecule_ids_from_bonds(n, bonds)
+        else:
+            self.molecule_id = []
         self.positions0 = None
         if positions0 is not None:
             p = np.asarray(positions0, dtype=np.float32)
@@ -58,6 +96,10 @@ class System:
 
     def n_atoms(self) -> int:
         return len(self.atoms["name"])
+
+    def molecule_id_for_atom(self, atom: int):
+        """system.rs:96-98: the atom's molecule id, or None."""
+        return self.molecule_id[atom] if atom < len(self.molecule_id) else None
 
     def atom_table(self) -> dict:
         return self.atoms
