@@ -270,6 +270,21 @@ wmd_status wmd_pairwise_store_stride(const wmd_pairwise *pw, size_t *stride);
 wmd_status wmd_pairwise_run_device(wmd_pairwise *pw, const float *d_store, size_t stride,
                                    size_t n_frames, float *d_out, double *d_out64, void *stream);
 void wmd_pairwise_destroy(wmd_pairwise *pw);
+/* TrajectoryClusterPlan k-means (crates/traj-engine/src/plans/analysis/
+ * clustering.rs:190-206, 384-519) over the frames appended to a pairwise handle:
+ * features = each frame's selection superposed onto frame 0 in f64
+ * (align_to_reference :394-418), then Lloyd iterations seeded by the
+ * reference's xorshift64 Fisher-Yates shuffle (:530-550), distances and sums in
+ * its order.  n_clusters is clamped to [1, F], max_iter to >= 1, tol to >= 0.
+ * labels [F], centroids and sizes [min(n_clusters, F)] (synchronous). */
+wmd_status wmd_pairwise_kmeans(wmd_pairwise *pw, size_t n_clusters, size_t max_iter, double tol,
+                               uint64_t seed, int32_t *labels, uint32_t *centroids, uint32_t *sizes);
+/* TrajectoryClusterPlan DBSCAN (clustering.rs:290-382) on host data: dist is a
+ * full [n][n] distance matrix (e.g. wmd_pairwise_finalize's); labels [n]
+ * (-1 noise), centroids / sizes [n] of which *n_clusters are written (the
+ * medoid of each cluster and its size). */
+wmd_status wmd_dbscan(const float *dist, size_t n, double eps, size_t min_samples, int32_t *labels,
+                      uint32_t *centroids, uint32_t *sizes, size_t *n_clusters);
 
 /* ---- Align / Superpose ------------------------------------------------------- */
 

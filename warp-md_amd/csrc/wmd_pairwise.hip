@@ -17,6 +17,7 @@
 // before the product keeps every sum free of cancellation; all arithmetic is
 // f64 like the reference's, and the result is cast to f32 as it does.
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <memory>
 
@@ -271,6 +272,107 @@ int pw_grid(long long total) {
     return (int)std::max<long long>(1, std::min<long long>((total + kPwBlock - 1) / kPwBlock, 65536));
 }
 
+// ---------------------------------------------------------------------------
+// TrajectoryClusterPlan's k-means (crates/traj-engine/src/plans/analysis/
+// clustering.rs:384-519) on the plan's store: features are every frame's
+// selected atoms superposed onto frame 0 (align_to_reference :394-418,
+// R (x - cx) + cy in f64 with R from the FP64 QCP fit), then Lloyd iterations
+// whose distance and sum loops run in the reference's order (sequential over
+// features, resp. over samples), so they are bit-exact given the features.
+// ---------------------------------------------------------------------------
+
+// features [F][3 n] f64; one workgroup per frame (grid-stride).
+__global__ __launch_bounds__(kPwBlock) void km_features_kernel(const float *__restrict__ soa, int stride,
+                                                               int n, int n_frames,
+                                                               double *__restrict__ feat) {
+    __shared__ double red[4];
+    __shared__ double rot[12];
+    const float *y = soa;  // frame 0 is the reference
+    double cy[3];
+    for (int d = 0; d < 3; ++d) {
+        double t = 0.0;
+        for (int k = threadIdx.x; k < n; k += kPwBlock) t += (double)y[d * stride + k];
+        cy[d] = block_sum(t, red) / (double)n;
+    }
+    double gy = 0.0;
+    for (int k = threadIdx.x; k < n; k += kPwBlock)
+        for (int d = 0; d < 3; ++d) {
+            const double e = (double)y[d * stride + k] - cy[d];
+            gy += e * e;
+        }
+    gy = block_sum(gy, red);
+    for (int f = blockIdx.x; f < n_frames; f += gridDim.x) {
+        const float *x = soa + (long long)f * 3 * stride;
+        double cx[3];
+        for (int d = 0; d < 3; ++d) {
+            double t = 0.0;
+            for (int k = threadIdx.x; k < n; k += kPwBlock) t += (double)x[d * stride + k];
+            cx[d] = block_sum(t, red) / (double)n;
+        }
+        double m[10];
+        for (int c = 0; c < 10; ++c) m[c] = 0.0;
+        for (int k = threadIdx.x; k < n; k += kPwBlock) {
+            const double xr[3] = {(double)x[k] - cx[0], (double)x[stride + k] - cx[1], (double)x[2 * stride + k] - cx[2]};
+            const double yr[3] = {(double)y[k] - cy[0], (double)y[stride + k] - cy[1], (double)y[2 * stride + k] - cy[2]};
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b) m[3 * a + b] += xr[a] * yr[b];
+            m[9] += xr[0] * xr[0] + xr[1] * xr[1] + xr[2] * xr[2];
+        }
+        for (int c = 0; c < 10; ++c) m[c] = block_sum(m[c], red);
+        if (threadIdx.x == 0) {
+            QcpResult res;
+            qcp_solve(m, m[9], gy, true, &res);
+            for (int i = 0; i < 9; ++i) rot[i] = res.rot[i];
+        }
+        __syncthreads();
+        double R[9];
+        for (int i = 0; i < 9; ++i) R[i] = rot[i];
+        double *o = feat + (long long)f * 3 * n;
+        for (int k = threadIdx.x; k < n; k += kPwBlock) {
+            const double c0 = (double)x[k] - cx[0], c1 = (double)x[stride + k] - cx[1],
+                         c2 = (double)x[2 * stride + k] - cx[2];
+            for (int a = 0; a < 3; ++a) o[3 * k + a] = (R[3 * a] * c0 + R[3 * a + 1] * c1 + R[3 * a + 2] * c2) + cy[a];
+        }
+        __syncthreads();  // rot is reused by the next frame
+    }
+}
+
+// dist [F][K]: sq_dist(sample i, centre c) summed over features in order (clustering.rs:521-528).
+__global__ __launch_bounds__(kPwBlock) void km_dist_kernel(const double *__restrict__ feat,
+                                                           const double *__restrict__ centers,
+                                                           long long n_feat, int n_samples, int k,
+                                                           double *__restrict__ dist) {
+    const long long total = (long long)n_samples * k;
+    for (long long t = (long long)blockIdx.x * kPwBlock + threadIdx.x; t < total;
+         t += (long long)gridDim.x * kPwBlock) {
+        const long long i = t / k, c = t - i * k;
+        const double *a = feat + i * n_feat, *b = centers + c * n_feat;
+        double s = 0.0;
+        for (long long q = 0; q < n_feat; ++q) {
+            const double d = a[q] - b[q];
+            s += d * d;
+        }
+        dist[t] = s;
+    }
+}
+
+// sums [K][n_feat]: per cluster and feature, the samples' values added in sample order.
+__global__ __launch_bounds__(kPwBlock) void km_sums_kernel(const double *__restrict__ feat,
+                                                           const int *__restrict__ labels,
+                                                           long long n_feat, int n_samples, int k,
+                                                           double *__restrict__ sums) {
+    const long long total = (long long)k * n_feat;
+    for (long long t = (long long)blockIdx.x * kPwBlock + threadIdx.x; t < total;
+         t += (long long)gridDim.x * kPwBlock) {
+        const int c = (int)(t / n_feat);
+        const long long q = t - (long long)c * n_feat;
+        double s = 0.0;
+        for (int i = 0; i < n_samples; ++i)
+            if (labels[i] == c) s += feat[(long long)i * n_feat + q];
+        sums[t] = s;
+    }
+}
+
 }  // namespace
 
 struct wmd_pairwise {
@@ -280,6 +382,7 @@ struct wmd_pairwise {
     int stride = 0;
     size_t frames = 0, cap = 0;
     DevBuf d_sel, d_aos, d_cen, d_g, d_out, d_tiles;
+    DevBuf d_feat, d_centers, d_dist, d_labels, d_sums;
     std::unique_ptr<DevBuf> d_store;
     HostBuf h_stage;
     std::vector<int2> tiles;
@@ -442,6 +545,210 @@ wmd_status wmd_pairwise_run_device(wmd_pairwise *h, const float *d_store, size_t
     WMD_REQUIRE(((uintptr_t)d_store & 15) == 0, WMD_ERR_INVALID, "device store must be 16-byte aligned");
     bind(h->ctx);
     pw_run(h, d_store, stride, n_frames, d_out, d_out64, pick_stream(h->ctx, stream));
+    WMD_API_END
+}
+
+// clustering.rs:530-550: xorshift64 and the Fisher-Yates shuffle it drives.
+static uint64_t km_xorshift64(uint64_t *state) {
+    uint64_t x = *state;
+    if (x == 0) x = 0x9E3779B97F4A7C15ull;
+    x ^= x << 13;
+    x ^= x >> 7;
+    x ^= x << 17;
+    *state = x;
+    return x;
+}
+
+static double km_sq_dist(const double *a, const double *b, size_t n) {
+    double s = 0.0;
+    for (size_t q = 0; q < n; ++q) {
+        const double d = a[q] - b[q];
+        s += d * d;
+    }
+    return s;
+}
+
+wmd_status wmd_pairwise_kmeans(wmd_pairwise *h, size_t n_clusters, size_t max_iter, double tol,
+                               uint64_t seed, int32_t *labels, uint32_t *centroids, uint32_t *sizes) {
+    WMD_API_BEGIN
+    WMD_REQUIRE(h != nullptr, WMD_ERR_INVALID, "null handle");
+    const size_t F = h->frames;
+    if (F == 0) return WMD_OK;
+    WMD_REQUIRE(labels && centroids && sizes, WMD_ERR_INVALID, "null output");
+    bind(h->ctx);
+    hipStream_t st = h->ctx->stream;
+    const int n = (int)h->sel.size();
+    const size_t K = std::min(std::max<size_t>(n_clusters, 1), F);  // clustering.rs:195
+    max_iter = std::max<size_t>(max_iter, 1);
+    tol = std::max(tol, 0.0);
+    const size_t nf = 3 * (size_t)n;
+    WMD_REQUIRE(F * std::max<size_t>(nf, 1) * 8 < (64ull << 30), WMD_ERR_UNSUPPORTED, "k-means features exceed 64 GB");
+    double *feat = (double *)h->d_feat.reserve(std::max<size_t>(F * nf, 1) * 8);
+    if (n > 0) {
+        hipLaunchKernelGGL(km_features_kernel, dim3((unsigned)std::min<size_t>(F, 65536)), dim3(kPwBlock), 0, st,
+                           (const float *)h->d_store->ptr, h->stride, n, (int)F, feat);
+        WMD_HIP_CHECK(hipGetLastError());
+    }
+    // kmeans_cluster (clustering.rs:420-519)
+    uint64_t state = seed == 0 ? 0x9E3779B97F4A7C15ull : seed;
+    std::vector<size_t> order(F);
+    for (size_t i = 0; i < F; ++i) order[i] = i;
+    if (F >= 2)
+        for (size_t i = F - 1; i >= 1; --i) {
+            const size_t j = (size_t)(km_xorshift64(&state) % (uint64_t)(i + 1));
+            std::swap(order[i], order[j]);
+        }
+    std::vector<double> centers(K * nf), sums(K * nf), dist(F * K);
+    std::vector<int32_t> lab(F, 0);
+    std::vector<size_t> counts(K, 0);
+    auto fetch_sample = [&](size_t i, double *dst) {
+        if (nf) WMD_HIP_CHECK(hipMemcpy(dst, feat + i * nf, nf * 8, hipMemcpyDeviceToHost));
+    };
+    for (size_t c = 0; c < K; ++c) fetch_sample(order[c], centers.data() + c * nf);
+    double *d_cent = (double *)h->d_centers.reserve(std::max<size_t>(K * nf, 1) * 8);
+    double *d_dist = (double *)h->d_dist.reserve(F * K * 8);
+    int *d_lab = (int *)h->d_labels.reserve(F * 4);
+    double *d_sums = (double *)h->d_sums.reserve(std::max<size_t>(K * nf, 1) * 8);
+    auto distances = [&]() {
+        if (nf) WMD_HIP_CHECK(hipMemcpyAsync(d_cent, centers.data(), K * nf * 8, hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(km_dist_kernel, dim3(pw_grid((long long)(F * K))), dim3(kPwBlock), 0, st,
+                           (const double *)feat, (const double *)d_cent, (long long)nf, (int)F, (int)K, d_dist);
+        WMD_HIP_CHECK(hipGetLastError());
+        WMD_HIP_CHECK(hipMemcpyAsync(dist.data(), d_dist, F * K * 8, hipMemcpyDeviceToHost, st));
+        WMD_HIP_CHECK(hipStreamSynchronize(st));
+    };
+    for (size_t it = 0; it < max_iter; ++it) {
+        distances();
+        std::fill(counts.begin(), counts.end(), 0);
+        for (size_t i = 0; i < F; ++i) {
+            size_t best = 0;
+            double best_d = INFINITY;
+            for (size_t c = 0; c < K; ++c)
+                if (dist[i * K + c] < best_d) {
+                    best_d = dist[i * K + c];
+                    best = c;
+                }
+            lab[i] = (int32_t)best;
+            counts[best] += 1;
+        }
+        WMD_HIP_CHECK(hipMemcpyAsync(d_lab, lab.data(), F * 4, hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(km_sums_kernel, dim3(pw_grid((long long)(K * std::max<size_t>(nf, 1)))), dim3(kPwBlock), 0, st,
+                           (const double *)feat, (const int *)d_lab, (long long)nf, (int)F, (int)K, d_sums);
+        WMD_HIP_CHECK(hipGetLastError());
+        if (nf) WMD_HIP_CHECK(hipMemcpyAsync(sums.data(), d_sums, K * nf * 8, hipMemcpyDeviceToHost, st));
+        WMD_HIP_CHECK(hipStreamSynchronize(st));
+        double max_shift = 0.0;
+        std::vector<double> nc(nf);
+        for (size_t c = 0; c < K; ++c) {
+            if (counts[c] == 0) {
+                const size_t idx = (size_t)(km_xorshift64(&state) % (uint64_t)F);
+                fetch_sample(idx, nc.data());
+            } else {
+                const double inv = 1.0 / (double)counts[c];
+                for (size_t q = 0; q < nf; ++q) nc[q] = sums[c * nf + q] * inv;
+            }
+            max_shift = std::max(max_shift, std::sqrt(km_sq_dist(centers.data() + c * nf, nc.data(), nf)));
+            std::copy(nc.begin(), nc.end(), centers.begin() + c * nf);
+        }
+        if (max_shift <= tol) break;
+    }
+    std::fill(counts.begin(), counts.end(), 0);
+    for (size_t i = 0; i < F; ++i) counts[(size_t)lab[i]] += 1;
+    distances();  // to the final centres
+    for (size_t c = 0; c < K; ++c) {
+        size_t best_i = 0;
+        double best_d = INFINITY;
+        for (size_t i = 0; i < F; ++i) {
+            if (lab[i] != (int32_t)c) continue;
+            if (dist[i * K + c] < best_d) {
+                best_d = dist[i * K + c];
+                best_i = i;
+            }
+        }
+        centroids[c] = (uint32_t)best_i;
+        sizes[c] = (uint32_t)counts[c];
+    }
+    for (size_t i = 0; i < F; ++i) labels[i] = lab[i];
+    WMD_API_END
+}
+
+// clustering.rs:290-382 dbscan_cluster + neighbors_within on a full [n][n]
+// distance matrix (the triangle's tri_get: the diagonal is 0).
+wmd_status wmd_dbscan(const float *dist, size_t n, double eps, size_t min_samples, int32_t *labels,
+                      uint32_t *centroids, uint32_t *sizes, size_t *n_clusters) {
+    WMD_API_BEGIN
+    WMD_REQUIRE(n_clusters != nullptr, WMD_ERR_INVALID, "null output");
+    *n_clusters = 0;
+    if (n == 0) return WMD_OK;
+    WMD_REQUIRE(dist && labels && centroids && sizes, WMD_ERR_INVALID, "null buffer");
+    const int32_t kUnclassified = -2, kNoise = -1;
+    auto d = [&](size_t i, size_t j) -> double { return i == j ? 0.0 : (double)dist[i * n + j]; };
+    std::vector<int32_t> lab(n, kUnclassified);
+    std::vector<char> visited(n, 0), queued(n, 0);
+    std::vector<size_t> neigh, queue;
+    auto neighbors = [&](size_t p) {
+        neigh.clear();
+        for (size_t j = 0; j < n; ++j)
+            if (d(p, j) <= eps) neigh.push_back(j);
+    };
+    int32_t cluster = 0;
+    for (size_t i = 0; i < n; ++i) {
+        if (visited[i]) continue;
+        visited[i] = 1;
+        neighbors(i);
+        if (neigh.size() < min_samples) {
+            lab[i] = kNoise;
+            continue;
+        }
+        lab[i] = cluster;
+        std::fill(queued.begin(), queued.end(), 0);
+        queue.clear();
+        size_t head = 0;
+        for (size_t p : neigh)
+            if (p != i && !queued[p]) {
+                queue.push_back(p);
+                queued[p] = 1;
+            }
+        while (head < queue.size()) {
+            const size_t p = queue[head++];
+            if (!visited[p]) {
+                visited[p] = 1;
+                neighbors(p);
+                if (neigh.size() >= min_samples)
+                    for (size_t q : neigh)
+                        if (!queued[q]) {
+                            queue.push_back(q);
+                            queued[q] = 1;
+                        }
+            }
+            if (lab[p] == kUnclassified || lab[p] == kNoise) lab[p] = cluster;
+        }
+        ++cluster;
+    }
+    // medoids: the member with the smallest summed distance to the others
+    for (int32_t c = 0; c < cluster; ++c) {
+        std::vector<size_t> members;
+        for (size_t i = 0; i < n; ++i)
+            if (lab[i] == c) members.push_back(i);
+        sizes[c] = (uint32_t)members.size();
+        if (members.empty()) {
+            centroids[c] = 0;
+            continue;
+        }
+        size_t best = members[0];
+        double best_sum = INFINITY;
+        for (size_t a : members) {
+            double sum = 0.0;
+            for (size_t b : members) sum += d(a, b);
+            if (sum < best_sum) {
+                best_sum = sum;
+                best = a;
+            }
+        }
+        centroids[c] = (uint32_t)best;
+    }
+    for (size_t i = 0; i < n; ++i) labels[i] = lab[i];
+    *n_clusters = (size_t)cluster;
     WMD_API_END
 }
 

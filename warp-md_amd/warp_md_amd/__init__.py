@@ -13,12 +13,12 @@ from .analysis import rdf
 from .executor import Device, normalize_frame_indices
 from .batch import UnevenBatch
 from .plans import (AlignPlan, PairDistPlan, PairwiseRmsdPlan, RdfPlan, RmsdPlan,
-                    StructureFactorPlan, SuperposePlan, finalize_rdf)
+                    StructureFactorPlan, SuperposePlan, TrajectoryClusterPlan, finalize_rdf)
 from .system import Selection, System, Trajectory
 
 __all__ = [
     "AlignPlan", "Device", "PairDistPlan", "PairwiseRmsdPlan", "RdfPlan", "RmsdPlan", "StructureFactorPlan",
-    "SuperposePlan", "Selection", "System", "Trajectory", "UnevenBatch", "WmdError",
+    "SuperposePlan", "TrajectoryClusterPlan", "Selection", "System", "Trajectory", "UnevenBatch", "WmdError",
     "device_count", "finalize_rdf", "library_built", "normalize_frame_indices", "rdf",
 ]
 __version__ = "0.1.0"

@@ -67,7 +67,7 @@ SYMBOLS = (
     "wmd_pairwise_create", "wmd_pairwise_append", "wmd_pairwise_frames", "wmd_pairwise_finalize",
     "wmd_pairwise_store_stride", "wmd_pairwise_run_device", "wmd_pairwise_destroy",
     "wmd_align_create", "wmd_align_set_reference", "wmd_align_run", "wmd_align_run_device",
-    "wmd_align_destroy", "wmd_rdf_create_groups",
+    "wmd_align_destroy", "wmd_rdf_create_groups", "wmd_pairwise_kmeans", "wmd_dbscan",
 )
 
 
@@ -184,6 +184,8 @@ def lib():
             "wmd_align_destroy": ([P], None),
             "wmd_rdf_create_groups": ([P, S, P, P, P, P, S, P, S, P, P, S, ctypes.c_float, I,
                                        ctypes.c_uint32, PP], St),
+            "wmd_pairwise_kmeans": ([P, S, S, ctypes.c_double, ctypes.c_uint64, P, P, P], St),
+            "wmd_dbscan": ([P, S, ctypes.c_double, S, P, P, P, ctypes.POINTER(S)], St),
         }
         for name, (args, res) in sig.items():
             fn = getattr(L, name)

@@ -727,6 +727,94 @@ class SuperposePlan(_SuperposeBase):
         return self._run(traj, system, chunk_frames, device, frame_indices, superpose=True)
 
 
+class TrajectoryClusterPlan:
+    """Frame clustering by DBSCAN on the pairwise RMSD matrix or k-means on
+    superposed coordinates.
+
+    Mirrors PyTrajectoryClusterPlan (crates/traj-py/src/io.rs:2805-2880) over
+    TrajectoryClusterPlan (crates/traj-engine/src/plans/analysis/
+    clustering.rs): the matrix and the k-means distance and sum passes run on
+    the device (``wmd_pairwise_*``), DBSCAN's graph walk on the host
+    (``wmd_dbscan``).  Returns dict(labels, centroids, sizes, method,
+    n_frames) like clustering_to_py (pack.rs:2614-2625).
+    """
+
+    def __init__(self, selection, method: str = "dbscan", eps: float = 2.0, min_samples: int = 5,
+                 n_clusters: int = 8, max_iter: int = 100, tol: float = 1.0e-4, seed: int = 0,
+                 memory_budget_bytes: Optional[int] = None):
+        m = str(method).strip().lower()
+        if m not in ("dbscan", "kmeans"):
+            raise RuntimeError("method must be 'dbscan' or 'kmeans'")
+        self.selection = _indices(selection)
+        self.method = m
+        self.eps = float(np.float32(eps))
+        self.min_samples = int(min_samples)
+        self.n_clusters = int(n_clusters)
+        self.max_iter = int(max_iter)
+        self.tol = float(np.float32(tol))
+        self.seed = int(seed)
+        self.memory_budget_bytes = memory_budget_bytes
+
+    def run(self, traj: Trajectory, system: System, chunk_frames: Optional[int] = None,
+            device: str = "auto", frame_indices: Optional[Sequence[int]] = None):
+        dev = Device.from_spec(device)
+        if traj.n_atoms() != system.n_atoms():
+            raise RuntimeError("mismatch: trajectory atom count does not match system")
+        ctx = dev.context()
+        n_atoms = system.n_atoms()
+        sel = self.selection
+        if sel.size and int(sel.max()) >= n_atoms:
+            raise RuntimeError("mismatch: trajectory_cluster selection index out of bounds")
+        h = ctypes.c_void_p()
+        check(lib().wmd_pairwise_create(ctx.handle, ptr(sel), sel.size, 0, ctypes.byref(h)))
+        try:
+            tr = select_frames(traj, frame_indices)
+            tr.reset()
+            step = resolve_chunk_frames(chunk_frames, max(1, n_atoms))
+            while True:
+                chunk = tr.read_chunk4(step)
+                if chunk is None:
+                    break
+                x4, _ = chunk
+                check(lib().wmd_pairwise_append(h, ptr(x4), n_atoms, x4.shape[0]))
+            nf = ctypes.c_size_t()
+            check(lib().wmd_pairwise_frames(h, ctypes.byref(nf)))
+            F = nf.value
+            out = {"labels": np.zeros(0, np.int32), "centroids": np.zeros(0, np.uint32),
+                   "sizes": np.zeros(0, np.uint32), "method": self.method, "n_frames": F}
+            if F == 0:  # clustering.rs:152-161
+                return out
+            if self.method == "dbscan":
+                if self.eps <= 0.0:
+                    raise RuntimeError("parse error: trajectory_cluster DBSCAN eps must be > 0")
+                pairs = F * (F - 1) // 2
+                need = pairs * 4
+                if self.memory_budget_bytes is not None and need > int(self.memory_budget_bytes):
+                    raise RuntimeError(
+                        f"unsupported: trajectory_cluster distance matrix requires {need} bytes but "
+                        f"memory_budget_bytes={int(self.memory_budget_bytes)}")
+                mat = np.zeros((F, F), np.float32)
+                check(lib().wmd_pairwise_finalize(h, ptr(mat)))
+                labels = np.zeros(F, np.int32)
+                cent = np.zeros(F, np.uint32)
+                sizes = np.zeros(F, np.uint32)
+                nc = ctypes.c_size_t()
+                check(lib().wmd_dbscan(ptr(mat), F, ctypes.c_double(self.eps), max(self.min_samples, 1),
+                                       ptr(labels), ptr(cent), ptr(sizes), ctypes.byref(nc)))
+                out.update(labels=labels, centroids=cent[:nc.value].copy(), sizes=sizes[:nc.value].copy())
+            else:
+                k = min(max(self.n_clusters, 1), F)
+                labels = np.zeros(F, np.int32)
+                cent = np.zeros(k, np.uint32)
+                sizes = np.zeros(k, np.uint32)
+                check(lib().wmd_pairwise_kmeans(h, k, max(self.max_iter, 1), ctypes.c_double(max(self.tol, 0.0)),
+                                                ctypes.c_uint64(self.seed), ptr(labels), ptr(cent), ptr(sizes)))
+                out.update(labels=labels, centroids=cent, sizes=sizes)
+            return out
+        finally:
+            lib().wmd_pairwise_destroy(h)
+
+
 def pairdist_centers(bins: int, w) -> np.ndarray:
     """Bin centres i * w + 0.5 * w, each f32 operation rounded (rdf.rs:1276-1279, 459)."""
     w = np.float32(w)
