@@ -255,7 +255,8 @@ def pairwise_bench(ctx, dev, F=5000, reps=3, cpu_frames=600):
                        "symmetric": bool(np.array_equal(m, m.T))},
             "roofline": {"kernel": "pw_gram_kernel", "bound": "mfma", "achieved": ach,
                          "peak": MFMA_F64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                         "frac": ach / MFMA_F64_PEAK_TFLOPS, "traffic": None,
+                         "frac": ach / MFMA_F64_PEAK_TFLOPS,
+                         "traffic": traffic_per_launch("pw_gram_kernel_chain_5000f_2000ca", F),
                          "algorithmic": f"9 x {n} x 2 f64 flops per frame pair x {pairs:.4g} pairs",
                          "avg_launch_ms": gram_avg},
             "cpu_baseline": cpu,
@@ -329,7 +330,7 @@ def align_bench(ctx, dev, F=5000, reps=3, cpu_frames=40):
             "kernel_ms": {"align_superpose": app_avg},
             "roofline": {"kernel": "al_fit_kernel (fit + transform)", "bound": "hbm", "achieved": ach,
                          "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
-                         "traffic": None,
+                         "traffic": traffic_per_launch("al_fit_kernel_chain_5000f_20k", F),
                          "algorithmic": f"28 B per atom (16 read, 12 written) x {n * F:.4g} atoms",
                          "avg_launch_ms": app_avg},
             "cpu_baseline": cpu,
