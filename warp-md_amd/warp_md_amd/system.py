@@ -78,11 +78,11 @@ class System:
         if mol is not None and len(mol):
             if len(mol) != n:
                 raise RuntimeError("mismatch: atom_table.molecule_id must match atom count")
-            self.molecule_id = [int(x) for x in mol]
+            self.molecule_id = np.asarray(mol, dtype=np.int64)
         elif bonds is not None and len(bonds) and n:
-            self.molecule_id = _molecule_ids_from_bonds(n, bonds)
+            self.molecule_id = np.asarray(_molecule_ids_from_bonds(n, bonds), dtype=np.int64)
         else:
-            self.molecule_id = []
+            self.molecule_id = np.zeros(0, np.int64)
         self.positions0 = None
         if positions0 is not None:
             p = np.asarray(positions0, dtype=np.float32)
@@ -99,7 +99,7 @@ class System:
 
     def molecule_id_for_atom(self, atom: int):
         """system.rs:96-98: the atom's molecule id, or None."""
-        return self.molecule_id[atom] if atom < len(self.molecule_id) else None
+        return int(self.molecule_id[atom]) if atom < len(self.molecule_id) else None
 
     def atom_table(self) -> dict:
         return self.atoms
