@@ -462,6 +462,85 @@ def sf_bymol_bench(ctx, dev, reps=2):
     return out
 
 
+def counter_sequence(ctx, dev):
+    """One launch of each measured kernel at its bench size, for hardware
+    counter runs (rocprofv3 --pmc): a plain 1 GiB device copy (the calibration
+    of FETCH_SIZE / WRITE_SIZE), one RDF accumulate of 100 frames of the
+    100,000-water box (500 bins, r_max 15), one RMSD run over 5,000 frames of
+    the 20,000-atom chain (fit on 2,000 CA), one 5,000 x 5,000 pairwise RMSD
+    matrix on the CA and one superposition of 5,000 x 20,000 atoms."""
+    import numpy as np
+    import torch
+
+    from warp_md_amd import workloads
+    from warp_md_amd._native import check, lib, ptr
+
+    L = lib()
+    V = lambda t: ctypes.c_void_p(t.data_ptr())
+    x = torch.ones(1 << 28, dtype=torch.float32, device=dev)  # 1 GiB
+    y = torch.empty_like(x)
+    y.copy_(x)
+    torch.cuda.synchronize()
+    del x, y
+    log("counters: 1 GiB copy done")
+    F = 100
+    store, n, stride, _ = workloads.water_lattice_device(47, 144.3, F, n_waters=100000, seed=1,
+                                                         device=dev)
+    boxes = np.repeat(np.diag([144.3] * 3).reshape(1, 9), F, 0).astype(np.float32)
+    sel = np.arange(n, dtype=np.uint32)
+    h = ctypes.c_void_p()
+    check(L.wmd_rdf_create(ctx.handle, ptr(sel), n, ptr(sel), n, 500, ctypes.c_float(15.0), 1, 0,
+                           ctypes.byref(h)))
+    check(L.wmd_rdf_accumulate_device(h, V(store), stride, F, ptr(boxes), None))
+    torch.cuda.synchronize()
+    L.wmd_rdf_destroy(h)
+    del store
+    log("counters: RDF 100 frames done")
+    c = workloads.chain(n_frames=0, seed=2)
+    nc = c["ref"].shape[0]
+    r4 = np.ones((nc, 4), np.float32)
+    r4[:, :3] = c["ref"]
+    allat = np.arange(nc, dtype=np.uint32)
+    hr = ctypes.c_void_p()
+    check(L.wmd_rmsd_create(ctx.handle, ptr(c["ca"]), c["ca"].size, ptr(allat), allat.size, ptr(r4),
+                            nc, 1, ctypes.byref(hr)))
+    ns, st = ctypes.c_size_t(), ctypes.c_size_t()
+    check(L.wmd_rmsd_store_layout(hr, None, 0, ctypes.byref(ns), ctypes.byref(st)))
+    atoms = np.zeros(st.value, np.uint32)
+    check(L.wmd_rmsd_store_layout(hr, ptr(atoms), atoms.size, ctypes.byref(ns), ctypes.byref(st)))
+    RF = 5000
+    cs = workloads.chain_store_device(c["ref"], atoms, RF, seed=2, device=dev)
+    o = torch.zeros(RF, dtype=torch.float32, device=dev)
+    check(L.wmd_rmsd_run_device(hr, V(cs), st.value, RF, V(o), None, None, None, None, None))
+    torch.cuda.synchronize()
+    L.wmd_rmsd_destroy(hr)
+    del cs
+    log("counters: RMSD 5000 frames done")
+    store = workloads.chain_store_device(c["ref"], c["ca"], 5000, seed=2, device=dev)
+    selp = np.arange(c["ca"].size, dtype=np.uint32)
+    hp = ctypes.c_void_p()
+    check(L.wmd_pairwise_create(ctx.handle, ptr(selp), selp.size, 0, ctypes.byref(hp)))
+    outp = torch.empty((5000, 5000), dtype=torch.float32, device=dev)
+    check(L.wmd_pairwise_run_device(hp, V(store), store.shape[2], 5000, V(outp), None, None))
+    torch.cuda.synchronize()
+    L.wmd_pairwise_destroy(hp)
+    del store, outp
+    log("counters: pairwise 5000 x 5000 done")
+    st2 = workloads.chain_store_device(c["ref"], allat, 5000, seed=2, device=dev)
+    x4 = torch.ones((5000, nc, 4), dtype=torch.float32, device=dev)
+    x4[:, :, :3] = st2.permute(0, 2, 1)
+    del st2
+    ha = ctypes.c_void_p()
+    check(L.wmd_align_create(ctx.handle, ptr(c["ca"]), c["ca"].size, None, ptr(r4), nc, 0,
+                             ctypes.byref(ha)))
+    rt64 = torch.zeros((5000, 12), dtype=torch.float64, device=dev)
+    xyz = torch.empty((5000, nc, 3), dtype=torch.float32, device=dev)
+    check(L.wmd_align_run_device(ha, V(x4), nc, 5000, V(rt64), None, V(xyz), None))
+    torch.cuda.synchronize()
+    L.wmd_align_destroy(ha)
+    log("counters: superposition 5000 x 20000 done")
+
+
 def uneven_batch_bench(ctx, dev, reps=3, autotune=True):
     """256 trajectories, atom counts log-uniform in [500, 50000], cubic boxes at
     0.1 atoms/A^3, 50 uniform-random frames each; RDF (300 bins, r_max 12 A)
@@ -563,10 +642,12 @@ def main():
     ap.add_argument("--no-sf-bymol", action="store_true", help="skip the structure-factor and bymol legs")
     ap.add_argument("--only-sf-bymol", action="store_true", help="time only the structure-factor and bymol legs")
     ap.add_argument("--only-superpose", action="store_true", help="time only the pairwise and superposition legs")
+    ap.add_argument("--counters", action="store_true",
+                    help="launch each measured kernel once (for rocprofv3 --pmc runs) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for "
-              f"--gpus {args.gpus}); the frame-sharded design is described in DESIGN.md.",
+              f"--gpus {args.gpus}); frames shard across GPUs by contiguous ranges, which is not built here.",
               file=sys.stderr)
         sys.exit(2)
 
@@ -581,6 +662,9 @@ def main():
     L = lib()
     V = lambda t: ctypes.c_void_p(t.data_ptr())
 
+    if args.counters:
+        counter_sequence(ctx, dev)
+        return
     if args.only_sf_bymol:
         print(json.dumps({"sf_bymol": sf_bymol_bench(ctx, dev)}), flush=True)
         return
@@ -666,6 +750,27 @@ def main():
         f"{pair_n // K} launches, build {build_ms / K:.2f} ms; rmsd moments {mom_ms / K:.3f} ms, "
         f"finalize {fin_ms / K:.3f} ms; step {ms_per_step:.2f} ms")
 
+    # candidate efficiency of the cell enumeration (untimed): the pair
+    # evaluations of 100 frames under edge diagnostics, against the in-range pairs
+    h_diag = ctypes.c_void_p()
+    check(L.wmd_rdf_create(ctx.handle, ptr(sel), n, ptr(sel), n, 500, ctypes.c_float(15.0), 1, 1,
+                           ctypes.byref(h_diag)))
+    FD = min(F, 100)
+    check(L.wmd_rdf_accumulate_device(h_diag, V(store), stride, FD, ptr(boxes[:FD]), None))
+    dcounts = np.zeros(500, np.uint64)
+    n_edge = np.zeros(1, np.uint64)
+    check(L.wmd_rdf_read(h_diag, ptr(dcounts), ptr(n_edge)))
+    n_eval = ctypes.c_uint64()
+    check(L.wmd_rdf_pair_evaluations(h_diag, ctypes.byref(n_eval)))
+    L.wmd_rdf_destroy(h_diag)
+    candidates = {"frames": FD, "pair_evaluations": int(n_eval.value),
+                  "in_range_pairs": int(dcounts.sum()) // 2,
+                  "efficiency": (int(dcounts.sum()) / 2.0) / max(1, int(n_eval.value)),
+                  "edge_pairs": int(n_edge[0])}
+    log(f"cell enumeration: {candidates['pair_evaluations']} evaluations for "
+        f"{candidates['in_range_pairs']} in-range pairs over {FD} frames "
+        f"(efficiency {candidates['efficiency']:.3f})")
+
     # triclinic variant of the RDF workload (10 deg xy tilt), timed separately
     tric = None
     if not args.no_triclinic:
@@ -747,6 +852,7 @@ def main():
             "algorithmic": f"{RDF_OPS_PER_PAIR} VALU lane-ops per unordered in-range pair x "
                            f"{p_in:.4g} pairs per step",
             "avg_launch_ms": pair_ms / max(pair_n, 1),
+            "candidates": candidates,
         },
         "roofline_rmsd": {
             "kernel": "rmsd_moments_kernel",

@@ -131,6 +131,13 @@ wmd_status wmd_rdf_accumulate_device(wmd_rdf *rdf, const float *d_soa, size_t st
  * pairs whose f32 distance lies within 1e-5 A of a bin edge (requires
  * WMD_RDF_EDGE_DIAGNOSTICS, else 0). */
 wmd_status wmd_rdf_read(wmd_rdf *rdf, uint64_t *counts, uint64_t *edge);
+/* Diagnostics with no counterpart in the reference: the number of pair
+ * distances the cell-list kernels evaluated since create / reset (unordered
+ * pairs; counted only with WMD_RDF_EDGE_DIAGNOSTICS, else 0; the brute-force
+ * path, which evaluates every pair of the selections, is not counted).  For
+ * one selection paired with itself, (sum of counts / 2) / evaluations is the
+ * enumeration's candidate efficiency.  Synchronises. */
+wmd_status wmd_rdf_pair_evaluations(wmd_rdf *rdf, uint64_t *evaluations);
 /* GpuContext::reset_counts (packing_constraints.rs:900-914). */
 wmd_status wmd_rdf_reset(wmd_rdf *rdf);
 void wmd_rdf_destroy(wmd_rdf *rdf);

@@ -31,10 +31,11 @@ def boxes9(box, n_frames, pbc):
 
 
 def rdf_counts(coords, sel_a, sel_b, bins, r_max, pbc="orthorhombic", box=None, flags=0,
-               cell_k=0, full_chunk=True, split=None):
+               cell_k=0, full_chunk=True, split=None, evals=False):
     """GPU counts through wmd_rdf_accumulate (full-system xyzw chunk).
 
-    Returns (counts u64, edge, path) where path is 1 brute / 2 cells."""
+    Returns (counts u64, edge, path) where path is 1 brute / 2 cells, and with
+    evals=True also the pair evaluations (wmd_rdf_pair_evaluations)."""
     c = np.ascontiguousarray(coords, np.float32)
     F, N, _ = c.shape
     x4 = np.ones((F, N, 4), np.float32)
@@ -57,8 +58,12 @@ def rdf_counts(coords, sel_a, sel_b, bins, r_max, pbc="orthorhombic", box=None, 
         check(lib().wmd_rdf_read(h, ptr(counts), ptr(edge)))
         path = ctypes.c_int()
         check(lib().wmd_rdf_last_path(h, ctypes.byref(path), None))
+        n_eval = ctypes.c_uint64()
+        check(lib().wmd_rdf_pair_evaluations(h, ctypes.byref(n_eval)))
     finally:
         lib().wmd_rdf_destroy(h)
+    if evals:
+        return counts, int(edge[0]), path.value, int(n_eval.value)
     return counts, int(edge[0]), path.value
 
 

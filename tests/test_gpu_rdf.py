@@ -132,6 +132,27 @@ def test_cell_shapes_agree(cell_k):
     np.testing.assert_array_equal(got, ref)
 
 
+@pytest.mark.parametrize("cell_k", [1, 2, 3])
+def test_pair_evaluation_count(cell_k):
+    """wmd_rdf_pair_evaluations under edge diagnostics: counts unchanged, every
+    in-range unordered pair evaluated, no more evaluations than unordered
+    pairs, zero without diagnostics, identical on a repeat."""
+    w = workloads.water_lattice(n_side=16, box=16 * 3.104, n_frames=2, seed=4, with_h=False)
+    n = w["coords"].shape[1]
+    sel = np.arange(n)
+    plain, _, path, e0 = _gpu.rdf_counts(w["coords"], sel, sel, 200, 10.0, box=w["box"],
+                                         cell_k=cell_k, evals=True)
+    diag, _, _, ev = _gpu.rdf_counts(w["coords"], sel, sel, 200, 10.0, box=w["box"], flags=0x1,
+                                     cell_k=cell_k, evals=True)
+    _, _, _, ev2 = _gpu.rdf_counts(w["coords"], sel, sel, 200, 10.0, box=w["box"], flags=0x1,
+                                   cell_k=cell_k, evals=True)
+    assert path == 2 and e0 == 0 and ev == ev2
+    np.testing.assert_array_equal(plain, diag)
+    in_range = int(diag.sum()) // 2
+    print(f"k={cell_k}: {ev} evaluations, {in_range} in range, efficiency {in_range / ev:.3f}")
+    assert in_range <= ev <= 2 * n * (n - 1) // 2
+
+
 def test_lattice_workload_full_exact_with_edges():
     """1,000 rigid waters (3,000 atoms), L = 31.04 A, 100 frames, OW-OW, 200 bins, r_max 10."""
     w = workloads.water_lattice(n_side=10, box=31.04, n_frames=100, seed=0)

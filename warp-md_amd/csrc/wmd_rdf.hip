@@ -1296,9 +1296,10 @@ wmd_status wmd_rdf_create(wmd_ctx *ctx, const uint32_t *sel_a, size_t n_a, const
                 }
             }
             hipStream_t st = ctx->stream;
-            h->d_counts.reserve((bins + 1) * 8);
-            hipLaunchKernelGGL(fill_u64_kernel, dim3((bins + 1 + kBlock - 1) / kBlock), dim3(kBlock), 0, st,
-                               h->d_counts.as<unsigned long long>(), (long long)bins + 1);
+            // counts [bins], edge pairs, pair evaluations
+            h->d_counts.reserve((bins + 2) * 8);
+            hipLaunchKernelGGL(fill_u64_kernel, dim3((bins + 2 + kBlock - 1) / kBlock), dim3(kBlock), 0, st,
+                               h->d_counts.as<unsigned long long>(), (long long)bins + 2);
             WMD_HIP_CHECK(hipGetLastError());
             h->d_thr.reserve((bins + 1) * 4);
             h->d_elo.reserve((bins + 1) * 4);
@@ -1556,14 +1557,26 @@ wmd_status wmd_rdf_read(wmd_rdf *h, uint64_t *counts, uint64_t *edge) {
     WMD_API_END
 }
 
+wmd_status wmd_rdf_pair_evaluations(wmd_rdf *h, uint64_t *evaluations) {
+    WMD_API_BEGIN
+    WMD_REQUIRE(h && evaluations, WMD_ERR_INVALID, "null handle or output");
+    bind(h->ctx);
+    *evaluations = 0;
+    if (h->bins == 0) return WMD_OK;
+    WMD_HIP_CHECK(hipDeviceSynchronize());
+    WMD_HIP_CHECK(hipMemcpy(evaluations, h->d_counts.as<unsigned long long>() + h->bins + 1, 8,
+                            hipMemcpyDeviceToHost));
+    WMD_API_END
+}
+
 wmd_status wmd_rdf_reset(wmd_rdf *h) {
     WMD_API_BEGIN
     WMD_REQUIRE(h != nullptr, WMD_ERR_INVALID, "null handle");
     bind(h->ctx);
     if (h->bins) {
-        int g = (h->bins + 1 + kBlock - 1) / kBlock;
+        int g = (h->bins + 2 + kBlock - 1) / kBlock;
         hipLaunchKernelGGL(fill_u64_kernel, dim3(g), dim3(kBlock), 0, h->ctx->stream,
-                           h->d_counts.as<unsigned long long>(), (long long)h->bins + 1);
+                           h->d_counts.as<unsigned long long>(), (long long)h->bins + 2);
         WMD_HIP_CHECK(hipGetLastError());
         WMD_HIP_CHECK(hipStreamSynchronize(h->ctx->stream));
     }

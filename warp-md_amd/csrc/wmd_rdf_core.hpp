@@ -1088,16 +1088,23 @@ __device__ __forceinline__ void cell_waves(const CellGrid &G, float4 *ring, unsi
 }
 
 // Plain histogram consumer: exact bins, u64 LDS counts.
+// With EDGE (diagnostics) it also counts the pair evaluations: a valid j lane
+// meets min(rank, n_home) home atoms in a home chunk and all n_home otherwise.
 template <int PBC, bool LT, bool WEIGHTED, bool EDGE, bool TABLE>
 struct PlainCons {
     HistParams p;
     CellLds L;
     unsigned long long edge_local;
+    unsigned long long evals_local;
     template <int MODE, bool HOME>
     __device__ __forceinline__ void chunk(const float4 *hp, int n_home, float xj, float yj, float zj,
                                           unsigned mj, int rank, unsigned, int, const FrameBox &b) {
         cell_chunk<PBC, LT, WEIGHTED, EDGE, TABLE, MODE, HOME>(hp, n_home, xj, yj, zj, mj, rank, b, p,
                                                                 L, edge_local);
+        if constexpr (EDGE) {
+            const int m = HOME ? min(rank, n_home) : n_home;
+            evals_local += xj == xj ? (unsigned long long)m : 0ull;
+        }
     }
     __device__ __forceinline__ void unit_begin(int, int, int) {}
     __device__ __forceinline__ void unit_end(int, int, int) {}
@@ -1121,7 +1128,7 @@ __global__ __launch_bounds__(kCellBlock) void rdf_cell_kernel(CellGrid G, HistPa
 
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform: unit indices in SGPRs
     const int lane = threadIdx.x & 63;
-    PlainCons<PBC, LT, WEIGHTED, EDGE, TABLE> cons{p, L, 0ull};
+    PlainCons<PBC, LT, WEIGHTED, EDGE, TABLE> cons{p, L, 0ull, 0ull};
     cell_waves<PBC, false>(G, L.ring + wave * kRing, nullptr, wave, cons);
 
     __syncthreads();
@@ -1130,9 +1137,13 @@ __global__ __launch_bounds__(kCellBlock) void rdf_cell_kernel(CellGrid G, HistPa
         if (v) atomicAdd(&p.counts[i], v);
     }
     if constexpr (EDGE) {
-        unsigned long long e = cons.edge_local;
-        for (int off = 32; off > 0; off >>= 1) e += __shfl_xor(e, off, 64);
+        unsigned long long e = cons.edge_local, v = cons.evals_local;
+        for (int off = 32; off > 0; off >>= 1) {
+            e += __shfl_xor(e, off, 64);
+            v += __shfl_xor(v, off, 64);
+        }
         if (lane == 0 && e) atomicAdd(p.edge, e);
+        if (lane == 0 && v) atomicAdd(p.edge + 1, v);
     }
 }
 

@@ -54,7 +54,7 @@ SYMBOLS = (
     "wmd_ctx_create", "wmd_ctx_destroy", "wmd_ctx_synchronize", "wmd_ctx_stream",
     "wmd_ctx_cu_count", "wmd_ctx_profile", "wmd_ctx_profile_read",
     "wmd_rdf_create", "wmd_rdf_io_selection", "wmd_rdf_accumulate",
-    "wmd_rdf_accumulate_selected", "wmd_rdf_accumulate_device", "wmd_rdf_read",
+    "wmd_rdf_accumulate_selected", "wmd_rdf_accumulate_device", "wmd_rdf_read", "wmd_rdf_pair_evaluations",
     "wmd_rdf_reset", "wmd_rdf_destroy", "wmd_rdf_last_path", "wmd_rdf_set_tuning",
     "wmd_rdf_finalize", "wmd_rdf_create_f64", "wmd_sf_finalize",
     "wmd_rmsd_create", "wmd_rmsd_store_layout", "wmd_rmsd_run", "wmd_rmsd_run_device",
@@ -134,6 +134,7 @@ def lib():
             "wmd_rdf_accumulate_selected": ([P, P, S, P], St),
             "wmd_rdf_accumulate_device": ([P, P, S, S, P, P], St),
             "wmd_rdf_read": ([P, P, P], St),
+            "wmd_rdf_pair_evaluations": ([P, P], St),
             "wmd_rdf_reset": ([P], St),
             "wmd_rdf_destroy": ([P], None),
             "wmd_rdf_last_path": ([P, ctypes.POINTER(I), P], St),
