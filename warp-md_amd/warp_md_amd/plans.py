@@ -77,8 +77,8 @@ class RdfPlan:
 
     # rdf.rs:378-385 common_atom_count
     def _common_count(self) -> int:
-        left = set(self.sel_a.tolist())
-        return int(sum(1 for i in self.sel_b.tolist() if i in left))
+        # entries of sel_b (with repeats) whose atom is in sel_a
+        return int(np.count_nonzero(np.isin(self.sel_b, self.sel_a)))
 
     def set_tuning(self, cell_k: int = 0):
         """Cell-list shape (cells per r_max, 1..3; 0 = automatic)."""
@@ -137,18 +137,18 @@ class RdfPlan:
                     if boxes is None:
                         raise RuntimeError(
                             "mismatch: RDF requires orthorhombic box for PBC or volume normalization")
-                    for m in boxes:
-                        if self.pbc == _native.PBC_TRICLINIC:
+                    if self.pbc == _native.PBC_TRICLINIC:
+                        for m in boxes:
                             volume_sum += abs(float(np.linalg.det(m.astype(np.float64).reshape(3, 3))))
-                        else:
-                            off = m[[1, 2, 3, 5, 6, 7]]
-                            if np.any(off != 0) or np.any(m[[0, 4, 8]] <= 0):
-                                raise RuntimeError("mismatch: RDF requires orthorhombic box for PBC or volume normalization")
-                            # rdf_box_measure (rdf.rs:480-485): f64 product of the f32 lengths
-                            if planar:
-                                volume_sum += float(m[0]) * float(m[4])
-                            else:
-                                volume_sum += float(m[0]) * float(m[4]) * float(m[8])
+                    else:
+                        if np.any(boxes[:, [1, 2, 3, 5, 6, 7]] != 0) or np.any(boxes[:, [0, 4, 8]] <= 0):
+                            raise RuntimeError("mismatch: RDF requires orthorhombic box for PBC or volume normalization")
+                        # rdf_box_measure (rdf.rs:480-485): f64 product of the f32
+                        # lengths, added frame by frame in order
+                        lens = boxes[:, [0, 4, 8]].astype(np.float64)
+                        vols = lens[:, 0] * lens[:, 1] if planar else lens[:, 0] * lens[:, 1] * lens[:, 2]
+                        for v in vols.tolist():
+                            volume_sum += v
                 frames += nf
                 # the full chunk; the engine gathers the IO selection on the device
                 check(lib().wmd_rdf_accumulate(h, ptr(coords), n_atoms, nf, ptr(boxes)))
