@@ -216,6 +216,31 @@ def test_full_size_properties():
     torch.cuda.empty_cache()
 
 
+@pytest.mark.parametrize("kind", ["orthorhombic", "triclinic"])
+def test_far_and_non_finite_atoms_exact(kind):
+    """Raw coordinates up to 1,000 box lengths away (frames with them take the
+    checked wrap) and NaN / inf coordinates (never counted), cell path and
+    brute force, against the oracle."""
+    n, F, bins, r_max = 1200, 3, 80, 7.3
+    frames, box = _random_case(kind, n, F, seed=17, L=32.0)
+    L = np.float32(32.0)
+    frames[0, 10:20, 0] += np.float32(20) * L      # 20 box lengths
+    frames[1, 30:40, 1] -= np.float32(150) * L     # 150 box lengths
+    frames[1, 50:55, 2] += np.float32(1000) * L    # 1,000 box lengths
+    frames[2, 60, :] = np.nan
+    frames[2, 61, 0] = np.inf
+    frames[2, 62, 1] = -np.inf
+    sel = np.arange(n)
+    for force_brute in (False, True):
+        got, _, path = _gpu.rdf_counts(frames, sel, sel, bins, r_max, pbc=kind, box=box,
+                                       flags=0x2 if force_brute else 0)
+        ref = oracle.rdf_counts(frames, sel, sel, bins, r_max, pbc=kind,
+                                boxes=_gpu.boxes9(box, F, kind))
+        assert path == (1 if force_brute else 2)
+        assert ref.sum() > 0
+        np.testing.assert_array_equal(got, ref)
+
+
 def test_empty_and_degenerate_inputs():
     frames = np.zeros((2, 4, 3), np.float32)
     got, _, _ = _gpu.rdf_counts(frames, [], [0, 1], 10, 5.0, pbc="none")

@@ -132,7 +132,9 @@ __device__ __forceinline__ float soft_force_over_r(float s, const SoftParams &sp
 }
 
 // Minimum-image displacement x_j - x_i of the cell stream's pair (MODE as in
-// cell_chunk), returned with its squared length.
+// cell_chunk; the soft kernels take the rint wrap for MODE 3 as well: the
+// smoothed histogram is this engine's own quantity), returned with its
+// squared length.
 template <int PBC, bool LT, int MODE>
 __device__ __forceinline__ float soft_disp(const float4 &pi, float xj, float yj, float zj, float sx,
                                            float sy, float sz, const FrameBox &b, float &dx,
@@ -143,7 +145,7 @@ __device__ __forceinline__ float soft_disp(const float4 &pi, float xj, float yj,
     if constexpr (PBC == WMD_PBC_ORTHO) {
         if constexpr (MODE == 1) {
             dx -= sx; dy -= sy; dz -= sz;
-        } else if constexpr (MODE == 2) {
+        } else if constexpr (MODE >= 2) {
             dx -= rintf(dx * b.ilx) * b.lx;
             dy -= rintf(dy * b.ily) * b.ly;
             dz -= rintf(dz * b.ilz) * b.lz;

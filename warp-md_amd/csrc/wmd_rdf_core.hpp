@@ -804,11 +804,13 @@ __device__ __forceinline__ void cell_dispatch(C &cons, int mode, bool home, cons
     if (home) {
         if (mode == 0) cons.template chunk<0, true>(hp, n_home, xj, yj, zj, mj, rank, jpos, f, b);
         else if (mode == 1) cons.template chunk<1, true>(hp, n_home, xj, yj, zj, mj, rank, jpos, f, b);
-        else cons.template chunk<2, true>(hp, n_home, xj, yj, zj, mj, rank, jpos, f, b);
+        else if (mode == 2) cons.template chunk<2, true>(hp, n_home, xj, yj, zj, mj, rank, jpos, f, b);
+        else cons.template chunk<3, true>(hp, n_home, xj, yj, zj, mj, rank, jpos, f, b);
     } else {
         if (mode == 0) cons.template chunk<0, false>(hp, n_home, xj, yj, zj, mj, rank, jpos, f, b);
         else if (mode == 1) cons.template chunk<1, false>(hp, n_home, xj, yj, zj, mj, rank, jpos, f, b);
-        else cons.template chunk<2, false>(hp, n_home, xj, yj, zj, mj, rank, jpos, f, b);
+        else if (mode == 2) cons.template chunk<2, false>(hp, n_home, xj, yj, zj, mj, rank, jpos, f, b);
+        else cons.template chunk<3, false>(hp, n_home, xj, yj, zj, mj, rank, jpos, f, b);
     }
 }
 
@@ -883,7 +885,10 @@ __device__ __forceinline__ void cell_unit(const CellGrid &G, int f, int h, const
                 diff |= (__float_as_uint(q.w) & 0xFFFFFFu) != m0;
             }
         }
-        const bool use_shift = G.shift_ok && uload(G.frame_flag, f) == 0u && !__any(diff);
+        // far: the frame has atoms beyond kImageLimit box lengths (or non-finite
+        // ones); its pairs take the checked wrap, exact for any quotient
+        const bool far = uload(G.frame_flag, f) != 0u;
+        const bool use_shift = G.shift_ok && !far && !__any(diff);
         const float mn0 = wave_min_key(kn0), mn1 = wave_min_key(kn1), mn2 = wave_min_key(kn2);
         const float mx0 = wave_max_key(kx0), mx1 = wave_max_key(kx1), mx2 = wave_max_key(kx2);
         // per-piece byte constants K: n + 128 = (m_j + 32) + K per byte, with
@@ -1027,8 +1032,9 @@ __device__ __forceinline__ void cell_unit(const CellGrid &G, int f, int h, const
                 const unsigned mj = v ? __float_as_uint(ej.w) : 0x808080u;
                 unsigned jpos = 0;
                 if constexpr (POS) jpos = v ? ringp[slot] : 0xFFFFFFFFu;
-                // 0: no lane has an image shift; 1: some lane has one; 2: rint wrap
-                const int mode = !use_shift ? 2 : (__any((mj & 0xFFFFFFu) != 0x808080u) ? 1 : 0);
+                // 0: no lane has an image shift; 1: some lane has one; 2: rint wrap;
+                // 3: checked wrap
+                const int mode = far ? 3 : !use_shift ? 2 : (__any((mj & 0xFFFFFFu) != 0x808080u) ? 1 : 0);
                 const bool home = head < n_home;
                 const int rank = home && head + lane < n_home ? head + lane : 0x7FFFFFFF;
                 cell_dispatch(cons, mode, home, hp, n_home, xj, yj, zj, mj, rank, jpos, f, b);
