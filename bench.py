@@ -110,6 +110,61 @@ def cpu_baseline(n_side=47, box=144.3, n_waters=100000):
     return out
 
 
+def small_box_leg(ctx, dev, reps=20):
+    """RdfPlan OW-OW on 1,000 rigid waters (3,000 atoms, cubic 31.04 A, 100
+    frames, 200 bins, r_max 10): the device path on a resident store against
+    the oracle's reference loop over all 100 frames, single-threaded and over
+    the host cores; counts compared."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import torch
+
+    import oracle
+    from warp_md_amd import workloads
+    from warp_md_amd._native import check, lib, ptr
+
+    L = lib()
+    w = workloads.water_lattice(n_side=10, box=31.04, n_frames=100, seed=0)
+    coords, ow = w["coords"], np.asarray(w["ow"])
+    F, n = coords.shape[0], ow.size
+    stride = (n + 3) // 4 * 4
+    soa = np.zeros((F, 3, stride), np.float32)
+    soa[:, :, :n] = np.transpose(coords[:, ow, :], (0, 2, 1))
+    store = torch.as_tensor(soa).to(dev)
+    boxes = np.repeat(np.diag(np.asarray(w["box"], np.float32)).reshape(1, 9), F, 0).astype(np.float32)
+    sel = np.arange(n, dtype=np.uint32)
+    h = ctypes.c_void_p()
+    check(L.wmd_rdf_create(ctx.handle, ptr(sel), n, ptr(sel), n, 200, ctypes.c_float(10.0), 1, 0,
+                           ctypes.byref(h)))
+    V = ctypes.c_void_p(store.data_ptr())
+    check(L.wmd_rdf_accumulate_device(h, V, stride, F, ptr(boxes), None))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        check(L.wmd_rdf_reset(h))
+        check(L.wmd_rdf_accumulate_device(h, V, stride, F, ptr(boxes), None))
+    torch.cuda.synchronize()
+    gpu_s = (time.perf_counter() - t0) / reps
+    counts = np.zeros(200, np.uint64)
+    check(L.wmd_rdf_read(h, ptr(counts), None))
+    L.wmd_rdf_destroy(h)
+    evals = F * (n * n - n)
+    t0 = time.perf_counter()
+    ref = oracle.rdf_counts(coords, ow, ow, 200, 10.0, boxes=w["box"])
+    cpu_s = time.perf_counter() - t0
+    threads = max(1, min(16, int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))))
+    t0 = time.perf_counter()
+    ref_omp = oracle.rdf_counts(coords, ow, ow, 200, 10.0, boxes=w["box"], method="omp", threads=threads)
+    omp_s = time.perf_counter() - t0
+    same = bool(np.array_equal(counts, ref) and np.array_equal(ref_omp, ref))
+    log(f"1,000 waters x 100 frames: GPU {gpu_s * 1e3:.3f} ms ({evals / gpu_s:.3g} pair-evals/s), "
+        f"oracle 1 thread {cpu_s:.2f} s, {threads} threads {omp_s:.2f} s, counts equal: {same}")
+    return {"pair_evals": evals, "gpu_ms": gpu_s * 1e3, "gpu_pair_evals_per_s": evals / gpu_s,
+            "cpu_1thread_pair_evals_per_s": evals / cpu_s,
+            "cpu_all_cores": {"pair_evals_per_s": evals / omp_s, "threads": threads},
+            "counts_equal": same, "in_range_pairs": int(counts.sum())}
+
+
 def soft_rdf_ratio(ctx, dev, n=20000, frames=64, reps=3):
     """Soft-RDF forward + backward against the plain RDF forward on the same
     input: 20,000 beads at 0.08 / A^3 (cubic 62.996 A), 64 frames of a uniform
@@ -808,6 +863,7 @@ def main():
     superpose = None if args.no_align else align_bench(ctx, dev)
     ingest = None if args.no_ingest else host_ingest_bench(ctx)
     sf_bymol = None if args.no_sf_bymol else sf_bymol_bench(ctx, dev)
+    small = None if args.no_cpu else small_box_leg(ctx, dev)
     cpu = None if args.no_cpu else cpu_baseline()
 
     line = {
@@ -841,6 +897,7 @@ def main():
         "superpose": superpose,
         "host_ingest": ingest,
         "sf_bymol": sf_bymol,
+        "rdf_1000_waters": small,
         "roofline": {
             "kernel": "rdf_cell_kernel",
             "bound": "valu",
