@@ -52,6 +52,11 @@ void Event::wait() {
     if (ev) WMD_HIP_CHECK(hipEventSynchronize(ev));
 }
 
+hipStream_t copy_stream(wmd_ctx *ctx) {
+    if (!ctx->copy) WMD_HIP_CHECK(hipStreamCreateWithFlags(&ctx->copy, hipStreamNonBlocking));
+    return ctx->copy;
+}
+
 void host_copy(void *dst, const void *src, size_t bytes) {
     const size_t kMin = 16ull << 20;  // below this one thread is as fast
     unsigned nt = std::thread::hardware_concurrency();
@@ -224,6 +229,7 @@ void wmd_ctx_destroy(wmd_ctx *ctx) {
     (void)hipStreamSynchronize(ctx->stream);
     (void)hipDeviceSynchronize();
     clear_marks(ctx);
+    if (ctx->copy) (void)hipStreamDestroy(ctx->copy);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }

@@ -110,6 +110,7 @@ void host_copy(void *dst, const void *src, size_t bytes);
 struct wmd_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t copy = nullptr;  // host-to-device copies of the host-chunk paths (created on first use)
     int cu_count = 0;
     int xcd_count = 8;
     size_t lds_per_block = 0;
@@ -129,6 +130,8 @@ namespace wmd {
 void bind(const wmd_ctx *ctx);
 // The stream a device-buffer entry point runs on: the caller's (NULL = null stream).
 hipStream_t pick_stream(const wmd_ctx *ctx, void *stream);
+// The context's copy stream (non-blocking, default priority), created on first use.
+hipStream_t copy_stream(wmd_ctx *ctx);
 
 inline size_t round_up(size_t x, size_t m) { return (x + m - 1) / m * m; }
 

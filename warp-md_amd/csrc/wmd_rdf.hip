@@ -903,7 +903,7 @@ struct wmd_rdf {
     // host-chunk pipeline: two staging / device buffers, events guarding reuse
     HostBuf p_stage[2];
     DevBuf p_aos[2], p_soa[2];
-    Event p_copied[2], ev_boxes;
+    Event p_copied[2], p_used[2], ev_boxes;
     // grouped modes (centre / byres / bymol / no_intramol) and PlanarXY: the
     // IO "atoms" are group centres computed from full chunks; exclusions are
     // the no_intramol ordered pairs (u, v, weight) in IO positions
@@ -1173,6 +1173,7 @@ void accumulate_host(wmd_rdf *h, const float *coords, size_t n_src, size_t n_fra
                             std::to_string(n_src) + " atoms");
     }
     hipStream_t st = h->ctx->stream;
+    hipStream_t cs = copy_stream(h->ctx);
     const size_t stride = round_up(std::max(n, 1), 4);
     // frames per batch (~48 MB of AoS per staging buffer)
     const size_t fbytes = n_src * 16;
@@ -1180,20 +1181,30 @@ void accumulate_host(wmd_rdf *h, const float *coords, size_t n_src, size_t n_fra
     for (int b = 0; b < 2; ++b) {
         if (h->p_stage[b].bytes < batch * fbytes || h->p_soa[b].bytes < batch * 3 * stride * 4) {
             WMD_HIP_CHECK(hipStreamSynchronize(st));  // growing frees buffers queued work may use
+            WMD_HIP_CHECK(hipStreamSynchronize(cs));
             h->p_stage[b].reserve(batch * fbytes);
             h->p_aos[b].reserve(batch * fbytes);
             h->p_soa[b].reserve(batch * 3 * stride * 4);
         }
     }
+    // Three overlapping stages: host copies into pinned staging buffer b, the
+    // H2D copy of b on the copy stream, and the gather and kernels on the
+    // context's stream.  p_copied[b]: the H2D of b is done (the host may refill
+    // the staging buffer, the kernels may read the device copy); p_used[b]: the
+    // gather has read the device copy (the next H2D into it may start).
+    WMD_HIP_CHECK(hipEventRecord(h->p_used[0].get(), st));
+    WMD_HIP_CHECK(hipEventRecord(h->p_used[1].get(), st));
     size_t k = 0;
     for (size_t f0 = 0; f0 < n_frames; f0 += batch, ++k) {
         const int b = (int)(k & 1);
         const size_t nf = std::min(batch, n_frames - f0);
         h->p_copied[b].wait();  // the copy out of this staging buffer two batches ago is done
         host_copy(h->p_stage[b].ptr, coords + f0 * n_src * 4, nf * fbytes);
+        WMD_HIP_CHECK(hipStreamWaitEvent(cs, h->p_used[b].get(), 0));
         WMD_HIP_CHECK(hipMemcpyAsync(h->p_aos[b].ptr, h->p_stage[b].ptr, nf * fbytes,
-                                     hipMemcpyHostToDevice, st));
-        WMD_HIP_CHECK(hipEventRecord(h->p_copied[b].get(), st));
+                                     hipMemcpyHostToDevice, cs));
+        WMD_HIP_CHECK(hipEventRecord(h->p_copied[b].get(), cs));
+        WMD_HIP_CHECK(hipStreamWaitEvent(st, h->p_copied[b].get(), 0));
         float *d_soa = (float *)h->p_soa[b].ptr;
         int g;
         grid1d((long long)stride * nf, &g);
@@ -1208,6 +1219,7 @@ void accumulate_host(wmd_rdf *h, const float *coords, size_t n_src, size_t n_fra
                                (int)stride, (int)nf, d_soa, (int)h->planar);
         }
         WMD_HIP_CHECK(hipGetLastError());
+        WMD_HIP_CHECK(hipEventRecord(h->p_used[b].get(), st));
         accumulate_device(h, d_soa, stride, nf, boxes ? boxes + 9 * f0 : nullptr, st, true);
         if (h->n_excl > 0) {
             grid1d(h->n_excl * (long long)nf, &g);

@@ -450,7 +450,7 @@ struct wmd_rmsd {
     // host-chunk pipeline: two staging / device buffers, events guarding reuse
     HostBuf p_stage[2];
     DevBuf p_aos[2], p_soa[2];
-    Event p_copied[2];
+    Event p_copied[2], p_used[2];
 };
 
 namespace {
@@ -629,13 +629,16 @@ wmd_status wmd_rmsd_run(wmd_rmsd *h, const float *coords, size_t n_atoms, size_t
                     "selection index out of range for the chunk's atom count");
     bind(h->ctx);
     hipStream_t st = h->ctx->stream;
-    // two-stage pipeline: the host copies batch k + 1 into the other pinned
-    // staging buffer while the GPU runs batch k (all device work in order on st)
+    hipStream_t cs = copy_stream(h->ctx);
+    // three overlapping stages: host copies into pinned staging buffer b, the
+    // H2D copy of b on the copy stream, the gather and kernels on st
+    // (p_copied[b]: the H2D of b is done; p_used[b]: the gather has read it)
     const size_t fbytes = n_atoms * 16;
     const size_t batch = std::max<size_t>(1, std::min<size_t>(n_frames, (48ull << 20) / std::max<size_t>(fbytes, 1)));
     for (int b = 0; b < 2; ++b) {
         if (h->p_stage[b].bytes < batch * fbytes || h->p_soa[b].bytes < batch * 3 * h->stride * 4) {
             WMD_HIP_CHECK(hipStreamSynchronize(st));  // growing frees buffers queued work may use
+            WMD_HIP_CHECK(hipStreamSynchronize(cs));
             h->p_stage[b].reserve(batch * fbytes);
             h->p_aos[b].reserve(batch * fbytes);
             h->p_soa[b].reserve(batch * 3 * h->stride * 4);
@@ -645,21 +648,26 @@ wmd_status wmd_rmsd_run(wmd_rmsd *h, const float *coords, size_t n_atoms, size_t
         WMD_HIP_CHECK(hipStreamSynchronize(st));
     float *d_o = (float *)h->d_out32.reserve(n_frames * 4);
     double *d_e = out_err ? (double *)h->d_err.reserve(n_frames * 8) : nullptr;
+    WMD_HIP_CHECK(hipEventRecord(h->p_used[0].get(), st));
+    WMD_HIP_CHECK(hipEventRecord(h->p_used[1].get(), st));
     size_t k = 0;
     for (size_t f0 = 0; f0 < n_frames; f0 += batch, ++k) {
         const int b = (int)(k & 1);
         const size_t nf = std::min(batch, n_frames - f0);
         h->p_copied[b].wait();  // the copy out of this staging buffer two batches ago is done
         host_copy(h->p_stage[b].ptr, coords + f0 * n_atoms * 4, nf * fbytes);
+        WMD_HIP_CHECK(hipStreamWaitEvent(cs, h->p_used[b].get(), 0));
         WMD_HIP_CHECK(hipMemcpyAsync(h->p_aos[b].ptr, h->p_stage[b].ptr, nf * fbytes,
-                                     hipMemcpyHostToDevice, st));
-        WMD_HIP_CHECK(hipEventRecord(h->p_copied[b].get(), st));
+                                     hipMemcpyHostToDevice, cs));
+        WMD_HIP_CHECK(hipEventRecord(h->p_copied[b].get(), cs));
+        WMD_HIP_CHECK(hipStreamWaitEvent(st, h->p_copied[b].get(), 0));
         float *d_soa = (float *)h->p_soa[b].ptr;
         long long total = (long long)h->stride * nf;
         int g = (int)std::min<long long>(65536, (total + kBlock - 1) / kBlock);
         hipLaunchKernelGGL(rmsd_gather_kernel, dim3(g), dim3(kBlock), 0, st, (const float4 *)h->p_aos[b].ptr,
                            (long long)n_atoms, h->d_atoms.as<unsigned>(), (int)h->stride, (int)nf, d_soa);
         WMD_HIP_CHECK(hipGetLastError());
+        WMD_HIP_CHECK(hipEventRecord(h->p_used[b].get(), st));
         run_device(h, d_soa, h->stride, nf, d_o + f0, nullptr, d_e ? d_e + f0 : nullptr, nullptr, nullptr, st);
     }
     WMD_HIP_CHECK(hipMemcpyAsync(out_rmsd, d_o, n_frames * 4, hipMemcpyDeviceToHost, st));
