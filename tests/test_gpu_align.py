@@ -103,6 +103,61 @@ def test_reference_known_answers():
         AlignPlan([0, 1, 2]).run(Trajectory.from_numpy(c), _system(3), device="cuda:0")
 
 
+def _device_superpose(c, sel, ref, masses=None, norotate=False):
+    """wmd_align_run_device with every atom transformed: (rt64 [F][12], xyz [F][N][3])."""
+    import torch
+
+    F, n, _ = c.shape
+    ref4 = np.ones((n, 4), np.float32)
+    ref4[:, :3] = ref
+    x4 = np.ones((F, n, 4), np.float32)
+    x4[..., :3] = c
+    sel = np.ascontiguousarray(sel, np.uint32)
+    m = None if masses is None else np.ascontiguousarray(masses, np.float32)
+    ctx = _native.context(0)
+    h = ctypes.c_void_p()
+    check(lib().wmd_align_create(ctx.handle, ptr(sel), sel.size, ptr(m), ptr(ref4), n, int(norotate),
+                                 ctypes.byref(h)))
+    try:
+        d = torch.as_tensor(x4).cuda()
+        rt64 = torch.zeros((F, 12), dtype=torch.float64, device="cuda:0")
+        xyz = torch.zeros((F, n, 3), dtype=torch.float32, device="cuda:0")
+        check(lib().wmd_align_run_device(h, ctypes.c_void_p(d.data_ptr()), n, F,
+                                         ctypes.c_void_p(rt64.data_ptr()), None,
+                                         ctypes.c_void_p(xyz.data_ptr()), None))
+        torch.cuda.synchronize()
+        return rt64.cpu().numpy(), xyz.cpu().numpy()
+    finally:
+        lib().wmd_align_destroy(h)
+
+
+@pytest.mark.parametrize("n", [4097, 12288, 12289, 20000, 27648, 27649])
+def test_superpose_frame_sizes(n):
+    """Frames of 4,097 to 27,649 atoms with masses, a selection that repeats
+    atoms and spans the frame, and norotate, against the oracle."""
+    rng = np.random.default_rng(n)
+    F = 5
+    ref = rng.normal(scale=15.0, size=(n, 3)).astype(np.float32)
+    c = np.empty((F, n, 3), np.float32)
+    for f in range(F):
+        q = rng.normal(size=4)
+        q /= np.linalg.norm(q)
+        w, a, b, d = q
+        R = np.array([[1 - 2 * (b * b + d * d), 2 * (a * b - d * w), 2 * (a * d + b * w)],
+                      [2 * (a * b + d * w), 1 - 2 * (a * a + d * d), 2 * (b * d - a * w)],
+                      [2 * (a * d - b * w), 2 * (b * d + a * w), 1 - 2 * (a * a + b * b)]])
+        c[f] = (ref @ R.T + rng.uniform(-20, 20, 3) + rng.normal(scale=0.5, size=(n, 3))).astype(np.float32)
+    sel = np.concatenate([np.arange(3, n, 7), [5, 5, n - 1, n - 1, n // 2]])
+    masses = rng.uniform(1, 16, sel.size).astype(np.float32)
+    print(f"superpose n={n}", flush=True)
+    for m, norot in ((None, False), (masses, False), (None, True)):
+        rt64, xyz = _device_superpose(c, sel, ref, masses=m, norotate=norot)
+        _, ort64, oxyz = oracle.align(c, sel, ref, weights=m, norotate=norot, superpose=True)
+        assert np.max(np.abs(rt64[:, :9] - ort64[:, :9])) <= 1e-9
+        assert np.max(np.abs(rt64[:, 9:] - ort64[:, 9:])) <= 1e-6
+        assert _close_xyz(xyz, oxyz)
+
+
 def test_full_chain_superpose():
     """5,000 frames x 20,000 atoms, fit on 2,000 CA: every atom superposed."""
     import torch

@@ -24,8 +24,9 @@ using namespace wmd;
 
 constexpr int kAlBlock = 256;
 
-// Sum of `cnt` f64 values over the workgroup (256 threads); every thread gets the totals.
-template <int CNT>
+// Sum of `CNT` f64 values over a workgroup of WAVES waves; every thread gets
+// the totals (waves added in index order).
+template <int CNT, int WAVES>
 __device__ __forceinline__ void al_block_sum(double *v, double *red) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
@@ -37,7 +38,57 @@ __device__ __forceinline__ void al_block_sum(double *v, double *red) {
         for (int c = 0; c < CNT; ++c) red[w * CNT + c] = v[c];
     __syncthreads();
 #pragma unroll
-    for (int c = 0; c < CNT; ++c) v[c] = red[c] + red[CNT + c] + red[2 * CNT + c] + red[3 * CNT + c];
+    for (int c = 0; c < CNT; ++c) {
+        double t = red[c];
+        for (int q = 1; q < WAVES; ++q) t += red[q * CNT + c];
+        v[c] = t;
+    }
+}
+
+// From summed moments m (Sx[3], W, Sxy[9], Sxx): cx, H, R (QCP, or I for
+// norotate) and t = cy - R cx, written to frame f's row and to rt_s (R
+// row-major, t).  Called by one thread.
+__device__ __forceinline__ void al_fit_row(const double *m, bool weighted, int n, double rs0,
+                                           double rs1, double rs2, double ref_g, double cy0,
+                                           double cy1, double cy2, int norotate, long long f,
+                                           double *__restrict__ rt64, float *__restrict__ rt32,
+                                           double *rt_s) {
+    const double wsum = weighted ? m[3] : (double)n;
+    double cx[3] = {0.0, 0.0, 0.0};
+    if (wsum != 0.0)
+        for (int d = 0; d < 3; ++d) cx[d] = m[d] / wsum;
+    const double rs[3] = {rs0, rs1, rs2};
+    double H[10];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) H[3 * a + c] = m[4 + 3 * a + c] - cx[a] * rs[c];
+    H[9] = m[13] - wsum * (cx[0] * cx[0] + cx[1] * cx[1] + cx[2] * cx[2]);
+    double R[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    if (!norotate && n > 0) {
+        QcpResult res;
+        qcp_solve(H, H[9] > 0.0 ? H[9] : 0.0, ref_g, true, &res);
+        for (int i = 0; i < 9; ++i) R[i] = res.rot[i];
+    }
+    const double cy[3] = {cy0, cy1, cy2};
+    double out[12];
+    for (int i = 0; i < 9; ++i) out[i] = R[i];
+    for (int a = 0; a < 3; ++a)
+        out[9 + a] = cy[a] - (R[3 * a] * cx[0] + R[3 * a + 1] * cx[1] + R[3 * a + 2] * cx[2]);
+    for (int i = 0; i < 12; ++i) {
+        rt64[f * 12 + i] = out[i];
+        if (rt32) rt32[f * 12 + i] = (float)out[i];
+        rt_s[i] = out[i];
+    }
+}
+
+// R x + t for one atom, f64 in the reference's order (plan_exec.rs:399-405),
+// cast to f32.
+__device__ __forceinline__ void al_transform(const double *q, float x, float y, float z, float *o) {
+    const double px = (double)x, py = (double)y, pz = (double)z;
+    o[0] = (float)(q[0] * px + q[1] * py + q[2] * pz + q[9]);
+    o[1] = (float)(q[3] * px + q[4] * py + q[5] * pz + q[10]);
+    o[2] = (float)(q[6] * px + q[7] * py + q[8] * pz + q[11]);
 }
 
 // One workgroup per frame (grid-stride), one pass over the selection:
@@ -77,36 +128,10 @@ __global__ __launch_bounds__(kAlBlock) void al_fit_kernel(
                 for (int c = 0; c < 3; ++c) m[4 + 3 * a + c] += xv[a] * yr[c] * w;
             m[13] += (xv[0] * xv[0] + xv[1] * xv[1] + xv[2] * xv[2]) * w;
         }
-        al_block_sum<14>(m, red);
-        const double wsum = wts ? m[3] : (double)n;
-        double cx[3] = {0.0, 0.0, 0.0};
-        if (wsum != 0.0)
-            for (int d = 0; d < 3; ++d) cx[d] = m[d] / wsum;
-        const double rs[3] = {rs0, rs1, rs2};
-        double H[10];
-#pragma unroll
-        for (int a = 0; a < 3; ++a)
-#pragma unroll
-            for (int c = 0; c < 3; ++c) H[3 * a + c] = m[4 + 3 * a + c] - cx[a] * rs[c];
-        H[9] = m[13] - wsum * (cx[0] * cx[0] + cx[1] * cx[1] + cx[2] * cx[2]);
-        if (threadIdx.x == 0) {
-            double R[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
-            if (!norotate && n > 0) {
-                QcpResult res;
-                qcp_solve(H, H[9] > 0.0 ? H[9] : 0.0, ref_g, true, &res);
-                for (int i = 0; i < 9; ++i) R[i] = res.rot[i];
-            }
-            const double cy[3] = {cy0, cy1, cy2};
-            double out[12];
-            for (int i = 0; i < 9; ++i) out[i] = R[i];
-            for (int a = 0; a < 3; ++a)
-                out[9 + a] = cy[a] - (R[3 * a] * cx[0] + R[3 * a + 1] * cx[1] + R[3 * a + 2] * cx[2]);
-            for (int i = 0; i < 12; ++i) {
-                rt64[(long long)f * 12 + i] = out[i];
-                if (rt32) rt32[(long long)f * 12 + i] = (float)out[i];
-                rt_s[i] = out[i];
-            }
-        }
+        al_block_sum<14, kAlBlock / 64>(m, red);
+        if (threadIdx.x == 0)
+            al_fit_row(m, wts != nullptr, n, rs0, rs1, rs2, ref_g, cy0, cy1, cy2, norotate, f, rt64,
+                       rt32, rt_s);
         __syncthreads();
         if (xyz) {
             // SuperposePlan: every atom of this frame while its lines are still cached
@@ -126,10 +151,7 @@ __global__ __launch_bounds__(kAlBlock) void al_fit_kernel(
                 for (int u = 0; u < U; ++u) {
                     const long long a = a0 + (long long)u * kAlBlock;
                     if (a >= n_src) break;
-                    const double px = (double)p[u].x, py = (double)p[u].y, pz = (double)p[u].z;
-                    o[3 * a] = (float)(q[0] * px + q[1] * py + q[2] * pz + q[9]);
-                    o[3 * a + 1] = (float)(q[3] * px + q[4] * py + q[5] * pz + q[10]);
-                    o[3 * a + 2] = (float)(q[6] * px + q[7] * py + q[8] * pz + q[11]);
+                    al_transform(q, p[u].x, p[u].y, p[u].z, o + 3 * a);
                 }
             }
         }
