@@ -490,6 +490,8 @@ __global__ __launch_bounds__(kBlock) void cell_scan_kernel(const unsigned *__res
 constexpr int kBuildBlock = 1024;
 constexpr int kBuildMaxCells = 16384;
 
+constexpr int kBuildUnroll = 16;  // atoms per thread in flight in the per-frame build
+
 __device__ __forceinline__ unsigned block_exclusive_scan_1024(unsigned v, unsigned *part,
                                                               unsigned *total) {
     part[threadIdx.x] = v;
@@ -517,12 +519,26 @@ __device__ __forceinline__ void build_frame(const float *__restrict__ X, int str
     for (int c = threadIdx.x; c < ncell; c += kBuildBlock) cnt[c] = 0u;
     if (threadIdx.x == 0) *far = 0u;
     __syncthreads();
-    for (int i = threadIdx.x; i < n; i += kBuildBlock) {
-        unsigned img;
-        bool far_out;
-        const int cid = assign_atom<PBC>(X[i], X[stride + i], X[2 * stride + i], b, ncx, ncy, ncz, img, far_out);
-        atomicAdd(&cnt[cid], 1u);
-        if (far_out) *far = 1u;
+    // both passes take kBuildUnroll atoms per thread at a time, loads first,
+    // so that several loads of a thread are in flight together
+    for (int i0 = threadIdx.x; i0 < n; i0 += kBuildUnroll * kBuildBlock) {
+        float xs[kBuildUnroll], ys[kBuildUnroll], zs[kBuildUnroll];
+#pragma unroll
+        for (int u = 0; u < kBuildUnroll; ++u) {
+            const int i = min(i0 + u * kBuildBlock, n - 1);
+            xs[u] = X[i];
+            ys[u] = X[stride + i];
+            zs[u] = X[2 * stride + i];
+        }
+#pragma unroll
+        for (int u = 0; u < kBuildUnroll; ++u) {
+            if (i0 + u * kBuildBlock >= n) break;
+            unsigned img;
+            bool far_out;
+            const int cid = assign_atom<PBC>(xs[u], ys[u], zs[u], b, ncx, ncy, ncz, img, far_out);
+            atomicAdd(&cnt[cid], 1u);
+            if (far_out) *far = 1u;
+        }
     }
     __syncthreads();
     // exclusive scan of cnt: thread t owns cells [t * per, (t + 1) * per)
@@ -543,16 +559,28 @@ __device__ __forceinline__ void build_frame(const float *__restrict__ X, int str
         *frame_flag = *far;
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < n; i += kBuildBlock) {
-        const float x = X[i], y = X[stride + i], z = X[2 * stride + i];
-        unsigned img;
-        bool far_out;
-        const int cid = assign_atom<PBC>(x, y, z, b, ncx, ncy, ncz, img, far_out);
-        const unsigned pos = atomicAdd(&cnt[cid], 1u);
-        const unsigned w = wts ? wts[i] : 0x0101u;  // mA | mB << 8
-        const unsigned meta = (img & 0xFFFFFFu) | ((w & 0xFu) << 24) | (((w >> 8) & 0xFu) << 28);
-        sorted[pos] = make_float4(x, y, z, __uint_as_float(meta));
-        if (order) order[pos] = (unsigned)i;
+    for (int i0 = threadIdx.x; i0 < n; i0 += kBuildUnroll * kBuildBlock) {
+        float xs[kBuildUnroll], ys[kBuildUnroll], zs[kBuildUnroll];
+#pragma unroll
+        for (int u = 0; u < kBuildUnroll; ++u) {
+            const int i = min(i0 + u * kBuildBlock, n - 1);
+            xs[u] = X[i];
+            ys[u] = X[stride + i];
+            zs[u] = X[2 * stride + i];
+        }
+#pragma unroll
+        for (int u = 0; u < kBuildUnroll; ++u) {
+            const int i = i0 + u * kBuildBlock;
+            if (i >= n) break;
+            unsigned img;
+            bool far_out;
+            const int cid = assign_atom<PBC>(xs[u], ys[u], zs[u], b, ncx, ncy, ncz, img, far_out);
+            const unsigned pos = atomicAdd(&cnt[cid], 1u);
+            const unsigned w = wts ? wts[i] : 0x0101u;  // mA | mB << 8
+            const unsigned meta = (img & 0xFFFFFFu) | ((w & 0xFu) << 24) | (((w >> 8) & 0xFu) << 28);
+            sorted[pos] = make_float4(xs[u], ys[u], zs[u], __uint_as_float(meta));
+            if (order) order[pos] = (unsigned)i;
+        }
     }
 }
 
