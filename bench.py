@@ -523,7 +523,8 @@ def counter_sequence(ctx, dev):
     of FETCH_SIZE / WRITE_SIZE), one RDF accumulate of 100 frames of the
     100,000-water box (500 bins, r_max 15), one RMSD run over 5,000 frames of
     the 20,000-atom chain (fit on 2,000 CA), one 5,000 x 5,000 pairwise RMSD
-    matrix on the CA and one superposition of 5,000 x 20,000 atoms."""
+    matrix on the CA, one superposition of 5,000 x 20,000 atoms and one soft-RDF
+    forward and backward (20,000 beads x 64 frames)."""
     import numpy as np
     import torch
 
@@ -594,6 +595,26 @@ def counter_sequence(ctx, dev):
     torch.cuda.synchronize()
     L.wmd_align_destroy(ha)
     log("counters: superposition 5000 x 20000 done")
+    del x4, xyz
+    # soft RDF forward and backward: 20,000 beads at 0.08 / A^3, 64 frames,
+    # 300 bins, r_max 12, sigma 0.05 (the soft-RDF ratio's workload)
+    from warp_md_amd import autograd
+
+    n, frames = 20000, 64
+    box = (n / 0.08) ** (1.0 / 3.0)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(4)
+    base = torch.rand((n, 3), generator=gen, device=dev, dtype=torch.float32) * box
+    xs = base[None] + 0.2 * torch.randn((frames, n, 3), generator=gen, device=dev, dtype=torch.float32)
+    stride = (n + 3) // 4 * 4
+    soa = torch.zeros((frames, 3, stride), dtype=torch.float32, device=dev)
+    soa[:, :, :n] = xs.permute(0, 2, 1)
+    boxes = np.repeat(np.diag([box] * 3).reshape(1, 9), frames, 0).astype(np.float32)
+    plan = autograd.SoftRdf(bins=300, r_max=12.0, sigma=0.05, pbc="orthorhombic")
+    c = plan.forward_store(soa, stride, n, boxes)
+    plan.backward_store(soa, stride, n, boxes, torch.ones_like(c))
+    torch.cuda.synchronize()
+    log("counters: soft RDF forward + backward done")
 
 
 def uneven_batch_bench(ctx, dev, reps=3, autotune=True):
