@@ -617,7 +617,7 @@ def counter_sequence(ctx, dev):
     log("counters: soft RDF forward + backward done")
 
 
-def uneven_batch_bench(ctx, dev, reps=3, autotune=True):
+def uneven_batch_bench(ctx, dev, reps=3, autotune=True, cpu_sample=True):
     """256 trajectories, atom counts log-uniform in [500, 50000], cubic boxes at
     0.1 atoms/A^3, 50 uniform-random frames each; RDF (300 bins, r_max 12 A)
     and RMSD to frame 0 of every trajectory in one captured-graph step, against
@@ -690,7 +690,27 @@ def uneven_batch_bench(ctx, dev, reps=3, autotune=True):
         lib().wmd_rmsd_destroy(hr)
     log(f"uneven batch: {ms:.2f} ms per graph step ({units} pair units, "
         f"{int((kinds == 0).sum())} brute-force trajectories), 256 single-plan calls {single_ms:.2f} ms")
-    return {"ms": ms, "pair_evals_per_s": ref_evals / (ms / 1e3),
+    # the oracle's reference loop on a bounded sample: frame 0 of the
+    # trajectories at the 25th, 50th, 75th and 90th size percentiles
+    cpu = None
+    if cpu_sample:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+
+        order = np.argsort(ns, kind="stable")
+        picks = [int(order[int(q * (len(ns) - 1))]) for q in (0.25, 0.5, 0.75, 0.9)]
+        evals, secs = 0.0, 0.0
+        for t in picks:
+            n = int(ns[t])
+            x = b.store_view(store, t)[0, :, :n].t().contiguous().cpu().numpy()[None]
+            t0 = time.perf_counter()
+            oracle.rdf_counts(x, np.arange(n), np.arange(n), 300, 12.0, boxes=spec[t]["box"])
+            secs += time.perf_counter() - t0
+            evals += float(n) * (n - 1)
+        cpu = {"value": evals / secs, "unit": "pair-evals/s", "cores": 1, "kind": "port",
+               "sample": f"frame 0 of the trajectories of {', '.join(str(int(ns[t])) for t in picks)} atoms "
+                         f"({evals:.3g} pair-evals, {secs:.1f} s, 1 thread, brute force in the reference loop order)"}
+    return {"ms": ms, "pair_evals_per_s": ref_evals / (ms / 1e3), "cpu_baseline": cpu,
             "rmsd_frames_per_s": F * len(spec) / (ms / 1e3),
             "unordered_pairs_in_range": pairs_in,
             "single_plan_calls_ms": single_ms, "speedup_vs_single_calls": single_ms / ms,
@@ -879,7 +899,7 @@ def main():
     L.wmd_rdf_destroy(h_rdf)
     L.wmd_rmsd_destroy(h_rmsd)
     soft = None if args.no_soft else soft_rdf_ratio(ctx, dev)
-    batch = None if args.no_batch else uneven_batch_bench(ctx, dev)
+    batch = None if args.no_batch else uneven_batch_bench(ctx, dev, cpu_sample=not args.no_cpu)
     pairwise = None if args.no_pairwise else pairwise_bench(ctx, dev)
     superpose = None if args.no_align else align_bench(ctx, dev)
     ingest = None if args.no_ingest else host_ingest_bench(ctx)
