@@ -447,6 +447,17 @@ def host_ingest_bench(ctx, frames=96, rmsd_frames=1000, reps=2):
         check(L.wmd_rmsd_run(hr, ptr(cx4), nc, rmsd_frames, ptr(out), None))
     rmsd_s = (time.perf_counter() - t0) / reps
     L.wmd_rmsd_destroy(hr)
+    # SuperposePlan's host path: the same chain chunks in, every atom superposed out
+    ha = ctypes.c_void_p()
+    check(L.wmd_align_create(ctx.handle, ptr(c["ca"]), c["ca"].size, None, ptr(r4), nc, 0, ctypes.byref(ha)))
+    rt = np.zeros((rmsd_frames, 12), np.float32)
+    xyz = np.zeros((rmsd_frames, nc, 3), np.float32)
+    check(L.wmd_align_run(ha, ptr(cx4[:8]), nc, 8, ptr(rt[:8]), ptr(xyz[:8])))
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        check(L.wmd_align_run(ha, ptr(cx4), nc, rmsd_frames, ptr(rt), ptr(xyz)))
+    sup_s = (time.perf_counter() - t0) / reps
+    L.wmd_align_destroy(ha)
     pair_evals = frames * (float(ow.size) * ow.size - ow.size)
     res = {"rdf": {"frames": frames, "atoms_per_frame": N, "s": rdf_s,
                    "host_GB_per_s": frames * N * 16 / rdf_s / 1e9,
@@ -454,10 +465,13 @@ def host_ingest_bench(ctx, frames=96, rmsd_frames=1000, reps=2):
                    "rdfplan_run_s": plan_s},
            "rmsd": {"frames": rmsd_frames, "atoms_per_frame": nc, "s": rmsd_s,
                     "host_GB_per_s": rmsd_frames * nc * 16 / rmsd_s / 1e9,
-                    "frames_per_s": rmsd_frames / rmsd_s}}
+                    "frames_per_s": rmsd_frames / rmsd_s},
+           "superpose": {"frames": rmsd_frames, "atoms_per_frame": nc, "s": sup_s,
+                         "host_GB_per_s": rmsd_frames * nc * 28 / sup_s / 1e9}}
     log(f"host ingest: RDF {frames} x {N} atoms {rdf_s * 1e3:.1f} ms (RdfPlan.run {plan_s * 1e3:.1f} ms) "
         f"({res['rdf']['host_GB_per_s']:.1f} GB/s of chunks, {res['rdf']['pair_evals_per_s']:.3g} pair-evals/s); "
-        f"RMSD {rmsd_frames} x {nc} atoms {rmsd_s * 1e3:.1f} ms ({res['rmsd']['host_GB_per_s']:.1f} GB/s)")
+        f"RMSD {rmsd_frames} x {nc} atoms {rmsd_s * 1e3:.1f} ms ({res['rmsd']['host_GB_per_s']:.1f} GB/s); "
+        f"superpose {rmsd_frames} x {nc} atoms {sup_s * 1e3:.1f} ms ({res['superpose']['host_GB_per_s']:.1f} GB/s in + out)")
     return res
 
 

@@ -170,8 +170,11 @@ struct wmd_align {
     double cy[3] = {0, 0, 0};
     double ref_s[3] = {0, 0, 0};  // sum w (y - cy): zero up to rounding
     double ref_g = 0.0;
-    DevBuf d_sel, d_wts, d_refc, d_aos, d_rt64, d_rt32, d_xyz;
-    HostBuf h_stage;
+    DevBuf d_sel, d_wts, d_refc;
+    // host-chunk pipeline (wmd_align_run): two of each
+    DevBuf p_aos[2], p_rt64[2], p_rt32[2], p_xyz[2];
+    HostBuf p_in[2], p_out[2], p_rt[2];
+    Event p_copied[2], p_used[2], p_out_ready[2];
 };
 
 namespace {
@@ -288,25 +291,69 @@ wmd_status wmd_align_run(wmd_align *h, const float *coords_xyzw, size_t n_atoms,
     WMD_REQUIRE(coords_xyzw != nullptr, WMD_ERR_INVALID, "null coords");
     bind(h->ctx);
     hipStream_t st = h->ctx->stream;
-    const size_t fbytes = n_atoms * 16;
-    const size_t batch = std::max<size_t>(1, std::min<size_t>(n_frames, (256ull << 20) / std::max<size_t>(fbytes, 1)));
-    float4 *d_aos = (float4 *)h->d_aos.reserve(batch * fbytes);
-    double *d_rt64 = (double *)h->d_rt64.reserve(batch * 12 * 8);
-    float *d_rt32 = (float *)h->d_rt32.reserve(batch * 12 * 4);
-    float *d_xyz = out_xyz ? (float *)h->d_xyz.reserve(batch * n_atoms * 12) : nullptr;
-    void *stage = h->h_stage.reserve(batch * fbytes);
-    for (size_t f0 = 0; f0 < n_frames; f0 += batch) {
-        const size_t nf = std::min(batch, n_frames - f0);
-        WMD_HIP_CHECK(hipStreamSynchronize(st));
-        std::memcpy(stage, coords_xyzw + f0 * n_atoms * 4, nf * fbytes);
-        WMD_HIP_CHECK(hipMemcpyAsync(d_aos, stage, nf * fbytes, hipMemcpyHostToDevice, st));
-        al_enqueue(h, d_aos, n_atoms, nf, d_rt64, d_rt32, d_xyz, st);
-        if (out_rt)
-            WMD_HIP_CHECK(hipMemcpyAsync(out_rt + f0 * 12, d_rt32, nf * 12 * 4, hipMemcpyDeviceToHost, st));
-        if (out_xyz)
-            WMD_HIP_CHECK(hipMemcpyAsync(out_xyz + f0 * n_atoms * 3, d_xyz, nf * n_atoms * 12,
-                                         hipMemcpyDeviceToHost, st));
+    hipStream_t cs = copy_stream(h->ctx);
+    // Pipeline over batches of <= 48 MB of chunk: host copy into pinned
+    // staging b, its H2D on the copy stream, the fit (and transform) and the
+    // D2H of the results into pinned staging on the context's stream, and the
+    // host copy of batch k - 2's results out while batch k is staged.
+    // p_copied[b]: the H2D of b is done; p_used[b]: the kernel has read the
+    // device copy of b; p_out_ready[b]: b's results are in pinned memory.
+    const size_t fbytes = n_atoms * 16, obytes = n_atoms * 12;
+    const size_t batch = std::max<size_t>(1, std::min<size_t>(n_frames, (48ull << 20) / std::max<size_t>(fbytes, 1)));
+    bool grow = false;
+    for (int b = 0; b < 2; ++b)
+        grow |= h->p_in[b].bytes < batch * fbytes || h->p_rt[b].bytes < batch * 48 ||
+                (out_xyz && h->p_out[b].bytes < batch * obytes);
+    if (grow) {
+        WMD_HIP_CHECK(hipStreamSynchronize(st));  // growing frees buffers queued work may use
+        WMD_HIP_CHECK(hipStreamSynchronize(cs));
+        for (int b = 0; b < 2; ++b) {
+            h->p_in[b].reserve(batch * fbytes);
+            h->p_aos[b].reserve(batch * fbytes);
+            h->p_rt[b].reserve(batch * 48);
+            h->p_rt64[b].reserve(batch * 96);
+            h->p_rt32[b].reserve(batch * 48);
+            if (out_xyz) {
+                h->p_out[b].reserve(batch * obytes);
+                h->p_xyz[b].reserve(batch * obytes);
+            }
+        }
     }
+    WMD_HIP_CHECK(hipEventRecord(h->p_used[0].get(), st));
+    WMD_HIP_CHECK(hipEventRecord(h->p_used[1].get(), st));
+    bool pend[2] = {false, false};
+    size_t pend_f0[2] = {0, 0}, pend_nf[2] = {0, 0};
+    auto drain = [&](int b) {
+        if (!pend[b]) return;
+        h->p_out_ready[b].wait();
+        if (out_rt) std::memcpy(out_rt + pend_f0[b] * 12, h->p_rt[b].ptr, pend_nf[b] * 48);
+        if (out_xyz) host_copy(out_xyz + pend_f0[b] * n_atoms * 3, h->p_out[b].ptr, pend_nf[b] * obytes);
+        pend[b] = false;
+    };
+    size_t k = 0;
+    for (size_t f0 = 0; f0 < n_frames; f0 += batch, ++k) {
+        const int b = (int)(k & 1);
+        const size_t nf = std::min(batch, n_frames - f0);
+        h->p_copied[b].wait();  // the H2D out of this staging buffer two batches ago is done
+        drain(b);               // and that batch's results are copied out
+        host_copy(h->p_in[b].ptr, coords_xyzw + f0 * n_atoms * 4, nf * fbytes);
+        WMD_HIP_CHECK(hipStreamWaitEvent(cs, h->p_used[b].get(), 0));
+        WMD_HIP_CHECK(hipMemcpyAsync(h->p_aos[b].ptr, h->p_in[b].ptr, nf * fbytes, hipMemcpyHostToDevice, cs));
+        WMD_HIP_CHECK(hipEventRecord(h->p_copied[b].get(), cs));
+        WMD_HIP_CHECK(hipStreamWaitEvent(st, h->p_copied[b].get(), 0));
+        al_enqueue(h, (const float4 *)h->p_aos[b].ptr, n_atoms, nf, (double *)h->p_rt64[b].ptr,
+                   (float *)h->p_rt32[b].ptr, out_xyz ? (float *)h->p_xyz[b].ptr : nullptr, st);
+        WMD_HIP_CHECK(hipEventRecord(h->p_used[b].get(), st));
+        WMD_HIP_CHECK(hipMemcpyAsync(h->p_rt[b].ptr, h->p_rt32[b].ptr, nf * 48, hipMemcpyDeviceToHost, st));
+        if (out_xyz)
+            WMD_HIP_CHECK(hipMemcpyAsync(h->p_out[b].ptr, h->p_xyz[b].ptr, nf * obytes, hipMemcpyDeviceToHost, st));
+        WMD_HIP_CHECK(hipEventRecord(h->p_out_ready[b].get(), st));
+        pend[b] = true;
+        pend_f0[b] = f0;
+        pend_nf[b] = nf;
+    }
+    drain(0);
+    drain(1);
     WMD_HIP_CHECK(hipStreamSynchronize(st));
     WMD_API_END
 }
