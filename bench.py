@@ -458,6 +458,15 @@ def host_ingest_bench(ctx, frames=96, rmsd_frames=1000, reps=2):
         check(L.wmd_align_run(ha, ptr(cx4), nc, rmsd_frames, ptr(rt), ptr(xyz)))
     sup_s = (time.perf_counter() - t0) / reps
     L.wmd_align_destroy(ha)
+    # PairwiseRmsdPlan's host path: the chunks' CA appended to the device store
+    hp = ctypes.c_void_p()
+    check(L.wmd_pairwise_create(ctx.handle, ptr(c["ca"]), c["ca"].size, 0, ctypes.byref(hp)))
+    check(L.wmd_pairwise_append(hp, ptr(cx4[:8]), nc, 8))
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        check(L.wmd_pairwise_append(hp, ptr(cx4), nc, rmsd_frames))
+    app_s = (time.perf_counter() - t0) / reps
+    L.wmd_pairwise_destroy(hp)
     pair_evals = frames * (float(ow.size) * ow.size - ow.size)
     res = {"rdf": {"frames": frames, "atoms_per_frame": N, "s": rdf_s,
                    "host_GB_per_s": frames * N * 16 / rdf_s / 1e9,
@@ -467,11 +476,14 @@ def host_ingest_bench(ctx, frames=96, rmsd_frames=1000, reps=2):
                     "host_GB_per_s": rmsd_frames * nc * 16 / rmsd_s / 1e9,
                     "frames_per_s": rmsd_frames / rmsd_s},
            "superpose": {"frames": rmsd_frames, "atoms_per_frame": nc, "s": sup_s,
-                         "host_GB_per_s": rmsd_frames * nc * 28 / sup_s / 1e9}}
+                         "host_GB_per_s": rmsd_frames * nc * 28 / sup_s / 1e9},
+           "pairwise_append": {"frames": rmsd_frames, "atoms_per_frame": nc, "s": app_s,
+                               "host_GB_per_s": rmsd_frames * nc * 16 / app_s / 1e9}}
     log(f"host ingest: RDF {frames} x {N} atoms {rdf_s * 1e3:.1f} ms (RdfPlan.run {plan_s * 1e3:.1f} ms) "
         f"({res['rdf']['host_GB_per_s']:.1f} GB/s of chunks, {res['rdf']['pair_evals_per_s']:.3g} pair-evals/s); "
         f"RMSD {rmsd_frames} x {nc} atoms {rmsd_s * 1e3:.1f} ms ({res['rmsd']['host_GB_per_s']:.1f} GB/s); "
-        f"superpose {rmsd_frames} x {nc} atoms {sup_s * 1e3:.1f} ms ({res['superpose']['host_GB_per_s']:.1f} GB/s in + out)")
+        f"superpose {rmsd_frames} x {nc} atoms {sup_s * 1e3:.1f} ms ({res['superpose']['host_GB_per_s']:.1f} GB/s in + out); "
+        f"pairwise append {rmsd_frames} x {nc} atoms {app_s * 1e3:.1f} ms ({res['pairwise_append']['host_GB_per_s']:.1f} GB/s)")
     return res
 
 

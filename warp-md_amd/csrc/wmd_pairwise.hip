@@ -381,10 +381,12 @@ struct wmd_pairwise {
     int metric = WMD_PAIRWISE_RMS;
     int stride = 0;
     size_t frames = 0, cap = 0;
-    DevBuf d_sel, d_aos, d_cen, d_g, d_out, d_tiles;
+    DevBuf d_sel, d_cen, d_g, d_out, d_tiles;
+    DevBuf p_aos[2];  // host-chunk pipeline of wmd_pairwise_append
     DevBuf d_feat, d_centers, d_dist, d_labels, d_sums;
     std::unique_ptr<DevBuf> d_store;
-    HostBuf h_stage;
+    HostBuf p_stage[2];
+    Event p_copied[2], p_used[2];
     std::vector<int2> tiles;
 };
 
@@ -488,20 +490,38 @@ wmd_status wmd_pairwise_append(wmd_pairwise *h, const float *coords_xyzw, size_t
         h->d_store = std::move(nb);
         h->cap = cap;
     }
+    // pipeline as in the RDF host path: host copies into pinned staging b,
+    // H2D on the copy stream (p_copied[b]), gather into the store on st
+    // (p_used[b]: the gather has read the device copy of b)
+    hipStream_t cs = copy_stream(h->ctx);
     const size_t fbytes = n_atoms * 16;
-    const size_t batch = std::max<size_t>(1, std::min<size_t>(n_frames, (256ull << 20) / std::max<size_t>(fbytes, 1)));
-    float4 *d_aos = (float4 *)h->d_aos.reserve(batch * fbytes);
-    void *stage = h->h_stage.reserve(batch * fbytes);
-    for (size_t f0 = 0; f0 < n_frames; f0 += batch) {
+    const size_t batch = std::max<size_t>(1, std::min<size_t>(n_frames, (48ull << 20) / std::max<size_t>(fbytes, 1)));
+    if (h->p_stage[0].bytes < batch * fbytes || h->p_stage[1].bytes < batch * fbytes) {
+        WMD_HIP_CHECK(hipStreamSynchronize(st));  // growing frees buffers queued work may use
+        WMD_HIP_CHECK(hipStreamSynchronize(cs));
+        for (int b = 0; b < 2; ++b) {
+            h->p_stage[b].reserve(batch * fbytes);
+            h->p_aos[b].reserve(batch * fbytes);
+        }
+    }
+    WMD_HIP_CHECK(hipEventRecord(h->p_used[0].get(), st));
+    WMD_HIP_CHECK(hipEventRecord(h->p_used[1].get(), st));
+    size_t k = 0;
+    for (size_t f0 = 0; f0 < n_frames; f0 += batch, ++k) {
+        const int b = (int)(k & 1);
         const size_t nf = std::min(batch, n_frames - f0);
-        WMD_HIP_CHECK(hipStreamSynchronize(st));
-        std::memcpy(stage, coords_xyzw + f0 * n_atoms * 4, nf * fbytes);
-        WMD_HIP_CHECK(hipMemcpyAsync(d_aos, stage, nf * fbytes, hipMemcpyHostToDevice, st));
+        h->p_copied[b].wait();  // the H2D out of this staging buffer two batches ago is done
+        host_copy(h->p_stage[b].ptr, coords_xyzw + f0 * n_atoms * 4, nf * fbytes);
+        WMD_HIP_CHECK(hipStreamWaitEvent(cs, h->p_used[b].get(), 0));
+        WMD_HIP_CHECK(hipMemcpyAsync(h->p_aos[b].ptr, h->p_stage[b].ptr, nf * fbytes, hipMemcpyHostToDevice, cs));
+        WMD_HIP_CHECK(hipEventRecord(h->p_copied[b].get(), cs));
+        WMD_HIP_CHECK(hipStreamWaitEvent(st, h->p_copied[b].get(), 0));
         float *dst = (float *)h->d_store->ptr + (h->frames + f0) * fl;
         hipLaunchKernelGGL(pw_gather_kernel, dim3(pw_grid((long long)h->stride * nf)), dim3(kPwBlock), 0,
-                           st, (const float4 *)d_aos, (long long)n_atoms, h->d_sel.as<unsigned>(),
+                           st, (const float4 *)h->p_aos[b].ptr, (long long)n_atoms, h->d_sel.as<unsigned>(),
                            (int)h->sel.size(), h->stride, (int)nf, dst);
         WMD_HIP_CHECK(hipGetLastError());
+        WMD_HIP_CHECK(hipEventRecord(h->p_used[b].get(), st));
     }
     WMD_HIP_CHECK(hipStreamSynchronize(st));
     h->frames += n_frames;
