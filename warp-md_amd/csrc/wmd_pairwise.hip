@@ -143,11 +143,15 @@ __device__ __forceinline__ void pw_store(double *panel, const float4 *v, int n, 
             if (k + 2 < n) v2 = (double)v[e].z - c;
             if (k + 3 < n) v3 = (double)v[e].w - c;
         }
+        // slots swizzled by row: rows with bit 3 set keep k % 4 = 0, 1 in slots
+        // 2, 3 (and 2, 3 in 0, 1), so that the 32 lanes of an operand read
+        // (16 rows x 2 k) fall on 64 distinct banks
         double *p = panel + (q * kPwS4 + r) * 4;
-        p[0] = v0;
-        p[1] = v1;
-        p[2] = v2;
-        p[3] = v3;
+        const int sw = ((r >> 3) & 1) << 1;
+        p[0 ^ sw] = v0;
+        p[1 ^ sw] = v1;
+        p[2 ^ sw] = v2;
+        p[3 ^ sw] = v3;
     }
 }
 
@@ -178,6 +182,7 @@ __global__ __launch_bounds__(kPwBlock) void pw_gram_kernel(
 #pragma unroll
         for (int j = 0; j < 3; ++j) acc[m][j] = (d4){0.0, 0.0, 0.0, 0.0};
     const int lr = lane & 15, lk = lane >> 4;
+    const int lsw = ((lr >> 3) & 1) << 1;  // the staging's slot swizzle (rows 16 b + lr)
     float4 va[kPwPer], vb[kPwPer];
     pw_load(va, soa, stride, n, rows, ra, 0);
     pw_load(vb, soa, stride, n, rows, rb, 0);
@@ -194,9 +199,9 @@ __global__ __launch_bounds__(kPwBlock) void pw_gram_kernel(
         for (int q = 0; q < kPwQuads; ++q) {
             double av[3], bv[3];
 #pragma unroll
-            for (int m = 0; m < 3; ++m) av[m] = pa[(q * kPwS4 + 16 * (mb0 + m) + lr) * 4 + lk];
+            for (int m = 0; m < 3; ++m) av[m] = pa[(q * kPwS4 + 16 * (mb0 + m) + lr) * 4 + (lk ^ lsw)];
 #pragma unroll
-            for (int j = 0; j < 3; ++j) bv[j] = pb[(q * kPwS4 + 16 * (nb0 + j) + lr) * 4 + lk];
+            for (int j = 0; j < 3; ++j) bv[j] = pb[(q * kPwS4 + 16 * (nb0 + j) + lr) * 4 + (lk ^ lsw)];
 #pragma unroll
             for (int m = 0; m < 3; ++m)
 #pragma unroll
