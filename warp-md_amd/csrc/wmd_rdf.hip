@@ -999,7 +999,11 @@ void launch_cells_t(wmd_rdf *h, hipStream_t st, const float4 *sorted, const unsi
     unsigned long long *queue = (unsigned long long *)h->d_queue.reserve(8 * 8);
     hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(kBlock), 0, st, queue, 8LL);
     WMD_HIP_CHECK(hipGetLastError());
-    CellGrid G{sorted, start, frame_flag, boxes, n, nf, nc[0], nc[1], nc[2], k, shift_ok, 8, cull2, queue};
+    // a larger grab halves the traffic on each XCD's queue counter (which
+    // otherwise stalls the waves of a long launch); short launches keep
+    // grabs of 2 so that the last grabs stay balanced
+    const int grab = n_units >= 64LL * grid * kCellWaves ? 4 : 2;
+    CellGrid G{sorted, start, frame_flag, boxes, n, nf, nc[0], nc[1], nc[2], k, shift_ok, 8, cull2, queue, grab};
     hipLaunchKernelGGL(kern, dim3(grid), dim3(kCellBlock), lds, st, G, p);
     WMD_HIP_CHECK(hipGetLastError());
 }

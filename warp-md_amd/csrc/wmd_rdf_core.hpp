@@ -821,6 +821,7 @@ struct CellGrid {
     int n, n_frames, ncx, ncy, ncz, k, shift_ok, n_xcd;
     float cull2;                // squared cull radius (pair reach plus rounding margin)
     unsigned long long *queue;  // [n_xcd] zeroed grab counters (dynamic schedule), or null
+    int grab;                   // units per grab of the dynamic schedule (0: kCellGrab)
 };
 
 // Dispatches one chunk to the consumer's pair loop for its image mode.
@@ -1077,9 +1078,9 @@ __device__ __forceinline__ void cell_unit(const CellGrid &G, int f, int h, const
 // The unit loop of one wave over a batch of frames.  XCD-aware: workgroup g
 // runs on XCD g % n_xcd and works on that XCD's contiguous slice of the
 // units.  Static schedule (G.queue null): the slice's units are dealt to its
-// waves round robin.  Dynamic schedule: each wave takes kCellGrab
-// consecutive units at a time from the slice's atomic counter until the
-// counter passes the slice end (every wave reaches that exit), which evens
+// waves round robin.  Dynamic schedule: each wave takes G.grab (default
+// kCellGrab) consecutive units at a time from the slice's atomic counter until
+// the counter passes the slice end (every wave reaches that exit), which evens
 // out units of uneven cost.
 constexpr int kCellGrab = 2;
 
@@ -1095,14 +1096,15 @@ __device__ __forceinline__ void cell_waves(const CellGrid &G, float4 *ring, unsi
     const long long u_hi = n_units * (xcd + 1) / n_xcd;
     if (G.queue) {
         const int lane = threadIdx.x & 63;
+        const int grab = G.grab > 0 ? G.grab : kCellGrab;
         while (true) {
             unsigned long long g0 = 0;
-            if (lane == 0) g0 = atomicAdd(&G.queue[xcd], (unsigned long long)kCellGrab);
+            if (lane == 0) g0 = atomicAdd(&G.queue[xcd], (unsigned long long)grab);
             const unsigned glo = __builtin_amdgcn_readfirstlane((unsigned)g0);
             const unsigned ghi = __builtin_amdgcn_readfirstlane((unsigned)(g0 >> 32));
             const long long u0 = u_lo + (long long)(((unsigned long long)ghi << 32) | glo);
             if (u0 >= u_hi) break;
-            const long long u1 = min(u0 + (long long)kCellGrab, u_hi);
+            const long long u1 = min(u0 + (long long)grab, u_hi);
             for (long long u = u0; u < u1; ++u) {
                 const int f = (int)(u / ncell);
                 const int h = (int)(u - (long long)f * ncell);
