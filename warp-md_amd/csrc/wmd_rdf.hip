@@ -1794,7 +1794,7 @@ void soft_run(wmd_softrdf *h, bool backward, const float *d_soa, size_t stride, 
         WMD_HIP_CHECK(hipGetLastError());
         CellGrid G{h->cells.sorted.as<float4>(), (const unsigned *)h->cells.start.ptr,
                    (const unsigned *)h->cells.flag.ptr, bx, n, nf, pl.nc[0], pl.nc[1], pl.nc[2], pl.k,
-                   shift_ok, 8, cull2, queue};
+                   shift_ok, 8, cull2, queue, 1};
         if (backward) {
             // per-batch gradient in cell order, scattered to the store afterwards
             sp.grad = (float *)h->d_gsorted.reserve((size_t)3 * n * nf * 4);
