@@ -419,7 +419,29 @@ def host_ingest_bench(ctx, frames=96, rmsd_frames=1000, reps=2):
     for _ in range(reps):
         check(L.wmd_rdf_accumulate(h, ptr(x4), N, frames, ptr(boxes)))
     rdf_s = (time.perf_counter() - t0) / reps
+    host_counts = np.zeros(500, np.uint64)
+    check(L.wmd_rdf_read(h, ptr(host_counts), None))
     L.wmd_rdf_destroy(h)
+    # The pipeline's counts (the 8 distinct frames once in the warm-up, then
+    # frames / 8 times per rep) must equal the device-resident path's counts
+    # of the same 8 frames times that multiplicity, bit for bit.
+    import torch
+    stride = (ow.size + 3) // 4 * 4
+    st8 = torch.zeros((8, 3, stride), dtype=torch.float32, device="cuda:0")
+    st8[:, :, :ow.size] = torch.as_tensor(np.ascontiguousarray(w["coords"][:, ow].transpose(0, 2, 1))).cuda()
+    hd = ctypes.c_void_p()
+    check(L.wmd_rdf_create(ctx.handle, ptr(ow), ow.size, ptr(ow), ow.size, 500, ctypes.c_float(15.0),
+                           1, 0, ctypes.byref(hd)))
+    dev_counts = np.zeros(500, np.uint64)
+    check(L.wmd_rdf_accumulate_device(hd, ctypes.c_void_p(st8.data_ptr()), stride, 8, ptr(boxes[:8]),
+                                      ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))
+    check(L.wmd_rdf_read(hd, ptr(dev_counts), None))
+    L.wmd_rdf_destroy(hd)
+    del st8
+    mult = 1 + reps * (frames // 8)
+    rdf_match = frames % 8 == 0 and np.array_equal(host_counts, dev_counts * np.uint64(mult))
+    if not rdf_match:
+        raise AssertionError("host-ingest RDF counts differ from the device-resident path")
     # the same through the Python plan API (Trajectory holds the FrameChunk layout)
     from warp_md_amd import RdfPlan, System, Trajectory
     traj = Trajectory(x4, boxes)
@@ -446,6 +468,27 @@ def host_ingest_bench(ctx, frames=96, rmsd_frames=1000, reps=2):
     for _ in range(reps):
         check(L.wmd_rmsd_run(hr, ptr(cx4), nc, rmsd_frames, ptr(out), None))
     rmsd_s = (time.perf_counter() - t0) / reps
+    # the pipeline's RMSD series against the device-resident path (same kernels
+    # on a store built on the host; only the tiling of the sums may differ)
+    n_store = ctypes.c_size_t()
+    rstride = ctypes.c_size_t()
+    check(L.wmd_rmsd_store_layout(hr, None, 0, ctypes.byref(n_store), ctypes.byref(rstride)))
+    slots = np.zeros(rstride.value, np.uint32)
+    check(L.wmd_rmsd_store_layout(hr, ptr(slots), slots.size, ctypes.byref(n_store), ctypes.byref(rstride)))
+    valid = slots != 0xFFFFFFFF
+    host_store = np.zeros((rmsd_frames, 3, slots.size), np.float32)
+    host_store[:, :, valid] = np.transpose(c["coords"][:, slots[valid].astype(np.int64)], (0, 2, 1))
+    dstore = torch.as_tensor(host_store).cuda()
+    del host_store
+    dout = torch.zeros(rmsd_frames, dtype=torch.float32, device="cuda:0")
+    check(L.wmd_rmsd_run_device(hr, ctypes.c_void_p(dstore.data_ptr()), slots.size, rmsd_frames,
+                                ctypes.c_void_p(dout.data_ptr()), None, None, None, None,
+                                ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))
+    torch.cuda.synchronize()
+    rmsd_diff = float(np.max(np.abs(dout.cpu().numpy().astype(np.float64) - out.astype(np.float64))))
+    del dstore, dout
+    if not rmsd_diff <= 2e-6:
+        raise AssertionError(f"host-ingest RMSD differs from the device-resident path by {rmsd_diff:.3e} A")
     L.wmd_rmsd_destroy(hr)
     # SuperposePlan's host path: the same chain chunks in, every atom superposed out
     ha = ctypes.c_void_p()
@@ -471,8 +514,10 @@ def host_ingest_bench(ctx, frames=96, rmsd_frames=1000, reps=2):
     res = {"rdf": {"frames": frames, "atoms_per_frame": N, "s": rdf_s,
                    "host_GB_per_s": frames * N * 16 / rdf_s / 1e9,
                    "pair_evals_per_s": pair_evals / rdf_s,
-                   "rdfplan_run_s": plan_s},
+                   "rdfplan_run_s": plan_s,
+                   "counts_equal_device_path": bool(rdf_match)},
            "rmsd": {"frames": rmsd_frames, "atoms_per_frame": nc, "s": rmsd_s,
+                    "max_abs_diff_vs_device_path_A": rmsd_diff,
                     "host_GB_per_s": rmsd_frames * nc * 16 / rmsd_s / 1e9,
                     "frames_per_s": rmsd_frames / rmsd_s},
            "superpose": {"frames": rmsd_frames, "atoms_per_frame": nc, "s": sup_s,

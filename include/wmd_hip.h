@@ -87,6 +87,15 @@ wmd_status wmd_ctx_synchronize(wmd_ctx *ctx);
 void *wmd_ctx_stream(wmd_ctx *ctx);
 /* Number of compute units of the context's device. */
 int wmd_ctx_cu_count(const wmd_ctx *ctx);
+/* Size of each of the two pinned staging buffers through which the host-chunk
+ * entry points (wmd_rdf_accumulate[_selected], wmd_rmsd_run, wmd_align_run,
+ * wmd_pairwise_append) stream a chunk to the device, a batch of frames at a
+ * time: the engine's side of the reference's chunk streaming
+ * (crates/traj-engine/src/executor.rs:486-509, chunk sizes from
+ * crates/traj-py/src/pack.rs:1447-1466).  bytes = 0 restores the default,
+ * 48 MiB; a batch holds at least one frame.  *previous (may be NULL) gets the
+ * size in force before the call.  Results do not depend on it. */
+wmd_status wmd_ctx_set_staging_bytes(wmd_ctx *ctx, size_t bytes, size_t *previous);
 /* Kernel timing with HIP events recorded around the engine's main kernels on
  * the stream they run on (enable = 1 starts recording and clears old marks).
  * wmd_ctx_profile_read synchronises and returns, for kernel family `name`

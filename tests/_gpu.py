@@ -19,6 +19,8 @@ def boxes9(box, n_frames, pbc):
     b = np.asarray(box, np.float32)
     if b.shape == (3,):
         b = np.diag(b).reshape(1, 9)
+    elif b.shape == (n_frames, 3) and pbc != "triclinic":
+        b = np.stack([np.diag(r).reshape(9) for r in b])
     elif b.shape == (3, 3):
         b = b.reshape(1, 9)
     elif b.ndim == 2 and b.shape[1] == 3:

@@ -292,14 +292,15 @@ wmd_status wmd_align_run(wmd_align *h, const float *coords_xyzw, size_t n_atoms,
     bind(h->ctx);
     hipStream_t st = h->ctx->stream;
     hipStream_t cs = copy_stream(h->ctx);
-    // Pipeline over batches of <= 48 MB of chunk: host copy into pinned
+    // Pipeline over batches of one staging buffer of chunk (48 MB unless the
+    // context sets another size): host copy into pinned
     // staging b, its H2D on the copy stream, the fit (and transform) and the
     // D2H of the results into pinned staging on the context's stream, and the
     // host copy of batch k - 2's results out while batch k is staged.
     // p_copied[b]: the H2D of b is done; p_used[b]: the kernel has read the
     // device copy of b; p_out_ready[b]: b's results are in pinned memory.
     const size_t fbytes = n_atoms * 16, obytes = n_atoms * 12;
-    const size_t batch = std::max<size_t>(1, std::min<size_t>(n_frames, (48ull << 20) / std::max<size_t>(fbytes, 1)));
+    const size_t batch = staging_frames(h->ctx, fbytes, n_frames);
     bool grow = false;
     for (int b = 0; b < 2; ++b)
         grow |= h->p_in[b].bytes < batch * fbytes || h->p_rt[b].bytes < batch * 48 ||

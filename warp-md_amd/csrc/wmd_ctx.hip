@@ -246,6 +246,14 @@ void *wmd_ctx_stream(wmd_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr;
 
 int wmd_ctx_cu_count(const wmd_ctx *ctx) { return ctx ? ctx->cu_count : 0; }
 
+wmd_status wmd_ctx_set_staging_bytes(wmd_ctx *ctx, size_t bytes, size_t *previous) {
+    WMD_API_BEGIN
+    WMD_REQUIRE(ctx != nullptr, WMD_ERR_INVALID, "null context");
+    if (previous) *previous = ctx->staging_bytes;
+    ctx->staging_bytes = bytes ? bytes : (size_t(48) << 20);
+    WMD_API_END
+}
+
 wmd_status wmd_ctx_profile(wmd_ctx *ctx, int enable) {
     WMD_API_BEGIN
     WMD_REQUIRE(ctx != nullptr, WMD_ERR_INVALID, "null context");

@@ -115,6 +115,8 @@ struct wmd_ctx {
     int xcd_count = 8;
     size_t lds_per_block = 0;
     std::string arch;
+    // bytes of host chunk per pinned staging buffer of the host-chunk pipelines
+    size_t staging_bytes = size_t(48) << 20;
     // kernel timing (wmd_ctx_profile)
     bool profile = false;
     struct Mark {
@@ -134,6 +136,13 @@ hipStream_t pick_stream(const wmd_ctx *ctx, void *stream);
 hipStream_t copy_stream(wmd_ctx *ctx);
 
 inline size_t round_up(size_t x, size_t m) { return (x + m - 1) / m * m; }
+
+// Frames per batch of a host-chunk pipeline: as many frames of `frame_bytes`
+// as fit one staging buffer of the context (at least one, at most n_frames).
+inline size_t staging_frames(const wmd_ctx *ctx, size_t frame_bytes, size_t n_frames) {
+    const size_t per = ctx->staging_bytes / (frame_bytes ? frame_bytes : 1);
+    return per < 1 ? 1 : (per < n_frames ? per : (n_frames ? n_frames : 1));
+}
 
 // Records HIP events around a launch sequence on `st` when profiling is on.
 struct ProfScope {

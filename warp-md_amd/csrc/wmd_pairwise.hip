@@ -500,7 +500,7 @@ wmd_status wmd_pairwise_append(wmd_pairwise *h, const float *coords_xyzw, size_t
     // (p_used[b]: the gather has read the device copy of b)
     hipStream_t cs = copy_stream(h->ctx);
     const size_t fbytes = n_atoms * 16;
-    const size_t batch = std::max<size_t>(1, std::min<size_t>(n_frames, (48ull << 20) / std::max<size_t>(fbytes, 1)));
+    const size_t batch = staging_frames(h->ctx, fbytes, n_frames);
     if (h->p_stage[0].bytes < batch * fbytes || h->p_stage[1].bytes < batch * fbytes) {
         WMD_HIP_CHECK(hipStreamSynchronize(st));  // growing frees buffers queued work may use
         WMD_HIP_CHECK(hipStreamSynchronize(cs));

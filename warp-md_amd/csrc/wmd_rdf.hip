@@ -1176,12 +1176,20 @@ void accumulate_host(wmd_rdf *h, const float *coords, size_t n_src, size_t n_fra
                         "selection index " + std::to_string(a) + " out of range for " +
                             std::to_string(n_src) + " atoms");
     }
+    // every frame's box is checked before anything is enqueued, so that a bad
+    // box in a later batch leaves the counts untouched (the reference checks
+    // the whole chunk's boxes before its pair loop, rdf.rs:746-788)
+    if (h->pbc != WMD_PBC_NONE) {
+        WMD_REQUIRE(boxes != nullptr, WMD_ERR_INVALID, "boxes required for PBC");
+        (void)make_boxes(h->pbc, boxes, n_frames);
+    }
     hipStream_t st = h->ctx->stream;
     hipStream_t cs = copy_stream(h->ctx);
     const size_t stride = round_up(std::max(n, 1), 4);
-    // frames per batch (~48 MB of AoS per staging buffer)
+    // frames per batch: one staging buffer of AoS (48 MB unless the context
+    // sets another size)
     const size_t fbytes = n_src * 16;
-    const size_t batch = std::max<size_t>(1, std::min<size_t>(n_frames, (48ull << 20) / std::max<size_t>(fbytes, 1)));
+    const size_t batch = staging_frames(h->ctx, fbytes, n_frames);
     for (int b = 0; b < 2; ++b) {
         if (h->p_stage[b].bytes < batch * fbytes || h->p_soa[b].bytes < batch * 3 * stride * 4) {
             WMD_HIP_CHECK(hipStreamSynchronize(st));  // growing frees buffers queued work may use

@@ -634,7 +634,7 @@ wmd_status wmd_rmsd_run(wmd_rmsd *h, const float *coords, size_t n_atoms, size_t
     // H2D copy of b on the copy stream, the gather and kernels on st
     // (p_copied[b]: the H2D of b is done; p_used[b]: the gather has read it)
     const size_t fbytes = n_atoms * 16;
-    const size_t batch = std::max<size_t>(1, std::min<size_t>(n_frames, (48ull << 20) / std::max<size_t>(fbytes, 1)));
+    const size_t batch = staging_frames(h->ctx, fbytes, n_frames);
     for (int b = 0; b < 2; ++b) {
         if (h->p_stage[b].bytes < batch * fbytes || h->p_soa[b].bytes < batch * 3 * h->stride * 4) {
             WMD_HIP_CHECK(hipStreamSynchronize(st));  // growing frees buffers queued work may use

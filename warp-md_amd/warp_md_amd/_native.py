@@ -52,7 +52,7 @@ _PREFIX = {
 SYMBOLS = (
     "wmd_last_error", "wmd_version", "wmd_device_count",
     "wmd_ctx_create", "wmd_ctx_destroy", "wmd_ctx_synchronize", "wmd_ctx_stream",
-    "wmd_ctx_cu_count", "wmd_ctx_profile", "wmd_ctx_profile_read",
+    "wmd_ctx_cu_count", "wmd_ctx_set_staging_bytes", "wmd_ctx_profile", "wmd_ctx_profile_read",
     "wmd_rdf_create", "wmd_rdf_io_selection", "wmd_rdf_accumulate",
     "wmd_rdf_accumulate_selected", "wmd_rdf_accumulate_device", "wmd_rdf_read", "wmd_rdf_pair_evaluations",
     "wmd_rdf_reset", "wmd_rdf_destroy", "wmd_rdf_last_path", "wmd_rdf_set_tuning",
@@ -125,6 +125,7 @@ def lib():
             "wmd_ctx_synchronize": ([P], St),
             "wmd_ctx_stream": ([P], P),
             "wmd_ctx_cu_count": ([P], I),
+            "wmd_ctx_set_staging_bytes": ([P, S, ctypes.POINTER(S)], St),
             "wmd_ctx_profile": ([P, I], St),
             "wmd_ctx_profile_read": ([P, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),
                                       ctypes.POINTER(I)], St),
@@ -236,6 +237,13 @@ class Context:
 
     def synchronize(self):
         check(lib().wmd_ctx_synchronize(self.handle))
+
+    def set_staging_bytes(self, nbytes: int) -> int:
+        """Sets the size of the host-chunk pipelines' staging buffers (0 = the
+        48 MiB default); returns the previous size."""
+        prev = ctypes.c_size_t()
+        check(lib().wmd_ctx_set_staging_bytes(self.handle, int(nbytes), ctypes.byref(prev)))
+        return int(prev.value)
 
     def profile(self, enable: bool = True):
         """Start (clearing old marks) or stop HIP-event timing of the engine's kernels."""

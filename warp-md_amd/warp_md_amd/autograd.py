@@ -46,15 +46,17 @@ def _boxes9(box, n_frames, pbc):
     if pbc == "none":
         return None, None
     b = np.asarray(box.detach().cpu().numpy() if hasattr(box, "detach") else box, np.float64)
+    # per-frame shapes are tested before the shared (3, 3) matrix, so that
+    # three frames of box lengths are never read as one box matrix
     if b.shape == (3,):
         m = np.tile(np.diag(b).reshape(1, 9), (n_frames, 1))
-    elif b.shape == (3, 3):
-        m = np.tile(b.reshape(1, 9), (n_frames, 1))
-    elif b.shape == (n_frames, 3):
+    elif b.shape == (n_frames, 3) and pbc != "triclinic":
         m = np.zeros((n_frames, 9))
         m[:, 0], m[:, 4], m[:, 8] = b[:, 0], b[:, 1], b[:, 2]
     elif b.shape == (n_frames, 3, 3):
         m = b.reshape(n_frames, 9)
+    elif b.shape == (3, 3):
+        m = np.tile(b.reshape(1, 9), (n_frames, 1))
     else:
         raise RuntimeError(f"mismatch: box shape {b.shape} for {n_frames} frames")
     vol = np.abs(np.linalg.det(m.reshape(-1, 3, 3)))

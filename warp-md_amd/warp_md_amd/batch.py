@@ -28,11 +28,16 @@ from .executor import Device
 from .plans import _parse_pbc, finalize_rdf
 
 
-def _boxes9(box, n_frames: int) -> np.ndarray:
-    """(F, 9) row-major boxes from lengths (3,), (F, 3), a (3, 3) matrix or (F, 3, 3)."""
+def _boxes9(box, n_frames: int, pbc: int = _native.PBC_ORTHO) -> np.ndarray:
+    """(F, 9) row-major boxes from lengths (3,), (F, 3), a (3, 3) matrix or (F, 3, 3).
+
+    A (3, 3) array is per-frame lengths when there are three frames and the
+    boxes are orthorhombic, else one box matrix shared by every frame."""
     b = np.asarray(box, np.float32)
     if b.shape == (3,):
         b = np.diag(b).reshape(1, 9)
+    elif b.shape == (n_frames, 3) and pbc != _native.PBC_TRICLINIC:
+        b = np.stack([np.diag(r).reshape(9) for r in b])
     elif b.shape == (3, 3):
         b = b.reshape(1, 9)
     elif b.ndim == 2 and b.shape[1] == 3:
@@ -119,7 +124,7 @@ class UnevenBatch:
             return
         if len(boxes_list) != self.n_traj:
             raise RuntimeError("mismatch: one box spec per trajectory required")
-        parts = [_boxes9(b, int(F)) for b, F in zip(boxes_list, self.n_frames)]
+        parts = [_boxes9(b, int(F), self.pbc) for b, F in zip(boxes_list, self.n_frames)]
         self.boxes = (np.ascontiguousarray(np.concatenate(parts)) if parts
                       else np.zeros((0, 9), np.float32))
         check(lib().wmd_batch_set_boxes(self.handle, ptr(self.boxes)))
