@@ -153,7 +153,8 @@ def test_rdf_grouped_many_batches(mode):
         Trajectory.from_numpy(c, box=box), system, device="cuda:0")
     ga = rdf_groups(sel_a, system, False, kw.get("byres1", False), False)
     gb = rdf_groups(sel_b, system, False, kw.get("byres2", False), False)
-    ref = oracle.rdf_grouped(c, ga, gb, 90, 9.0, "orthorhombic", box, None, kw.get("no_intramol", False))
+    masses = np.asarray(system.atoms["mass"], np.float32)  # RdfPlan's default mass_weighted=True
+    ref = oracle.rdf_grouped(c, ga, gb, 90, 9.0, "orthorhombic", box, masses, kw.get("no_intramol", False))
     np.testing.assert_array_equal(got, ref)
     np.testing.assert_array_equal(got, one)
 
