@@ -239,6 +239,24 @@ wmd_status wmd_rmsd_run(wmd_rmsd *rmsd, const float *coords_xyzw, size_t n_atoms
 wmd_status wmd_rmsd_run_device(wmd_rmsd *rmsd, const float *d_store, size_t stride,
                                size_t n_frames, float *d_rmsd, double *d_rmsd64, double *d_err,
                                double *d_rot, double *d_cx, void *stream);
+/* Allocation-free variant of wmd_rmsd_run_device.  It has no counterpart in
+ * the reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, outputs, results and stream semantics as
+ * wmd_rmsd_run_device; the per-(frame, tile) moment partials go to a fixed
+ * 16 MiB device array of the library (one per GPU, held until the process
+ * ends) instead of a buffer the handle allocates, and the call allocates
+ * nothing (kernel timing by wmd_ctx_profile is not recorded for it).
+ * Returns WMD_ERR_UNSUPPORTED, before enqueuing anything, when the frames'
+ * partials exceed that array: n_frames x tiles x moments x 8 B, with 13
+ * moments per (frame, tile) for fit == RMSD selection, 26 for a split fit,
+ * 1 unfitted, and one tile per frame beyond 12,288 frames (so at most about
+ * 161,000 / 80,000 / 2,000,000 frames per call).  Every call on a GPU
+ * shares that GPU's array: calls that overlap on one GPU (other streams,
+ * other handles, other threads) race; serialise them. */
+wmd_status wmd_rmsd_run_device_noalloc(wmd_rmsd *rmsd, const float *d_store, size_t stride,
+                                       size_t n_frames, float *d_rmsd, double *d_rmsd64,
+                                       double *d_err, double *d_rot, double *d_cx, void *stream);
 /* Backward of wmd_rmsd_run_device (no counterpart in the reference: the adjoint
  * of structure-matching losses, the differentiable companion of RmsdPlan).
  * d_gout [F] f32 = dL/dRMSD per frame; d_rmsd64, d_rot, d_cx: the outputs of

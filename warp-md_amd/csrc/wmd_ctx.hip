@@ -96,7 +96,7 @@ void *HostBuf::reserve(size_t n) {
 void bind(const wmd_ctx *ctx) { WMD_HIP_CHECK(hipSetDevice(ctx->device)); }
 
 ProfScope::ProfScope(wmd_ctx *c, hipStream_t s, const char *name) : ctx(c), st(s) {
-    if (!ctx->profile) return;
+    if (!ctx || !ctx->profile) return;  // ctx NULL: never record
     wmd_ctx::Mark m;
     m.name = name;
     if (hipEventCreate(&m.start) != hipSuccess) return;

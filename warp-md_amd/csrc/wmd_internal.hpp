@@ -144,7 +144,8 @@ inline size_t staging_frames(const wmd_ctx *ctx, size_t frame_bytes, size_t n_fr
     return per < 1 ? 1 : (per < n_frames ? per : (n_frames ? n_frames : 1));
 }
 
-// Records HIP events around a launch sequence on `st` when profiling is on.
+// Records HIP events around a launch sequence on `st` when profiling is on
+// (c == NULL: records nothing).
 struct ProfScope {
     wmd_ctx *ctx;
     hipStream_t st;

@@ -26,11 +26,20 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <mutex>
 
 #include "wmd_internal.hpp"
 #include "wmd_qcp.hpp"
 
 using namespace wmd;
+
+// Scratch of the allocation-free entry point (wmd_rmsd_run_device_noalloc):
+// the moments partials, at most kScratchBytes, in a fixed array that every
+// GPU the library runs on holds until the process ends.  In host code the
+// name `scratch` is only a host-side stand-in: kernels get the address that
+// hipGetSymbolAddress returns for the current device (scratch_address).
+static constexpr size_t kScratchBytes = size_t(16) << 20;
+static __device__ __align__(256) double scratch[kScratchBytes / sizeof(double)];
 
 namespace {
 
@@ -39,6 +48,22 @@ constexpr uint32_t kPad = 0xFFFFFFFFu;
 constexpr int kMom = 13;  // Sx[3], Sxy[9], Sxx
 
 enum { MODE_SAME = 0, MODE_SPLIT = 1, MODE_RAW = 2 };
+
+std::mutex g_scratch_mu;
+std::map<int, double *> g_scratch_addr;  // device -> its own copy of `scratch`
+
+// The device address of `scratch` on `device` (which must be current),
+// resolved once per device.
+double *scratch_address(int device) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    auto it = g_scratch_addr.find(device);
+    if (it != g_scratch_addr.end()) return it->second;
+    void *p = nullptr;
+    WMD_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    WMD_REQUIRE(p != nullptr, WMD_ERR_DEVICE, "no device address for the RMSD scratch array");
+    g_scratch_addr[device] = static_cast<double *>(p);
+    return static_cast<double *>(p);
+}
 
 struct RefMoments {
     double n_a, n_b;
@@ -447,6 +472,7 @@ struct wmd_rmsd {
     size_t n_ref_atoms = 0;
     RefMoments rm{};
     DevBuf d_ref, d_atoms, d_partials, d_out32, d_out64, d_err;
+    double *scratch = nullptr;  // this device's address of the fixed scratch array
     // host-chunk pipeline: two staging / device buffers, events guarding reuse
     HostBuf p_stage[2];
     DevBuf p_aos[2], p_soa[2];
@@ -455,24 +481,34 @@ struct wmd_rmsd {
 
 namespace {
 
-template <int MODE>
-void launch_rmsd(wmd_rmsd *h, const float *d_store, size_t stride, size_t n_frames, float *d_rmsd,
-                 double *d_r64, double *d_err, double *d_rot, double *d_cx, hipStream_t st) {
-    constexpr int NV = MODE == MODE_RAW ? 1 : (MODE == MODE_SAME ? kMom : 2 * kMom);
+int moment_count(int mode) { return mode == MODE_RAW ? 1 : (mode == MODE_SAME ? kMom : 2 * kMom); }
+
+// Tiles per frame of the moments kernel: aim for ~12k waves in flight, each
+// lane streaming >= 32 float4 groups.  gpt: groups per tile.
+void rmsd_tiling(const wmd_rmsd *h, size_t n_frames, int *n_tiles_out, int *gpt_out) {
     const int n_groups = (int)(h->stride / 4);
-    // tiles per frame: aim for ~12k waves in flight, each lane streaming >= 32 groups
     long long want = std::max<long long>(1, (12288 + (long long)n_frames - 1) / (long long)n_frames);
     long long cap = std::max<long long>(1, n_groups / (64 * 32));
     int n_tiles = (int)std::min(want, cap);
     int gpt = (n_groups + n_tiles - 1) / n_tiles;
     n_tiles = (n_groups + gpt - 1) / std::max(gpt, 1);
-    n_tiles = std::max(n_tiles, 1);
-    double *partials = (double *)h->d_partials.reserve((size_t)n_frames * n_tiles * NV * 8);
+    *n_tiles_out = std::max(n_tiles, 1);
+    *gpt_out = gpt;
+}
+
+// partials: [n_frames][n_tiles][moments] doubles (every entry is written by
+// the moments kernel before the finalize reads it, so nothing is cleared).
+// prof: record HIP events around the kernels when the context profiles.
+template <int MODE>
+void launch_rmsd(wmd_rmsd *h, const float *d_store, size_t stride, size_t n_frames, float *d_rmsd,
+                 double *d_r64, double *d_err, double *d_rot, double *d_cx, hipStream_t st,
+                 double *partials, int n_tiles, int gpt, bool prof) {
+    const int n_groups = (int)(h->stride / 4);
     long long waves = (long long)n_frames * n_tiles;
     long long blocks = (waves + (kBlock / 64) - 1) / (kBlock / 64);
     WMD_REQUIRE(blocks < (1ll << 31), WMD_ERR_UNSUPPORTED, "too many frames for one launch");
     {
-    ProfScope prof(h->ctx, st, "rmsd_moments");
+    ProfScope scope(prof ? h->ctx : nullptr, st, "rmsd_moments");
     hipLaunchKernelGGL(rmsd_moments_kernel<MODE>, dim3((unsigned)blocks), dim3(kBlock), 0, st,
                        d_store, 3 * (long long)stride, (int)stride, (int)n_frames,
                        h->d_ref.as<float>(), n_groups, h->seg1, h->seg2, n_tiles, gpt, partials);
@@ -481,22 +517,32 @@ void launch_rmsd(wmd_rmsd *h, const float *d_store, size_t stride, size_t n_fram
     double lane_terms = 4.0 * std::ceil((double)gpt / 64.0) + 6.0;
     int fb = (int)((n_frames + 63) / 64);
     {
-    ProfScope prof(h->ctx, st, "rmsd_finalize");
+    ProfScope scope(prof ? h->ctx : nullptr, st, "rmsd_finalize");
     hipLaunchKernelGGL(rmsd_finalize_kernel<MODE>, dim3(fb), dim3(64), 0, st, partials,
                        (int)n_frames, n_tiles, h->rm, lane_terms, d_rmsd, d_r64, d_err, d_rot, d_cx);
     }
     WMD_HIP_CHECK(hipGetLastError());
 }
 
+// scratch_partials: NULL (the handle's own partials buffer, grown as needed)
+// or the address of the library's fixed scratch array on this device, which
+// must hold n_frames x tiles x moments doubles (checked by the caller).
 void run_device(wmd_rmsd *h, const float *d_store, size_t stride, size_t n_frames, float *d_rmsd,
-                double *d_r64, double *d_err, double *d_rot, double *d_cx, hipStream_t st) {
+                double *d_r64, double *d_err, double *d_rot, double *d_cx, hipStream_t st,
+                double *scratch_partials = nullptr) {
     if (n_frames == 0) return;
     WMD_REQUIRE(stride == h->stride, WMD_ERR_INVALID,
                 "store stride must equal the plan's stride (" + std::to_string(h->stride) + ")");
     WMD_REQUIRE(d_store && d_rmsd, WMD_ERR_INVALID, "null device buffer");
-    if (h->mode == MODE_SAME) launch_rmsd<MODE_SAME>(h, d_store, stride, n_frames, d_rmsd, d_r64, d_err, d_rot, d_cx, st);
-    else if (h->mode == MODE_SPLIT) launch_rmsd<MODE_SPLIT>(h, d_store, stride, n_frames, d_rmsd, d_r64, d_err, d_rot, d_cx, st);
-    else launch_rmsd<MODE_RAW>(h, d_store, stride, n_frames, d_rmsd, d_r64, d_err, d_rot, d_cx, st);
+    int n_tiles = 1, gpt = 1;
+    rmsd_tiling(h, n_frames, &n_tiles, &gpt);
+    double *partials = scratch_partials;
+    if (!partials)
+        partials = (double *)h->d_partials.reserve((size_t)n_frames * n_tiles * moment_count(h->mode) * 8);
+    const bool prof = scratch_partials == nullptr;  // event creation is left out of the allocation-free path
+    if (h->mode == MODE_SAME) launch_rmsd<MODE_SAME>(h, d_store, stride, n_frames, d_rmsd, d_r64, d_err, d_rot, d_cx, st, partials, n_tiles, gpt, prof);
+    else if (h->mode == MODE_SPLIT) launch_rmsd<MODE_SPLIT>(h, d_store, stride, n_frames, d_rmsd, d_r64, d_err, d_rot, d_cx, st, partials, n_tiles, gpt, prof);
+    else launch_rmsd<MODE_RAW>(h, d_store, stride, n_frames, d_rmsd, d_r64, d_err, d_rot, d_cx, st, partials, n_tiles, gpt, prof);
 }
 
 }  // namespace
@@ -597,6 +643,7 @@ wmd_status wmd_rmsd_create(wmd_ctx *ctx, const uint32_t *fit_sel, size_t n_fit,
         WMD_HIP_CHECK(hipMemcpy(h->d_ref.ptr, refs.data(), refs.size() * 4, hipMemcpyHostToDevice));
         h->d_atoms.reserve(h->atoms.size() * 4);
         WMD_HIP_CHECK(hipMemcpy(h->d_atoms.ptr, h->atoms.data(), h->atoms.size() * 4, hipMemcpyHostToDevice));
+        h->scratch = scratch_address(ctx->device);  // the context's device is current (bind)
     } catch (...) {
         delete h;
         throw;
@@ -684,6 +731,26 @@ wmd_status wmd_rmsd_run_device(wmd_rmsd *h, const float *d_store, size_t stride,
     bind(h->ctx);
     run_device(h, d_store, stride, n_frames, d_rmsd, d_rmsd64, d_err, d_rot, d_cx,
                pick_stream(h->ctx, stream));
+    WMD_API_END
+}
+
+wmd_status wmd_rmsd_run_device_noalloc(wmd_rmsd *h, const float *d_store, size_t stride,
+                                       size_t n_frames, float *d_rmsd, double *d_rmsd64,
+                                       double *d_err, double *d_rot, double *d_cx, void *stream) {
+    WMD_API_BEGIN
+    WMD_REQUIRE(h != nullptr, WMD_ERR_INVALID, "null handle");
+    if (n_frames == 0) return WMD_OK;
+    WMD_REQUIRE(h->scratch != nullptr, WMD_ERR_DEVICE, "RMSD scratch array not resolved");
+    int n_tiles = 1, gpt = 1;
+    rmsd_tiling(h, n_frames, &n_tiles, &gpt);
+    const size_t need = n_frames * (size_t)n_tiles * (size_t)moment_count(h->mode) * sizeof(double);
+    WMD_REQUIRE(need <= kScratchBytes, WMD_ERR_UNSUPPORTED,
+                "allocation-free RMSD: " + std::to_string(n_frames) + " frames need " +
+                    std::to_string(need) + " bytes of scratch, more than the fixed " +
+                    std::to_string(kScratchBytes) + "; pass fewer frames per call");
+    bind(h->ctx);
+    run_device(h, d_store, stride, n_frames, d_rmsd, d_rmsd64, d_err, d_rot, d_cx,
+               pick_stream(h->ctx, stream), h->scratch);
     WMD_API_END
 }
 

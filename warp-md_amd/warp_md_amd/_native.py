@@ -58,6 +58,7 @@ SYMBOLS = (
     "wmd_rdf_reset", "wmd_rdf_destroy", "wmd_rdf_last_path", "wmd_rdf_set_tuning",
     "wmd_rdf_finalize", "wmd_rdf_create_f64", "wmd_sf_finalize",
     "wmd_rmsd_create", "wmd_rmsd_store_layout", "wmd_rmsd_run", "wmd_rmsd_run_device",
+    "wmd_rmsd_run_device_noalloc",
     "wmd_rmsd_backward_device", "wmd_rmsd_destroy", "wmd_qcp_host",
     "wmd_softrdf_create", "wmd_softrdf_forward", "wmd_softrdf_backward",
     "wmd_softrdf_last_path", "wmd_softrdf_set_tuning", "wmd_softrdf_destroy",
@@ -150,6 +151,7 @@ def lib():
             "wmd_rmsd_store_layout": ([P, P, S, ctypes.POINTER(S), ctypes.POINTER(S)], St),
             "wmd_rmsd_run": ([P, P, S, S, P, P], St),
             "wmd_rmsd_run_device": ([P, P, S, S, P, P, P, P, P, P], St),
+            "wmd_rmsd_run_device_noalloc": ([P, P, S, S, P, P, P, P, P, P], St),
             "wmd_rmsd_backward_device": ([P, P, S, S, P, P, P, P, P, P], St),
             "wmd_rmsd_destroy": ([P], None),
             "wmd_softrdf_create": ([P, S, ctypes.c_float, ctypes.c_float, I, PP], St),
