@@ -282,14 +282,32 @@ enum { WMD_PAIRWISE_RMS = 0, WMD_PAIRWISE_NOFIT = 1, WMD_PAIRWISE_DME = 2 };
  * superposition residual) or "nofit" (rmsd_raw, plan.rs:1081-1091).  The
  * matrix is symmetric with a zero diagonal; a selection of fewer than 2 atoms
  * gives the zero matrix (plan.rs:597-603).  The O(F^2 n) cross-covariances are
- * one FP64 Gram product on the matrix cores.  WMD_PAIRWISE_DME has no device
- * path (WMD_ERR_UNSUPPORTED). */
+ * one FP64 Gram product on the matrix cores.  WMD_PAIRWISE_DME is the
+ * distance-matrix RMSD of plan.rs:612-650 without PBC (see
+ * wmd_pairwise_create_dme). */
 wmd_status wmd_pairwise_create(wmd_ctx *ctx, const uint32_t *sel, size_t n_sel, int metric,
                                wmd_pairwise **out);
+/* PairwiseRmsdPlan::new(selection, PairwiseMetric::Dme, pbc) (plan.rs:67-73,
+ * 612-650): for every two frames sqrt(sum_k (d_i[k] - d_j[k])^2 / n_pairs)
+ * over the compact pair-distance vectors of pair_distances_compact
+ * (plan.rs:1279-1302: i < j in selection order, the f32 difference cast to
+ * f64, apply_pbc on the frame's box lengths for WMD_PBC_ORTHO, sqrt in f64).
+ * The sums run in the reference's k order with its roundings.  pbc:
+ * WMD_PBC_NONE or WMD_PBC_ORTHO (else WMD_ERR_PARSE); with WMD_PBC_ORTHO the
+ * frames must be appended with wmd_pairwise_append_boxes. */
+wmd_status wmd_pairwise_create_dme(wmd_ctx *ctx, const uint32_t *sel, size_t n_sel, int pbc,
+                                   wmd_pairwise **out);
 /* PairwiseRmsdPlan::process_chunk (plan.rs:533-558): appends the selected atoms
  * of a host chunk [n_frames][n_atoms][4] to the plan's device store. */
 wmd_status wmd_pairwise_append(wmd_pairwise *pw, const float *coords_xyzw, size_t n_atoms,
                                size_t n_frames);
+/* The same with the chunk's boxes [n_frames][9] (FrameChunk.box_, plan.rs:
+ * 549-556): a dme handle with WMD_PBC_ORTHO keeps each frame's box lengths
+ * (WMD_ERR_MISMATCH "orthorhombic box required for PBC" for a box with
+ * off-diagonal terms, as box_lengths, pbc_math.rs:11-15); other handles ignore
+ * them.  wmd_pairwise_append on such a handle returns WMD_ERR_MISMATCH. */
+wmd_status wmd_pairwise_append_boxes(wmd_pairwise *pw, const float *coords_xyzw, size_t n_atoms,
+                                     size_t n_frames, const float *boxes);
 wmd_status wmd_pairwise_frames(const wmd_pairwise *pw, size_t *n_frames);
 /* PairwiseRmsdPlan::finalize (plan.rs:586-675) over the appended frames:
  * out [F][F] f32, row-major (synchronous). */
@@ -300,7 +318,8 @@ wmd_status wmd_pairwise_store_stride(const wmd_pairwise *pw, size_t *stride);
 /* Hot path on a device store [n_frames][3][stride] holding the selection's
  * atoms in slots 0 .. n_sel - 1 (16-byte aligned): d_out [F][F] f32 and
  * d_out64 (NULL or [F][F] f64, the value before the cast).  Asynchronous on
- * `stream`. */
+ * `stream`.  A dme handle with WMD_PBC_ORTHO returns WMD_ERR_UNSUPPORTED
+ * (its boxes come with wmd_pairwise_append_boxes). */
 wmd_status wmd_pairwise_run_device(wmd_pairwise *pw, const float *d_store, size_t stride,
                                    size_t n_frames, float *d_out, double *d_out64, void *stream);
 void wmd_pairwise_destroy(wmd_pairwise *pw);

@@ -89,6 +89,8 @@ def lib():
     L.orc_sf_finalize.restype = ctypes.c_int
     L.orc_pairwise_rmsd.argtypes = [P, S, S, S, P, S, ctypes.c_int, P, P]
     L.orc_pairwise_rmsd.restype = ctypes.c_int
+    L.orc_pairwise_dme.argtypes = [P, S, S, S, P, S, ctypes.c_int, P, P, P]
+    L.orc_pairwise_dme.restype = ctypes.c_int
     L.orc_align.argtypes = [P, S, S, S, P, S, P, P, S, ctypes.c_int, P, P, P]
     L.orc_align.restype = ctypes.c_int
     L.orc_rdf_grouped.argtypes = [P, S, S, S, P, P, S, P, P, P, P, S, P, P, P, ctypes.c_int,
@@ -263,6 +265,26 @@ def pairwise_rmsd(coords, sel, metric="rms"):
     st = L.orc_pairwise_rmsd(_ptr(c), stride, N, F, _ptr(s), s.size, m, _ptr(out), _ptr(out64))
     if st != 0:
         raise RuntimeError(f"oracle pairwise rmsd failed {st}")
+    return out, out64
+
+
+def pairwise_dme(coords, sel, pbc="none", boxes=None):
+    """PairwiseRmsdPlan metric "dme" CPU path (rmsd/plan.rs:612-650, 1279-1302);
+    boxes: (F, 3) box lengths for pbc "orthorhombic".  Returns ((F, F) f32,
+    (F, F) f64 before the cast)."""
+    L = lib()
+    c = _coords(coords)
+    F, N, stride = c.shape
+    s = np.ascontiguousarray(sel, dtype=np.uint32)
+    kind = PBC_KINDS[pbc]
+    bx = None
+    if kind == 1:
+        bx = np.ascontiguousarray(np.broadcast_to(np.asarray(boxes, np.float32).reshape(-1, 3), (F, 3)))
+    out = np.zeros((F, F), np.float32)
+    out64 = np.zeros((F, F), np.float64)
+    st = L.orc_pairwise_dme(_ptr(c), stride, N, F, _ptr(s), s.size, kind, _ptr(bx), _ptr(out), _ptr(out64))
+    if st != 0:
+        raise RuntimeError(f"oracle pairwise dme failed {st}")
     return out, out64
 
 

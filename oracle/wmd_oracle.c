@@ -26,6 +26,10 @@
  *   orc_min_image_f64    pbc_math.rs:166-221 known vectors.
  *   orc_pairdist         reference known answers (correlation_structure_tests.rs:2193-2231,
  *                        2313-2422); equal to orc_rdf_brute where the pair rules coincide.
+ *   orc_pairwise_dme     a hand-computed 3-atom case; exact doubling of every coordinate (the
+ *                        matrix entry is then the RMS of the frame's pair distances); scipy
+ *                        pdist pair order and values; invariance under rigid motions and
+ *                        under lattice translations of single atoms with PBC.
  *   orc_pairwise_rmsd    pairwise_rmsd_plan_basic (geometry_matrix_tests.rs:1438-1460),
  *                        equal to orc_rmsd_plan per pair where the two coincide, numpy.linalg.svd
  *                        Kabsch, rigid-motion zero, symmetry, triangle inequality (nofit).
@@ -834,6 +838,69 @@ int orc_pairwise_rmsd(const float *coords, size_t stride, size_t n_atoms, size_t
             if (out64) out64[i * n_frames + j] = out64[j * n_frames + i] = v;
         }
     free(fs);
+    return ORC_OK;
+}
+
+/*
+ * PairwiseRmsdPlan metric "dme" (rmsd/plan.rs:612-650): per frame the
+ * compact pair-distance vector of pair_distances_compact (:1279-1302) -- for
+ * i < j in selection order, the f32 difference cast to f64, the f64
+ * orthorhombic wrap of apply_pbc (pbc_math.rs:17-27) on the frame's box
+ * lengths when pbc is ORTHO, sqrt in f64 -- then for every frame pair i < j
+ * sqrt(sum_k (d_i[k] - d_j[k])^2 / n_pairs) summed over k in order, cast to
+ * f32.  boxes: [F][3] box lengths (f32, read as f64) for PBC_ORTHO, else
+ * unused.  out [F][F] f32 (symmetric, zero diagonal; all zero when n_sel < 2),
+ * out64 NULL or the f64 values before the cast.
+ */
+int orc_pairwise_dme(const float *coords, size_t stride, size_t n_atoms, size_t n_frames,
+                     const uint32_t *sel, size_t n_sel, int pbc, const float *boxes, float *out,
+                     double *out64) {
+    if (pbc != PBC_NONE && pbc != PBC_ORTHO) return ORC_ERR_INVALID;
+    if (pbc == PBC_ORTHO && !boxes) return ORC_ERR_MISMATCH;
+    for (size_t i = 0; i < n_sel; ++i)
+        if (sel[i] >= n_atoms) return ORC_ERR_INVALID;
+    for (size_t t = 0; t < n_frames * n_frames; ++t) {
+        out[t] = 0.0f;
+        if (out64) out64[t] = 0.0;
+    }
+    if (n_sel < 2 || n_frames == 0) return ORC_OK;
+    const size_t n_pairs = n_sel * (n_sel - 1) / 2;
+    double *dist = (double *)malloc(n_frames * n_pairs * sizeof(double));
+    if (!dist) return ORC_ERR_INVALID;
+    for (size_t f = 0; f < n_frames; ++f) {
+        double lx = 0.0, ly = 0.0, lz = 0.0;
+        if (pbc == PBC_ORTHO) {
+            lx = boxes[3 * f];
+            ly = boxes[3 * f + 1];
+            lz = boxes[3 * f + 2];
+        }
+        double *d = dist + f * n_pairs;
+        size_t k = 0;
+        for (size_t i = 0; i < n_sel; ++i) {
+            const float *pa = coords + (f * n_atoms + sel[i]) * stride;
+            for (size_t j = i + 1; j < n_sel; ++j) {
+                const float *pb = coords + (f * n_atoms + sel[j]) * stride;
+                double dx = (double)(pb[0] - pa[0]);
+                double dy = (double)(pb[1] - pa[1]);
+                double dz = (double)(pb[2] - pa[2]);
+                if (pbc == PBC_ORTHO) apply_pbc_f64(&dx, &dy, &dz, lx, ly, lz);
+                d[k++] = sqrt(dx * dx + dy * dy + dz * dz);
+            }
+        }
+    }
+    for (size_t i = 0; i < n_frames; ++i)
+        for (size_t j = i + 1; j < n_frames; ++j) {
+            const double *a = dist + i * n_pairs, *b = dist + j * n_pairs;
+            double sum = 0.0;
+            for (size_t k = 0; k < n_pairs; ++k) {
+                const double diff = a[k] - b[k];
+                sum += diff * diff;
+            }
+            const double v = sqrt(sum / (double)n_pairs);
+            out[i * n_frames + j] = out[j * n_frames + i] = (float)v;
+            if (out64) out64[i * n_frames + j] = out64[j * n_frames + i] = v;
+        }
+    free(dist);
     return ORC_OK;
 }
 

@@ -27,8 +27,8 @@ def _device_run(coords, sel, metric):
 
     ctx = _native.context(0)
     h = ctypes.c_void_p()
-    check(lib().wmd_pairwise_create(ctx.handle, ptr(sel), sel.size, {"rms": 0, "nofit": 1}[metric],
-                                    ctypes.byref(h)))
+    check(lib().wmd_pairwise_create(ctx.handle, ptr(sel), sel.size,
+                                    {"rms": 0, "nofit": 1, "dme": 2}[metric], ctypes.byref(h)))
     try:
         F = coords.shape[0]
         stride = (sel.size + 3) // 4 * 4
@@ -85,8 +85,6 @@ def test_reference_known_answer_and_edges():
     assert out.shape == (2, 2) and np.all(out == 0)
     out = PairwiseRmsdPlan([0, 1]).run(Trajectory.from_numpy(c[:1]), sysm, device="cuda:0")
     assert out.shape == (1, 1) and out[0, 0] == 0
-    with pytest.raises(RuntimeError, match="unsupported"):
-        PairwiseRmsdPlan([0, 1], metric="dme").run(Trajectory.from_numpy(c), sysm, device="cuda:0")
     with pytest.raises(RuntimeError, match="metric must be"):
         PairwiseRmsdPlan([0, 1], metric="xyz")
 
@@ -123,3 +121,70 @@ def test_full_chain_5000_frames():
         worst = max(worst, float(d.max()))
     print(f"5000 x 5000: max |gpu - oracle| over 4 columns {worst:.2e} A", flush=True)
     assert worst <= 1e-5
+
+
+# ---- metric "dme" (plan.rs:612-650): the reference's k-ordered f64 sums ----
+
+@pytest.mark.parametrize("F,n", [(2, 2), (5, 37), (70, 61), (130, 200)])
+def test_dme_matches_oracle_bit_for_bit(F, n):
+    c = _chainish(F, n + 3, seed=3 * F + n)
+    sel = np.arange(2, n + 2, dtype=np.uint32)[::-1].copy()  # selection order matters for k
+    print(f"pairwise dme F={F} n={n}", flush=True)
+    out, out64 = _device_run(c, sel, "dme")
+    ref, ref64 = oracle.pairwise_dme(c, sel)
+    np.testing.assert_array_equal(out64, ref64)
+    np.testing.assert_array_equal(out, ref)
+
+
+def _dme_plan(c, sel, pbc="none", box=None, chunk=None):
+    sysm = System.from_arrays({"name": np.array(["X"] * c.shape[1])})
+    return PairwiseRmsdPlan(sel, metric="dme", pbc=pbc).run(Trajectory.from_numpy(c, box=box), sysm,
+                                                            chunk_frames=chunk, device="cuda:0")
+
+
+def test_dme_plan_orthorhombic_matches_oracle():
+    rng = np.random.default_rng(31)
+    F, n, L = 40, 90, 12.0
+    c = rng.uniform(0, L, size=(F, n, 3)).astype(np.float32)
+    box = np.stack([np.array([L, L * 1.1, L * 0.9], np.float32) * np.float32(1 + 0.01 * f) for f in range(F)])
+    sel = np.arange(3, 88, dtype=np.uint32)
+    print("pairwise dme plan orthorhombic", flush=True)
+    out = _dme_plan(c, sel, "orthorhombic", box, chunk=7)
+    ref, ref64 = oracle.pairwise_dme(c, sel, pbc="orthorhombic", boxes=box)
+    np.testing.assert_array_equal(out, ref)
+    plain = _dme_plan(c, sel, chunk=7)
+    assert np.max(np.abs(plain - out)) > 0.1  # the wrap matters for this box
+    np.testing.assert_array_equal(plain, oracle.pairwise_dme(c, sel)[0])
+
+
+def test_dme_errors():
+    c = np.zeros((2, 3, 3), np.float32)
+    c[1, 1, 0] = 1.0
+    sysm = System.from_arrays({"name": np.array(["X"] * 3)})
+    with pytest.raises(RuntimeError, match="orthorhombic box required"):
+        PairwiseRmsdPlan([0, 1, 2], metric="dme", pbc="orthorhombic").run(Trajectory.from_numpy(c), sysm,
+                                                                          device="cuda:0")
+    tri = np.array([[10, 0, 0], [1, 10, 0], [0, 0, 10]], np.float32)
+    with pytest.raises(RuntimeError, match="orthorhombic box required"):
+        PairwiseRmsdPlan([0, 1, 2], metric="dme", pbc="orthorhombic").run(
+            Trajectory.from_numpy(c, box=tri), sysm, device="cuda:0")
+    with pytest.raises(RuntimeError, match="pbc must be"):
+        PairwiseRmsdPlan([0, 1, 2], metric="dme", pbc="triclinic")
+    # one atom: the zero matrix (plan.rs:597-603)
+    assert not _dme_plan(c, [1]).any()
+
+
+def test_dme_many_pair_blocks():
+    """2,048 frames x 600 atoms (179,700 pair distances in 11 blocks of 16,384):
+    a sample of frame pairs against the oracle, bit for bit."""
+    F, n = 2048, 600
+    c = _chainish(F, n, seed=41, noise=0.8)
+    sel = np.arange(n, dtype=np.uint32)
+    print("pairwise dme 2048 x 600", flush=True)
+    out, out64 = _device_run(c, sel, "dme")
+    assert np.array_equal(out, out.T) and np.all(np.diag(out) == 0)
+    rng = np.random.default_rng(42)
+    for _ in range(12):
+        i, j = rng.choice(F, 2, replace=False)
+        _, r64 = oracle.pairwise_dme(c[[i, j]], sel)
+        assert out64[i, j] == r64[0, 1], (i, j)

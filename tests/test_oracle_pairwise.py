@@ -94,3 +94,89 @@ def test_fewer_than_two_atoms_gives_zeros():
     c = _frames(4, 5, seed=5)
     out, _ = oracle.pairwise_rmsd(c, [2], "rms")
     assert np.all(out == 0) and out.shape == (4, 4)
+
+
+# ---- metric "dme" (rmsd/plan.rs:612-650, pair_distances_compact :1279-1302) ----
+
+def test_dme_three_atom_closed_form():
+    # frame 0: 3-4-5 triangle; frame 1: the same triangle doubled (6-8-10)
+    c = np.array([[[0, 0, 0], [3, 0, 0], [0, 4, 0]],
+                  [[0, 0, 0], [6, 0, 0], [0, 8, 0]]], np.float32)
+    out, out64 = oracle.pairwise_dme(c, [0, 1, 2])
+    want = np.sqrt((3.0 ** 2 + 4.0 ** 2 + 5.0 ** 2) / 3.0)
+    assert out64[0, 1] == want and out64[1, 0] == want
+    assert out64[0, 0] == 0 and out64[1, 1] == 0
+    assert out[0, 1] == np.float32(want)
+
+
+def test_dme_exact_doubling_is_rms_of_distances():
+    # doubling every coordinate doubles every f32 difference and every distance
+    # exactly, so DME = sqrt(mean(d^2)) of the first frame, with d from scipy's pdist
+    from scipy.spatial.distance import pdist
+
+    rng = np.random.default_rng(7)
+    # dyadic coordinates: the f32 differences are exact, as pdist's f64 ones
+    x = (rng.integers(-1280, 1280, size=(37, 3)) / 64.0).astype(np.float32)
+    c = np.stack([x, 2 * x])
+    out, out64 = oracle.pairwise_dme(c, np.arange(37))
+    d = pdist(x.astype(np.float64))
+    assert abs(out64[0, 1] - np.sqrt(np.mean(d * d))) <= 1e-12 * out64[0, 1]
+
+
+def test_dme_selection_order_and_subset():
+    rng = np.random.default_rng(8)
+    c = rng.uniform(-10, 10, size=(4, 20, 3)).astype(np.float32)
+    sel = np.array([5, 2, 17, 9, 11, 3])
+    out, out64 = oracle.pairwise_dme(c, sel)
+    out2, out64b = oracle.pairwise_dme(c[:, sel], np.arange(sel.size))
+    np.testing.assert_array_equal(out64, out64b)
+    # the DME of a frame pair is independent of the other frames and symmetric
+    for i in range(4):
+        for j in range(4):
+            if i != j:
+                _, pair = oracle.pairwise_dme(c[[i, j]], sel)
+                assert pair[0, 1] == out64[i, j] == out64[j, i]
+
+
+def test_dme_rigid_motion_is_near_zero():
+    rng = np.random.default_rng(9)
+    x = rng.uniform(-15, 15, size=(50, 3))
+    q = rng.normal(size=4)
+    q /= np.linalg.norm(q)
+    w, a, b, d = q
+    R = np.array([[1 - 2 * (b * b + d * d), 2 * (a * b - d * w), 2 * (a * d + b * w)],
+                  [2 * (a * b + d * w), 1 - 2 * (a * a + d * d), 2 * (b * d - a * w)],
+                  [2 * (a * d - b * w), 2 * (b * d + a * w), 1 - 2 * (a * a + b * b)]])
+    c = np.stack([x, x @ R.T + 7.5]).astype(np.float32)
+    out, out64 = oracle.pairwise_dme(c, np.arange(50))
+    assert out64[0, 1] < 1e-4  # f32 rounding of the moved coordinates only
+    noisy = c.copy()
+    noisy[1] += np.float32(0.3) * rng.normal(size=(50, 3)).astype(np.float32)
+    assert oracle.pairwise_dme(noisy, np.arange(50))[1][0, 1] > 0.05
+
+
+def test_dme_orthorhombic_wrap():
+    # two atoms 2 A apart across the boundary of a 10 A box, then 4 A apart inside it
+    c = np.array([[[1, 0, 0], [9, 0, 0]], [[1, 0, 0], [5, 0, 0]]], np.float32)
+    box = np.array([[10, 10, 10], [10, 10, 10]], np.float32)
+    _, wrapped = oracle.pairwise_dme(c, [0, 1], pbc="orthorhombic", boxes=box)
+    _, plain = oracle.pairwise_dme(c, [0, 1])
+    assert wrapped[0, 1] == 2.0 and plain[0, 1] == 4.0
+    # dyadic coordinates: moving single atoms by whole box vectors is exact and
+    # leaves every wrapped distance, hence the DME, at exactly 0
+    rng = np.random.default_rng(10)
+    x = (rng.integers(0, 128, size=(30, 3)) / 8.0).astype(np.float32)
+    y = x.copy()
+    y[::3, 0] += 16.0
+    y[1::4, 2] -= 32.0
+    c = np.stack([x, y])
+    box = np.array([[16, 16, 16], [16, 16, 16]], np.float32)
+    _, w = oracle.pairwise_dme(c, np.arange(30), pbc="orthorhombic", boxes=box)
+    assert w[0, 1] == 0.0
+    assert oracle.pairwise_dme(c, np.arange(30))[1][0, 1] > 1.0
+
+
+def test_dme_fewer_than_two_atoms_is_zero():
+    c = np.ones((3, 4, 3), np.float32)
+    out, out64 = oracle.pairwise_dme(c, [2])
+    assert not out.any() and not out64.any()

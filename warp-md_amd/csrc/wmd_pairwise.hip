@@ -378,6 +378,157 @@ __global__ __launch_bounds__(kPwBlock) void km_sums_kernel(const double *__restr
     }
 }
 
+// ---------------------------------------------------------------------------
+// Metric "dme" (plan.rs:612-650): the RMS difference of two frames' compact
+// pair-distance vectors (pair_distances_compact, plan.rs:1279-1302).  The
+// store is transposed to atom-major planes so that consecutive threads read
+// consecutive frames; blocks of K pair distances D[k][f] (f64) are computed
+// for every frame, and the frame-pair sums S[i][j] = sum_k (D[k][i] -
+// D[k][j])^2 advance block by block in k order with the reference's own
+// roundings (a separate multiply and add, no contraction), so every S[i][j]
+// is the reference's sequential f64 sum.
+// ---------------------------------------------------------------------------
+constexpr int kDmeTile = 64;  // frames per tile side (4 x 4 per thread)
+constexpr int kDmeKc = 16;    // pair distances per LDS chunk
+
+// store [F][3][stride] -> T [3][n][fp], fp = F rounded up to kDmeTile (0 pads).
+__global__ __launch_bounds__(kPwBlock) void dme_transpose_kernel(const float *__restrict__ store,
+                                                                 int stride, int n, int F, int fp,
+                                                                 float *__restrict__ T) {
+    const long long total = (long long)n * fp;
+    for (long long t = (long long)blockIdx.x * kPwBlock + threadIdx.x; t < total;
+         t += (long long)gridDim.x * kPwBlock) {
+        const int a = (int)(t / fp);
+        const int f = (int)(t - (long long)a * fp);
+        float x = 0.f, y = 0.f, z = 0.f;
+        if (f < F) {
+            const float *s = store + (long long)f * 3 * stride;
+            x = s[a];
+            y = s[stride + a];
+            z = s[2 * stride + a];
+        }
+        T[(long long)a * fp + f] = x;
+        T[((long long)n + a) * fp + f] = y;
+        T[(2LL * n + a) * fp + f] = z;
+    }
+}
+
+// Pair k of the compact order (i < j, i major): row i starts at i (2n - i - 1) / 2.
+__device__ __forceinline__ void dme_pair(long long k, int n, int *pi, int *pj) {
+    const double b = 2.0 * n - 1.0;
+    int i = (int)floor((b - sqrt(fmax(b * b - 8.0 * (double)k, 0.0))) * 0.5);
+    i = max(0, min(i, n - 2));
+    while (i > 0 && (long long)i * (2LL * n - i - 1) / 2 > k) --i;
+    while (i < n - 2 && (long long)(i + 1) * (2LL * n - i - 2) / 2 <= k) ++i;
+    *pi = i;
+    *pj = (int)(k - (long long)i * (2LL * n - i - 1) / 2) + i + 1;
+}
+
+// D[kk][f] = distance of pair k0 + kk in frame f (0 for padding frames):
+// the f32 difference pb - pa cast to f64, apply_pbc (pbc_math.rs:17-27) on
+// the frame's f64 box lengths when box3 is given, sqrt in f64.
+__global__ __launch_bounds__(kPwBlock) void dme_dist_kernel(const float *__restrict__ T, int n,
+                                                            int F, int fp, long long k0, int K,
+                                                            const double *__restrict__ box3,
+                                                            double *__restrict__ D) {
+    for (int kk = blockIdx.x; kk < K; kk += gridDim.x) {
+        int i, j;
+        dme_pair(k0 + kk, n, &i, &j);
+        const float *xi = T + (long long)i * fp, *xj = T + (long long)j * fp;
+        const float *yi = T + ((long long)n + i) * fp, *yj = T + ((long long)n + j) * fp;
+        const float *zi = T + (2LL * n + i) * fp, *zj = T + (2LL * n + j) * fp;
+        double *out = D + (long long)kk * fp;
+        for (int f = threadIdx.x; f < fp; f += kPwBlock) {
+            double d = 0.0;
+            if (f < F) {
+                double dx = (double)(xj[f] - xi[f]);
+                double dy = (double)(yj[f] - yi[f]);
+                double dz = (double)(zj[f] - zi[f]);
+                if (box3) {
+                    const double lx = box3[3 * f], ly = box3[3 * f + 1], lz = box3[3 * f + 2];
+                    if (lx > 0.0) dx -= round(dx / lx) * lx;
+                    if (ly > 0.0) dy -= round(dy / ly) * ly;
+                    if (lz > 0.0) dz -= round(dz / lz) * lz;
+                }
+                d = sqrt(dx * dx + dy * dy + dz * dz);
+            }
+            out[f] = d;
+        }
+    }
+}
+
+// For tiles (ti <= tj) of kDmeTile x kDmeTile frame pairs: S[i][j] (row
+// stride fp) continues its sum over the K pair distances of the block, k in
+// order (first block: from 0).  Thread (ty, tx) owns rows ty + 16 p and
+// columns tx + 16 q of the tile.
+__global__ __launch_bounds__(kPwBlock) void dme_sum_kernel(const double *__restrict__ D, int fp, int K,
+                                                           const int2 *__restrict__ tiles, int n_tiles,
+                                                           int first, double *__restrict__ S) {
+    __shared__ double A[kDmeKc][kDmeTile], B[kDmeKc][kDmeTile];
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    for (int tt = blockIdx.x; tt < n_tiles; tt += gridDim.x) {
+        const int2 tl = tiles[tt];
+        const int i0 = tl.x * kDmeTile, j0 = tl.y * kDmeTile;
+        double acc[4][4];
+#pragma unroll
+        for (int p = 0; p < 4; ++p)
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                acc[p][q] = first ? 0.0 : S[(long long)(i0 + ty + 16 * p) * fp + j0 + tx + 16 * q];
+        for (int c0 = 0; c0 < K; c0 += kDmeKc) {
+            // rows past K stage zeros on both sides: (0 - 0)^2 adds an exact 0
+            for (int e = threadIdx.x; e < kDmeKc * kDmeTile; e += kPwBlock) {
+                const int r = e / kDmeTile, c = e - r * kDmeTile;
+                const bool in = c0 + r < K;
+                A[r][c] = in ? D[(long long)(c0 + r) * fp + i0 + c] : 0.0;
+                B[r][c] = in ? D[(long long)(c0 + r) * fp + j0 + c] : 0.0;
+            }
+            __syncthreads();
+#pragma unroll 4
+            for (int r = 0; r < kDmeKc; ++r) {
+                double a[4], b[4];
+#pragma unroll
+                for (int p = 0; p < 4; ++p) {
+                    a[p] = A[r][ty + 16 * p];
+                    b[p] = B[r][tx + 16 * p];
+                }
+#pragma unroll
+                for (int p = 0; p < 4; ++p)
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const double d = a[p] - b[q];
+                        acc[p][q] = acc[p][q] + d * d;  // two roundings, as the reference
+                    }
+            }
+            __syncthreads();
+        }
+#pragma unroll
+        for (int p = 0; p < 4; ++p)
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                S[(long long)(i0 + ty + 16 * p) * fp + j0 + tx + 16 * q] = acc[p][q];
+    }
+}
+
+// out[i][j] = out[j][i] = (f32) sqrt(S / n_pairs) (plan.rs:637-644); 0 on the diagonal.
+__global__ __launch_bounds__(kPwBlock) void dme_finalize_kernel(const double *__restrict__ S, int fp,
+                                                                int F, double n_pairs,
+                                                                float *__restrict__ out,
+                                                                double *__restrict__ out64) {
+    const long long total = (long long)F * F;
+    for (long long t = (long long)blockIdx.x * kPwBlock + threadIdx.x; t < total;
+         t += (long long)gridDim.x * kPwBlock) {
+        const int i = (int)(t / F), j = (int)(t - (long long)i * F);
+        double v = 0.0;
+        if (i != j) {
+            const double sum = i < j ? S[(long long)i * fp + j] : S[(long long)j * fp + i];
+            v = sqrt(sum / n_pairs);
+        }
+        out[t] = (float)v;
+        if (out64) out64[t] = v;
+    }
+}
+
 }  // namespace
 
 struct wmd_pairwise {
@@ -393,13 +544,65 @@ struct wmd_pairwise {
     HostBuf p_stage[2];
     Event p_copied[2], p_used[2];
     std::vector<int2> tiles;
+    // metric "dme": pbc (none / orthorhombic) and, for orthorhombic, the box
+    // lengths of every appended frame ([F][3], f64 of the f32 lengths)
+    int pbc = WMD_PBC_NONE;
+    std::vector<double> box3;
+    DevBuf d_dme_t, d_dme_d, d_dme_s, d_box3, d_dme_tiles;
+    size_t dme_tiles_n = 0;
 };
 
 namespace {
 
-// The F x F matrix of a store [F][3][stride] on stream st.
+// Metric "dme" on a store [F][3][stride]; d_box3: NULL or [F][3] f64 box lengths.
+void dme_run(wmd_pairwise *h, const float *d_soa, size_t stride, size_t F, float *d_out,
+             double *d_out64, const double *d_box3, hipStream_t st) {
+    const int n = (int)h->sel.size();
+    const long long n_pairs = (long long)n * (n - 1) / 2;
+    const int T = (int)((F + kDmeTile - 1) / kDmeTile);
+    const int fp = T * kDmeTile;
+    WMD_REQUIRE((size_t)fp * fp * 8 <= (size_t(32) << 30), WMD_ERR_UNSUPPORTED,
+                "dme: the frame-pair sums exceed 32 GB");
+    float *d_t = (float *)h->d_dme_t.reserve((size_t)3 * n * fp * 4);
+    hipLaunchKernelGGL(dme_transpose_kernel, dim3(pw_grid((long long)n * fp)), dim3(kPwBlock), 0, st,
+                       d_soa, (int)stride, n, (int)F, fp, d_t);
+    WMD_HIP_CHECK(hipGetLastError());
+    const size_t nt = (size_t)T * (T + 1) / 2;
+    if (h->dme_tiles_n != nt) {
+        std::vector<int2> tl;
+        tl.reserve(nt);
+        for (int i = 0; i < T; ++i)
+            for (int j = i; j < T; ++j) tl.push_back(make_int2(i, j));
+        int2 *d = (int2 *)h->d_dme_tiles.reserve(nt * sizeof(int2));
+        WMD_HIP_CHECK(hipMemcpyAsync(d, tl.data(), nt * sizeof(int2), hipMemcpyHostToDevice, st));
+        WMD_HIP_CHECK(hipStreamSynchronize(st));  // tl dies at scope exit
+        h->dme_tiles_n = nt;
+    }
+    // blocks of pair distances: about 256 MB of D per block
+    const long long per = std::max<long long>(kDmeKc, (long long)((size_t(256) << 20) / ((size_t)fp * 8)) / kDmeKc * kDmeKc);
+    const long long K = std::min<long long>(per, n_pairs);
+    double *d_d = (double *)h->d_dme_d.reserve((size_t)K * fp * 8);
+    double *d_s = (double *)h->d_dme_s.reserve((size_t)fp * fp * 8);
+    ProfScope prof(h->ctx, st, "pairwise_dme");
+    for (long long k0 = 0; k0 < n_pairs; k0 += K) {
+        const int kb = (int)std::min<long long>(K, n_pairs - k0);
+        hipLaunchKernelGGL(dme_dist_kernel, dim3((unsigned)std::min(kb, 65536)), dim3(kPwBlock), 0, st,
+                           (const float *)d_t, n, (int)F, fp, k0, kb, d_box3, d_d);
+        WMD_HIP_CHECK(hipGetLastError());
+        hipLaunchKernelGGL(dme_sum_kernel, dim3((unsigned)std::min<size_t>(nt, 65536)), dim3(kPwBlock), 0, st,
+                           (const double *)d_d, fp, kb, (const int2 *)h->d_dme_tiles.ptr, (int)nt,
+                           k0 == 0 ? 1 : 0, d_s);
+        WMD_HIP_CHECK(hipGetLastError());
+    }
+    hipLaunchKernelGGL(dme_finalize_kernel, dim3(pw_grid((long long)(F * F))), dim3(kPwBlock), 0, st,
+                       (const double *)d_s, fp, (int)F, (double)n_pairs, d_out, d_out64);
+    WMD_HIP_CHECK(hipGetLastError());
+}
+
+// The F x F matrix of a store [F][3][stride] on stream st (d_box3: the dme
+// metric's box lengths, NULL without PBC).
 void pw_run(wmd_pairwise *h, const float *d_soa, size_t stride, size_t F, float *d_out,
-            double *d_out64, hipStream_t st) {
+            double *d_out64, hipStream_t st, const double *d_box3 = nullptr) {
     const int n = (int)h->sel.size();
     if (F == 0) return;
     WMD_REQUIRE(stride >= (size_t)n && stride % 4 == 0, WMD_ERR_INVALID,
@@ -410,6 +613,10 @@ void pw_run(wmd_pairwise *h, const float *d_soa, size_t stride, size_t F, float 
         hipLaunchKernelGGL(pw_zero_kernel, dim3(pw_grid((long long)(F * F))), dim3(kPwBlock), 0, st,
                            d_out, d_out64, (long long)(F * F));
         WMD_HIP_CHECK(hipGetLastError());
+        return;
+    }
+    if (h->metric == WMD_PAIRWISE_DME) {
+        dme_run(h, d_soa, stride, F, d_out, d_out64, d_box3, st);
         return;
     }
     double *cen = (double *)h->d_cen.reserve(F * 3 * sizeof(double));
@@ -455,8 +662,6 @@ wmd_status wmd_pairwise_create(wmd_ctx *ctx, const uint32_t *sel, size_t n_sel, 
     WMD_REQUIRE(metric == WMD_PAIRWISE_RMS || metric == WMD_PAIRWISE_NOFIT ||
                     metric == WMD_PAIRWISE_DME,
                 WMD_ERR_PARSE, "metric must be rms, nofit or dme");
-    WMD_REQUIRE(metric != WMD_PAIRWISE_DME, WMD_ERR_UNSUPPORTED,
-                "the dme metric has no device path");
     WMD_REQUIRE(n_sel < (1u << 30), WMD_ERR_UNSUPPORTED, "selection too large");
     bind(ctx);
     auto h = std::make_unique<wmd_pairwise>();
@@ -471,11 +676,51 @@ wmd_status wmd_pairwise_create(wmd_ctx *ctx, const uint32_t *sel, size_t n_sel, 
     WMD_API_END
 }
 
+wmd_status wmd_pairwise_create_dme(wmd_ctx *ctx, const uint32_t *sel, size_t n_sel, int pbc,
+                                   wmd_pairwise **out) {
+    if (pbc != WMD_PBC_NONE && pbc != WMD_PBC_ORTHO) {  // parse_pbc (pack.rs:2665-2673)
+        if (out) *out = nullptr;
+        set_error("pbc must be 'orthorhombic' or 'none'");
+        return WMD_ERR_PARSE;
+    }
+    wmd_status st = wmd_pairwise_create(ctx, sel, n_sel, WMD_PAIRWISE_DME, out);
+    if (st == WMD_OK) (*out)->pbc = pbc;
+    return st;
+}
+
+static wmd_status pairwise_append_impl(wmd_pairwise *h, const float *coords_xyzw, size_t n_atoms,
+                                       size_t n_frames, const float *boxes, bool with_boxes);
+
 wmd_status wmd_pairwise_append(wmd_pairwise *h, const float *coords_xyzw, size_t n_atoms,
                                size_t n_frames) {
+    return pairwise_append_impl(h, coords_xyzw, n_atoms, n_frames, nullptr, false);
+}
+
+wmd_status wmd_pairwise_append_boxes(wmd_pairwise *h, const float *coords_xyzw, size_t n_atoms,
+                                     size_t n_frames, const float *boxes) {
+    return pairwise_append_impl(h, coords_xyzw, n_atoms, n_frames, boxes, true);
+}
+
+static wmd_status pairwise_append_impl(wmd_pairwise *h, const float *coords_xyzw, size_t n_atoms,
+                                       size_t n_frames, const float *boxes, bool with_boxes) {
     WMD_API_BEGIN
     WMD_REQUIRE(h != nullptr, WMD_ERR_INVALID, "null handle");
     if (n_frames == 0) return WMD_OK;
+    // dme with PBC keeps every frame's box lengths (plan.rs:549-556, box_lengths
+    // pbc_math.rs:11-15); other metrics ignore boxes, as the reference does
+    std::vector<double> new_box3;
+    if (h->metric == WMD_PAIRWISE_DME && h->pbc == WMD_PBC_ORTHO) {
+        WMD_REQUIRE(with_boxes && boxes != nullptr, WMD_ERR_MISMATCH, "pairwise_rmsd requires orthorhombic box");
+        new_box3.resize(3 * n_frames);
+        for (size_t f = 0; f < n_frames; ++f) {
+            const float *m = boxes + 9 * f;
+            WMD_REQUIRE(m[1] == 0.f && m[2] == 0.f && m[3] == 0.f && m[5] == 0.f && m[6] == 0.f && m[7] == 0.f,
+                        WMD_ERR_MISMATCH, "orthorhombic box required for PBC");
+            new_box3[3 * f] = m[0];
+            new_box3[3 * f + 1] = m[4];
+            new_box3[3 * f + 2] = m[8];
+        }
+    }
     WMD_REQUIRE(coords_xyzw != nullptr, WMD_ERR_INVALID, "null coords");
     for (uint32_t a : h->sel)
         WMD_REQUIRE(a < n_atoms, WMD_ERR_MISMATCH,
@@ -530,6 +775,7 @@ wmd_status wmd_pairwise_append(wmd_pairwise *h, const float *coords_xyzw, size_t
     }
     WMD_HIP_CHECK(hipStreamSynchronize(st));
     h->frames += n_frames;
+    h->box3.insert(h->box3.end(), new_box3.begin(), new_box3.end());
     WMD_API_END
 }
 
@@ -549,7 +795,14 @@ wmd_status wmd_pairwise_finalize(wmd_pairwise *h, float *out) {
     bind(h->ctx);
     hipStream_t st = h->ctx->stream;
     float *d_out = (float *)h->d_out.reserve(F * F * 4);
-    pw_run(h, (const float *)h->d_store->ptr, (size_t)h->stride, F, d_out, nullptr, st);
+    const double *d_box3 = nullptr;
+    if (h->metric == WMD_PAIRWISE_DME && h->pbc == WMD_PBC_ORTHO) {
+        WMD_REQUIRE(h->box3.size() == 3 * F, WMD_ERR_MISMATCH, "pairwise_rmsd requires orthorhombic box");
+        double *d = (double *)h->d_box3.reserve(h->box3.size() * 8);
+        WMD_HIP_CHECK(hipMemcpyAsync(d, h->box3.data(), h->box3.size() * 8, hipMemcpyHostToDevice, st));
+        d_box3 = d;
+    }
+    pw_run(h, (const float *)h->d_store->ptr, (size_t)h->stride, F, d_out, nullptr, st, d_box3);
     WMD_HIP_CHECK(hipMemcpyAsync(out, d_out, F * F * 4, hipMemcpyDeviceToHost, st));
     WMD_HIP_CHECK(hipStreamSynchronize(st));
     WMD_API_END
@@ -568,6 +821,8 @@ wmd_status wmd_pairwise_run_device(wmd_pairwise *h, const float *d_store, size_t
     WMD_REQUIRE(h != nullptr, WMD_ERR_INVALID, "null handle");
     WMD_REQUIRE(n_frames == 0 || (d_store && d_out), WMD_ERR_INVALID, "null device buffer");
     WMD_REQUIRE(((uintptr_t)d_store & 15) == 0, WMD_ERR_INVALID, "device store must be 16-byte aligned");
+    WMD_REQUIRE(!(h->metric == WMD_PAIRWISE_DME && h->pbc == WMD_PBC_ORTHO), WMD_ERR_UNSUPPORTED,
+                "dme with PBC takes per-frame boxes: use wmd_pairwise_append_boxes and wmd_pairwise_finalize");
     bind(h->ctx);
     pw_run(h, d_store, stride, n_frames, d_out, d_out64, pick_stream(h->ctx, stream));
     WMD_API_END

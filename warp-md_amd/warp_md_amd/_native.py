@@ -65,7 +65,8 @@ SYMBOLS = (
     "wmd_batch_create", "wmd_batch_store_layout", "wmd_batch_set_boxes", "wmd_batch_set_tuning",
     "wmd_batch_autotune", "wmd_batch_run_device", "wmd_batch_read_counts", "wmd_batch_reset",
     "wmd_batch_plan_info", "wmd_batch_destroy", "wmd_autotune_export", "wmd_autotune_import",
-    "wmd_pairwise_create", "wmd_pairwise_append", "wmd_pairwise_frames", "wmd_pairwise_finalize",
+    "wmd_pairwise_create", "wmd_pairwise_create_dme", "wmd_pairwise_append",
+    "wmd_pairwise_append_boxes", "wmd_pairwise_frames", "wmd_pairwise_finalize",
     "wmd_pairwise_store_stride", "wmd_pairwise_run_device", "wmd_pairwise_destroy",
     "wmd_align_create", "wmd_align_set_reference", "wmd_align_run", "wmd_align_run_device",
     "wmd_align_destroy", "wmd_rdf_create_groups", "wmd_pairwise_kmeans", "wmd_dbscan",
@@ -175,7 +176,9 @@ def lib():
             "wmd_autotune_export": ([ctypes.c_char_p, S, ctypes.POINTER(S)], St),
             "wmd_autotune_import": ([ctypes.c_char_p], St),
             "wmd_pairwise_create": ([P, P, S, I, PP], St),
+            "wmd_pairwise_create_dme": ([P, P, S, I, PP], St),
             "wmd_pairwise_append": ([P, P, S, S], St),
+            "wmd_pairwise_append_boxes": ([P, P, S, S, P], St),
             "wmd_pairwise_frames": ([P, ctypes.POINTER(S)], St),
             "wmd_pairwise_finalize": ([P, P], St),
             "wmd_pairwise_store_stride": ([P, ctypes.POINTER(S)], St),

@@ -496,15 +496,19 @@ class PairwiseRmsdPlan:
 
     Mirrors PyPairwiseRmsdPlan (crates/traj-py/src/io.rs:2759-2800) over
     PairwiseRmsdPlan (crates/traj-engine/src/plans/rmsd/plan.rs:504-675):
-    metric "rms" (after optimal superposition) or "nofit"; "dme" has no
-    device path and is refused.  Frames go to a device store as they are
-    read; the matrix is one FP64 Gram product plus per-pair QCP
+    metric "rms" (after optimal superposition), "nofit", or "dme" (the RMS
+    difference of the frames' pair-distance vectors, with the orthorhombic
+    minimum image when pbc="orthorhombic", plan.rs:612-650).  Frames go to a
+    device store as they are read; "rms" / "nofit" are one FP64 Gram product
+    plus per-pair QCP, "dme" blocks of f64 pair distances and frame-pair sums
     (``wmd_pairwise_*``).  Returns a (F, F) float32 array.
     """
 
     def __init__(self, selection, metric: str = "rms", pbc: str = "none"):
         if metric not in _PAIRWISE_METRICS:
             raise RuntimeError("metric must be 'rms', 'nofit', or 'dme'")  # pack.rs:2675-2684
+        if pbc not in ("orthorhombic", "none"):  # parse_pbc, pack.rs:2665-2673
+            raise RuntimeError("pbc must be 'orthorhombic' or 'none'")
         self.metric = metric
         self.pbc = _parse_pbc(pbc)
         self.selection = _indices(selection)
@@ -520,8 +524,13 @@ class PairwiseRmsdPlan:
         if sel.size and int(sel.max()) >= n_atoms:
             raise RuntimeError("invalid selection: atom index out of range")
         h = ctypes.c_void_p()
-        check(lib().wmd_pairwise_create(ctx.handle, ptr(sel), sel.size,
-                                        _PAIRWISE_METRICS[self.metric], ctypes.byref(h)))
+        dme_pbc = self.metric == "dme" and self.pbc == _native.PBC_ORTHO
+        if self.metric == "dme":
+            check(lib().wmd_pairwise_create_dme(ctx.handle, ptr(sel), sel.size, self.pbc,
+                                                ctypes.byref(h)))
+        else:
+            check(lib().wmd_pairwise_create(ctx.handle, ptr(sel), sel.size,
+                                            _PAIRWISE_METRICS[self.metric], ctypes.byref(h)))
         try:
             tr = select_frames(traj, frame_indices)
             tr.reset()
@@ -530,8 +539,14 @@ class PairwiseRmsdPlan:
                 chunk = tr.read_chunk4(step)
                 if chunk is None:
                     break
-                x4, _ = chunk
-                check(lib().wmd_pairwise_append(h, ptr(x4), n_atoms, x4.shape[0]))
+                x4, boxes = chunk
+                if dme_pbc:
+                    if boxes is None:  # box_lengths of Box3::None (pbc_math.rs:11-15)
+                        raise RuntimeError("mismatch: orthorhombic box required for PBC")
+                    check(lib().wmd_pairwise_append_boxes(h, ptr(x4), n_atoms, x4.shape[0],
+                                                          ptr(np.ascontiguousarray(boxes, np.float32))))
+                else:
+                    check(lib().wmd_pairwise_append(h, ptr(x4), n_atoms, x4.shape[0]))
             nf = ctypes.c_size_t()
             check(lib().wmd_pairwise_frames(h, ctypes.byref(nf)))
             out = np.zeros((nf.value, nf.value), np.float32)
