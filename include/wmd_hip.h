@@ -214,7 +214,8 @@ wmd_status wmd_sf_finalize(const uint64_t *counts, size_t bins, double r_max, do
  * over (NULL => same as fit_sel, the RmsdPlan case; the split form is the
  * fit-on-A / RMSD-over-B composition of rms.rs:643-676).  ref_xyzw:
  * [n_ref_atoms][4] reference coordinates indexed by atom.  align = 0 gives the
- * unfitted RMSD (rmsd/plan.rs:1093-1110; requires rmsd_sel == fit_sel). */
+ * unfitted RMSD (rmsd/plan.rs:1093-1110) over rmsd_sel (fit_sel when rmsd_sel
+ * is NULL); the split form then leaves fit_sel unused, as rms.rs:664-666. */
 wmd_status wmd_rmsd_create(wmd_ctx *ctx, const uint32_t *fit_sel, size_t n_fit,
                            const uint32_t *rmsd_sel, size_t n_rmsd, const float *ref_xyzw,
                            size_t n_ref_atoms, int align, wmd_rmsd **out);
@@ -262,9 +263,15 @@ wmd_status wmd_rmsd_run_device_noalloc(wmd_rmsd *rmsd, const float *d_store, siz
  * d_gout [F] f32 = dL/dRMSD per frame; d_rmsd64, d_rot, d_cx: the outputs of
  * the forward run on the same store.  Writes d_grad [F][3][stride] (store
  * layout, 0 in padding slots) = dL/dx: ((x - cx) - R^T (y - cy)) / (n RMSD)
- * per atom for the fitted RMSD (envelope theorem; rmsd_sel must equal
- * fit_sel, else WMD_ERR_UNSUPPORTED), (x - y) / (n RMSD) unfitted.  The
- * reference coordinates are constants.  Asynchronous on `stream`. */
+ * per atom for the fitted RMSD over its own fit set (envelope theorem);
+ * (x - y) / (n RMSD) unfitted; for a fit on set A and the RMSD over set B,
+ * with u = x - cx, v = y - cy, r = R u - v, the adjoint through the rotation:
+ *   dRMSD/dx_i = (2 R^T r_i [i in B] - (2 / n_A) R^T sum_B r [i in A]
+ *                 + 2 G^T v_i [i in A]) / (2 n_B RMSD),
+ * G = [z]x R, z = (tr(P) I - P)^-1 q, P = R H (H the fit set's centred
+ * cross-covariance), q the axial vector of R sum_B u r^T (the store is read
+ * twice).  The reference coordinates are constants.  Asynchronous on
+ * `stream`. */
 wmd_status wmd_rmsd_backward_device(wmd_rmsd *rmsd, const float *d_store, size_t stride,
                                     size_t n_frames, const double *d_rmsd64, const double *d_rot,
                                     const double *d_cx, const float *d_gout, float *d_grad,

@@ -189,3 +189,60 @@ def fit_rmsd_grad(x, y):
     x = np.asarray(x, np.float64)
     y = np.asarray(y, np.float64)
     return ((x - cx) - (y - cy) @ R) / (len(x) * rmsd)
+
+
+def split_rmsd_parts(x, y, fit, rmsd_sel):
+    """Fit on atoms `fit`, RMSD over atoms `rmsd_sel` (float64), the
+    composition of kabsch_rotation (geometry_math.rs:169-297) and the residual
+    of rms.rs:656-672: R (x - cx) ~ (y - cy) from the fit set's centroids and
+    cross-covariance H; returns (rmsd, R, cx, cy, H)."""
+    x = np.asarray(x, np.float64)
+    y = np.asarray(y, np.float64)
+    fit = np.asarray(fit, np.int64)
+    rsel = np.asarray(rmsd_sel, np.int64)
+    cx = x[fit].mean(0)
+    cy = y[fit].mean(0)
+    H = (x[fit] - cx).T @ (y[fit] - cy)
+    U, _, Vt = np.linalg.svd(H)
+    D = np.eye(3)
+    D[2, 2] = np.sign(np.linalg.det(Vt.T @ U.T))
+    R = Vt.T @ D @ U.T
+    res = (x[rsel] - cx) @ R.T - (y[rsel] - cy)
+    return math.sqrt((res * res).sum() / len(rsel)), R, cx, cy, H
+
+
+def split_rmsd_grad(x, y, fit, rmsd_sel):
+    """dRMSD_B/dx for a fit on set A and the RMSD over set B (float64).
+
+    With u_i = x_i - cx, v_i = y_i - cy, r_i = R u_i - v_i and
+    E = sum_B |r_i|^2 (RMSD = sqrt(E / n_B)):
+      dE/dx_i = 2 R^T r_i [i in B] - (2 / n_A) R^T sum_B r [i in A] + 2 G^T v_i [i in A].
+    The last term is the adjoint through the rotation: the optimum keeps
+    P = R H symmetric, so a perturbation dR = [k]x R obeys
+    (tr(P) I - P) k = -2 axial(skew(R dH)); with W = sum_B u_i r_i^T,
+    Q = R W, q = (Q23 - Q32, Q31 - Q13, Q12 - Q21), z = (tr(P) I - P)^-1 q and
+    G = [z]x R, the rotation's share of dE is 2 tr(G dH).  For A = B it
+    vanishes (q = 0), leaving the envelope-theorem form of fit_rmsd_grad.
+    Duplicated indices add their terms."""
+    x = np.asarray(x, np.float64)
+    y = np.asarray(y, np.float64)
+    fit = np.asarray(fit, np.int64)
+    rsel = np.asarray(rmsd_sel, np.int64)
+    rmsd, R, cx, cy, H = split_rmsd_parts(x, y, fit, rsel)
+    g = np.zeros_like(x)
+    if rmsd == 0.0:
+        return g
+    u = x[rsel] - cx
+    v = y[rsel] - cy
+    r = u @ R.T - v
+    np.add.at(g, rsel, 2.0 * (r @ R))
+    Q = R @ (u.T @ r)
+    q = np.array([Q[1, 2] - Q[2, 1], Q[2, 0] - Q[0, 2], Q[0, 1] - Q[1, 0]])
+    P = R @ H
+    P = 0.5 * (P + P.T)
+    z = np.linalg.solve(np.trace(P) * np.eye(3) - P, q)
+    Z = np.array([[0.0, -z[2], z[1]], [z[2], 0.0, -z[0]], [-z[1], z[0], 0.0]])
+    G = Z @ R
+    np.add.at(g, fit, 2.0 * ((y[fit] - cy) @ G))
+    np.add.at(g, fit, -(2.0 / len(fit)) * (r.sum(0) @ R))
+    return g / (2.0 * len(rsel) * rmsd)

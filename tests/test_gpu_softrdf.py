@@ -179,29 +179,101 @@ def test_rmsd_backward_fd():
             assert abs(g[a, d] - fd) <= 1e-4 * np.abs(g).max()
 
 
-def test_split_rmsd_backward_unsupported():
+def _chain300(frames, seed):
+    """A 300-atom random-walk chain, frames randomly moved and noised (the
+    split tests fit on every 10th atom)."""
+    y, x = _chain(300, frames, seed=seed)
+    return y, x
+
+
+@pytest.mark.parametrize("align", [True, False])
+def test_split_rmsd_backward_matches_oracle(align):
+    """Fit on every 10th atom, RMSD over all 300: the adjoint through the
+    rotation against the float64 restatement (itself pinned by FP64 central
+    differences and torch autograd), and the forward against the oracle."""
+    from warp_md_amd.autograd import FittedRmsd
+    y, x = _chain300(4, seed=21)
+    fit = np.arange(0, 300, 10)
+    plan = FittedRmsd(y, align=align, fit_selection=fit)
+    xt = torch.tensor(x, device="cuda", requires_grad=True)
+    r = plan(xt)
+    w = torch.tensor([0.5, -1.0, 2.0, 1.5], dtype=torch.float64, device="cuda")
+    (r * w).sum().backward()
+    g = xt.grad.cpu().numpy()
+    allat = np.arange(300)
+    for f in range(4):
+        x64, y64 = x[f].astype(np.float64), y.astype(np.float64)
+        if align:
+            ref_r = S.split_rmsd_parts(x64, y64, fit, allat)[0]
+            ref_g = S.split_rmsd_grad(x64, y64, fit, allat)
+        else:
+            diff = x64 - y64
+            ref_r = math.sqrt((diff * diff).sum() / len(x64))
+            ref_g = diff / (len(x64) * ref_r)
+        assert float(r[f].detach()) == pytest.approx(ref_r, abs=1e-5)
+        assert _rel(g[f], float(w[f]) * ref_g) < 1e-5
+
+
+def test_split_rmsd_backward_fd_and_duplicates():
+    """FP64 central differences of the oracle's split RMSD on the same f32
+    inputs, with duplicated fit atoms and a partial RMSD set (atoms in both
+    sets, fit-only and RMSD-only)."""
     from warp_md_amd import _native as N
+    y, x = _chain300(1, seed=22)
+    fit = np.concatenate([np.arange(0, 300, 10), [10, 20]]).astype(np.uint32)
+    rsel = np.arange(100, 300, dtype=np.uint32)
     ctx = N.context(0)
-    y, x = _chain(40, 1, seed=1)
-    xyzw = np.zeros((40, 4), np.float32)
+    xyzw = np.zeros((300, 4), np.float32)
     xyzw[:, :3] = y
-    fit = np.arange(0, 40, 2, dtype=np.uint32)
-    allat = np.arange(40, dtype=np.uint32)
     h = ctypes.c_void_p()
-    N.check(N.lib().wmd_rmsd_create(ctx.handle, N.ptr(fit), fit.size, N.ptr(allat), allat.size,
-                                    N.ptr(xyzw), 40, 1, ctypes.byref(h)))
+    N.check(N.lib().wmd_rmsd_create(ctx.handle, N.ptr(fit), fit.size, N.ptr(rsel), rsel.size,
+                                    N.ptr(xyzw), 300, 1, ctypes.byref(h)))
     try:
-        st = torch.zeros((1, 3, 64), device="cuda")
-        one = torch.ones(1, device="cuda")
-        r64 = torch.ones(1, dtype=torch.float64, device="cuda")
-        rot = torch.zeros((1, 9), dtype=torch.float64, device="cuda")
-        status = N.lib().wmd_rmsd_backward_device(
-            h, ctypes.c_void_p(st.data_ptr()), 64, 1, ctypes.c_void_p(r64.data_ptr()),
-            ctypes.c_void_p(rot.data_ptr()), ctypes.c_void_p(rot.data_ptr()),
-            ctypes.c_void_p(one.data_ptr()), ctypes.c_void_p(st.data_ptr()), None)
-        assert status == N.WMD_ERR_UNSUPPORTED
+        import _gpu
+        atoms = _gpu.rmsd_layout(h)
+        store = torch.tensor(_gpu.host_store(x, atoms), device="cuda")
+        stride = atoms.size
+        r32 = torch.empty(1, dtype=torch.float32, device="cuda")
+        r64 = torch.empty(1, dtype=torch.float64, device="cuda")
+        rot = torch.empty((1, 9), dtype=torch.float64, device="cuda")
+        cx = torch.empty((1, 3), dtype=torch.float64, device="cuda")
+        V = lambda t: ctypes.c_void_p(t.data_ptr())
+        N.check(N.lib().wmd_rmsd_run_device(h, V(store), stride, 1, V(r32), V(r64), None, V(rot), V(cx), None))
+        gout = torch.ones(1, dtype=torch.float32, device="cuda")
+        gs = torch.empty_like(store)
+        N.check(N.lib().wmd_rmsd_backward_device(h, V(store), stride, 1, V(r64), V(rot), V(cx), V(gout),
+                                                 V(gs), None))
+        torch.cuda.synchronize()
+        gs = gs.cpu().numpy()[0]
     finally:
         N.lib().wmd_rmsd_destroy(h)
+    g = np.zeros((300, 3))
+    valid = atoms != 0xFFFFFFFF
+    np.add.at(g, atoms[valid].astype(np.int64), gs[:, valid].T.astype(np.float64))
+    x64, y64 = x[0].astype(np.float64), y.astype(np.float64)
+    assert float(r64.cpu()[0]) == pytest.approx(S.split_rmsd_parts(x64, y64, fit, rsel)[0], abs=1e-9)
+    hh = 1e-6
+    scale = np.abs(g).max()
+    for a in [0, 10, 20, 50, 99, 100, 150, 200, 290, 299]:
+        for d in range(3):
+            xp = x64.copy(); xp[a, d] += hh
+            xm = x64.copy(); xm[a, d] -= hh
+            fd = (S.split_rmsd_parts(xp, y64, fit, rsel)[0] - S.split_rmsd_parts(xm, y64, fit, rsel)[0]) / (2 * hh)
+            assert abs(g[a, d] - fd) <= 1e-5 * scale, (a, d, g[a, d], fd)
+
+
+def test_split_unfitted_rmsd_is_rmsd_over_the_rmsd_set():
+    """align=False with a fit set: the unfitted residual over the RMSD set
+    alone (rms.rs:664-666), through RmsdPlan."""
+    from warp_md_amd import RmsdPlan, System, Trajectory
+    y, x = _chain300(3, seed=23)
+    sysm = System.from_arrays({"name": np.array(["X"] * 300)}, positions0=y)
+    rsel = np.arange(50, 300)
+    out = RmsdPlan(rsel, align=False, fit_selection=np.arange(0, 300, 10)).run(
+        Trajectory.from_numpy(x), sysm, device="cuda:0")
+    diff = x[:, rsel].astype(np.float64) - y[rsel].astype(np.float64)
+    want = np.sqrt((diff * diff).sum(axis=(1, 2)) / rsel.size)
+    assert np.max(np.abs(out.astype(np.float64) - want)) <= 1e-5
 
 
 def test_sigma_outside_window_rejected():

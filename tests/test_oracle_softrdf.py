@@ -189,3 +189,84 @@ def test_oracle_gradient_matches_dense_torch_autograd():
     assert float(lt) == pytest.approx(loss, rel=1e-12)
     gt = xt.grad.numpy()
     assert np.max(np.abs(gt - g)) <= 1e-10 * np.max(np.abs(g))
+
+
+def _chain300(seed, noise=0.5):
+    rng = np.random.default_rng(seed)
+    d = rng.normal(size=(30, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    ca = np.cumsum(3.8 * d, axis=0)
+    y = np.repeat(ca, 10, axis=0) + 1.5 * rng.normal(size=(300, 3)) * (np.arange(300) % 10 != 0)[:, None]
+    q = rng.normal(size=4)
+    q /= np.linalg.norm(q)
+    w, a, b, c = q
+    R = np.array([[1 - 2 * (b * b + c * c), 2 * (a * b - c * w), 2 * (a * c + b * w)],
+                  [2 * (a * b + c * w), 1 - 2 * (a * a + c * c), 2 * (b * c - a * w)],
+                  [2 * (a * c - b * w), 2 * (b * c + a * w), 1 - 2 * (a * a + b * b)]])
+    x = y @ R.T + 5.0 + noise * rng.normal(size=(300, 3))
+    return x, y
+
+
+def test_split_rmsd_gradient_matches_finite_differences():
+    """300-atom chain fitted on every 10th atom, RMSD over all 300 atoms:
+    the adjoint through the rotation against FP64 central differences."""
+    x, y = _chain300(11)
+    fit = np.arange(0, 300, 10)
+    rsel = np.arange(300)
+    g = S.split_rmsd_grad(x, y, fit, rsel)
+    h = 1e-6
+    worst = 0.0
+    scale = np.abs(g).max()
+    for i in list(range(0, 300, 7)) + [0, 10, 290]:
+        for d in range(3):
+            xp = x.copy()
+            xm = x.copy()
+            xp[i, d] += h
+            xm[i, d] -= h
+            fd = (S.split_rmsd_parts(xp, y, fit, rsel)[0] - S.split_rmsd_parts(xm, y, fit, rsel)[0]) / (2 * h)
+            worst = max(worst, abs(fd - g[i, d]))
+    assert worst <= 1e-7 * scale, (worst, scale)
+    # the rotation term matters here: dropping it fails the same check
+    assert np.abs(g - _without_rotation_term(x, y, fit, rsel)).max() > 1e-3 * scale
+
+
+def _without_rotation_term(x, y, fit, rsel):
+    rmsd, R, cx, cy, _ = S.split_rmsd_parts(x, y, fit, rsel)
+    r = (x[rsel] - cx) @ R.T - (y[rsel] - cy)
+    g = np.zeros_like(x)
+    np.add.at(g, rsel, 2.0 * (r @ R))
+    np.add.at(g, fit, -(2.0 / len(fit)) * (r.sum(0) @ R))
+    return g / (2.0 * len(rsel) * rmsd)
+
+
+def test_split_rmsd_gradient_matches_torch_autograd():
+    """A dense float64 torch restatement (autograd through torch.linalg.svd)."""
+    import torch
+
+    x, y = _chain300(12, noise=0.8)
+    fit = np.concatenate([np.arange(0, 300, 10), [10, 20]])  # duplicates weight the fit
+    rsel = np.concatenate([np.arange(150, 300), [151]])
+    xt = torch.tensor(x, dtype=torch.float64, requires_grad=True)
+    yt = torch.tensor(y, dtype=torch.float64)
+    fa = torch.as_tensor(fit)
+    rb = torch.as_tensor(rsel)
+    cx = xt[fa].mean(0)
+    cy = yt[fa].mean(0)
+    H = (xt[fa] - cx).T @ (yt[fa] - cy)
+    U, _, Vh = torch.linalg.svd(H)
+    dsign = torch.sign(torch.linalg.det(Vh.T @ U.T)).detach()
+    D = torch.diag(torch.stack([torch.ones((), dtype=torch.float64), torch.ones((), dtype=torch.float64), dsign]))
+    R = Vh.T @ D @ U.T
+    res = (xt[rb] - cx) @ R.T - (yt[rb] - cy)
+    rmsd = torch.sqrt((res * res).sum() / len(rsel))
+    rmsd.backward()
+    g = S.split_rmsd_grad(x, y, fit, rsel)
+    assert abs(rmsd.item() - S.split_rmsd_parts(x, y, fit, rsel)[0]) < 1e-12
+    np.testing.assert_allclose(g, xt.grad.numpy(), rtol=0, atol=1e-10 * np.abs(g).max())
+
+
+def test_split_rmsd_gradient_same_sets_is_envelope_form():
+    x, y = _chain300(13)
+    sel = np.arange(300)
+    np.testing.assert_allclose(S.split_rmsd_grad(x, y, sel, sel), S.fit_rmsd_grad(x, y), rtol=0,
+                               atol=1e-12)

@@ -104,6 +104,18 @@ __device__ __forceinline__ void acc_raw(double *a, float xf, float yf, float zf,
     a[0] = fma(dx, dx, fma(dy, dy, fma(dz, dz, a[0])));
 }
 
+// The six second moments x x^T (xx, xy, xz, yy, yz, zz) of the RMSD set, for
+// the adjoint of a split fit.
+__device__ __forceinline__ void acc_xx(double *C, float xf, float yf, float zf) {
+    const double x = xf, y = yf, z = zf;
+    C[0] = fma(x, x, C[0]);
+    C[1] = fma(x, y, C[1]);
+    C[2] = fma(x, z, C[2]);
+    C[3] = fma(y, y, C[3]);
+    C[4] = fma(y, z, C[4]);
+    C[5] = fma(z, z, C[5]);
+}
+
 template <int MODE>
 __device__ __forceinline__ void acc_group(double *A, double *B, int seg, const float4 &x4,
                                           const float4 &y4, const float4 &z4, const float4 &rx,
@@ -134,13 +146,15 @@ __device__ __forceinline__ void acc_group(double *A, double *B, int seg, const f
     }
 }
 
-// partials: [F][n_tiles][NV] doubles, NV = 1 (raw), 13 (same), 26 (split)
-template <int MODE>
+// partials: [F][n_tiles][NV] doubles, NV = 1 (raw), 13 (same), 26 (split;
+// 32 with XX: the RMSD set's second moments for the split-fit adjoint)
+template <int MODE, bool XX = false>
 __global__ __launch_bounds__(kBlock) void rmsd_moments_kernel(
     const float *__restrict__ store, long long frame_stride, int stride, int n_frames,
     const float *__restrict__ ref, int n_groups, int seg1, int seg2, int n_tiles, int gpt,
     double *__restrict__ partials) {
-    constexpr int NV = MODE == MODE_RAW ? 1 : (MODE == MODE_SAME ? kMom : 2 * kMom);
+    static_assert(!XX || MODE == MODE_SPLIT, "second moments only for a split fit");
+    constexpr int NV = MODE == MODE_RAW ? 1 : (MODE == MODE_SAME ? kMom : 2 * kMom + (XX ? 6 : 0));
     const int wave = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
     const long long unit = (long long)blockIdx.x * (kBlock / 64) + wave;
@@ -157,10 +171,23 @@ __global__ __launch_bounds__(kBlock) void rmsd_moments_kernel(
     const float4 *RZ = (const float4 *)(ref + 2 * stride);
     double A[MODE == MODE_RAW ? 1 : kMom];
     double B[MODE == MODE_SPLIT ? kMom : 1];
+    double C[XX ? 6 : 1];
 #pragma unroll
     for (int k = 0; k < (MODE == MODE_RAW ? 1 : kMom); ++k) A[k] = 0.0;
 #pragma unroll
     for (int k = 0; k < (MODE == MODE_SPLIT ? kMom : 1); ++k) B[k] = 0.0;
+#pragma unroll
+    for (int k = 0; k < (XX ? 6 : 1); ++k) C[k] = 0.0;
+    auto second = [&](int seg, const float4 &x4, const float4 &y4, const float4 &z4) {
+        if constexpr (XX) {
+            if (seg != 1) {  // RMSD set: both or RMSD-only
+                acc_xx(C, x4.x, y4.x, z4.x);
+                acc_xx(C, x4.y, y4.y, z4.y);
+                acc_xx(C, x4.z, y4.z, z4.z);
+                acc_xx(C, x4.w, y4.w, z4.w);
+            }
+        }
+    };
     int g = g0 + lane;
     // two groups in flight per lane
     for (; g + 64 < g1; g += 128) {
@@ -172,12 +199,15 @@ __global__ __launch_bounds__(kBlock) void rmsd_moments_kernel(
         const int s1 = (g + 64 >= seg1) + (g + 64 >= seg2);
         acc_group<MODE>(A, B, s0, x0, y0, z0, a0, b0, c0);
         acc_group<MODE>(A, B, s1, x1, y1, z1, a1, b1, c1);
+        second(s0, x0, y0, z0);
+        second(s1, x1, y1, z1);
     }
     if (g < g1) {
         float4 x0 = X[g], y0 = Y[g], z0 = Z[g];
         float4 a0 = RX[g], b0 = RY[g], c0 = RZ[g];
         const int s0 = (g >= seg1) + (g >= seg2);
         acc_group<MODE>(A, B, s0, x0, y0, z0, a0, b0, c0);
+        second(s0, x0, y0, z0);
     }
     double *out = partials + ((long long)f * n_tiles + t) * NV;
 #pragma unroll
@@ -190,6 +220,13 @@ __global__ __launch_bounds__(kBlock) void rmsd_moments_kernel(
         for (int k = 0; k < kMom; ++k) {
             double v = wave_sum(B[k]);
             if (lane == 0) out[kMom + k] = v;
+        }
+    }
+    if constexpr (XX) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            double v = wave_sum(C[k]);
+            if (lane == 0) out[2 * kMom + k] = v;
         }
     }
 }
@@ -348,6 +385,148 @@ __global__ __launch_bounds__(kBlock) void rmsd_backward_kernel(
     }
 }
 
+// Adjoint of the split fit (fit on set A, RMSD over set B; the composition of
+// rms.rs:643-676), per frame from the moments with second moments (NV = 32)
+// and the forward's rotation.  With u = x - cx, v = y - cy, r = R u - v and
+// E = sum_B |r|^2:
+//   dE/dx_i = 2 R^T r_i [i in B] - (2 / n_A) R^T sum_B r [i in A] + 2 G^T v_i [i in A],
+// where G = [z]x R, z = (tr(P) I - P)^-1 q, P = R H (H the fit set's centred
+// cross-covariance, kept symmetric by the optimum), q the axial vector
+// (Q23 - Q32, Q31 - Q13, Q12 - Q21) of Q = R W, W = sum_B u r^T
+// = Suu R^T - sum_B u v^T.  coef [F][16]: cx (3), G (9), the centroid term
+// (2 / n_A) R^T sum_B r (3), and sc = dL/dRMSD / (2 n_B RMSD) (0 when RMSD = 0).
+__global__ void rmsd_split_adjoint_kernel(const double *__restrict__ partials, int n_frames,
+                                          int n_tiles, RefMoments rm,
+                                          const double *__restrict__ rot,
+                                          const double *__restrict__ rmsd64,
+                                          const float *__restrict__ gout,
+                                          double *__restrict__ coef) {
+    constexpr int NV = 2 * kMom + 6;
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= n_frames) return;
+    double S[NV];
+    for (int k = 0; k < NV; ++k) S[k] = 0.0;
+    for (int t = 0; t < n_tiles; ++t) {
+        const double *p = partials + ((long long)f * n_tiles + t) * NV;
+        for (int k = 0; k < NV; ++k) S[k] += p[k];
+    }
+    const double *SA = S, *SB = S + kMom, *XXB = S + 2 * kMom;
+    double R[9];
+    for (int k = 0; k < 9; ++k) R[k] = rot[(long long)f * 9 + k];
+    double cx[3] = {0, 0, 0}, cy[3] = {0, 0, 0};
+    if (rm.n_a > 0)
+        for (int d = 0; d < 3; ++d) {
+            cx[d] = SA[d] / rm.n_a;
+            cy[d] = rm.sy_a[d] / rm.n_a;
+        }
+    double H[9], MB[9], gx, gy;
+    centred(SA, rm.sy_a, rm.syy_a, rm.n_a, cx, cy, H, &gx, &gy);
+    centred(SB, rm.sy_b, rm.syy_b, rm.n_b, cx, cy, MB, &gx, &gy);
+    // Suu = sum_B (x - cx)(x - cx)^T from the raw second moments
+    const int ix[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+    double Suu[9];
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b)
+            Suu[a * 3 + b] = XXB[ix[a][b]] - cx[a] * SB[b] - SB[a] * cx[b] + rm.n_b * cx[a] * cx[b];
+    double W[9], Q[9], P[9];
+    for (int a = 0; a < 3; ++a)
+        for (int c = 0; c < 3; ++c) {
+            double w = 0.0;
+            for (int b = 0; b < 3; ++b) w += Suu[a * 3 + b] * R[c * 3 + b];
+            W[a * 3 + c] = w - MB[a * 3 + c];
+        }
+    for (int a = 0; a < 3; ++a)
+        for (int c = 0; c < 3; ++c) {
+            double q = 0.0, pp = 0.0;
+            for (int b = 0; b < 3; ++b) {
+                q += R[a * 3 + b] * W[b * 3 + c];
+                pp += R[a * 3 + b] * H[b * 3 + c];
+            }
+            Q[a * 3 + c] = q;
+            P[a * 3 + c] = pp;
+        }
+    const double qv[3] = {Q[5] - Q[7], Q[6] - Q[2], Q[1] - Q[3]};
+    const double tr = P[0] + P[4] + P[8];
+    // M = tr(P) I - sym(P), solved by its adjugate
+    double M[9];
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b)
+            M[a * 3 + b] = (a == b ? tr : 0.0) - 0.5 * (P[a * 3 + b] + P[b * 3 + a]);
+    double adj[9];
+    adj[0] = M[4] * M[8] - M[5] * M[7];
+    adj[1] = M[2] * M[7] - M[1] * M[8];
+    adj[2] = M[1] * M[5] - M[2] * M[4];
+    adj[3] = M[5] * M[6] - M[3] * M[8];
+    adj[4] = M[0] * M[8] - M[2] * M[6];
+    adj[5] = M[2] * M[3] - M[0] * M[5];
+    adj[6] = M[3] * M[7] - M[4] * M[6];
+    adj[7] = M[1] * M[6] - M[0] * M[7];
+    adj[8] = M[0] * M[4] - M[1] * M[3];
+    const double det = M[0] * adj[0] + M[1] * adj[3] + M[2] * adj[6];
+    double z[3] = {0, 0, 0};
+    const double scale = fabs(tr) + 1e-300;
+    if (fabs(det) > 1e-14 * scale * scale * scale)  // else a degenerate fit: no rotation term
+        for (int a = 0; a < 3; ++a) z[a] = (adj[a * 3] * qv[0] + adj[a * 3 + 1] * qv[1] + adj[a * 3 + 2] * qv[2]) / det;
+    const double Z[9] = {0.0, -z[2], z[1], z[2], 0.0, -z[0], -z[1], z[0], 0.0};
+    double *o = coef + (long long)f * 16;
+    for (int d = 0; d < 3; ++d) o[d] = cx[d];
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) {
+            double g = 0.0;
+            for (int c = 0; c < 3; ++c) g += Z[a * 3 + c] * R[c * 3 + b];
+            o[3 + a * 3 + b] = g;
+        }
+    // sum_B r = R sum_B u - sum_B v
+    double su[3], sr[3];
+    for (int d = 0; d < 3; ++d) su[d] = SB[d] - rm.n_b * cx[d];
+    for (int a = 0; a < 3; ++a)
+        sr[a] = R[a * 3] * su[0] + R[a * 3 + 1] * su[1] + R[a * 3 + 2] * su[2] - (rm.sy_b[a] - rm.n_b * cy[a]);
+    const double ca = rm.n_a > 0 ? 2.0 / rm.n_a : 0.0;
+    for (int b = 0; b < 3; ++b) o[12 + b] = ca * (R[b] * sr[0] + R[3 + b] * sr[1] + R[6 + b] * sr[2]);
+    const double r = rmsd64[f];
+    o[15] = (r > 0.0 && rm.n_b > 0) ? (double)gout[f] / (2.0 * rm.n_b * r) : 0.0;
+}
+
+// grad[f][d][slot] of the split fit from the per-frame coefficients (0 for
+// padding slots); a slot is in the fit set unless it is RMSD-only, in the
+// RMSD set unless it is fit-only.
+__global__ __launch_bounds__(kBlock) void rmsd_split_backward_kernel(
+    const float *__restrict__ store, int stride, int n_frames, const float *__restrict__ ref,
+    const unsigned *__restrict__ atoms, int seg1, int seg2, double cy0, double cy1, double cy2,
+    const double *__restrict__ rot, const double *__restrict__ coef, float *__restrict__ grad) {
+    const long long total = (long long)n_frames * stride;
+    for (long long t = (long long)blockIdx.x * kBlock + threadIdx.x; t < total;
+         t += (long long)gridDim.x * kBlock) {
+        const int f = (int)(t / stride);
+        const int k = (int)(t - (long long)f * stride);
+        const double *o = coef + (long long)f * 16;
+        const float *X = store + (long long)f * 3 * stride;
+        float *Gr = grad + (long long)f * 3 * stride;
+        double g[3] = {0.0, 0.0, 0.0};
+        const double sc = o[15];
+        if (atoms[k] != kPad && sc != 0.0) {
+            const double *R = rot + (long long)f * 9;
+            const int seg = (k / 4 >= seg1) + (k / 4 >= seg2);
+            const double u[3] = {X[k] - o[0], X[stride + k] - o[1], X[2 * stride + k] - o[2]};
+            const double v[3] = {ref[k] - cy0, ref[stride + k] - cy1, ref[2 * stride + k] - cy2};
+            if (seg != 1) {  // RMSD set: 2 R^T (R u - v)
+                double rr[3];
+                for (int a = 0; a < 3; ++a) rr[a] = R[a * 3] * u[0] + R[a * 3 + 1] * u[1] + R[a * 3 + 2] * u[2] - v[a];
+                for (int b = 0; b < 3; ++b) g[b] += 2.0 * (R[b] * rr[0] + R[3 + b] * rr[1] + R[6 + b] * rr[2]);
+            }
+            if (seg != 2) {  // fit set: 2 G^T v - (2 / n_A) R^T sum_B r
+                const double *G = o + 3;
+                for (int b = 0; b < 3; ++b)
+                    g[b] += 2.0 * (G[b] * v[0] + G[3 + b] * v[1] + G[6 + b] * v[2]) - o[12 + b];
+            }
+            for (int b = 0; b < 3; ++b) g[b] *= sc;
+        }
+        Gr[k] = (float)g[0];
+        Gr[stride + k] = (float)g[1];
+        Gr[2 * stride + k] = (float)g[2];
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Batched RMSD to frame 0 (the RMSD half of an uneven batch; RmsdPlan with
 // reference = frame0 and align = true, rmsd/plan.rs:336-354, per trajectory).
@@ -473,6 +652,7 @@ struct wmd_rmsd {
     RefMoments rm{};
     DevBuf d_ref, d_atoms, d_partials, d_out32, d_out64, d_err;
     double *scratch = nullptr;  // this device's address of the fixed scratch array
+    DevBuf d_bwd_partials, d_bwd_coef;  // split-fit adjoint: moments with second moments, per-frame terms
     // host-chunk pipeline: two staging / device buffers, events guarding reuse
     HostBuf p_stage[2];
     DevBuf p_aos[2], p_soa[2];
@@ -575,15 +755,20 @@ wmd_status wmd_rmsd_create(wmd_ctx *ctx, const uint32_t *fit_sel, size_t n_fit,
     *out = nullptr;
     WMD_REQUIRE(n_fit == 0 || fit_sel, WMD_ERR_INVALID, "null fit selection");
     WMD_REQUIRE(ref_xyzw || n_ref_atoms == 0, WMD_ERR_INVALID, "null reference");
-    const bool same = rmsd_sel == nullptr;
+    bool same = rmsd_sel == nullptr;
     if (same) n_rmsd = n_fit;
     WMD_REQUIRE(same || n_rmsd == 0 || rmsd_sel, WMD_ERR_INVALID, "null rmsd selection");
-    if (!align) WMD_REQUIRE(same, WMD_ERR_UNSUPPORTED, "align=false needs a single selection");
     for (size_t i = 0; i < n_fit; ++i)
         WMD_REQUIRE(fit_sel[i] < n_ref_atoms, WMD_ERR_MISMATCH, "selection index out of range of the reference");
     if (!same)
         for (size_t i = 0; i < n_rmsd; ++i)
             WMD_REQUIRE(rmsd_sel[i] < n_ref_atoms, WMD_ERR_MISMATCH, "selection index out of range of the reference");
+    if (!align && !same) {
+        // unfitted: the residual over the RMSD set alone, the fit set unused (rms.rs:664-666)
+        fit_sel = rmsd_sel;
+        n_fit = n_rmsd;
+        same = true;
+    }
     bind(ctx);
     auto *h = new wmd_rmsd();
     try {
@@ -761,8 +946,6 @@ wmd_status wmd_rmsd_backward_device(wmd_rmsd *h, const float *d_store, size_t st
     WMD_API_BEGIN
     WMD_REQUIRE(h != nullptr, WMD_ERR_INVALID, "null handle");
     if (n_frames == 0) return WMD_OK;
-    WMD_REQUIRE(h->mode != MODE_SPLIT, WMD_ERR_UNSUPPORTED,
-                "the RMSD backward needs rmsd_sel == fit_sel (a split fit needs the adjoint through the rotation)");
     WMD_REQUIRE(stride == h->stride, WMD_ERR_INVALID,
                 "store stride must equal the plan's stride (" + std::to_string(h->stride) + ")");
     WMD_REQUIRE(d_store && d_rmsd64 && d_gout && d_grad, WMD_ERR_INVALID, "null device buffer");
@@ -777,6 +960,30 @@ wmd_status wmd_rmsd_backward_device(wmd_rmsd *h, const float *d_store, size_t st
     const long long total = (long long)n_frames * (long long)stride;
     const int g = (int)std::min<long long>(65536, (total + kBlock - 1) / kBlock);
     ProfScope prof(h->ctx, st, "rmsd_backward");
+    if (h->mode == MODE_SPLIT) {
+        // moments again, with the RMSD set's second moments, then the
+        // per-frame adjoint terms, then the per-slot gradient
+        int n_tiles = 1, gpt = 1;
+        rmsd_tiling(h, n_frames, &n_tiles, &gpt);
+        constexpr int NV = 2 * kMom + 6;
+        double *partials = (double *)h->d_bwd_partials.reserve((size_t)n_frames * n_tiles * NV * 8);
+        double *coef = (double *)h->d_bwd_coef.reserve((size_t)n_frames * 16 * 8);
+        const long long waves = (long long)n_frames * n_tiles;
+        const long long blocks = (waves + (kBlock / 64) - 1) / (kBlock / 64);
+        WMD_REQUIRE(blocks < (1ll << 31), WMD_ERR_UNSUPPORTED, "too many frames for one launch");
+        hipLaunchKernelGGL((rmsd_moments_kernel<MODE_SPLIT, true>), dim3((unsigned)blocks), dim3(kBlock), 0, st,
+                           d_store, 3 * (long long)stride, (int)stride, (int)n_frames, h->d_ref.as<float>(),
+                           (int)(h->stride / 4), h->seg1, h->seg2, n_tiles, gpt, partials);
+        WMD_HIP_CHECK(hipGetLastError());
+        hipLaunchKernelGGL(rmsd_split_adjoint_kernel, dim3((unsigned)((n_frames + 63) / 64)), dim3(64), 0, st,
+                           (const double *)partials, (int)n_frames, n_tiles, h->rm, d_rot, d_rmsd64, d_gout, coef);
+        WMD_HIP_CHECK(hipGetLastError());
+        hipLaunchKernelGGL(rmsd_split_backward_kernel, dim3(g), dim3(kBlock), 0, st, d_store, (int)stride,
+                           (int)n_frames, h->d_ref.as<float>(), h->d_atoms.as<unsigned>(), h->seg1, h->seg2,
+                           cy[0], cy[1], cy[2], d_rot, (const double *)coef, d_grad);
+        WMD_HIP_CHECK(hipGetLastError());
+        return WMD_OK;
+    }
     if (h->mode == MODE_SAME)
         hipLaunchKernelGGL(rmsd_backward_kernel<true>, dim3(g), dim3(kBlock), 0, st, d_store, (int)stride,
                            (int)n_frames, h->d_ref.as<float>(), h->d_atoms.as<unsigned>(), cy[0], cy[1], cy[2],

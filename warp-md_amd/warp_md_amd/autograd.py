@@ -225,7 +225,8 @@ def _torch_function():
                 ctypes.c_void_p(ctx.cx.data_ptr()), ctypes.c_void_p(gout.data_ptr()),
                 ctypes.c_void_p(gs.data_ptr()), _stream(gs.device)))
             gx = torch.zeros(ctx.shape, dtype=torch.float32, device=gs.device)
-            gx[:, plan.atom_idx, :] = gs[:, :, plan.slot_idx].permute(0, 2, 1)
+            # an atom in both sets, or repeated in one, owns several slots: add them
+            gx.index_add_(1, plan.atom_idx, gs[:, :, plan.slot_idx].permute(0, 2, 1))
             return gx.to(ctx.x_dtype), None
 
     return _SoftRdf, _Rmsd
@@ -249,9 +250,13 @@ _RmsdFn = _Lazy(1)
 class FittedRmsd:
     """Per-frame RMSD to a fixed reference after the optimal superposition
     (align=True) or without it (align=False), over all N atoms; differentiable
-    in the positions.  The reference is a constant."""
+    in the positions.  The reference is a constant.
 
-    def __init__(self, reference, align: bool = True, device="auto"):
+    ``fit_selection`` (atom indices) fits on that subset and takes the RMSD
+    over all atoms (the composition of rms.rs:643-676); its backward runs the
+    adjoint through the rotation."""
+
+    def __init__(self, reference, align: bool = True, device="auto", fit_selection=None):
         ref = np.asarray(reference.detach().cpu().numpy() if hasattr(reference, "detach")
                          else reference, np.float32)
         if ref.ndim != 2 or ref.shape[1] != 3:
@@ -263,9 +268,17 @@ class FittedRmsd:
         xyzw = np.zeros((self.n, 4), np.float32)
         xyzw[:, :3] = ref
         h = ctypes.c_void_p()
-        _native.check(L.wmd_rmsd_create(self._ctx.handle, _native.ptr(sel), self.n, None, 0,
-                                        _native.ptr(xyzw), self.n, int(bool(align)),
-                                        ctypes.byref(h)))
+        if fit_selection is None:
+            _native.check(L.wmd_rmsd_create(self._ctx.handle, _native.ptr(sel), self.n, None, 0,
+                                            _native.ptr(xyzw), self.n, int(bool(align)),
+                                            ctypes.byref(h)))
+        else:
+            fit = np.ascontiguousarray(np.asarray(fit_selection, np.int64).astype(np.uint32))
+            if fit.size and int(fit.max()) >= self.n:
+                raise RuntimeError("invalid selection: atom index out of range")
+            _native.check(L.wmd_rmsd_create(self._ctx.handle, _native.ptr(fit), fit.size,
+                                            _native.ptr(sel), self.n, _native.ptr(xyzw), self.n,
+                                            int(bool(align)), ctypes.byref(h)))
         self._h = h
         n_store, stride = ctypes.c_size_t(), ctypes.c_size_t()
         _native.check(L.wmd_rmsd_store_layout(h, None, 0, ctypes.byref(n_store), ctypes.byref(stride)))

@@ -220,8 +220,10 @@ class RmsdPlan:
 
     ``fit_selection`` (an addition of this engine) fits on one selection and
     takes the RMSD over ``selection`` -- the composition of kabsch_rotation
-    and the residual of rms.rs:643-676.  ``None`` keeps RmsdPlan's behaviour
-    (fit and RMSD over the same atoms).
+    and the residual of rms.rs:643-676; with ``align=False`` that composition
+    takes the unfitted residual over ``selection`` and leaves the fit set
+    unused (rms.rs:664-666).  ``None`` keeps RmsdPlan's behaviour (fit and
+    RMSD over the same atoms).
     """
 
     def __init__(self, selection, reference: str = "topology", align: bool = True,
@@ -237,8 +239,6 @@ class RmsdPlan:
         dev = Device.from_spec(device)
         if traj.n_atoms() != system.n_atoms():
             raise RuntimeError("mismatch: trajectory atom count does not match system")
-        if self.fit_selection is not None and not self.align:
-            raise RuntimeError("unsupported: fit_selection requires align=True")
         ctx = dev.context()
         n_atoms = system.n_atoms()
         fit = self.fit_selection if self.fit_selection is not None else self.selection
@@ -263,7 +263,8 @@ class RmsdPlan:
                                         ptr(ref4), n_atoms, int(self.align), ctypes.byref(h)))
         else:
             check(lib().wmd_rmsd_create(ctx.handle, ptr(fit), fit.size, ptr(self.selection),
-                                        self.selection.size, ptr(ref4), n_atoms, 1, ctypes.byref(h)))
+                                        self.selection.size, ptr(ref4), n_atoms, int(self.align),
+                                        ctypes.byref(h)))
         try:
             out = np.zeros(tr.n_frames(), np.float32)
             err = np.zeros(tr.n_frames(), np.float64)
