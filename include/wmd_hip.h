@@ -112,7 +112,9 @@ wmd_status wmd_ctx_profile_read(wmd_ctx *ctx, const char *name, double *total_ms
  * sel_a / sel_b: atom indices (duplicates allowed, they weight pairs like the
  * reference's loops).  pbc: WMD_PBC_*.  Pairs with the same atom index are
  * skipped (rdf.rs:917-919).  The IO selection is the order-preserving union of
- * sel_a then sel_b (rdf.rs:498-526). */
+ * sel_a then sel_b (rdf.rs:498-526).  bins <= 4194304: a histogram larger
+ * than the pair kernels' LDS is counted one slice of bins per pass (each
+ * pass enumerates the pairs again). */
 wmd_status wmd_rdf_create(wmd_ctx *ctx, const uint32_t *sel_a, size_t n_a, const uint32_t *sel_b,
                           size_t n_b, size_t bins, float r_max, int pbc, uint32_t flags,
                           wmd_rdf **out);
@@ -422,7 +424,9 @@ void wmd_softrdf_destroy(wmd_softrdf *h);
 
 typedef struct wmd_batch wmd_batch;
 
-/* n_atoms, n_frames: [n_traj].  bins <= 1024, r_max > 0, pbc WMD_PBC_*. */
+/* n_atoms, n_frames: [n_traj].  bins <= 4194304 (histograms beyond the pair
+ * kernel's per-wave LDS copies are counted one slice of bins per pass),
+ * r_max > 0, pbc WMD_PBC_*. */
 wmd_status wmd_batch_create(wmd_ctx *ctx, size_t n_traj, const uint32_t *n_atoms,
                             const uint32_t *n_frames, size_t bins, float r_max, int pbc,
                             uint32_t flags, wmd_batch **out);

@@ -236,3 +236,26 @@ def test_full_uneven_batch_matches_single_plans():
         worst = max(worst, d)
     print(f"256 trajectories: counts identical, RMSD max diff {worst:.2e} A", flush=True)
     assert worst <= 1e-5
+
+
+@pytest.mark.parametrize("graph", [True, False])
+def test_batch_histogram_larger_than_lds(graph):
+    """5,000 bins: the per-wave LDS histograms cannot hold them, so the pair
+    launch runs one pass per slice of bins; counts bit-exact per trajectory
+    (cells and brute force)."""
+    sizes = [300, 1500, 8000]
+    coords, boxes = _case(sizes, n_frames=2, seed=9)
+    bins = 5000
+    b = UnevenBatch(sizes, [2] * len(sizes), bins=bins, r_max=RMAX, pbc="orthorhombic", rmsd=False,
+                    graph=graph, device="cuda:0")
+    b.set_boxes(boxes)
+    store = b.store_from_numpy(coords)
+    print(f"batch bins={bins} graph={graph} plan {list(b.plan()[0])}", flush=True)
+    b.run(store)
+    b.run(store)  # a replay adds the same counts again
+    counts = b.counts()
+    for t, c in enumerate(coords):
+        allat = np.arange(c.shape[1])
+        ref = oracle.rdf_counts(c, allat, allat, bins, RMAX, pbc="orthorhombic", boxes=boxes[t],
+                                method="omp" if c.shape[1] > 3000 else "brute")
+        assert np.array_equal(counts[t], 2 * ref), f"trajectory {t}"

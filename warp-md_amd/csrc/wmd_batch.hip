@@ -187,9 +187,10 @@ struct BatchPairArgs {
     const unsigned *start;
     const unsigned *frame_flag;
     const FrameBox *boxes;
-    unsigned long long *counts;  // [n_traj][bins]
+    unsigned long long *counts;  // [n_traj][row_bins] (+ the pass's first bin)
     unsigned long long *queue;   // [kSlices] grab counters, zero at launch
     long long slice_lo[kSlices + 1];
+    int row_bins;                // bins of a trajectory's histogram row
     HistParams p;
 };
 
@@ -204,14 +205,16 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Adds a wave's LDS histogram to out[bins] and clears it.
+// Adds a wave's LDS histogram to out[bins - skip] (local bins below skip are
+// dropped) and clears it.
 __device__ __forceinline__ void wave_hist_flush(unsigned long long *hist,
-                                                unsigned long long *out, int bins, int lane) {
+                                                unsigned long long *out, int bins, int skip,
+                                                int lane) {
     wave_lds_sync();
     for (int i = lane; i < bins; i += 64) {
         const unsigned long long v = hist[i];
         if (v) {
-            atomicAdd(out + i, v);
+            if (i >= skip) atomicAdd(out + (i - skip), v);
             hist[i] = 0ull;
         }
     }
@@ -286,7 +289,7 @@ __global__ __launch_bounds__(kBatchBlock) void batch_pair_kernel(BatchPairArgs a
         const long long u1 = min(u0 + (long long)kGrab, hi);
         for (long long u = u0; u < u1; ++u) {
             if (u >= cur_end) {
-                if (cur >= 0) wave_hist_flush(L.hist, a.counts + (long long)cur * bins, bins, lane);
+                if (cur >= 0) wave_hist_flush(L.hist, a.counts + (long long)cur * a.row_bins, bins, p.skip, lane);
                 cur = __builtin_amdgcn_readfirstlane(
                     find_last_le(a.n_traj, u, [&](int i) { return a.traj[i].unit0; }));
                 T = uload(a.traj, cur);
@@ -308,7 +311,7 @@ __global__ __launch_bounds__(kBatchBlock) void batch_pair_kernel(BatchPairArgs a
             brute_unit(G, f, h, cons);
         }
     }
-    if (cur >= 0) wave_hist_flush(L.hist, a.counts + (long long)cur * bins, bins, lane);
+    if (cur >= 0) wave_hist_flush(L.hist, a.counts + (long long)cur * a.row_bins, bins, p.skip, lane);
 }
 
 // ---- autotuner cache ---------------------------------------------------------
@@ -351,6 +354,10 @@ struct wmd_batch {
     // device
     DevBuf d_counts, d_thr, d_tab2, d_traj, d_rtraj, d_boxes, d_flag, d_count, d_start, d_cell_of,
         d_slot, d_image, d_sorted, d_queue, d_partials;
+    // bins beyond the per-wave LDS histograms: slice_bins per pass (0 = one
+    // pass) and per slice [slice_bins + 2] local thresholds
+    int slice_bins = 0;
+    DevBuf d_sthr;
     HostBuf h_boxes;
     // graph
     hipStream_t cap = nullptr;
@@ -576,8 +583,9 @@ void enqueue_pairs(wmd_batch *b, hipStream_t st, const float *d_store) {
     p.tab_sh = b->tables.tab_sh;
     p.tab_base = b->tables.tab_base;
     p.counts = a.counts;
-    const size_t lds = batch_lds_bytes(b->bins, p.tab_n);
-    auto launch = [&](auto kern) {
+    p.count_evals = 1;
+    a.row_bins = b->bins;
+    auto launch = [&](auto kern, size_t lds) {
         int occ = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, kBatchBlock, lds) != hipSuccess || occ < 1)
             occ = 1;
@@ -585,8 +593,34 @@ void enqueue_pairs(wmd_batch *b, hipStream_t st, const float *d_store) {
         hipLaunchKernelGGL(kern, dim3(grid), dim3(kBatchBlock), lds, st, a);
         WMD_HIP_CHECK(hipGetLastError());
     };
-    if (b->table) launch(batch_pair_kernel<PBC, true>);
-    else launch(batch_pair_kernel<PBC, false>);
+    if (b->slice_bins == 0) {
+        const size_t lds = batch_lds_bytes(b->bins, p.tab_n);
+        if (b->table) launch(batch_pair_kernel<PBC, true>, lds);
+        else launch(batch_pair_kernel<PBC, false>, lds);
+        return;
+    }
+    // bins beyond the per-wave LDS histograms: one pass (a full enumeration)
+    // per slice of bins, with a local bin for the pairs below the slice
+    const int S = b->slice_bins;
+    const int n_pass = (b->bins + S - 1) / S;
+    for (int k = 0; k < n_pass; ++k) {
+        const int b0 = k * S, nb = std::min(S, b->bins - b0);
+        if (k > 0) {
+            hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(kBlock), 0, st, b->d_queue.as<unsigned long long>(),
+                               (long long)kSlices);
+            WMD_HIP_CHECK(hipGetLastError());
+        }
+        p.bins = nb + 1;
+        p.cut = b->tables.thr[b0 + nb];
+        p.thr = b->d_sthr.as<float>() + (size_t)k * (S + 2);
+        p.tab2 = nullptr;
+        p.tab_n = 0;
+        p.shift = b0 - 1;
+        p.skip = 1;
+        a.counts = b->d_counts.as<unsigned long long>() + b0;
+        p.counts = a.counts;
+        launch(batch_pair_kernel<PBC, false>, batch_lds_bytes(nb + 1, 0));
+    }
 }
 
 // The whole step on `st`: clears, cell build, pair kernel, RMSD.
@@ -671,7 +705,7 @@ wmd_batch *create_batch(wmd_ctx *ctx, size_t n_traj, const uint32_t *n_atoms, co
     WMD_REQUIRE(n_traj < (1u << 20), WMD_ERR_UNSUPPORTED, "too many trajectories");
     WMD_REQUIRE(pbc >= WMD_PBC_NONE && pbc <= WMD_PBC_TRICLINIC, WMD_ERR_PARSE,
                 "pbc must be none, orthorhombic or triclinic");
-    WMD_REQUIRE(bins <= 1024, WMD_ERR_UNSUPPORTED, "a batch supports at most 1024 bins");
+    WMD_REQUIRE(bins <= (size_t(1) << 22), WMD_ERR_UNSUPPORTED, "a batch supports at most 4194304 bins");
     WMD_REQUIRE(std::isfinite(r_max) && r_max > 0.0f, WMD_ERR_PARSE, "r_max must be finite and positive");
     WMD_REQUIRE((flags & ~(uint32_t)(WMD_BATCH_RDF | WMD_BATCH_RMSD | WMD_BATCH_NO_GRAPH)) == 0,
                 WMD_ERR_INVALID, "unknown batch flags");
@@ -719,8 +753,26 @@ wmd_batch *create_batch(wmd_ctx *ctx, size_t n_traj, const uint32_t *n_atoms, co
             }
         }
         WMD_HIP_CHECK(hipStreamSynchronize(st));
-        WMD_REQUIRE(batch_lds_bytes((int)bins, b->table ? (int)b->tables.tab2.size() : 0) <= ctx->lds_per_block,
-                    WMD_ERR_UNSUPPORTED, "too many bins for the batch kernel's LDS");
+        if (bins > 0 &&
+            batch_lds_bytes((int)bins, b->table ? (int)b->tables.tab2.size() : 0) > ctx->lds_per_block) {
+            // slices of bins, one pass each: the largest multiple of 64 whose
+            // per-wave histograms (plus a bin below the slice) fit LDS
+            int S = (int)std::min<size_t>(bins, 1 << 16) / 64 * 64;
+            while (S > 64 && batch_lds_bytes(S + 1, 0) > ctx->lds_per_block) S -= 64;
+            WMD_REQUIRE(S >= 64 && batch_lds_bytes(S + 1, 0) <= ctx->lds_per_block, WMD_ERR_UNSUPPORTED,
+                        "too many bins for the batch kernel's LDS");
+            const int n_pass = ((int)bins + S - 1) / S;
+            const size_t per = (size_t)S + 2;
+            std::vector<float> sthr(per * n_pass, 0.0f);
+            for (int k = 0; k < n_pass; ++k) {
+                const int b0 = k * S, nb = std::min(S, (int)bins - b0);
+                for (int j = 0; j <= nb; ++j) sthr[k * per + j + 1] = b->tables.thr[b0 + j];
+            }
+            b->d_sthr.reserve(sthr.size() * 4);
+            WMD_HIP_CHECK(hipMemcpy(b->d_sthr.ptr, sthr.data(), sthr.size() * 4, hipMemcpyHostToDevice));
+            b->slice_bins = S;
+            b->table = false;
+        }
     } catch (...) {
         delete b;
         throw;

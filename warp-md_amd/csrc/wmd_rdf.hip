@@ -917,6 +917,11 @@ struct wmd_rdf {
     double r_max64 = 0.0, scale64 = 1.0;
     std::vector<double> thr64;
     DevBuf d_thr64;
+    // bin slices of a histogram larger than the pair kernels' LDS: slice_bins
+    // global bins per pass; per slice [slice_bins + 2] local thresholds (and
+    // edge bounds): a bin below the slice, then the slice's own
+    int slice_bins = 0;
+    DevBuf d_sthr, d_selo, d_sehi;
 };
 
 namespace {
@@ -962,6 +967,64 @@ HistParams hist_params(wmd_rdf *h, bool table) {
 
 size_t lds_hist_bytes(const HistParams &p, bool edge) {
     return lds_hist_size(p.bins, p.hist_copies, edge, p.tab_n);
+}
+
+// The histogram passes of one pair enumeration.  When the kernel's LDS
+// (lds_of(local bins, with table)) holds every bin: one pass with p.  Else
+// the bins are counted slice by slice, each pass a full enumeration whose
+// histogram is one local bin for the pairs below the slice (dropped) plus
+// the slice's bins; pairs above it fail the pass's cut.  *table is cleared
+// for sliced passes (the bin estimate from sqrt(s) needs no table).
+template <class LdsOf>
+std::vector<HistParams> hist_passes(wmd_rdf *h, const HistParams &p, bool edge, bool *table,
+                                    LdsOf lds_of, hipStream_t st) {
+    const size_t limit = h->ctx->lds_per_block;
+    if (lds_of(h->bins, *table) <= limit) return {p};
+    int S = std::min(h->bins, 1 << 16) / 64 * 64;
+    while (S > 64 && lds_of(S + 1, false) > limit) S -= 64;
+    WMD_REQUIRE(S >= 64 && lds_of(S + 1, false) <= limit, WMD_ERR_UNSUPPORTED, "too many bins for LDS");
+    const int n_slices = (h->bins + S - 1) / S;
+    if (h->slice_bins != S) {
+        const size_t per = (size_t)S + 2;
+        std::vector<float> thr(per * n_slices, 0.0f), elo(per * n_slices, 0.0f), ehi(per * n_slices, 0.0f);
+        for (int k = 0; k < n_slices; ++k) {
+            const int b0 = k * S, nb = std::min(S, h->bins - b0);
+            for (int j = 0; j <= nb; ++j) {
+                thr[k * per + j + 1] = h->thr[b0 + j];
+                elo[k * per + j + 1] = h->elo[b0 + j];
+                ehi[k * per + j + 1] = h->ehi[b0 + j];
+            }
+        }
+        WMD_HIP_CHECK(hipStreamSynchronize(st));  // growing frees buffers queued work may use
+        h->d_sthr.reserve(thr.size() * 4);
+        h->d_selo.reserve(elo.size() * 4);
+        h->d_sehi.reserve(ehi.size() * 4);
+        WMD_HIP_CHECK(hipMemcpyAsync(h->d_sthr.ptr, thr.data(), thr.size() * 4, hipMemcpyHostToDevice, st));
+        WMD_HIP_CHECK(hipMemcpyAsync(h->d_selo.ptr, elo.data(), elo.size() * 4, hipMemcpyHostToDevice, st));
+        WMD_HIP_CHECK(hipMemcpyAsync(h->d_sehi.ptr, ehi.data(), ehi.size() * 4, hipMemcpyHostToDevice, st));
+        WMD_HIP_CHECK(hipStreamSynchronize(st));  // the host vectors die at scope exit
+        h->slice_bins = S;
+    }
+    std::vector<HistParams> out;
+    for (int k = 0; k < n_slices; ++k) {
+        const int b0 = k * S, nb = std::min(S, h->bins - b0);
+        HistParams q = p;
+        q.bins = nb + 1;
+        q.cut = h->thr[b0 + nb];
+        q.thr = h->d_sthr.as<float>() + (size_t)k * (S + 2);
+        q.elo = h->d_selo.as<float>() + (size_t)k * (S + 2);
+        q.ehi = h->d_sehi.as<float>() + (size_t)k * (S + 2);
+        q.tab = nullptr;
+        q.tab2 = nullptr;
+        q.tab_n = 0;
+        q.shift = b0 - 1;
+        q.skip = 1;
+        q.counts = p.counts + b0;
+        q.count_evals = k == 0;
+        out.push_back(q);
+    }
+    *table = false;
+    return out;
 }
 
 // Dispatch helpers over the template flags.
@@ -1125,8 +1188,13 @@ void accumulate_device(wmd_rdf *h, const float *d_soa, size_t stride, size_t n_f
         const int n_tp = upload_tile_pairs(h, n, st);
         const int2 *d_tp = (const int2 *)h->d_tiles.ptr;
         long long fs = 3 * (long long)stride;
-        brute_dispatch(!fast_wrap, h->weighted, edge, table, h->pbc, h, st, d_soa, fs, (int)stride,
-                       n, (int)n_frames, 0LL, d_boxes, (const int2 *)d_tp, n_tp, p);
+        bool tb = table;
+        const auto passes = hist_passes(h, p, edge, &tb, [&](int bins, bool t) {
+            return lds_hist_size(bins, p.hist_copies, edge, t ? p.tab_n : 0) + (size_t)kTile * 16;
+        }, st);
+        for (const HistParams &q : passes)
+            brute_dispatch(!fast_wrap, h->weighted, edge, tb, h->pbc, h, st, d_soa, fs, (int)stride,
+                           n, (int)n_frames, 0LL, d_boxes, (const int2 *)d_tp, n_tp, q);
         h->last_path = 1;
         h->last_nc[0] = h->last_nc[1] = h->last_nc[2] = 0;
         return;
@@ -1142,14 +1210,18 @@ void accumulate_device(wmd_rdf *h, const float *d_soa, size_t stride, size_t n_f
     h->cells.reserve(n, ncell, batch, false);
     const unsigned *d_w = h->weighted ? h->d_wts.as<unsigned>() : nullptr;
     const bool lt = h->pbc == WMD_PBC_TRICLINIC && B.lower_tri;
-    const bool cell_table = table && (int)h->tab.size() <= kMaxTab2;
+    bool cell_table = table && (int)h->tab.size() <= kMaxTab2;
+    const auto passes = hist_passes(h, p, edge, &cell_table, [&](int bins, bool t) {
+        return cell_lds_bytes(bins, edge, t ? p.tab_n : 0);
+    }, st);
     for (size_t f0 = 0; f0 < n_frames; f0 += batch) {
         const int nf = (int)std::min(batch, n_frames - f0);
         const FrameBox *bx = d_boxes + f0;
         build_cells(h->ctx, st, h->pbc, d_soa, (int)stride, n, nf, (long long)f0, bx, nc, d_w, h->cells, false);
-        cells_dispatch(h->pbc, lt, h->weighted, edge, cell_table, h, st, h->cells.sorted.as<float4>(),
-                       (const unsigned *)h->cells.start.ptr, (const unsigned *)h->cells.flag.ptr, n, nf,
-                       bx, (const int *)nc, k_use, shift_ok, cull2, p);
+        for (const HistParams &q : passes)
+            cells_dispatch(h->pbc, lt, h->weighted, edge, cell_table, h, st, h->cells.sorted.as<float4>(),
+                           (const unsigned *)h->cells.start.ptr, (const unsigned *)h->cells.flag.ptr, n, nf,
+                           bx, (const int *)nc, k_use, shift_ok, cull2, q);
     }
     h->last_path = 2;
     h->last_nc[0] = nc[0]; h->last_nc[1] = nc[1]; h->last_nc[2] = nc[2];
@@ -1258,7 +1330,7 @@ wmd_status wmd_rdf_create(wmd_ctx *ctx, const uint32_t *sel_a, size_t n_a, const
     WMD_REQUIRE((n_a == 0 || sel_a) && (n_b == 0 || sel_b), WMD_ERR_INVALID, "null selection");
     WMD_REQUIRE(pbc >= WMD_PBC_NONE && pbc <= WMD_PBC_TRICLINIC, WMD_ERR_PARSE,
                 "pbc must be none, orthorhombic or triclinic");
-    WMD_REQUIRE(bins <= 12288, WMD_ERR_UNSUPPORTED, "at most 12288 bins are supported");
+    WMD_REQUIRE(bins <= (size_t(1) << 22), WMD_ERR_UNSUPPORTED, "at most 4194304 bins are supported");
     WMD_REQUIRE(std::isfinite(r_max) && r_max > 0.0f, WMD_ERR_PARSE, "r_max must be finite and positive");
     bind(ctx);
     auto *h = new wmd_rdf();

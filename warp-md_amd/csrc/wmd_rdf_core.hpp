@@ -41,8 +41,14 @@ struct HistParams {
     const unsigned short *tab;  // [tab_n] s-bits bin table (TABLE variants)
     const uint2 *tab2;          // [tab_n] per bucket (split threshold bits, 8 x bin below it)
     int tab_n, tab_sh, tab_base;
-    unsigned long long *counts;  // [bins]
-    unsigned long long *edge;    // [1]
+    unsigned long long *counts;  // [bins - skip]: counts[b - skip] gets local bin b
+    unsigned long long *edge;    // [2]: edge pairs, pair evaluations
+    // Bin slices (a histogram larger than LDS, counted one slice of bins per
+    // pass): local bin b is global bin b + shift of the plain tables, local
+    // bins below `skip` collect the pairs under the slice and are dropped.
+    // shift = skip = 0 for an unsliced histogram.
+    int shift = 0, skip = 0;
+    int count_evals = 1;  // EDGE variants: add the pair evaluations (first pass only)
 };
 
 // LDS layout shared by the pair kernels:
@@ -102,7 +108,7 @@ __device__ void lds_hist_flush(const LdsHist &L, const HistParams &p, unsigned &
             sum += L.hist[c * p.bins + i];
             L.hist[c * p.bins + i] = 0u;
         }
-        if (sum) atomicAdd(&p.counts[i], sum);
+        if (sum && i >= p.skip) atomicAdd(&p.counts[i - p.skip], sum);
     }
     if constexpr (EDGE) {
         if (edge_local) atomicAdd(p.edge, (unsigned long long)edge_local);
@@ -122,8 +128,9 @@ __device__ __forceinline__ int exact_bin(float s, const LdsHist &L, const HistPa
         return b;
     } else {
         float r_est = __builtin_amdgcn_sqrtf(s);
-        int b = (int)(r_est * p.inv_w);
+        int b = (int)(r_est * p.inv_w) - p.shift;
         b = b < p.bins - 1 ? b : p.bins - 1;
+        b = b > 0 ? b : 0;
         float lo = L.thr[b];
         float hi = L.thr[b + 1];
         b += (int)(s >= hi) - (int)(s < lo);
@@ -140,7 +147,7 @@ __device__ __forceinline__ void count_pair(float s, unsigned w, const HistParams
         atomicAdd(&hist[b], w);
         if constexpr (EDGE) {
             bool near = (s >= L.elo[b] && s < L.ehi[b]) || (s >= L.elo[b + 1] && s < L.ehi[b + 1]);
-            edge_local += near ? w : 0u;
+            edge_local += (near && b >= p.skip) ? w : 0u;
         }
     }
 }
@@ -689,8 +696,9 @@ __device__ __forceinline__ unsigned cell_bin8(float s, const CellLds &L, const H
         return e.y + (s >= __uint_as_float(e.x) ? 8u : 0u);
     } else {
         float r_est = __builtin_amdgcn_sqrtf(s);
-        int b = (int)(r_est * p.inv_w);
+        int b = (int)(r_est * p.inv_w) - p.shift;
         b = b < p.bins - 1 ? b : p.bins - 1;
+        b = b > 0 ? b : 0;
         b += (int)(s >= L.thr[b + 1]) - (int)(s < L.thr[b]);
         return (unsigned)b * 8u;
     }
@@ -746,7 +754,7 @@ __device__ __forceinline__ void cell_count(float s, unsigned mi, unsigned mAj, u
             const unsigned bin = off >> 3;
             bool near = (s >= L.elo[bin] && s < L.ehi[bin]) ||
                         (s >= L.elo[bin + 1] && s < L.ehi[bin + 1]);
-            edge_local += near ? w : 0u;
+            edge_local += (near && bin >= (unsigned)p.skip) ? w : 0u;
         }
     }
 }
@@ -1170,7 +1178,7 @@ __global__ __launch_bounds__(kCellBlock) void rdf_cell_kernel(CellGrid G, HistPa
     __syncthreads();
     for (int i = threadIdx.x; i < p.bins; i += kCellBlock) {
         const unsigned long long v = L.hist[i];
-        if (v) atomicAdd(&p.counts[i], v);
+        if (v && i >= p.skip) atomicAdd(&p.counts[i - p.skip], v);
     }
     if constexpr (EDGE) {
         unsigned long long e = cons.edge_local, v = cons.evals_local;
@@ -1179,7 +1187,7 @@ __global__ __launch_bounds__(kCellBlock) void rdf_cell_kernel(CellGrid G, HistPa
             v += __shfl_xor(v, off, 64);
         }
         if (lane == 0 && e) atomicAdd(p.edge, e);
-        if (lane == 0 && v) atomicAdd(p.edge + 1, v);
+        if (lane == 0 && v && p.count_evals) atomicAdd(p.edge + 1, v);
     }
 }
 
