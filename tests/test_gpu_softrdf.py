@@ -90,9 +90,13 @@ def test_loss_gradient_matches_fp64_finite_differences(small):
         assert abs(grad[f, a, d] - fd) <= 1e-4 * scale, (f, a, d, grad[f, a, d], fd)
 
 
-@pytest.mark.parametrize("n,tilt", [(6000, None), (12000, None), (12000, 10.0)])
-def test_cell_path_matches_brute(n, tilt):
-    L = (n / 0.08) ** (1.0 / 3.0)
+@pytest.mark.parametrize("n,tilt,k,rho", [(6000, None, 0, 0.08), (12000, None, 0, 0.08),
+                                          (12000, 10.0, 0, 0.08), (12000, None, 1, 0.08),
+                                          (12000, None, 1, 0.2)])
+def test_cell_path_matches_brute(n, tilt, k, rho):
+    """Cell enumeration against brute force; k = 1 gives cells of ~140
+    (rho 0.08) and ~350 (rho 0.2) atoms: home atoms beyond the LDS slots."""
+    L = (n / rho) ** (1.0 / 3.0)
     x = _beads(n, L, 2, seed=n)
     if tilt:
         M = np.array([[L, 0, 0], [L * math.tan(math.radians(tilt)), L, 0], [0, 0, L]], np.float32)
@@ -101,7 +105,7 @@ def test_cell_path_matches_brute(n, tilt):
     else:
         box, pbc = np.array([L, L, L], np.float32), "orthorhombic"
     xt = torch.tensor(x, device="cuda")
-    cell = _plan(bins=BINS, r_max=RMAX, sigma=SIG, pbc=pbc)
+    cell = _plan(bins=BINS, r_max=RMAX, sigma=SIG, pbc=pbc, cell_k=k)
     brute = _plan(bins=BINS, r_max=RMAX, sigma=SIG, pbc=pbc, cell_k=-1)
     cc = cell.counts(xt, box)
     assert cell.last_path[0] == 2

@@ -85,8 +85,6 @@ namespace {
 constexpr int kSoftNodesPerBin = 4;   // h = w / 4
 constexpr int kSoftPadL = 1;          // node padding below node 0
 constexpr int kSoftPadR = 3;          // and above the last node
-constexpr int kSoftMaxHome = 64;      // home atoms with an LDS gradient slot per wave
-constexpr int kSoftJCopies = 1;       // copies of the j gradient sums (power of two)
 
 struct SoftParams {
     int n_nodes;        // nodes 0 .. n_nodes - 1 carry weight
@@ -122,11 +120,14 @@ __device__ __forceinline__ void soft_deposit(float s, const SoftParams &sp, doub
 }
 
 // One unordered pair's dL/dr divided by r, from the padded (A, S) table.
+// Pairs within reach have u <= n_nodes, so t = u - rint(u) needs no clamp;
+// the index is clamped for the lanes a caller masks out.
 __device__ __forceinline__ float soft_force_over_r(float s, const SoftParams &sp, const float2 *tab) {
     const float rinv = __builtin_amdgcn_rsqf(s);
     const float u = s * rinv * sp.inv_h;
-    float t;
-    const int n0 = bspline2(u, sp.n_nodes, t);
+    const float n0f = rintf(u);
+    const float t = u - n0f;
+    const int n0 = min((int)n0f, sp.n_nodes);
     const float2 as = tab[kSoftPadL + n0];
     return __builtin_fmaf(t, as.y, as.x) * rinv;
 }
@@ -163,7 +164,7 @@ __device__ __forceinline__ float soft_disp(const float4 &pi, float xj, float yj,
         dy = f0 * b.c[1] + f1 * b.c[4] + f2 * b.c[7];
         dz = f0 * b.c[2] + f1 * b.c[5] + f2 * b.c[8];
     }
-    return dx * dx + dy * dy + dz * dz;
+    return __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
 }
 
 template <int MODE, int PBC>
@@ -230,41 +231,60 @@ struct SoftFwdCons {
     __device__ __forceinline__ void unit_end(int, int, int) {}
 };
 
-// Backward consumer.  For every home atom i of a chunk the in-range pairs are
-// compacted into a per-wave buffer (displacement, s, and i, j lane, rank in
-// i's run); 64 buffered pairs at a time then evaluate dL/dr with every lane
-// busy.  A pair's gradient g goes to the chunk's j slot in LDS (+g) and to
-// home atom i (-g).  The buffer holds each i's pairs as one contiguous run,
-// so i's share of a batch is a difference of two wave prefix sums, taken at
-// the run's last lane (one LDS add per run instead of one per pair on the
-// same address).  The j adds of one batch come from several runs that
-// mostly hit the same j slots; LDS float adds of one instruction to one
-// address are very slow, so run q of a batch adds into copy q mod
-// kSoftJCopies of the j sums (one run never repeats a j slot).  At the end of a chunk the buffer is drained and each j lane
-// adds its accumulated gradient to the output once; home atoms beyond
-// kSoftMaxHome add theirs directly.  The output is indexed by cell-sorted
-// position, where a chunk's lanes mostly hold consecutive atoms, so the
-// atomics of one instruction fall on a few cache lines.
-__device__ __forceinline__ float wave_prefix_sum(float v) {
-    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x111, 0xF, 0xF, false));  // row_shr:1
-    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x112, 0xF, 0xF, false));  // row_shr:2
-    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x114, 0xF, 0xF, false));  // row_shr:4
-    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x118, 0xF, 0xF, false));  // row_shr:8
-    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x142, 0xA, 0xF, false));  // row_bcast:15
-    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x143, 0xC, 0xF, false));  // row_bcast:31
-    return v;
+// Backward consumer.  Lanes hold the chunk's j atoms and the home atoms i
+// come four at a time (uniform, scalar loads); every lane evaluates its pair
+// with each of them without compaction (masked lanes give a zero force).  A
+// pair's gradient g = (dL/dr / r) (x_j - x_i) goes to j (+g) and to i (-g):
+//  * j's share stays in the lane's registers for the whole chunk and is
+//    added to the output once;
+//  * i's share is summed over lanes: the 4 home atoms' 3 components (plus a
+//    zero pad, 16 values) are reduced over each 16-lane row by recursive
+//    halving (4 DPP steps; afterwards lane l holds the row sum of value
+//    v(l) = 8 b0 + 4 b1 + 2 b2 + b3 of its lane bits), and each lane adds its
+//    value to a per-row LDS slot of the home atom (64 distinct addresses per
+//    instruction); the 4 row copies are summed once at the end of the unit.
+// Home atoms past the consumer's MAXH LDS slots add their row sums to the
+// output directly (MAXH = 32 for the usual cells of about 20 atoms, 128 for
+// dense cells; the smaller LDS footprint keeps three workgroups per CU).
+// The output is indexed by cell-sorted position, where a chunk's lanes mostly
+// hold consecutive atoms, so the atomics of one instruction fall on a few
+// cache lines.
+// floats per row copy of the home sums (padded: rows on distinct banks)
+__host__ __device__ constexpr int soft_gi_row(int maxh) { return maxh * 4 + 16; }
+
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
 }
 
-template <int PBC, bool LT>
+// The value of lane l ^ (1 << B) within a 16-lane row.
+template <int B>
+__device__ __forceinline__ float row_xor(float v) {
+    if constexpr (B == 0) return dpp_f<0xB1>(v);                 // quad_perm [1,0,3,2]
+    else if constexpr (B == 1) return dpp_f<0x4E>(v);            // quad_perm [2,3,0,1]
+    else if constexpr (B == 2) return dpp_f<0x141>(dpp_f<0x1B>(v));   // half-row mirror of quad_perm [3,2,1,0]
+    else return dpp_f<0x141>(dpp_f<0x140>(v));                   // half-row mirror of the row mirror
+}
+
+// One recursive-halving step over lane bit B: M values -> M / 2, a lane with
+// the bit set keeping the upper half.
+template <int M, int B>
+__device__ __forceinline__ void row_halve(float (&v)[16], bool hi) {
+#pragma unroll
+    for (int k = 0; k < M / 2; ++k) {
+        const float keep = hi ? v[k + M / 2] : v[k];
+        const float send = hi ? v[k] : v[k + M / 2];
+        v[k] = keep + row_xor<B>(send);
+    }
+}
+
+template <int PBC, bool LT, int MAXH>
 struct SoftBwdCons {
+    static constexpr int kGiRow = soft_gi_row(MAXH);
     SoftParams sp;
     const float2 *tab;  // LDS (A, S) table (padded)
-    double *gi;         // this wave's [3][kSoftMaxHome] home-atom gradient sums
-    double *gj;         // this wave's [kSoftJCopies][3][64] gradient sums of the chunk's j atoms
-    float4 *cbuf;       // this wave's [128] buffered pairs (dx, dy, dz, s)
-    unsigned *cid;      // and their i | j lane << 20 | rank in i's run << 26
+    float *gi;          // this wave's [4][kGiRow] row copies of the home-atom sums ([i][4] per row)
     int hs;             // the current unit's first home position
-    int ph, pt;         // buffer head / tail (uniform)
     __device__ __forceinline__ void add_grad(int f, unsigned pos, float gx, float gy, float gz) {
         float *g = sp.grad + (long long)f * 3 * sp.n + pos;
         atomicAdd(g, gx);
@@ -276,50 +296,6 @@ struct SoftBwdCons {
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
-    // Evaluates cnt (<= 64) buffered pairs.
-    __device__ __forceinline__ void drain(int cnt, int f) {
-        const int lane = threadIdx.x & 63;
-        const bool act = lane < cnt;
-        float gx = 0.f, gy = 0.f, gz = 0.f;
-        unsigned id = act ? cid[(ph + lane) & 127] : 0u;
-        // index of this lane's run within the batch (runs start at lane 0 and where rank = 0)
-        const unsigned long long starts = __ballot(act && (lane == 0 || (id >> 26) == 0u));
-        const int run = (int)__popcll(starts & ((2ull << lane) - 1ull)) - 1;
-        double *gjc = gj + (run & (kSoftJCopies - 1)) * 192;
-        if (act) {
-            const int slot = (ph + lane) & 127;
-            const float4 e = cbuf[slot];
-            const float fr = soft_force_over_r(e.w, sp, tab);  // (dL/dr) / r
-            gx = fr * e.x;
-            gy = fr * e.y;
-            gz = fr * e.z;
-            const unsigned jl = (id >> 20) & 63u;
-            atomicAdd(&gjc[jl], (double)gx);
-            atomicAdd(&gjc[64 + jl], (double)gy);
-            atomicAdd(&gjc[128 + jl], (double)gz);
-        }
-        // i's share: prefix sums at the run's last lane minus those before its first
-        const float px = wave_prefix_sum(gx), py = wave_prefix_sum(gy), pz = wave_prefix_sum(gz);
-        const int r = (int)(id >> 26);
-        const int first = lane - r;  // < 0 when the run began in an earlier batch
-        const int nr = __shfl_down((int)(id >> 26), 1, 64);
-        const int src = first > 0 ? first - 1 : lane;
-        const float bx = __shfl(px, src, 64), by = __shfl(py, src, 64), bz = __shfl(pz, src, 64);
-        if (act && (lane == cnt - 1 || nr == 0)) {
-            const float sx = first > 0 ? px - bx : px;
-            const float sy = first > 0 ? py - by : py;
-            const float sz = first > 0 ? pz - bz : pz;
-            const unsigned i = id & 0xFFFFFu;
-            if (i < (unsigned)kSoftMaxHome) {
-                atomicAdd(&gi[i], (double)sx);
-                atomicAdd(&gi[kSoftMaxHome + i], (double)sy);
-                atomicAdd(&gi[2 * kSoftMaxHome + i], (double)sz);
-            } else {
-                add_grad(f, (unsigned)hs + i, -sx, -sy, -sz);
-            }
-        }
-        ph += cnt;
-    }
     __device__ __forceinline__ void unit_begin(int, int h0, int) { hs = h0; }
     template <int MODE, bool HOME>
     __device__ __forceinline__ void chunk(const float4 *hp, int n_home, float xj, float yj, float zj,
@@ -327,67 +303,92 @@ struct SoftBwdCons {
                                           const FrameBox &b) {
         float sx, sy, sz;
         soft_shift<MODE, PBC>(mj, b, sx, sy, sz);
+        // a full chunk has finite positions in every lane, so a masked force
+        // of zero gives zero products; lanes past a partial chunk hold NaN
+        // positions and their products are masked instead
+        if (__builtin_amdgcn_readfirstlane((int)__any(xj != xj)))
+            pairs<MODE, HOME, false>(hp, n_home, xj, yj, zj, sx, sy, sz, rank, jpos, f, b);
+        else
+            pairs<MODE, HOME, true>(hp, n_home, xj, yj, zj, sx, sy, sz, rank, jpos, f, b);
+    }
+    // One pair's gradient g (masked lanes: exact zeros); returns it through gx, gy, gz.
+    template <int MODE, bool HOME, bool FULL>
+    __device__ __forceinline__ void pair_grad(const float4 &a, int i, int n_home, float xj, float yj, float zj,
+                                              float sx, float sy, float sz, int rank, const FrameBox &b,
+                                              float &gx, float &gy, float &gz) {
+        float dx, dy, dz;
+        const float s = soft_disp<PBC, LT, MODE>(a, xj, yj, zj, sx, sy, sz, b, dx, dy, dz);
+        const bool in = i < n_home && s < sp.cut2 && s > 0.f && (!HOME || i < rank);
+        const float fr = soft_force_over_r(s, sp, tab);  // (dL/dr) / r
+        if constexpr (FULL) {
+            const float frm = in ? fr : 0.f;
+            gx = frm * dx;
+            gy = frm * dy;
+            gz = frm * dz;
+        } else {
+            gx = in ? fr * dx : 0.f;
+            gy = in ? fr * dy : 0.f;
+            gz = in ? fr * dz : 0.f;
+        }
+    }
+    template <int MODE, bool HOME, bool FULL>
+    __device__ __forceinline__ void pairs(const float4 *hp, int n_home, float xj, float yj, float zj,
+                                          float sx, float sy, float sz, int rank, unsigned jpos, int f,
+                                          const FrameBox &b) {
         const int lane = threadIdx.x & 63;
-        auto pair = [&](int i, const float4 &a) {
-            float dx, dy, dz;
-            const float s = soft_disp<PBC, LT, MODE>(a, xj, yj, zj, sx, sy, sz, b, dx, dy, dz);
-            const bool in = s < sp.cut2 && (!HOME || i < rank) && s > 0.f;
-            const unsigned long long bal = __ballot(in);
-            if (bal) {
-                if (in) {
-                    const int r = lanes_below(bal);
-                    const int slot = (pt + r) & 127;
-                    cbuf[slot] = make_float4(dx, dy, dz, s);
-                    cid[slot] = (unsigned)i | ((unsigned)lane << 20) | ((unsigned)r << 26);
-                }
-                pt += (int)__popcll(bal);
-                if (pt - ph >= 64) {
-                    wave_sync();
-                    drain(64, f);
-                }
-            }
-        };
-        // home atoms four at a time (scalar loads)
-        int i = 0;
-        for (; i + 4 <= n_home; i += 4) {
-            const float4 a0 = uload(hp, i), a1 = uload(hp, i + 1), a2 = uload(hp, i + 2),
-                         a3 = uload(hp, i + 3);
-            pair(i, a0);
-            pair(i + 1, a1);
-            pair(i + 2, a2);
-            pair(i + 3, a3);
-        }
-        for (; i < n_home; ++i) pair(i, uload(hp, i));
-        // the chunk's remaining pairs, then its j gradients
-        wave_sync();
-        if (pt > ph) drain(pt - ph, f);
-        wave_sync();
-        double tx = 0.0, ty = 0.0, tz = 0.0;
+        const int row = lane >> 4;
+        const int v = ((lane & 1) << 3) | ((lane & 2) << 1) | ((lane & 4) >> 1) | ((lane & 8) >> 3);
+        const bool b0 = lane & 1, b1 = lane & 2, b2 = lane & 4, b3 = lane & 8;
+        float gjx = 0.f, gjy = 0.f, gjz = 0.f;
+        for (int i0 = 0; i0 < n_home; i0 += 4) {
+            float g[16];
 #pragma unroll
-        for (int c = 0; c < kSoftJCopies; ++c) {
-            double *q = gj + c * 192;
-            tx += q[lane];
-            ty += q[64 + lane];
-            tz += q[128 + lane];
-            q[lane] = 0.0;
-            q[64 + lane] = 0.0;
-            q[128 + lane] = 0.0;
+            for (int ii = 0; ii < 4; ++ii) {
+                const int i = i0 + ii;
+                const float4 a = uload(hp, min(i, n_home - 1));
+                float gx, gy, gz;
+                pair_grad<MODE, HOME, FULL>(a, i, n_home, xj, yj, zj, sx, sy, sz, rank, b, gx, gy, gz);
+                gjx += gx;
+                gjy += gy;
+                gjz += gz;
+                g[4 * ii] = gx;
+                g[4 * ii + 1] = gy;
+                g[4 * ii + 2] = gz;
+                g[4 * ii + 3] = 0.f;
+            }
+            row_halve<16, 0>(g, b0);
+            row_halve<8, 1>(g, b1);
+            row_halve<4, 2>(g, b2);
+            row_halve<2, 3>(g, b3);
+            // lane: row sum of value v = 4 ii + component (component 3: the zero pad)
+            if (i0 < MAXH) {
+                atomicAdd(&gi[row * kGiRow + i0 * 4 + v], g[0]);
+            } else if ((v & 3) != 3 && i0 + (v >> 2) < n_home) {
+                float *o = sp.grad + (long long)f * 3 * sp.n + (v & 3) * sp.n + hs + i0 + (v >> 2);
+                atomicAdd(o, -g[0]);
+            }
         }
-        const float gx = (float)tx, gy = (float)ty, gz = (float)tz;
-        if (jpos != 0xFFFFFFFFu && (gx != 0.f || gy != 0.f || gz != 0.f)) add_grad(f, jpos, gx, gy, gz);
-        wave_sync();
+        if (jpos != 0xFFFFFFFFu && (gjx != 0.f || gjy != 0.f || gjz != 0.f)) add_grad(f, jpos, gjx, gjy, gjz);
     }
     __device__ __forceinline__ void unit_end(int f, int, int n_home) {
-        wave_sync();
         const int lane = threadIdx.x & 63;
-        if (lane < min(n_home, kSoftMaxHome)) {
-            const float gx = (float)gi[lane], gy = (float)gi[kSoftMaxHome + lane],
-                        gz = (float)gi[2 * kSoftMaxHome + lane];
-            add_grad(f, (unsigned)(hs + lane), -gx, -gy, -gz);
-            gi[lane] = 0.0;
-            gi[kSoftMaxHome + lane] = 0.0;
-            gi[2 * kSoftMaxHome + lane] = 0.0;
+        wave_sync();
+        for (int i = lane; i < min(n_home, MAXH); i += 64) {
+            float t[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                float acc = 0.f;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) acc += gi[r * kGiRow + i * 4 + c];
+                t[c] = acc;
+            }
+            add_grad(f, (unsigned)(hs + i), -t[0], -t[1], -t[2]);
         }
+        wave_sync();
+        // clear what the unit used (the pad components included)
+        const int used = min(n_home, MAXH) * 4;
+        for (int r = 0; r < 4; ++r)
+            for (int e = lane; e < used; e += 64) gi[r * kGiRow + e] = 0.f;
         wave_sync();
     }
 };
@@ -437,36 +438,29 @@ __global__ __launch_bounds__(kSoftBlock) void soft_cell_fwd_kernel(CellGrid G, S
     }
 }
 
-constexpr int kSoftBwdBlock = 512;  // 8 waves (two workgroups per CU within the LDS budget)
+constexpr int kSoftBwdBlock = 512;  // 8 waves (three workgroups per CU within the LDS budget at 300 bins)
 constexpr int kSoftBwdWaves = kSoftBwdBlock / 64;
 
-// LDS of the backward cell kernel: (A, S) table | per wave: cbuf[128] float4,
-// ring[kRing] float4, gj[kSoftJCopies][3][64] and gi[3][kSoftMaxHome] f64,
-// cid[128], ring positions[kRing].
-__host__ __device__ inline size_t soft_bwd_lds_bytes(int tn) {
-    return align16((size_t)tn * 8) +
-           (size_t)kSoftBwdWaves * ((128 + kRing) * 16 + (kSoftJCopies * 192 + 3 * kSoftMaxHome) * 8 +
-                                    128 * 4 + kRing * 4);
+// LDS of the backward cell kernel: (A, S) table | per wave: ring[kRing]
+// float4, ring positions [kRing], the home-atom row sums [4][soft_gi_row(maxh)].
+__host__ __device__ inline size_t soft_bwd_lds_bytes(int tn, int maxh) {
+    return align16((size_t)tn * 8) + (size_t)kSoftBwdWaves * (kRing * 16 + kRing * 4 + 4 * soft_gi_row(maxh) * 4);
 }
 
-template <int PBC, bool LT>
+template <int PBC, bool LT, int MAXH>
 __global__ __launch_bounds__(kSoftBwdBlock) void soft_cell_bwd_kernel(CellGrid G, SoftParams sp) {
+    constexpr int kGiRow = soft_gi_row(MAXH);
     extern __shared__ __attribute__((aligned(16))) char lds[];
     const int tn = sp.n_nodes + kSoftPadL + kSoftPadR;
     float2 *tab = (float2 *)lds;
-    float4 *cbufs = (float4 *)(lds + align16((size_t)tn * 8));
-    float4 *rings = cbufs + kSoftBwdWaves * 128;
-    double *gjs = (double *)(rings + kSoftBwdWaves * kRing);
-    double *gis = gjs + kSoftBwdWaves * kSoftJCopies * 192;
-    unsigned *cids = (unsigned *)(gis + kSoftBwdWaves * 3 * kSoftMaxHome);
-    unsigned *ringps = cids + kSoftBwdWaves * 128;
+    float4 *rings = (float4 *)(lds + align16((size_t)tn * 8));
+    unsigned *ringps = (unsigned *)(rings + kSoftBwdWaves * kRing);
+    float *gis = (float *)(ringps + kSoftBwdWaves * kRing);
     for (int i = threadIdx.x; i < tn; i += kSoftBwdBlock) tab[i] = sp.table[i];
-    for (int i = threadIdx.x; i < kSoftBwdWaves * (kSoftJCopies * 192 + 3 * kSoftMaxHome); i += kSoftBwdBlock)
-        gjs[i] = 0.0;  // gj and gi are contiguous
+    for (int i = threadIdx.x; i < kSoftBwdWaves * 4 * kGiRow; i += kSoftBwdBlock) gis[i] = 0.f;
     __syncthreads();
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    SoftBwdCons<PBC, LT> cons{sp, tab, gis + wave * 3 * kSoftMaxHome, gjs + wave * kSoftJCopies * 192, cbufs + wave * 128,
-                              cids + wave * 128, 0, 0, 0};
+    SoftBwdCons<PBC, LT, MAXH> cons{sp, tab, gis + wave * 4 * kGiRow, 0};
     cell_waves<PBC, true>(G, rings + wave * kRing, ringps + wave * kRing, wave, cons);
 }
 
@@ -1892,13 +1886,20 @@ void soft_run(wmd_softrdf *h, bool backward, const float *d_soa, size_t stride, 
             else
                 launch_soft_cells(h, st, soft_cell_fwd_kernel<WMD_PBC_TRICLINIC, false>, kSoftBlock, lds, G, sp, "softrdf_fwd");
         } else {
-            const size_t lds = soft_bwd_lds_bytes(tn);
+            // home slots in LDS: 32 for cells of up to ~24 atoms on average, else 128
+            const double per_cell = (double)n / ((double)pl.nc[0] * pl.nc[1] * pl.nc[2]);
+            auto bwd = [&](auto k32, auto k128) {
+                if (per_cell <= 24.0)
+                    launch_soft_cells(h, st, k32, kSoftBwdBlock, soft_bwd_lds_bytes(tn, 32), G, sp, "softrdf_bwd");
+                else
+                    launch_soft_cells(h, st, k128, kSoftBwdBlock, soft_bwd_lds_bytes(tn, 128), G, sp, "softrdf_bwd");
+            };
             if (h->pbc == WMD_PBC_ORTHO)
-                launch_soft_cells(h, st, soft_cell_bwd_kernel<WMD_PBC_ORTHO, false>, kSoftBwdBlock, lds, G, sp, "softrdf_bwd");
+                bwd(soft_cell_bwd_kernel<WMD_PBC_ORTHO, false, 32>, soft_cell_bwd_kernel<WMD_PBC_ORTHO, false, 128>);
             else if (pl.lt)
-                launch_soft_cells(h, st, soft_cell_bwd_kernel<WMD_PBC_TRICLINIC, true>, kSoftBwdBlock, lds, G, sp, "softrdf_bwd");
+                bwd(soft_cell_bwd_kernel<WMD_PBC_TRICLINIC, true, 32>, soft_cell_bwd_kernel<WMD_PBC_TRICLINIC, true, 128>);
             else
-                launch_soft_cells(h, st, soft_cell_bwd_kernel<WMD_PBC_TRICLINIC, false>, kSoftBwdBlock, lds, G, sp, "softrdf_bwd");
+                bwd(soft_cell_bwd_kernel<WMD_PBC_TRICLINIC, false, 32>, soft_cell_bwd_kernel<WMD_PBC_TRICLINIC, false, 128>);
             int g;
             grid1d((long long)n * nf, &g);
             hipLaunchKernelGGL(soft_unsort_kernel, dim3(g), dim3(kBlock), 0, st, (const float *)sp.grad,
