@@ -726,6 +726,21 @@ def uneven_batch_bench(ctx, dev, reps=3, autotune=True, cpu_sample=True):
     counts = b.counts()
     pairs_in = float(counts.sum()) / 2.0 / reps  # unordered in-range pairs per step
     ref_evals = float(np.sum(F * ns * (ns - 1)))
+    # the pair kernel's own duration from its per-slice wall-clock marks (first
+    # wave start to last wave end), over further replays of the same graph
+    marks = []
+    for _ in range(reps):
+        b.run(store, rm)
+        marks.append(b.slice_times())
+    marks = np.array(marks)
+    pair_ms = float(np.mean(marks[:, :, 2].max(axis=1)))
+    drained = marks[:, :, 1]
+    ach = RDF_OPS_PER_PAIR * pairs_in / (pair_ms / 1e3) / 1e12
+    k_t, bh_t, g_t = b.plan_tuning()
+    tuning = {}
+    for k, bh, g in zip(k_t.tolist(), bh_t.tolist(), g_t.tolist()):
+        key = f"k={k}" + (f" home={bh}" if k == 0 else "") + f" grab={g}"
+        tuning[key] = tuning.get(key, 0) + 1
     # the same work as one plan call per trajectory (RDF accumulate + RMSD run)
     V = lambda t: ctypes.c_void_p(t.data_ptr())
     handles = []
@@ -760,7 +775,8 @@ def uneven_batch_bench(ctx, dev, reps=3, autotune=True, cpu_sample=True):
         lib().wmd_rdf_destroy(h)
         lib().wmd_rmsd_destroy(hr)
     log(f"uneven batch: {ms:.2f} ms per graph step ({units} pair units, "
-        f"{int((kinds == 0).sum())} brute-force trajectories), 256 single-plan calls {single_ms:.2f} ms")
+        f"{int((kinds == 0).sum())} brute-force trajectories), pair kernel {pair_ms:.2f} ms "
+        f"({ach / VALU_PEAK_TOPS:.3f} of VALU peak), 256 single-plan calls {single_ms:.2f} ms")
     # the oracle's reference loop on a bounded sample: frame 0 of the
     # trajectories at the 25th, 50th, 75th and 90th size percentiles
     cpu = None
@@ -786,6 +802,17 @@ def uneven_batch_bench(ctx, dev, reps=3, autotune=True, cpu_sample=True):
             "unordered_pairs_in_range": pairs_in,
             "single_plan_calls_ms": single_ms, "speedup_vs_single_calls": single_ms / ms,
             "autotune_s": tune_s, "cell_k_histogram": {str(k): int((kinds == k).sum()) for k in sorted(set(kinds.tolist()))},
+            "tuning_histogram": tuning,
+            "roofline": {"kernel": "batch_pair_kernel", "bound": "valu", "achieved": ach, "peak": VALU_PEAK_TOPS,
+                         "unit": "TOP/s", "frac": ach / VALU_PEAK_TOPS, "traffic": None,
+                         "algorithmic": f"{RDF_OPS_PER_PAIR} VALU lane-ops per unordered in-range pair x "
+                                        f"{pairs_in:.4g} pairs per step",
+                         "avg_launch_ms": pair_ms,
+                         "timing": "wall-clock marks inside the kernel (first wave start to last wave end), "
+                                   f"mean of {reps} graph replays"},
+            "slices": {"drained_ms": [round(float(x), 3) for x in drained.mean(axis=0)],
+                       "end_ms": [round(float(x), 3) for x in marks[:, :, 2].mean(axis=0)],
+                       "tasks_helped": [int(x) for x in marks[:, :, 3].mean(axis=0)]},
             "workload": "256 trajectories, N log-uniform [500, 50000] (seed 3, sum N = 2,612,540), "
                         "density 0.1 /A^3, 50 frames, RDF 300 bins r_max 12 A + RMSD to frame 0"}
 
@@ -801,6 +828,7 @@ def main():
     ap.add_argument("--no-triclinic", action="store_true")
     ap.add_argument("--no-soft", action="store_true", help="skip the soft-RDF ratio")
     ap.add_argument("--no-batch", action="store_true", help="skip the uneven batch")
+    ap.add_argument("--only-batch", action="store_true", help="time only the uneven batch")
     ap.add_argument("--no-pairwise", action="store_true", help="skip the pairwise RMSD matrix")
     ap.add_argument("--only-pairwise", action="store_true", help="time only the pairwise RMSD matrix")
     ap.add_argument("--no-align", action="store_true", help="skip the superposition leg")
@@ -831,6 +859,9 @@ def main():
 
     if args.counters:
         counter_sequence(ctx, dev)
+        return
+    if args.only_batch:
+        print(json.dumps({"uneven_batch": uneven_batch_bench(ctx, dev, cpu_sample=not args.no_cpu)}), flush=True)
         return
     if args.only_sf_bymol:
         print(json.dumps({"sf_bymol": sf_bymol_bench(ctx, dev)}), flush=True)

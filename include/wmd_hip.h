@@ -155,9 +155,11 @@ void wmd_rdf_destroy(wmd_rdf *rdf);
 /* Which pair kernel the last accumulate used: 0 none yet, 1 brute force,
  * 2 cell list; *cells_per_side receives the last cell grid (or zeros). */
 wmd_status wmd_rdf_last_path(const wmd_rdf *rdf, int *path, int *cells_per_side);
-/* Cell-list shape override from an autotuner: k = cells per r_max (1..3, 0 =
- * automatic).  Takes effect on the next accumulate. */
-wmd_status wmd_rdf_set_tuning(wmd_rdf *rdf, int cell_k, int block_threads);
+/* Cell-list overrides from an autotuner: k = cells per r_max (1..3, 0 =
+ * automatic) and grab = cell units a wave takes per visit of its XCD's work
+ * queue (1..64, 0 = automatic: 4 for launches of at least 64 units per wave,
+ * else 2).  Take effect on the next accumulate. */
+wmd_status wmd_rdf_set_tuning(wmd_rdf *rdf, int cell_k, int grab);
 
 /* RdfPlan::finalize normalisation (rdf.rs:943-1016) on host data: r, g [bins]
  * and, with WMD_RDF_INTEGRAL, integral [bins] (else may be NULL).  flags:
@@ -440,14 +442,20 @@ wmd_status wmd_batch_store_layout(const wmd_batch *b, uint64_t *offsets, uint32_
  * WMD_PBC_NONE).  Plans the enumeration (cell list or brute force per
  * trajectory, units, XCD slices); must precede the first run. */
 wmd_status wmd_batch_set_boxes(wmd_batch *b, const float *boxes);
-/* Per-trajectory enumeration override: cell_k[t] = 0 automatic, -1 brute
- * force, 1..3 cells per r_max (NULL = all automatic).  Re-plans. */
-wmd_status wmd_batch_set_tuning(wmd_batch *b, const int *cell_k);
+/* Per-trajectory tuning override, each array [n_traj] or NULL (all 0):
+ * cell_k[t] 0 automatic, -1 brute force, 1..3 cells per r_max;
+ * home_block[t] 0 (64) or a multiple of 64 up to 512: home atoms of one
+ * brute-force unit; grab[t] 0 (2) or 1..64: units per queue task.  Re-plans. */
+wmd_status wmd_batch_set_tuning(wmd_batch *b, const int *cell_k, const int *home_block,
+                                const int *grab);
 /* Project autotuner: for every class of trajectories sharing (device arch,
- * pbc, bins, r_max, atom-count bucket, density bucket) times brute force and
- * cells per r_max k = 1, 2, 3 on the class's median trajectory (reps timed
- * runs each, after a warm-up) and keeps the fastest; choices are cached per
- * process (see wmd_autotune_export / import) and applied to the batch. */
+ * pbc, bins, r_max, atom-count bucket, density bucket) it times, on a
+ * sub-batch of the class's members (a frame prefix of each, replicated until
+ * one run holds about 4e6 atom-frames; reps timed runs after a warm-up):
+ * brute force and cells per r_max k = 1, 2, 3; then, for brute force, home
+ * blocks of 128 and 256 atoms; then 1, 4 and 8 units per task, keeping the
+ * fastest at each step.  Choices are cached per process (see
+ * wmd_autotune_export / import) and applied to the batch. */
 wmd_status wmd_batch_autotune(wmd_batch *b, const float *d_store, int reps, void *stream);
 /* One step over the device store: adds every trajectory's counts to the
  * batch's u64 histograms [n_traj][bins] (WMD_BATCH_RDF) and writes d_rmsd
@@ -461,14 +469,23 @@ wmd_status wmd_batch_run_device(wmd_batch *b, const float *d_store, float *d_rms
 wmd_status wmd_batch_read_counts(wmd_batch *b, uint64_t *counts);
 /* Zeroes the histograms. */
 wmd_status wmd_batch_reset(wmd_batch *b);
-/* The plan: cell_k[t] (0 = brute force) and the number of pair units. */
-wmd_status wmd_batch_plan_info(const wmd_batch *b, int *cell_k, long long *n_units);
+/* The plan, arrays [n_traj] or NULL: cell_k[t] (0 = brute force),
+ * home_block[t] (brute force; 0 for cells), grab[t] (units per task), and the
+ * number of pair units. */
+wmd_status wmd_batch_plan_info(const wmd_batch *b, int *cell_k, int *home_block, int *grab,
+                               long long *n_units);
+/* Wall-clock marks of the last step's pair kernel per queue slice (8 slices,
+ * one per XCD): out[8][4] = (first wave start, slice drained, last wave end)
+ * in ms from the earliest start (-1 if unset), and the number of the slice's
+ * tasks that waves of other slices took.  Synchronises. */
+wmd_status wmd_batch_slice_times(wmd_batch *b, double *out);
 void wmd_batch_destroy(wmd_batch *b);
 /* The autotuner's process-wide cache as text, one choice per line
- * ("arch pbc bins r_max_bits n_bucket density_bucket k"); *len gets the full
+ * ("arch pbc bins r_max_bits n_bucket density_bucket k home_block grab"); *len gets the full
  * length, buf (cap bytes, NUL-terminated) may be NULL. */
 wmd_status wmd_autotune_export(char *buf, size_t cap, size_t *len);
-/* Adds lines in the export format to the cache. */
+/* Adds lines in the export format to the cache (lines of the first seven
+ * fields only keep the default home block and task size). */
 wmd_status wmd_autotune_import(const char *text);
 
 /* QCP superposition of one set of centred moments, evaluated on the host with

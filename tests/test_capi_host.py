@@ -152,3 +152,25 @@ def test_system_selection_subset():
     np.testing.assert_array_equal(sysm.select("index 1-2").indices, [1, 2])
     with pytest.raises(RuntimeError, match="invalid selection"):
         sysm.select("within 5 of name OW")
+
+
+def test_autotune_cache_text_round_trip():
+    """The tuning cache's text form: nine fields per line (k, home block, units
+    per task after the key); seven-field lines keep the defaults; malformed or
+    out-of-range lines are refused without changing the cache."""
+    from warp_md_amd.batch import export_tuning, import_tuning
+
+    import_tuning("gfx950 1 300 1094713344 60 -14 2 0 4\n\ngfx950 1 300 1094713344 61 -14 -1\n")
+    lines = [l.split() for l in export_tuning().splitlines()]
+    mine = {tuple(l[:6]): l[6:] for l in lines}
+    assert mine[("gfx950", "1", "300", "1094713344", "60", "-14")] == ["2", "0", "4"]
+    assert mine[("gfx950", "1", "300", "1094713344", "61", "-14")] == ["-1", "0", "0"]
+    before = export_tuning()
+    for bad in ["gfx950 1 300 1094713344 62 -14 -1 128",      # eight fields
+                "gfx950 1 300 1094713344 62 -14 4 0 0",       # k out of range
+                "gfx950 1 300 1094713344 62 -14 -1 96 0",     # home block not a multiple of 64
+                "gfx950 1 300 1094713344 62 -14 -1 64 65",    # task size out of range
+                "gfx950 1 300 1094713344 62 -14 -1 64 2 9"]:  # trailing field
+        with pytest.raises(RuntimeError, match="parse"):
+            import_tuning(bad)
+    assert export_tuning() == before

@@ -172,6 +172,39 @@ def test_forced_enumerations_agree():
         assert np.array_equal(b.counts()[0], ref), (k, kinds)
 
 
+def test_home_blocks_and_task_sizes_agree():
+    """Brute-force home blocks of 64..512 atoms (sizes that are and are not
+    multiples of the block, and one below a single wave) and 1..64 units per
+    task, set per trajectory, all give the oracle's counts."""
+    sizes = [37, 300, 700, 1100, 5000]
+    coords, boxes = _case(sizes, n_frames=2, seed=21)
+    refs = [_oracle_counts(c, box, "orthorhombic") for c, box in zip(coords, boxes)]
+    for bh, g in [(64, 1), (128, 3), (256, 8), (512, 64), (192, 2)]:
+        b, store = _batch(coords, boxes, rmsd=False)
+        b.set_tuning([-1, -1, -1, -1, 0], [bh] * len(sizes), [g] * len(sizes))
+        kinds, homes, grabs = b.plan_tuning()
+        assert list(kinds[:4]) == [0, 0, 0, 0] and kinds[4] > 0
+        assert list(homes[:4]) == [bh] * 4 and list(grabs) == [g] * len(sizes)
+        b.run(store)
+        counts = b.counts()
+        for t in range(len(sizes)):
+            assert np.array_equal(counts[t], refs[t]), (bh, g, t)
+
+
+def test_slice_times_are_consistent():
+    coords, boxes = _case([3000, 8000, 20000, 900], n_frames=4, seed=22)
+    b, store = _batch(coords, boxes, rmsd=False)
+    b.run(store)
+    st = b.slice_times()
+    print("slice times", st.tolist(), flush=True)
+    assert st.shape == (8, 4)
+    started = st[:, 0] >= 0
+    assert started.all()
+    assert np.all(st[:, 2] >= st[:, 0])
+    assert np.all((st[:, 1] < 0) | (st[:, 1] <= st[:, 2] + 1e-3))
+    assert np.all(st[:, 3] >= 0)
+
+
 def test_full_uneven_batch_matches_single_plans():
     """256 trajectories, log-uniform N in [500, 50000], density 0.1, 50 frames:
     the batch equals the single-trajectory engine paths on every trajectory."""

@@ -20,14 +20,21 @@
 // brute-force trajectories) is built by batched assign / scan / scatter
 // kernels over all trajectories at once.
 //
-// The pair kernel is persistent with a work queue: the unit range is split
-// into 8 slices of equal estimated cost (slice q is served by workgroups
-// g % 8 == q, i.e. one XCD under round-robin dispatch), and each wave takes
-// kGrab consecutive units at a time from its slice's atomic counter until the
-// counter passes the slice's end (every wave reaches that exit).  Each wave
-// keeps a private u64 LDS histogram for the trajectory it is on and merges it
-// into that trajectory's global counts when it moves to another trajectory
-// and at the end.
+// The pair kernel is persistent with a work queue.  A trajectory's units are
+// grouped into tasks of `grab` consecutive units (a task never spans two
+// trajectories); the task range is split into 8 slices of equal estimated
+// cost (slice q is served by workgroups g % 8 == q, i.e. one XCD under
+// round-robin dispatch), and each wave takes one task at a time from its
+// slice's atomic counter.  A wave whose slice is drained moves on to the next
+// slices in turn and helps there, so a slice whose cost was underestimated
+// does not leave the other XCDs idle; every wave exits after it has found all
+// 8 counters past their slice ends.  Each wave keeps a private u64 LDS
+// histogram for the trajectory it is on and merges it into that trajectory's
+// global counts when it moves to another trajectory and at the end.
+//
+// Tuning per trajectory: cells per r_max k (or brute force), the brute-force
+// home block (home atoms per unit, a multiple of 64) and the units per task;
+// wmd_batch_autotune picks them per class of similar trajectories by timing.
 //
 // The whole step (clears by fill kernels, cell build, pair kernel, RMSD
 // moments and finalize) is captured once into a HIP graph and replayed;
@@ -48,9 +55,12 @@ namespace {
 
 constexpr int kBatchBlock = 512;  // 8 waves
 constexpr int kBatchWaves = kBatchBlock / 64;
-constexpr int kBruteHome = 64;    // i atoms per brute-force unit
-constexpr int kGrab = 4;          // units per queue grab
+constexpr int kBruteHome = 64;    // default home atoms per brute-force unit
+constexpr int kMaxBruteHome = 512;
+constexpr int kGrab = 2;          // default units per task
+constexpr int kMaxGrab = 64;
 constexpr int kSlices = 8;        // queue slices (one per XCD)
+constexpr int kClock = 4;         // per slice: first start, drained, last end, tasks taken by helpers
 
 enum { KIND_CELLS = 0, KIND_BRUTE = 1 };
 
@@ -60,9 +70,11 @@ struct BatchTraj {
     long long start_off;  // u32 offset of frame 0's cell starts [F][ncell + 1] (cells)
     long long count_off;  // u32 offset of frame 0's cell counts [F][ncell] (cells)
     long long unit0;      // first pair unit
+    long long task0;      // first task (grab consecutive units of this trajectory)
     long long scan0;      // first scan workgroup (cells)
     int stride, n, n_frames, box0;  // box0: index of frame 0's box and frame flag
     int kind, k, ncx, ncy, ncz, shift_ok, upf;  // upf: units per frame
+    int bh, grab;         // brute-force home atoms per unit; units per task
     float cull2;
 };
 
@@ -188,8 +200,9 @@ struct BatchPairArgs {
     const unsigned *frame_flag;
     const FrameBox *boxes;
     unsigned long long *counts;  // [n_traj][row_bins] (+ the pass's first bin)
-    unsigned long long *queue;   // [kSlices] grab counters, zero at launch
-    long long slice_lo[kSlices + 1];
+    unsigned long long *queue;   // [kSlices] task counters, zero at launch
+    unsigned long long *clock;   // [kSlices][kClock] wall-clock marks, zero at the step's start
+    long long slice_lo[kSlices + 1];  // task ranges of the slices
     int row_bins;                // bins of a trajectory's histogram row
     HistParams p;
 };
@@ -221,14 +234,15 @@ __device__ __forceinline__ void wave_hist_flush(unsigned long long *hist,
     wave_lds_sync();
 }
 
-// Brute-force unit: home block [I0, I0 + 64) of frame f against every j > i.
+// Brute-force unit: home block [I0, I0 + bh) of frame f against every j > i.
+// Chunks of j inside the home block pair only with the home atoms below them.
 template <class C>
-__device__ __forceinline__ void brute_unit(const CellGrid &G, int f, int blk, C &cons) {
+__device__ __forceinline__ void brute_unit(const CellGrid &G, int f, int blk, int bh, C &cons) {
     const int lane = threadIdx.x & 63;
     const float nanv = __builtin_nanf("");
     const float4 *fp = G.sorted + (long long)f * G.n;
-    const int I0 = blk * kBruteHome;
-    const int n_home = min(kBruteHome, G.n - I0);
+    const int I0 = blk * bh;
+    const int n_home = min(bh, G.n - I0);
     const float4 *hp = fp + I0;
     const FrameBox b = uload(G.boxes, f);
     for (int j0 = I0; j0 < G.n; j0 += 64) {
@@ -236,12 +250,14 @@ __device__ __forceinline__ void brute_unit(const CellGrid &G, int f, int blk, C 
         float4 q = make_float4(nanv, nanv, nanv, __uint_as_float(0x11202020u));
         if (j < G.n) q = fp[j];
         const unsigned mj = __float_as_uint(q.w);
-        if (j0 == I0)  // the home block itself: pairs (i, j) with i < j
-            cons.template chunk<3, true>(hp, n_home, q.x, q.y, q.z, mj, lane, 0u, f, b);
+        if (j0 < I0 + n_home)  // pairs (i, j) with i < j inside the home block
+            cons.template chunk<3, true>(hp, min(n_home, j0 - I0 + 64), q.x, q.y, q.z, mj, j - I0, 0u, f, b);
         else
             cons.template chunk<3, false>(hp, n_home, q.x, q.y, q.z, mj, 0x7FFFFFFF, 0u, f, b);
     }
 }
+
+__device__ __forceinline__ unsigned long long clock_now() { return (unsigned long long)wall_clock64(); }
 
 template <int PBC, bool TABLE>
 __global__ __launch_bounds__(kBatchBlock) void batch_pair_kernel(BatchPairArgs a) {
@@ -272,54 +288,65 @@ __global__ __launch_bounds__(kBatchBlock) void batch_pair_kernel(BatchPairArgs a
     L.ring = rings + wave * kRing;
     PlainCons<PBC, false, false, false, TABLE> cons{p, L, 0ull};
 
-    const int q = blockIdx.x % kSlices;
-    const long long lo = a.slice_lo[q], hi = a.slice_lo[q + 1];
+    const int home_q = blockIdx.x % kSlices;
+    if (a.clock && lane == 0) atomicMax(&a.clock[home_q * kClock], ~clock_now());
+    int q = home_q, visited = 0;
     int cur = -1;
-    long long cur_end = 0;
     BatchTraj T{};
     CellGrid G{};
     StencilLane sl{0, 0, 0, -1};
-    while (true) {
+    while (visited < kSlices) {
         unsigned long long g0 = 0;
-        if (lane == 0) g0 = atomicAdd(&a.queue[q], (unsigned long long)kGrab);
+        if (lane == 0) g0 = atomicAdd(&a.queue[q], 1ull);
         const unsigned g_lo = __builtin_amdgcn_readfirstlane((unsigned)g0);
         const unsigned g_hi = __builtin_amdgcn_readfirstlane((unsigned)(g0 >> 32));
-        const long long u0 = lo + (long long)(((unsigned long long)g_hi << 32) | g_lo);
-        if (u0 >= hi) break;
-        const long long u1 = min(u0 + (long long)kGrab, hi);
+        const long long task = a.slice_lo[q] + (long long)(((unsigned long long)g_hi << 32) | g_lo);
+        if (task >= a.slice_lo[q + 1]) {  // slice drained: help the next one
+            if (a.clock && lane == 0 && visited == 0) atomicMax(&a.clock[q * kClock + 1], ~clock_now());
+            ++visited;
+            q = q + 1 == kSlices ? 0 : q + 1;
+            continue;
+        }
+        if (a.clock && lane == 0 && visited > 0) atomicAdd(&a.clock[q * kClock + 3], 1ull);
+        if (cur < 0 || task < T.task0 || task >= T.task0 + ((long long)T.n_frames * T.upf + T.grab - 1) / T.grab) {
+            if (cur >= 0) wave_hist_flush(L.hist, a.counts + (long long)cur * a.row_bins, bins, p.skip, lane);
+            cur = __builtin_amdgcn_readfirstlane(
+                find_last_le(a.n_traj, task, [&](int i) { return a.traj[i].task0; }));
+            T = uload(a.traj, cur);
+            G = CellGrid{a.sorted + T.af0, a.start + T.start_off, a.frame_flag + T.box0,
+                         a.boxes + T.box0, T.n, T.n_frames, T.ncx, T.ncy, T.ncz, T.k,
+                         T.shift_ok, kSlices, T.cull2, nullptr, 0};
+            sl = stencil_lane(lane, T.kind == KIND_CELLS ? T.k : 1);
+        }
+        const long long u0 = (task - T.task0) * T.grab;
+        const long long u1 = min(u0 + (long long)T.grab, (long long)T.n_frames * T.upf);
         for (long long u = u0; u < u1; ++u) {
-            if (u >= cur_end) {
-                if (cur >= 0) wave_hist_flush(L.hist, a.counts + (long long)cur * a.row_bins, bins, p.skip, lane);
-                cur = __builtin_amdgcn_readfirstlane(
-                    find_last_le(a.n_traj, u, [&](int i) { return a.traj[i].unit0; }));
-                T = uload(a.traj, cur);
-                cur_end = T.unit0 + (long long)T.n_frames * T.upf;
-                G = CellGrid{a.sorted + T.af0, a.start + T.start_off, a.frame_flag + T.box0,
-                             a.boxes + T.box0, T.n, T.n_frames, T.ncx, T.ncy, T.ncz, T.k,
-                             T.shift_ok, kSlices, T.cull2, nullptr};
-                sl = stencil_lane(lane, T.kind == KIND_CELLS ? T.k : 1);
-            }
-            const long long local = u - T.unit0;
-            const int f = (int)(local / T.upf);
-            const int h = (int)(local - (long long)f * T.upf);
+            const int f = (int)(u / T.upf);
+            const int h = (int)(u - (long long)f * T.upf);
             if constexpr (PBC != WMD_PBC_NONE) {
                 if (T.kind == KIND_CELLS) {
                     cell_unit<PBC, false>(G, f, h, sl, L.ring, nullptr, cons);
                     continue;
                 }
             }
-            brute_unit(G, f, h, cons);
+            brute_unit(G, f, h, T.bh, cons);
         }
     }
     if (cur >= 0) wave_hist_flush(L.hist, a.counts + (long long)cur * a.row_bins, bins, p.skip, lane);
+    if (a.clock && lane == 0) atomicMax(&a.clock[home_q * kClock + 2], clock_now());
 }
 
 // ---- autotuner cache ---------------------------------------------------------
-// key: (arch, pbc, bins, r_max bits, atom-count bucket, density bucket) -> k
-// (-1 brute force, 1..3 cells per r_max).  Buckets: quarter steps of log2.
+// key: (arch, pbc, bins, r_max bits, atom-count bucket, density bucket) ->
+// (k: -1 brute force or 1..3 cells per r_max, brute-force home atoms per unit
+// (0 default), units per task (0 default)).  Buckets: quarter steps of log2.
 using TuneKey = std::tuple<std::string, int, int, uint32_t, int, int>;
+struct TuneVal {
+    int k, bh, grab;
+};
 std::mutex g_tune_mu;
-std::map<TuneKey, int> g_tune;
+std::map<TuneKey, TuneVal> g_tune;
+constexpr double kTuneAtomFrames = 4e6;  // work of one timed tuning run (atom-frames)
 
 }  // namespace
 
@@ -339,19 +366,22 @@ struct wmd_batch {
     uint32_t flags = 0;
     BinTables tables;
     bool table = false;
-    std::vector<int> tune_k;   // per trajectory: 0 automatic, -1 brute, 1..3
+    std::vector<int> tune_k;     // per trajectory: 0 automatic, -1 brute, 1..3
+    std::vector<int> tune_bh;    // per trajectory: brute-force home atoms per unit (0 default)
+    std::vector<int> tune_grab;  // per trajectory: units per task (0 default)
     bool use_graph = true;
     // plan (set_boxes)
     bool planned = false;
     std::vector<float> boxes_host;
     std::vector<BatchTraj> traj;
     std::vector<BatchRmsdTraj> rtraj;
-    long long total_af = 0, n_units = 0, scan_blocks = 0, cell_counts = 0, cell_starts = 0;
+    long long total_af = 0, n_units = 0, n_tasks = 0, scan_blocks = 0, cell_counts = 0, cell_starts = 0;
     long long rmsd_units = 0;
     long long slice_lo[kSlices + 1] = {};
     bool any_cells = false;
     int max_ncell = 0;  // largest cell grid of a cell-list trajectory
     // device
+    // d_queue: [kSlices] task counters, then [kSlices][kClock] wall-clock marks
     DevBuf d_counts, d_thr, d_tab2, d_traj, d_rtraj, d_boxes, d_flag, d_count, d_start, d_cell_of,
         d_slot, d_image, d_sorted, d_queue, d_partials;
     // bins beyond the per-wave LDS histograms: slice_bins per pass (0 = one
@@ -406,7 +436,7 @@ void plan_batch(wmd_batch *b) {
     std::vector<FrameBox> fb;
     fb.reserve((size_t)b->total_frames);
     std::vector<double> cost(T, 0.0);
-    long long af = 0, units = 0, scan = 0, cnt = 0, sta = 0, box0 = 0, runits = 0, out0 = 0;
+    long long af = 0, units = 0, tasks = 0, scan = 0, cnt = 0, sta = 0, box0 = 0, runits = 0, out0 = 0;
     b->any_cells = false;
     b->max_ncell = 0;
     for (int t = 0; t < T; ++t) {
@@ -419,6 +449,7 @@ void plan_batch(wmd_batch *b) {
         d.af0 = af;
         d.box0 = (int)box0;
         d.unit0 = units;
+        d.task0 = tasks;
         d.scan0 = scan;
         d.count_off = cnt;
         d.start_off = sta;
@@ -451,11 +482,14 @@ void plan_batch(wmd_batch *b) {
             d.kind = KIND_BRUTE;
             d.k = 0;
             d.ncx = d.ncy = d.ncz = 1;
-            d.upf = n > 1 ? (n + kBruteHome - 1) / kBruteHome : 0;
-            cost[t] = (double)F * d.upf * (kBruteHome * 0.5 * n * 1.5 + 2000.0);
+            d.bh = b->tune_bh[t] > 0 ? b->tune_bh[t] : kBruteHome;
+            d.upf = n > 1 ? (n + d.bh - 1) / d.bh : 0;
+            cost[t] = (double)F * d.upf * (d.bh * 0.5 * n * 1.5 + 2000.0);
         }
         if (F == 0) d.upf = 0;
+        d.grab = b->tune_grab[t] > 0 ? b->tune_grab[t] : kGrab;
         units += (long long)F * d.upf;
+        tasks += ((long long)F * d.upf + d.grab - 1) / d.grab;
         af += (long long)F * n;
         box0 += F;
         // RMSD descriptor: tiles of up to 2048 float4 groups
@@ -474,11 +508,12 @@ void plan_batch(wmd_batch *b) {
     }
     b->total_af = af;
     b->n_units = units;
+    b->n_tasks = tasks;
     b->scan_blocks = scan;
     b->cell_counts = cnt;
     b->cell_starts = sta;
     b->rmsd_units = runits;
-    // slices of equal estimated cost, cut at unit granularity
+    // slices of equal estimated cost, cut at task granularity
     double total = 0.0;
     for (double c : cost) total += c;
     b->slice_lo[0] = 0;
@@ -488,15 +523,16 @@ void plan_batch(wmd_batch *b) {
         for (int q = 1; q < kSlices; ++q) {
             const double target = total * q / kSlices;
             while (t < T && acc + cost[t] < target) acc += cost[t++];
-            long long cut = units;
+            long long cut = tasks;
             if (t < T) {
-                const long long tu = (long long)b->traj[t].n_frames * b->traj[t].upf;
+                const BatchTraj &d = b->traj[t];
+                const long long tt = ((long long)d.n_frames * d.upf + d.grab - 1) / d.grab;
                 const double frac = cost[t] > 0 ? (target - acc) / cost[t] : 0.0;
-                cut = b->traj[t].unit0 + (long long)std::llround(frac * (double)tu);
+                cut = d.task0 + (long long)std::llround(frac * (double)tt);
             }
-            b->slice_lo[q] = std::max(b->slice_lo[q - 1], std::min(cut, units));
+            b->slice_lo[q] = std::max(b->slice_lo[q - 1], std::min(cut, tasks));
         }
-        b->slice_lo[kSlices] = units;
+        b->slice_lo[kSlices] = tasks;
     }
     // device buffers
     const size_t nT = std::max(T, 1);
@@ -515,7 +551,7 @@ void plan_batch(wmd_batch *b) {
         }
         b->d_sorted.reserve(16 * afc);
     }
-    b->d_queue.reserve(8 * kSlices);
+    b->d_queue.reserve(8 * kSlices * (1 + kClock));
     b->d_partials.reserve(8 * 13 * std::max<long long>(runits, 1));
     const size_t tb = sizeof(BatchTraj) * T, rb = sizeof(BatchRmsdTraj) * T, bb = sizeof(FrameBox) * fb.size();
     char *hs = (char *)b->h_boxes.reserve(std::max<size_t>(tb + rb + bb, 1));
@@ -570,6 +606,7 @@ void enqueue_pairs(wmd_batch *b, hipStream_t st, const float *d_store) {
     a.boxes = b->d_boxes.as<FrameBox>();
     a.counts = b->d_counts.as<unsigned long long>();
     a.queue = b->d_queue.as<unsigned long long>();
+    a.clock = a.queue + kSlices;
     for (int q = 0; q <= kSlices; ++q) a.slice_lo[q] = b->slice_lo[q];
     HistParams &p = a.p;
     std::memset(&p, 0, sizeof(p));
@@ -634,7 +671,7 @@ void enqueue_step(wmd_batch *b, hipStream_t st, const float *d_store, float *d_o
         hipLaunchKernelGGL(fill_u32_kernel, dim3(grid_1d(std::max<long long>(b->total_frames, 1))), dim3(kBlock), 0, st,
                            b->d_flag.as<unsigned>(), b->total_frames, 0u);
         hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(kBlock), 0, st, b->d_queue.as<unsigned long long>(),
-                           (long long)kSlices);
+                           (long long)kSlices * (1 + kClock));
         WMD_HIP_CHECK(hipGetLastError());
         if (b->pbc == WMD_PBC_NONE) enqueue_pairs<WMD_PBC_NONE>(b, st, d_store);
         else if (b->pbc == WMD_PBC_ORTHO) enqueue_pairs<WMD_PBC_ORTHO>(b, st, d_store);
@@ -698,8 +735,11 @@ TuneKey tune_key(const wmd_batch *b, int t) {
     return TuneKey(b->ctx->arch, b->pbc, b->bins, rb, nb, db);
 }
 
+// offsets / strides (optional, both or neither): an explicit store layout, as
+// the autotuner's sub-batches use to read members of the caller's store.
 wmd_batch *create_batch(wmd_ctx *ctx, size_t n_traj, const uint32_t *n_atoms, const uint32_t *n_frames,
-                        size_t bins, float r_max, int pbc, uint32_t flags) {
+                        size_t bins, float r_max, int pbc, uint32_t flags,
+                        const long long *offsets = nullptr, const uint32_t *strides = nullptr) {
     WMD_REQUIRE(ctx, WMD_ERR_INVALID, "null context");
     WMD_REQUIRE(n_traj == 0 || (n_atoms && n_frames), WMD_ERR_INVALID, "null size arrays");
     WMD_REQUIRE(n_traj < (1u << 20), WMD_ERR_UNSUPPORTED, "too many trajectories");
@@ -722,13 +762,15 @@ wmd_batch *create_batch(wmd_ctx *ctx, size_t n_traj, const uint32_t *n_atoms, co
         b->flags = flags;
         b->use_graph = !(flags & WMD_BATCH_NO_GRAPH);
         b->tune_k.assign(n_traj, 0);
+        b->tune_bh.assign(n_traj, 0);
+        b->tune_grab.assign(n_traj, 0);
         long long off = 0, frames = 0;
         for (size_t t = 0; t < n_traj; ++t) {
             WMD_REQUIRE(n_atoms[t] < (1u << 27), WMD_ERR_UNSUPPORTED, "trajectory too large");
             const uint32_t s = (uint32_t)round_up(std::max<uint32_t>(n_atoms[t], 1), 4);
-            b->store_off.push_back(off);
-            b->stride.push_back(s);
-            off += 3LL * s * n_frames[t];
+            b->store_off.push_back(offsets ? offsets[t] : off);
+            b->stride.push_back(strides ? strides[t] : s);
+            off = std::max(off, b->store_off.back() + 3LL * b->stride.back() * n_frames[t]);
             frames += n_frames[t];
         }
         b->store_floats = off;
@@ -824,13 +866,22 @@ wmd_status wmd_batch_set_boxes(wmd_batch *b, const float *boxes) {
     WMD_API_END
 }
 
-wmd_status wmd_batch_set_tuning(wmd_batch *b, const int *cell_k) {
+wmd_status wmd_batch_set_tuning(wmd_batch *b, const int *cell_k, const int *home_block, const int *grab) {
     WMD_API_BEGIN
     WMD_REQUIRE(b, WMD_ERR_INVALID, "null batch");
     for (int t = 0; t < b->n_traj; ++t) {
         const int k = cell_k ? cell_k[t] : 0;
+        const int bh = home_block ? home_block[t] : 0;
+        const int g = grab ? grab[t] : 0;
         WMD_REQUIRE(k >= -1 && k <= 3, WMD_ERR_INVALID, "cell_k must be -1, 0 or 1..3");
-        b->tune_k[t] = k;
+        WMD_REQUIRE(bh == 0 || (bh % 64 == 0 && bh >= 64 && bh <= kMaxBruteHome), WMD_ERR_INVALID,
+                    "home_block must be 0 or a multiple of 64 in [64, 512]");
+        WMD_REQUIRE(g >= 0 && g <= kMaxGrab, WMD_ERR_INVALID, "grab must be 0..64");
+    }
+    for (int t = 0; t < b->n_traj; ++t) {
+        b->tune_k[t] = cell_k ? cell_k[t] : 0;
+        b->tune_bh[t] = home_block ? home_block[t] : 0;
+        b->tune_grab[t] = grab ? grab[t] : 0;
     }
     if (b->planned) {
         bind(b->ctx);
@@ -873,13 +924,44 @@ wmd_status wmd_batch_reset(wmd_batch *b) {
     WMD_API_END
 }
 
-wmd_status wmd_batch_plan_info(const wmd_batch *b, int *cell_k, long long *n_units) {
+wmd_status wmd_batch_plan_info(const wmd_batch *b, int *cell_k, int *home_block, int *grab,
+                               long long *n_units) {
     WMD_API_BEGIN
     WMD_REQUIRE(b, WMD_ERR_INVALID, "null batch");
     WMD_REQUIRE(b->planned, WMD_ERR_INVALID, "wmd_batch_set_boxes must be called first");
-    for (int t = 0; t < b->n_traj; ++t)
-        if (cell_k) cell_k[t] = b->traj[t].kind == KIND_CELLS ? b->traj[t].k : 0;
+    for (int t = 0; t < b->n_traj; ++t) {
+        const BatchTraj &d = b->traj[t];
+        if (cell_k) cell_k[t] = d.kind == KIND_CELLS ? d.k : 0;
+        if (home_block) home_block[t] = d.kind == KIND_CELLS ? 0 : d.bh;
+        if (grab) grab[t] = d.grab;
+    }
     if (n_units) *n_units = b->n_units;
+    WMD_API_END
+}
+
+wmd_status wmd_batch_slice_times(wmd_batch *b, double *out) {
+    WMD_API_BEGIN
+    WMD_REQUIRE(b && out, WMD_ERR_INVALID, "null argument");
+    WMD_REQUIRE(b->planned, WMD_ERR_INVALID, "wmd_batch_set_boxes must be called first");
+    bind(b->ctx);
+    WMD_HIP_CHECK(hipDeviceSynchronize());
+    unsigned long long c[kSlices * kClock];
+    WMD_HIP_CHECK(hipMemcpy(c, b->d_queue.as<unsigned long long>() + kSlices, sizeof(c), hipMemcpyDeviceToHost));
+    int khz = 0;
+    WMD_HIP_CHECK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, b->ctx->device));
+    WMD_REQUIRE(khz > 0, WMD_ERR_DEVICE, "device reports no wall clock rate");
+    unsigned long long t0 = ~0ull;
+    for (int q = 0; q < kSlices; ++q)
+        if (c[q * kClock]) t0 = std::min(t0, ~c[q * kClock]);
+    const double per_ms = (double)khz;
+    for (int q = 0; q < kSlices; ++q) {
+        const unsigned long long *r = c + q * kClock;
+        auto rel = [&](unsigned long long t) { return t && t0 != ~0ull ? ((double)t - (double)t0) / per_ms : -1.0; };
+        out[q * 4 + 0] = r[0] ? rel(~r[0]) : -1.0;
+        out[q * 4 + 1] = r[1] ? rel(~r[1]) : -1.0;
+        out[q * 4 + 2] = rel(r[2]);
+        out[q * 4 + 3] = (double)r[3];
+    }
     WMD_API_END
 }
 
@@ -891,7 +973,6 @@ wmd_status wmd_batch_autotune(wmd_batch *b, const float *d_store, int reps, void
     bind(b->ctx);
     hipStream_t st = pick_stream(b->ctx, stream);
     reps = std::max(1, std::min(reps, 20));
-    // classes of trajectories sharing a key; representative = median atom count
     std::map<TuneKey, std::vector<int>> classes;
     for (int t = 0; t < b->n_traj; ++t)
         if (b->n_atoms[t] > 1 && b->n_frames[t] > 0) classes[tune_key(b, t)].push_back(t);
@@ -900,55 +981,93 @@ wmd_status wmd_batch_autotune(wmd_batch *b, const float *d_store, int reps, void
     hipEvent_t e0, e1;
     WMD_HIP_CHECK(hipEventCreate(&e0));
     WMD_HIP_CHECK(hipEventCreate(&e1));
+    wmd_batch *s = nullptr;
     try {
         for (auto &kv : classes) {
-            int choice;
+            TuneVal choice{0, 0, 0};
+            bool cached;
             {
                 std::lock_guard<std::mutex> lk(g_tune_mu);
                 auto it = g_tune.find(kv.first);
-                choice = it == g_tune.end() ? 0 : it->second;
+                cached = it != g_tune.end();
+                if (cached) choice = it->second;
             }
-            if (choice == 0) {
-                std::vector<int> &ts = kv.second;
-                std::sort(ts.begin(), ts.end(), [&](int x, int y) { return b->n_atoms[x] < b->n_atoms[y]; });
-                const int t = ts[ts.size() / 2];
-                const uint32_t na = b->n_atoms[t], nf = b->n_frames[t];
-                double best = 1e300;
-                for (int cand : {-1, 1, 2, 3}) {
-                    wmd_batch *s = create_batch(b->ctx, 1, &na, &nf, b->bins, b->r_max, b->pbc,
-                                                WMD_BATCH_RDF | WMD_BATCH_NO_GRAPH);
-                    try {
-                        s->tune_k[0] = cand;
-                        set_boxes(s, b->boxes_host.empty() ? nullptr : b->boxes_host.data() + 9 * box0[t]);
-                        const bool taken = cand < 0 ? s->traj[0].kind == KIND_BRUTE
-                                                    : (s->traj[0].kind == KIND_CELLS && s->traj[0].k == cand);
-                        if (taken) {
-                            const float *sub = d_store + b->store_off[t];
-                            run_batch(s, sub, nullptr, nullptr, nullptr, st, false);
-                            WMD_HIP_CHECK(hipEventRecord(e0, st));
-                            for (int r = 0; r < reps; ++r) run_batch(s, sub, nullptr, nullptr, nullptr, st, false);
-                            WMD_HIP_CHECK(hipEventRecord(e1, st));
-                            WMD_HIP_CHECK(hipEventSynchronize(e1));
-                            float ms = 0.f;
-                            WMD_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
-                            if (ms < best) {
-                                best = ms;
-                                choice = cand;
-                            }
-                        }
-                    } catch (...) {
-                        delete s;
-                        throw;
+            if (!cached) {
+                // A sub-batch of the class's members (a frame prefix of each,
+                // replicated until one run holds about kTuneAtomFrames
+                // atom-frames, so that it fills the device) reading the
+                // caller's store in place.
+                const std::vector<int> &ts = kv.second;
+                double atoms = 0.0;
+                for (int t : ts) atoms += b->n_atoms[t];
+                const int fmax = (int)std::max(1.0, std::ceil(kTuneAtomFrames / atoms));
+                double af = 0.0;
+                for (int t : ts) af += (double)b->n_atoms[t] * std::min<int>(b->n_frames[t], fmax);
+                const int rep = (int)std::max(1.0, std::min(64.0, std::ceil(kTuneAtomFrames / af)));
+                std::vector<uint32_t> na, nf, sd;
+                std::vector<long long> off;
+                std::vector<float> bx;
+                for (int r = 0; r < rep; ++r)
+                    for (int t : ts) {
+                        const uint32_t f = (uint32_t)std::min<int>(b->n_frames[t], fmax);
+                        na.push_back(b->n_atoms[t]);
+                        nf.push_back(f);
+                        off.push_back(b->store_off[t]);
+                        sd.push_back(b->stride[t]);
+                        if (!b->boxes_host.empty())
+                            bx.insert(bx.end(), b->boxes_host.begin() + 9 * box0[t],
+                                      b->boxes_host.begin() + 9 * (box0[t] + f));
                     }
-                    delete s;
+                s = create_batch(b->ctx, na.size(), na.data(), nf.data(), b->bins, b->r_max, b->pbc,
+                                 WMD_BATCH_RDF | WMD_BATCH_NO_GRAPH, off.data(), sd.data());
+                s->boxes_host = bx;
+                auto time_cfg = [&](const TuneVal &c) -> double {
+                    std::fill(s->tune_k.begin(), s->tune_k.end(), c.k);
+                    std::fill(s->tune_bh.begin(), s->tune_bh.end(), c.bh);
+                    std::fill(s->tune_grab.begin(), s->tune_grab.end(), c.grab);
+                    plan_batch(s);
+                    const BatchTraj &d = s->traj[0];
+                    const bool taken = c.k < 0 ? d.kind == KIND_BRUTE : (d.kind == KIND_CELLS && d.k == c.k);
+                    if (!taken) return 1e300;
+                    run_batch(s, d_store, nullptr, nullptr, nullptr, st, false);
+                    WMD_HIP_CHECK(hipEventRecord(e0, st));
+                    for (int r = 0; r < reps; ++r) run_batch(s, d_store, nullptr, nullptr, nullptr, st, false);
+                    WMD_HIP_CHECK(hipEventRecord(e1, st));
+                    WMD_HIP_CHECK(hipEventSynchronize(e1));
+                    float ms = 0.f;
+                    WMD_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+                    return ms;
+                };
+                // coordinate search: enumeration, then home block (brute
+                // force), then units per task
+                double best = 1e300;
+                choice = TuneVal{-1, 0, 0};
+                for (int cand : {-1, 1, 2, 3}) {
+                    const double ms = time_cfg(TuneVal{cand, 0, 0});
+                    if (ms < best) { best = ms; choice = TuneVal{cand, 0, 0}; }
                 }
-                if (choice == 0) choice = -1;
+                if (choice.k < 0)
+                    for (int bh : {128, 256}) {
+                        const double ms = time_cfg(TuneVal{-1, bh, 0});
+                        if (ms < best) { best = ms; choice.bh = bh; }
+                    }
+                for (int g : {1, 4, 8}) {
+                    const double ms = time_cfg(TuneVal{choice.k, choice.bh, g});
+                    if (ms < best) { best = ms; choice.grab = g; }
+                }
+                delete s;
+                s = nullptr;
                 std::lock_guard<std::mutex> lk(g_tune_mu);
                 g_tune[kv.first] = choice;
             }
-            for (int t : kv.second) b->tune_k[t] = choice;
+            for (int t : kv.second) {
+                b->tune_k[t] = choice.k;
+                b->tune_bh[t] = choice.bh;
+                b->tune_grab[t] = choice.grab;
+            }
         }
     } catch (...) {
+        delete s;
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         throw;
@@ -967,7 +1086,8 @@ wmd_status wmd_autotune_export(char *buf, size_t cap, size_t *len) {
         for (const auto &kv : g_tune)
             os << std::get<0>(kv.first) << ' ' << std::get<1>(kv.first) << ' ' << std::get<2>(kv.first)
                << ' ' << std::get<3>(kv.first) << ' ' << std::get<4>(kv.first) << ' '
-               << std::get<5>(kv.first) << ' ' << kv.second << '\n';
+               << std::get<5>(kv.first) << ' ' << kv.second.k << ' ' << kv.second.bh << ' '
+               << kv.second.grab << '\n';
     }
     const std::string s = os.str();
     if (len) *len = s.size();
@@ -984,17 +1104,26 @@ wmd_status wmd_autotune_import(const char *text) {
     WMD_REQUIRE(text, WMD_ERR_INVALID, "null text");
     std::istringstream is(text);
     std::string line;
-    std::map<TuneKey, int> add;
+    std::map<TuneKey, TuneVal> add;
     while (std::getline(is, line)) {
-        if (line.empty()) continue;
+        if (line.find_first_not_of(" \t\r") == std::string::npos) continue;
         std::istringstream ls(line);
         std::string arch;
         int pbc, bins, nb, db, k;
         uint32_t rb;
         WMD_REQUIRE(bool(ls >> arch >> pbc >> bins >> rb >> nb >> db >> k), WMD_ERR_PARSE,
                     "malformed autotune line: " + line);
+        int bh = 0, g = 0;
+        if (ls >> bh) {  // the seven-field form (k only) keeps the default home block and task size
+            WMD_REQUIRE(bool(ls >> g), WMD_ERR_PARSE, "malformed autotune line: " + line);
+        }
+        std::string rest;
+        WMD_REQUIRE(!(ls >> rest), WMD_ERR_PARSE, "malformed autotune line: " + line);
         WMD_REQUIRE(k == -1 || (k >= 1 && k <= 3), WMD_ERR_PARSE, "autotune choice out of range: " + line);
-        add[TuneKey(arch, pbc, bins, rb, nb, db)] = k;
+        WMD_REQUIRE(bh == 0 || (bh % 64 == 0 && bh >= 64 && bh <= kMaxBruteHome), WMD_ERR_PARSE,
+                    "autotune home block out of range: " + line);
+        WMD_REQUIRE(g >= 0 && g <= kMaxGrab, WMD_ERR_PARSE, "autotune task size out of range: " + line);
+        add[TuneKey(arch, pbc, bins, rb, nb, db)] = TuneVal{k, bh, g};
     }
     std::lock_guard<std::mutex> lk(g_tune_mu);
     for (auto &kv : add) g_tune[kv.first] = kv.second;

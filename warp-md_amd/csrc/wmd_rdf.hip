@@ -887,6 +887,7 @@ struct wmd_rdf {
     std::vector<uint2> tab2;
     int tab_sh = 0, tab_base = 0;
     int cell_k = 0;        // 0 = automatic
+    int grab = 0;          // cell units per queue grab, 0 = automatic
     int last_path = 0;
     int last_nc[3] = {0, 0, 0};
     DevBuf d_counts, d_thr, d_elo, d_ehi, d_tab, d_tab2, d_wts, d_io, d_boxes;
@@ -1059,7 +1060,7 @@ void launch_cells_t(wmd_rdf *h, hipStream_t st, const float4 *sorted, const unsi
     // a larger grab halves the traffic on each XCD's queue counter (which
     // otherwise stalls the waves of a long launch); short launches keep
     // grabs of 2 so that the last grabs stay balanced
-    const int grab = n_units >= 64LL * grid * kCellWaves ? 4 : 2;
+    const int grab = h->grab > 0 ? h->grab : n_units >= 64LL * grid * kCellWaves ? 4 : 2;
     CellGrid G{sorted, start, frame_flag, boxes, n, nf, nc[0], nc[1], nc[2], k, shift_ok, 8, cull2, queue, grab};
     hipLaunchKernelGGL(kern, dim3(grid), dim3(kCellBlock), lds, st, G, p);
     WMD_HIP_CHECK(hipGetLastError());
@@ -1691,13 +1692,13 @@ wmd_status wmd_rdf_last_path(const wmd_rdf *h, int *path, int *nc) {
     WMD_API_END
 }
 
-wmd_status wmd_rdf_set_tuning(wmd_rdf *h, int cell_k, int block_threads) {
+wmd_status wmd_rdf_set_tuning(wmd_rdf *h, int cell_k, int grab) {
     WMD_API_BEGIN
     WMD_REQUIRE(h != nullptr, WMD_ERR_INVALID, "null handle");
     WMD_REQUIRE(cell_k >= 0 && cell_k <= 3, WMD_ERR_INVALID, "cell_k must be 0..3");
-    WMD_REQUIRE(block_threads == 0 || block_threads == kBlock, WMD_ERR_UNSUPPORTED,
-                "only 256-thread workgroups are built");
+    WMD_REQUIRE(grab >= 0 && grab <= 64, WMD_ERR_INVALID, "grab must be 0..64");
     h->cell_k = cell_k;
+    h->grab = grab;
     WMD_API_END
 }
 

@@ -64,7 +64,7 @@ SYMBOLS = (
     "wmd_softrdf_last_path", "wmd_softrdf_set_tuning", "wmd_softrdf_destroy",
     "wmd_batch_create", "wmd_batch_store_layout", "wmd_batch_set_boxes", "wmd_batch_set_tuning",
     "wmd_batch_autotune", "wmd_batch_run_device", "wmd_batch_read_counts", "wmd_batch_reset",
-    "wmd_batch_plan_info", "wmd_batch_destroy", "wmd_autotune_export", "wmd_autotune_import",
+    "wmd_batch_plan_info", "wmd_batch_slice_times", "wmd_batch_destroy", "wmd_autotune_export", "wmd_autotune_import",
     "wmd_pairwise_create", "wmd_pairwise_create_dme", "wmd_pairwise_append",
     "wmd_pairwise_append_boxes", "wmd_pairwise_frames", "wmd_pairwise_finalize",
     "wmd_pairwise_store_stride", "wmd_pairwise_run_device", "wmd_pairwise_destroy",
@@ -166,12 +166,13 @@ def lib():
             "wmd_batch_create": ([P, S, P, P, S, ctypes.c_float, I, ctypes.c_uint32, PP], St),
             "wmd_batch_store_layout": ([P, P, P, ctypes.POINTER(ctypes.c_uint64)], St),
             "wmd_batch_set_boxes": ([P, P], St),
-            "wmd_batch_set_tuning": ([P, P], St),
+            "wmd_batch_set_tuning": ([P, P, P, P], St),
             "wmd_batch_autotune": ([P, P, I, P], St),
             "wmd_batch_run_device": ([P, P, P, P, P, P], St),
             "wmd_batch_read_counts": ([P, P], St),
             "wmd_batch_reset": ([P], St),
-            "wmd_batch_plan_info": ([P, P, ctypes.POINTER(ctypes.c_longlong)], St),
+            "wmd_batch_plan_info": ([P, P, P, P, ctypes.POINTER(ctypes.c_longlong)], St),
+            "wmd_batch_slice_times": ([P, P], St),
             "wmd_batch_destroy": ([P], None),
             "wmd_autotune_export": ([ctypes.c_char_p, S, ctypes.POINTER(S)], St),
             "wmd_autotune_import": ([ctypes.c_char_p], St),

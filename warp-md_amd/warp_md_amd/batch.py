@@ -130,10 +130,15 @@ class UnevenBatch:
         check(lib().wmd_batch_set_boxes(self.handle, ptr(self.boxes)))
 
     # ---- tuning -------------------------------------------------------------
-    def set_tuning(self, cell_k: Optional[Sequence[int]] = None) -> None:
-        """Per-trajectory enumeration: 0 automatic, -1 brute force, 1..3 cells per r_max."""
-        k = None if cell_k is None else np.ascontiguousarray(cell_k, np.int32)
-        check(lib().wmd_batch_set_tuning(self.handle, ptr(k)))
+    def set_tuning(self, cell_k: Optional[Sequence[int]] = None,
+                   home_block: Optional[Sequence[int]] = None,
+                   grab: Optional[Sequence[int]] = None) -> None:
+        """Per-trajectory overrides (None or 0 = default): enumeration (-1 brute
+        force, 1..3 cells per r_max), brute-force home atoms per unit (a
+        multiple of 64 up to 512) and units per queue task (1..64)."""
+        arr = lambda v: None if v is None else np.ascontiguousarray(v, np.int32)
+        k, bh, g = arr(cell_k), arr(home_block), arr(grab)
+        check(lib().wmd_batch_set_tuning(self.handle, ptr(k), ptr(bh), ptr(g)))
 
     def autotune(self, store, reps: int = 2, cache: Optional[str] = None) -> None:
         """Times the candidate enumerations per trajectory class and applies the
@@ -149,8 +154,25 @@ class UnevenBatch:
         """(cell_k per trajectory with 0 = brute force, number of pair units)."""
         k = np.zeros(self.n_traj, np.int32)
         u = ctypes.c_longlong()
-        check(lib().wmd_batch_plan_info(self.handle, ptr(k), ctypes.byref(u)))
+        check(lib().wmd_batch_plan_info(self.handle, ptr(k), None, None, ctypes.byref(u)))
         return k, int(u.value)
+
+    def plan_tuning(self):
+        """Per trajectory: (cell_k with 0 = brute force, brute-force home block
+        (0 for cells), units per task)."""
+        k = np.zeros(self.n_traj, np.int32)
+        bh = np.zeros(self.n_traj, np.int32)
+        g = np.zeros(self.n_traj, np.int32)
+        check(lib().wmd_batch_plan_info(self.handle, ptr(k), ptr(bh), ptr(g), None))
+        return k, bh, g
+
+    def slice_times(self) -> np.ndarray:
+        """[8, 4] per queue slice of the last step's pair kernel: first wave
+        start, slice drained, last wave end (ms from the earliest start) and
+        the slice's tasks taken by waves of other slices."""
+        out = np.zeros((8, 4), np.float64)
+        check(lib().wmd_batch_slice_times(self.handle, ptr(out)))
+        return out
 
     # ---- runs ---------------------------------------------------------------
     def run(self, store, rmsd_out=None, rmsd64_out=None, err_out=None, stream=None):

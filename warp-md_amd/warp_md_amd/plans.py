@@ -74,15 +74,18 @@ class RdfPlan:
         self.last_edge_pairs = 0
         self.last_path = None
         self._cell_k = 0
+        self._grab = 0
 
     # rdf.rs:378-385 common_atom_count
     def _common_count(self) -> int:
         # entries of sel_b (with repeats) whose atom is in sel_a
         return int(np.count_nonzero(np.isin(self.sel_b, self.sel_a)))
 
-    def set_tuning(self, cell_k: int = 0):
-        """Cell-list shape (cells per r_max, 1..3; 0 = automatic)."""
+    def set_tuning(self, cell_k: int = 0, grab: int = 0):
+        """Cell-list shape (cells per r_max, 1..3) and cell units per queue
+        grab (1..64); 0 = automatic."""
         self._cell_k = int(cell_k)
+        self._grab = int(grab)
 
     def run(self, traj: Trajectory, system: System, chunk_frames: Optional[int] = None,
             device: str = "auto", frame_indices: Optional[Sequence[int]] = None):
@@ -119,8 +122,8 @@ class RdfPlan:
                                        self.sel_b.size, self.bins, ctypes.c_float(self.r_max),
                                        self.pbc, flags, ctypes.byref(h)))
         try:
-            if self._cell_k:
-                check(lib().wmd_rdf_set_tuning(h, self._cell_k, 0))
+            if self._cell_k or self._grab:
+                check(lib().wmd_rdf_set_tuning(h, self._cell_k, self._grab))
             tr = select_frames(traj, frame_indices)
             tr.reset()
             frames = 0
