@@ -38,6 +38,14 @@ def log(msg):
 
 
 def cpu_model():
+    """The host CPU's model string (/proc/cpuinfo, else lscpu)."""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.lower().startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
     try:
         out = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
         for line in out.splitlines():
@@ -46,6 +54,21 @@ def cpu_model():
     except Exception:
         pass
     return platform.processor()
+
+
+def host_threads():
+    """Threads for the multi-core CPU lines: the process's CPU share (the
+    OMP_NUM_THREADS the host sets for one GPU's job, within the affinity
+    mask), with nproc and the mask size recorded beside it."""
+    nproc = os.cpu_count() or 1
+    try:
+        mask = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        mask = nproc
+    share = int(os.environ.get("OMP_NUM_THREADS", mask) or mask)
+    threads = max(1, min(share, mask))
+    return threads, {"nproc": nproc, "affinity_cpus": mask, "omp_num_threads": os.environ.get("OMP_NUM_THREADS"),
+                     "threads_used": threads, "cpu": cpu_model()}
 
 
 def traffic_per_launch(kernel, frames):
@@ -62,6 +85,18 @@ def traffic_per_launch(kernel, frames):
         return None
     per_frame = (k["fetch_kib"] * t["fetch_scale"] + k["write_kib"] * t["write_scale"]) * 1024.0 / k["frames"]
     return per_frame * frames
+
+
+def traffic_stamp():
+    """Commit and date of the counter runs behind traffic_per_launch (the
+    counters need rocprofv3 around the process, so they are not re-read in
+    a bench run); None when unrecorded."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "hbm_traffic.json")) as fh:
+            t = json.load(fh)
+        return t.get("measured_at")
+    except (OSError, ValueError):
+        return None
 
 
 def cpu_baseline(n_side=47, box=144.3, n_waters=100000):
@@ -94,14 +129,19 @@ def cpu_baseline(n_side=47, box=144.3, n_waters=100000):
                      f"({evals:.3g} pair-evals, {dt:.1f} s, 1 thread, brute force in the "
                      f"reference loop order)",
            "cpu": cpu_model()}
-    # all-cores line of the same loop (OpenMP over sel_a), labelled as such
-    threads = max(1, min(16, int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))))
+    # multi-core line of the same loop (OpenMP over sel_a) on the process's
+    # CPU share, labelled with the thread count and the host's core counts
+    threads, host = host_threads()
     rows2 = int(min(n, rows * threads))
     t0 = time.perf_counter()
     oracle.rdf_counts(w["coords"], allat[:rows2], allat, 500, 15.0, boxes=w["box"], method="omp",
                       threads=threads)
     dt2 = time.perf_counter() - t0
-    out["all_cores"] = {"value": rows2 * (n - 1) / dt2, "threads": threads}
+    out["host"] = host
+    out["all_cores"] = {"value": rows2 * (n - 1) / dt2, "threads": threads,
+                        "label": f"{threads} threads (the job's CPU share) of {host['nproc']} host CPUs",
+                        "sample": f"sel_a = first {rows2} OW x all {n} OW ({rows2 * (n - 1):.3g} pair-evals, "
+                                  f"{dt2:.1f} s)"}
     # RMSD frames/s of the oracle's FP64 Kabsch (fit on CA, RMSD over all atoms)
     c = workloads.chain(n_frames=100, seed=2)
     t0 = time.perf_counter()
@@ -152,7 +192,7 @@ def small_box_leg(ctx, dev, reps=20):
     t0 = time.perf_counter()
     ref = oracle.rdf_counts(coords, ow, ow, 200, 10.0, boxes=w["box"])
     cpu_s = time.perf_counter() - t0
-    threads = max(1, min(16, int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))))
+    threads, _ = host_threads()
     t0 = time.perf_counter()
     ref_omp = oracle.rdf_counts(coords, ow, ow, 200, 10.0, boxes=w["box"], method="omp", threads=threads)
     omp_s = time.perf_counter() - t0
@@ -321,7 +361,8 @@ def pairwise_bench(ctx, dev, F=5000, reps=3, cpu_frames=600):
 def align_bench(ctx, dev, F=5000, reps=3, cpu_frames=40):
     """SuperposePlan over the 5,000 frames of the 20,000-atom chain, fit on the
     2,000 CA atoms, from device frames [F][n][4]: one workgroup per frame
-    fits in f64 and writes R x + t for every atom (al_fit_kernel; HBM-bound,
+    fits in f64 and writes R x + t for every atom from the frame held in
+    registers and LDS (al_superpose_reg_kernel; HBM-bound,
     algorithmic 16 B read + 12 B written per atom)."""
     import numpy as np
     import torch
@@ -383,9 +424,10 @@ def align_bench(ctx, dev, F=5000, reps=3, cpu_frames=40):
     return {"frames": F, "atoms": n, "fit_atoms": int(ca.size), "ms": wall_ms,
             "frames_per_s": F / (wall_ms / 1e3),
             "kernel_ms": {"align_superpose": app_avg},
-            "roofline": {"kernel": "al_fit_kernel (fit + transform)", "bound": "hbm", "achieved": ach,
+            "roofline": {"kernel": "al_superpose_reg_kernel (fit + transform)", "bound": "hbm", "achieved": ach,
                          "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
-                         "traffic": traffic_per_launch("al_fit_kernel_chain_5000f_20k", F),
+                         "traffic": traffic_per_launch("al_superpose_reg_kernel_chain_5000f_20k", F),
+                         "traffic_measured_at": traffic_stamp(),
                          "algorithmic": f"28 B per atom (16 read, 12 written) x {n * F:.4g} atoms",
                          "avg_launch_ms": app_avg},
             "cpu_baseline": cpu,
@@ -415,10 +457,21 @@ def host_ingest_bench(ctx, frames=96, rmsd_frames=1000, reps=2):
     check(L.wmd_rdf_create(ctx.handle, ptr(ow), ow.size, ptr(ow), ow.size, 500, ctypes.c_float(15.0),
                            1, 0, ctypes.byref(h)))
     check(L.wmd_rdf_accumulate(h, ptr(x4[:8]), N, 8, ptr(boxes[:8])))  # warm-up
+    ctx.profile(True)
     t0 = time.perf_counter()
     for _ in range(reps):
         check(L.wmd_rdf_accumulate(h, ptr(x4), N, frames, ptr(boxes)))
     rdf_s = (time.perf_counter() - t0) / reps
+    # the device-side selection gather (300,000-atom AoS frames -> the
+    # 100,000 OW SoA store the pair kernels read), timed on its own
+    gather_ms, gather_n = ctx.profile_read("rdf_gather")
+    ctx.profile(False)
+    gather_frame_ms = gather_ms / (reps * frames)
+    gather = {"ms_per_frame": gather_frame_ms, "ms_per_1000_frames": gather_frame_ms * 1000.0,
+              "launches": gather_n,
+              "algorithmic_GBps": (16.0 * ow.size * 3 + 12.0 * ow.size) / (gather_frame_ms / 1e3) / 1e9,
+              "note": "reads the 300,000-atom frame's rows (OW every third atom: every line) and writes "
+                      "the OW SoA store; beside the headline, which starts from that store"}
     host_counts = np.zeros(500, np.uint64)
     check(L.wmd_rdf_read(h, ptr(host_counts), None))
     L.wmd_rdf_destroy(h)
@@ -515,7 +568,8 @@ def host_ingest_bench(ctx, frames=96, rmsd_frames=1000, reps=2):
                    "host_GB_per_s": frames * N * 16 / rdf_s / 1e9,
                    "pair_evals_per_s": pair_evals / rdf_s,
                    "rdfplan_run_s": plan_s,
-                   "counts_equal_device_path": bool(rdf_match)},
+                   "counts_equal_device_path": bool(rdf_match),
+                   "device_selection_gather": gather},
            "rmsd": {"frames": rmsd_frames, "atoms_per_frame": nc, "s": rmsd_s,
                     "max_abs_diff_vs_device_path_A": rmsd_diff,
                     "host_GB_per_s": rmsd_frames * nc * 16 / rmsd_s / 1e9,
@@ -524,7 +578,8 @@ def host_ingest_bench(ctx, frames=96, rmsd_frames=1000, reps=2):
                          "host_GB_per_s": rmsd_frames * nc * 28 / sup_s / 1e9},
            "pairwise_append": {"frames": rmsd_frames, "atoms_per_frame": nc, "s": app_s,
                                "host_GB_per_s": rmsd_frames * nc * 16 / app_s / 1e9}}
-    log(f"host ingest: RDF {frames} x {N} atoms {rdf_s * 1e3:.1f} ms (RdfPlan.run {plan_s * 1e3:.1f} ms) "
+    log(f"host ingest: RDF {frames} x {N} atoms {rdf_s * 1e3:.1f} ms (RdfPlan.run {plan_s * 1e3:.1f} ms, "
+        f"device selection gather {gather_frame_ms * 1e3:.1f} us per frame) "
         f"({res['rdf']['host_GB_per_s']:.1f} GB/s of chunks, {res['rdf']['pair_evals_per_s']:.3g} pair-evals/s); "
         f"RMSD {rmsd_frames} x {nc} atoms {rmsd_s * 1e3:.1f} ms ({res['rmsd']['host_GB_per_s']:.1f} GB/s); "
         f"superpose {rmsd_frames} x {nc} atoms {sup_s * 1e3:.1f} ms ({res['superpose']['host_GB_per_s']:.1f} GB/s in + out); "
@@ -1039,6 +1094,10 @@ def main():
         "pairwise_rmsd": pairwise,
         "superpose": superpose,
         "host_ingest": ingest,
+        "selection_gather_300k_atoms": None if ingest is None else {
+            "ms_per_1000_frames": ingest["rdf"]["device_selection_gather"]["ms_per_1000_frames"],
+            "note": "device gather of the 100,000 OW from 300,000-atom frames, not in the timed step "
+                    "(the headline starts from the selected store, the reference's selected-chunk contract)"},
         "sf_bymol": sf_bymol,
         "rdf_1000_waters": small,
         "roofline": {
@@ -1049,6 +1108,7 @@ def main():
             "unit": "TOP/s",
             "frac": rdf_ach / VALU_PEAK_TOPS,
             "traffic": traffic_per_launch("rdf_cell_kernel_100k_waters", F / max(pair_n // K, 1)),
+            "traffic_measured_at": traffic_stamp(),
             "algorithmic": f"{RDF_OPS_PER_PAIR} VALU lane-ops per unordered in-range pair x "
                            f"{p_in:.4g} pairs per step",
             "avg_launch_ms": pair_ms / max(pair_n, 1),
@@ -1062,6 +1122,7 @@ def main():
             "unit": "GB/s",
             "frac": rmsd_ach / HBM_PEAK_GBS,
             "traffic": traffic_per_launch("rmsd_moments_kernel_chain_20k", RF),
+            "traffic_measured_at": traffic_stamp(),
             "avg_launch_ms": mom_ms / max(mom_n, 1),
         },
         "checks": {

@@ -99,7 +99,8 @@ wmd_status wmd_ctx_set_staging_bytes(wmd_ctx *ctx, size_t bytes, size_t *previou
 /* Kernel timing with HIP events recorded around the engine's main kernels on
  * the stream they run on (enable = 1 starts recording and clears old marks).
  * wmd_ctx_profile_read synchronises and returns, for kernel family `name`
- * ("rdf_pair", "rmsd_moments", "rmsd_finalize", "cell_build"), the summed
+ * ("rdf_pair", "rdf_gather" (the host-chunk path's selection gather),
+ * "rmsd_moments", "rmsd_finalize", "cell_build", ...), the summed
  * duration in ms and the number of launches since profiling was enabled. */
 wmd_status wmd_ctx_profile(wmd_ctx *ctx, int enable);
 wmd_status wmd_ctx_profile_read(wmd_ctx *ctx, const char *name, double *total_ms, int *launches);

@@ -11,7 +11,8 @@
 // cross-covariance in f64, and the FP64 QCP solve (wmd_qcp.hpp) supplies R.
 // The transform of SuperposePlan is evaluated in f64 in the reference's
 // operation order (plan_exec.rs:399-405) and cast to f32, by the same
-// workgroup right after the fit, while the frame's lines are still cached.
+// workgroup right after the fit: from registers and LDS holding the whole
+// frame for frames of up to 20,480 atoms, else from a second read of it.
 #include <algorithm>
 #include <cstring>
 #include <memory>
@@ -159,6 +160,120 @@ __global__ __launch_bounds__(kAlBlock) void al_fit_kernel(
     }
 }
 
+// SuperposePlan with the frame held on chip: each of the workgroup's
+// kRegBlock threads loads atoms t + u kRegBlock (u < UR) into registers and
+// atoms t + (UR + v) kRegBlock (v < UL) into LDS (SoA x, y, z planes) before
+// the fit, all as coalesced float4 rows; the fit gathers the selection
+// exactly as al_fit_kernel does (its lines were just fetched, so the gather
+// reads them from L2), and the transform works from registers and LDS: each
+// frame crosses HBM once in and once out.  The LDS part keeps the registers
+// live across the QCP solve within two waves per SIMD (256 VGPRs).  Same
+// moments, rotation and outputs as al_fit_kernel.  One workgroup per CU,
+// frames grid-strided; n_src <= (UR + UL) kRegBlock; dynamic LDS
+// UL kRegBlock x 12 bytes.
+constexpr int kRegBlock = 512;
+constexpr int kRegU = 14;  // register-resident atoms per thread
+// Raw buffer descriptor word 3 (32-bit data format, untyped access); with a
+// zero stride the hardware bounds every access by num_records bytes, the
+// whole offset being in the VGPR operand.
+constexpr int kBufWord3 = 0x00020000;
+typedef float v3f __attribute__((ext_vector_type(3)));
+
+template <int UR, int UL>
+__global__ __launch_bounds__(kRegBlock, 1) void al_superpose_reg_kernel(
+    const float4 *__restrict__ aos, long long n_src, int n_frames, const unsigned *__restrict__ sel,
+    const float *__restrict__ wts, int n, const double *__restrict__ ref_c, double rs0, double rs1,
+    double rs2, double ref_g, double cy0, double cy1, double cy2, int norotate,
+    double *__restrict__ rt64, float *__restrict__ rt32, float *__restrict__ xyz) {
+    __shared__ double red[(kRegBlock / 64) * 14];
+    __shared__ double rt_s[12];
+    extern __shared__ float lxyz[];  // [3][UL * kRegBlock]
+    for (int f = blockIdx.x; f < n_frames; f += gridDim.x) {
+        const float4 *x = aos + (long long)f * n_src;
+        // raw buffer loads bounded by the frame (atoms past n_src read as 0
+        // and are never written out): no branches, all loads in flight at once
+        const __amdgpu_buffer_rsrc_t in =
+            __builtin_amdgcn_make_buffer_rsrc((void *)x, (short)0, (int)(n_src * 16), kBufWord3);
+        float px[UR], py[UR], pz[UR];
+#pragma unroll
+        for (int u = 0; u < UR; ++u) {
+            const v3f p = __builtin_amdgcn_raw_buffer_load_b96(in, (int)(threadIdx.x + u * kRegBlock) * 16, 0, 0);
+            px[u] = p.x;
+            py[u] = p.y;
+            pz[u] = p.z;
+        }
+#pragma unroll
+        for (int v = 0; v < UL; ++v) {
+            const v3f p =
+                __builtin_amdgcn_raw_buffer_load_b96(in, (int)(threadIdx.x + (UR + v) * kRegBlock) * 16, 0, 0);
+            const int l = v * kRegBlock + threadIdx.x;
+            lxyz[l] = p.x;
+            lxyz[UL * kRegBlock + l] = p.y;
+            lxyz[2 * UL * kRegBlock + l] = p.z;
+        }
+        double m[14];
+#pragma unroll
+        for (int c = 0; c < 14; ++c) m[c] = 0.0;
+        for (int k = threadIdx.x; k < n; k += kRegBlock) {
+            const float4 p = x[sel[k]];
+            const double w = wts ? (double)wts[k] : 1.0;
+            const double xv[3] = {(double)p.x, (double)p.y, (double)p.z};
+            const double yr[3] = {ref_c[3 * k], ref_c[3 * k + 1], ref_c[3 * k + 2]};
+#pragma unroll
+            for (int a = 0; a < 3; ++a) m[a] += xv[a] * w;
+            m[3] += w;
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) m[4 + 3 * a + c] += xv[a] * yr[c] * w;
+            m[13] += (xv[0] * xv[0] + xv[1] * xv[1] + xv[2] * xv[2]) * w;
+        }
+        // wave sums, then thread 0 adds the waves in index order (a loop that
+        // is not unrolled: all 8 x 14 partials live at once would not fit
+        // beside the frame)
+        const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+        for (int c = 0; c < 14; ++c)
+            for (int off = 32; off > 0; off >>= 1) m[c] += __shfl_down(m[c], off, 64);
+        if (lane == 0)
+#pragma unroll
+            for (int c = 0; c < 14; ++c) red[wv * 14 + c] = m[c];
+        __syncthreads();
+        if (threadIdx.x == 0) {
+#pragma unroll
+            for (int c = 0; c < 14; ++c) m[c] = red[c];
+#pragma unroll 1
+            for (int w = 1; w < kRegBlock / 64; ++w)
+#pragma unroll
+                for (int c = 0; c < 14; ++c) m[c] += red[w * 14 + c];
+            al_fit_row(m, wts != nullptr, n, rs0, rs1, rs2, ref_g, cy0, cy1, cy2, norotate, f, rt64,
+                       rt32, rt_s);
+        }
+        __syncthreads();
+        double q[12];
+#pragma unroll
+        for (int i = 0; i < 12; ++i) q[i] = rt_s[i];
+        const __amdgpu_buffer_rsrc_t out = __builtin_amdgcn_make_buffer_rsrc(
+            (void *)(xyz + (long long)f * n_src * 3), (short)0, (int)(n_src * 12), kBufWord3);
+#pragma unroll
+        for (int u = 0; u < UR; ++u) {
+            float o[3];
+            al_transform(q, px[u], py[u], pz[u], o);
+            const v3f ov = {o[0], o[1], o[2]};
+            __builtin_amdgcn_raw_buffer_store_b96(ov, out, (int)(threadIdx.x + u * kRegBlock) * 12, 0, 0);
+        }
+#pragma unroll 2
+        for (int v = 0; v < UL; ++v) {
+            const int l = v * kRegBlock + threadIdx.x;
+            float o[3];
+            al_transform(q, lxyz[l], lxyz[UL * kRegBlock + l], lxyz[2 * UL * kRegBlock + l], o);
+            const v3f ov = {o[0], o[1], o[2]};
+            __builtin_amdgcn_raw_buffer_store_b96(ov, out, (int)(threadIdx.x + (UR + v) * kRegBlock) * 12, 0, 0);
+        }
+        __syncthreads();  // red, rt_s and the LDS frame part are reused by the next frame
+    }
+}
+
 }  // namespace
 
 struct wmd_align {
@@ -229,12 +344,38 @@ void al_enqueue(wmd_align *h, const float4 *d_aos, size_t n_atoms, size_t F, dou
     if (F == 0) return;
     {
         ProfScope prof(h->ctx, st, d_xyz ? "align_superpose" : "align_fit");
-        hipLaunchKernelGGL(al_fit_kernel, dim3((unsigned)std::min<size_t>(F, 65536)), dim3(kAlBlock), 0,
-                           st, d_aos, (long long)n_atoms, (int)F, h->d_sel.as<unsigned>(),
-                           h->wts.empty() ? nullptr : h->d_wts.as<float>(), (int)h->sel.size(),
-                           h->d_refc.as<double>(), h->ref_s[0], h->ref_s[1], h->ref_s[2], h->ref_g,
-                           h->cy[0], h->cy[1], h->cy[2],
-                           (int)h->norotate, d_rt64, d_rt32, d_xyz);
+        const float *wts = h->wts.empty() ? nullptr : h->d_wts.as<float>();
+        // frame on chip: the smallest variant that holds n_atoms and whose
+        // LDS part fits the device
+        const size_t per_ul = (size_t)kRegBlock * 12;
+        int ul = -1;
+        if (d_xyz) {
+            for (int c : {0, 8, 16, 26})
+                if (n_atoms <= (size_t)(kRegU + c) * kRegBlock && (size_t)c * per_ul <= h->ctx->lds_per_block) {
+                    ul = c;
+                    break;
+                }
+        }
+        if (ul >= 0) {
+            const unsigned grid = (unsigned)std::min<size_t>(F, (size_t)h->ctx->cu_count);
+            auto launch = [&](auto kern) {
+                hipLaunchKernelGGL(kern, dim3(grid), dim3(kRegBlock), (size_t)ul * per_ul, st, d_aos,
+                                   (long long)n_atoms, (int)F, h->d_sel.as<unsigned>(), wts, (int)h->sel.size(),
+                                   h->d_refc.as<double>(), h->ref_s[0], h->ref_s[1], h->ref_s[2], h->ref_g,
+                                   h->cy[0], h->cy[1], h->cy[2], (int)h->norotate, d_rt64, d_rt32, d_xyz);
+            };
+            if (n_atoms <= 8u * kRegBlock) launch(al_superpose_reg_kernel<8, 0>);
+            else if (ul == 0) launch(al_superpose_reg_kernel<kRegU, 0>);
+            else if (ul == 8) launch(al_superpose_reg_kernel<kRegU, 8>);
+            else if (ul == 16) launch(al_superpose_reg_kernel<kRegU, 16>);
+            else launch(al_superpose_reg_kernel<kRegU, 26>);
+        } else {
+            hipLaunchKernelGGL(al_fit_kernel, dim3((unsigned)std::min<size_t>(F, 65536)), dim3(kAlBlock), 0,
+                               st, d_aos, (long long)n_atoms, (int)F, h->d_sel.as<unsigned>(), wts,
+                               (int)h->sel.size(), h->d_refc.as<double>(), h->ref_s[0], h->ref_s[1],
+                               h->ref_s[2], h->ref_g, h->cy[0], h->cy[1], h->cy[2], (int)h->norotate,
+                               d_rt64, d_rt32, d_xyz);
+        }
         WMD_HIP_CHECK(hipGetLastError());
     }
 }

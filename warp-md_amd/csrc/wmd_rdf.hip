@@ -1287,6 +1287,8 @@ void accumulate_host(wmd_rdf *h, const float *coords, size_t n_src, size_t n_fra
         float *d_soa = (float *)h->p_soa[b].ptr;
         int g;
         grid1d((long long)stride * nf, &g);
+        {
+        ProfScope prof(h->ctx, st, "rdf_gather");
         if (h->grouped) {
             hipLaunchKernelGGL(group_center_kernel, dim3(g), dim3(kBlock), 0, st,
                                (const float4 *)h->p_aos[b].ptr, (long long)n_src,
@@ -1298,6 +1300,7 @@ void accumulate_host(wmd_rdf *h, const float *coords, size_t n_src, size_t n_fra
                                (int)stride, (int)nf, d_soa, (int)h->planar);
         }
         WMD_HIP_CHECK(hipGetLastError());
+        }
         WMD_HIP_CHECK(hipEventRecord(h->p_used[b].get(), st));
         accumulate_device(h, d_soa, stride, nf, boxes ? boxes + 9 * f0 : nullptr, st, true);
         if (h->n_excl > 0) {
