@@ -741,6 +741,23 @@ def counter_sequence(ctx, dev):
     plan.backward_store(soa, stride, n, boxes, torch.ones_like(c))
     torch.cuda.synchronize()
     log("counters: soft RDF forward + backward done")
+    # the plain RDF forward on the same frames (the soft-RDF ratio's
+    # denominator), then its pair evaluations under edge diagnostics
+    sel = np.arange(n, dtype=np.uint32)
+    for flags in (0, 1):
+        h = ctypes.c_void_p()
+        check(L.wmd_rdf_create(ctx.handle, ptr(sel), n, ptr(sel), n, 300, ctypes.c_float(12.0), 1,
+                               flags, ctypes.byref(h)))
+        check(L.wmd_rdf_accumulate_device(h, V(soa), stride, frames, ptr(boxes), None))
+        torch.cuda.synchronize()
+        if flags:
+            ev, cnt = ctypes.c_uint64(), np.zeros(300, np.uint64)
+            check(L.wmd_rdf_pair_evaluations(h, ctypes.byref(ev)))
+            check(L.wmd_rdf_read(h, ptr(cnt), None))
+            log(f"counters: soft workload plain RDF: {ev.value} evaluations, "
+                f"{int(cnt.sum()) // 2} unordered in-range pairs")
+        L.wmd_rdf_destroy(h)
+    log("counters: plain RDF on the soft workload done")
 
 
 def uneven_batch_bench(ctx, dev, reps=3, autotune=True, cpu_sample=True):

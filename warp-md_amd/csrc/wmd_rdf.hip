@@ -107,16 +107,22 @@ __device__ __forceinline__ int bspline2(float u, int n_max, float &t) {
 }
 
 // Deposits one unordered pair (both orders) at squared distance s onto the
-// padded node array (nodes[m + kSoftPadL] holds node m).
-__device__ __forceinline__ void soft_deposit(float s, const SoftParams &sp, double *nodes) {
+// padded node array (nodes[m + kSoftPadL] holds node m) as unsigned fixed
+// point, 2^30 per unit: the three weights (at most 1.5) are formed in f32 and
+// converted once, so integer LDS atomics add them exactly and the node sums
+// of a workgroup do not depend on the order of its deposits.  A node sum
+// overflows only past 1.1e10 deposits in one workgroup.
+constexpr float kSoftFix = 1073741824.0f;  // 2^30
+constexpr double kSoftUnfix = 1.0 / 1073741824.0;
+__device__ __forceinline__ void soft_deposit(float s, const SoftParams &sp, unsigned long long *nodes) {
     const float u = __builtin_amdgcn_sqrtf(s) * sp.inv_h;
     float t;
     const int n0 = bspline2(u, sp.n_nodes, t);
     const float a = 0.5f - t, c = 0.5f + t;
-    double *p = nodes + kSoftPadL + n0;
-    atomicAdd(p - 1, (double)(a * a));              // 2 x (1/2 - t)^2 / 2
-    atomicAdd(p, (double)(1.5f - 2.0f * t * t));    // 2 x (3/4 - t^2)
-    atomicAdd(p + 1, (double)(c * c));              // 2 x (1/2 + t)^2 / 2
+    unsigned long long *p = nodes + kSoftPadL + n0;
+    atomicAdd(p - 1, (unsigned long long)(unsigned)((a * a) * kSoftFix));              // 2 x (1/2 - t)^2 / 2
+    atomicAdd(p, (unsigned long long)(unsigned)((1.5f - 2.0f * t * t) * kSoftFix));    // 2 x (3/4 - t^2)
+    atomicAdd(p + 1, (unsigned long long)(unsigned)((c * c) * kSoftFix));              // 2 x (1/2 + t)^2 / 2
 }
 
 // One unordered pair's dL/dr divided by r, from the padded (A, S) table.
@@ -132,54 +138,51 @@ __device__ __forceinline__ float soft_force_over_r(float s, const SoftParams &sp
     return __builtin_fmaf(t, as.y, as.x) * rinv;
 }
 
-// Minimum-image displacement x_j - x_i of the cell stream's pair (MODE as in
-// cell_chunk; the soft kernels take the rint wrap for MODE 3 as well: the
-// smoothed histogram is this engine's own quantity), returned with its
-// squared length.
+// Minimum-image displacement x_j - x_i of the cell stream's pair, returned
+// with its squared length.  MODE 0: the j atoms sit next to the home cell
+// (image shifts of mode-1 chunks already applied by soft_preshift); 2 and 3:
+// the rint wrap (the smoothed histogram is this engine's own quantity, so the
+// checked wrap of mode 3 is not needed).
 template <int PBC, bool LT, int MODE>
-__device__ __forceinline__ float soft_disp(const float4 &pi, float xj, float yj, float zj, float sx,
-                                           float sy, float sz, const FrameBox &b, float &dx,
-                                           float &dy, float &dz) {
+__device__ __forceinline__ float soft_disp(const float4 &pi, float xj, float yj, float zj,
+                                           const FrameBox &b, float &dx, float &dy, float &dz) {
     dx = xj - pi.x;
     dy = yj - pi.y;
     dz = zj - pi.z;
-    if constexpr (PBC == WMD_PBC_ORTHO) {
-        if constexpr (MODE == 1) {
-            dx -= sx; dy -= sy; dz -= sz;
-        } else if constexpr (MODE >= 2) {
+    if constexpr (MODE >= 2) {
+        if constexpr (PBC == WMD_PBC_ORTHO) {
             dx -= rintf(dx * b.ilx) * b.lx;
             dy -= rintf(dy * b.ily) * b.ly;
             dz -= rintf(dz * b.ilz) * b.lz;
-        }
-    } else if constexpr (MODE != 0) {
-        float f0 = b.r[0] * dx + b.r[3] * dy + b.r[6] * dz;
-        float f1 = b.r[1] * dx + b.r[4] * dy + b.r[7] * dz;
-        float f2 = b.r[2] * dx + b.r[5] * dy + b.r[8] * dz;
-        if constexpr (MODE == 1) {
-            f0 -= sx; f1 -= sy; f2 -= sz;
         } else {
+            float f0 = b.r[0] * dx + b.r[3] * dy + b.r[6] * dz;
+            float f1 = b.r[1] * dx + b.r[4] * dy + b.r[7] * dz;
+            float f2 = b.r[2] * dx + b.r[5] * dy + b.r[8] * dz;
             f0 -= rintf(f0); f1 -= rintf(f1); f2 -= rintf(f2);
+            dx = f0 * b.c[0] + f1 * b.c[3] + f2 * b.c[6];
+            dy = f0 * b.c[1] + f1 * b.c[4] + f2 * b.c[7];
+            dz = f0 * b.c[2] + f1 * b.c[5] + f2 * b.c[8];
         }
-        dx = f0 * b.c[0] + f1 * b.c[3] + f2 * b.c[6];
-        dy = f0 * b.c[1] + f1 * b.c[4] + f2 * b.c[7];
-        dz = f0 * b.c[2] + f1 * b.c[5] + f2 * b.c[8];
     }
     return __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
 }
 
-template <int MODE, int PBC>
-__device__ __forceinline__ void soft_shift(unsigned mj, const FrameBox &b, float &sx, float &sy,
-                                           float &sz) {
-    sx = sy = sz = 0.f;
-    if constexpr (MODE == 1) {
-        sx = (float)((int)(mj & 0xFFu) - 128);
-        sy = (float)((int)((mj >> 8) & 0xFFu) - 128);
-        sz = (float)((int)((mj >> 16) & 0xFFu) - 128);
-        if constexpr (PBC == WMD_PBC_ORTHO) {
-            sx *= b.lx;
-            sy *= b.ly;
-            sz *= b.lz;
-        }
+// Moves a stream atom to its image next to the home cell: x - n . box, with
+// the image shift n + 128 per byte of the meta word (cell_unit).
+template <int PBC>
+__device__ __forceinline__ void soft_preshift(float &xj, float &yj, float &zj, unsigned mj,
+                                              const FrameBox &b) {
+    const float n0 = (float)((int)(mj & 0xFFu) - 128);
+    const float n1 = (float)((int)((mj >> 8) & 0xFFu) - 128);
+    const float n2 = (float)((int)((mj >> 16) & 0xFFu) - 128);
+    if constexpr (PBC == WMD_PBC_ORTHO) {
+        xj -= n0 * b.lx;
+        yj -= n1 * b.ly;
+        zj -= n2 * b.lz;
+    } else {
+        xj -= n0 * b.c[0] + n1 * b.c[3] + n2 * b.c[6];
+        yj -= n0 * b.c[1] + n1 * b.c[4] + n2 * b.c[7];
+        zj -= n0 * b.c[2] + n1 * b.c[5] + n2 * b.c[8];
     }
 }
 
@@ -188,7 +191,7 @@ __device__ __forceinline__ void soft_shift(unsigned mj, const FrameBox &b, float
 template <int PBC, bool LT>
 struct SoftFwdCons {
     SoftParams sp;
-    double *hist;   // LDS, padded
+    unsigned long long *hist;   // LDS, padded, fixed point
     float *pbuf;    // this wave's 128-entry pair buffer
     int ph, pt;     // buffer head / tail (uniform)
     __device__ __forceinline__ void drain(int cnt) {
@@ -196,14 +199,17 @@ struct SoftFwdCons {
         if (lane < cnt) soft_deposit(pbuf[(ph + lane) & 127], sp, hist);
         ph += cnt;
     }
+    static constexpr bool kPreShift = true;
+    __device__ __forceinline__ void preshift(float &xj, float &yj, float &zj, unsigned mj,
+                                             const FrameBox &b) {
+        soft_preshift<PBC>(xj, yj, zj, mj, b);
+    }
     template <int MODE, bool HOME>
     __device__ __forceinline__ void chunk(const float4 *hp, int n_home, float xj, float yj, float zj,
-                                          unsigned mj, int rank, unsigned, int, const FrameBox &b) {
-        float sx, sy, sz;
-        soft_shift<MODE, PBC>(mj, b, sx, sy, sz);
+                                          unsigned, int rank, unsigned, int, const FrameBox &b) {
         auto pair = [&](int i, const float4 &a) {
             float dx, dy, dz;
-            const float s = soft_disp<PBC, LT, MODE>(a, xj, yj, zj, sx, sy, sz, b, dx, dy, dz);
+            const float s = soft_disp<PBC, LT, MODE>(a, xj, yj, zj, b, dx, dy, dz);
             const bool in = s < sp.cut2 && (!HOME || i < rank);
             const unsigned long long bal = __ballot(in);
             if (in) pbuf[(pt + lanes_below(bal)) & 127] = s;
@@ -241,14 +247,18 @@ struct SoftFwdCons {
 //    zero pad, 16 values) are reduced over each 16-lane row by recursive
 //    halving (4 DPP steps; afterwards lane l holds the row sum of value
 //    v(l) = 8 b0 + 4 b1 + 2 b2 + b3 of its lane bits), and each lane adds its
-//    value to a per-row LDS slot of the home atom (64 distinct addresses per
-//    instruction); the 4 row copies are summed once at the end of the unit.
+//    value to its own running sum (one register per 4 home atoms for the
+//    first 32, a per-row LDS slot of the home atom beyond: 64 distinct
+//    addresses per instruction); the 4 row copies are summed once at the end
+//    of the unit.
 // Home atoms past the consumer's MAXH LDS slots add their row sums to the
 // output directly (MAXH = 32 for the usual cells of about 20 atoms, 128 for
 // dense cells; the smaller LDS footprint keeps three workgroups per CU).
 // The output is indexed by cell-sorted position, where a chunk's lanes mostly
 // hold consecutive atoms, so the atomics of one instruction fall on a few
-// cache lines.
+// cache lines.  (Keeping each lane's home-atom sums in registers across the
+// unit instead, reduced once per unit, needs 3 VGPRs per home atom; at 24
+// home atoms the kernel spilled and ran slower at three waves per SIMD.)
 // floats per row copy of the home sums (padded: rows on distinct banks)
 __host__ __device__ constexpr int soft_gi_row(int maxh) { return maxh * 4 + 16; }
 
@@ -278,13 +288,26 @@ __device__ __forceinline__ void row_halve(float (&v)[16], bool hi) {
     }
 }
 
+// Row sums of 16 values per lane (see above): afterwards v[0] of lane l holds
+// the sum over its 16-lane row of value v(l).
+__device__ __forceinline__ void row_reduce16(float (&g)[16]) {
+    const int lane = threadIdx.x & 63;
+    row_halve<16, 0>(g, lane & 1);
+    row_halve<8, 1>(g, lane & 2);
+    row_halve<4, 2>(g, lane & 4);
+    row_halve<2, 3>(g, lane & 8);
+}
+
 template <int PBC, bool LT, int MAXH>
 struct SoftBwdCons {
     static constexpr int kGiRow = soft_gi_row(MAXH);
+    static constexpr bool kPreShift = true;
+    static constexpr int kRegGroups = 8;  // home atoms 0..31: row sums kept in registers
     SoftParams sp;
     const float2 *tab;  // LDS (A, S) table (padded)
     float *gi;          // this wave's [4][kGiRow] row copies of the home-atom sums ([i][4] per row)
     int hs;             // the current unit's first home position
+    float racc[kRegGroups];  // this lane's row sum of value v for home atoms 4q .. 4q + 3
     __device__ __forceinline__ void add_grad(int f, unsigned pos, float gx, float gy, float gz) {
         float *g = sp.grad + (long long)f * 3 * sp.n + pos;
         atomicAdd(g, gx);
@@ -296,82 +319,99 @@ struct SoftBwdCons {
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
-    __device__ __forceinline__ void unit_begin(int, int h0, int) { hs = h0; }
+    __device__ __forceinline__ void preshift(float &xj, float &yj, float &zj, unsigned mj,
+                                             const FrameBox &b) {
+        soft_preshift<PBC>(xj, yj, zj, mj, b);
+    }
+    __device__ __forceinline__ void unit_begin(int, int h0, int) {
+        hs = h0;
+#pragma unroll
+        for (int q = 0; q < kRegGroups; ++q) racc[q] = 0.f;
+    }
     template <int MODE, bool HOME>
     __device__ __forceinline__ void chunk(const float4 *hp, int n_home, float xj, float yj, float zj,
-                                          unsigned mj, int rank, unsigned jpos, int f,
-                                          const FrameBox &b) {
-        float sx, sy, sz;
-        soft_shift<MODE, PBC>(mj, b, sx, sy, sz);
+                                          unsigned, int rank, unsigned jpos, int f, const FrameBox &b) {
         // a full chunk has finite positions in every lane, so a masked force
         // of zero gives zero products; lanes past a partial chunk hold NaN
         // positions and their products are masked instead
         if (__builtin_amdgcn_readfirstlane((int)__any(xj != xj)))
-            pairs<MODE, HOME, false>(hp, n_home, xj, yj, zj, sx, sy, sz, rank, jpos, f, b);
+            pairs<MODE, HOME, false>(hp, n_home, xj, yj, zj, rank, jpos, f, b);
         else
-            pairs<MODE, HOME, true>(hp, n_home, xj, yj, zj, sx, sy, sz, rank, jpos, f, b);
+            pairs<MODE, HOME, true>(hp, n_home, xj, yj, zj, rank, jpos, f, b);
     }
-    // One pair's gradient g (masked lanes: exact zeros); returns it through gx, gy, gz.
+    // Home atoms i0 .. i0 + 3 against the chunk: j's share added to gj, the
+    // returned value is this lane's row sum of value v (the home share).
     template <int MODE, bool HOME, bool FULL>
-    __device__ __forceinline__ void pair_grad(const float4 &a, int i, int n_home, float xj, float yj, float zj,
-                                              float sx, float sy, float sz, int rank, const FrameBox &b,
-                                              float &gx, float &gy, float &gz) {
-        float dx, dy, dz;
-        const float s = soft_disp<PBC, LT, MODE>(a, xj, yj, zj, sx, sy, sz, b, dx, dy, dz);
-        const bool in = i < n_home && s < sp.cut2 && s > 0.f && (!HOME || i < rank);
-        const float fr = soft_force_over_r(s, sp, tab);  // (dL/dr) / r
-        if constexpr (FULL) {
-            const float frm = in ? fr : 0.f;
-            gx = frm * dx;
-            gy = frm * dy;
-            gz = frm * dz;
-        } else {
-            gx = in ? fr * dx : 0.f;
-            gy = in ? fr * dy : 0.f;
-            gz = in ? fr * dz : 0.f;
+    __device__ __forceinline__ float group(const float4 *hp, int i0, int n_home, float xj, float yj,
+                                           float zj, int rank, const FrameBox &b, float &gjx,
+                                           float &gjy, float &gjz) {
+        float g[16];
+#pragma unroll
+        for (int ii = 0; ii < 4; ++ii) {
+            const int i = i0 + ii;
+            const float4 a = uload(hp, min(i, n_home - 1));
+            float dx, dy, dz;
+            const float s = soft_disp<PBC, LT, MODE>(a, xj, yj, zj, b, dx, dy, dz);
+            const bool in = i < n_home && s < sp.cut2 && s > 0.f && (!HOME || i < rank);
+            const float fr = soft_force_over_r(s, sp, tab);  // (dL/dr) / r
+            float gx, gy, gz;
+            if constexpr (FULL) {
+                const float frm = in ? fr : 0.f;
+                gx = frm * dx;
+                gy = frm * dy;
+                gz = frm * dz;
+            } else {
+                gx = in ? fr * dx : 0.f;
+                gy = in ? fr * dy : 0.f;
+                gz = in ? fr * dz : 0.f;
+            }
+            gjx += gx;
+            gjy += gy;
+            gjz += gz;
+            g[4 * ii] = gx;
+            g[4 * ii + 1] = gy;
+            g[4 * ii + 2] = gz;
+            g[4 * ii + 3] = 0.f;
         }
+        row_reduce16(g);
+        return g[0];
     }
     template <int MODE, bool HOME, bool FULL>
     __device__ __forceinline__ void pairs(const float4 *hp, int n_home, float xj, float yj, float zj,
-                                          float sx, float sy, float sz, int rank, unsigned jpos, int f,
-                                          const FrameBox &b) {
+                                          int rank, unsigned jpos, int f, const FrameBox &b) {
         const int lane = threadIdx.x & 63;
         const int row = lane >> 4;
         const int v = ((lane & 1) << 3) | ((lane & 2) << 1) | ((lane & 4) >> 1) | ((lane & 8) >> 3);
-        const bool b0 = lane & 1, b1 = lane & 2, b2 = lane & 4, b3 = lane & 8;
         float gjx = 0.f, gjy = 0.f, gjz = 0.f;
-        for (int i0 = 0; i0 < n_home; i0 += 4) {
-            float g[16];
+        // the first 32 home atoms: this lane's row sums stay in registers
 #pragma unroll
-            for (int ii = 0; ii < 4; ++ii) {
-                const int i = i0 + ii;
-                const float4 a = uload(hp, min(i, n_home - 1));
-                float gx, gy, gz;
-                pair_grad<MODE, HOME, FULL>(a, i, n_home, xj, yj, zj, sx, sy, sz, rank, b, gx, gy, gz);
-                gjx += gx;
-                gjy += gy;
-                gjz += gz;
-                g[4 * ii] = gx;
-                g[4 * ii + 1] = gy;
-                g[4 * ii + 2] = gz;
-                g[4 * ii + 3] = 0.f;
-            }
-            row_halve<16, 0>(g, b0);
-            row_halve<8, 1>(g, b1);
-            row_halve<4, 2>(g, b2);
-            row_halve<2, 3>(g, b3);
+        for (int q = 0; q < kRegGroups; ++q) {
+            if (4 * q >= n_home) break;
+            racc[q] += group<MODE, HOME, FULL>(hp, 4 * q, n_home, xj, yj, zj, rank, b, gjx, gjy, gjz);
+        }
+        // dense cells: LDS row slots up to MAXH, the output beyond
+        for (int i0 = 4 * kRegGroups; i0 < n_home; i0 += 4) {
+            const float r = group<MODE, HOME, FULL>(hp, i0, n_home, xj, yj, zj, rank, b, gjx, gjy, gjz);
             // lane: row sum of value v = 4 ii + component (component 3: the zero pad)
             if (i0 < MAXH) {
-                atomicAdd(&gi[row * kGiRow + i0 * 4 + v], g[0]);
+                atomicAdd(&gi[row * kGiRow + i0 * 4 + v], r);
             } else if ((v & 3) != 3 && i0 + (v >> 2) < n_home) {
                 float *o = sp.grad + (long long)f * 3 * sp.n + (v & 3) * sp.n + hs + i0 + (v >> 2);
-                atomicAdd(o, -g[0]);
+                atomicAdd(o, -r);
             }
         }
         if (jpos != 0xFFFFFFFFu && (gjx != 0.f || gjy != 0.f || gjz != 0.f)) add_grad(f, jpos, gjx, gjy, gjz);
     }
     __device__ __forceinline__ void unit_end(int f, int, int n_home) {
         const int lane = threadIdx.x & 63;
+        const int row = lane >> 4;
+        const int v = ((lane & 1) << 3) | ((lane & 2) << 1) | ((lane & 4) >> 1) | ((lane & 8) >> 3);
+        // register row sums into this lane's own slots (no other lane writes them)
+#pragma unroll
+        for (int q = 0; q < kRegGroups; ++q) {
+            if (4 * q >= n_home) break;
+            gi[row * kGiRow + q * 16 + v] = racc[q];
+        }
         wave_sync();
         for (int i = lane; i < min(n_home, MAXH); i += 64) {
             float t[3];
@@ -385,10 +425,10 @@ struct SoftBwdCons {
             add_grad(f, (unsigned)(hs + i), -t[0], -t[1], -t[2]);
         }
         wave_sync();
-        // clear what the unit used (the pad components included)
+        // clear the slots the atomics use (past the register groups)
         const int used = min(n_home, MAXH) * 4;
         for (int r = 0; r < 4; ++r)
-            for (int e = lane; e < used; e += 64) gi[r * kGiRow + e] = 0.f;
+            for (int e = 16 * kRegGroups + lane; e < used; e += 64) gi[r * kGiRow + e] = 0.f;
         wave_sync();
     }
 };
@@ -419,10 +459,10 @@ template <int PBC, bool LT>
 __global__ __launch_bounds__(kSoftBlock) void soft_cell_fwd_kernel(CellGrid G, SoftParams sp) {
     extern __shared__ __attribute__((aligned(16))) char lds[];
     const int hn = sp.n_nodes + kSoftPadL + kSoftPadR;
-    double *hist = (double *)lds;
+    unsigned long long *hist = (unsigned long long *)lds;
     float *pbufs = (float *)(lds + align16((size_t)hn * 8));
     float4 *rings = (float4 *)((char *)pbufs + kSoftWaves * 128 * 4);
-    for (int i = threadIdx.x; i < hn; i += kSoftBlock) hist[i] = 0.0;
+    for (int i = threadIdx.x; i < hn; i += kSoftBlock) hist[i] = 0ull;
     __syncthreads();
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     SoftFwdCons<PBC, LT> cons{sp, hist, pbufs + wave * 128, 0, 0};
@@ -433,8 +473,8 @@ __global__ __launch_bounds__(kSoftBlock) void soft_cell_fwd_kernel(CellGrid G, S
     if (cons.pt > cons.ph) cons.drain(cons.pt - cons.ph);
     __syncthreads();
     for (int i = threadIdx.x; i < sp.n_nodes; i += kSoftBlock) {
-        const double v = hist[kSoftPadL + i];
-        if (v != 0.0) atomicAdd(&sp.nodes[i], v);
+        const unsigned long long v = hist[kSoftPadL + i];
+        if (v != 0ull) atomicAdd(&sp.nodes[i], (double)v * kSoftUnfix);
     }
 }
 
@@ -448,7 +488,7 @@ __host__ __device__ inline size_t soft_bwd_lds_bytes(int tn, int maxh) {
 }
 
 template <int PBC, bool LT, int MAXH>
-__global__ __launch_bounds__(kSoftBwdBlock) void soft_cell_bwd_kernel(CellGrid G, SoftParams sp) {
+__global__ __launch_bounds__(kSoftBwdBlock) __attribute__((amdgpu_waves_per_eu(6))) void soft_cell_bwd_kernel(CellGrid G, SoftParams sp) {
     constexpr int kGiRow = soft_gi_row(MAXH);
     extern __shared__ __attribute__((aligned(16))) char lds[];
     const int tn = sp.n_nodes + kSoftPadL + kSoftPadR;
@@ -499,8 +539,8 @@ __global__ __launch_bounds__(kBlock) void soft_brute_fwd_kernel(const float *__r
                                                                 SoftParams sp) {
     extern __shared__ __attribute__((aligned(16))) char lds[];
     const int hn = sp.n_nodes + kSoftPadL + kSoftPadR;
-    double *hist = (double *)lds;
-    for (int t = threadIdx.x; t < hn; t += kBlock) hist[t] = 0.0;
+    unsigned long long *hist = (unsigned long long *)lds;
+    for (int t = threadIdx.x; t < hn; t += kBlock) hist[t] = 0ull;
     __syncthreads();
     const int f = blockIdx.y;
     const float *X = soa + (long long)f * 3 * stride;
@@ -516,8 +556,8 @@ __global__ __launch_bounds__(kBlock) void soft_brute_fwd_kernel(const float *__r
     }
     __syncthreads();
     for (int t = threadIdx.x; t < sp.n_nodes; t += kBlock) {
-        const double v = hist[kSoftPadL + t];
-        if (v != 0.0) atomicAdd(&sp.nodes[t], v);
+        const unsigned long long v = hist[kSoftPadL + t];
+        if (v != 0ull) atomicAdd(&sp.nodes[t], (double)v * kSoftUnfix);
     }
 }
 

@@ -832,12 +832,30 @@ struct CellGrid {
     int grab;                   // units per grab of the dynamic schedule (0: kCellGrab)
 };
 
+// A consumer declaring kPreShift = true moves the chunk's j atoms to their
+// images next to the home cell itself (cons.preshift) and takes image-shift
+// chunks (mode 1) in its mode-0 pair loop.
+template <class C, class = void>
+struct cons_preshift {
+    static constexpr bool value = false;
+};
+template <class C>
+struct cons_preshift<C, decltype((void)C::kPreShift)> {
+    static constexpr bool value = C::kPreShift;
+};
+
 // Dispatches one chunk to the consumer's pair loop for its image mode.
 template <class C>
 __device__ __forceinline__ void cell_dispatch(C &cons, int mode, bool home, const float4 *hp,
                                               int n_home, float xj, float yj, float zj,
                                               unsigned mj, int rank, unsigned jpos, int f,
                                               const FrameBox &b) {
+    if constexpr (cons_preshift<C>::value) {
+        if (mode == 1) {
+            cons.preshift(xj, yj, zj, mj, b);
+            mode = 0;
+        }
+    }
     if (home) {
         if (mode == 0) cons.template chunk<0, true>(hp, n_home, xj, yj, zj, mj, rank, jpos, f, b);
         else if (mode == 1) cons.template chunk<1, true>(hp, n_home, xj, yj, zj, mj, rank, jpos, f, b);

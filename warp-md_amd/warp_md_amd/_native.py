@@ -11,7 +11,9 @@ import os
 import threading
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "lib", "libwmd_hip.so")
+# WMD_HIP_LIB names another build of the same library (for example one built
+# with different compile-time options to compare kernels side by side).
+LIB_PATH = os.environ.get("WMD_HIP_LIB") or os.path.join(HERE, "lib", "libwmd_hip.so")
 
 WMD_OK = 0
 WMD_ERR_UNSUPPORTED = 1
