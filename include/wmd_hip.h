@@ -401,8 +401,9 @@ wmd_status wmd_softrdf_forward(wmd_softrdf *h, const float *d_soa, size_t stride
                                size_t n_frames, const float *boxes_host, double *d_counts,
                                void *stream);
 /* Adds dL/dx to d_grad [F][3][stride] (f32, device) given d_dcounts [bins]
- * (f64, device) = dL/dc_k.  The pairs are enumerated again (nothing is kept
- * from the forward).  Asynchronous on `stream`. */
+ * (f64, device) = dL/dc_k.  The pairs are enumerated again (no per-pair term
+ * is kept from the forward; see wmd_softrdf_keep_cells for its cell lists).
+ * Asynchronous on `stream`. */
 wmd_status wmd_softrdf_backward(wmd_softrdf *h, const float *d_soa, size_t stride, size_t n_atoms,
                                 size_t n_frames, const float *boxes_host, const double *d_dcounts,
                                 float *d_grad, void *stream);
@@ -410,6 +411,13 @@ wmd_status wmd_softrdf_backward(wmd_softrdf *h, const float *d_soa, size_t strid
 wmd_status wmd_softrdf_last_path(const wmd_softrdf *h, int *path, int *nc);
 /* cell_k: stencil reach in cells (0 = automatic, 1..3; -1 = brute force). */
 wmd_status wmd_softrdf_set_tuning(wmd_softrdf *h, int cell_k);
+/* keep != 0: a forward whose cell lists cover all its frames keeps them, and
+ * the next backward called with the same d_soa, stride, n_atoms, n_frames and
+ * boxes takes them instead of sorting the frames again; the caller promises
+ * that the store's contents did not change in between (an autograd wrapper
+ * that owns the store).  Default 0.  No counterpart in the reference (the
+ * differentiable path is new). */
+wmd_status wmd_softrdf_keep_cells(wmd_softrdf *h, int keep);
 void wmd_softrdf_destroy(wmd_softrdf *h);
 
 /* ---- uneven batches ---------------------------------------------------------

@@ -317,3 +317,46 @@ def test_run_to_run_spread_is_bounded():
     assert _rel(c1, c0) < 1e-12
     assert _rel(g1, g0) < 1e-5
     print(f"run-to-run: counts {_rel(c1, c0):.2e}, gradients {_rel(g1, g0):.2e} (relative to max)")
+
+
+def test_backward_reuses_the_forward_cells():
+    """A backward on the forward's own store takes its cell lists: the
+    gradient equals a plan that sorts the frames again; a second forward on
+    other positions in between (another store) makes the backward sort again;
+    a changed box does too."""
+    n = 6000
+    L = (n / 0.08) ** (1.0 / 3.0)
+    box = np.array([L, L, L], np.float32)
+    xa = torch.tensor(_beads(n, L, 3, seed=21), device="cuda")
+    xb = torch.tensor(_beads(n, L, 3, seed=22), device="cuda")
+    dldc = torch.tensor(np.random.default_rng(23).normal(size=BINS), device="cuda")
+    keep = _plan(bins=BINS, r_max=RMAX, sigma=SIG, keep_cells=True)
+    fresh = _plan(bins=BINS, r_max=RMAX, sigma=SIG, keep_cells=False)
+
+    def grad(plan, x, b=box):
+        xg = x.clone().requires_grad_(True)
+        plan.counts(xg, b).backward(dldc)
+        return xg.grad.cpu().numpy()
+
+    ga, gb = grad(fresh, xa), grad(fresh, xb)
+    assert _rel(grad(keep, xa), ga) < 1e-5
+    assert keep.last_path[0] == 2
+    # forward a, forward b, then the backward of a (the cells hold b)
+    x1 = xa.clone().requires_grad_(True)
+    x2 = xb.clone().requires_grad_(True)
+    c1 = keep.counts(x1, box)
+    c2 = keep.counts(x2, box)
+    c1.backward(dldc)
+    c2.backward(dldc)
+    assert _rel(x1.grad.cpu().numpy(), ga) < 1e-5
+    assert _rel(x2.grad.cpu().numpy(), gb) < 1e-5
+    # the same store pointer with another box: raw calls on one store
+    stride = (n + 3) // 4 * 4
+    soa = torch.zeros((3, 3, stride), dtype=torch.float32, device="cuda")
+    soa[:, :, :n] = xa.permute(0, 2, 1)
+    b1 = np.repeat(np.diag(box).reshape(1, 9), 3, 0).astype(np.float32)
+    b2 = (b1 * np.float32(1.01)).astype(np.float32)
+    keep.forward_store(soa, stride, n, b1)
+    g_keep = keep.backward_store(soa, stride, n, b2, dldc)
+    g_fresh = fresh.backward_store(soa, stride, n, b2, dldc)
+    assert _rel(g_keep.cpu().numpy(), g_fresh.cpu().numpy()) < 1e-5

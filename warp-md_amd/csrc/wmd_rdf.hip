@@ -243,12 +243,12 @@ struct SoftFwdCons {
 // pair's gradient g = (dL/dr / r) (x_j - x_i) goes to j (+g) and to i (-g):
 //  * j's share stays in the lane's registers for the whole chunk and is
 //    added to the output once;
-//  * i's share is summed over lanes: the 4 home atoms' 3 components (plus a
+//  * i's share is summed over lanes: 5 home atoms' 3 components (plus a
 //    zero pad, 16 values) are reduced over each 16-lane row by recursive
 //    halving (4 DPP steps; afterwards lane l holds the row sum of value
 //    v(l) = 8 b0 + 4 b1 + 2 b2 + b3 of its lane bits), and each lane adds its
-//    value to its own running sum (one register per 4 home atoms for the
-//    first 32, a per-row LDS slot of the home atom beyond: 64 distinct
+//    value to its own running sum (one register per 5 home atoms for the
+//    first 35, a per-row LDS slot of the home atom beyond: 64 distinct
 //    addresses per instruction); the 4 row copies are summed once at the end
 //    of the unit.
 // Home atoms past the consumer's MAXH LDS slots add their row sums to the
@@ -302,12 +302,17 @@ template <int PBC, bool LT, int MAXH>
 struct SoftBwdCons {
     static constexpr int kGiRow = soft_gi_row(MAXH);
     static constexpr bool kPreShift = true;
-    static constexpr int kRegGroups = 8;  // home atoms 0..31: row sums kept in registers
+    static constexpr int kG = 5;          // home atoms per row reduction (15 values + a pad)
+    static constexpr int kRegGroups = 7;  // home atoms 0..34: row sums kept in registers
+    // home atoms with LDS row slots (the register groups' sums land there at
+    // the end of the unit); soft_gi_row(32) = 144 floats hold 35 atoms
+    static constexpr int kSlots = MAXH > kG * kRegGroups ? MAXH : kG * kRegGroups;
+    static_assert(kGiRow >= 4 * kSlots, "row slots");
     SoftParams sp;
     const float2 *tab;  // LDS (A, S) table (padded)
     float *gi;          // this wave's [4][kGiRow] row copies of the home-atom sums ([i][4] per row)
     int hs;             // the current unit's first home position
-    float racc[kRegGroups];  // this lane's row sum of value v for home atoms 4q .. 4q + 3
+    float racc[kRegGroups];  // this lane's row sum of value v for home atoms kG q .. kG q + kG - 1
     __device__ __forceinline__ void add_grad(int f, unsigned pos, float gx, float gy, float gz) {
         float *g = sp.grad + (long long)f * 3 * sp.n + pos;
         atomicAdd(g, gx);
@@ -318,6 +323,11 @@ struct SoftBwdCons {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    // value v = 3 ii + component of a lane after row_reduce16 (v = 15: the pad)
+    __device__ __forceinline__ static int lane_value() {
+        const int lane = threadIdx.x & 63;
+        return ((lane & 1) << 3) | ((lane & 2) << 1) | ((lane & 4) >> 1) | ((lane & 8) >> 3);
     }
     __device__ __forceinline__ void preshift(float &xj, float &yj, float &zj, unsigned mj,
                                              const FrameBox &b) {
@@ -339,15 +349,15 @@ struct SoftBwdCons {
         else
             pairs<MODE, HOME, true>(hp, n_home, xj, yj, zj, rank, jpos, f, b);
     }
-    // Home atoms i0 .. i0 + 3 against the chunk: j's share added to gj, the
-    // returned value is this lane's row sum of value v (the home share).
+    // Home atoms i0 .. i0 + kG - 1 against the chunk: j's share added to gj,
+    // the returned value is this lane's row sum of value v (the home share).
     template <int MODE, bool HOME, bool FULL>
     __device__ __forceinline__ float group(const float4 *hp, int i0, int n_home, float xj, float yj,
                                            float zj, int rank, const FrameBox &b, float &gjx,
                                            float &gjy, float &gjz) {
         float g[16];
 #pragma unroll
-        for (int ii = 0; ii < 4; ++ii) {
+        for (int ii = 0; ii < kG; ++ii) {
             const int i = i0 + ii;
             const float4 a = uload(hp, min(i, n_home - 1));
             float dx, dy, dz;
@@ -368,36 +378,34 @@ struct SoftBwdCons {
             gjx += gx;
             gjy += gy;
             gjz += gz;
-            g[4 * ii] = gx;
-            g[4 * ii + 1] = gy;
-            g[4 * ii + 2] = gz;
-            g[4 * ii + 3] = 0.f;
+            g[3 * ii] = gx;
+            g[3 * ii + 1] = gy;
+            g[3 * ii + 2] = gz;
         }
+        g[15] = 0.f;
         row_reduce16(g);
         return g[0];
     }
     template <int MODE, bool HOME, bool FULL>
     __device__ __forceinline__ void pairs(const float4 *hp, int n_home, float xj, float yj, float zj,
                                           int rank, unsigned jpos, int f, const FrameBox &b) {
-        const int lane = threadIdx.x & 63;
-        const int row = lane >> 4;
-        const int v = ((lane & 1) << 3) | ((lane & 2) << 1) | ((lane & 4) >> 1) | ((lane & 8) >> 3);
+        const int row = (threadIdx.x & 63) >> 4;
+        const int v = lane_value();
+        const int ii = v / 3, c = v - 3 * ii;
         float gjx = 0.f, gjy = 0.f, gjz = 0.f;
-        // the first 32 home atoms: this lane's row sums stay in registers
+        // the first kG kRegGroups home atoms: this lane's row sums stay in registers
 #pragma unroll
         for (int q = 0; q < kRegGroups; ++q) {
-            if (4 * q >= n_home) break;
-            racc[q] += group<MODE, HOME, FULL>(hp, 4 * q, n_home, xj, yj, zj, rank, b, gjx, gjy, gjz);
+            if (kG * q >= n_home) break;
+            racc[q] += group<MODE, HOME, FULL>(hp, kG * q, n_home, xj, yj, zj, rank, b, gjx, gjy, gjz);
         }
         // dense cells: LDS row slots up to MAXH, the output beyond
-        for (int i0 = 4 * kRegGroups; i0 < n_home; i0 += 4) {
+        for (int i0 = kG * kRegGroups; i0 < n_home; i0 += kG) {
             const float r = group<MODE, HOME, FULL>(hp, i0, n_home, xj, yj, zj, rank, b, gjx, gjy, gjz);
-            // lane: row sum of value v = 4 ii + component (component 3: the zero pad)
-            if (i0 < MAXH) {
-                atomicAdd(&gi[row * kGiRow + i0 * 4 + v], r);
-            } else if ((v & 3) != 3 && i0 + (v >> 2) < n_home) {
-                float *o = sp.grad + (long long)f * 3 * sp.n + (v & 3) * sp.n + hs + i0 + (v >> 2);
-                atomicAdd(o, -r);
+            const int i = i0 + ii;
+            if (v < 15 && i < n_home) {
+                if (i < MAXH) atomicAdd(&gi[row * kGiRow + i * 4 + c], r);
+                else atomicAdd(sp.grad + (long long)f * 3 * sp.n + c * sp.n + hs + i, -r);
             }
         }
         if (jpos != 0xFFFFFFFFu && (gjx != 0.f || gjy != 0.f || gjz != 0.f)) add_grad(f, jpos, gjx, gjy, gjz);
@@ -405,22 +413,23 @@ struct SoftBwdCons {
     __device__ __forceinline__ void unit_end(int f, int, int n_home) {
         const int lane = threadIdx.x & 63;
         const int row = lane >> 4;
-        const int v = ((lane & 1) << 3) | ((lane & 2) << 1) | ((lane & 4) >> 1) | ((lane & 8) >> 3);
+        const int v = lane_value();
+        const int ii = v / 3, c = v - 3 * ii;
         // register row sums into this lane's own slots (no other lane writes them)
 #pragma unroll
         for (int q = 0; q < kRegGroups; ++q) {
-            if (4 * q >= n_home) break;
-            gi[row * kGiRow + q * 16 + v] = racc[q];
+            if (kG * q >= n_home) break;
+            if (v < 15) gi[row * kGiRow + (kG * q + ii) * 4 + c] = racc[q];
         }
         wave_sync();
-        for (int i = lane; i < min(n_home, MAXH); i += 64) {
+        for (int i = lane; i < min(n_home, kSlots); i += 64) {
             float t[3];
 #pragma unroll
-            for (int c = 0; c < 3; ++c) {
+            for (int cc = 0; cc < 3; ++cc) {
                 float acc = 0.f;
 #pragma unroll
-                for (int r = 0; r < 4; ++r) acc += gi[r * kGiRow + i * 4 + c];
-                t[c] = acc;
+                for (int r = 0; r < 4; ++r) acc += gi[r * kGiRow + i * 4 + cc];
+                t[cc] = acc;
             }
             add_grad(f, (unsigned)(hs + i), -t[0], -t[1], -t[2]);
         }
@@ -428,24 +437,25 @@ struct SoftBwdCons {
         // clear the slots the atomics use (past the register groups)
         const int used = min(n_home, MAXH) * 4;
         for (int r = 0; r < 4; ++r)
-            for (int e = 16 * kRegGroups + lane; e < used; e += 64) gi[r * kGiRow + e] = 0.f;
+            for (int e = 4 * kG * kRegGroups + lane; e < used; e += 64) gi[r * kGiRow + e] = 0.f;
         wave_sync();
     }
 };
 
-// grad[f][d][order[f][p]] += gs[f][d][p]: cell order back to store slots.
+// grad[f][d][i] += gs[f][d][pos[f][i]]: cell order back to store slots
+// (pos: store slot -> sorted position; writes in store order, reads gathered
+// from the frame's cell-order gradient, which stays in L2).
 __global__ __launch_bounds__(kBlock) void soft_unsort_kernel(const float *__restrict__ gs,
-                                                             const unsigned *__restrict__ order,
+                                                             const unsigned *__restrict__ pos,
                                                              int n, int n_frames, int stride,
                                                              float *__restrict__ grad) {
     const long long total = (long long)n * n_frames;
     for (long long t = (long long)blockIdx.x * kBlock + threadIdx.x; t < total;
          t += (long long)gridDim.x * kBlock) {
         const int f = (int)(t / n);
-        const int p = (int)(t - (long long)f * n);
-        const unsigned slot = order[t];
-        const float *src = gs + (long long)f * 3 * n + p;
-        float *dst = grad + (long long)f * 3 * stride + slot;
+        const int i = (int)(t - (long long)f * n);
+        const float *src = gs + (long long)f * 3 * n + pos[t];
+        float *dst = grad + (long long)f * 3 * stride + i;
         dst[0] += src[0];
         dst[stride] += src[n];
         dst[2 * stride] += src[2 * n];
@@ -615,10 +625,10 @@ __global__ void soft_conv_kernel(const double *__restrict__ nodes, int n_nodes, 
 
 // D_m = sum over |m h - r_k| <= 6 sigma of dL/dc_k w phi(r_k - m h) into the
 // padded node array (zero in the padding), one thread per entry.
-__global__ void soft_node_table_kernel(const double *__restrict__ dldc, int bins, double w, double h,
-                                       double sigma, int n_nodes, double *__restrict__ D) {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= n_nodes + kSoftPadL + kSoftPadR) return;
+// D_m = sum_k dL/dc_k w phi_sigma(r_k - m h) for a padded node e (node
+// e - kSoftPadL; zero outside 0 .. n_nodes - 1).
+__device__ __forceinline__ double soft_node_d(const double *__restrict__ dldc, int bins, double w,
+                                              double h, double sigma, int n_nodes, int e) {
     const int m = e - kSoftPadL;
     double v = 0.0;
     if (m >= 0 && m < n_nodes) {
@@ -632,22 +642,24 @@ __global__ void soft_node_table_kernel(const double *__restrict__ dldc, int bins
             if (fabs(d) <= reach) v += dldc[k] * norm * exp(-0.5 * d * d / (sigma * sigma));
         }
     }
-    D[e] = v;
+    return v;
 }
 
-// (A, S) per padded node from D (entries outside the array count as zero).
-__global__ void soft_slope_kernel(const double *__restrict__ D, int tn, double inv_h,
-                                  float2 *__restrict__ table) {
+// The padded (A, S) table of the backward: A_e = (D_{e+1} - D_{e-1}) / h,
+// S_e = 2 (D_{e-1} - 2 D_e + D_{e+1}) / h, each thread forming the three
+// node values it needs.
+__global__ void soft_table_kernel(const double *__restrict__ dldc, int bins, double w, double h,
+                                  double sigma, int n_nodes, float2 *__restrict__ table) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    const int tn = n_nodes + kSoftPadL + kSoftPadR;
     if (e >= tn) return;
-    const double dm = e > 0 ? D[e - 1] : 0.0;
-    const double dp = e + 1 < tn ? D[e + 1] : 0.0;
-    table[e] = make_float2((float)((dp - dm) * inv_h), (float)(2.0 * (dm - 2.0 * D[e] + dp) * inv_h));
+    const double dm = e > 0 ? soft_node_d(dldc, bins, w, h, sigma, n_nodes, e - 1) : 0.0;
+    const double d0 = soft_node_d(dldc, bins, w, h, sigma, n_nodes, e);
+    const double dp = e + 1 < tn ? soft_node_d(dldc, bins, w, h, sigma, n_nodes, e + 1) : 0.0;
+    const double inv_h = 1.0 / h;
+    table[e] = make_float2((float)((dp - dm) * inv_h), (float)(2.0 * (dm - 2.0 * d0 + dp) * inv_h));
 }
 
-// AoS [F][n_src][4] -> SoA [F][3][stride] through a gather list (kPad -> 0);
-// planar: z = 0, so s = dx^2 + dy^2 (+ 0) is the PlanarXY distance
-// (rdf_distance, rdf.rs:473-478) bit for bit.
 __global__ __launch_bounds__(kBlock) void gather_soa_kernel(const float4 *__restrict__ aos,
                                                             long long n_src, const unsigned *__restrict__ idx,
                                                             int n_store, int stride, int n_frames,
@@ -1760,7 +1772,15 @@ struct wmd_softrdf {
     int last_nc[3] = {0, 0, 0};
     DevBuf d_boxes, d_table, d_nodes, d_gsorted, d_queue;
     HostBuf h_boxes;
+    Event ev_boxes;                    // the last box upload (guards h_boxes)
+    std::vector<FrameBox> boxes_up;    // what d_boxes holds
     CellScratch cells;
+    // cell lists of the last forward, kept for the backward (keep_cells):
+    // valid when they cover every frame of that call
+    bool keep_cells = false, cells_valid = false;
+    const float *cells_soa = nullptr;
+    size_t cells_stride = 0, cells_n = 0, cells_frames = 0;
+    int cells_nc[3] = {0, 0, 0};
 };
 
 namespace {
@@ -1828,19 +1848,25 @@ void soft_run(wmd_softrdf *h, bool backward, const float *d_soa, size_t stride, 
     WMD_REQUIRE(h->pbc == WMD_PBC_NONE || boxes_host, WMD_ERR_INVALID, "boxes required for PBC");
     const int n = (int)n_atoms;
     const BoxSet B = make_boxes(h->pbc, boxes_host, n_frames);
-    const FrameBox *d_boxes = upload_boxes(B, h->h_boxes, h->d_boxes, st);
+    // boxes: uploaded when they differ from what the device copy holds (the
+    // backward usually repeats the forward's), without a stream sync
+    const bool same_boxes = h->d_boxes.ptr && h->boxes_up.size() == B.fb.size() &&
+                            std::memcmp(h->boxes_up.data(), B.fb.data(), B.fb.size() * sizeof(FrameBox)) == 0;
+    const FrameBox *d_boxes = same_boxes ? (const FrameBox *)h->d_boxes.ptr
+                                         : upload_boxes(B, h->h_boxes, h->d_boxes, st, &h->ev_boxes);
+    if (!same_boxes) {
+        h->boxes_up = B.fb;
+        h->cells_valid = false;
+    }
     SoftParams sp = soft_params(h);
     sp.grad = d_grad;
     sp.stride = (int)stride;
     sp.n = n;
     const int tn = sp.n_nodes + kSoftPadL + kSoftPadR;
     if (backward) {
-        double *D = (double *)h->d_nodes.reserve((size_t)tn * 8);
         float2 *tab = (float2 *)h->d_table.reserve((size_t)tn * 8);
-        hipLaunchKernelGGL(soft_node_table_kernel, dim3((tn + 255) / 256), dim3(256), 0, st, d_dc,
-                           h->bins, (double)h->w, soft_h(h), (double)h->sigma, sp.n_nodes, D);
-        hipLaunchKernelGGL(soft_slope_kernel, dim3((tn + 255) / 256), dim3(256), 0, st,
-                           (const double *)D, tn, 1.0 / soft_h(h), tab);
+        hipLaunchKernelGGL(soft_table_kernel, dim3((tn + 255) / 256), dim3(256), 0, st, d_dc,
+                           h->bins, (double)h->w, soft_h(h), (double)h->sigma, sp.n_nodes, tab);
         WMD_HIP_CHECK(hipGetLastError());
         sp.table = tab;
     } else {
@@ -1892,6 +1918,7 @@ void soft_run(wmd_softrdf *h, bool backward, const float *d_soa, size_t stride, 
             WMD_HIP_CHECK(hipGetLastError());
         }
         conv();
+        h->cells_valid = false;
         h->last_path = 1;
         h->last_nc[0] = h->last_nc[1] = h->last_nc[2] = 0;
         return;
@@ -1899,13 +1926,23 @@ void soft_run(wmd_softrdf *h, bool backward, const float *d_soa, size_t stride, 
     const int ncell = pl.nc[0] * pl.nc[1] * pl.nc[2];
     const int shift_ok = (pl.nc[0] >= 2 * pl.k + 3 && pl.nc[1] >= 2 * pl.k + 3 && pl.nc[2] >= 2 * pl.k + 3) ? 1 : 0;
     const float cull2 = cull2_for(soft_reach(h), B);
-    const size_t batch = h->cells.frames_for(n, ncell, n_frames, backward);
-    h->cells.reserve(n, ncell, batch, backward);
+    // with keep_cells the forward also writes the slot map, and a backward
+    // on the same store, sizes and boxes takes the forward's cell lists
+    const bool want_order = backward || h->keep_cells;
+    const size_t batch = h->cells.frames_for(n, ncell, n_frames, want_order);
+    const bool reuse = backward && h->keep_cells && h->cells_valid && h->cells_soa == d_soa &&
+                       h->cells_stride == stride && h->cells_n == n_atoms && h->cells_frames == n_frames &&
+                       h->cells_nc[0] == pl.nc[0] && h->cells_nc[1] == pl.nc[1] && h->cells_nc[2] == pl.nc[2];
+    if (!reuse) {
+        h->cells_valid = false;
+        h->cells.reserve(n, ncell, batch, want_order);
+    }
     for (size_t f0 = 0; f0 < n_frames; f0 += batch) {
         const int nf = (int)std::min(batch, n_frames - f0);
         const FrameBox *bx = d_boxes + f0;
-        build_cells(h->ctx, st, h->pbc, d_soa, (int)stride, n, nf, (long long)f0, bx, pl.nc, nullptr,
-                    h->cells, backward);
+        if (!reuse)
+            build_cells(h->ctx, st, h->pbc, d_soa, (int)stride, n, nf, (long long)f0, bx, pl.nc, nullptr,
+                        h->cells, want_order);
         // dynamic unit schedule: the per-XCD grab counters start at zero
         unsigned long long *queue = (unsigned long long *)h->d_queue.reserve(8 * 8);
         hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(kBlock), 0, st, queue, 8LL);
@@ -1953,6 +1990,15 @@ void soft_run(wmd_softrdf *h, bool backward, const float *d_soa, size_t stride, 
         }
     }
     conv();
+    if (h->keep_cells && batch >= n_frames) {
+        // the cell lists now hold every frame of this call
+        h->cells_valid = true;
+        h->cells_soa = d_soa;
+        h->cells_stride = stride;
+        h->cells_n = n_atoms;
+        h->cells_frames = n_frames;
+        for (int d = 0; d < 3; ++d) h->cells_nc[d] = pl.nc[d];
+    }
     h->last_path = 2;
     h->last_nc[0] = pl.nc[0]; h->last_nc[1] = pl.nc[1]; h->last_nc[2] = pl.nc[2];
 }
@@ -2020,6 +2066,15 @@ wmd_status wmd_softrdf_set_tuning(wmd_softrdf *h, int cell_k) {
     WMD_REQUIRE(h != nullptr, WMD_ERR_INVALID, "null handle");
     WMD_REQUIRE(cell_k >= -1 && cell_k <= 3, WMD_ERR_INVALID, "cell_k must be -1..3");
     h->cell_k = cell_k;
+    h->cells_valid = false;
+    WMD_API_END
+}
+
+wmd_status wmd_softrdf_keep_cells(wmd_softrdf *h, int keep) {
+    WMD_API_BEGIN
+    WMD_REQUIRE(h != nullptr, WMD_ERR_INVALID, "null handle");
+    h->keep_cells = keep != 0;
+    h->cells_valid = false;
     WMD_API_END
 }
 

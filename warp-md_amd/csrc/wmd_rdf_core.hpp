@@ -586,7 +586,7 @@ __device__ __forceinline__ void build_frame(const float *__restrict__ X, int str
             const unsigned w = wts ? wts[i] : 0x0101u;  // mA | mB << 8
             const unsigned meta = (img & 0xFFFFFFu) | ((w & 0xFu) << 24) | (((w >> 8) & 0xFu) << 28);
             sorted[pos] = make_float4(xs[u], ys[u], zs[u], __uint_as_float(meta));
-            if (order) order[pos] = (unsigned)i;
+            if (order) order[i] = pos;  // store slot -> sorted position
         }
     }
 }
@@ -633,7 +633,7 @@ __global__ __launch_bounds__(kBlock) void cell_scatter_kernel(
         v.z = fx[2 * stride + i];
         v.w = __uint_as_float(meta);
         sorted[(long long)f * n + pos] = v;
-        if (order) order[(long long)f * n + pos] = (unsigned)i;  // sorted position -> store slot
+        if (order) order[t] = pos;  // store slot -> sorted position
     }
 }
 
@@ -1309,7 +1309,8 @@ inline BinTables make_bin_tables(int bins, float r_max) {
 
 // Scratch of the cell build.
 struct CellScratch {
-    DevBuf cell_of, cell_slot, image, count, start, flag, sorted, order;
+    DevBuf cell_of, cell_slot, image, count, start, flag, sorted;
+    DevBuf order;  // [F][n] store slot -> sorted position (want_order)
     size_t frames_for(int n, int ncell, size_t n_frames, bool want_order) const {
         const size_t per_frame = (size_t)n * (16 + 12 + (want_order ? 4 : 0)) + (size_t)(2 * ncell + 2) * 4;
         return std::max<size_t>(1, std::min<size_t>(n_frames, (size_t)(768ull << 20) / per_frame));

@@ -63,7 +63,7 @@ SYMBOLS = (
     "wmd_rmsd_run_device_noalloc",
     "wmd_rmsd_backward_device", "wmd_rmsd_destroy", "wmd_qcp_host",
     "wmd_softrdf_create", "wmd_softrdf_forward", "wmd_softrdf_backward",
-    "wmd_softrdf_last_path", "wmd_softrdf_set_tuning", "wmd_softrdf_destroy",
+    "wmd_softrdf_last_path", "wmd_softrdf_set_tuning", "wmd_softrdf_keep_cells", "wmd_softrdf_destroy",
     "wmd_batch_create", "wmd_batch_store_layout", "wmd_batch_set_boxes", "wmd_batch_set_tuning",
     "wmd_batch_autotune", "wmd_batch_run_device", "wmd_batch_read_counts", "wmd_batch_reset",
     "wmd_batch_plan_info", "wmd_batch_slice_times", "wmd_batch_destroy", "wmd_autotune_export", "wmd_autotune_import",
@@ -162,6 +162,7 @@ def lib():
             "wmd_softrdf_backward": ([P, P, S, S, S, P, P, P, P], St),
             "wmd_softrdf_last_path": ([P, ctypes.POINTER(I), P], St),
             "wmd_softrdf_set_tuning": ([P, I], St),
+            "wmd_softrdf_keep_cells": ([P, I], St),
             "wmd_softrdf_destroy": ([P], None),
             "wmd_qcp_host": ([P, ctypes.c_double, ctypes.c_double, ctypes.c_double, P, P, P,
                               ctypes.POINTER(I)], St),

@@ -92,7 +92,8 @@ class SoftRdf:
     """
 
     def __init__(self, bins: int = 300, r_max: float = 12.0, sigma: float = 0.05,
-                 pbc: str = "orthorhombic", device="auto", cell_k: int = 0):
+                 pbc: str = "orthorhombic", device="auto", cell_k: int = 0,
+                 keep_cells: bool = True):
         if pbc not in _PBC:
             raise RuntimeError(f"unsupported: pbc '{pbc}'")
         self.bins = int(bins)
@@ -107,6 +108,10 @@ class SoftRdf:
         self._h = h
         if cell_k:
             _native.check(_native.lib().wmd_softrdf_set_tuning(h, int(cell_k)))
+        # the backward of the forward that just ran on the same store takes its
+        # cell lists (counts() owns its store; forward_store / backward_store
+        # callers keep the store unchanged in between, or pass keep_cells=False)
+        _native.check(_native.lib().wmd_softrdf_keep_cells(h, 1 if keep_cells else 0))
         w = np.float32(self.r_max) / np.float32(self.bins)
         self.bin_width = float(w)
         self.bin_centers = (np.arange(self.bins) + 0.5) * float(w)
