@@ -192,12 +192,12 @@ template <int PBC, bool LT>
 struct SoftFwdCons {
     SoftParams sp;
     unsigned long long *hist;   // LDS, padded, fixed point
-    float *pbuf;    // this wave's 128-entry pair buffer
-    int ph, pt;     // buffer head / tail (uniform)
+    float *pbuf;    // this wave's 128-entry pair buffer, filled from its start
+    int pt;         // entries in the buffer (uniform, below 64 between pairs)
+    // Deposits the first cnt entries (cnt = 64, or the rest at the end).
     __device__ __forceinline__ void drain(int cnt) {
         const int lane = threadIdx.x & 63;
-        if (lane < cnt) soft_deposit(pbuf[(ph + lane) & 127], sp, hist);
-        ph += cnt;
+        if (lane < cnt) soft_deposit(pbuf[lane], sp, hist);
     }
     static constexpr bool kPreShift = true;
     __device__ __forceinline__ void preshift(float &xj, float &yj, float &zj, unsigned mj,
@@ -212,13 +212,24 @@ struct SoftFwdCons {
             const float s = soft_disp<PBC, LT, MODE>(a, xj, yj, zj, b, dx, dy, dz);
             const bool in = s < sp.cut2 && (!HOME || i < rank);
             const unsigned long long bal = __ballot(in);
-            if (in) pbuf[(pt + lanes_below(bal)) & 127] = s;
+            // slot pt + (in-range lanes below): pt stays below 64, so the
+            // slot stays below 128 without a wrap
+            if (in) pbuf[__builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32),
+                                                   __builtin_amdgcn_mbcnt_lo((unsigned)bal, (unsigned)pt))] = s;
             pt += (int)__popcll(bal);
-            if (pt - ph >= 64) {
+            if (pt >= 64) {
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
                 drain(64);
+                // move the entries past the first 64 to the front
+                const int lane = threadIdx.x & 63;
+                const float rest = pbuf[64 + lane];
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                pt -= 64;
+                if (lane < pt) pbuf[lane] = rest;
             }
         };
         // home atoms four at a time (scalar loads)
@@ -246,7 +257,7 @@ struct SoftFwdCons {
 //  * i's share is summed over lanes: 5 home atoms' 3 components (plus a
 //    zero pad, 16 values) are reduced over each 16-lane row by recursive
 //    halving (4 DPP steps; afterwards lane l holds the row sum of value
-//    v(l) = 8 b0 + 4 b1 + 2 b2 + b3 of its lane bits), and each lane adds its
+//    v(l) = 8 b3 + 4 b2 + 2 b0 + b1 of its lane bits), and each lane adds its
 //    value to its own running sum (one register per 5 home atoms for the
 //    first 35, a per-row LDS slot of the home atom beyond: 64 distinct
 //    addresses per instruction); the 4 row copies are summed once at the end
@@ -288,14 +299,44 @@ __device__ __forceinline__ void row_halve(float (&v)[16], bool hi) {
     }
 }
 
-// Row sums of 16 values per lane (see above): afterwards v[0] of lane l holds
-// the sum over its 16-lane row of value v(l).
+// Halving steps over lane bits 3 and 2, where the lanes that keep the lower
+// and the upper half fill whole 4-lane banks: one DPP add per half, each
+// writing only its own banks (bank_mask), instead of two selects and an add.
+// d = lo + lo of lane l ^ 8 in lanes 0-7 of a row, hi + hi of lane l ^ 8 in
+// lanes 8-15 (row_ror:8 reads lane (l - 8) mod 16).  The s_nop covers the
+// VALU-write-to-DPP-read wait of the inputs.
+__device__ __forceinline__ float bank_halve_b3(float lo, float hi) {
+    float d;
+    asm volatile("s_nop 1\n\t"
+                 "v_add_f32_dpp %0, %1, %1 row_ror:8 row_mask:0xf bank_mask:0x3\n\t"
+                 "v_add_f32_dpp %0, %2, %2 row_ror:8 row_mask:0xf bank_mask:0xc"
+                 : "=&v"(d)
+                 : "v"(lo), "v"(hi));
+    return d;
+}
+// d = lo + lo of lane l + 4 in banks 0 and 2 (row_ror:12), hi + hi of lane
+// l - 4 in banks 1 and 3 (row_ror:4).
+__device__ __forceinline__ float bank_halve_b2(float lo, float hi) {
+    float d;
+    asm volatile("s_nop 1\n\t"
+                 "v_add_f32_dpp %0, %1, %1 row_ror:12 row_mask:0xf bank_mask:0x5\n\t"
+                 "v_add_f32_dpp %0, %2, %2 row_ror:4 row_mask:0xf bank_mask:0xa"
+                 : "=&v"(d)
+                 : "v"(lo), "v"(hi));
+    return d;
+}
+
+// Row sums of 16 values per lane: afterwards v[0] of lane l holds the sum
+// over its 16-lane row of value v(l) = 8 b3 + 4 b2 + 2 b0 + b1 of its lane
+// bits (bits 3 and 2 by bank-masked DPP adds, then bits 0 and 1).
 __device__ __forceinline__ void row_reduce16(float (&g)[16]) {
     const int lane = threadIdx.x & 63;
-    row_halve<16, 0>(g, lane & 1);
-    row_halve<8, 1>(g, lane & 2);
-    row_halve<4, 2>(g, lane & 4);
-    row_halve<2, 3>(g, lane & 8);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) g[k] = bank_halve_b3(g[k], g[k + 8]);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) g[k] = bank_halve_b2(g[k], g[k + 4]);
+    row_halve<4, 0>(g, lane & 1);
+    row_halve<2, 1>(g, lane & 2);
 }
 
 template <int PBC, bool LT, int MAXH>
@@ -313,6 +354,13 @@ struct SoftBwdCons {
     float *gi;          // this wave's [4][kGiRow] row copies of the home-atom sums ([i][4] per row)
     int hs;             // the current unit's first home position
     float racc[kRegGroups];  // this lane's row sum of value v for home atoms kG q .. kG q + kG - 1
+    // a full chunk of unshifted, non-home j atoms waiting for a second one
+    // (two j atoms per lane halve the row reductions per pair)
+    bool held;
+    float hx, hy, hz;
+    unsigned hpos;
+    const float4 *hhp;     // the unit's home atoms
+    const FrameBox *hbox;  // the unit's box (alive until unit_end)
     __device__ __forceinline__ void add_grad(int f, unsigned pos, float gx, float gy, float gz) {
         float *g = sp.grad + (long long)f * 3 * sp.n + pos;
         atomicAdd(g, gx);
@@ -327,7 +375,7 @@ struct SoftBwdCons {
     // value v = 3 ii + component of a lane after row_reduce16 (v = 15: the pad)
     __device__ __forceinline__ static int lane_value() {
         const int lane = threadIdx.x & 63;
-        return ((lane & 1) << 3) | ((lane & 2) << 1) | ((lane & 4) >> 1) | ((lane & 8) >> 3);
+        return (lane & 12) | ((lane & 1) << 1) | ((lane >> 1) & 1);
     }
     __device__ __forceinline__ void preshift(float &xj, float &yj, float &zj, unsigned mj,
                                              const FrameBox &b) {
@@ -335,6 +383,7 @@ struct SoftBwdCons {
     }
     __device__ __forceinline__ void unit_begin(int, int h0, int) {
         hs = h0;
+        held = false;
 #pragma unroll
         for (int q = 0; q < kRegGroups; ++q) racc[q] = 0.f;
     }
@@ -344,10 +393,86 @@ struct SoftBwdCons {
         // a full chunk has finite positions in every lane, so a masked force
         // of zero gives zero products; lanes past a partial chunk hold NaN
         // positions and their products are masked instead
-        if (__builtin_amdgcn_readfirstlane((int)__any(xj != xj)))
-            pairs<MODE, HOME, false>(hp, n_home, xj, yj, zj, rank, jpos, f, b);
-        else
-            pairs<MODE, HOME, true>(hp, n_home, xj, yj, zj, rank, jpos, f, b);
+        const bool full = !__builtin_amdgcn_readfirstlane((int)__any(xj != xj));
+        if constexpr (MODE == 0 && !HOME) {
+            if (full) {
+                if (held) {
+                    pairs2(hp, n_home, hx, hy, hz, hpos, xj, yj, zj, jpos, f, b);
+                    held = false;
+                } else {
+                    hx = xj; hy = yj; hz = zj; hpos = jpos;
+                    hhp = hp;
+                    hbox = &b;
+                    held = true;
+                }
+                return;
+            }
+        }
+        flush(hp, n_home, f, b);
+        if (full) pairs<MODE, HOME, true>(hp, n_home, xj, yj, zj, rank, jpos, f, b);
+        else pairs<MODE, HOME, false>(hp, n_home, xj, yj, zj, rank, jpos, f, b);
+    }
+    // the held chunk alone
+    __device__ __forceinline__ void flush(const float4 *hp, int n_home, int f, const FrameBox &b) {
+        if (held) {
+            pairs<0, false, true>(hp, n_home, hx, hy, hz, 0x7FFFFFFF, hpos, f, b);
+            held = false;
+        }
+    }
+    // Two full chunks of unshifted, non-home j atoms (one each per lane): the
+    // row reductions take both lanes' shares at once.
+    __device__ __forceinline__ void pairs2(const float4 *hp, int n_home, float xa, float ya, float za,
+                                           unsigned pa, float xb, float yb, float zb, unsigned pb,
+                                           int f, const FrameBox &b) {
+        float gax = 0.f, gay = 0.f, gaz = 0.f, gbx = 0.f, gby = 0.f, gbz = 0.f;
+#pragma unroll
+        for (int q = 0; q < kRegGroups; ++q) {
+            if (kG * q >= n_home) break;
+            racc[q] += group2(hp, kG * q, n_home, xa, ya, za, xb, yb, zb, b, gax, gay, gaz, gbx, gby, gbz);
+        }
+        const int row = (threadIdx.x & 63) >> 4;
+        const int v = lane_value();
+        const int ii = v / 3, c = v - 3 * ii;
+        for (int i0 = kG * kRegGroups; i0 < n_home; i0 += kG) {
+            const float r = group2(hp, i0, n_home, xa, ya, za, xb, yb, zb, b, gax, gay, gaz, gbx, gby, gbz);
+            const int i = i0 + ii;
+            if (v < 15 && i < n_home) {
+                if (i < MAXH) atomicAdd(&gi[row * kGiRow + i * 4 + c], r);
+                else atomicAdd(sp.grad + (long long)f * 3 * sp.n + c * sp.n + hs + i, -r);
+            }
+        }
+        if (gax != 0.f || gay != 0.f || gaz != 0.f) add_grad(f, pa, gax, gay, gaz);
+        if (gbx != 0.f || gby != 0.f || gbz != 0.f) add_grad(f, pb, gbx, gby, gbz);
+    }
+    __device__ __forceinline__ float group2(const float4 *hp, int i0, int n_home, float xa, float ya,
+                                            float za, float xb, float yb, float zb, const FrameBox &b,
+                                            float &gax, float &gay, float &gaz, float &gbx, float &gby,
+                                            float &gbz) {
+        float g[16];
+#pragma unroll
+        for (int ii = 0; ii < kG; ++ii) {
+            const int i = i0 + ii;
+            const float4 a = uload(hp, min(i, n_home - 1));
+            float dx, dy, dz, ex, ey, ez;
+            const float s = soft_disp<PBC, LT, 0>(a, xa, ya, za, b, dx, dy, dz);
+            const float t = soft_disp<PBC, LT, 0>(a, xb, yb, zb, b, ex, ey, ez);
+            const bool in_s = i < n_home && s < sp.cut2 && s > 0.f;
+            const bool in_t = i < n_home && t < sp.cut2 && t > 0.f;
+            const float fs0 = soft_force_over_r(s, sp, tab);  // both evaluated, then masked
+            const float ft0 = soft_force_over_r(t, sp, tab);
+            const float fs = in_s ? fs0 : 0.f;
+            const float ft = in_t ? ft0 : 0.f;
+            const float g0x = fs * dx, g0y = fs * dy, g0z = fs * dz;
+            const float g1x = ft * ex, g1y = ft * ey, g1z = ft * ez;
+            gax += g0x; gay += g0y; gaz += g0z;
+            gbx += g1x; gby += g1y; gbz += g1z;
+            g[3 * ii] = g0x + g1x;
+            g[3 * ii + 1] = g0y + g1y;
+            g[3 * ii + 2] = g0z + g1z;
+        }
+        g[15] = 0.f;
+        row_reduce16(g);
+        return g[0];
     }
     // Home atoms i0 .. i0 + kG - 1 against the chunk: j's share added to gj,
     // the returned value is this lane's row sum of value v (the home share).
@@ -411,6 +536,7 @@ struct SoftBwdCons {
         if (jpos != 0xFFFFFFFFu && (gjx != 0.f || gjy != 0.f || gjz != 0.f)) add_grad(f, jpos, gjx, gjy, gjz);
     }
     __device__ __forceinline__ void unit_end(int f, int, int n_home) {
+        if (held) flush(hhp, n_home, f, *hbox);
         const int lane = threadIdx.x & 63;
         const int row = lane >> 4;
         const int v = lane_value();
@@ -475,12 +601,12 @@ __global__ __launch_bounds__(kSoftBlock) void soft_cell_fwd_kernel(CellGrid G, S
     for (int i = threadIdx.x; i < hn; i += kSoftBlock) hist[i] = 0ull;
     __syncthreads();
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    SoftFwdCons<PBC, LT> cons{sp, hist, pbufs + wave * 128, 0, 0};
+    SoftFwdCons<PBC, LT> cons{sp, hist, pbufs + wave * 128, 0};
     cell_waves<PBC, false>(G, rings + wave * kRing, nullptr, wave, cons);
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    if (cons.pt > cons.ph) cons.drain(cons.pt - cons.ph);
+    if (cons.pt > 0) cons.drain(cons.pt);
     __syncthreads();
     for (int i = threadIdx.x; i < sp.n_nodes; i += kSoftBlock) {
         const unsigned long long v = hist[kSoftPadL + i];
@@ -1206,14 +1332,14 @@ void accumulate_f64(wmd_rdf *h, const float *d_soa, size_t stride, size_t n_fram
 
 // Core: accumulate frames of an IO-selection SoA store already on the device.
 void accumulate_device(wmd_rdf *h, const float *d_soa, size_t stride, size_t n_frames,
-                       const float *boxes_host, hipStream_t st, bool pipelined = false) {
+                       const float *boxes_host, hipStream_t st) {
     const int n = (int)h->io.size();
     if (n_frames == 0 || h->bins == 0) return;
     WMD_REQUIRE(stride >= (size_t)n && stride % 4 == 0, WMD_ERR_INVALID,
                 "stride must be a multiple of 4 and >= the IO selection size");
     WMD_REQUIRE(h->pbc == WMD_PBC_NONE || boxes_host, WMD_ERR_INVALID, "boxes required for PBC");
     const BoxSet B = make_boxes(h->pbc, boxes_host, n_frames);
-    const FrameBox *d_boxes = upload_boxes(B, h->h_boxes, h->d_boxes, st, pipelined ? &h->ev_boxes : nullptr);
+    const FrameBox *d_boxes = upload_boxes(B, h->h_boxes, h->d_boxes, st, &h->ev_boxes);
 
     if (h->f64) {
         accumulate_f64(h, d_soa, stride, n_frames, d_boxes, st);
@@ -1354,7 +1480,7 @@ void accumulate_host(wmd_rdf *h, const float *coords, size_t n_src, size_t n_fra
         WMD_HIP_CHECK(hipGetLastError());
         }
         WMD_HIP_CHECK(hipEventRecord(h->p_used[b].get(), st));
-        accumulate_device(h, d_soa, stride, nf, boxes ? boxes + 9 * f0 : nullptr, st, true);
+        accumulate_device(h, d_soa, stride, nf, boxes ? boxes + 9 * f0 : nullptr, st);
         if (h->n_excl > 0) {
             grid1d(h->n_excl * (long long)nf, &g);
             auto kern = h->pbc == WMD_PBC_ORTHO ? rdf_excl_kernel<WMD_PBC_ORTHO> : rdf_excl_kernel<WMD_PBC_NONE>;
