@@ -241,12 +241,14 @@ def soft_rdf_ratio(ctx, dev, n=20000, frames=64, reps=10):
         check(L.wmd_rdf_reset(h))
         check(L.wmd_rdf_accumulate_device(h, V(soa), stride, frames, ptr(boxes), None))
 
+    two_over_norm, neg_target = 2.0 / norm, -target
+
     def soft_step():
-        # the loss stays on the device (read after the timed steps)
+        # L = sum (c / norm - g_t)^2, dL/dc = 2 (c / norm - g_t) / norm; the
+        # loss stays on the device (read after the timed steps)
         c = plan.forward_store(soa, stride, n, boxes)
-        g = c / norm
-        dldc = 2.0 * (g - target) / norm
-        return plan.backward_store(soa, stride, n, boxes, dldc), ((g - target) ** 2).sum()
+        d = torch.addcdiv(neg_target, c, norm)
+        return plan.backward_store(soa, stride, n, boxes, d * two_over_norm), torch.dot(d, d)
 
     def rmsd_step():
         xr = x.clone().requires_grad_(True)

@@ -463,12 +463,13 @@ struct SoftBwdCons {
             const float fs = in_s ? fs0 : 0.f;
             const float ft = in_t ? ft0 : 0.f;
             const float g0x = fs * dx, g0y = fs * dy, g0z = fs * dz;
-            const float g1x = ft * ex, g1y = ft * ey, g1z = ft * ez;
             gax += g0x; gay += g0y; gaz += g0z;
-            gbx += g1x; gby += g1y; gbz += g1z;
-            g[3 * ii] = g0x + g1x;
-            g[3 * ii + 1] = g0y + g1y;
-            g[3 * ii + 2] = g0z + g1z;
+            gbx = __builtin_fmaf(ft, ex, gbx);
+            gby = __builtin_fmaf(ft, ey, gby);
+            gbz = __builtin_fmaf(ft, ez, gbz);
+            g[3 * ii] = __builtin_fmaf(ft, ex, g0x);
+            g[3 * ii + 1] = __builtin_fmaf(ft, ey, g0y);
+            g[3 * ii + 2] = __builtin_fmaf(ft, ez, g0z);
         }
         g[15] = 0.f;
         row_reduce16(g);
@@ -732,41 +733,36 @@ __global__ __launch_bounds__(kBlock) void soft_brute_bwd_kernel(const float *__r
 }
 
 // c_k += sum over |m h - r_k| <= 6 sigma of H_m w phi(r_k - m h), one thread per bin.
-__global__ void soft_conv_kernel(const double *__restrict__ nodes, int n_nodes, int bins, double w,
-                                 double h, double sigma, double *__restrict__ counts) {
+// The bin-node weights depend only on j = kSoftNodesPerBin k + kSoftNodesPerBin / 2 - m
+// (r_k - m h = j h): wt[J + j] = w phi_sigma(j h) for |j| <= J (6 sigma / h),
+// tabulated once per plan.
+constexpr int kSoftMid = kSoftNodesPerBin / 2;
+
+// c_k += sum_m H_m wt[J + j(k, m)].
+__global__ void soft_conv_kernel(const double *__restrict__ nodes, int n_nodes, int bins,
+                                 const double *__restrict__ wt, int J, double *__restrict__ counts) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= bins) return;
-    const double rk = (k + 0.5) * w;
-    const double reach = 6.0 * sigma;
-    const int m0 = max(0, (int)ceil((rk - reach) / h));
-    const int m1 = min(n_nodes - 1, (int)floor((rk + reach) / h));
-    const double norm = w / (sigma * 2.5066282746310002);
+    const int c0 = kSoftNodesPerBin * k + kSoftMid;
+    const int m0 = max(0, c0 - J), m1 = min(n_nodes - 1, c0 + J);
     double c = 0.0;
-    for (int m = m0; m <= m1; ++m) {
-        const double d = rk - m * h;
-        c += nodes[m] * norm * exp(-0.5 * d * d / (sigma * sigma));
-    }
+    for (int m = m0; m <= m1; ++m) c += nodes[m] * wt[J + c0 - m];
     counts[k] += c;
 }
 
-// D_m = sum over |m h - r_k| <= 6 sigma of dL/dc_k w phi(r_k - m h) into the
-// padded node array (zero in the padding), one thread per entry.
-// D_m = sum_k dL/dc_k w phi_sigma(r_k - m h) for a padded node e (node
-// e - kSoftPadL; zero outside 0 .. n_nodes - 1).
-__device__ __forceinline__ double soft_node_d(const double *__restrict__ dldc, int bins, double w,
-                                              double h, double sigma, int n_nodes, int e) {
+// D_m = sum_k dL/dc_k wt[J + j(k, m)] for a padded node e (node e - kSoftPadL;
+// zero outside 0 .. n_nodes - 1).
+__device__ __forceinline__ double soft_node_d(const double *__restrict__ dldc, int bins,
+                                              const double *__restrict__ wt, int J, int n_nodes,
+                                              int e) {
     const int m = e - kSoftPadL;
     double v = 0.0;
     if (m >= 0 && m < n_nodes) {
-        const double rm = m * h;
-        const double reach = 6.0 * sigma;
-        const int k0 = max(0, (int)ceil((rm - reach) / w - 0.5));
-        const int k1 = min(bins - 1, (int)floor((rm + reach) / w - 0.5));
-        const double norm = w / (sigma * 2.5066282746310002);
-        for (int k = k0; k <= k1; ++k) {
-            const double d = (k + 0.5) * w - rm;
-            if (fabs(d) <= reach) v += dldc[k] * norm * exp(-0.5 * d * d / (sigma * sigma));
-        }
+        // k with |kSoftNodesPerBin k + kSoftMid - m| <= J (floor divisions)
+        auto fdiv = [](int a) { return a >= 0 ? a / kSoftNodesPerBin : -((-a + kSoftNodesPerBin - 1) / kSoftNodesPerBin); };
+        const int k0 = max(0, -fdiv(-(m - kSoftMid - J)));
+        const int k1 = min(bins - 1, fdiv(m - kSoftMid + J));
+        for (int k = k0; k <= k1; ++k) v += dldc[k] * wt[J + kSoftNodesPerBin * k + kSoftMid - m];
     }
     return v;
 }
@@ -774,15 +770,15 @@ __device__ __forceinline__ double soft_node_d(const double *__restrict__ dldc, i
 // The padded (A, S) table of the backward: A_e = (D_{e+1} - D_{e-1}) / h,
 // S_e = 2 (D_{e-1} - 2 D_e + D_{e+1}) / h, each thread forming the three
 // node values it needs.
-__global__ void soft_table_kernel(const double *__restrict__ dldc, int bins, double w, double h,
-                                  double sigma, int n_nodes, float2 *__restrict__ table) {
+__global__ void soft_table_kernel(const double *__restrict__ dldc, int bins,
+                                  const double *__restrict__ wt, int J, double inv_h, int n_nodes,
+                                  float2 *__restrict__ table) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     const int tn = n_nodes + kSoftPadL + kSoftPadR;
     if (e >= tn) return;
-    const double dm = e > 0 ? soft_node_d(dldc, bins, w, h, sigma, n_nodes, e - 1) : 0.0;
-    const double d0 = soft_node_d(dldc, bins, w, h, sigma, n_nodes, e);
-    const double dp = e + 1 < tn ? soft_node_d(dldc, bins, w, h, sigma, n_nodes, e + 1) : 0.0;
-    const double inv_h = 1.0 / h;
+    const double dm = e > 0 ? soft_node_d(dldc, bins, wt, J, n_nodes, e - 1) : 0.0;
+    const double d0 = soft_node_d(dldc, bins, wt, J, n_nodes, e);
+    const double dp = e + 1 < tn ? soft_node_d(dldc, bins, wt, J, n_nodes, e + 1) : 0.0;
     table[e] = make_float2((float)((dp - dm) * inv_h), (float)(2.0 * (dm - 2.0 * d0 + dp) * inv_h));
 }
 
@@ -1897,6 +1893,8 @@ struct wmd_softrdf {
     int last_path = 0;
     int last_nc[3] = {0, 0, 0};
     DevBuf d_boxes, d_table, d_nodes, d_gsorted, d_queue;
+    DevBuf d_wt;                       // bin-node weights wt[2 wt_j + 1] (f64)
+    int wt_j = 0;
     HostBuf h_boxes;
     Event ev_boxes;                    // the last box upload (guards h_boxes)
     std::vector<FrameBox> boxes_up;    // what d_boxes holds
@@ -1992,7 +1990,7 @@ void soft_run(wmd_softrdf *h, bool backward, const float *d_soa, size_t stride, 
     if (backward) {
         float2 *tab = (float2 *)h->d_table.reserve((size_t)tn * 8);
         hipLaunchKernelGGL(soft_table_kernel, dim3((tn + 255) / 256), dim3(256), 0, st, d_dc,
-                           h->bins, (double)h->w, soft_h(h), (double)h->sigma, sp.n_nodes, tab);
+                           h->bins, (const double *)h->d_wt.ptr, h->wt_j, 1.0 / soft_h(h), sp.n_nodes, tab);
         WMD_HIP_CHECK(hipGetLastError());
         sp.table = tab;
     } else {
@@ -2005,8 +2003,8 @@ void soft_run(wmd_softrdf *h, bool backward, const float *d_soa, size_t stride, 
     auto conv = [&]() {
         if (backward) return;
         hipLaunchKernelGGL(soft_conv_kernel, dim3((h->bins + 255) / 256), dim3(256), 0, st,
-                           (const double *)sp.nodes, sp.n_nodes, h->bins, (double)h->w, soft_h(h),
-                           (double)h->sigma, d_counts);
+                           (const double *)sp.nodes, sp.n_nodes, h->bins, (const double *)h->d_wt.ptr,
+                           h->wt_j, d_counts);
         WMD_HIP_CHECK(hipGetLastError());
     };
     const SoftPlanOut pl = soft_plan(h, B, n);
@@ -2152,6 +2150,21 @@ wmd_status wmd_softrdf_create(wmd_ctx *ctx, size_t bins, float r_max, float sigm
     h->sigma = sigma;
     h->w = w;
     h->pbc = pbc;
+    {
+        // wt[J + j] = w phi_sigma(j h), |j h| <= 6 sigma (J rounded so that an
+        // exact multiple of h at 6 sigma is kept)
+        const double hh = soft_h(h), sg = (double)sigma;
+        const int J = (int)std::floor(6.0 * sg / hh * (1.0 + 1e-12));
+        std::vector<double> wt(2 * J + 1);
+        const double norm = (double)w / (sg * 2.5066282746310002);
+        for (int j = -J; j <= J; ++j) {
+            const double d = j * hh;
+            wt[J + j] = norm * std::exp(-0.5 * d * d / (sg * sg));
+        }
+        h->wt_j = J;
+        WMD_HIP_CHECK(hipMemcpy(h->d_wt.reserve(wt.size() * 8), wt.data(), wt.size() * 8,
+                                hipMemcpyHostToDevice));
+    }
     *out = h;
     WMD_API_END
 }
