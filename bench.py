@@ -762,6 +762,21 @@ def counter_sequence(ctx, dev):
                 f"{int(cnt.sum()) // 2} unordered in-range pairs")
         L.wmd_rdf_destroy(h)
     log("counters: plain RDF on the soft workload done")
+    del soa, xs, base
+    # the uneven batch (256 trajectories, 50 frames, default tuning), one
+    # eager step: its pair kernel, cell build and RMSD kernels
+    from warp_md_amd import workloads as wl
+    from warp_md_amd.batch import UnevenBatch
+
+    spec = wl.uneven_batch(sizes_only=True)
+    ns = np.array([s["n"] for s in spec], np.int64)
+    b = UnevenBatch(ns, [50] * len(spec), bins=300, r_max=12.0, graph=False, device=dev)
+    b.set_boxes([s["box"] for s in spec])
+    store = wl.uneven_store_device(b, spec, seed=3)
+    rm = torch.zeros(50 * len(spec), dtype=torch.float32, device=dev)
+    b.run(store, rm)
+    torch.cuda.synchronize()
+    log(f"counters: uneven batch step done ({float(b.counts().sum()) / 2:.4g} unordered in-range pairs)")
 
 
 def uneven_batch_bench(ctx, dev, reps=3, autotune=True, cpu_sample=True):

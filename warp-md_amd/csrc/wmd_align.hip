@@ -173,6 +173,7 @@ __global__ __launch_bounds__(kAlBlock) void al_fit_kernel(
 // UL kRegBlock x 12 bytes.
 constexpr int kRegBlock = 512;
 constexpr int kRegU = 14;  // register-resident atoms per thread
+constexpr int kRegSel = 4;  // selected atoms per thread fetched ahead of the frame
 // Raw buffer descriptor word 3 (32-bit data format, untyped access); with a
 // zero stride the hardware bounds every access by num_records bytes, the
 // whole offset being in the VGPR operand.
@@ -194,6 +195,19 @@ __global__ __launch_bounds__(kRegBlock, 1) void al_superpose_reg_kernel(
         // and are never written out): no branches, all loads in flight at once
         const __amdgpu_buffer_rsrc_t in =
             __builtin_amdgcn_make_buffer_rsrc((void *)x, (short)0, (int)(n_src * 16), kBufWord3);
+        // the fit's first kRegSel selected atoms per thread are requested
+        // before the frame itself: the frame's rows then find those lines in
+        // L2 (gathered after the rows, they came from HBM a second time)
+        float4 gs[kRegSel];
+        if (n > 0) {
+            // clamped indices, no branches: the index loads, then the atom
+            // loads, each group in flight together
+            unsigned si[kRegSel];
+#pragma unroll
+            for (int u = 0; u < kRegSel; ++u) si[u] = sel[min((int)threadIdx.x + u * kRegBlock, n - 1)];
+#pragma unroll
+            for (int u = 0; u < kRegSel; ++u) gs[u] = x[si[u]];
+        }
         float px[UR], py[UR], pz[UR];
 #pragma unroll
         for (int u = 0; u < UR; ++u) {
@@ -214,8 +228,15 @@ __global__ __launch_bounds__(kRegBlock, 1) void al_superpose_reg_kernel(
         double m[14];
 #pragma unroll
         for (int c = 0; c < 14; ++c) m[c] = 0.0;
-        for (int k = threadIdx.x; k < n; k += kRegBlock) {
-            const float4 p = x[sel[k]];
+        for (int k = threadIdx.x, u = 0; k < n; k += kRegBlock, ++u) {
+            float4 p;
+            if (u < kRegSel) {
+#pragma unroll
+                for (int v = 0; v < kRegSel; ++v)
+                    if (v == u) p = gs[v];
+            } else {
+                p = x[sel[k]];
+            }
             const double w = wts ? (double)wts[k] : 1.0;
             const double xv[3] = {(double)p.x, (double)p.y, (double)p.z};
             const double yr[3] = {ref_c[3 * k], ref_c[3 * k + 1], ref_c[3 * k + 2]};
