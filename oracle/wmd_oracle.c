@@ -159,9 +159,13 @@ static int load_box(int kind, const float *m, frame_box *fb) {
 
 /* The per-pair f32 expression.  Orthorhombic: rdf.rs:921-929 (f32 subtract,
  * dx -= round(dx/lx)*lx with Rust round = C roundf, left-to-right adds in
- * rdf_distance rdf.rs:473-478, IEEE sqrt).  Triclinic: the reference's own f32
- * device formula wrap_triclinic (traj-kernels part1.cu:12-27), evaluated
- * without FMA contraction.  Returns the f32 distance. */
+ * rdf_distance rdf.rs:473-478, IEEE sqrt).  Triclinic (the reference RdfPlan
+ * refuses it, rdf.rs:797-802; this order is the engine's own definition): the
+ * image n = roundf(f) of the f32 fractional displacement f = inv . d as in the
+ * reference device formula wrap_triclinic (traj-kernels part1.cu:12-27), then
+ * the lattice vector taken off in Cartesian space, d -= (n0 a + n1 b) + n2 c,
+ * each component one f32 sum of f32 products; no FMA contraction.  Returns the
+ * f32 distance. */
 static inline float pair_r32(const float *pa, const float *pb, const frame_box *fb) {
     float dx = pb[0] - pa[0];
     float dy = pb[1] - pa[1];
@@ -171,15 +175,12 @@ static inline float pair_r32(const float *pa, const float *pb, const frame_box *
         dy -= roundf(dy / fb->ly) * fb->ly;
         dz -= roundf(dz / fb->lz) * fb->lz;
     } else if (fb->kind == PBC_TRICLINIC) {
-        float f0 = fb->r[0][0] * dx + fb->r[1][0] * dy + fb->r[2][0] * dz;
-        float f1 = fb->r[0][1] * dx + fb->r[1][1] * dy + fb->r[2][1] * dz;
-        float f2 = fb->r[0][2] * dx + fb->r[1][2] * dy + fb->r[2][2] * dz;
-        f0 -= roundf(f0);
-        f1 -= roundf(f1);
-        f2 -= roundf(f2);
-        dx = f0 * fb->c[0][0] + f1 * fb->c[1][0] + f2 * fb->c[2][0];
-        dy = f0 * fb->c[0][1] + f1 * fb->c[1][1] + f2 * fb->c[2][1];
-        dz = f0 * fb->c[0][2] + f1 * fb->c[1][2] + f2 * fb->c[2][2];
+        const float n0 = roundf(fb->r[0][0] * dx + fb->r[1][0] * dy + fb->r[2][0] * dz);
+        const float n1 = roundf(fb->r[0][1] * dx + fb->r[1][1] * dy + fb->r[2][1] * dz);
+        const float n2 = roundf(fb->r[0][2] * dx + fb->r[1][2] * dy + fb->r[2][2] * dz);
+        dx -= n0 * fb->c[0][0] + n1 * fb->c[1][0] + n2 * fb->c[2][0];
+        dy -= n0 * fb->c[0][1] + n1 * fb->c[1][1] + n2 * fb->c[2][1];
+        dz -= n0 * fb->c[0][2] + n1 * fb->c[1][2] + n2 * fb->c[2][2];
     }
     return sqrtf(dx * dx + dy * dy + dz * dz);
 }

@@ -82,14 +82,15 @@ def _py_pair_r(pa, pb, kind, box):
         dz = f32(dz - f32(_round_half_away(f32(dz / lz)) * lz))
     elif kind == "triclinic":
         c, r = box
-        f = []
+        n = []
         for k in range(3):
             v = f32(f32(f32(r[0][k] * dx) + f32(r[1][k] * dy)) + f32(r[2][k] * dz))
-            f.append(f32(v - _round_half_away(v)))
-        out = []
+            n.append(f32(_round_half_away(v)))
+        d = [dx, dy, dz]
         for k in range(3):
-            out.append(f32(f32(f32(f[0] * c[0][k]) + f32(f[1] * c[1][k])) + f32(f[2] * c[2][k])))
-        dx, dy, dz = out
+            shift = f32(f32(f32(n[0] * c[0][k]) + f32(n[1] * c[1][k])) + f32(n[2] * c[2][k]))
+            d[k] = f32(d[k] - shift)
+        dx, dy, dz = d
     s = f32(f32(f32(dx * dx) + f32(dy * dy)) + f32(dz * dz))
     return f32(np.sqrt(s))
 
@@ -243,3 +244,22 @@ def test_min_image_known_vectors(case):
     out = oracle.min_image_f64(case["a"], case["b"], box=box, kind=case["kind"],
                                length_scale=case["scale"])
     np.testing.assert_allclose(out, case["expect"], atol=case["tol"], rtol=0)
+
+
+@pytest.mark.parametrize("shear", [10.0, 25.0])
+def test_triclinic_f32_agrees_with_f64_apply_pbc(shear):
+    # the f32 triclinic order (n = round(inv . d), d -= (n0 a + n1 b) + n2 c) must
+    # bin every pair as the f64 apply_pbc_triclinic (pbc_math.rs:92-108) does,
+    # except pairs within 1e-5 A of a bin edge; a wrong image, sign or index
+    # in the lattice shift moves thousands of pairs
+    rng = np.random.default_rng(int(shear))
+    n, F, L = 1500, 2, 32.0
+    m = np.array([[L, 0, 0], [L * math.tan(math.radians(shear)), L, 0], [1.5, -2.0, L]], np.float64)
+    frac = rng.random((F, n, 3)) * 1.2 - 0.1  # some atoms outside the primary cell
+    frames = (frac @ m).astype(np.float32)
+    boxes = np.repeat(m.reshape(1, 9), F, 0).astype(np.float32)
+    counts, edge = oracle.rdf_counts(frames, np.arange(n), np.arange(n), 120, 12.0,
+                                     pbc="triclinic", boxes=boxes, edges=True)
+    assert counts.sum() > 100_000
+    assert int(edge[2]) <= int(edge[1]) + int(edge[0])
+    assert int(edge[2]) <= 1e-4 * counts.sum()

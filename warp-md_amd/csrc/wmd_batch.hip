@@ -286,7 +286,7 @@ __global__ __launch_bounds__(kBatchBlock) void batch_pair_kernel(BatchPairArgs a
     L.elo = thr;
     L.ehi = thr;
     L.ring = rings + wave * kRing;
-    PlainCons<PBC, false, false, false, TABLE> cons{p, L, 0ull};
+    PlainCons<PBC, false, false, TABLE> cons{p, L, 0ull};
 
     const int home_q = blockIdx.x % kSlices;
     if (a.clock && lane == 0) atomicMax(&a.clock[home_q * kClock], ~clock_now());

@@ -24,10 +24,14 @@
 //    the image separation is < (k+1)/nc < 1/2 box.  The per-j value n*L (the
 //    reference's own f32 product round(dx/L)*L) is staged in LDS and the pair
 //    costs dx = (xj - xi) - S: the reference's two f32 roundings, nothing else.
-//    Triclinic boxes stage the fractional image n and evaluate the reference's
-//    f32 device formula wrap_triclinic (part1.cu:12-27) with f - n in place of
-//    f - round(f), skipping the zero entries of a lower-triangular box (adding
-//    or multiplying by an exact zero changes at most the sign of a zero).
+//    Triclinic boxes (refused by the reference RdfPlan, rdf.rs:797-802) take
+//    n = round(f) of the reference's f32 fractional displacement f = inv . d
+//    (wrap_triclinic, part1.cu:12-27) and subtract the lattice vector in
+//    Cartesian space, S = (n0 a + n1 b) + n2 c per component in f32; the shift
+//    path stages that S per j, so a pair costs what an orthorhombic one does.
+//    For a pair in range, |f - n| < 1/2 on every axis (r_max is below half the
+//    smallest plane spacing), so the cells' image is round(f); any other image
+//    lies at least half a plane spacing away and is counted by neither.
 //  * rint: when r_max < half the smallest plane spacing (less 1e-5 relative),
 //    rint(dx * (1/L)) equals roundf(dx / L) for every pair in range; a pair
 //    where they could differ lies near L/2 > r_max and is counted by neither.
@@ -1211,11 +1215,11 @@ void launch_brute_t(wmd_rdf *h, hipStream_t st, const float *soa, long long fram
     WMD_HIP_CHECK(hipGetLastError());
 }
 
-template <int PBC, bool LT, bool WEIGHTED, bool EDGE, bool TABLE>
+template <int PBC, bool WEIGHTED, bool EDGE, bool TABLE>
 void launch_cells_t(wmd_rdf *h, hipStream_t st, const float4 *sorted, const unsigned *start,
                     const unsigned *frame_flag, int n, int nf, const FrameBox *boxes,
                     const int *nc, int k, int shift_ok, float cull2, const HistParams &p) {
-    auto kern = rdf_cell_kernel<PBC, LT, WEIGHTED, EDGE, TABLE>;
+    auto kern = rdf_cell_kernel<PBC, WEIGHTED, EDGE, TABLE>;
     ProfScope prof(h->ctx, st, "rdf_pair");
     size_t lds = cell_lds_bytes(p.bins, EDGE, TABLE ? p.tab_n : 0);
     WMD_REQUIRE(lds <= h->ctx->lds_per_block, WMD_ERR_UNSUPPORTED, "too many bins for LDS");
@@ -1263,18 +1267,17 @@ void brute_dispatch(bool exact, bool weighted, bool edge, bool table, int pbc, A
 }
 
 template <class... A>
-void cells_dispatch(int pbc, bool lt, bool weighted, bool edge, bool table, A... a) {
-#define WMD_CELLS(P, LT_, W, D, T)                                                \
-    if (pbc == P && lt == LT_ && weighted == W && edge == D && table == T)        \
-        return launch_cells_t<P, LT_, W, D, T>(a...);
-#define WMD_CELLS_P(P, LT_) \
-    WMD_CELLS(P, LT_, false, false, false) WMD_CELLS(P, LT_, false, false, true) \
-    WMD_CELLS(P, LT_, false, true, false) WMD_CELLS(P, LT_, false, true, true)   \
-    WMD_CELLS(P, LT_, true, false, false) WMD_CELLS(P, LT_, true, false, true)   \
-    WMD_CELLS(P, LT_, true, true, false) WMD_CELLS(P, LT_, true, true, true)
-    WMD_CELLS_P(WMD_PBC_ORTHO, false)
-    WMD_CELLS_P(WMD_PBC_TRICLINIC, false)
-    WMD_CELLS_P(WMD_PBC_TRICLINIC, true)
+void cells_dispatch(int pbc, bool weighted, bool edge, bool table, A... a) {
+#define WMD_CELLS(P, W, D, T)                                       \
+    if (pbc == P && weighted == W && edge == D && table == T)       \
+        return launch_cells_t<P, W, D, T>(a...);
+#define WMD_CELLS_P(P) \
+    WMD_CELLS(P, false, false, false) WMD_CELLS(P, false, false, true) \
+    WMD_CELLS(P, false, true, false) WMD_CELLS(P, false, true, true)   \
+    WMD_CELLS(P, true, false, false) WMD_CELLS(P, true, false, true)   \
+    WMD_CELLS(P, true, true, false) WMD_CELLS(P, true, true, true)
+    WMD_CELLS_P(WMD_PBC_ORTHO)
+    WMD_CELLS_P(WMD_PBC_TRICLINIC)
 #undef WMD_CELLS_P
 #undef WMD_CELLS
     fail(WMD_ERR_INVALID, "no cell-list kernel for this combination");
@@ -1378,7 +1381,6 @@ void accumulate_device(wmd_rdf *h, const float *d_soa, size_t stride, size_t n_f
     const size_t batch = h->cells.frames_for(n, ncell, n_frames, false);
     h->cells.reserve(n, ncell, batch, false);
     const unsigned *d_w = h->weighted ? h->d_wts.as<unsigned>() : nullptr;
-    const bool lt = h->pbc == WMD_PBC_TRICLINIC && B.lower_tri;
     bool cell_table = table && (int)h->tab.size() <= kMaxTab2;
     const auto passes = hist_passes(h, p, edge, &cell_table, [&](int bins, bool t) {
         return cell_lds_bytes(bins, edge, t ? p.tab_n : 0);
@@ -1388,7 +1390,7 @@ void accumulate_device(wmd_rdf *h, const float *d_soa, size_t stride, size_t n_f
         const FrameBox *bx = d_boxes + f0;
         build_cells(h->ctx, st, h->pbc, d_soa, (int)stride, n, nf, (long long)f0, bx, nc, d_w, h->cells, false);
         for (const HistParams &q : passes)
-            cells_dispatch(h->pbc, lt, h->weighted, edge, cell_table, h, st, h->cells.sorted.as<float4>(),
+            cells_dispatch(h->pbc, h->weighted, edge, cell_table, h, st, h->cells.sorted.as<float4>(),
                            (const unsigned *)h->cells.start.ptr, (const unsigned *)h->cells.flag.ptr, n, nf,
                            bx, (const int *)nc, k_use, shift_ok, cull2, q);
     }

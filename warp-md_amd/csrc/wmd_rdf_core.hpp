@@ -153,7 +153,14 @@ __device__ __forceinline__ void count_pair(float s, unsigned w, const HistParams
 }
 
 // -------------------------------------------------------------------------
-// The per-pair squared distance, reference operation order (rint / exact wrap).
+// The per-pair squared distance (rint / exact wrap).  Orthorhombic: the
+// reference's operations in its order, dx -= round(dx / L) * L.  Triclinic
+// (which the reference RdfPlan refuses, rdf.rs:797-802): the image n is the
+// reference's f32 fractional rounding n = round(f), f = inv . d (wrap_triclinic,
+// part1.cu:12-27), and the lattice shift is taken off in Cartesian space,
+// d -= (n0 a + n1 b) + n2 c, each component one f32 sum of f32 products.  The
+// shift path below subtracts the same per-j vector, so a pair costs the same
+// as an orthorhombic one; the oracle states this order.
 // -------------------------------------------------------------------------
 template <int PBC, bool EXACT>
 __device__ __forceinline__ float pair_s(float xi, float yi, float zi, float xj, float yj, float zj,
@@ -172,63 +179,41 @@ __device__ __forceinline__ float pair_s(float xi, float yi, float zi, float xj, 
             dz -= rintf(dz * b.ilz) * b.lz;
         }
     } else if constexpr (PBC == WMD_PBC_TRICLINIC) {
-        float f0 = b.r[0] * dx + b.r[3] * dy + b.r[6] * dz;
-        float f1 = b.r[1] * dx + b.r[4] * dy + b.r[7] * dz;
-        float f2 = b.r[2] * dx + b.r[5] * dy + b.r[8] * dz;
-        if constexpr (EXACT) {
-            f0 -= roundf(f0);
-            f1 -= roundf(f1);
-            f2 -= roundf(f2);
-        } else {
-            f0 -= rintf(f0);
-            f1 -= rintf(f1);
-            f2 -= rintf(f2);
-        }
-        dx = f0 * b.c[0] + f1 * b.c[3] + f2 * b.c[6];
-        dy = f0 * b.c[1] + f1 * b.c[4] + f2 * b.c[7];
-        dz = f0 * b.c[2] + f1 * b.c[5] + f2 * b.c[8];
+        const float f0 = b.r[0] * dx + b.r[3] * dy + b.r[6] * dz;
+        const float f1 = b.r[1] * dx + b.r[4] * dy + b.r[7] * dz;
+        const float f2 = b.r[2] * dx + b.r[5] * dy + b.r[8] * dz;
+        const float n0 = EXACT ? roundf(f0) : rintf(f0);
+        const float n1 = EXACT ? roundf(f1) : rintf(f1);
+        const float n2 = EXACT ? roundf(f2) : rintf(f2);
+        dx -= n0 * b.c[0] + n1 * b.c[3] + n2 * b.c[6];
+        dy -= n0 * b.c[1] + n1 * b.c[4] + n2 * b.c[7];
+        dz -= n0 * b.c[2] + n1 * b.c[5] + n2 * b.c[8];
     }
     return dx * dx + dy * dy + dz * dz;
 }
 
-// Shift path: the image is known; (sx, sy, sz) hold n*L (orthorhombic) or n
-// (triclinic fractional).  LT: lower-triangular box (r[1] = r[2] = r[5] = 0 and
-// c[1] = c[2] = c[5] = 0), whose zero terms are skipped.
-template <int PBC, bool LT>
-__device__ __forceinline__ float pair_s_shift(float xi, float yi, float zi, float xj, float yj,
-                                              float zj, float sx, float sy, float sz,
-                                              const FrameBox &b) {
-    float dx = xj - xi;
-    float dy = yj - yi;
-    float dz = zj - zi;
+// The lattice shift of image n as pair_s subtracts it: n * L (orthorhombic,
+// the reference's f32 product round(dx / L) * L) or (n0 a + n1 b) + n2 c.
+template <int PBC>
+__device__ __forceinline__ void image_shift(float n0, float n1, float n2, const FrameBox &b,
+                                            float &sx, float &sy, float &sz) {
     if constexpr (PBC == WMD_PBC_ORTHO) {
-        dx -= sx;
-        dy -= sy;
-        dz -= sz;
+        sx = n0 * b.lx;
+        sy = n1 * b.ly;
+        sz = n2 * b.lz;
     } else {
-        float f0, f1, f2;
-        if constexpr (LT) {
-            f0 = b.r[0] * dx + b.r[3] * dy + b.r[6] * dz;
-            f1 = b.r[4] * dy + b.r[7] * dz;
-            f2 = b.r[8] * dz;
-        } else {
-            f0 = b.r[0] * dx + b.r[3] * dy + b.r[6] * dz;
-            f1 = b.r[1] * dx + b.r[4] * dy + b.r[7] * dz;
-            f2 = b.r[2] * dx + b.r[5] * dy + b.r[8] * dz;
-        }
-        f0 -= sx;
-        f1 -= sy;
-        f2 -= sz;
-        if constexpr (LT) {
-            dx = f0 * b.c[0] + f1 * b.c[3] + f2 * b.c[6];
-            dy = f1 * b.c[4] + f2 * b.c[7];
-            dz = f2 * b.c[8];
-        } else {
-            dx = f0 * b.c[0] + f1 * b.c[3] + f2 * b.c[6];
-            dy = f0 * b.c[1] + f1 * b.c[4] + f2 * b.c[7];
-            dz = f0 * b.c[2] + f1 * b.c[5] + f2 * b.c[8];
-        }
+        sx = n0 * b.c[0] + n1 * b.c[3] + n2 * b.c[6];
+        sy = n0 * b.c[1] + n1 * b.c[4] + n2 * b.c[7];
+        sz = n0 * b.c[2] + n1 * b.c[5] + n2 * b.c[8];
     }
+}
+
+// Shift path: the image is known and (sx, sy, sz) = image_shift(n).
+__device__ __forceinline__ float pair_s_shift(float xi, float yi, float zi, float xj, float yj,
+                                              float zj, float sx, float sy, float sz) {
+    const float dx = (xj - xi) - sx;
+    const float dy = (yj - yi) - sy;
+    const float dz = (zj - zi) - sz;
     return dx * dx + dy * dy + dz * dz;
 }
 
@@ -766,30 +751,25 @@ __device__ __forceinline__ void cell_count(float s, unsigned mi, unsigned mAj, u
 // 3: the reference's roundf(dx / L) wrap (pair_s_checked).
 // HOME: the chunk holds home atoms (stream positions below n_home), which
 // pair only with home atoms of lower index.
-template <int PBC, bool LT, bool WEIGHTED, bool EDGE, bool TABLE, int MODE, bool HOME>
+template <int PBC, bool WEIGHTED, bool EDGE, bool TABLE, int MODE, bool HOME>
 __device__ __forceinline__ void cell_chunk(const float4 *__restrict__ hp, int n_home, float xj,
                                            float yj, float zj, unsigned mj, int rank,
                                            const FrameBox &b, const HistParams &p,
                                            const CellLds &L, unsigned long long &edge_local) {
     float sx = 0.f, sy = 0.f, sz = 0.f;
     if constexpr (MODE == 1) {
-        sx = (float)((int)(mj & 0xFFu) - 128);  // image shift n + 128 per byte
-        sy = (float)((int)((mj >> 8) & 0xFFu) - 128);
-        sz = (float)((int)((mj >> 16) & 0xFFu) - 128);
-        if constexpr (PBC == WMD_PBC_ORTHO) {
-            sx *= b.lx;  // the reference's f32 product round(dx/L) * L
-            sy *= b.ly;
-            sz *= b.lz;
-        }
+        // image shift n + 128 per byte
+        image_shift<PBC>((float)((int)(mj & 0xFFu) - 128), (float)((int)((mj >> 8) & 0xFFu) - 128),
+                         (float)((int)((mj >> 16) & 0xFFu) - 128), b, sx, sy, sz);
     }
     const unsigned mAj = (mj >> 24) & 0xFu, mBj = mj >> 28;
     const float nanv = __builtin_nanf("");
     auto dist = [&](const float4 &pi) {
-        if constexpr (MODE == 0 && PBC == WMD_PBC_ORTHO) {
+        if constexpr (MODE == 0) {  // no image shift in the chunk
             const float dx = xj - pi.x, dy = yj - pi.y, dz = zj - pi.z;
             return dx * dx + dy * dy + dz * dz;
-        } else if constexpr (MODE <= 1) {
-            return pair_s_shift<PBC, LT>(pi.x, pi.y, pi.z, xj, yj, zj, sx, sy, sz, b);
+        } else if constexpr (MODE == 1) {
+            return pair_s_shift(pi.x, pi.y, pi.z, xj, yj, zj, sx, sy, sz);
         } else if constexpr (MODE == 2) {
             return pair_s<PBC, false>(pi.x, pi.y, pi.z, xj, yj, zj, b);
         } else {
@@ -1152,7 +1132,7 @@ __device__ __forceinline__ void cell_waves(const CellGrid &G, float4 *ring, unsi
 // Plain histogram consumer: exact bins, u64 LDS counts.
 // With EDGE (diagnostics) it also counts the pair evaluations: a valid j lane
 // meets min(rank, n_home) home atoms in a home chunk and all n_home otherwise.
-template <int PBC, bool LT, bool WEIGHTED, bool EDGE, bool TABLE>
+template <int PBC, bool WEIGHTED, bool EDGE, bool TABLE>
 struct PlainCons {
     HistParams p;
     CellLds L;
@@ -1161,7 +1141,7 @@ struct PlainCons {
     template <int MODE, bool HOME>
     __device__ __forceinline__ void chunk(const float4 *hp, int n_home, float xj, float yj, float zj,
                                           unsigned mj, int rank, unsigned, int, const FrameBox &b) {
-        cell_chunk<PBC, LT, WEIGHTED, EDGE, TABLE, MODE, HOME>(hp, n_home, xj, yj, zj, mj, rank, b, p,
+        cell_chunk<PBC, WEIGHTED, EDGE, TABLE, MODE, HOME>(hp, n_home, xj, yj, zj, mj, rank, b, p,
                                                                 L, edge_local);
         if constexpr (EDGE) {
             const int m = HOME ? min(rank, n_home) : n_home;
@@ -1172,7 +1152,7 @@ struct PlainCons {
     __device__ __forceinline__ void unit_end(int, int, int) {}
 };
 
-template <int PBC, bool LT, bool WEIGHTED, bool EDGE, bool TABLE>
+template <int PBC, bool WEIGHTED, bool EDGE, bool TABLE>
 __global__ __launch_bounds__(kCellBlock) void rdf_cell_kernel(CellGrid G, HistParams p) {
     extern __shared__ __attribute__((aligned(16))) char lds[];
     const CellLds L = cell_lds_layout(lds, p, EDGE, TABLE);
@@ -1190,7 +1170,7 @@ __global__ __launch_bounds__(kCellBlock) void rdf_cell_kernel(CellGrid G, HistPa
 
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform: unit indices in SGPRs
     const int lane = threadIdx.x & 63;
-    PlainCons<PBC, LT, WEIGHTED, EDGE, TABLE> cons{p, L, 0ull, 0ull};
+    PlainCons<PBC, WEIGHTED, EDGE, TABLE> cons{p, L, 0ull, 0ull};
     cell_waves<PBC, false>(G, L.ring + wave * kRing, nullptr, wave, cons);
 
     __syncthreads();
