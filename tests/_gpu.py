@@ -33,11 +33,12 @@ def boxes9(box, n_frames, pbc):
 
 
 def rdf_counts(coords, sel_a, sel_b, bins, r_max, pbc="orthorhombic", box=None, flags=0,
-               cell_k=0, full_chunk=True, split=None, evals=False):
+               cell_k=0, full_chunk=True, split=None, evals=False, cells=False):
     """GPU counts through wmd_rdf_accumulate (full-system xyzw chunk).
 
-    Returns (counts u64, edge, path) where path is 1 brute / 2 cells, and with
-    evals=True also the pair evaluations (wmd_rdf_pair_evaluations)."""
+    Returns (counts u64, edge, path) where path is 1 brute / 2 cells, with
+    evals=True also the pair evaluations (wmd_rdf_pair_evaluations), and with
+    cells=True also the cells per side of the last cell grid."""
     c = np.ascontiguousarray(coords, np.float32)
     F, N, _ = c.shape
     x4 = np.ones((F, N, 4), np.float32)
@@ -59,14 +60,18 @@ def rdf_counts(coords, sel_a, sel_b, bins, r_max, pbc="orthorhombic", box=None, 
         edge = np.zeros(1, np.uint64)
         check(lib().wmd_rdf_read(h, ptr(counts), ptr(edge)))
         path = ctypes.c_int()
-        check(lib().wmd_rdf_last_path(h, ctypes.byref(path), None))
+        nc = (ctypes.c_int * 3)()
+        check(lib().wmd_rdf_last_path(h, ctypes.byref(path), nc))
         n_eval = ctypes.c_uint64()
         check(lib().wmd_rdf_pair_evaluations(h, ctypes.byref(n_eval)))
     finally:
         lib().wmd_rdf_destroy(h)
+    out = (counts, int(edge[0]), path.value)
     if evals:
-        return counts, int(edge[0]), path.value, int(n_eval.value)
-    return counts, int(edge[0]), path.value
+        out += (int(n_eval.value),)
+    if cells:
+        out += (tuple(nc),)
+    return out
 
 
 def rdf_counts_device(store, n, sel_len, bins, r_max, pbc, boxes, flags=0, cell_k=0):

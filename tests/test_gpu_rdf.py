@@ -120,6 +120,24 @@ def test_exact_wrap_small_box(kind):
     np.testing.assert_array_equal(got, ref)
 
 
+@pytest.mark.parametrize("kind", ["orthorhombic", "triclinic"])
+@pytest.mark.parametrize("L,r_max,cell_k,want", [(34.0, 12.0, 2, 5), (40.0, 12.0, 2, 6),
+                                                   (42.0, 12.0, 1, 3), (23.5, 9.0, 3, 7)])
+def test_small_grid_shift_path(kind, L, r_max, cell_k, want):
+    # grids of 2k+1 and 2k+2 cells per side: the stencil spans more than half
+    # the box, yet every counted pair takes its image from the cells
+    n, F, bins = 3000, 3, 120
+    frames, box = _random_case(kind, n, F, seed=int(L * 10 + cell_k), L=L)
+    sel = np.arange(n)
+    got, _, path, nc = _gpu.rdf_counts(frames, sel, sel, bins, r_max, pbc=kind, box=box,
+                                       cell_k=cell_k, cells=True)
+    ref = oracle.rdf_counts(frames, sel, sel, bins, r_max, pbc=kind,
+                            boxes=_gpu.boxes9(box, F, kind))
+    assert path == 2 and min(nc) == want, (path, nc)
+    assert ref.sum() > 0
+    np.testing.assert_array_equal(got, ref)
+
+
 @pytest.mark.parametrize("cell_k", [1, 2, 3])
 def test_cell_shapes_agree(cell_k):
     w = workloads.water_lattice(n_side=14, box=14 * 3.104, n_frames=3, seed=9, with_h=False)

@@ -466,7 +466,7 @@ void plan_batch(wmd_batch *b) {
             d.kind = KIND_CELLS;
             d.k = k;
             d.ncx = nc[0]; d.ncy = nc[1]; d.ncz = nc[2];
-            d.shift_ok = (nc[0] >= 2 * k + 3 && nc[1] >= 2 * k + 3 && nc[2] >= 2 * k + 3) ? 1 : 0;
+            d.shift_ok = (nc[0] >= 2 * k + 1 && nc[1] >= 2 * k + 1 && nc[2] >= 2 * k + 1) ? 1 : 0;
             d.cull2 = cull2_for((double)b->r_max, B);
             d.upf = ncell;
             scan += F;

@@ -18,12 +18,17 @@
 //
 // Minimum image.  The reference wraps each component with round(dx / L)
 // (roundf).  Three exact ways are used:
-//  * shift: with a periodic cell grid of >= 2k+3 cells per side, every pair the
-//    stencil visits has round(dx / L) = m_j - w - m_i exactly, where m is the
-//    atom's image index floor(x / L) and w the stencil's periodic wrap, because
-//    the image separation is < (k+1)/nc < 1/2 box.  The per-j value n*L (the
-//    reference's own f32 product round(dx/L)*L) is staged in LDS and the pair
-//    costs dx = (xj - xi) - S: the reference's two f32 roundings, nothing else.
+//  * shift: the cell path runs only when r_max is below half the smallest
+//    plane spacing (less 1e-5 relative) on a periodic grid of >= 2k+1 cells per
+//    side, cells >= (r_max + 1e-2 A) / k wide.  A pair the reference counts has
+//    its minimum image within k cells of the home cell, where the stencil
+//    visits it exactly once (offsets -k..k are distinct cells mod nc), with
+//    wrap w such that round(dx / L) = m_j - w - m_i (m: the atom's image index
+//    floor(x / L)).  A visited pair whose cell image is not round(dx / L) lies
+//    at least L/2 > r_max away in that image and is counted by neither.  The
+//    per-j value n*L (the reference's own f32 product round(dx/L)*L) is staged
+//    in LDS and the pair costs dx = (xj - xi) - S: the reference's two f32
+//    roundings, nothing else.
 //    Triclinic boxes (refused by the reference RdfPlan, rdf.rs:797-802) take
 //    n = round(f) of the reference's f32 fractional displacement f = inv . d
 //    (wrap_triclinic, part1.cu:12-27) and subtract the lattice vector in
@@ -1374,9 +1379,9 @@ void accumulate_device(wmd_rdf *h, const float *d_soa, size_t stride, size_t n_f
 
     // ---- cell list ----------------------------------------------------------
     const int ncell = nc[0] * nc[1] * nc[2];
-    // the image index of every stencil pair is determined by the cells when the
-    // stencil spans less than half the box: (k + 1) / nc < 1/2
-    const int shift_ok = (nc[0] >= 2 * k_use + 3 && nc[1] >= 2 * k_use + 3 && nc[2] >= 2 * k_use + 3) ? 1 : 0;
+    // every counted pair takes its image from the cells on a grid of >= 2k+1
+    // cells per side (r_max is below half the smallest plane spacing here)
+    const int shift_ok = (nc[0] >= 2 * k_use + 1 && nc[1] >= 2 * k_use + 1 && nc[2] >= 2 * k_use + 1) ? 1 : 0;
     const float cull2 = cull2_for((double)h->r_max, B);
     const size_t batch = h->cells.frames_for(n, ncell, n_frames, false);
     h->cells.reserve(n, ncell, batch, false);
@@ -2050,7 +2055,7 @@ void soft_run(wmd_softrdf *h, bool backward, const float *d_soa, size_t stride, 
         return;
     }
     const int ncell = pl.nc[0] * pl.nc[1] * pl.nc[2];
-    const int shift_ok = (pl.nc[0] >= 2 * pl.k + 3 && pl.nc[1] >= 2 * pl.k + 3 && pl.nc[2] >= 2 * pl.k + 3) ? 1 : 0;
+    const int shift_ok = (pl.nc[0] >= 2 * pl.k + 1 && pl.nc[1] >= 2 * pl.k + 1 && pl.nc[2] >= 2 * pl.k + 1) ? 1 : 0;
     const float cull2 = cull2_for(soft_reach(h), B);
     // with keep_cells the forward also writes the slot map, and a backward
     // on the same store, sizes and boxes takes the forward's cell lists
