@@ -53,7 +53,7 @@ using namespace wmd;
 
 namespace {
 
-constexpr int kBatchBlock = 512;  // 8 waves
+constexpr int kBatchBlock = 768;  // 12 waves: two workgroups per CU with u32 wave histograms
 constexpr int kBatchWaves = kBatchBlock / 64;
 constexpr int kBruteHome = 64;    // default home atoms per brute-force unit
 constexpr int kMaxBruteHome = 512;
@@ -208,30 +208,8 @@ struct BatchPairArgs {
 };
 
 __host__ __device__ inline size_t batch_lds_bytes(int bins, int tab_n) {
-    return align16((size_t)kBatchWaves * bins * 8) + align16((size_t)tab_n * 8) +
+    return align16((size_t)kBatchWaves * bins * 4) + align16((size_t)tab_n * 8) +
            align16(((size_t)bins + 1) * 4) + (size_t)kBatchWaves * kRing * 16;
-}
-
-__device__ __forceinline__ void wave_lds_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-// Adds a wave's LDS histogram to out[bins - skip] (local bins below skip are
-// dropped) and clears it.
-__device__ __forceinline__ void wave_hist_flush(unsigned long long *hist,
-                                                unsigned long long *out, int bins, int skip,
-                                                int lane) {
-    wave_lds_sync();
-    for (int i = lane; i < bins; i += 64) {
-        const unsigned long long v = hist[i];
-        if (v) {
-            if (i >= skip) atomicAdd(out + (i - skip), v);
-            hist[i] = 0ull;
-        }
-    }
-    wave_lds_sync();
 }
 
 // Brute-force unit: home block [I0, I0 + bh) of frame f against every j > i.
@@ -264,14 +242,14 @@ __global__ __launch_bounds__(kBatchBlock) void batch_pair_kernel(BatchPairArgs a
     extern __shared__ __attribute__((aligned(16))) char lds[];
     const HistParams &p = a.p;
     const int bins = p.bins;
-    unsigned long long *hist_all = (unsigned long long *)lds;
-    size_t off = align16((size_t)kBatchWaves * bins * 8);
+    unsigned *hist_all = (unsigned *)lds;  // [kBatchWaves][bins] u32, one histogram per wave
+    size_t off = align16((size_t)kBatchWaves * bins * 4);
     uint2 *tab2 = (uint2 *)(lds + off);
     off += align16((size_t)(TABLE ? p.tab_n : 0) * 8);
     float *thr = (float *)(lds + off);
     off += align16(((size_t)bins + 1) * 4);
     float4 *rings = (float4 *)(lds + off);
-    for (int i = threadIdx.x; i < kBatchWaves * bins; i += kBatchBlock) hist_all[i] = 0ull;
+    for (int i = threadIdx.x; i < kBatchWaves * bins; i += kBatchBlock) hist_all[i] = 0u;
     for (int i = threadIdx.x; i <= bins; i += kBatchBlock) thr[i] = p.thr[i];
     if constexpr (TABLE)
         for (int i = threadIdx.x; i < p.tab_n; i += kBatchBlock) tab2[i] = p.tab2[i];
@@ -280,13 +258,14 @@ __global__ __launch_bounds__(kBatchBlock) void batch_pair_kernel(BatchPairArgs a
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     CellLds L;
-    L.hist = hist_all + wave * bins;
+    L.hist = (unsigned long long *)(hist_all + wave * bins);
     L.tab2 = tab2;
     L.thr = thr;
     L.elo = thr;
     L.ehi = thr;
     L.ring = rings + wave * kRing;
-    PlainCons<PBC, false, false, TABLE> cons{p, L, 0ull};
+    // u32 counters (table offsets 4 x bin), drained into the trajectory's row
+    PlainCons<PBC, false, false, TABLE, 4> cons{p, L, 0ull, 0ull};
 
     const int home_q = blockIdx.x % kSlices;
     if (a.clock && lane == 0) atomicMax(&a.clock[home_q * kClock], ~clock_now());
@@ -309,9 +288,10 @@ __global__ __launch_bounds__(kBatchBlock) void batch_pair_kernel(BatchPairArgs a
         }
         if (a.clock && lane == 0 && visited > 0) atomicAdd(&a.clock[q * kClock + 3], 1ull);
         if (cur < 0 || task < T.task0 || task >= T.task0 + ((long long)T.n_frames * T.upf + T.grab - 1) / T.grab) {
-            if (cur >= 0) wave_hist_flush(L.hist, a.counts + (long long)cur * a.row_bins, bins, p.skip, lane);
+            if (cur >= 0) cons.drain();
             cur = __builtin_amdgcn_readfirstlane(
                 find_last_le(a.n_traj, task, [&](int i) { return a.traj[i].task0; }));
+            cons.row = a.counts + (long long)cur * a.row_bins;
             T = uload(a.traj, cur);
             G = CellGrid{a.sorted + T.af0, a.start + T.start_off, a.frame_flag + T.box0,
                          a.boxes + T.box0, T.n, T.n_frames, T.ncx, T.ncy, T.ncz, T.k,
@@ -332,7 +312,7 @@ __global__ __launch_bounds__(kBatchBlock) void batch_pair_kernel(BatchPairArgs a
             brute_unit(G, f, h, T.bh, cons);
         }
     }
-    if (cur >= 0) wave_hist_flush(L.hist, a.counts + (long long)cur * a.row_bins, bins, p.skip, lane);
+    if (cur >= 0) cons.drain();
     if (a.clock && lane == 0) atomicMax(&a.clock[home_q * kClock + 2], clock_now());
 }
 
@@ -782,6 +762,7 @@ wmd_batch *create_batch(wmd_ctx *ctx, size_t n_traj, const uint32_t *n_atoms, co
                            dim3(kBlock), 0, st, b->d_counts.as<unsigned long long>(),
                            (long long)(n_traj * bins));
         WMD_HIP_CHECK(hipGetLastError());
+        std::vector<uint2> t4;  // host source of the table copy, alive until the sync below
         if (bins > 0) {
             b->tables = make_bin_tables((int)bins, r_max);
             b->table = !b->tables.tab2.empty() && (int)b->tables.tab2.size() <= kMaxTab2;
@@ -789,9 +770,12 @@ wmd_batch *create_batch(wmd_ctx *ctx, size_t n_traj, const uint32_t *n_atoms, co
             WMD_HIP_CHECK(hipMemcpyAsync(b->d_thr.ptr, b->tables.thr.data(), (bins + 1) * 4,
                                          hipMemcpyHostToDevice, st));
             if (b->table) {
-                b->d_tab2.reserve(b->tables.tab2.size() * 8);
-                WMD_HIP_CHECK(hipMemcpyAsync(b->d_tab2.ptr, b->tables.tab2.data(), b->tables.tab2.size() * 8,
-                                             hipMemcpyHostToDevice, st));
+                // the pair kernel's wave histograms hold u32 counters: byte
+                // offsets 4 x bin in place of the table's 8 x bin
+                t4 = b->tables.tab2;
+                for (uint2 &e : t4) e.y >>= 1;
+                b->d_tab2.reserve(t4.size() * 8);
+                WMD_HIP_CHECK(hipMemcpyAsync(b->d_tab2.ptr, t4.data(), t4.size() * 8, hipMemcpyHostToDevice, st));
             }
         }
         WMD_HIP_CHECK(hipStreamSynchronize(st));

@@ -672,20 +672,21 @@ __device__ __forceinline__ CellLds cell_lds_layout(char *base, const HistParams 
     return L;
 }
 
-// Byte offset (8 x bin) of the exact bin of a counted pair (s < cut).
-template <bool TABLE>
+// Byte offset (HB x bin) of the exact bin of a counted pair (s < cut) in a
+// histogram of HB-byte counters; the table's offsets are scaled by HB.
+template <bool TABLE, int HB = 8>
 __device__ __forceinline__ unsigned cell_bin8(float s, const CellLds &L, const HistParams &p) {
     if constexpr (TABLE) {
         const unsigned t = __builtin_elementwise_sub_sat(__float_as_uint(s), (unsigned)p.tab_base);
         const uint2 e = L.tab2[t >> p.tab_sh];
-        return e.y + (s >= __uint_as_float(e.x) ? 8u : 0u);
+        return e.y + (s >= __uint_as_float(e.x) ? (unsigned)HB : 0u);
     } else {
         float r_est = __builtin_amdgcn_sqrtf(s);
         int b = (int)(r_est * p.inv_w) - p.shift;
         b = b < p.bins - 1 ? b : p.bins - 1;
         b = b > 0 ? b : 0;
         b += (int)(s >= L.thr[b + 1]) - (int)(s < L.thr[b]);
-        return (unsigned)b * 8u;
+        return (unsigned)b * (unsigned)HB;
     }
 }
 
@@ -725,18 +726,21 @@ __device__ __forceinline__ int lanes_below(unsigned long long m) {
                                           __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
 }
 
-// Counts one pair (s, i's meta word) of a chunk.
-template <bool WEIGHTED, bool EDGE, bool TABLE>
+// Counts one pair (s, i's meta word) of a chunk into HB-byte counters.
+template <bool WEIGHTED, bool EDGE, bool TABLE, int HB = 8>
 __device__ __forceinline__ void cell_count(float s, unsigned mi, unsigned mAj, unsigned mBj,
                                            const HistParams &p, const CellLds &L,
                                            unsigned long long &edge_local) {
     if (s < p.cut) {
         unsigned w = 2u;
         if constexpr (WEIGHTED) w = ((mi >> 24) & 0xFu) * mBj + (mi >> 28) * mAj;
-        const unsigned off = cell_bin8<TABLE>(s, L, p);
-        atomicAdd((unsigned long long *)((char *)L.hist + off), (unsigned long long)w);
+        const unsigned off = cell_bin8<TABLE, HB>(s, L, p);
+        if constexpr (HB == 8)
+            atomicAdd((unsigned long long *)((char *)L.hist + off), (unsigned long long)w);
+        else
+            atomicAdd((unsigned *)((char *)L.hist + off), w);
         if constexpr (EDGE) {
-            const unsigned bin = off >> 3;
+            const unsigned bin = off / (unsigned)HB;
             bool near = (s >= L.elo[bin] && s < L.ehi[bin]) ||
                         (s >= L.elo[bin + 1] && s < L.ehi[bin + 1]);
             edge_local += (near && bin >= (unsigned)p.skip) ? w : 0u;
@@ -751,7 +755,7 @@ __device__ __forceinline__ void cell_count(float s, unsigned mi, unsigned mAj, u
 // 3: the reference's roundf(dx / L) wrap (pair_s_checked).
 // HOME: the chunk holds home atoms (stream positions below n_home), which
 // pair only with home atoms of lower index.
-template <int PBC, bool WEIGHTED, bool EDGE, bool TABLE, int MODE, bool HOME>
+template <int PBC, bool WEIGHTED, bool EDGE, bool TABLE, int MODE, bool HOME, int HB = 8>
 __device__ __forceinline__ void cell_chunk(const float4 *__restrict__ hp, int n_home, float xj,
                                            float yj, float zj, unsigned mj, int rank,
                                            const FrameBox &b, const HistParams &p,
@@ -787,16 +791,16 @@ __device__ __forceinline__ void cell_chunk(const float4 *__restrict__ hp, int n_
             s2 = i + 2 >= rank ? nanv : s2;
             s3 = i + 3 >= rank ? nanv : s3;
         }
-        cell_count<WEIGHTED, EDGE, TABLE>(s0, __float_as_uint(a0.w), mAj, mBj, p, L, edge_local);
-        cell_count<WEIGHTED, EDGE, TABLE>(s1, __float_as_uint(a1.w), mAj, mBj, p, L, edge_local);
-        cell_count<WEIGHTED, EDGE, TABLE>(s2, __float_as_uint(a2.w), mAj, mBj, p, L, edge_local);
-        cell_count<WEIGHTED, EDGE, TABLE>(s3, __float_as_uint(a3.w), mAj, mBj, p, L, edge_local);
+        cell_count<WEIGHTED, EDGE, TABLE, HB>(s0, __float_as_uint(a0.w), mAj, mBj, p, L, edge_local);
+        cell_count<WEIGHTED, EDGE, TABLE, HB>(s1, __float_as_uint(a1.w), mAj, mBj, p, L, edge_local);
+        cell_count<WEIGHTED, EDGE, TABLE, HB>(s2, __float_as_uint(a2.w), mAj, mBj, p, L, edge_local);
+        cell_count<WEIGHTED, EDGE, TABLE, HB>(s3, __float_as_uint(a3.w), mAj, mBj, p, L, edge_local);
     }
     for (; i < n_home; ++i) {
         const float4 a0 = uload(hp, i);
         float s0 = dist(a0);
         if constexpr (HOME) s0 = i >= rank ? nanv : s0;
-        cell_count<WEIGHTED, EDGE, TABLE>(s0, __float_as_uint(a0.w), mAj, mBj, p, L, edge_local);
+        cell_count<WEIGHTED, EDGE, TABLE, HB>(s0, __float_as_uint(a0.w), mAj, mBj, p, L, edge_local);
     }
 }
 
@@ -1129,24 +1133,56 @@ __device__ __forceinline__ void cell_waves(const CellGrid &G, float4 *ring, unsi
     }
 }
 
-// Plain histogram consumer: exact bins, u64 LDS counts.
+// Plain histogram consumer: exact bins, LDS counts of HB bytes (u64, or
+// u32 for a wave-private histogram: HB = 4).
 // With EDGE (diagnostics) it also counts the pair evaluations: a valid j lane
 // meets min(rank, n_home) home atoms in a home chunk and all n_home otherwise.
-template <int PBC, bool WEIGHTED, bool EDGE, bool TABLE>
+// With HB = 4 it bounds what any counter can have gained since the last drain
+// (a chunk adds at most 2 x 64 x n_home to one bin) and drains the wave's
+// histogram into `row` (u64 global counts of its bins from p.skip on) before
+// a counter could wrap.
+template <int PBC, bool WEIGHTED, bool EDGE, bool TABLE, int HB = 8>
 struct PlainCons {
+    static_assert(HB == 8 || (HB == 4 && !WEIGHTED), "u32 counters take unit weights only");
     HistParams p;
     CellLds L;
     unsigned long long edge_local;
     unsigned long long evals_local;
+    unsigned long long *row = nullptr;
+    unsigned pend = 0;
     template <int MODE, bool HOME>
     __device__ __forceinline__ void chunk(const float4 *hp, int n_home, float xj, float yj, float zj,
                                           unsigned mj, int rank, unsigned, int, const FrameBox &b) {
-        cell_chunk<PBC, WEIGHTED, EDGE, TABLE, MODE, HOME>(hp, n_home, xj, yj, zj, mj, rank, b, p,
+        cell_chunk<PBC, WEIGHTED, EDGE, TABLE, MODE, HOME, HB>(hp, n_home, xj, yj, zj, mj, rank, b, p,
                                                                 L, edge_local);
         if constexpr (EDGE) {
             const int m = HOME ? min(rank, n_home) : n_home;
             evals_local += xj == xj ? (unsigned long long)m : 0ull;
         }
+        if constexpr (HB == 4) {
+            pend += 128u * (unsigned)n_home;
+            if (pend >= 0x80000000u) drain();
+        }
+    }
+    // Adds the wave's u32 histogram to row and clears it (every lane of the
+    // wave, uniform control flow).
+    __device__ __forceinline__ void drain() {
+        unsigned *h = (unsigned *)L.hist;
+        const int lane = threadIdx.x & 63;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        for (int i = lane; i < p.bins; i += 64) {
+            const unsigned v = h[i];
+            if (v) {
+                if (i >= p.skip) atomicAdd(row + (i - p.skip), (unsigned long long)v);
+                h[i] = 0u;
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        pend = 0;
     }
     __device__ __forceinline__ void unit_begin(int, int, int) {}
     __device__ __forceinline__ void unit_end(int, int, int) {}
