@@ -895,7 +895,9 @@ def uneven_batch_bench(ctx, dev, reps=3, autotune=True, cpu_sample=True):
             "autotune_s": tune_s, "cell_k_histogram": {str(k): int((kinds == k).sum()) for k in sorted(set(kinds.tolist()))},
             "tuning_histogram": tuning,
             "roofline": {"kernel": "batch_pair_kernel", "bound": "valu", "achieved": ach, "peak": VALU_PEAK_TOPS,
-                         "unit": "TOP/s", "frac": ach / VALU_PEAK_TOPS, "traffic": None,
+                         "unit": "TOP/s", "frac": ach / VALU_PEAK_TOPS,
+                         "traffic": traffic_per_launch("batch_pair_kernel_256_uneven_50f", 1),
+                         "traffic_measured_at": traffic_stamp(),
                          "algorithmic": f"{RDF_OPS_PER_PAIR} VALU lane-ops per unordered in-range pair x "
                                         f"{pairs_in:.4g} pairs per step",
                          "avg_launch_ms": pair_ms,
