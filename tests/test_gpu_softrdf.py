@@ -296,6 +296,18 @@ def test_sigma_outside_window_rejected():
         _plan(bins=300, r_max=12.0, sigma=0.2)
 
 
+def test_node_histogram_beyond_lds_is_refused():
+    """8,192 bins put 32,772 f64 nodes (262 KB) in a workgroup's LDS: the
+    launch is refused with a clean 'unsupported' error before any kernel runs,
+    on the brute-force and the cell-list paths."""
+    for n in (500, 6000):
+        L = (n / 0.08) ** (1.0 / 3.0)
+        x = torch.tensor(_beads(n, L, 2, seed=31), device="cuda")
+        plan = _plan(bins=8192, r_max=RMAX, sigma=0.002)
+        with pytest.raises(RuntimeError, match="unsupported"):
+            plan.counts(x, np.array([L, L, L], np.float32))
+
+
 def test_run_to_run_spread_is_bounded():
     """The soft kernels add f64 node sums and f32 gradients with atomics whose
     order varies between runs: two identical runs agree to 1e-12 relative in

@@ -598,11 +598,11 @@ __global__ __launch_bounds__(kBlock) void soft_unsort_kernel(const float *__rest
     }
 }
 
-constexpr int kSoftBlock = 768;  // 12 waves
+constexpr int kSoftBlock = 1024;  // 16 waves: two workgroups per CU, 8 waves per SIMD (registers capped for 8)
 constexpr int kSoftWaves = kSoftBlock / 64;
 
 template <int PBC, bool LT>
-__global__ __launch_bounds__(kSoftBlock) void soft_cell_fwd_kernel(CellGrid G, SoftParams sp) {
+__global__ __launch_bounds__(kSoftBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) void soft_cell_fwd_kernel(CellGrid G, SoftParams sp) {
     extern __shared__ __attribute__((aligned(16))) char lds[];
     const int hn = sp.n_nodes + kSoftPadL + kSoftPadR;
     unsigned long long *hist = (unsigned long long *)lds;

@@ -638,7 +638,7 @@ __global__ __launch_bounds__(kBlock) void cell_scatter_kernel(
 // only with home atoms of lower index (unordered pairs, weight 2).  The
 // histogram is a u64 LDS array shared by the workgroup's waves.
 // -------------------------------------------------------------------------
-constexpr int kCellBlock = 768;  // 12 waves, 3 per SIMD: two workgroups per CU within the kernel's SGPR budget
+constexpr int kCellBlock = 1024;  // 16 waves: two workgroups per CU, 8 waves per SIMD (registers capped for 8 waves)
 constexpr int kCellWaves = kCellBlock / 64;
 constexpr int kRing = 128;       // per-wave ring entries (float4 each)
 constexpr int kMaxTab2 = 5120;   // split-table entries the cell kernel keeps in LDS
@@ -1189,7 +1189,7 @@ struct PlainCons {
 };
 
 template <int PBC, bool WEIGHTED, bool EDGE, bool TABLE>
-__global__ __launch_bounds__(kCellBlock) void rdf_cell_kernel(CellGrid G, HistParams p) {
+__global__ __launch_bounds__(kCellBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) void rdf_cell_kernel(CellGrid G, HistParams p) {
     extern __shared__ __attribute__((aligned(16))) char lds[];
     const CellLds L = cell_lds_layout(lds, p, EDGE, TABLE);
     for (int i = threadIdx.x; i < p.bins; i += kCellBlock) L.hist[i] = 0ull;
