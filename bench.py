@@ -205,7 +205,7 @@ def small_box_leg(ctx, dev, reps=20):
             "counts_equal": same, "in_range_pairs": int(counts.sum())}
 
 
-def soft_rdf_ratio(ctx, dev, n=20000, frames=64, reps=10):
+def soft_rdf_ratio(ctx, dev, n=20000, frames=64, reps=20):
     """Soft-RDF forward + backward against the plain RDF forward on the same
     input: 20,000 beads at 0.08 / A^3 (cubic 62.996 A), 64 frames of a uniform
     base configuration plus N(0, 0.2^2) jitter, 300 bins, r_max 12 A, kernel

@@ -121,17 +121,21 @@ __device__ __forceinline__ int bspline2(float u, int n_max, float &t) {
 // converted once, so integer LDS atomics add them exactly and the node sums
 // of a workgroup do not depend on the order of its deposits.  A node sum
 // overflows only past 1.1e10 deposits in one workgroup.
-constexpr float kSoftFix = 1073741824.0f;  // 2^30
 constexpr double kSoftUnfix = 1.0 / 1073741824.0;
 __device__ __forceinline__ void soft_deposit(float s, const SoftParams &sp, unsigned long long *nodes) {
     const float u = __builtin_amdgcn_sqrtf(s) * sp.inv_h;
     float t;
     const int n0 = bspline2(u, sp.n_nodes, t);
-    const float a = 0.5f - t, c = 0.5f + t;
+    // The scale 2^30 is folded into the operands: scaling by a power of two
+    // commutes with rounding, so each product is bit-identical to the f32
+    // weight times 2^30 (2^15 (1/2 -+ t) is one rounding of 1/2 -+ t, scaled).
+    const float a = __builtin_fmaf(-t, 32768.0f, 16384.0f);                 // 2^15 (1/2 - t)
+    const float c = __builtin_fmaf(t, 32768.0f, 16384.0f);                  // 2^15 (1/2 + t)
+    const float b = __builtin_fmaf(t * t, -2147483648.0f, 1610612736.0f);   // 2^30 (3/2 - 2 t^2)
     unsigned long long *p = nodes + kSoftPadL + n0;
-    atomicAdd(p - 1, (unsigned long long)(unsigned)((a * a) * kSoftFix));              // 2 x (1/2 - t)^2 / 2
-    atomicAdd(p, (unsigned long long)(unsigned)((1.5f - 2.0f * t * t) * kSoftFix));    // 2 x (3/4 - t^2)
-    atomicAdd(p + 1, (unsigned long long)(unsigned)((c * c) * kSoftFix));              // 2 x (1/2 + t)^2 / 2
+    atomicAdd(p - 1, (unsigned long long)(unsigned)(a * a));  // 2 x (1/2 - t)^2 / 2
+    atomicAdd(p, (unsigned long long)(unsigned)b);            // 2 x (3/4 - t^2)
+    atomicAdd(p + 1, (unsigned long long)(unsigned)(c * c));  // 2 x (1/2 + t)^2 / 2
 }
 
 // One unordered pair's dL/dr divided by r, from the padded (A, S) table.
@@ -230,10 +234,12 @@ struct SoftFwdCons {
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                drain(64);
-                // move the entries past the first 64 to the front
+                // both halves read together (one LDS wait), the first 64
+                // deposited, the entries past them moved to the front
                 const int lane = threadIdx.x & 63;
+                const float first = pbuf[lane];
                 const float rest = pbuf[64 + lane];
+                soft_deposit(first, sp, hist);
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -273,7 +279,7 @@ struct SoftFwdCons {
 //    of the unit.
 // Home atoms past the consumer's MAXH LDS slots add their row sums to the
 // output directly (MAXH = 32 for the usual cells of about 20 atoms, 128 for
-// dense cells; the smaller LDS footprint keeps three workgroups per CU).
+// dense cells, whose LDS footprint is larger).
 // The output is indexed by cell-sorted position, where a chunk's lanes mostly
 // hold consecutive atoms, so the atomics of one instruction fall on a few
 // cache lines.  (Keeping each lane's home-atom sums in registers across the
@@ -624,7 +630,7 @@ __global__ __launch_bounds__(kSoftBlock) __attribute__((amdgpu_waves_per_eu(8, 8
     }
 }
 
-constexpr int kSoftBwdBlock = 512;  // 8 waves (three workgroups per CU within the LDS budget at 300 bins)
+constexpr int kSoftBwdBlock = 512;  // 8 waves: two workgroups per CU, 126 VGPRs without spills
 constexpr int kSoftBwdWaves = kSoftBwdBlock / 64;
 
 // LDS of the backward cell kernel: (A, S) table | per wave: ring[kRing]
@@ -634,7 +640,7 @@ __host__ __device__ inline size_t soft_bwd_lds_bytes(int tn, int maxh) {
 }
 
 template <int PBC, bool LT, int MAXH>
-__global__ __launch_bounds__(kSoftBwdBlock) __attribute__((amdgpu_waves_per_eu(6))) void soft_cell_bwd_kernel(CellGrid G, SoftParams sp) {
+__global__ __launch_bounds__(kSoftBwdBlock) void soft_cell_bwd_kernel(CellGrid G, SoftParams sp) {
     constexpr int kGiRow = soft_gi_row(MAXH);
     extern __shared__ __attribute__((aligned(16))) char lds[];
     const int tn = sp.n_nodes + kSoftPadL + kSoftPadR;
