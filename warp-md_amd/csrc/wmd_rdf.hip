@@ -1389,7 +1389,12 @@ void accumulate_device(wmd_rdf *h, const float *d_soa, size_t stride, size_t n_f
     // cells per side (r_max is below half the smallest plane spacing here)
     const int shift_ok = (nc[0] >= 2 * k_use + 1 && nc[1] >= 2 * k_use + 1 && nc[2] >= 2 * k_use + 1) ? 1 : 0;
     const float cull2 = cull2_for((double)h->r_max, B);
-    const size_t batch = h->cells.frames_for(n, ncell, n_frames, false);
+    // frames per build: a whole number of CUs' worth when the scratch holds
+    // more than one (the build runs one workgroup per frame and per CU, so
+    // 276 frames take two rounds where 256 take one)
+    size_t batch = h->cells.frames_for(n, ncell, n_frames, false);
+    const size_t cus = (size_t)std::max(1, h->ctx->cu_count);
+    if (batch < n_frames && batch > cus) batch = batch / cus * cus;
     h->cells.reserve(n, ncell, batch, false);
     const unsigned *d_w = h->weighted ? h->d_wts.as<unsigned>() : nullptr;
     bool cell_table = table && (int)h->tab.size() <= kMaxTab2;
