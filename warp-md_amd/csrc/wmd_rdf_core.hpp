@@ -1329,7 +1329,7 @@ struct CellScratch {
     DevBuf order;  // [F][n] store slot -> sorted position (want_order)
     size_t frames_for(int n, int ncell, size_t n_frames, bool want_order) const {
         const size_t per_frame = (size_t)n * (16 + 12 + (want_order ? 4 : 0)) + (size_t)(2 * ncell + 2) * 4;
-        return std::max<size_t>(1, std::min<size_t>(n_frames, (size_t)(768ull << 20) / per_frame));
+        return std::max<size_t>(1, std::min<size_t>(n_frames, (size_t)(1536ull << 20) / per_frame));
     }
     void reserve(int n, int ncell, size_t batch, bool want_order) {
         cell_of.reserve((size_t)n * batch * 4);
