@@ -1228,7 +1228,7 @@ void launch_brute_t(wmd_rdf *h, hipStream_t st, const float *soa, long long fram
 
 template <int PBC, bool WEIGHTED, bool EDGE, bool TABLE>
 void launch_cells_t(wmd_rdf *h, hipStream_t st, const float4 *sorted, const unsigned *start,
-                    const unsigned *frame_flag, int n, int nf, const FrameBox *boxes,
+                    const unsigned *frame_flag, const float4 *bbox, int n, int nf, const FrameBox *boxes,
                     const int *nc, int k, int shift_ok, float cull2, const HistParams &p) {
     auto kern = rdf_cell_kernel<PBC, WEIGHTED, EDGE, TABLE>;
     ProfScope prof(h->ctx, st, "rdf_pair");
@@ -1250,7 +1250,7 @@ void launch_cells_t(wmd_rdf *h, hipStream_t st, const float4 *sorted, const unsi
     // otherwise stalls the waves of a long launch); short launches keep
     // grabs of 2 so that the last grabs stay balanced
     const int grab = h->grab > 0 ? h->grab : n_units >= 64LL * grid * kCellWaves ? 4 : 2;
-    CellGrid G{sorted, start, frame_flag, boxes, n, nf, nc[0], nc[1], nc[2], k, shift_ok, 8, cull2, queue, grab};
+    CellGrid G{sorted, start, frame_flag, boxes, n, nf, nc[0], nc[1], nc[2], k, shift_ok, 8, cull2, queue, grab, bbox};
     hipLaunchKernelGGL(kern, dim3(grid), dim3(kCellBlock), lds, st, G, p);
     WMD_HIP_CHECK(hipGetLastError());
 }
@@ -1404,10 +1404,11 @@ void accumulate_device(wmd_rdf *h, const float *d_soa, size_t stride, size_t n_f
     for (size_t f0 = 0; f0 < n_frames; f0 += batch) {
         const int nf = (int)std::min(batch, n_frames - f0);
         const FrameBox *bx = d_boxes + f0;
-        build_cells(h->ctx, st, h->pbc, d_soa, (int)stride, n, nf, (long long)f0, bx, nc, d_w, h->cells, false);
+        const float4 *bbox = build_cells(h->ctx, st, h->pbc, d_soa, (int)stride, n, nf, (long long)f0, bx, nc, d_w,
+                                         h->cells, false);
         for (const HistParams &q : passes)
             cells_dispatch(h->pbc, h->weighted, edge, cell_table, h, st, h->cells.sorted.as<float4>(),
-                           (const unsigned *)h->cells.start.ptr, (const unsigned *)h->cells.flag.ptr, n, nf,
+                           (const unsigned *)h->cells.start.ptr, (const unsigned *)h->cells.flag.ptr, bbox, n, nf,
                            bx, (const int *)nc, k_use, shift_ok, cull2, q);
     }
     h->last_path = 2;
@@ -2071,27 +2072,29 @@ void soft_run(wmd_softrdf *h, bool backward, const float *d_soa, size_t stride, 
     // with keep_cells the forward also writes the slot map, and a backward
     // on the same store, sizes and boxes takes the forward's cell lists
     const bool want_order = backward || h->keep_cells;
-    const size_t batch = h->cells.frames_for(n, ncell, n_frames, want_order);
+    const size_t batch = h->cells.frames_for(n, ncell, n_frames, want_order, true);
     const bool reuse = backward && h->keep_cells && h->cells_valid && h->cells_soa == d_soa &&
                        h->cells_stride == stride && h->cells_n == n_atoms && h->cells_frames == n_frames &&
                        h->cells_nc[0] == pl.nc[0] && h->cells_nc[1] == pl.nc[1] && h->cells_nc[2] == pl.nc[2];
     if (!reuse) {
         h->cells_valid = false;
-        h->cells.reserve(n, ncell, batch, want_order);
+        h->cells.reserve(n, ncell, batch, want_order, true);
     }
     for (size_t f0 = 0; f0 < n_frames; f0 += batch) {
         const int nf = (int)std::min(batch, n_frames - f0);
         const FrameBox *bx = d_boxes + f0;
         if (!reuse)
             build_cells(h->ctx, st, h->pbc, d_soa, (int)stride, n, nf, (long long)f0, bx, pl.nc, nullptr,
-                        h->cells, want_order);
+                        h->cells, want_order, true);
+        // the per-cell boxes of this batch's (or the reused) build, when its path writes them
+        const float4 *bbox = ncell <= kBuildMaxCells ? h->cells.bbox.as<float4>() : nullptr;
         // dynamic unit schedule: the per-XCD grab counters start at zero
         unsigned long long *queue = (unsigned long long *)h->d_queue.reserve(8 * 8);
         hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(kBlock), 0, st, queue, 8LL);
         WMD_HIP_CHECK(hipGetLastError());
         CellGrid G{h->cells.sorted.as<float4>(), (const unsigned *)h->cells.start.ptr,
                    (const unsigned *)h->cells.flag.ptr, bx, n, nf, pl.nc[0], pl.nc[1], pl.nc[2], pl.k,
-                   shift_ok, 8, cull2, queue, 1};
+                   shift_ok, 8, cull2, queue, 1, bbox};
         if (backward) {
             // per-batch gradient in cell order, scattered to the store afterwards
             sp.grad = (float *)h->d_gsorted.reserve((size_t)3 * n * nf * 4);

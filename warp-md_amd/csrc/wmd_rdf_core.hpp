@@ -506,7 +506,8 @@ __device__ __forceinline__ void build_frame(const float *__restrict__ X, int str
                                             const unsigned *__restrict__ wts,
                                             unsigned *__restrict__ start, float4 *__restrict__ sorted,
                                             unsigned *__restrict__ order, unsigned *frame_flag,
-                                            unsigned *cnt, unsigned *part, unsigned *far) {
+                                            unsigned *cnt, unsigned *part, unsigned *far,
+                                            float4 *__restrict__ bbox = nullptr) {
     const int ncell = ncx * ncy * ncz;
     for (int c = threadIdx.x; c < ncell; c += kBuildBlock) cnt[c] = 0u;
     if (threadIdx.x == 0) *far = 0u;
@@ -574,6 +575,40 @@ __device__ __forceinline__ void build_frame(const float *__restrict__ X, int str
             if (order) order[i] = pos;  // store slot -> sorted position
         }
     }
+    if (bbox) {
+        // per cell: the bounding box of its atoms' raw coordinates and whether
+        // they share one image index (bbox[2c] = min x, y, z and the flag
+        // 1.0 when they do, bbox[2c + 1] = max x, y, z); the cell kernels read
+        // it instead of reducing over the home atoms in every unit.  After
+        // pass 2, cnt[c] is the end of cell c; this thread wrote start[c].
+        // The first kBuildUnroll atoms of a cell are loaded together (the
+        // last one repeated past the cell's end, harmless for a min or max).
+        __syncthreads();
+        for (int c = lo; c < hi; ++c) {
+            const unsigned beg = start[c], end = cnt[c];
+            if (beg == end) continue;
+            float4 q[kBuildUnroll];
+#pragma unroll
+            for (int u = 0; u < kBuildUnroll; ++u) q[u] = sorted[min(beg + (unsigned)u, end - 1u)];
+            const unsigned m0 = __float_as_uint(q[0].w) & 0xFFFFFFu;
+            float mnx = q[0].x, mny = q[0].y, mnz = q[0].z, mxx = q[0].x, mxy = q[0].y, mxz = q[0].z;
+            bool same = true;
+#pragma unroll
+            for (int u = 1; u < kBuildUnroll; ++u) {
+                mnx = fminf(mnx, q[u].x); mny = fminf(mny, q[u].y); mnz = fminf(mnz, q[u].z);
+                mxx = fmaxf(mxx, q[u].x); mxy = fmaxf(mxy, q[u].y); mxz = fmaxf(mxz, q[u].z);
+                same = same && (__float_as_uint(q[u].w) & 0xFFFFFFu) == m0;
+            }
+            for (unsigned a = beg + kBuildUnroll; a < end; ++a) {
+                const float4 r = sorted[a];
+                mnx = fminf(mnx, r.x); mny = fminf(mny, r.y); mnz = fminf(mnz, r.z);
+                mxx = fmaxf(mxx, r.x); mxy = fmaxf(mxy, r.y); mxz = fmaxf(mxz, r.z);
+                same = same && (__float_as_uint(r.w) & 0xFFFFFFu) == m0;
+            }
+            bbox[2 * c] = make_float4(mnx, mny, mnz, same ? 1.0f : 0.0f);
+            bbox[2 * c + 1] = make_float4(mxx, mxy, mxz, 0.0f);
+        }
+    }
 }
 
 template <int PBC>
@@ -581,7 +616,7 @@ __global__ __launch_bounds__(kBuildBlock) void cell_build_frame_kernel(
     const float *__restrict__ soa, long long frame_stride, int stride, int n, long long frame0,
     const FrameBox *__restrict__ boxes, int ncx, int ncy, int ncz, const unsigned *__restrict__ wts,
     unsigned *__restrict__ start, float4 *__restrict__ sorted, unsigned *__restrict__ order,
-    unsigned *__restrict__ frame_flag) {
+    unsigned *__restrict__ frame_flag, float4 *__restrict__ bbox) {
     extern __shared__ __attribute__((aligned(16))) char lds[];
     unsigned *part = (unsigned *)lds;
     unsigned *far = part + kBuildBlock;
@@ -590,7 +625,7 @@ __global__ __launch_bounds__(kBuildBlock) void cell_build_frame_kernel(
     const int ncell = ncx * ncy * ncz;
     build_frame<PBC>(soa + (frame0 + f) * frame_stride, stride, n, boxes[f], ncx, ncy, ncz, wts,
                      start + f * (ncell + 1), sorted + f * n, order ? order + f * n : nullptr,
-                     frame_flag + f, cnt, part, far);
+                     frame_flag + f, cnt, part, far, bbox ? bbox + f * 2 * ncell : nullptr);
 }
 
 __host__ __device__ inline size_t build_lds_bytes(int ncell) {
@@ -814,6 +849,7 @@ struct CellGrid {
     float cull2;                // squared cull radius (pair reach plus rounding margin)
     unsigned long long *queue;  // [n_xcd] zeroed grab counters (dynamic schedule), or null
     int grab;                   // units per grab of the dynamic schedule (0: kCellGrab)
+    const float4 *bbox = nullptr;  // [F][ncell][2] per-cell bounding boxes from the build, or null
 };
 
 // A consumer declaring kPreShift = true moves the chunk's j atoms to their
@@ -909,27 +945,38 @@ __device__ __forceinline__ void cell_unit(const CellGrid &G, int f, int h, const
         cons.unit_begin(f, hs, n_home);
 
         // home atoms: bounding box and image uniformity (the box is only used
-        // on the shift path, whose frames hold finite coordinates)
+        // on the shift path, whose frames hold finite coordinates), from the
+        // build's per-cell boxes when it wrote them
         const unsigned m0 = __float_as_uint(uload(hp, 0).w) & 0xFFFFFFu;
-        int kn0 = 0x7FFFFFFF, kn1 = 0x7FFFFFFF, kn2 = 0x7FFFFFFF;
-        int kx0 = (int)0x80000000, kx1 = (int)0x80000000, kx2 = (int)0x80000000;
-        bool diff = false;
-        for (int hb = 0; hb < n_home; hb += 64) {
-            if (hb + lane < n_home) {
-                const float4 q = hp[hb + lane];
-                const int k0 = f_key(q.x), k1 = f_key(q.y), k2 = f_key(q.z);
-                kn0 = min(kn0, k0); kx0 = max(kx0, k0);
-                kn1 = min(kn1, k1); kx1 = max(kx1, k1);
-                kn2 = min(kn2, k2); kx2 = max(kx2, k2);
-                diff |= (__float_as_uint(q.w) & 0xFFFFFFu) != m0;
-            }
-        }
         // far: the frame has atoms beyond kImageLimit box lengths (or non-finite
         // ones); its pairs take the checked wrap, exact for any quotient
         const bool far = uload(G.frame_flag, f) != 0u;
-        const bool use_shift = G.shift_ok && !far && !__any(diff);
-        const float mn0 = wave_min_key(kn0), mn1 = wave_min_key(kn1), mn2 = wave_min_key(kn2);
-        const float mx0 = wave_max_key(kx0), mx1 = wave_max_key(kx1), mx2 = wave_max_key(kx2);
+        float mn0, mn1, mn2, mx0, mx1, mx2;
+        bool use_shift;
+        if (G.bbox) {
+            const float4 lo4 = uload(G.bbox, ((long long)f * ncell + h) * 2);
+            const float4 hi4 = uload(G.bbox, ((long long)f * ncell + h) * 2 + 1);
+            mn0 = lo4.x; mn1 = lo4.y; mn2 = lo4.z;
+            mx0 = hi4.x; mx1 = hi4.y; mx2 = hi4.z;
+            use_shift = G.shift_ok && !far && lo4.w != 0.0f;
+        } else {
+            int kn0 = 0x7FFFFFFF, kn1 = 0x7FFFFFFF, kn2 = 0x7FFFFFFF;
+            int kx0 = (int)0x80000000, kx1 = (int)0x80000000, kx2 = (int)0x80000000;
+            bool diff = false;
+            for (int hb = 0; hb < n_home; hb += 64) {
+                if (hb + lane < n_home) {
+                    const float4 q = hp[hb + lane];
+                    const int k0 = f_key(q.x), k1 = f_key(q.y), k2 = f_key(q.z);
+                    kn0 = min(kn0, k0); kx0 = max(kx0, k0);
+                    kn1 = min(kn1, k1); kx1 = max(kx1, k1);
+                    kn2 = min(kn2, k2); kx2 = max(kx2, k2);
+                    diff |= (__float_as_uint(q.w) & 0xFFFFFFu) != m0;
+                }
+            }
+            use_shift = G.shift_ok && !far && !__any(diff);
+            mn0 = wave_min_key(kn0); mn1 = wave_min_key(kn1); mn2 = wave_min_key(kn2);
+            mx0 = wave_max_key(kx0); mx1 = wave_max_key(kx1); mx2 = wave_max_key(kx2);
+        }
         // per-piece byte constants K: n + 128 = (m_j + 32) + K per byte, with
         // K = 128 - w - (m_home + 32) (no carries: every byte stays below 256)
         const unsigned k_home = ((128u - (m0 & 0xFFu)) & 0xFFu) | (((128u - ((m0 >> 8) & 0xFFu)) & 0xFFu) << 8) |
@@ -1325,13 +1372,14 @@ inline BinTables make_bin_tables(int bins, float r_max) {
 
 // Scratch of the cell build.
 struct CellScratch {
-    DevBuf cell_of, cell_slot, image, count, start, flag, sorted;
+    DevBuf cell_of, cell_slot, image, count, start, flag, sorted, bbox;
     DevBuf order;  // [F][n] store slot -> sorted position (want_order)
-    size_t frames_for(int n, int ncell, size_t n_frames, bool want_order) const {
-        const size_t per_frame = (size_t)n * (16 + 12 + (want_order ? 4 : 0)) + (size_t)(2 * ncell + 2) * 4;
+    size_t frames_for(int n, int ncell, size_t n_frames, bool want_order, bool want_bbox = false) const {
+        const size_t per_frame = (size_t)n * (16 + 12 + (want_order ? 4 : 0)) + (size_t)(2 * ncell + 2) * 4 +
+                                 (want_bbox ? (size_t)ncell * 32 : 0);
         return std::max<size_t>(1, std::min<size_t>(n_frames, (size_t)(1536ull << 20) / per_frame));
     }
-    void reserve(int n, int ncell, size_t batch, bool want_order) {
+    void reserve(int n, int ncell, size_t batch, bool want_order, bool want_bbox = false) {
         cell_of.reserve((size_t)n * batch * 4);
         cell_slot.reserve((size_t)n * batch * 4);
         image.reserve((size_t)n * batch * 4);
@@ -1339,6 +1387,7 @@ struct CellScratch {
         start.reserve((size_t)(ncell + 1) * batch * 4);
         flag.reserve(batch * 4);
         sorted.reserve((size_t)n * batch * 16);
+        if (want_bbox) bbox.reserve((size_t)ncell * batch * 32);
         if (want_order) order.reserve((size_t)n * batch * 4);
     }
 };
@@ -1446,21 +1495,28 @@ float cull2_for(double reach, const BoxSet &B) {
 }
 
 // Sorts frames [f0, f0 + nf) of an SoA store into cells (bx: their boxes).
-void build_cells(wmd_ctx *ctx, hipStream_t st, int pbc, const float *d_soa, int stride, int n,
-                 int nf, long long f0, const FrameBox *bx, const int *nc, const unsigned *d_w,
-                 CellScratch &S, bool want_order) {
+// With want_bbox the one-workgroup-per-frame build also writes per-cell
+// bounding boxes (S.bbox), returned; null otherwise and on the three-kernel
+// path for grids beyond kBuildMaxCells.  The extra pass re-reads the sorted
+// frame, which costs the plain histogram more than its units' own box
+// reductions; the soft-RDF kernels, whose forward and backward share one
+// build and whose units are heavier, take the boxes.
+const float4 *build_cells(wmd_ctx *ctx, hipStream_t st, int pbc, const float *d_soa, int stride, int n,
+                          int nf, long long f0, const FrameBox *bx, const int *nc, const unsigned *d_w,
+                          CellScratch &S, bool want_order, bool want_bbox = false) {
     ProfScope build_prof(ctx, st, "cell_build");
     const int ncell = nc[0] * nc[1] * nc[2];
     const long long fs = 3 * (long long)stride;
     if (ncell <= kBuildMaxCells) {
         const size_t lds = build_lds_bytes(ncell);
         unsigned *ord = want_order ? S.order.as<unsigned>() : (unsigned *)nullptr;
+        float4 *bb = want_bbox ? S.bbox.as<float4>() : (float4 *)nullptr;
         if (pbc == WMD_PBC_ORTHO)
-            hipLaunchKernelGGL(cell_build_frame_kernel<WMD_PBC_ORTHO>, dim3(nf), dim3(kBuildBlock), lds, st, d_soa, fs, stride, n, f0, bx, nc[0], nc[1], nc[2], d_w, S.start.as<unsigned>(), S.sorted.as<float4>(), ord, S.flag.as<unsigned>());
+            hipLaunchKernelGGL(cell_build_frame_kernel<WMD_PBC_ORTHO>, dim3(nf), dim3(kBuildBlock), lds, st, d_soa, fs, stride, n, f0, bx, nc[0], nc[1], nc[2], d_w, S.start.as<unsigned>(), S.sorted.as<float4>(), ord, S.flag.as<unsigned>(), bb);
         else
-            hipLaunchKernelGGL(cell_build_frame_kernel<WMD_PBC_TRICLINIC>, dim3(nf), dim3(kBuildBlock), lds, st, d_soa, fs, stride, n, f0, bx, nc[0], nc[1], nc[2], d_w, S.start.as<unsigned>(), S.sorted.as<float4>(), ord, S.flag.as<unsigned>());
+            hipLaunchKernelGGL(cell_build_frame_kernel<WMD_PBC_TRICLINIC>, dim3(nf), dim3(kBuildBlock), lds, st, d_soa, fs, stride, n, f0, bx, nc[0], nc[1], nc[2], d_w, S.start.as<unsigned>(), S.sorted.as<float4>(), ord, S.flag.as<unsigned>(), bb);
         WMD_HIP_CHECK(hipGetLastError());
-        return;
+        return bb;
     }
     unsigned *d_count = S.count.as<unsigned>(), *d_fflag = S.flag.as<unsigned>();
     int g;
@@ -1481,6 +1537,7 @@ void build_cells(wmd_ctx *ctx, hipStream_t st, int pbc, const float *d_soa, int 
                        (const unsigned *)S.image.ptr, (const unsigned *)S.start.ptr, ncell, d_w,
                        S.sorted.as<float4>(), want_order ? S.order.as<unsigned>() : (unsigned *)nullptr);
     WMD_HIP_CHECK(hipGetLastError());
+    return nullptr;
 }
 
 }  // namespace
