@@ -1013,18 +1013,22 @@ __device__ __forceinline__ void cell_unit(const CellGrid &G, int f, int h, const
             pk = k_home + (unsigned)(-wx) + ((unsigned)(-wy) << 8) + ((unsigned)(-wz) << 16);
         }
         // stream order: home, runs without a box wrap, runs with one (so that
-        // chunks of unshifted atoms come first); permuted through the empty ring
+        // chunks of unshifted atoms come first), empty runs dropped (most of
+        // the wrapped halves are empty); permuted through the empty ring
+        int n_runs;  // non-empty runs after the home cell
         {
-            const bool act = lane <= n_pieces;
+            const bool act = lane <= n_pieces && plen > 0;
             const bool wrapped = pk != k_home;
             const unsigned long long b0m = __ballot(act && !wrapped);
             const unsigned long long b1m = __ballot(act && wrapped);
-            const int dst = wrapped ? (int)__popcll(b0m) + lanes_below(b1m) : lanes_below(b0m);
+            const int n0 = (int)__popcll(b0m);
+            n_runs = n0 + (int)__popcll(b1m) - 1;
+            const int dst = wrapped ? n0 + lanes_below(b1m) : lanes_below(b0m);
             if (act) ring[dst] = make_float4(__int_as_float(pbeg), __int_as_float(plen), __uint_as_float(pk), 0.f);
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            if (act) {
+            if (lane <= n_runs) {
                 const float4 t = ring[lane];
                 pbeg = __float_as_int(t.x);
                 plen = __float_as_int(t.y);
@@ -1042,7 +1046,7 @@ __device__ __forceinline__ void cell_unit(const CellGrid &G, int f, int h, const
         unsigned c_k = k_home;
         int P0 = 0;
         while (true) {
-            while (c_end <= P0 && c < n_pieces) {
+            while (c_end <= P0 && c < n_runs) {
                 ++c;
                 c_beg = __builtin_amdgcn_readlane(pbeg, c);
                 c_off = c_end;
@@ -1055,7 +1059,7 @@ __device__ __forceinline__ void cell_unit(const CellGrid &G, int f, int h, const
                 int src = c_beg + (pos - c_off);
                 unsigned kk = c_k;
                 int e = c, e_end = c_end;
-                while (e_end < P0 + 64 && e < n_pieces) {
+                while (e_end < P0 + 64 && e < n_runs) {
                     ++e;
                     const int eb = __builtin_amdgcn_readlane(pbeg, e);
                     const int el = __builtin_amdgcn_readlane(plen, e);
