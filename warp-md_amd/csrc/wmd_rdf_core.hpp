@@ -725,35 +725,39 @@ __device__ __forceinline__ unsigned cell_bin8(float s, const CellLds &L, const H
     }
 }
 
-// Wave-wide min / max of non-NaN floats through DPP row operations on their
-// order-preserving integer keys; every lane must be active.
-__device__ __forceinline__ int f_key(float x) {
-    const int b = __float_as_int(x);
-    return b ^ ((b >> 31) & 0x7FFFFFFF);
+// Wave-wide min of (a0, a1, a2) and max of (b0, b1, b2); every lane must be
+// active.  Six DPP row steps per value, each one v_min_f32 / v_max_f32 that
+// reads the permuted operand (quad swaps, half-row and row mirrors, then
+// row_bcast:15 into rows 1 and 3 and row_bcast:31 into rows 2 and 3, whose
+// other rows keep their value), after which lane 63 holds the result.  The six
+// chains are interleaved, so each DPP read comes five instructions after the
+// write of its operand (two wait states are required).
+#define WMD_BBOX_STEP(ctrl)                      \
+    "v_min_f32_dpp %0, %0, %0 " ctrl "\n\t"      \
+    "v_min_f32_dpp %1, %1, %1 " ctrl "\n\t"      \
+    "v_min_f32_dpp %2, %2, %2 " ctrl "\n\t"      \
+    "v_max_f32_dpp %3, %3, %3 " ctrl "\n\t"      \
+    "v_max_f32_dpp %4, %4, %4 " ctrl "\n\t"      \
+    "v_max_f32_dpp %5, %5, %5 " ctrl "\n\t"
+__device__ __forceinline__ void wave_bbox(float &a0, float &a1, float &a2, float &b0, float &b1,
+                                          float &b2) {
+    asm volatile("s_nop 1\n\t"
+                 WMD_BBOX_STEP("quad_perm:[1,0,3,2] row_mask:0xf bank_mask:0xf")
+                 WMD_BBOX_STEP("quad_perm:[2,3,0,1] row_mask:0xf bank_mask:0xf")
+                 WMD_BBOX_STEP("row_half_mirror row_mask:0xf bank_mask:0xf")
+                 WMD_BBOX_STEP("row_mirror row_mask:0xf bank_mask:0xf")
+                 WMD_BBOX_STEP("row_bcast:15 row_mask:0xa bank_mask:0xf")
+                 WMD_BBOX_STEP("row_bcast:31 row_mask:0xc bank_mask:0xf")
+                 "s_nop 1"
+                 : "+v"(a0), "+v"(a1), "+v"(a2), "+v"(b0), "+v"(b1), "+v"(b2));
+    a0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(a0), 63));
+    a1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(a1), 63));
+    a2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(a2), 63));
+    b0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(b0), 63));
+    b1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(b1), 63));
+    b2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(b2), 63));
 }
-__device__ __forceinline__ float key_f(int k) { return __int_as_float(k ^ ((k >> 31) & 0x7FFFFFFF)); }
-template <int CTRL, int ROWS>
-__device__ __forceinline__ int dpp_i(int v) {
-    return __builtin_amdgcn_update_dpp(v, v, CTRL, ROWS, 0xF, false);
-}
-__device__ __forceinline__ float wave_min_key(int v) {
-    v = min(v, dpp_i<0xB1, 0xF>(v));   // quad_perm [1,0,3,2]
-    v = min(v, dpp_i<0x4E, 0xF>(v));   // quad_perm [2,3,0,1]
-    v = min(v, dpp_i<0x141, 0xF>(v));  // row_half_mirror
-    v = min(v, dpp_i<0x140, 0xF>(v));  // row_mirror
-    v = min(v, dpp_i<0x142, 0xA>(v));  // row_bcast:15 into rows 1, 3
-    v = min(v, dpp_i<0x143, 0xC>(v));  // row_bcast:31 into rows 2, 3
-    return key_f(__builtin_amdgcn_readlane(v, 63));
-}
-__device__ __forceinline__ float wave_max_key(int v) {
-    v = max(v, dpp_i<0xB1, 0xF>(v));
-    v = max(v, dpp_i<0x4E, 0xF>(v));
-    v = max(v, dpp_i<0x141, 0xF>(v));
-    v = max(v, dpp_i<0x140, 0xF>(v));
-    v = max(v, dpp_i<0x142, 0xA>(v));
-    v = max(v, dpp_i<0x143, 0xC>(v));
-    return key_f(__builtin_amdgcn_readlane(v, 63));
-}
+#undef WMD_BBOX_STEP
 
 // Number of set bits of m below this lane.
 __device__ __forceinline__ int lanes_below(unsigned long long m) {
@@ -960,22 +964,22 @@ __device__ __forceinline__ void cell_unit(const CellGrid &G, int f, int h, const
             mx0 = hi4.x; mx1 = hi4.y; mx2 = hi4.z;
             use_shift = G.shift_ok && !far && lo4.w != 0.0f;
         } else {
-            int kn0 = 0x7FFFFFFF, kn1 = 0x7FFFFFFF, kn2 = 0x7FFFFFFF;
-            int kx0 = (int)0x80000000, kx1 = (int)0x80000000, kx2 = (int)0x80000000;
+            // (only used when every coordinate of the frame is finite)
+            const float inf = __builtin_inff();
+            mn0 = inf; mn1 = inf; mn2 = inf;
+            mx0 = -inf; mx1 = -inf; mx2 = -inf;
             bool diff = false;
             for (int hb = 0; hb < n_home; hb += 64) {
                 if (hb + lane < n_home) {
                     const float4 q = hp[hb + lane];
-                    const int k0 = f_key(q.x), k1 = f_key(q.y), k2 = f_key(q.z);
-                    kn0 = min(kn0, k0); kx0 = max(kx0, k0);
-                    kn1 = min(kn1, k1); kx1 = max(kx1, k1);
-                    kn2 = min(kn2, k2); kx2 = max(kx2, k2);
+                    mn0 = fminf(mn0, q.x); mx0 = fmaxf(mx0, q.x);
+                    mn1 = fminf(mn1, q.y); mx1 = fmaxf(mx1, q.y);
+                    mn2 = fminf(mn2, q.z); mx2 = fmaxf(mx2, q.z);
                     diff |= (__float_as_uint(q.w) & 0xFFFFFFu) != m0;
                 }
             }
             use_shift = G.shift_ok && !far && !__any(diff);
-            mn0 = wave_min_key(kn0); mn1 = wave_min_key(kn1); mn2 = wave_min_key(kn2);
-            mx0 = wave_max_key(kx0); mx1 = wave_max_key(kx1); mx2 = wave_max_key(kx2);
+            wave_bbox(mn0, mn1, mn2, mx0, mx1, mx2);
         }
         // per-piece byte constants K: n + 128 = (m_j + 32) + K per byte, with
         // K = 128 - w - (m_home + 32) (no carries: every byte stays below 256)
@@ -1071,8 +1075,7 @@ __device__ __forceinline__ void cell_unit(const CellGrid &G, int f, int h, const
                     e_end += el;
                 }
                 const bool valid = pos < e_end;
-                float4 q = make_float4(nanv, nanv, nanv, 0.f);
-                if (valid) q = fp[src];
+                const float4 q = fp[valid ? src : hs];  // lanes past the stream end load the home start
                 unsigned meta = __float_as_uint(q.w);
                 bool keep = valid;
                 if (!use_shift) {
