@@ -209,7 +209,7 @@ struct BatchPairArgs {
 
 __host__ __device__ inline size_t batch_lds_bytes(int bins, int tab_n) {
     return align16((size_t)kBatchWaves * bins * 4) + align16((size_t)tab_n * 8) +
-           align16(((size_t)bins + 1) * 4) + (size_t)kBatchWaves * kRing * 16;
+           align16(((size_t)bins + 1) * 4) + (size_t)kBatchWaves * kRingStride * 16;
 }
 
 // Brute-force unit: home block [I0, I0 + bh) of frame f against every j > i.
@@ -263,7 +263,7 @@ __global__ __launch_bounds__(kBatchBlock) void batch_pair_kernel(BatchPairArgs a
     L.thr = thr;
     L.elo = thr;
     L.ehi = thr;
-    L.ring = rings + wave * kRing;
+    L.ring = rings + wave * kRingStride;
     // u32 counters (table offsets 4 x bin), drained into the trajectory's row
     PlainCons<PBC, false, false, TABLE, 4> cons{p, L, 0ull, 0ull};
 

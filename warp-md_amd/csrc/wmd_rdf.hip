@@ -618,7 +618,7 @@ __global__ __launch_bounds__(kSoftBlock) __attribute__((amdgpu_waves_per_eu(8, 8
     __syncthreads();
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     SoftFwdCons<PBC, LT> cons{sp, hist, pbufs + wave * 128, 0};
-    cell_waves<PBC, false>(G, rings + wave * kRing, nullptr, wave, cons);
+    cell_waves<PBC, false>(G, rings + wave * kRingStride, nullptr, wave, cons);
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -633,10 +633,10 @@ __global__ __launch_bounds__(kSoftBlock) __attribute__((amdgpu_waves_per_eu(8, 8
 constexpr int kSoftBwdBlock = 512;  // 8 waves: two workgroups per CU, 126 VGPRs without spills
 constexpr int kSoftBwdWaves = kSoftBwdBlock / 64;
 
-// LDS of the backward cell kernel: (A, S) table | per wave: ring[kRing]
+// LDS of the backward cell kernel: (A, S) table | per wave: ring[kRingStride]
 // float4, ring positions [kRing], the home-atom row sums [4][soft_gi_row(maxh)].
 __host__ __device__ inline size_t soft_bwd_lds_bytes(int tn, int maxh) {
-    return align16((size_t)tn * 8) + (size_t)kSoftBwdWaves * (kRing * 16 + kRing * 4 + 4 * soft_gi_row(maxh) * 4);
+    return align16((size_t)tn * 8) + (size_t)kSoftBwdWaves * (kRingStride * 16 + kRing * 4 + 4 * soft_gi_row(maxh) * 4);
 }
 
 template <int PBC, bool LT, int MAXH>
@@ -646,14 +646,14 @@ __global__ __launch_bounds__(kSoftBwdBlock) void soft_cell_bwd_kernel(CellGrid G
     const int tn = sp.n_nodes + kSoftPadL + kSoftPadR;
     float2 *tab = (float2 *)lds;
     float4 *rings = (float4 *)(lds + align16((size_t)tn * 8));
-    unsigned *ringps = (unsigned *)(rings + kSoftBwdWaves * kRing);
+    unsigned *ringps = (unsigned *)(rings + kSoftBwdWaves * kRingStride);
     float *gis = (float *)(ringps + kSoftBwdWaves * kRing);
     for (int i = threadIdx.x; i < tn; i += kSoftBwdBlock) tab[i] = sp.table[i];
     for (int i = threadIdx.x; i < kSoftBwdWaves * 4 * kGiRow; i += kSoftBwdBlock) gis[i] = 0.f;
     __syncthreads();
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     SoftBwdCons<PBC, LT, MAXH> cons{sp, tab, gis + wave * 4 * kGiRow, 0};
-    cell_waves<PBC, true>(G, rings + wave * kRing, ringps + wave * kRing, wave, cons);
+    cell_waves<PBC, true>(G, rings + wave * kRingStride, ringps + wave * kRing, wave, cons);
 }
 
 // Brute-force soft kernels (boxes too small for the cell list): one thread per
@@ -2104,7 +2104,7 @@ void soft_run(wmd_softrdf *h, bool backward, const float *d_soa, size_t stride, 
         }
         if (!backward) {
             const size_t lds = align16((size_t)tn * 8) + (size_t)kSoftWaves * 128 * 4 +
-                               (size_t)kSoftWaves * kRing * 16;
+                               (size_t)kSoftWaves * kRingStride * 16;
             if (h->pbc == WMD_PBC_ORTHO)
                 launch_soft_cells(h, st, soft_cell_fwd_kernel<WMD_PBC_ORTHO, false>, kSoftBlock, lds, G, sp, "softrdf_fwd");
             else if (pl.lt)

@@ -676,18 +676,19 @@ __global__ __launch_bounds__(kBlock) void cell_scatter_kernel(
 constexpr int kCellBlock = 1024;  // 16 waves: two workgroups per CU, 8 waves per SIMD (registers capped for 8 waves)
 constexpr int kCellWaves = kCellBlock / 64;
 constexpr int kRing = 128;       // per-wave ring entries (float4 each)
-constexpr int kMaxTab2 = 5120;   // split-table entries the cell kernel keeps in LDS
+constexpr int kRingStride = kRing + 2;  // per wave: the ring, then the unit's cull bounds
+constexpr int kMaxTab2 = 5056;   // split-table entries the cell kernel keeps in LDS
 
 struct CellLds {
     unsigned long long *hist;
     uint2 *tab2;
     float *thr, *elo, *ehi;
-    float4 *ring;  // [kCellWaves][kRing]
+    float4 *ring;  // [kCellWaves][kRingStride]
 };
 
 __host__ __device__ inline size_t cell_lds_bytes(int bins, bool edge, int tab_n) {
     return align16((size_t)bins * 8) + align16((size_t)tab_n * 8) +
-           align16(((size_t)bins + 1) * 4 * (edge ? 3 : 1)) + (size_t)kCellWaves * kRing * 16;
+           align16(((size_t)bins + 1) * 4 * (edge ? 3 : 1)) + (size_t)kCellWaves * kRingStride * 16;
 }
 
 __device__ __forceinline__ CellLds cell_lds_layout(char *base, const HistParams &p, bool edge,
@@ -981,6 +982,14 @@ __device__ __forceinline__ void cell_unit(const CellGrid &G, int f, int h, const
             use_shift = G.shift_ok && !far && !__any(diff);
             wave_bbox(mn0, mn1, mn2, mx0, mx1, mx2);
         }
+        // the cull's bounds go to the two LDS entries past the wave's ring and
+        // are read there in the stream loop (held in SGPRs across it, they are
+        // spilled to VGPR lanes and read back with one v_readlane each)
+        float4 *const ubox = ring + kRing;
+        if (lane == 0) {
+            ubox[0] = make_float4(mn0, mn1, mn2, cull2);
+            ubox[1] = make_float4(mx0, mx1, mx2, 0.f);
+        }
         // per-piece byte constants K: n + 128 = (m_j + 32) + K per byte, with
         // K = 128 - w - (m_home + 32) (no carries: every byte stays below 256)
         const unsigned k_home = ((128u - (m0 & 0xFFu)) & 0xFFu) | (((128u - ((m0 >> 8) & 0xFFu)) & 0xFFu) << 8) |
@@ -996,23 +1005,19 @@ __device__ __forceinline__ void cell_unit(const CellGrid &G, int f, int h, const
             cy -= wy * ncy;
             cz -= wz * ncz;
             const int xa = hx + sl.xa, xb = hx + sl.xb;
-            int lo, hi, wx = 0;
-            if (lane & 1) {
-                lo = max(xa, 0); hi = min(xb, ncx - 1);
-            } else if (xa < 0) {
-                lo = xa + ncx; hi = ncx - 1; wx = -1;
-            } else if (xb >= ncx) {
-                lo = 0; hi = xb - ncx; wx = 1;
-            } else {
-                lo = 1; hi = 0;
-            }
-            pbeg = 0;
-            plen = 0;
-            if (lo <= hi) {
-                const int rowbase = (cz * ncy + cy) * ncx;
-                pbeg = (int)fs[rowbase + lo];
-                plen = (int)fs[rowbase + hi + 1] - pbeg;
-            }
+            // odd lane: [max(xa, 0), min(xb, ncx - 1)]; even lane: the part
+            // below 0 (wrapped up, wx = -1) or above ncx - 1 (wrapped down,
+            // wx = +1), else empty (grids have at least 2k + 1 cells per side,
+            // so not both).  Selects, not branches: lo and hi + 1 stay within
+            // [0, ncx] for every lane, empty runs included.
+            const bool odd = (lane & 1) != 0, wl = xa < 0, wr = xb >= ncx;
+            const int lo = odd ? max(xa, 0) : wl ? xa + ncx : wr ? 0 : 1;
+            const int hi = odd ? min(xb, ncx - 1) : wl ? ncx - 1 : wr ? xb - ncx : 0;
+            const int wx = odd ? 0 : wl ? -1 : wr ? 1 : 0;
+            const int rowbase = (cz * ncy + cy) * ncx;
+            pbeg = (int)fs[rowbase + lo];
+            const int pend = (int)fs[rowbase + hi + 1];
+            plen = lo <= hi ? pend - pbeg : 0;
             // K per byte: 128 - w - (m_home + 32); w = +1 subtracts 1, w = -1 adds 1
             pk = k_home + (unsigned)(-wx) + ((unsigned)(-wy) << 8) + ((unsigned)(-wz) << 16);
         }
@@ -1097,10 +1102,11 @@ __device__ __forceinline__ void cell_unit(const CellGrid &G, int f, int h, const
                         py = __builtin_fmaf(-fx, b.c[1], __builtin_fmaf(-fy, b.c[4], __builtin_fmaf(-fz, b.c[7], q.y)));
                         pz = __builtin_fmaf(-fx, b.c[2], __builtin_fmaf(-fy, b.c[5], __builtin_fmaf(-fz, b.c[8], q.z)));
                     }
-                    const float ex = fmaxf(fmaxf(mn0 - px, px - mx0), 0.f);
-                    const float ey = fmaxf(fmaxf(mn1 - py, py - mx1), 0.f);
-                    const float ez = fmaxf(fmaxf(mn2 - pz, pz - mx2), 0.f);
-                    keep = valid && ex * ex + ey * ey + ez * ez <= cull2;
+                    const float4 lo = ubox[0], hi = ubox[1];
+                    const float ex = fmaxf(fmaxf(lo.x - px, px - hi.x), 0.f);
+                    const float ey = fmaxf(fmaxf(lo.y - py, py - hi.y), 0.f);
+                    const float ez = fmaxf(fmaxf(lo.z - pz, pz - hi.z), 0.f);
+                    keep = valid && ex * ex + ey * ey + ez * ez <= lo.w;
                 }
                 const unsigned long long bal = __ballot(keep);
                 if (keep) {
@@ -1261,7 +1267,7 @@ __global__ __launch_bounds__(kCellBlock) __attribute__((amdgpu_waves_per_eu(8, 8
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform: unit indices in SGPRs
     const int lane = threadIdx.x & 63;
     PlainCons<PBC, WEIGHTED, EDGE, TABLE> cons{p, L, 0ull, 0ull};
-    cell_waves<PBC, false>(G, L.ring + wave * kRing, nullptr, wave, cons);
+    cell_waves<PBC, false>(G, L.ring + wave * kRingStride, nullptr, wave, cons);
 
     __syncthreads();
     for (int i = threadIdx.x; i < p.bins; i += kCellBlock) {
