@@ -985,10 +985,22 @@ __device__ __forceinline__ void cell_unit(const CellGrid &G, int f, int h, const
         // the cull's bounds go to the two LDS entries past the wave's ring and
         // are read there in the stream loop (held in SGPRs across it, they are
         // spilled to VGPR lanes and read back with one v_readlane each)
+        // The bounds are stored shifted by -128 lattice vectors (S), so that
+        // the cull can take the biased image index n + 128 of each byte as it
+        // is (its image positions are then off by the same -S; the rounding
+        // this adds is covered by cull2_for's margin).
         float4 *const ubox = ring + kRing;
         if (lane == 0) {
-            ubox[0] = make_float4(mn0, mn1, mn2, cull2);
-            ubox[1] = make_float4(mx0, mx1, mx2, 0.f);
+            float sx, sy, sz;
+            if constexpr (PBC == WMD_PBC_ORTHO) {
+                sx = 128.f * b.lx; sy = 128.f * b.ly; sz = 128.f * b.lz;
+            } else {
+                sx = 128.f * (b.c[0] + b.c[3] + b.c[6]);
+                sy = 128.f * (b.c[1] + b.c[4] + b.c[7]);
+                sz = 128.f * (b.c[2] + b.c[5] + b.c[8]);
+            }
+            ubox[0] = make_float4(mn0 - sx, mn1 - sy, mn2 - sz, cull2);
+            ubox[1] = make_float4(mx0 - sx, mx1 - sy, mx2 - sz, 0.f);
         }
         // per-piece byte constants K: n + 128 = (m_j + 32) + K per byte, with
         // K = 128 - w - (m_home + 32) (no carries: every byte stays below 256)
@@ -1081,33 +1093,33 @@ __device__ __forceinline__ void cell_unit(const CellGrid &G, int f, int h, const
                 }
                 const bool valid = pos < e_end;
                 const float4 q = fp[valid ? src : hs];  // lanes past the stream end load the home start
-                unsigned meta = __float_as_uint(q.w);
-                bool keep = valid;
-                if (!use_shift) {
-                    meta = (meta & 0xFF000000u) | 0x808080u;  // multiplicities only
-                } else {
+                const float4 lo = ubox[0];
+                unsigned nb = 0x808080u;  // without the shift path: multiplicities only
+                float d2 = 0.f;           // squared distance to the home atoms' box (shift path)
+                if (use_shift) {
                     // n + 128 per byte; home atoms get n = 0 and lie inside the box
-                    const unsigned nb = (meta & 0xFFFFFFu) + kk;
-                    meta = (meta & 0xFF000000u) | nb;
-                    const float fx = (float)(nb & 0xFFu) - 128.f;
-                    const float fy = (float)((nb >> 8) & 0xFFu) - 128.f;
-                    const float fz = (float)((nb >> 16) & 0xFFu) - 128.f;
-                    float px, py, pz;  // image of j next to the home atoms (approximate)
+                    nb = (__float_as_uint(q.w) & 0xFFFFFFu) + kk;
+                    const float bx = (float)(nb & 0xFFu);
+                    const float by = (float)((nb >> 8) & 0xFFu);
+                    const float bz = (float)((nb >> 16) & 0xFFu);
+                    float px, py, pz;  // image of j next to the home atoms, less S (approximate)
                     if constexpr (PBC == WMD_PBC_ORTHO) {
-                        px = __builtin_fmaf(-fx, b.lx, q.x);
-                        py = __builtin_fmaf(-fy, b.ly, q.y);
-                        pz = __builtin_fmaf(-fz, b.lz, q.z);
+                        px = __builtin_fmaf(-bx, b.lx, q.x);
+                        py = __builtin_fmaf(-by, b.ly, q.y);
+                        pz = __builtin_fmaf(-bz, b.lz, q.z);
                     } else {
-                        px = __builtin_fmaf(-fx, b.c[0], __builtin_fmaf(-fy, b.c[3], __builtin_fmaf(-fz, b.c[6], q.x)));
-                        py = __builtin_fmaf(-fx, b.c[1], __builtin_fmaf(-fy, b.c[4], __builtin_fmaf(-fz, b.c[7], q.y)));
-                        pz = __builtin_fmaf(-fx, b.c[2], __builtin_fmaf(-fy, b.c[5], __builtin_fmaf(-fz, b.c[8], q.z)));
+                        px = __builtin_fmaf(-bx, b.c[0], __builtin_fmaf(-by, b.c[3], __builtin_fmaf(-bz, b.c[6], q.x)));
+                        py = __builtin_fmaf(-bx, b.c[1], __builtin_fmaf(-by, b.c[4], __builtin_fmaf(-bz, b.c[7], q.y)));
+                        pz = __builtin_fmaf(-bx, b.c[2], __builtin_fmaf(-by, b.c[5], __builtin_fmaf(-bz, b.c[8], q.z)));
                     }
-                    const float4 lo = ubox[0], hi = ubox[1];
+                    const float4 hi = ubox[1];
                     const float ex = fmaxf(fmaxf(lo.x - px, px - hi.x), 0.f);
                     const float ey = fmaxf(fmaxf(lo.y - py, py - hi.y), 0.f);
                     const float ez = fmaxf(fmaxf(lo.z - pz, pz - hi.z), 0.f);
-                    keep = valid && ex * ex + ey * ey + ez * ez <= lo.w;
+                    d2 = ex * ex + ey * ey + ez * ez;
                 }
+                const unsigned meta = (__float_as_uint(q.w) & 0xFF000000u) | nb;
+                const bool keep = valid && d2 <= lo.w;
                 const unsigned long long bal = __ballot(keep);
                 if (keep) {
                     const int slot = (tail + lanes_below(bal)) & (kRing - 1);
@@ -1501,8 +1513,13 @@ float cull2_for(double reach, const BoxSet &B) {
         for (int i = 0; i < 9; ++i) row += std::fabs((double)b.c[i]);
         coord_bound = std::max(coord_bound, row);
     }
+    // the cull's biased image positions q - (n + 128) . box (n + 128 in
+    // [111, 145] for frames within kImageLimit box lengths, |q| below 9 box
+    // lengths) and its bounds shifted by -128 lattice vectors carry at most
+    // five roundings of magnitude below 160 x row per axis
+    const double biased = std::sqrt(3.0) * 5.0 * 160.0 * coord_bound * std::ldexp(1.0, -24);
     coord_bound *= (double)(kImageLimit + 1);
-    const double margin = 0.05 + 32.0 * coord_bound * std::ldexp(1.0, -24);
+    const double margin = 0.05 + 32.0 * coord_bound * std::ldexp(1.0, -24) + biased;
     const float cull = (float)(reach + margin);
     return cull * cull;
 }
