@@ -226,9 +226,11 @@ struct SoftFwdCons {
             const bool in = s < sp.cut2 && (!HOME || i < rank);
             const unsigned long long bal = __ballot(in);
             // slot pt + (in-range lanes below): pt stays below 64, so the
-            // slot stays below 128 without a wrap
-            if (in) pbuf[__builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32),
-                                                   __builtin_amdgcn_mbcnt_lo((unsigned)bal, (unsigned)pt))] = s;
+            // slot stays below 128 without a wrap (pt goes into the uniform
+            // base address, not into the lane count)
+            float *const dst = pbuf + pt;
+            if (in) dst[__builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32),
+                                                  __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0u))] = s;
             pt += (int)__popcll(bal);
             if (pt >= 64) {
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
